@@ -1,0 +1,225 @@
+"""MI355X engine for BTE-Barna's deviational Monte Carlo (RTAMC2D,
+beRTAMC2D) and its Krylov propagator builder, driven through the C ABI of
+libbte_amd.so.
+
+Python here is a thin host mirror used by the tests and the benchmark; the
+C++ front-ends in apps/ are the drop-in executables.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._abi import (BertCfg, BertReport, BteError, FRAME_CB, KrylovReport, PREC, ARNOLDI, RtaCfg,
+                   RtaReport, check, lib, LIBPATH)
+from . import workloads  # noqa: F401
+
+__all__ = ["Context", "ModeTable", "Geometry", "Propagator", "rtamc2d", "bertamc2d", "build_P",
+           "synthetic_modes", "synthetic_B", "alias_build", "BteError", "workloads", "LIBPATH"]
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def synthetic_modes(n, seed, fmin=0.1, fmax=15.0, vmin=0.5, vmax=8.0, tmin=1.0, tmax=1000.0,
+                    mirror=False):
+    """Seeded synthetic mode table -> dict(omega, vx, vy, w)."""
+    out = {k: np.zeros(n) for k in ("omega", "vx", "vy", "w")}
+    check(lib().bte_synthetic_modes(n, seed, fmin, fmax, vmin, vmax, tmin, tmax, int(mirror),
+                                    _dp(out["omega"]), _dp(out["vx"]), _dp(out["vy"]), _dp(out["w"])))
+    return out
+
+
+def synthetic_B(C_modes, w, density=0.05, seed=1, iters=200):
+    """Synthetic energy-conserving collision operator B = -Omega (column-major)."""
+    C_modes, w = _f64(C_modes), _f64(w)
+    n = len(C_modes)
+    B = np.zeros((n, n), order="F")
+    check(lib().bte_synthetic_B(n, _dp(C_modes), _dp(w), density, seed, iters,
+                                B.ctypes.data_as(C.POINTER(C.c_double))))
+    return B
+
+
+def alias_build(pmf):
+    pmf = _f64(pmf)
+    n = len(pmf)
+    th = np.zeros(n, np.uint32)
+    al = np.zeros(n, np.uint32)
+    check(lib().bte_alias_build(n, _dp(pmf), th.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                al.ctypes.data_as(C.POINTER(C.c_uint32))))
+    return th, al
+
+
+class _Handle:
+    _destroy = None
+
+    def __init__(self):
+        self.h = C.c_void_p()
+
+    def close(self):
+        if self.h and self.h.value:
+            check(getattr(lib(), self._destroy)(self.h))
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context(_Handle):
+    _destroy = "bte_ctx_destroy"
+
+    def __init__(self, device=0):
+        super().__init__()
+        check(lib().bte_ctx_create(device, C.byref(self.h)))
+
+    def sync(self):
+        check(lib().bte_sync(self.h))
+
+
+class ModeTable(_Handle):
+    """Phonon modes of one material: omega [rad/ps], v [nm/ps], w [1/ps],
+    unit-cell volume V [nm^3], lattvec(2,2) [nm], nq q-points."""
+    _destroy = "bte_modes_destroy"
+
+    def __init__(self, ctx, omega, vx, vy, w, V, lattz=1.0, nq=1):
+        super().__init__()
+        self.ctx = ctx
+        self.omega, self.vx, self.vy, self.w = map(_f64, (omega, vx, vy, w))
+        self.V, self.lattz, self.nq = float(V), float(lattz), int(nq)
+        self.n = len(self.omega)
+        check(lib().bte_modes_create(ctx.h, self.n, self.nq, _dp(self.omega), _dp(self.vx), _dp(self.vy),
+                                     _dp(self.w), self.V, self.lattz, C.byref(self.h)))
+
+
+class Geometry(_Handle):
+    """Rectangular boxes: rect rows (xmin, xmax, ymin, ymax) [nm];
+    kind 0 box, 1 reservoir."""
+    _destroy = "bte_geom_destroy"
+
+    def __init__(self, ctx, rect, kind, Teq, Tinit=None):
+        super().__init__()
+        self.ctx = ctx
+        self.rect = _f64(rect).reshape(-1, 4)
+        self.kind = np.ascontiguousarray(kind, dtype=np.int32)
+        self.Teq = _f64(Teq)
+        self.Tinit = None if Tinit is None else _f64(Tinit)
+        self.nbox = len(self.kind)
+        check(lib().bte_geom_create_rects(ctx.h, self.nbox, _dp(self.rect),
+                                          self.kind.ctypes.data_as(C.POINTER(C.c_int32)), _dp(self.Teq),
+                                          None if self.Tinit is None else _dp(self.Tinit), C.byref(self.h)))
+
+    def contacts(self, maxc=8):
+        nc = np.zeros(self.nbox, np.int32)
+        nbr = np.zeros((self.nbox, maxc), np.int32)
+        geo = np.zeros((self.nbox, maxc, 6))
+        check(lib().bte_geom_contacts(self.h, maxc, nc.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      nbr.ctypes.data_as(C.POINTER(C.c_int32)), _dp(geo)))
+        return nc, nbr, geo
+
+
+class Propagator(_Handle):
+    """Dense propagator P (ncol x ncol) acting on modes [ngamma, ngamma+ncol)."""
+    _destroy = "bte_prop_destroy"
+
+    def __init__(self, ctx, P, ngamma=0, T=0.0):
+        super().__init__()
+        P = np.asfortranarray(P, dtype=np.float64)
+        self.ncol = P.shape[0]
+        check(lib().bte_prop_create(ctx.h, self.ncol, P.ctypes.data_as(C.POINTER(C.c_double)), ngamma, T,
+                                    C.byref(self.h)))
+
+
+class RtaRun(_Handle):
+    """RTAMC2D session: device tables are built once, execute() runs the
+    particle histories (the timed hot path)."""
+    _destroy = "bte_rta_destroy"
+
+    def __init__(self, ctx, modes, geom, thickness, T0, dt, maxtime, nparticles, seed=1,
+                 ballistic=False, sigma_bounce=0.25):
+        super().__init__()
+        self.cfg = RtaCfg(thickness, T0, dt, maxtime, nparticles, seed, int(ballistic), sigma_bounce)
+        self.nbox = geom.nbox
+        self.ntime = int(np.floor(maxtime / dt + 1e-6)) + 1
+        self._keep = (ctx, modes, geom)
+        check(lib().bte_rta_create(ctx.h, modes.h, geom.h, C.byref(self.cfg), C.byref(self.h)))
+
+    def execute(self):
+        nt = self.ntime - 1
+        T = np.zeros((nt, self.nbox))
+        jx = np.zeros((nt, self.nbox))
+        jy = np.zeros((nt, self.nbox))
+        rep = RtaReport()
+        check(lib().bte_rta_execute(self.h, _dp(T), _dp(jx), _dp(jy), C.byref(rep)))
+        return {"T": T, "jx": jx, "jy": jy, "events": rep.events, "crossings": rep.crossings,
+                "Eeff": rep.Eeff, "kernel_ms": rep.kernel_ms}
+
+
+def rtamc2d(ctx, modes, geom, **kw):
+    r = RtaRun(ctx, modes, geom, **kw)
+    try:
+        return r.execute()
+    finally:
+        r.close()
+
+
+class BertRun(_Handle):
+    """beRTAMC2D session (device tables built once, execute() runs the steps)."""
+    _destroy = "bte_bert_destroy"
+
+    def __init__(self, ctx, modes, geom, prop, thickness, dt, maxtime, max_steps=0, Ed=-1.0,
+                 Efactor=0, Tmax=0.0, Tmin=0.0, rta=False, pcancel_each=4, seed=1, capacity=0):
+        super().__init__()
+        self.cfg = BertCfg(thickness, dt, maxtime, max_steps, Ed, Efactor, Tmax, Tmin, int(rta),
+                           pcancel_each, seed, capacity)
+        self.nbox = geom.nbox
+        nst = 0
+        t = 0.0
+        while t <= maxtime:
+            nst += 1
+            t += dt
+        self.nsteps = min(max_steps, nst) if max_steps > 0 else nst
+        self._keep = (ctx, modes, geom, prop)
+        check(lib().bte_bert_create(ctx.h, modes.h, geom.h, None if prop is None else prop.h,
+                                    C.byref(self.cfg), C.byref(self.h)))
+
+    def execute(self, frame_cb=None):
+        props = np.zeros((self.nsteps + 1, self.nbox, 3))
+        npart = np.zeros(self.nsteps + 1, np.int64)
+        pending = np.zeros(self.nsteps + 1, np.int64)
+        rep = BertReport()
+        cb = FRAME_CB(0) if frame_cb is None else FRAME_CB(
+            lambda u, s, t, e, nb, nm, H: int(bool(frame_cb(s, t, e, np.ctypeslib.as_array(H, (nb, nm)).copy()))))
+        check(lib().bte_bert_execute(self.h, _dp(props), npart.ctypes.data_as(C.POINTER(C.c_int64)),
+                                     pending.ctypes.data_as(C.POINTER(C.c_int64)), cb, None, C.byref(rep)))
+        n = rep.nsteps
+        return {"props": props[:n + 1], "npart": npart[:n + 1], "pending": pending[:n + 1], "nsteps": n,
+                "Eeff": rep.Eeff,
+                "particle_steps": rep.particle_steps, "max_particles": rep.max_particles,
+                "loop_ms": rep.loop_ms, "step_kernel_ms": rep.step_kernel_ms}
+
+
+def bertamc2d(ctx, modes, geom, prop, **kw):
+    b = BertRun(ctx, modes, geom, prop, **kw)
+    try:
+        return b.execute()
+    finally:
+        b.close()
+
+
+def build_P(ctx, B, dt, m=30, precision="fp64", arnoldi="textbook"):
+    """P = exp(dt*B) for all columns (block Krylov); returns (P, report)."""
+    B = np.asfortranarray(B, dtype=np.float64)
+    N = B.shape[0]
+    P = np.zeros((N, N), order="F")
+    rep = KrylovReport()
+    check(lib().bte_krylov_build(ctx.h, N, B.ctypes.data_as(C.POINTER(C.c_double)), dt, m, PREC[precision],
+                                 ARNOLDI[arnoldi], P.ctypes.data_as(C.POINTER(C.c_double)), C.byref(rep)))
+    return P, {"build_ms": rep.build_ms, "gemm_ms": rep.gemm_ms, "gemm_tflops": rep.gemm_tflops,
+               "max_col_sum_err": rep.max_col_sum_err, "breakdowns": rep.breakdowns}

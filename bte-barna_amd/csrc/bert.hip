@@ -1,0 +1,933 @@
+// beRTAMC2D on the GPU: time-stepped deviational MC with a dense propagator.
+//
+// Per step s (reference order in src/beyondRTA_MC2d.cpp:2138-2425):
+//   bert_step_kernel   cancellation of step s-1 (2295-2370) for particles
+//                      that reach it, advection by dt (evolparticle 708-970),
+//                      Scattering (1073-1210) or ScatteringRTA (983-1062),
+//                      histogram (2281-2289), compacted write-out
+//   bert_source_kernel Isothermal_walls (446-585), emitted particles
+//                      advected U*dt, scattered, histogrammed (only launched
+//                      when reservoirs inject)
+//   bert_walls_kernel  adiabatic_diffuse_walls (281-435): one workgroup per
+//                      box side; border temperature from the J(T) spline,
+//                      |Eborder| particles re-emitted, advected U*dt,
+//                      scattered, histogrammed
+//   bert_props_kernel  calculate_prop (1535-1602) for every box, post-cancel
+//                      particle count, buffer recycling
+// Particle state is 32 bytes (x, y f64; mode; box|sign; 64-bit id). The
+// random stream of a particle in a step is PCG32 seeded from (seed, id,
+// step), so a particle's fate does not depend on where it sits in the buffer.
+#include "runtime.hpp"
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+
+namespace bte {
+
+struct __align__(16) BPart {
+    double x, y;
+    uint32_t mode;
+    uint32_t boxsign;  // box << 1 | (sign < 0)
+    uint64_t pid;
+};
+static_assert(sizeof(BPart) == 32, "particle record is 32 bytes");
+
+struct BertParams {
+    const double2* vel;      // (vx, vy) per mode
+    const double* omega;
+    const double* w;
+    const DevBox* boxes;
+    const DevContact* cts;
+    const uint2* palias;     // propagator alias, ncol x ncol column-major
+    const uint2* rta_alias;  // BE_derivative alias per box-temperature slot
+    const int32_t* box_rta_slot;
+    const double* box_Teq;
+    const double* spl;       // spline coefficients: per table 6 x nseg
+    const int32_t* side_spl; // per (box, side) spline table index, -1 none
+    BPart* pin;
+    BPart* pout;
+    uint32_t* count;         // [2] live counts
+    int32_t* H;              // [2][nbox][n]
+    uint32_t* tickets;       // [nbox][n]
+    int32_t* Eb;             // [2][nbox*4]
+    double* props;           // [steps+1][nbox][3]
+    int64_t* npart;          // [steps+1]
+    int64_t* pending;        // [steps+1] signed wall counts awaiting re-emission
+    uint32_t* flags;         // [0] overflow, [1] bad mode, [2] stuck
+    unsigned long long* stats;  // [0] particle-steps
+    uint64_t cap;
+    uint64_t seed;
+    double dt, Eeff, thickness;
+    int32_t n, ngamma, ncol, nbox, nseg;
+    int32_t rta, max_children;
+};
+
+__device__ inline void seed_stream(Pcg32& r, uint64_t seed, uint64_t pid, uint32_t step, uint32_t salt) {
+    r.seed(mix64(seed ^ mix64(pid + 0x632be59bd9b4e019ULL * ((uint64_t)step + 1) + salt)), pid);
+}
+
+// evolparticle (beyondRTA_MC2d.cpp:708-970): ballistic advection by dt
+// through box crossings; a void side adds the particle sign to that side's
+// Eborder and ends the particle. Returns false if the particle ended.
+__device__ inline bool advect(const BertParams& P, double& x, double& y, int& box, int sign,
+                              double vx, double vy, double dt, Pcg32& rng, int32_t* Eb) {
+    for (int it = 0; it < 4096; it++) {
+        const DevBox B = P.boxes[box];
+        if (B.kind == 1) return false;  // reservoir: deleted
+        double nx = x + dt * vx, ny = y + dt * vy;
+        if (box_inside(B, nx, ny)) {
+            x = nx;
+            y = ny;
+            return true;
+        }
+        int mask;
+        double ex, ey;
+        double tf = box_exit(B, x, y, vx, vy, mask, ex, ey);
+        double left = dt - tf;
+        if (almost_equal(left, 0.)) left = 0.;
+        int pb[4];
+        int npb = find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
+        if (npb == 0) {
+            int s = __popc(mask) == 1 ? __ffs(mask) - 1
+                                      : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next());
+            atomicAdd(Eb + box * 4 + s, sign);
+            return false;
+        }
+        x = ex;
+        y = ey;
+        if (npb == 1) {
+            box = pb[0];
+        } else {
+            int cand[4], nc = 0;
+            double tau = 1.0e-6;
+            while (nc == 0 && tau >= 1.0e-15) {
+                for (int k = 0; k < npb && k < 4; k++)
+                    if (box_inside(P.boxes[pb[k]], ex + tau * vx, ey + tau * vy)) cand[nc++] = pb[k];
+                tau /= 10.0;
+            }
+            if (nc == 0) {
+                double dx = 0.5 * (B.xmin + B.xmax) - ex, dy = 0.5 * (B.ymin + B.ymax) - ey;
+                double dn = sqrt(dx * dx + dy * dy);
+                x = ex + 1.0e-6 * dx / dn;
+                y = ey + 1.0e-6 * dy / dn;
+            } else {
+                box = cand[rng.bounded((uint32_t)nc)];
+            }
+        }
+        dt = left;
+        if (dt == 0.) return true;
+    }
+    atomicOr(P.flags + 2, 1u);
+    return false;
+}
+
+// Scattering with the propagator (beyondRTA_MC2d.cpp:1133-1192) or
+// ScatteringRTA (1040-1055). Replays from a copy of the stream: pass 0
+// counts children, pass 1 emits. Children of a sign flip re-queue the
+// parent twice (1189-1192).
+template <bool EMIT>
+__device__ inline int scatter(const BertParams& P, Pcg32 rng, int mode, int sign, int box,
+                              BPart* out, uint64_t slot, double x, double y, uint64_t pid,
+                              uint32_t step) {
+    if (P.rta) {
+        double R = rng.uniform();
+        int nm = mode;
+        if (R < 1.0 - exp(-P.w[mode] * P.dt)) {
+            bool ng;
+            int sl = P.box_rta_slot[box];
+            nm = (int)alias_pick(P.rta_alias + (size_t)sl * P.n, (uint32_t)P.n, rng, ng);
+        }
+        if (EMIT) {
+            atomicAdd(P.H + (size_t)box * P.n + nm, sign);
+            if (slot < P.cap) {
+                BPart q{x, y, (uint32_t)nm, ((uint32_t)box << 1) | (sign < 0 ? 1u : 0u), pid};
+                out[slot] = q;
+            }
+        }
+        return 1;
+    }
+    int col = mode - P.ngamma;
+    if (col < 0 || col >= P.ncol) {
+        atomicOr(P.flags + 1, 1u);
+        return 0;
+    }
+    const uint2* tab = P.palias + (size_t)col * P.ncol;
+    int pending = 1, nchild = 0;
+    while (pending > 0 && nchild < P.max_children) {
+        pending--;
+        bool neg;
+        int k = (int)alias_pick(tab, (uint32_t)P.ncol, rng, neg);
+        int nm = k + P.ngamma;
+        int ns = neg ? -sign : sign;
+        if (EMIT) {
+            atomicAdd(P.H + (size_t)box * P.n + nm, ns);
+            uint64_t s = slot + nchild;
+            if (s < P.cap) {
+                uint64_t cpid = nchild == 0 ? pid : mix64(pid ^ ((uint64_t)step << 40) ^ (uint64_t)nchild);
+                BPart q{x, y, (uint32_t)nm, ((uint32_t)box << 1) | (ns < 0 ? 1u : 0u), cpid};
+                out[s] = q;
+            }
+        }
+        nchild++;
+        if (neg) pending += 2;
+    }
+    if (pending > 0) atomicOr(P.flags + 2, 2u);
+    return nchild;
+}
+
+// wave-wide exclusive prefix sum of c; returns the base slot for this lane
+__device__ inline uint64_t wave_reserve(uint32_t* counter, int c, uint64_t cap, uint32_t* flags) {
+    const int lane = threadIdx.x & 63;
+    int incl = c;
+    for (int d = 1; d < 64; d <<= 1) {
+        int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    int total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(counter, (uint32_t)total);
+    base = __shfl(base, 63, 64);
+    uint64_t b = (uint64_t)base + (uint64_t)(incl - c);
+    if (c > 0 && b + c > cap) atomicOr(flags, 1u);
+    return b;
+}
+
+__global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t step, int cur,
+                                                        int cancel_prev) {
+    const uint32_t nin = min(P.count[cur], (uint32_t)P.cap);
+    int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
+    const int32_t* Hprev = P.H + (size_t)(cur ^ 1) * P.nbox * P.n;
+    int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
+    BertParams Q = P;
+    Q.H = H;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // whole waves iterate together so wave_reserve sees every lane
+    const uint64_t nround = ((uint64_t)nin + 63) & ~63ULL;
+    unsigned long long done = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i - (threadIdx.x & 63) < nround; i += stride) {
+        bool valid = i < nin;
+        BPart p{};
+        int box = 0, sign = 1, mode = 0;
+        bool alive = false;
+        Pcg32 rng;
+        if (valid) {
+            p = P.pin[i];
+            box = (int)(p.boxsign >> 1);
+            sign = (p.boxsign & 1u) ? -1 : 1;
+            mode = (int)p.mode;
+            alive = true;
+            if (cancel_prev) {
+                // cancellation (beyondRTA_MC2d.cpp:2342-2359): keep |h| particles
+                // of sign(h) per (box, mode); tickets order the survivors
+                int h = Hprev[(size_t)box * P.n + mode];
+                if (h == 0 || ((sign < 0) != (h < 0))) {
+                    alive = false;
+                } else {
+                    uint32_t tk = atomicAdd(P.tickets + (size_t)box * P.n + mode, 1u);
+                    if (tk >= (uint32_t)(h < 0 ? -h : h)) alive = false;
+                }
+            }
+            if (alive) {
+                seed_stream(rng, P.seed, p.pid, step, 0);
+                const double2 v = P.vel[mode];
+                alive = advect(P, p.x, p.y, box, sign, v.x, v.y, P.dt, rng, Eb);
+            }
+        }
+        int c = alive ? scatter<false>(Q, rng, mode, sign, box, nullptr, 0, 0, 0, 0, 0) : 0;
+        uint64_t slot = wave_reserve(P.count + (cur ^ 1), c, P.cap, P.flags);
+        if (c > 0) scatter<true>(Q, rng, mode, sign, box, P.pout, slot, p.x, p.y, p.pid, step);
+        done += alive ? 1 : 0;
+    }
+    // one atomic per wave for the particle-step counter
+    for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
+    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
+}
+
+// One workgroup per (box, side): adiabatic diffuse wall re-emission
+// (beyondRTA_MC2d.cpp:281-435). Needs n <= BERT_WALL_MAXN.
+#define BERT_WALL_MAXN 7680
+__global__ __launch_bounds__(256) void bert_walls_kernel(BertParams P, uint32_t step, int cur) {
+    __shared__ double cdf[BERT_WALL_MAXN];
+    __shared__ double wsum[256];
+    const int b = blockIdx.x;  // box*4 + side
+    const int box = b >> 2, side = b & 3;
+    int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
+    int32_t* Ebn = P.Eb + (size_t)((step + 1) & 1) * P.nbox * 4;
+    const int E = Eb[b];
+    if (E == 0) return;
+    const int si = P.side_spl[b];
+    __syncthreads();
+    if (threadIdx.x == 0) Eb[b] = 0;
+    if (si < 0) return;
+    const DevBox B = P.boxes[box];
+    const double Teq = P.box_Teq[box];
+    double p1x, p1y, ndx, ndy;
+    if (side == 0) { p1x = B.xmin; p1y = B.ymin; ndx = 0; ndy = B.ymax - B.ymin; }
+    else if (side == 1) { p1x = B.xmin; p1y = B.ymin; ndx = B.xmax - B.xmin; ndy = 0; }
+    else if (side == 2) { p1x = B.xmin; p1y = B.ymax; ndx = B.xmax - B.xmin; ndy = 0; }
+    else { p1x = B.xmax; p1y = B.ymax; ndx = 0; ndy = B.ymin - B.ymax; }
+    const double L = sqrt(ndx * ndx + ndy * ndy);
+    // border temperature from the J(T) spline
+    const double J = P.Eeff * (double)E / P.dt / (L * P.thickness);
+    const double* S = P.spl + (size_t)si * 6 * P.nseg;  // a b c d x0 xe
+    int lo = 0, hi = P.nseg;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (S[5 * P.nseg + mid] < J) lo = mid + 1; else hi = mid;
+    }
+    if (lo >= P.nseg) lo = P.nseg - 1;
+    const double dx = J - S[4 * P.nseg + lo];
+    const double Tb = S[lo] + dx * (S[P.nseg + lo] + dx * (S[2 * P.nseg + lo] + S[3 * P.nseg + lo] * dx));
+    double nx, ny;
+    side_normal(side, nx, ny);
+    nx = -nx;  // emission normal points into the box
+    ny = -ny;
+    // wall pmf (sampling.cpp:227-240) and its inclusive scan in LDS
+    const int n = P.n;
+    const int per = (n + blockDim.x - 1) / blockDim.x;
+    const int i0 = threadIdx.x * per, i1 = min(n, i0 + per);
+    double acc = 0;
+    for (int i = i0; i < i1; i++) {
+        const double2 v = P.vel[i];
+        const double om = P.omega[i];
+        double vn = nx * v.x + ny * v.y;
+        double pm = 0.;
+        if (vn > 0. && !almost_equal(0., om))
+            pm = vn * om * fabs(bose_einstein(om, Tb) - bose_einstein(om, Teq));
+        acc += pm;
+        cdf[i] = acc;
+    }
+    wsum[threadIdx.x] = acc;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        double v = threadIdx.x >= d ? wsum[threadIdx.x - d] : 0.;
+        __syncthreads();
+        wsum[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const double off = threadIdx.x > 0 ? wsum[threadIdx.x - 1] : 0.;
+    for (int i = i0; i < i1; i++) cdf[i] += off;
+    __syncthreads();
+    const double total = cdf[n - 1];
+    const int sign = (Tb - Teq) >= 0 ? 1 : -1;
+    const int ng = (int)(fabs((double)E) + 1.0e-6);
+    int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
+    BertParams Q = P;
+    Q.H = H;
+    const int nloop = (ng + 63) & ~63;
+    unsigned long long done = 0;
+    for (int k = threadIdx.x; k - (threadIdx.x & 63) < nloop; k += blockDim.x) {
+        bool valid = k < ng && total > 0.;
+        Pcg32 rng;
+        int mode = 0, bx = box;
+        double x = 0, y = 0;
+        bool alive = false;
+        uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)b << 20) ^ (uint64_t)k ^ 0xad1aULL);
+        if (valid) {
+            seed_stream(rng, P.seed, pid, step, 1);
+            double u = rng.uniform() * total;
+            int l = 0, h = n - 1;
+            while (l < h) {
+                int mid = (l + h) >> 1;
+                if (cdf[mid] < u) l = mid + 1; else h = mid;
+            }
+            mode = l;
+            double t = rng.uniform();
+            x = p1x + t * ndx;
+            y = p1y + t * ndy;
+            const double2 v = P.vel[mode];
+            double rdt = P.dt * rng.uniform();
+            alive = advect(P, x, y, bx, sign, v.x, v.y, rdt, rng, Ebn);
+        }
+        int c = alive ? scatter<false>(Q, rng, mode, sign, bx, nullptr, 0, 0, 0, 0, 0) : 0;
+        uint64_t slot = wave_reserve(P.count + (cur ^ 1), c, P.cap, P.flags);
+        if (c > 0) scatter<true>(Q, rng, mode, sign, bx, P.pout, slot, x, y, pid, step);
+        done += alive ? 1 : 0;
+    }
+    for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
+    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
+}
+
+// Reservoir injection (Isothermal_walls, beyondRTA_MC2d.cpp:446-585): one
+// workgroup per (reservoir, contact) source with a fixed alias table.
+struct BertSource {
+    double p1x, p1y, ndx, ndy;
+    double expected;  // particles per step
+    int32_t box, sign;
+    int32_t tab;      // alias table offset (in entries)
+    int32_t pad;
+};
+__global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src,
+                                                          const uint2* stab, uint32_t step, int cur) {
+    const BertSource S = src[blockIdx.x];
+    int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
+    Pcg32 cr;
+    seed_stream(cr, P.seed, 0x5ce0000000000000ULL + blockIdx.x, step, 2);
+    const double fl = floor(S.expected);
+    const int ng = (int)fl + (cr.uniform() <= S.expected - fl ? 1 : 0);
+    int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
+    BertParams Q = P;
+    Q.H = H;
+    const int nloop = (ng + 63) & ~63;
+    unsigned long long done = 0;
+    for (int k = threadIdx.x; k - (threadIdx.x & 63) < nloop; k += blockDim.x) {
+        bool valid = k < ng;
+        Pcg32 rng;
+        int mode = 0, bx = S.box;
+        double x = 0, y = 0;
+        bool alive = false;
+        uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)blockIdx.x << 20) ^ (uint64_t)k ^ 0x15a0ULL);
+        if (valid) {
+            seed_stream(rng, P.seed, pid, step, 3);
+            double t = rng.uniform();
+            x = S.p1x + t * S.ndx;
+            y = S.p1y + t * S.ndy;
+            const DevBox B = P.boxes[S.box];
+            if (!box_inside(B, x, y)) {
+                double dx = 0.5 * (B.xmin + B.xmax) - x, dy = 0.5 * (B.ymin + B.ymax) - y;
+                double dn = sqrt(dx * dx + dy * dy);
+                x += 1.0e-6 * dx / dn;
+                y += 1.0e-6 * dy / dn;
+            }
+            bool ngf;
+            mode = (int)alias_pick(stab + S.tab, (uint32_t)P.n, rng, ngf);
+            const double2 v = P.vel[mode];
+            double rdt = P.dt * rng.uniform();
+            alive = advect(P, x, y, bx, S.sign, v.x, v.y, rdt, rng, Eb);
+        }
+        int c = alive ? scatter<false>(Q, rng, mode, S.sign, bx, nullptr, 0, 0, 0, 0, 0) : 0;
+        uint64_t slot = wave_reserve(P.count + (cur ^ 1), c, P.cap, P.flags);
+        if (c > 0) scatter<true>(Q, rng, mode, S.sign, bx, P.pout, slot, x, y, pid, step);
+        done += alive ? 1 : 0;
+    }
+    for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
+    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
+}
+
+// calculate_prop (beyondRTA_MC2d.cpp:1535-1602) from the step histogram, the
+// post-cancellation particle count, and buffer recycling for the next step.
+__global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, int cur,
+                                                         int cancel_now, int clear_tickets, int ebn) {
+    const int box = blockIdx.x;
+    const int32_t* H = P.H + (size_t)cur * P.nbox * P.n + (size_t)box * P.n;
+    int32_t* Hold = P.H + (size_t)(cur ^ 1) * P.nbox * P.n + (size_t)box * P.n;
+    __shared__ double red[3][256];
+    __shared__ long long redc[2][256];
+    double s = 0, fx = 0, fy = 0;
+    long long cnt = 0, acnt = 0;
+    for (int k = threadIdx.x; k < P.n; k += blockDim.x) {
+        int h = H[k];
+        if (h != 0) {
+            const double2 v = P.vel[k];
+            s += h;
+            fx += h * v.x;
+            fy += h * v.y;
+            cnt += h;
+            acnt += h < 0 ? -h : h;
+        }
+        Hold[k] = 0;
+        if (clear_tickets) P.tickets[(size_t)box * P.n + k] = 0;
+    }
+    red[0][threadIdx.x] = s;
+    red[1][threadIdx.x] = fx;
+    red[2][threadIdx.x] = fy;
+    redc[0][threadIdx.x] = cnt;
+    redc[1][threadIdx.x] = acnt;
+    __syncthreads();
+    for (int d = 128; d > 0; d >>= 1) {
+        if (threadIdx.x < d) {
+            for (int q = 0; q < 3; q++) red[q][threadIdx.x] += red[q][threadIdx.x + d];
+            redc[0][threadIdx.x] += redc[0][threadIdx.x + d];
+            redc[1][threadIdx.x] += redc[1][threadIdx.x + d];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const DevBox B = P.boxes[box];
+        double vol = (B.xmax - B.xmin) * (B.ymax - B.ymin) * P.thickness;
+        bool skip = B.kind != 0;
+        double* o = P.props + ((size_t)row * P.nbox + box) * 3;
+        o[0] = skip ? 0. : P.Eeff / vol * 1.0e27 * red[0][0];
+        o[1] = skip ? 0. : P.Eeff / vol * 1.0e30 * red[1][0];
+        o[2] = skip ? 0. : P.Eeff / vol * 1.0e30 * red[2][0];
+        // particles left after this step: sum |h| over regular boxes when
+        // the step cancels, else every particle written this step
+        if (cancel_now) {
+            if (!skip) atomicAdd((unsigned long long*)(P.npart + row), (unsigned long long)redc[1][0]);
+        } else if (box == 0) {
+            P.npart[row] = (int64_t)P.count[cur ^ 1];
+        }
+        if (box == 0) P.count[cur] = 0;
+        const int32_t* E = P.Eb + (size_t)ebn * P.nbox * 4 + (size_t)box * 4;
+        long long pend = (long long)E[0] + E[1] + E[2] + E[3];
+        if (pend) atomicAdd((unsigned long long*)(P.pending + row), (unsigned long long)pend);
+    }
+}
+
+// initial particles: histogram into H[cur] (properties row 0)
+__global__ __launch_bounds__(256) void bert_hist_kernel(BertParams P, int cur) {
+    const uint32_t nin = min(P.count[cur], (uint32_t)P.cap);
+    int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nin;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const BPart p = P.pin[i];
+        int box = (int)(p.boxsign >> 1);
+        int sign = (p.boxsign & 1u) ? -1 : 1;
+        atomicAdd(H + (size_t)box * P.n + p.mode, sign);
+    }
+}
+
+// initial particles from outTref distributions (beyondRTA_MC2d.cpp:211-269):
+// one workgroup per box with its count and alias table
+struct BertInit {
+    double xmin, xmax, ymin, ymax;
+    uint64_t first, count;
+    int32_t box, sign, tab, pad;
+};
+__global__ __launch_bounds__(256) void bert_init_kernel(BertParams P, const BertInit* ini,
+                                                        const uint2* itab) {
+    const BertInit I = ini[blockIdx.x];
+    for (uint64_t k = threadIdx.x; k < I.count; k += blockDim.x) {
+        uint64_t pid = I.first + k;
+        Pcg32 rng;
+        seed_stream(rng, P.seed, pid, 0xffffffffu, 4);
+        double x = I.xmin + rng.uniform() * (I.xmax - I.xmin);
+        double y = I.ymin + rng.uniform() * (I.ymax - I.ymin);
+        bool ng;
+        int mode = (int)alias_pick(itab + I.tab, (uint32_t)P.n, rng, ng);
+        BPart q{x, y, (uint32_t)mode, ((uint32_t)I.box << 1) | (I.sign < 0 ? 1u : 0u), pid};
+        P.pin[pid] = q;
+    }
+}
+
+struct Bert {
+    Ctx* ctx = nullptr;
+    const Modes* m = nullptr;
+    const Geom* g = nullptr;
+    const Prop* p = nullptr;
+    bte_bert_cfg cfg{};
+    double Eeff = 0;
+    int max_steps = 0;
+    uint64_t n0 = 0, cap = 0;
+    int nseg = 99;
+    DevBuf<double2> d_vel;
+    DevBuf<BPart> d_parts;
+    DevBuf<uint32_t> d_count, d_tickets, d_flags;
+    DevBuf<int32_t> d_H, d_Eb, d_side_spl, d_rta_slot;
+    DevBuf<double> d_props, d_spl, d_Teq;
+    DevBuf<int64_t> d_npart, d_pending;
+    DevBuf<unsigned long long> d_stats;
+    DevBuf<uint2> d_rta_alias, d_src_tab, d_init_tab;
+    DevBuf<BertSource> d_src;
+    DevBuf<BertInit> d_init;
+    int nsrc = 0, ninit = 0;
+    std::vector<int32_t> hH;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~Bert() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+};
+
+static double calc_Eeff_host(const Modes& m, double Th, double Tc, double Tr, double vol, uint64_t np) {
+    long double dh = 0, dc = 0;
+    for (int i = 0; i < m.n; i++) {
+        if (almost_equal(0., m.omega[i])) continue;
+        dh += 1e12 * hbar * m.omega[i] * std::fabs(h_bose_einstein(m.omega[i], Th) - h_bose_einstein(m.omega[i], Tr));
+        dc += 1e12 * hbar * m.omega[i] * std::fabs(h_bose_einstein(m.omega[i], Tc) - h_bose_einstein(m.omega[i], Tr));
+    }
+    return (double)(vol * (dc + dh) / m.nq / np);
+}
+
+static BertParams make_params(Bert* b) {
+    BertParams P{};
+    P.vel = b->d_vel.p;
+    P.omega = b->m->d_omega.p;
+    P.w = b->m->d_w.p;
+    P.boxes = b->g->d_boxes.p;
+    P.cts = b->g->d_contacts.p;
+    P.palias = b->p ? b->p->d_alias.p : nullptr;
+    P.rta_alias = b->d_rta_alias.p;
+    P.box_rta_slot = b->d_rta_slot.p;
+    P.box_Teq = b->d_Teq.p;
+    P.spl = b->d_spl.p;
+    P.side_spl = b->d_side_spl.p;
+    P.count = b->d_count.p;
+    P.H = b->d_H.p;
+    P.tickets = b->d_tickets.p;
+    P.Eb = b->d_Eb.p;
+    P.props = b->d_props.p;
+    P.npart = b->d_npart.p;
+    P.pending = b->d_pending.p;
+    P.flags = b->d_flags.p;
+    P.stats = b->d_stats.p;
+    P.cap = b->cap;
+    P.seed = b->cfg.seed;
+    P.dt = b->cfg.dt;
+    P.Eeff = b->Eeff;
+    P.thickness = b->cfg.thickness;
+    P.n = b->m->n;
+    P.ngamma = b->p ? b->p->ngamma : 0;
+    P.ncol = b->p ? b->p->ncol : 0;
+    P.nbox = (int)b->g->boxes.size();
+    P.nseg = b->nseg;
+    P.rta = b->cfg.rta;
+    P.max_children = 1 << 20;
+    return P;
+}
+
+}  // namespace bte
+
+using namespace bte;
+
+extern "C" {
+
+int bte_prop_create(bte_ctx* ctx, int ncol, const double* P, int ngamma, double T, bte_prop** out) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !P || !out || ncol <= 0 || ngamma < 0 || ncol >= (1 << 30))
+        throw Error(BTE_EINVAL, "bad arguments to bte_prop_create");
+    HIPCHECK(hipSetDevice(c->device));
+    auto* p = new Prop();
+    std::unique_ptr<Prop> guard(p);
+    p->ctx = c;
+    p->ncol = ncol;
+    p->ngamma = ngamma;
+    p->T = T;
+    p->P.assign(P, P + (size_t)ncol * ncol);
+    // propagator_H per column (collision_operator.cpp:854-870) as an alias
+    // table over |P_kj| with the entry signs in the alias word
+    std::vector<uint2> tab((size_t)ncol * ncol);
+    std::vector<double> pm(ncol);
+    std::vector<uint32_t> th(ncol), al(ncol);
+    p->abssum.resize(ncol);
+    for (int j = 0; j < ncol; j++) {
+        const double* col = P + (size_t)j * ncol;
+        double s = 0;
+        for (int k = 0; k < ncol; k++) {
+            pm[k] = std::fabs(col[k]);
+            s += pm[k];
+        }
+        p->abssum[j] = s;
+        if (!build_alias(pm.data(), ncol, th.data(), al.data()))
+            throw Error(BTE_EINVAL, "propagator column " + std::to_string(j) + " is zero");
+        for (int k = 0; k < ncol; k++) {
+            uint32_t a = al[k];
+            uint32_t word = a | (col[k] < 0. ? 0x80000000u : 0u) | (col[a] < 0. ? 0x40000000u : 0u);
+            tab[(size_t)j * ncol + k] = make_uint2(th[k], word);
+        }
+    }
+    p->d_alias.upload(tab, c->stream);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    *out = reinterpret_cast<bte_prop*>(guard.release());
+    BTE_GUARD_END
+}
+
+int bte_prop_destroy(bte_prop* pp) {
+    BTE_GUARD_BEGIN
+    delete reinterpret_cast<Prop*>(pp);
+    BTE_GUARD_END
+}
+
+int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const bte_prop* pp,
+                    const bte_bert_cfg* cfg, bte_bert** out) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    auto* m = reinterpret_cast<const Modes*>(mm);
+    auto* g = reinterpret_cast<const Geom*>(gg);
+    auto* p = reinterpret_cast<const Prop*>(pp);
+    if (!c || !m || !g || !cfg || !out) throw Error(BTE_EINVAL, "null argument");
+    if (!cfg->rta && !p) throw Error(BTE_EINVAL, "a propagator is required unless rta is set");
+    if (p && p->ngamma + p->ncol != m->n) throw Error(BTE_EINVAL, "propagator size does not match the mode table");
+    if (!(cfg->dt > 0) || !(cfg->thickness > 0)) throw Error(BTE_EINVAL, "bad beRTA configuration");
+    if (m->n > BERT_WALL_MAXN) throw Error(BTE_EINVAL, "mode count above the wall-emission limit");
+    HIPCHECK(hipSetDevice(c->device));
+    auto* b = new Bert();
+    std::unique_ptr<Bert> guard(b);
+    b->ctx = c;
+    b->m = m;
+    b->g = g;
+    b->p = cfg->rta ? nullptr : p;
+    b->cfg = *cfg;
+    if (b->cfg.pcancel_each <= 0) b->cfg.pcancel_each = 4;
+    hipStream_t s = c->stream;
+    const int n = m->n, nbox = (int)g->boxes.size();
+    double vuc = m->V * cfg->thickness / m->lattz;
+    // steps: while (time <= maxtime) (beyondRTA_MC2d.cpp:2138)
+    int nst = 0;
+    {
+        double t = 0;
+        while (t <= cfg->maxtime) { nst++; t += cfg->dt; }
+    }
+    b->max_steps = cfg->max_steps > 0 ? std::min(cfg->max_steps, nst) : nst;
+    // Eeff (beyondRTA_MC2d.cpp:2026-2070)
+    if (cfg->Ed < 0.) {
+        double maxT = 0, minT = 1.416784e+32, meanT = 0;
+        int nb_ = 0;
+        for (auto& bx : g->boxes) {
+            meanT += bx.Teq;
+            nb_++;
+            minT = std::min(minT, bx.Teq);
+            maxT = std::max(maxT, bx.Teq);
+        }
+        if (cfg->Tmax != 0) { maxT = cfg->Tmax; minT = cfg->Tmin; }
+        meanT /= nb_;
+        if (cfg->Efactor == 0) throw Error(BTE_EINVAL, "Eeff needs Ed or particles");
+        double e = 1e300;
+        for (auto& bx : g->boxes) {
+            if (bx.kind != 0) continue;
+            e = std::min(e, calc_Eeff_host(*m, maxT, minT, meanT, bx.area() * cfg->thickness / vuc, cfg->Efactor));
+        }
+        b->Eeff = e;
+    } else {
+        b->Eeff = cfg->Ed;
+    }
+    if (!(b->Eeff > 0)) throw Error(BTE_EINVAL, "particle energy Eeff must be positive");
+    // initial particles (outTref per box with Tinit != Teq), counts drawn on
+    // the host like Ntogenerate (sampling.cpp:313-325)
+    Pcg32 hr;
+    hr.seed(cfg->seed, 0xb0a7u);
+    std::vector<BertInit> inits;
+    std::vector<uint2> itab;
+    std::vector<double> pmf(n);
+    std::vector<uint32_t> th(n), al(n);
+    uint64_t total = 0;
+    for (int i = 0; i < nbox; i++) {
+        const RectBox& bx = g->boxes[i];
+        if (bx.kind != 0 || almost_equal(bx.Teq, bx.Tinit)) continue;
+        if (bx.Tinit <= 0 || bx.Teq <= 0) throw Error(BTE_EINVAL, "Invalid temperature value");
+        double acc = 0;
+        for (int k = 0; k < n; k++) {
+            pmf[k] = almost_equal(m->omega[k], 0.) ? 0. :
+                m->omega[k] * std::fabs(h_bose_einstein(m->omega[k], bx.Tinit) - h_bose_einstein(m->omega[k], bx.Teq));
+            acc += pmf[k];
+        }
+        double cumulsum = 1.0e12 * hbar * acc;
+        double gen = (bx.area() * cfg->thickness / vuc) * cumulsum / (b->Eeff * m->nq);
+        uint64_t ng = (uint64_t)gen;
+        if (hr.uniform() <= gen - std::floor(gen)) ng++;
+        build_alias(pmf.data(), n, th.data(), al.data());
+        BertInit I{bx.xmin, bx.xmax, bx.ymin, bx.ymax, total, ng, i, (bx.Tinit - bx.Teq) >= 0 ? 1 : -1,
+                   (int32_t)itab.size(), 0};
+        for (int k = 0; k < n; k++) itab.push_back(make_uint2(th[k], al[k]));
+        inits.push_back(I);
+        total += ng;
+    }
+    b->n0 = total;
+    b->ninit = (int)inits.size();
+    b->cap = cfg->capacity ? cfg->capacity : std::max<uint64_t>(1u << 20, 2 * total + (total >> 1));
+    if (b->cap >= 0xffffffffULL) throw Error(BTE_EINVAL, "capacity above 2^32 particles");
+    // device buffers
+    std::vector<double2> vel(n);
+    for (int k = 0; k < n; k++) vel[k] = make_double2(m->vx[k], m->vy[k]);
+    b->d_vel.upload(vel, s);
+    b->d_parts.alloc(2 * b->cap);
+    b->d_count.alloc(2);
+    b->d_flags.alloc(4);
+    b->d_stats.alloc(2);
+    b->d_H.alloc((size_t)2 * nbox * n);
+    b->d_tickets.alloc((size_t)nbox * n);
+    b->d_Eb.alloc((size_t)2 * nbox * 4);
+    b->d_props.alloc((size_t)(b->max_steps + 1) * nbox * 3);
+    b->d_npart.alloc((size_t)(b->max_steps + 1));
+    b->d_pending.alloc((size_t)(b->max_steps + 1));
+    std::vector<double> Teq(nbox);
+    for (int i = 0; i < nbox; i++) Teq[i] = g->boxes[i].Teq;
+    b->d_Teq.upload(Teq, s);
+    if (!inits.empty()) {
+        b->d_init.upload(inits, s);
+        b->d_init_tab.upload(itab, s);
+    }
+    // RTA samplers per distinct box temperature (ScatteringRTA)
+    std::vector<int32_t> slot(nbox, 0);
+    if (cfg->rta) {
+        std::map<double, int> slots;
+        std::vector<uint2> rt;
+        for (int i = 0; i < nbox; i++) {
+            double T = g->boxes[i].Teq;
+            if (!slots.count(T)) {
+                int id = (int)slots.size();
+                slots[T] = id;
+                for (int k = 0; k < n; k++) pmf[k] = m->omega[k] <= 0. ? 0. : h_be_kernel(m->omega[k], T) * m->w[k];
+                build_alias(pmf.data(), n, th.data(), al.data());
+                for (int k = 0; k < n; k++) rt.push_back(make_uint2(th[k], al[k]));
+            }
+            slot[i] = slots[T];
+        }
+        b->d_rta_alias.upload(rt, s);
+    } else {
+        b->d_rta_alias.alloc(1);
+    }
+    b->d_rta_slot.upload(slot, s);
+    // adiabatic wall splines J(T), T = 10..1000 K (beyondRTA_MC2d.cpp:294-350)
+    std::vector<double> Ts(100);
+    for (int i = 0; i < 100; i++) Ts[i] = 10.0 + 10.0 * i;
+    std::map<std::pair<double, int>, int> spl_id;
+    std::vector<double> spl;
+    std::vector<int32_t> side_spl((size_t)nbox * 4, -1);
+    for (int i = 0; i < nbox; i++) {
+        const RectBox& bx = g->boxes[i];
+        if (bx.kind != 0) continue;
+        for (int sd = 0; sd < 4; sd++) {
+            auto key = std::make_pair(bx.Teq, sd);
+            if (!spl_id.count(key)) {
+                double nx, ny;
+                side_normal(sd, nx, ny);
+                std::vector<double> J(100);
+                for (int t = 0; t < 100; t++) {
+                    double cum = wall_pmf(*m, Ts[t], bx.Teq, -nx, -ny, pmf.data());
+                    int sg = (Ts[t] - bx.Teq) >= 0 ? 1 : -1;
+                    J[t] = sg * cum / (m->nq * vuc);
+                }
+                Spline sp;
+                sp.build(J, Ts);
+                int id = (int)spl_id.size();
+                spl_id[key] = id;
+                for (auto* v : {&sp.a, &sp.b, &sp.c, &sp.d, &sp.x0, &sp.xe}) spl.insert(spl.end(), v->begin(), v->end());
+            }
+            side_spl[(size_t)i * 4 + sd] = spl_id[key];
+        }
+    }
+    b->d_spl.upload(spl.empty() ? std::vector<double>(1, 0.) : spl, s);
+    b->d_side_spl.upload(side_spl, s);
+    // reservoir sources (Isothermal_walls)
+    std::vector<BertSource> srcs;
+    std::vector<uint2> stab;
+    for (int sid = 0; sid < nbox; sid++) {
+        const RectBox& sb = g->boxes[sid];
+        if (sb.kind != 1) continue;
+        for (auto& hc : g->contacts[sid]) {
+            const RectBox& cb = g->boxes[hc.box];
+            if (cb.kind != 0 || almost_equal(hc.length(), 0.) || almost_equal(sb.Teq, cb.Teq)) continue;
+            double cum = wall_pmf(*m, sb.Teq, cb.Teq, hc.npx, hc.npy, pmf.data());
+            double spf = hc.length() * cfg->thickness / vuc;
+            build_alias(pmf.data(), n, th.data(), al.data());
+            BertSource S{hc.p1x, hc.p1y, hc.p2x - hc.p1x, hc.p2y - hc.p1y,
+                         cfg->dt * spf * cum / (b->Eeff * m->nq), hc.box,
+                         (sb.Teq - cb.Teq) >= 0 ? 1 : -1, (int32_t)stab.size(), 0};
+            for (int k = 0; k < n; k++) stab.push_back(make_uint2(th[k], al[k]));
+            srcs.push_back(S);
+        }
+    }
+    b->nsrc = (int)srcs.size();
+    if (b->nsrc) {
+        b->d_src.upload(srcs, s);
+        b->d_src_tab.upload(stab, s);
+    }
+    b->hH.resize((size_t)nbox * n);
+    HIPCHECK(hipEventCreate(&b->ev0));
+    HIPCHECK(hipEventCreate(&b->ev1));
+    HIPCHECK(hipStreamSynchronize(s));
+    *out = reinterpret_cast<bte_bert*>(guard.release());
+    BTE_GUARD_END
+}
+
+int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pending, bte_frame_cb cb,
+                     void* user, bte_bert_report* rep) {
+    BTE_GUARD_BEGIN
+    auto* b = reinterpret_cast<Bert*>(bb);
+    if (!b || !props || !npart) throw Error(BTE_EINVAL, "null argument");
+    Ctx* c = b->ctx;
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int n = b->m->n, nbox = (int)b->g->boxes.size();
+    BertParams P = make_params(b);
+    BPart* buf[2] = {b->d_parts.p, b->d_parts.p + b->cap};
+    if (b->n0 > b->cap) throw Error(BTE_ECAPACITY, "initial particles exceed the capacity");
+    HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)2 * nbox * n * sizeof(int32_t), s));
+    HIPCHECK(hipMemsetAsync(b->d_tickets.p, 0, (size_t)nbox * n * sizeof(uint32_t), s));
+    HIPCHECK(hipMemsetAsync(b->d_Eb.p, 0, (size_t)2 * nbox * 4 * sizeof(int32_t), s));
+    HIPCHECK(hipMemsetAsync(b->d_flags.p, 0, 4 * sizeof(uint32_t), s));
+    HIPCHECK(hipMemsetAsync(b->d_stats.p, 0, 2 * sizeof(unsigned long long), s));
+    HIPCHECK(hipMemsetAsync(b->d_npart.p, 0, (size_t)(b->max_steps + 1) * sizeof(int64_t), s));
+    HIPCHECK(hipMemsetAsync(b->d_pending.p, 0, (size_t)(b->max_steps + 1) * sizeof(int64_t), s));
+    uint32_t c0[2] = {(uint32_t)b->n0, 0u};
+    HIPCHECK(hipMemcpyAsync(b->d_count.p, c0, sizeof(c0), hipMemcpyHostToDevice, s));
+    P.pin = buf[0];
+    P.pout = buf[1];
+    if (b->ninit) {
+        hipLaunchKernelGGL(bert_init_kernel, dim3(b->ninit), dim3(256), 0, s, P, b->d_init.p, b->d_init_tab.p);
+        HIPCHECK(hipGetLastError());
+    }
+    // row 0: properties of the initial particles
+    hipLaunchKernelGGL(bert_hist_kernel, dim3(c->cu_count * 4), dim3(256), 0, s, P, 0);
+    HIPCHECK(hipGetLastError());
+    // props of H[0] into row 0; the kernel clears H[1] and count[0]; restore
+    // count[0] afterwards because here the initial particles are the input
+    hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, 0, 0, 0, 0, 0);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(b->d_count.p, c0, sizeof(c0), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)nbox * n * sizeof(int32_t), s));
+    HIPCHECK(hipMemcpyAsync(b->d_npart.p, &b->n0, sizeof(int64_t), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    const int step_blocks = c->cu_count * 8;
+    const int pc = b->cfg.pcancel_each;
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHECK(hipEventRecord(b->ev0, s));
+    int istep = 0;
+    for (; istep < b->max_steps; istep++) {
+        const int cur = istep & 1;
+        P.pin = buf[cur];
+        P.pout = buf[cur ^ 1];
+        const int cancel_prev = istep > 0 && ((istep - 1) % pc == 0);
+        const int cancel_now = (istep % pc == 0);
+        hipLaunchKernelGGL(bert_step_kernel, dim3(step_blocks), dim3(256), 0, s, P, (uint32_t)istep, cur,
+                           cancel_prev);
+        if (b->nsrc)
+            hipLaunchKernelGGL(bert_source_kernel, dim3(b->nsrc), dim3(256), 0, s, P, b->d_src.p,
+                               b->d_src_tab.p, (uint32_t)istep, cur);
+        hipLaunchKernelGGL(bert_walls_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep, cur);
+        // H of this step lives in H[cur]; H[cur^1] (previous step) is cleared
+        // after the step kernel used it for cancellation
+        hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, istep + 1, cur, cancel_now,
+                           cancel_now, (istep + 1) & 1);
+        HIPCHECK(hipGetLastError());
+        if (cb) {
+            // the histogram of this step is in H[cur] until the next step's
+            // props kernel clears it
+            HIPCHECK(hipMemcpyAsync(b->hH.data(), b->d_H.p + (size_t)cur * nbox * n,
+                                    (size_t)nbox * n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            if (cb(user, istep, (istep + 1) * b->cfg.dt, b->Eeff, nbox, n, b->hH.data())) {
+                istep++;
+                break;
+            }
+        }
+    }
+    HIPCHECK(hipEventRecord(b->ev1, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    auto t1 = std::chrono::steady_clock::now();
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    uint32_t flags[4];
+    unsigned long long stats[2];
+    HIPCHECK(hipMemcpy(flags, b->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(stats, b->d_stats.p, sizeof(stats), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(props, b->d_props.p, (size_t)(istep + 1) * nbox * 3 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(npart, b->d_npart.p, (size_t)(istep + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (pending)
+        HIPCHECK(hipMemcpy(pending, b->d_pending.p, (size_t)(istep + 1) * sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (flags[0]) throw Error(BTE_ECAPACITY, "particle buffer capacity exceeded; raise capacity");
+    if (flags[1]) throw Error(BTE_EINVAL, "a particle reached a mode outside the propagator");
+    if (flags[2]) throw Error(BTE_EGEOM, "advection or branching exceeded its iteration bound");
+    if (rep) {
+        rep->nsteps = istep;
+        rep->Eeff = b->Eeff;
+        rep->particle_steps = stats[0];
+        int64_t mx = 0;
+        for (int i = 0; i <= istep; i++) mx = std::max(mx, npart[i]);
+        rep->max_particles = mx;
+        rep->loop_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        rep->step_kernel_ms = ms;
+    }
+    BTE_GUARD_END
+}
+
+int bte_bert_destroy(bte_bert* bb) {
+    BTE_GUARD_BEGIN
+    delete reinterpret_cast<Bert*>(bb);
+    BTE_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// Device-side building blocks shared by the MC kernels: PCG32 streams,
+// alias sampling, rectangle geometry and Bose-Einstein helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bte {
+
+constexpr double kB = 1.3806488e-23;      // include/constants.hpp:37
+constexpr double hbar = 1.054571726e-34;  // include/constants.hpp:47
+
+// PCG32 (setseq_xsh_rr_64_32): one stream per particle, state in registers.
+struct Pcg32 {
+    uint64_t state, inc;
+    __device__ __host__ inline void seed(uint64_t initstate, uint64_t initseq) {
+        inc = (initseq << 1u) | 1u;
+        state = (initstate + inc) * 6364136223846793005ULL + inc;
+    }
+    __device__ __host__ inline uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    // 53-bit uniform in [0,1)
+    __device__ __host__ inline double uniform() {
+        uint64_t a = next() >> 5, b = next() >> 6;
+        return (double)((a << 26) | b) * (1.0 / 9007199254740992.0);
+    }
+    // exact uniform integer in [0, bound) (multiply-shift with rejection)
+    __device__ __host__ inline uint32_t bounded(uint32_t bound) {
+        uint64_t m = (uint64_t)next() * bound;
+        uint32_t l = (uint32_t)m;
+        if (l < bound) {
+            uint32_t t = (0u - bound) % bound;
+            while (l < t) {
+                m = (uint64_t)next() * bound;
+                l = (uint32_t)m;
+            }
+        }
+        return (uint32_t)(m >> 32);
+    }
+};
+
+// splitmix64 finaliser: derives per-(particle, step) seeds
+__device__ __host__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// Alias table entry: accept column k with probability thresh/2^32, else alias.
+// For propagator columns bit 31 of .y flags a negative own entry, bit 30 a
+// negative alias entry; the index uses the low 30 bits.
+__device__ inline uint32_t alias_pick(const uint2* tab, uint32_t n, Pcg32& r, bool& neg) {
+    uint32_t k = r.bounded(n);
+    uint2 e = tab[k];
+    bool own = r.next() < e.x;
+    neg = own ? ((e.y >> 31) & 1u) : ((e.y >> 30) & 1u);
+    return own ? k : (e.y & 0x3fffffffu);
+}
+
+// include/utilities.hpp:214-223
+__device__ __host__ inline bool almost_equal(double a, double b, double abstol = 1e-8,
+                                             double reltol = 1e-5) {
+    return fabs(a - b) < fabs(abstol) + fabs(reltol * b);
+}
+
+__device__ inline double bose_einstein(double omega, double T) {
+    constexpr double pf = 1e12 * hbar / kB;
+    return 1. / (exp(pf * omega / T) - 1.);
+}
+
+// Rectangle box. Sides in the reference's border order for vertex order
+// (xmin,ymin),(xmin,ymax),(xmax,ymax),(xmax,ymin): 0 left, 1 bottom, 2 top,
+// 3 right; outward normals (-1,0),(0,-1),(0,1),(1,0).
+struct DevBox {
+    double xmin, xmax, ymin, ymax;
+    int32_t kind;      // 0 regular, 1 reservoir
+    int32_t c0, c1;    // contacts [c0, c1)
+    int32_t pad;
+};
+struct DevContact {
+    int32_t box;
+    int32_t pad;
+    double npx, npy;   // contact border normal, pointing out of the owner
+};
+
+__device__ __host__ inline bool box_inside(const DevBox& b, double x, double y) {
+    return x >= b.xmin && x <= b.xmax && y >= b.ymin && y <= b.ymax;
+}
+__device__ __host__ inline void side_normal(int s, double& nx, double& ny) {
+    nx = s == 0 ? -1. : (s == 3 ? 1. : 0.);
+    ny = s == 1 ? -1. : (s == 2 ? 1. : 0.);
+}
+
+// Exit of the ray r + t v from box b: time of the first side crossed with
+// n.v > 0 (geometry_2d.cpp:160-293 for rectangles). side_mask gets bit s for
+// every side reached at that time (corner ties set two bits).
+__device__ __host__ inline double box_exit(const DevBox& b, double x, double y, double vx,
+                                           double vy, int& side_mask, double& ex, double& ey) {
+    double tx = 1e300, ty = 1e300;
+    int sx = -1, sy = -1;
+    if (vx > 0.) { tx = (b.xmax - x) / vx; sx = 3; }
+    else if (vx < 0.) { tx = (b.xmin - x) / vx; sx = 0; }
+    if (vy > 0.) { ty = (b.ymax - y) / vy; sy = 2; }
+    else if (vy < 0.) { ty = (b.ymin - y) / vy; sy = 1; }
+    if (tx < 0.) tx = 0.;
+    if (ty < 0.) ty = 0.;
+    double t;
+    if (tx < ty) {
+        t = tx;
+        side_mask = 1 << sx;
+        ex = sx == 3 ? b.xmax : b.xmin;
+        ey = y + t * vy;
+    } else if (ty < tx) {
+        t = ty;
+        side_mask = 1 << sy;
+        ex = x + t * vx;
+        ey = sy == 2 ? b.ymax : b.ymin;
+    } else {
+        t = tx;
+        side_mask = (1 << sx) | (1 << sy);
+        ex = sx == 3 ? b.xmax : b.xmin;
+        ey = sy == 2 ? b.ymax : b.ymin;
+    }
+    if (almost_equal(t, 0.)) t = 0.;
+    return t;
+}
+
+// Neighbour search at an exit point (RTA_MC2D.cpp:1130-1163): contacts c
+// whose closed box holds rnew or rnew + 1e-6 v/|v| with n_c.v > 0 and not
+// parallel. Returns the count (up to 4 ids stored).
+__device__ __host__ inline int find_pboxes(const DevBox* boxes, const DevContact* cts,
+                                           const DevBox& cur, double rx, double ry, double vx,
+                                           double vy, int* out) {
+    double vn = sqrt(vx * vx + vy * vy);
+    double ex = rx + 1.0e-6 * vx / vn, ey = ry + 1.0e-6 * vy / vn;
+    int n = 0;
+    for (int c = cur.c0; c < cur.c1; c++) {
+        const DevContact ct = cts[c];
+        double d = ct.npx * vx + ct.npy * vy;
+        if (!(d > 0.) || almost_equal(d, 0.)) continue;
+        const DevBox nb = boxes[ct.box];
+        if (box_inside(nb, rx, ry) || box_inside(nb, ex, ey)) {
+            if (n < 4) out[n] = ct.box;
+            n++;
+        }
+    }
+    return n;
+}
+
+// Void-side choice among tied sides (RTA_MC2D.cpp:1205-1236): a side is void
+// when no contact box holds the point moved 1e-4 along its normal.
+__device__ __host__ inline int pick_void_side(const DevBox* boxes, const DevContact* cts,
+                                              const DevBox& cur, int mask, double rx, double ry,
+                                              uint32_t rnd) {
+    int cand[2], nc = 0, first = -1;
+    for (int s = 0; s < 4; s++) {
+        if (!((mask >> s) & 1)) continue;
+        if (first < 0) first = s;
+        double nx, ny;
+        side_normal(s, nx, ny);
+        double cx = rx + 1.0e-4 * nx, cy = ry + 1.0e-4 * ny;
+        bool isvoid = true;
+        for (int c = cur.c0; c < cur.c1; c++)
+            if (box_inside(boxes[cts[c].box], cx, cy)) { isvoid = false; break; }
+        if (isvoid && nc < 2) cand[nc++] = s;
+    }
+    if (nc == 0) return first;
+    return cand[nc == 1 ? 0 : (rnd & 1u)];
+}
+
+}  // namespace bte

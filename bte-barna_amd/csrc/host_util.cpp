@@ -1,0 +1,393 @@
+// Host side of the C ABI: errors, context, material and geometry handles,
+// synthetic inputs, alias tables and the small host-side physics used to
+// build device tables.
+#include "runtime.hpp"
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace bte {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& s) { g_last_error = s; }
+
+double h_bose_einstein(double omega, double T) {
+    constexpr double pf = 1e12 * hbar / kB;
+    return 1. / (std::exp(pf * omega / T) - 1.);
+}
+// include/utilities.hpp:401-415
+double h_be_kernel(double omega, double T) {
+    constexpr double pf = 0.5 * 1e12 * hbar / kB;
+    double x = pf * omega / T;
+    if (x < 1e-320) return 1.;
+    if (x > 300.) return 0.;
+    double s = x / std::sinh(x);
+    return s * s;
+}
+// include/bulk_properties.hpp:32-47
+double h_calc_cv(const Modes& m, double T) {
+    double acc = 0;
+    for (int i = 0; i < m.n; i++) acc += h_be_kernel(m.omega[i], T);
+    return acc * kB / m.nq / m.V;
+}
+
+bool build_alias(const double* pmf, int n, uint32_t* thresh, uint32_t* alias) {
+    double tot = 0;
+    for (int i = 0; i < n; i++) {
+        if (!(pmf[i] >= 0.)) throw Error(BTE_EINVAL, "probabilities must be >= 0.");
+        tot += pmf[i];
+    }
+    if (!(tot > 0.)) {
+        for (int i = 0; i < n; i++) { thresh[i] = 0xffffffffu; alias[i] = (uint32_t)i; }
+        return false;
+    }
+    std::vector<double> q(n);
+    std::vector<int> small, large;
+    small.reserve(n);
+    large.reserve(n);
+    for (int i = 0; i < n; i++) {
+        q[i] = pmf[i] * n / tot;
+        (q[i] < 1.0 ? small : large).push_back(i);
+    }
+    for (int i = 0; i < n; i++) alias[i] = (uint32_t)i;
+    while (!small.empty() && !large.empty()) {
+        int s = small.back();
+        small.pop_back();
+        int l = large.back();
+        double qs = q[s];
+        double t = std::ldexp(qs, 32);
+        thresh[s] = t >= 4294967295.0 ? 0xffffffffu : (uint32_t)std::llround(t);
+        alias[s] = (uint32_t)l;
+        q[l] = (q[l] + qs) - 1.0;
+        if (q[l] < 1.0) {
+            large.pop_back();
+            small.push_back(l);
+        }
+    }
+    for (int l : large) { thresh[l] = 0xffffffffu; alias[l] = (uint32_t)l; }
+    for (int s : small) { thresh[s] = 0xffffffffu; alias[s] = (uint32_t)s; }
+    return true;
+}
+
+// Isothermal_wall_distribution pmf (sampling.cpp:189-269); normal points into
+// the box the particles are emitted into.
+double wall_pmf(const Modes& m, double Twall, double Teq, double nx, double ny, double* pmf) {
+    double acc = 0;
+    for (int i = 0; i < m.n; i++) {
+        double vn = nx * m.vx[i] + ny * m.vy[i];
+        if (vn <= 0. || almost_equal(0., m.omega[i]))
+            pmf[i] = 0.;
+        else
+            pmf[i] = vn * m.omega[i] *
+                     std::fabs(h_bose_einstein(m.omega[i], Twall) - h_bose_einstein(m.omega[i], Teq));
+        acc += pmf[i];
+    }
+    return 1.0e12 * hbar * acc;
+}
+
+void Spline::build(const std::vector<double>& X, const std::vector<double>& Y) {
+    int n = (int)Y.size() - 1;
+    std::vector<double> A(Y), B(n), D(n), h(n), alpha(n, 0.), C(n + 1), l(n + 1), mu(n + 1), z(n + 1);
+    for (int i = 0; i < n; i++) h[i] = X[i + 1] - X[i];
+    for (int i = 1; i < n; i++)
+        alpha[i] = 3. / h[i] * (A[i + 1] - A[i]) - 3. / h[i - 1] * (A[i] - A[i - 1]);
+    l[0] = 1; mu[0] = 0; z[0] = 0;
+    for (int i = 1; i < n; i++) {
+        l[i] = 2 * (X[i + 1] - X[i - 1]) - h[i - 1] * mu[i - 1];
+        mu[i] = h[i] / l[i];
+        z[i] = (alpha[i] - h[i - 1] * z[i - 1]) / l[i];
+    }
+    l[n] = 1; z[n] = 0; C[n] = 0;
+    for (int j = n - 1; j >= 0; j--) {
+        C[j] = z[j] - mu[j] * C[j + 1];
+        B[j] = (A[j + 1] - A[j]) / h[j] - h[j] * (C[j + 1] + 2 * C[j]) / 3;
+        D[j] = (C[j + 1] - C[j]) / (3 * h[j]);
+    }
+    a.assign(A.begin(), A.begin() + n);
+    b = B;
+    c.assign(C.begin(), C.begin() + n);
+    d = D;
+    x0.assign(X.begin(), X.begin() + n);
+    xe.assign(X.begin() + 1, X.end());
+}
+double Spline::eval(double x) const {
+    size_t idx = std::lower_bound(xe.begin(), xe.end(), x) - xe.begin();
+    if (idx >= a.size()) idx--;
+    double dx = x - x0[idx];
+    return a[idx] + dx * (b[idx] + dx * (c[idx] + d[idx] * dx));
+}
+
+}  // namespace bte
+
+using namespace bte;
+
+extern "C" {
+
+int bte_last_error(char* buf, size_t len) {
+    if (!buf || len == 0) return BTE_EINVAL;
+    std::strncpy(buf, g_last_error.c_str(), len - 1);
+    buf[len - 1] = 0;
+    return BTE_OK;
+}
+
+int bte_version(void) { return 100; }
+
+int bte_ctx_create(int device, bte_ctx** out) {
+    BTE_GUARD_BEGIN
+    if (!out) throw Error(BTE_EINVAL, "null output");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        throw Error(BTE_ENODEV, "no HIP device visible");
+    if (device < 0 || device >= ndev) throw Error(BTE_EINVAL, "device index out of range");
+    HIPCHECK(hipSetDevice(device));
+    auto* c = new Ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    HIPCHECK(hipGetDeviceProperties(&prop, device));
+    c->cu_count = prop.multiProcessorCount;
+    c->lds_per_block = prop.sharedMemPerBlock;
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    *out = reinterpret_cast<bte_ctx*>(c);
+    BTE_GUARD_END
+}
+
+int bte_ctx_destroy(bte_ctx* ctx) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c) return BTE_OK;
+    HIPCHECK(hipSetDevice(c->device));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipStreamDestroy(c->stream));
+    delete c;
+    BTE_GUARD_END
+}
+
+int bte_sync(bte_ctx* ctx) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    BTE_GUARD_END
+}
+
+int bte_synthetic_modes(int n, uint64_t seed, double fmin, double fmax, double vmin, double vmax,
+                        double tmin, double tmax, int mirror, double* omega, double* vx,
+                        double* vy, double* w) {
+    BTE_GUARD_BEGIN
+    if (n <= 0 || !omega || !vx || !vy || !w) throw Error(BTE_EINVAL, "bad arguments");
+    if (mirror && (n % 2)) throw Error(BTE_EINVAL, "mirror tables need an even mode count");
+    Pcg32 r;
+    r.seed(seed, 0x6d6f6465u);
+    const double PI = 3.14159265358979323846;
+    int nbase = mirror ? n / 2 : n;
+    double l0 = std::log10(tmin), l1 = std::log10(tmax);
+    for (int i = 0; i < nbase; i++) {
+        double f = fmin + (fmax - fmin) * r.uniform();
+        double sp = vmin + (vmax - vmin) * r.uniform();
+        double th = 2.0 * PI * r.uniform();
+        double tau = std::pow(10.0, l0 + (l1 - l0) * r.uniform());
+        omega[i] = 2.0 * PI * f;
+        vx[i] = sp * std::cos(th);
+        vy[i] = sp * std::sin(th);
+        w[i] = 1.0 / tau;
+    }
+    if (mirror)
+        for (int i = 0; i < nbase; i++) {
+            omega[nbase + i] = omega[i];
+            vx[nbase + i] = vx[i];
+            vy[nbase + i] = -vy[i];
+            w[nbase + i] = w[i];
+        }
+    BTE_GUARD_END
+}
+
+int bte_synthetic_B(int N, const double* C, const double* w, double density, uint64_t seed,
+                    int iters, double* B) {
+    BTE_GUARD_BEGIN
+    if (N <= 1 || !C || !w || !B || !(density > 0. && density <= 1.))
+        throw Error(BTE_EINVAL, "bad arguments");
+    Pcg32 rng;
+    rng.seed(seed, 0x6f6du);
+    std::vector<int> ri, cj;
+    std::vector<double> val;
+    uint32_t thr = (uint32_t)(density * 4294967296.0);
+    for (int j = 1; j < N; j++)
+        for (int i = 0; i < j; i++) {
+            uint32_t r = rng.next();
+            if (r < thr) {
+                ri.push_back(i);
+                cj.push_back(j);
+                val.push_back((rng.next() >> 8) * (1.0 / 16777216.0) + 1.0 / 33554432.0);
+            }
+        }
+    std::vector<double> d(N, 1.0), rs(N);
+    for (int it = 0; it < iters; it++) {
+        std::fill(rs.begin(), rs.end(), 0.);
+        for (size_t e = 0; e < val.size(); e++) {
+            rs[ri[e]] += val[e] * d[cj[e]];
+            rs[cj[e]] += val[e] * d[ri[e]];
+        }
+        // stop at the last finite scaling when the pattern admits no exact one
+        std::vector<double> nd(N);
+        bool ok = true;
+        for (int i = 0; i < N; i++) {
+            nd[i] = std::sqrt(d[i] * C[i] * w[i] / (rs[i] > 0 ? rs[i] : 1e-300));
+            if (!(nd[i] > 0. && nd[i] < 1e300)) ok = false;
+        }
+        if (!ok) break;
+        d.swap(nd);
+    }
+    std::fill(B, B + (size_t)N * N, 0.);
+    std::vector<double> srow(N, 0.);
+    for (size_t e = 0; e < val.size(); e++) {
+        double s = d[ri[e]] * val[e] * d[cj[e]];
+        B[ri[e] + (size_t)cj[e] * N] = s / C[cj[e]];
+        B[cj[e] + (size_t)ri[e] * N] = s / C[ri[e]];
+        srow[ri[e]] += s;
+        srow[cj[e]] += s;
+    }
+    for (int j = 0; j < N; j++) B[j + (size_t)j * N] = -srow[j] / C[j];
+    BTE_GUARD_END
+}
+
+int bte_alias_build(int n, const double* pmf, uint32_t* thresh, uint32_t* alias) {
+    BTE_GUARD_BEGIN
+    if (n <= 0 || !pmf || !thresh || !alias) throw Error(BTE_EINVAL, "bad arguments");
+    if (!build_alias(pmf, n, thresh, alias)) throw Error(BTE_EINVAL, "pmf sums to zero");
+    BTE_GUARD_END
+}
+
+int bte_modes_create(bte_ctx* ctx, int n, int nq, const double* omega, const double* vx,
+                     const double* vy, const double* w, double V, double lattz, bte_modes** out) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !out || n <= 0 || nq <= 0 || n % nq || !omega || !vx || !vy || !w || !(V > 0) ||
+        !(lattz > 0))
+        throw Error(BTE_EINVAL, "bad arguments to bte_modes_create");
+    if (n >= (1 << 30)) throw Error(BTE_EINVAL, "too many modes");
+    auto* m = new Modes();
+    m->ctx = c;
+    m->n = n;
+    m->nq = nq;
+    m->omega.assign(omega, omega + n);
+    m->vx.assign(vx, vx + n);
+    m->vy.assign(vy, vy + n);
+    m->w.assign(w, w + n);
+    // get_v cleaning (RTA_MC2D.cpp:843-856): components almost_equal to 0 are 0
+    for (int i = 0; i < n; i++) {
+        if (almost_equal(m->vx[i], 0.)) m->vx[i] = 0.;
+        if (almost_equal(m->vy[i], 0.)) m->vy[i] = 0.;
+    }
+    m->V = V;
+    m->lattz = lattz;
+    HIPCHECK(hipSetDevice(c->device));
+    m->d_vx.upload(m->vx, c->stream);
+    m->d_vy.upload(m->vy, c->stream);
+    m->d_w.upload(m->w, c->stream);
+    m->d_omega.upload(m->omega, c->stream);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    *out = reinterpret_cast<bte_modes*>(m);
+    BTE_GUARD_END
+}
+
+int bte_modes_destroy(bte_modes* mm) {
+    BTE_GUARD_BEGIN
+    delete reinterpret_cast<Modes*>(mm);
+    BTE_GUARD_END
+}
+
+int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int32_t* kind,
+                          const double* Teq, const double* Tinit, bte_geom** out) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !out || nbox <= 0 || !rect || !kind || !Teq) throw Error(BTE_EINVAL, "bad arguments");
+    auto* g = new Geom();
+    g->ctx = c;
+    g->boxes.resize(nbox);
+    for (int i = 0; i < nbox; i++) {
+        RectBox& b = g->boxes[i];
+        b.xmin = rect[4 * i]; b.xmax = rect[4 * i + 1]; b.ymin = rect[4 * i + 2]; b.ymax = rect[4 * i + 3];
+        if (!(b.xmax > b.xmin && b.ymax > b.ymin)) { delete g; throw Error(BTE_EGEOM, "degenerate box"); }
+        if (kind[i] == 2) { delete g; throw Error(BTE_EGEOM, "periodic boxes are not supported"); }
+        if (kind[i] != 0 && kind[i] != 1) { delete g; throw Error(BTE_EINVAL, "bad box kind"); }
+        b.kind = kind[i];
+        b.Teq = Teq[i];
+        b.Tinit = Tinit ? Tinit[i] : Teq[i];
+    }
+    // calculate_contacts (geometry_2d.cpp:295-381) for rectangles
+    g->contacts.assign(nbox, {});
+    for (int i = 0; i < nbox; i++) {
+        const RectBox& a = g->boxes[i];
+        for (int j = 0; j < nbox; j++) {
+            if (j == i) continue;
+            const RectBox& e = g->boxes[j];
+            double x0 = std::max(a.xmin, e.xmin), x1 = std::min(a.xmax, e.xmax);
+            double y0 = std::max(a.ymin, e.ymin), y1 = std::min(a.ymax, e.ymax);
+            if (x0 > x1 || y0 > y1) continue;
+            HostContact hc{j, 0, 0, x0, y0, x1, y1};
+            if (x0 == x1 && y0 == y1) {
+                double nx = x0 - a.cx(), ny = y0 - a.cy(), nn = std::hypot(nx, ny);
+                hc.npx = nx / nn;
+                hc.npy = ny / nn;
+            } else if (x0 != x1 && y0 != y1) {
+                delete g;
+                throw Error(BTE_EGEOM, "overlapping boxes");
+            } else {
+                double ndx = x1 - x0, ndy = y1 - y0;
+                double mx = 0.5 * (x0 + x1) - a.cx(), my = 0.5 * (y0 + y1) - a.cy();
+                double nx = ndy, ny = -ndx;
+                if (nx * mx + ny * my < 0.) { nx = -nx; ny = -ny; }
+                double nn = std::hypot(nx, ny);
+                hc.npx = nx / nn;
+                hc.npy = ny / nn;
+            }
+            g->contacts[i].push_back(hc);
+        }
+        if (g->contacts[i].empty() && nbox > 1) {
+            delete g;
+            throw Error(BTE_EGEOM, "There are unconnected boxes");
+        }
+    }
+    g->dboxes.resize(nbox);
+    for (int i = 0; i < nbox; i++) {
+        DevBox& d = g->dboxes[i];
+        const RectBox& b = g->boxes[i];
+        d.xmin = b.xmin; d.xmax = b.xmax; d.ymin = b.ymin; d.ymax = b.ymax;
+        d.kind = b.kind;
+        d.c0 = (int32_t)g->dcontacts.size();
+        for (auto& hc : g->contacts[i]) g->dcontacts.push_back(DevContact{hc.box, 0, hc.npx, hc.npy});
+        d.c1 = (int32_t)g->dcontacts.size();
+        d.pad = 0;
+    }
+    HIPCHECK(hipSetDevice(c->device));
+    g->d_boxes.upload(g->dboxes, c->stream);
+    g->d_contacts.upload(g->dcontacts, c->stream);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    *out = reinterpret_cast<bte_geom*>(g);
+    BTE_GUARD_END
+}
+
+int bte_geom_destroy(bte_geom* gg) {
+    BTE_GUARD_BEGIN
+    delete reinterpret_cast<Geom*>(gg);
+    BTE_GUARD_END
+}
+
+int bte_geom_contacts(const bte_geom* gg, int maxc, int32_t* ncont, int32_t* nbr, double* geo) {
+    BTE_GUARD_BEGIN
+    auto* g = reinterpret_cast<const Geom*>(gg);
+    if (!g || maxc <= 0 || !ncont || !nbr || !geo) throw Error(BTE_EINVAL, "bad arguments");
+    for (size_t i = 0; i < g->boxes.size(); i++) {
+        int k = 0;
+        for (auto& hc : g->contacts[i]) {
+            if (k >= maxc) break;
+            nbr[i * maxc + k] = hc.box;
+            double* q = geo + (i * maxc + k) * 6;
+            q[0] = hc.npx; q[1] = hc.npy; q[2] = hc.p1x; q[3] = hc.p1y; q[4] = hc.p2x; q[5] = hc.p2y;
+            k++;
+        }
+        ncont[i] = k;
+    }
+    BTE_GUARD_END
+}
+
+}  // extern "C"

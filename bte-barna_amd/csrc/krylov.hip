@@ -1,0 +1,422 @@
+// Krylov propagator P = exp(dt*B) for all columns at once.
+//
+// Reference: build_P (src/collision_operator.cpp:1015-1064) runs one
+// Arnoldi_algorithm (815-851) per column with b = e_i and keeps
+// P(:,i) = (V exp(H))(:,0). Here every column advances together: step k is
+// one N x N x N GEMM Z = dt*B*V_k (rocBLAS, FP64 / FP32 / BF16 hi+lo split
+// with FP32 accumulation), followed by a per-column FP64 modified
+// Gram-Schmidt kernel, a per-column FP64 exponential of the small Hessenberg
+// matrix and a combine kernel that forms P(:,i) and applies the FP64 column
+// energy projection (collision_operator.cpp:786-807 applied to P: columns
+// sum to one).
+#include "runtime.hpp"
+#include <rocblas/rocblas.h>
+#include <hip/hip_bf16.h>
+#include <chrono>
+#include <cstring>
+
+namespace bte {
+
+#define RBCHECK(x)                                                                    \
+    do {                                                                              \
+        rocblas_status s_ = (x);                                                      \
+        if (s_ != rocblas_status_success)                                             \
+            throw ::bte::Error(BTE_EHIP, std::string(#x) + ": rocblas status " +       \
+                                             std::to_string((int)s_));                \
+    } while (0)
+
+// FP64 -> compute format
+__global__ void kr_to_f32(const double* __restrict__ a, float* __restrict__ o, size_t n, double scale) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = (float)(a[i] * scale);
+}
+__global__ void kr_to_bf16x2(const double* __restrict__ a, __hip_bfloat16* __restrict__ hi,
+                             __hip_bfloat16* __restrict__ lo, size_t n, double scale) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        float v = (float)(a[i] * scale);
+        __hip_bfloat16 h = __float2bfloat16(v);
+        hi[i] = h;
+        lo[i] = __float2bfloat16(v - __bfloat162float(h));
+    }
+}
+__global__ void kr_scale64(const double* __restrict__ a, double* __restrict__ o, size_t n, double scale) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = a[i] * scale;
+}
+__global__ void kr_f32_to_f64(const float* __restrict__ a, double* __restrict__ o, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        o[i] = (double)a[i];
+}
+
+__device__ inline double block_sum(double v, double* red) {
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+    const int nw = blockDim.x >> 6;
+    for (int q = 0; q < nw; q++) s += red[q];
+    return s;
+}
+
+// Basis storage: FP64 (prec 0), FP32 (1) or BF16 hi/lo (2); column i of V_j
+// starts at (j*N + i)*N.
+struct Basis {
+    double* f64;
+    float* f32;
+    __hip_bfloat16* hi;
+    __hip_bfloat16* lo;
+    int prec;
+    __device__ inline double load(size_t idx) const {
+        if (prec == 0) return f64[idx];
+        if (prec == 1) return (double)f32[idx];
+        return (double)__bfloat162float(hi[idx]) + (double)__bfloat162float(lo[idx]);
+    }
+    __device__ inline void store(size_t idx, double v) const {
+        if (prec == 0) { f64[idx] = v; return; }
+        if (prec == 1) { f32[idx] = (float)v; return; }
+        float f = (float)v;
+        __hip_bfloat16 h = __float2bfloat16(f);
+        hi[idx] = h;
+        lo[idx] = __float2bfloat16(f - __bfloat162float(h));
+    }
+};
+
+// Modified Gram-Schmidt step k for every column (Arnoldi_algorithm,
+// collision_operator.cpp:831-850). z comes from the GEMM (FP32 or FP64), the
+// projections run in FP64. V_0 = I is implicit.
+template <int E>
+__global__ __launch_bounds__(256) void kr_mgs(const float* __restrict__ Z32, const double* __restrict__ Z64,
+                                              Basis V, double* __restrict__ H, int* __restrict__ closed,
+                                              int N, int m, int k, int textbook) {
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    if (closed[i]) return;
+    double z[E];
+    const size_t zc = (size_t)i * N;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        int r = threadIdx.x + 256 * e;
+        z[e] = r < N ? (Z64 ? Z64[zc + r] : (double)Z32[zc + r]) : 0.;
+    }
+    double* Hi = H + (size_t)i * m * m;  // column-major m x m
+    const int jmax = textbook ? k + 1 : k;
+    for (int j = 0; j < jmax; j++) {
+        double d = 0;
+        if (j == 0) {
+            // V_0(:,i) = e_i
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                if (threadIdx.x + 256 * e == i) d = z[e];
+        } else {
+            const size_t vc = ((size_t)j * N + i) * N;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                int r = threadIdx.x + 256 * e;
+                if (r < N) d += V.load(vc + r) * z[e];
+            }
+        }
+        double h = block_sum(d, red);
+        if (threadIdx.x == 0) Hi[j + (size_t)k * m] = h;
+        if (j == 0) {
+#pragma unroll
+            for (int e = 0; e < E; e++)
+                if (threadIdx.x + 256 * e == i) z[e] -= h;
+        } else {
+            const size_t vc = ((size_t)j * N + i) * N;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                int r = threadIdx.x + 256 * e;
+                if (r < N) z[e] -= h * V.load(vc + r);
+            }
+        }
+    }
+    double nn = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) nn += z[e] * z[e];
+    double zn = sqrt(block_sum(nn, red));
+    if (almost_equal(zn, 0., 1.0e-12, 1.0e-9)) {
+        if (threadIdx.x == 0) closed[i] = 1;
+        return;
+    }
+    if (k + 1 < m) {
+        if (threadIdx.x == 0) Hi[(k + 1) + (size_t)k * m] = zn;
+        const size_t vc = ((size_t)(k + 1) * N + i) * N;
+        const double inv = 1.0 / zn;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            int r = threadIdx.x + 256 * e;
+            if (r < N) V.store(vc + r, z[e] * inv);
+        }
+    }
+}
+
+// c = exp(H_i) e_1 for every column: one wave per column, H in LDS;
+// exp(H) e_1 = (exp(H/2^s))^(2^s) e_1 with each factor a Taylor polynomial
+// of degree <= 24 applied to the vector (||H/2^s||_1 <= 1/2).
+__global__ __launch_bounds__(64) void kr_expm_e1(const double* __restrict__ H, double* __restrict__ C,
+                                                 int m) {
+    extern __shared__ double sh[];
+    double* Hs = sh;            // m*m
+    double* v = sh + m * m;     // m
+    double* t = v + m;          // m
+    const int i = blockIdx.x, l = threadIdx.x;
+    const double* Hi = H + (size_t)i * m * m;
+    for (int q = l; q < m * m; q += 64) Hs[q] = Hi[q];
+    __syncthreads();
+    // 1-norm
+    double cs = 0;
+    if (l < m)
+        for (int r = 0; r < m; r++) cs += fabs(Hs[r + l * m]);
+    for (int d = 32; d > 0; d >>= 1) cs = fmax(cs, __shfl_xor(cs, d, 64));
+    // a non-finite or huge Hessenberg norm yields NaN instead of an unbounded loop
+    if (!(cs < 8.0e6)) {
+        if (l < m) C[(size_t)i * m + l] = __builtin_nan("");
+        return;
+    }
+    int s = 0;
+    if (cs > 0.5) s = (int)ceil(log2(cs / 0.5));
+    const double sc = ldexp(1.0, -s);
+    for (int q = l; q < m * m; q += 64) Hs[q] *= sc;
+    if (l < m) v[l] = l == 0 ? 1.0 : 0.0;
+    __syncthreads();
+    const long reps = 1L << s;
+    for (long rp = 0; rp < reps; rp++) {
+        double acc = l < m ? v[l] : 0.;
+        double term = acc;
+        for (int kk = 1; kk <= 24; kk++) {
+            if (l < m) t[l] = term;
+            __syncthreads();
+            double nt = 0;
+            if (l < m)
+                for (int c = 0; c < m; c++) nt += Hs[l + c * m] * t[c];
+            nt /= kk;
+            __syncthreads();
+            term = nt;
+            acc += nt;
+            double mag = fabs(nt);
+            for (int d = 32; d > 0; d >>= 1) mag = fmax(mag, __shfl_xor(mag, d, 64));
+            if (mag < 1e-18) break;
+        }
+        if (l < m) v[l] = acc;
+        __syncthreads();
+    }
+    if (l < m) C[(size_t)i * m + l] = v[l];
+}
+
+// P(:,i) = sum_k c_k V_k(:,i), then the column energy projection
+// P(:,i) += (1 - sum_r P(r,i)) / N. colerr gets |sum - 1| before projection.
+template <int E>
+__global__ __launch_bounds__(256) void kr_combine(Basis V, const double* __restrict__ C,
+                                                  double* __restrict__ P, double* __restrict__ colerr,
+                                                  int N, int m, int project) {
+    __shared__ double red[4];
+    const int i = blockIdx.x;
+    const double* c = C + (size_t)i * m;
+    double acc[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        int r = threadIdx.x + 256 * e;
+        acc[e] = r == i ? c[0] : 0.;
+    }
+    for (int k = 1; k < m; k++) {
+        const double ck = c[k];
+        if (ck == 0.) continue;
+        const size_t vc = ((size_t)k * N + i) * N;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            int r = threadIdx.x + 256 * e;
+            if (r < N) acc[e] += ck * V.load(vc + r);
+        }
+    }
+    double s = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) s += acc[e];
+    double tot = block_sum(s, red);
+    if (threadIdx.x == 0) colerr[i] = fabs(tot - 1.0);
+    const double corr = project ? (1.0 - tot) / N : 0.;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        int r = threadIdx.x + 256 * e;
+        if (r < N) P[(size_t)i * N + r] = acc[e] + corr;
+    }
+}
+
+template <int E>
+static void launch_mgs(hipStream_t s, int N, const float* z32, const double* z64, Basis V, double* H,
+                       int* closed, int m, int k, int tb) {
+    hipLaunchKernelGGL((kr_mgs<E>), dim3(N), dim3(256), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+}
+template <int E>
+static void launch_combine(hipStream_t s, int N, Basis V, const double* C, double* P, double* err, int m) {
+    hipLaunchKernelGGL((kr_combine<E>), dim3(N), dim3(256), 0, s, V, C, P, err, N, m, 1);
+}
+
+static int elems_for(int N) {
+    if (N <= 1024) return 4;
+    if (N <= 2048) return 8;
+    if (N <= 4096) return 16;
+    if (N <= 8192) return 32;
+    if (N <= 16384) return 64;
+    return -1;
+}
+
+}  // namespace bte
+
+using namespace bte;
+
+extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt, int m, int prec,
+                                int mode, double* P, bte_krylov_report* rep) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !B || !P || N < 2 || m < 2 || m > 64 || prec < 0 || prec > 2 || (mode != 0 && mode != 1))
+        throw Error(BTE_EINVAL, "bad arguments to bte_krylov_build");
+    int E = elems_for(N);
+    if (E < 0) throw Error(BTE_EINVAL, "N above 16384 is not supported");
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const size_t NN = (size_t)N * N;
+    rocblas_handle hb;
+    RBCHECK(rocblas_create_handle(&hb));
+    struct HandleGuard {
+        rocblas_handle h;
+        ~HandleGuard() { rocblas_destroy_handle(h); }
+    } hg{hb};
+    RBCHECK(rocblas_set_stream(hb, s));
+    DevBuf<double> dB, dZ64, dH, dC, dP, dErr;
+    DevBuf<float> dA32, dZ32, dV32;
+    DevBuf<double> dV64;
+    DevBuf<__hip_bfloat16> dAhi, dAlo, dVhi, dVlo;
+    DevBuf<int> dClosed;
+    dB.upload(B, NN, s);
+    Basis V{nullptr, nullptr, nullptr, nullptr, prec};
+    const int cb = 4 * c->cu_count;
+    // A = dt*B in the compute format; Z_0 = dt*B*V_0 = dt*B
+    if (prec == 0) {
+        dZ64.alloc(NN);
+        dV64.alloc(NN * m);
+        V.f64 = dV64.p;
+        hipLaunchKernelGGL(kr_scale64, dim3(cb), dim3(256), 0, s, dB.p, dZ64.p, NN, dt);
+    } else if (prec == 1) {
+        dA32.alloc(NN);
+        dZ32.alloc(NN);
+        dV32.alloc(NN * m);
+        V.f32 = dV32.p;
+        hipLaunchKernelGGL(kr_to_f32, dim3(cb), dim3(256), 0, s, dB.p, dA32.p, NN, dt);
+        HIPCHECK(hipMemcpyAsync(dZ32.p, dA32.p, NN * sizeof(float), hipMemcpyDeviceToDevice, s));
+    } else {
+        dAhi.alloc(NN);
+        dAlo.alloc(NN);
+        dZ32.alloc(NN);
+        dVhi.alloc(NN * m);
+        dVlo.alloc(NN * m);
+        V.hi = dVhi.p;
+        V.lo = dVlo.p;
+        hipLaunchKernelGGL(kr_to_bf16x2, dim3(cb), dim3(256), 0, s, dB.p, dAhi.p, dAlo.p, NN, dt);
+        hipLaunchKernelGGL(kr_to_f32, dim3(cb), dim3(256), 0, s, dB.p, dZ32.p, NN, dt);
+    }
+    HIPCHECK(hipGetLastError());
+    dH.alloc((size_t)N * m * m);
+    dC.alloc((size_t)N * m);
+    dP.alloc(NN);
+    dErr.alloc(N);
+    dClosed.alloc(N);
+    HIPCHECK(hipMemsetAsync(dH.p, 0, (size_t)N * m * m * sizeof(double), s));
+    HIPCHECK(hipMemsetAsync(dClosed.p, 0, N * sizeof(int), s));
+    if (prec != 0) dB.release();
+    // columns of a basis that closes early stay zero
+    if (prec == 0) HIPCHECK(hipMemsetAsync(dV64.p, 0, NN * m * sizeof(double), s));
+    if (prec == 1) HIPCHECK(hipMemsetAsync(dV32.p, 0, NN * m * sizeof(float), s));
+    if (prec == 2) {
+        HIPCHECK(hipMemsetAsync(dVhi.p, 0, NN * m * sizeof(__hip_bfloat16), s));
+        HIPCHECK(hipMemsetAsync(dVlo.p, 0, NN * m * sizeof(__hip_bfloat16), s));
+    }
+    hipEvent_t e0, e1, g0, g1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    HIPCHECK(hipEventCreate(&g0));
+    HIPCHECK(hipEventCreate(&g1));
+    struct EvGuard {
+        hipEvent_t a, b, c, d;
+        ~EvGuard() { hipEventDestroy(a); hipEventDestroy(b); hipEventDestroy(c); hipEventDestroy(d); }
+    } eg{e0, e1, g0, g1};
+    HIPCHECK(hipEventRecord(e0, s));
+    double gemm_ms = 0;
+    int gemms = 0;
+    const double one = 1.0, zero = 0.0;
+    const float onef = 1.0f, zerof = 0.0f;
+    for (int k = 0; k < m; k++) {
+        if (k > 0) {
+            // Z = (dt*B) * V_k
+            HIPCHECK(hipEventRecord(g0, s));
+            const size_t vo = (size_t)k * NN;
+            if (prec == 0) {
+                RBCHECK(rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, N, N, N, &dt, dB.p, N,
+                                      dV64.p + vo, N, &zero, dZ64.p, N));
+            } else if (prec == 1) {
+                RBCHECK(rocblas_sgemm(hb, rocblas_operation_none, rocblas_operation_none, N, N, N, &onef, dA32.p,
+                                      N, dV32.p + vo, N, &zerof, dZ32.p, N));
+            } else {
+                const __hip_bfloat16* As[3] = {dAhi.p, dAhi.p, dAlo.p};
+                const __hip_bfloat16* Vs[3] = {dVhi.p + vo, dVlo.p + vo, dVhi.p + vo};
+                for (int q = 0; q < 3; q++) {
+                    const float beta = q == 0 ? zerof : onef;
+                    RBCHECK(rocblas_gemm_ex(hb, rocblas_operation_none, rocblas_operation_none, N, N, N, &onef,
+                                            As[q], rocblas_datatype_bf16_r, N, Vs[q], rocblas_datatype_bf16_r, N,
+                                            &beta, dZ32.p, rocblas_datatype_f32_r, N, dZ32.p,
+                                            rocblas_datatype_f32_r, N, rocblas_datatype_f32_r,
+                                            rocblas_gemm_algo_standard, 0, 0));
+                }
+            }
+            HIPCHECK(hipEventRecord(g1, s));
+            HIPCHECK(hipEventSynchronize(g1));
+            float gm = 0;
+            HIPCHECK(hipEventElapsedTime(&gm, g0, g1));
+            gemm_ms += gm;
+            gemms++;
+        }
+        const float* z32 = prec == 0 ? nullptr : dZ32.p;
+        const double* z64 = prec == 0 ? dZ64.p : nullptr;
+        switch (E) {
+            case 4: launch_mgs<4>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 8: launch_mgs<8>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 16: launch_mgs<16>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 32: launch_mgs<32>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            default: launch_mgs<64>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+        }
+        HIPCHECK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(kr_expm_e1, dim3(N), dim3(64), (size_t)(m * m + 2 * m) * sizeof(double), s, dH.p, dC.p, m);
+    HIPCHECK(hipGetLastError());
+    switch (E) {
+        case 4: launch_combine<4>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+        case 8: launch_combine<8>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+        case 16: launch_combine<16>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+        case 32: launch_combine<32>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+        default: launch_combine<64>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(e1, s));
+    HIPCHECK(hipMemcpyAsync(P, dP.p, NN * sizeof(double), hipMemcpyDeviceToHost, s));
+    std::vector<double> err(N);
+    std::vector<int> closed(N);
+    HIPCHECK(hipMemcpyAsync(err.data(), dErr.p, N * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(closed.data(), dClosed.p, N * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (rep) {
+        rep->build_ms = ms;
+        rep->gemm_ms = gemm_ms;
+        double flops = 2.0 * (double)N * N * N * gemms * (prec == 2 ? 3 : 1);
+        rep->gemm_tflops = gemm_ms > 0 ? flops / (gemm_ms * 1e-3) / 1e12 : 0;
+        double mx = 0;
+        for (double e : err) mx = std::max(mx, e);
+        rep->max_col_sum_err = mx;
+        int nb = 0;
+        for (int cl : closed) nb += cl;
+        rep->breakdowns = nb;
+    }
+    BTE_GUARD_END
+}

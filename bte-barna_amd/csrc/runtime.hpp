@@ -1,0 +1,141 @@
+// Host-side runtime objects behind the C ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include <map>
+#include <memory>
+#include "../../include/bte_abi.h"
+#include "device_common.hpp"
+
+namespace bte {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& s) : std::runtime_error(s), code(c) {}
+};
+
+void set_error(const std::string& s);
+
+#define HIPCHECK(x)                                                                 \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess)                                                       \
+            throw ::bte::Error(BTE_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define BTE_GUARD_BEGIN try {
+#define BTE_GUARD_END                                 \
+    }                                                 \
+    catch (const ::bte::Error& e) {                   \
+        ::bte::set_error(e.what());                   \
+        return e.code;                                \
+    }                                                 \
+    catch (const std::exception& e) {                 \
+        ::bte::set_error(e.what());                   \
+        return BTE_EINVAL;                            \
+    }                                                 \
+    return BTE_OK;
+
+// Device buffer owned by a handle.
+template <class T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(BTE_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        if (count > n) alloc(count);
+        HIPCHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        alloc(h.size());
+        HIPCHECK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    size_t lds_per_block = 0;
+};
+
+struct Modes {
+    Ctx* ctx = nullptr;
+    int n = 0, nq = 1;
+    std::vector<double> omega, vx, vy, w;
+    double V = 1.0, lattz = 1.0;
+    DevBuf<double> d_vx, d_vy, d_w, d_omega;
+};
+
+struct RectBox {
+    double xmin, xmax, ymin, ymax;
+    int kind;
+    double Teq, Tinit;
+    double cx() const { return 0.5 * (xmin + xmax); }
+    double cy() const { return 0.5 * (ymin + ymax); }
+    double area() const { return (xmax - xmin) * (ymax - ymin); }
+};
+struct HostContact {
+    int box;
+    double npx, npy, p1x, p1y, p2x, p2y;
+    double length() const { return std::hypot(p2x - p1x, p2y - p1y); }
+};
+
+struct Geom {
+    Ctx* ctx = nullptr;
+    std::vector<RectBox> boxes;
+    std::vector<std::vector<HostContact>> contacts;  // ordered by neighbour id
+    std::vector<DevBox> dboxes;
+    std::vector<DevContact> dcontacts;
+    DevBuf<DevBox> d_boxes;
+    DevBuf<DevContact> d_contacts;
+};
+
+struct Prop {
+    Ctx* ctx = nullptr;
+    int ncol = 0, ngamma = 0;
+    double T = 0;
+    std::vector<double> P;  // host copy (column-major)
+    DevBuf<uint2> d_alias;  // ncol x ncol entries, column-major
+    std::vector<double> abssum;
+};
+
+// ---- host helpers (host_util.cpp) ----
+double h_bose_einstein(double omega, double T);
+double h_be_kernel(double omega, double T);
+double h_calc_cv(const Modes& m, double T);
+// Vose alias construction; pmf >= 0, returns false if the pmf sums to 0
+bool build_alias(const double* pmf, int n, uint32_t* thresh, uint32_t* alias);
+// isothermal wall pmf (sampling.cpp:189-269) into pmf; returns cumulsum
+double wall_pmf(const Modes& m, double Twall, double Teq, double nx, double ny, double* pmf);
+// natural cubic spline coefficients (src/interpolation.cpp:29-100):
+// per segment (a, b, c, d, x0) and segment right ends xe
+struct Spline {
+    std::vector<double> a, b, c, d, x0, xe;
+    void build(const std::vector<double>& X, const std::vector<double>& Y);
+    double eval(double x) const;
+};
+
+}  // namespace bte

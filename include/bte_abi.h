@@ -1,0 +1,184 @@
+/* C ABI of the MI355X deviational Monte Carlo + Krylov propagator engine.
+ *
+ * Plain pointers and sizes; no exceptions cross this boundary. Every entry
+ * point returns BTE_OK (0) or a negative BTE_E* code; bte_last_error() gives
+ * the thread's last message. Host arrays are caller-owned; device memory is
+ * owned by the handles. Calls on one context are externally synchronised.
+ *
+ * The reference has no FFI: its boundary is the three executables and the
+ * C++ interfaces they link statically (src/CMakeLists.txt:2-21). Each entry
+ * point below names the reference interface it replaces.
+ */
+#ifndef BTE_ABI_H
+#define BTE_ABI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BTE_OK 0
+#define BTE_EINVAL -1    /* bad argument (reference: value_error/input_error) */
+#define BTE_EGEOM -2     /* geometry_error (include/exceptions.hpp:33-54) */
+#define BTE_EHIP -3      /* HIP runtime failure */
+#define BTE_ENOMEM -4    /* device allocation failed */
+#define BTE_ECAPACITY -5 /* particle buffer capacity exceeded */
+#define BTE_ENODEV -6    /* no GPU visible */
+
+typedef struct bte_ctx bte_ctx;
+typedef struct bte_modes bte_modes;
+typedef struct bte_geom bte_geom;
+typedef struct bte_prop bte_prop;
+typedef struct bte_rta bte_rta;
+typedef struct bte_bert bte_bert;
+
+/* ---- errors, version ---- */
+int bte_last_error(char* buf, size_t len);
+int bte_version(void);
+
+/* ---- context: one HIP device + one stream ---- */
+int bte_ctx_create(int device, bte_ctx** out);
+int bte_ctx_destroy(bte_ctx* ctx);
+int bte_sync(bte_ctx* ctx);
+
+/* ---- host utilities (no GPU use) ---- */
+/* Seeded synthetic mode table: f ~ U(fmin,fmax) THz, omega = 2 pi f,
+ * |v| ~ U(vmin,vmax) nm/ps at angle ~ U(0,2pi), tau = 10^U(log tmin, log tmax)
+ * ps, w = 1/tau; mirror != 0 repeats the first half with vy -> -vy. Stands in
+ * for the Gamma_grid loaded by load_bulk_hdf5 (src/bulk_hdf5.cpp:443). */
+int bte_synthetic_modes(int n, uint64_t seed, double fmin, double fmax, double vmin,
+                        double vmax, double tmin, double tmax, int mirror, double* omega,
+                        double* vx, double* vy, double* w);
+/* Synthetic collision operator B = -Omega (N x N, column-major): symmetric
+ * nonnegative pattern of the given density scaled so that 1^T B = 0 and
+ * B C = 0. Stands in for get_collision_operator_dense
+ * (src/collision_operator.cpp:38-812). */
+int bte_synthetic_B(int n, const double* C, const double* w, double density, uint64_t seed,
+                    int sinkhorn_iters, double* B);
+/* Vose alias table of a nonnegative pmf: thresh in units of 2^-32. */
+int bte_alias_build(int n, const double* pmf, uint32_t* thresh, uint32_t* alias);
+
+/* ---- material: phonon modes (replaces alma::Gamma_grid + Crystal_structure
+ * as used by the MC solvers: omega, vg, V, lattvec(2,2);
+ * include/dynamical_matrix.hpp:88-95, include/structures.hpp) ---- */
+int bte_modes_create(bte_ctx* ctx, int n, int nq, const double* omega, const double* vx,
+                     const double* vy, const double* w, double V, double lattz,
+                     bte_modes** out);
+int bte_modes_destroy(bte_modes* m);
+
+/* ---- geometry: axis-aligned rectangular boxes (replaces
+ * alma::read_geometry_XML -> std::vector<alma::geometry_2d>,
+ * src/geometry_2d.cpp:408-522). rect: xmin,xmax,ymin,ymax per box.
+ * kind: 0 regular box, 1 reservoir (<Reservoir/>), 2 periodic (rejected).
+ * Tinit may be NULL (Tinit = Teq). ---- */
+int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int32_t* kind,
+                          const double* Teq, const double* Tinit, bte_geom** out);
+int bte_geom_destroy(bte_geom* g);
+/* Contacts (geometry_2d::calculate_contacts, src/geometry_2d.cpp:295-381):
+ * for each box up to maxc neighbours with (npx, npy, p1x, p1y, p2x, p2y). */
+int bte_geom_contacts(const bte_geom* g, int maxc, int32_t* ncont, int32_t* nbr, double* geo);
+
+/* ---- RTAMC2D transient solver (replaces the particle loop of
+ * src/RTA_MC2D.cpp:2188-2435: Init_generator, evolparticle, process_segment,
+ * diffusive_bouncing and the post-processing to temperature/jx/jy) ---- */
+typedef struct {
+    double thickness;    /* <material thickness> [nm] */
+    double T0;           /* <material T0> [K] */
+    double dt, maxtime;  /* <time dt maxtime> [ps] */
+    uint64_t nparticles; /* <particles N> */
+    uint64_t seed;
+    int ballistic;       /* <ballistic/> */
+    double sigma_bounce; /* Lambert broadening per mode [rad/ps] */
+} bte_rta_cfg;
+
+typedef struct {
+    uint64_t events;    /* scatters + wall bounces + reservoir absorptions */
+    uint64_t crossings; /* box-to-box crossings */
+    double Eeff;        /* particle power [J/ps] */
+    double kernel_ms;   /* device time of the history kernel */
+    int ntime;          /* number of time rows written (ntime-1 bins) */
+} bte_rta_report;
+
+/* Outputs are [ntime-1][nbox] (the CSV rows of temperature_/jx_/jy_ without
+ * the time column, RTA_MC2D.cpp:2437-2471). */
+int bte_rta_create(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_cfg* cfg,
+                   bte_rta** out);
+int bte_rta_execute(bte_rta* r, double* T_out, double* jx_out, double* jy_out,
+                    bte_rta_report* rep);
+int bte_rta_destroy(bte_rta* r);
+int bte_rta_run(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_cfg* cfg,
+                double* T_out, double* jx_out, double* jy_out, bte_rta_report* rep);
+
+/* ---- propagator: dense P (ncol x ncol, column-major) for modes
+ * [ngamma, ngamma+ncol) (replaces load_P + propagator_H per column,
+ * src/collision_operator.cpp:854-1013, beyondRTA_MC2d.cpp:1356-1378) ---- */
+int bte_prop_create(bte_ctx* ctx, int ncol, const double* P, int ngamma, double T,
+                    bte_prop** out);
+int bte_prop_destroy(bte_prop* p);
+
+/* ---- beRTAMC2D (replaces the step loop of src/beyondRTA_MC2d.cpp:
+ * 2138-2425: advection, adiabatic/isothermal wall generators, Scattering or
+ * ScatteringRTA, histogram, cancellation, calculate_prop, frame saving) ---- */
+typedef struct {
+    double thickness;  /* [nm] */
+    double dt, maxtime;
+    int max_steps;     /* cap on steps (<=0: run while time <= maxtime) */
+    double Ed;         /* <Eeff Ed>; < 0 to derive from Efactor/Tmax/Tmin */
+    uint64_t Efactor;  /* <Eeff particles> */
+    double Tmax, Tmin; /* <Eeff Tmax Tmin>, 0 if absent */
+    int rta;           /* <RTA/> */
+    int pcancel_each;  /* cancellation period (4 in the reference) */
+    uint64_t seed;
+    uint64_t capacity; /* particle buffer capacity; 0 = automatic */
+} bte_bert_cfg;
+
+typedef struct {
+    int nsteps;
+    double Eeff;
+    uint64_t particle_steps; /* sum over steps of particles scattered */
+    int64_t max_particles;
+    double loop_ms;          /* wall time of the step loop (device-synchronised) */
+    double step_kernel_ms;   /* device time of the main step kernel summed */
+} bte_bert_report;
+
+/* Frame callback: called after each step with the (box, mode) histogram
+ * [nbox][nmodes] (properties.msgpack.bin, beyondRTA_MC2d.cpp:1604-1769).
+ * Return nonzero to stop. */
+typedef int (*bte_frame_cb)(void* user, int istep, double time, double Eeff, int nbox,
+                            int nmodes, const int32_t* H);
+
+/* props: [nsteps+1][nbox][3] (ed J/m^3, jx, jy W/m^2); npart: [nsteps+1]
+ * (the stdout columns of beyondRTA_MC2d.cpp:2418-2423); pending (may be
+ * NULL): [nsteps+1] signed wall counts (Eborder) waiting for re-emission at
+ * the end of each step, so that npart-weighted signs + pending is conserved
+ * with adiabatic walls and a nonnegative propagator. Capacity: max_steps+1. */
+int bte_bert_create(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_prop* p,
+                    const bte_bert_cfg* cfg, bte_bert** out);
+int bte_bert_execute(bte_bert* b, double* props, int64_t* npart, int64_t* pending, bte_frame_cb cb,
+                     void* user, bte_bert_report* rep);
+int bte_bert_destroy(bte_bert* b);
+
+/* ---- Krylov propagator (replaces build_P + Arnoldi_algorithm,
+ * src/collision_operator.cpp:815-851, 1015-1064): P = exp(dt*B), all columns
+ * at once as block-Krylov GEMMs. ---- */
+#define BTE_PREC_FP64 0
+#define BTE_PREC_FP32 1
+#define BTE_PREC_BF16 2
+#define BTE_ARNOLDI_REFERENCE 0 /* projections over j<k only (Q1) */
+#define BTE_ARNOLDI_TEXTBOOK 1  /* modified Gram-Schmidt over j<=k */
+typedef struct {
+    double build_ms;   /* device time of the whole build */
+    double gemm_ms;    /* device time in the block GEMMs */
+    double gemm_tflops;
+    double max_col_sum_err; /* max_j |sum_i P_ij - 1| after the build */
+    int breakdowns;    /* columns whose Krylov space closed early */
+} bte_krylov_report;
+
+int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt, int m, int precision,
+                     int arnoldi_mode, double* P, bte_krylov_report* rep);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,438 @@
+// Oracle (test infrastructure only): restatement of beRTAMC2D
+// (src/beyondRTA_MC2d.cpp) for rectangular geometries and one material.
+//   calc_Eeff                  beyondRTA_MC2d.cpp:100-128, main 2044-2070
+//   init_particles (outTref)   beyondRTA_MC2d.cpp:211-269
+//   adiabatic_diffuse_walls    beyondRTA_MC2d.cpp:281-435
+//   Isothermal_walls           beyondRTA_MC2d.cpp:446-585
+//   evolparticle               beyondRTA_MC2d.cpp:708-970
+//   ScatteringRTA              beyondRTA_MC2d.cpp:983-1062
+//   Scattering                 beyondRTA_MC2d.cpp:1073-1210
+//   propagator_H               collision_operator.hpp:241-323, .cpp:854-916
+//   delete_particles           beyondRTA_MC2d.cpp:1214-1220
+//   histogram / cancellation   beyondRTA_MC2d.cpp:2277-2370
+//   calculate_prop (histogram) beyondRTA_MC2d.cpp:1535-1602
+//   step loop                  beyondRTA_MC2d.cpp:2138-2425
+// Deviations applied identically on the GPU path: per-particle PCG32 streams
+// instead of thread-duplicated pcg64 (Q2); explicit dead flag (Q6);
+// gradient sources are zero (uniform Teq) and are rejected otherwise.
+#include "orc_geometry.hpp"
+#include <omp.h>
+#include <map>
+
+namespace orc {
+
+// propagator_H (collision_operator.cpp:854-870, hpp:256-262, cpp:893-904)
+struct PropH {
+    std::vector<double> signed_cumm;
+    double abssum = 0;
+    explicit PropH(const double* b, int n) : signed_cumm(n) {
+        abssum = 0;
+        for (int i = 0; i < n; i++) abssum += std::fabs(b[i]);
+        double acc = 0;
+        for (int i = 0; i < n; i++) {
+            acc += std::fabs(b[i]);
+            signed_cumm[i] = b[i] < 0. ? -acc / abssum : acc / abssum;
+        }
+    }
+    // returns (sign, index) with index relative to the first column mode
+    std::pair<int, int> get(double R) const {
+        size_t im = std::lower_bound(signed_cumm.begin(), signed_cumm.end(), R,
+                                     [](double a, double b) { return std::fabs(a) < std::fabs(b); }) -
+                    signed_cumm.begin();
+        if (im >= signed_cumm.size()) im = signed_cumm.size() - 1;
+        return {signed_cumm[im] > 0. ? 1 : -1, (int)im};
+    }
+};
+
+struct BertSetup {
+    Modes m;
+    std::vector<Box> sys;
+    double thickness = 1.0, dt = 1.0, maxtime = 10.0;
+    int ngamma = 0;              // leading modes not in P (3 in the reference)
+    std::vector<double> P;       // (n-ngamma)^2 column-major
+    bool rta = false;
+    double Ed = -1.;             // fixed particle energy, or
+    uint64_t Efactor = 0;        // number of particles for calc_Eeff
+    double Tmax = 0, Tmin = 0;
+    int pcancel_each = 4;
+
+    std::vector<PropH> hist;
+    double Eeff = 0;
+};
+
+static double calc_Eeff(const Modes& m, double Th, double Tc, double Tr, double vol, uint64_t np) {
+    long double dh = 0, dc = 0;
+    for (int i = 0; i < m.n; i++) {
+        if (almost_equal(0., m.omega[i])) continue;
+        dh += 1e12 * hbar * m.omega[i] * std::fabs(bose_einstein(m.omega[i], Th) - bose_einstein(m.omega[i], Tr));
+        dc += 1e12 * hbar * m.omega[i] * std::fabs(bose_einstein(m.omega[i], Tc) - bose_einstein(m.omega[i], Tr));
+    }
+    return (double)(vol * (dc + dh) / m.nq / np);
+}
+
+struct BPart {
+    double x, y;
+    int mode, box, sign;
+    bool dead;
+};
+
+// evolparticle (beyondRTA_MC2d.cpp:708-970)
+static void bert_evol(BertSetup& S, BPart& p, double vx, double vy, double dt,
+                      pcg32& rng, std::vector<std::vector<int>>& Eb, int depth = 0) {
+    auto& sys = S.sys;
+    if (depth > 100000) throw geometry_error("runaway recursion");
+    Box& box = sys[p.box];
+    if (box.reservoir) { p.dead = true; return; }
+    if (box.periodic) throw geometry_error("periodic boxes are not supported");
+    double nx = p.x + dt * vx, ny = p.y + dt * vy;
+    if (box.inside(nx, ny)) { p.x = nx; p.y = ny; return; }
+    Box::Hit hit;
+    try {
+        hit = box.get_inter_side(p.x, p.y, vx, vy, dt);
+    } catch (const geometry_error&) {
+        double dx = box.cx - p.x, dy = box.cy - p.y, dn = std::hypot(dx, dy);
+        p.x += 1.0e-6 * dx / dn;
+        p.y += 1.0e-6 * dy / dn;
+        bert_evol(S, p, vx, vy, dt, rng, Eb, depth + 1);
+        return;
+    }
+    if (hit.time < 0) throw geometry_error("Error in kinematics");
+    double left = dt - hit.time;
+    if (almost_equal(left, 0.)) left = 0.;
+    double rx = hit.rx, ry = hit.ry;
+    double vn = std::hypot(vx, vy);
+    double ex = rx + 1.0e-6 * vx / vn, ey = ry + 1.0e-6 * vy / vn;
+    std::vector<int> pboxes;
+    for (auto& [c, bc] : box.contacts) {
+        if (c == p.box) continue;
+        double d = bc.npx * vx + bc.npy * vy;
+        if ((sys[c].inside(rx, ry) || sys[c].inside(ex, ey)) && !almost_equal(d, 0.) && d > 0.)
+            pboxes.push_back(c);
+    }
+    if (pboxes.empty()) {
+        std::vector<int> cids;
+        if (in_corner3(rx, ry, sys, cids)) {
+            auto cs = correct_corner_problem(rx, ry, vx, vy, sys, cids, rng);
+            p.box = cs.first;
+            p.x = rx; p.y = ry;
+            Eb[p.box][cs.second] += p.sign;
+        } else {
+            int ib;
+            if (hit.bids.size() > 1) {
+                std::vector<int> voidb;
+                for (int tvb : hit.bids) {
+                    auto& mb = box.borders[tvb];
+                    double cx = rx + 1.0e-4 * mb.npx, cy = ry + 1.0e-4 * mb.npy;
+                    bool isvoid = true;
+                    for (auto& [c, bc] : box.contacts)
+                        if (sys[c].inside(cx, cy)) { isvoid = false; break; }
+                    if (isvoid) voidb.push_back(tvb);
+                }
+                if (voidb.empty()) throw geometry_error("Error in void borders");
+                ib = voidb[rng.bounded((uint32_t)voidb.size())];
+            } else {
+                ib = hit.bids[0];
+            }
+            Eb[p.box][ib] += p.sign;
+        }
+        p.dead = true;
+        return;
+    }
+    if (pboxes.size() == 1) {
+        p.box = pboxes[0];
+        p.x = rx; p.y = ry;
+        bert_evol(S, p, vx, vy, left, rng, Eb, depth + 1);
+        return;
+    }
+    std::vector<int> pbox2;
+    double tau = 1.0e-6;
+    while (true) {
+        double cx = rx + tau * vx, cy = ry + tau * vy;
+        for (int pb : pboxes)
+            if (sys[pb].inside(cx, cy)) pbox2.push_back(pb);
+        if (!pbox2.empty()) break;
+        tau /= 10.0;
+        if (tau < 1.0e-15) {
+            double dx = box.cx - rx, dy = box.cy - ry, dn = std::hypot(dx, dy);
+            p.x = rx + 1.0e-6 * dx / dn;
+            p.y = ry + 1.0e-6 * dy / dn;
+            bert_evol(S, p, vx, vy, left, rng, Eb, depth + 1);
+            return;
+        }
+    }
+    p.box = pbox2[rng.bounded((uint32_t)pbox2.size())];
+    p.x = rx; p.y = ry;
+    bert_evol(S, p, vx, vy, left, rng, Eb, depth + 1);
+}
+
+// Runs the full beRTAMC2D loop. props: [nsteps+1][nbox][3] (ed, jx, jy) with
+// row 0 the initial state (beyondRTA_MC2d.cpp:2105-2117); npart: [nsteps+1].
+// Returns the number of steps done (<= max_steps).
+int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
+             double* props, int64_t* npart, int32_t* H_last, double* Eeff_out, int64_t* pending) {
+    auto& sys = S.sys;
+    const Modes& m = S.m;
+    int nb = (int)sys.size();
+    int ncol = m.n - S.ngamma;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    double vuc = m.V * S.thickness / m.lattz;
+    std::vector<double> vols(nb), volsf(nb);
+    for (int i = 0; i < nb; i++) {
+        vols[i] = sys[i].area * S.thickness;
+        volsf[i] = vols[i] / vuc;
+    }
+    // Eeff (beyondRTA_MC2d.cpp:2026-2070)
+    if (S.Ed < 0.) {
+        double maxT = 0, minT = 1.416784e+32, meanT = 0;
+        int nb_ = 0;
+        for (auto& s : sys) {
+            if (s.periodic) continue;
+            meanT += s.Teq;
+            nb_++;
+            minT = std::min(minT, s.Teq);
+            maxT = std::max(maxT, s.Teq);
+        }
+        if (S.Tmax != 0) { maxT = S.Tmax; minT = S.Tmin; }
+        meanT /= nb_;
+        double e = 1e300;
+        for (int i = 0; i < nb; i++) {
+            if (sys[i].periodic || sys[i].reservoir) continue;
+            e = std::min(e, calc_Eeff(m, maxT, minT, meanT, volsf[i], S.Efactor));
+        }
+        S.Eeff = e;
+    } else {
+        S.Eeff = S.Ed;
+    }
+    const double Eeff = S.Eeff;
+    if (Eeff_out) *Eeff_out = Eeff;
+    if (!S.rta) {
+        S.hist.clear();
+        for (int c = 0; c < ncol; c++) S.hist.emplace_back(&S.P[(size_t)c * ncol], ncol);
+    }
+    std::vector<GridDist> rta_sampler;
+    std::map<double, GridDist> rta_by_T;
+    if (S.rta)
+        for (auto& s : sys)
+            if (!rta_by_T.count(s.Teq)) rta_by_T[s.Teq] = be_derivative(m, s.Teq);
+
+    pcg32 hostrng(seed, 0xb0a7u);  // the serial generators' stream
+    std::vector<BPart> parts;
+    // init_particles from Tinit != Teq (beyondRTA_MC2d.cpp:211-269)
+    for (auto& s : sys) {
+        if (s.reservoir || s.periodic) continue;
+        if (almost_equal(s.Teq, s.Treal)) continue;
+        GridDist d = out_tref(m, s.Treal, s.Teq);
+        size_t ngen = n_to_generate((s.area * S.thickness / vuc) * d.cumulsum / (Eeff * m.nq), hostrng);
+        for (size_t i = 0; i < ngen; i++) {
+            double x = s.xmin + hostrng.uniform() * (s.xmax - s.xmin);
+            double y = s.ymin + hostrng.uniform() * (s.ymax - s.ymin);
+            size_t pos = d.sample(hostrng);
+            parts.push_back({x, y, (int)pos, s.id, d.signs[pos], false});
+        }
+    }
+    auto calc_prop = [&](const std::vector<std::vector<int>>& H, int row) {
+        for (int b = 0; b < nb; b++) {
+            double ed = 0, fx = 0, fy = 0;
+            if (!(sys[b].reservoir || sys[b].periodic)) {
+                long long s = 0;
+                for (int k = 0; k < m.n; k++) {
+                    s += H[b][k];
+                    fx += H[b][k] * m.vx[k];
+                    fy += H[b][k] * m.vy[k];
+                }
+                ed = Eeff / vols[b] * 1.0e27 * (double)s;
+                fx *= Eeff / vols[b] * 1.0e30;
+                fy *= Eeff / vols[b] * 1.0e30;
+            }
+            props[((size_t)row * nb + b) * 3 + 0] = ed;
+            props[((size_t)row * nb + b) * 3 + 1] = fx;
+            props[((size_t)row * nb + b) * 3 + 2] = fy;
+        }
+    };
+    // row 0: properties of the initial particles (calculate_prop over the
+    // particle list, beyondRTA_MC2d.cpp:1494-1532)
+    {
+        std::vector<std::vector<int>> H0(nb, std::vector<int>(m.n, 0));
+        for (auto& p : parts) H0[p.box][p.mode] += p.sign;
+        calc_prop(H0, 0);
+        npart[0] = (int64_t)parts.size();
+        if (pending) pending[0] = 0;
+    }
+    // adiabatic wall spline tables J(T) per (box, border)
+    std::vector<double> Ts(100);
+    for (int i = 0; i < 100; i++) Ts[i] = 10.0 + 10.0 * i;
+    std::map<std::pair<int, int>, CubicSpline1D> interp;
+    std::vector<std::vector<int>> Eb(nb, std::vector<int>(4, 0));
+    double time = 0.;
+    int istep = 0;
+    uint64_t stepcount = 0;
+    while (time <= S.maxtime && istep < max_steps) {
+        // 1. advection
+        {
+            int nth = omp_get_max_threads();
+            std::vector<std::vector<std::vector<int>>> Ebt(nth, std::vector<std::vector<int>>(nb, std::vector<int>(4, 0)));
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < (long long)parts.size(); i++) {
+                int tid = omp_get_thread_num();
+                pcg32 rng(seed ^ 0x9e3779b97f4a7c15ULL, (stepcount << 32) ^ (uint64_t)i);
+                double vx, vy;
+                get_v(m, parts[i].mode, vx, vy);
+                bert_evol(S, parts[i], vx, vy, S.dt, rng, Ebt[tid]);
+            }
+            for (auto& e : Ebt)
+                for (int b = 0; b < nb; b++)
+                    for (int k = 0; k < 4; k++) Eb[b][k] += e[b][k];
+        }
+        auto advect_new = [&](std::vector<BPart>& np_, uint64_t salt) {
+            for (size_t i = 0; i < np_.size(); i++) {
+                pcg32 rng(seed ^ salt, (stepcount << 32) ^ (uint64_t)i);
+                double vx, vy;
+                get_v(m, np_[i].mode, vx, vy);
+                double rdt = S.dt * rng.uniform();
+                bert_evol(S, np_[i], vx, vy, rdt, rng, Eb);
+            }
+            parts.insert(parts.end(), np_.begin(), np_.end());
+        };
+        // 2. gradient sources: zero for uniform Teq (calculate_gradientT)
+        // 3. Isothermal walls (reservoir injection)
+        {
+            std::vector<BPart> np_;
+            for (auto& s : sys) {
+                if (!s.reservoir) continue;
+                for (auto& [c, b] : s.contacts) {
+                    if (sys[c].reservoir || sys[c].periodic) continue;
+                    if (almost_equal(b.length(), 0.)) continue;
+                    if (almost_equal(s.Teq, sys[c].Teq)) continue;
+                    GridDist g = isothermal_wall(m, s.Teq, sys[c].Teq, b.npx, b.npy);
+                    double spf = b.length() * S.thickness / vuc;
+                    size_t ng = n_to_generate(S.dt * spf * g.cumulsum / (Eeff * m.nq), hostrng);
+                    for (size_t k = 0; k < ng; k++) {
+                        double u = hostrng.uniform();
+                        double x = b.p1x + u * b.ndx, y = b.p1y + u * b.ndy;
+                        if (!sys[c].inside(x, y)) {
+                            double dx = sys[c].cx - x, dy = sys[c].cy - y, dn = std::hypot(dx, dy);
+                            x += 1.0e-6 * dx / dn;
+                            y += 1.0e-6 * dy / dn;
+                        }
+                        size_t pos = g.sample(hostrng);
+                        np_.push_back({x, y, (int)pos, c, g.signs[pos], false});
+                    }
+                }
+            }
+            advect_new(np_, 0x15a0ULL);
+        }
+        // 4. adiabatic diffuse walls
+        {
+            std::vector<BPart> np_;
+            for (auto& s : sys) {
+                if (s.reservoir || s.periodic) continue;
+                for (int ib = 0; ib < (int)s.borders.size(); ib++) {
+                    auto& b = s.borders[ib];
+                    auto key = std::make_pair(s.id, ib);
+                    if (!interp.count(key)) {
+                        std::vector<double> J(100);
+                        for (int t = 0; t < 100; t++) {
+                            GridDist g = isothermal_wall(m, Ts[t], s.Teq, -b.npx, -b.npy);
+                            J[t] = wall_flux(g, vuc);
+                        }
+                        interp[key] = CubicSpline1D(J, Ts);
+                    }
+                    if (Eb[s.id][ib] != 0) {
+                        double Jb = Eeff * Eb[s.id][ib] / S.dt / (b.length() * S.thickness);
+                        double Tb = interp[key].interpolate(Jb);
+                        GridDist g = isothermal_wall(m, Tb, s.Teq, -b.npx, -b.npy);
+                        size_t ng = (size_t)(std::abs(Eb[s.id][ib]) + 1.0e-6);
+                        for (size_t k = 0; k < ng; k++) {
+                            double u = hostrng.uniform();
+                            double x = b.p1x + u * b.ndx, y = b.p1y + u * b.ndy;
+                            size_t pos = g.sample(hostrng);
+                            np_.push_back({x, y, (int)pos, s.id, g.signs[pos], false});
+                        }
+                        Eb[s.id][ib] = 0;
+                    }
+                }
+            }
+            advect_new(np_, 0xad1aULL);
+        }
+        // 5. delete
+        parts.erase(std::remove_if(parts.begin(), parts.end(), [](const BPart& p) { return p.dead; }), parts.end());
+        // 6. scattering
+        if (S.rta) {
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < (long long)parts.size(); i++) {
+                pcg32 rng(seed ^ 0x5ca7ULL, (stepcount << 32) ^ (uint64_t)i);
+                auto& p = parts[i];
+                double R = rng.uniform();
+                if (R < 1 - std::exp(-m.w[p.mode] * S.dt))
+                    p.mode = (int)rta_by_T.at(sys[p.box].Teq).sample(rng);
+            }
+        } else {
+            std::vector<BPart> out;
+            out.reserve(parts.size() * 2);
+            int nth = omp_get_max_threads();
+            std::vector<std::vector<BPart>> outs(nth);
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < (long long)parts.size(); i++) {
+                int tid = omp_get_thread_num();
+                pcg32 rng(seed ^ 0x5ca7ULL, (stepcount << 32) ^ (uint64_t)i);
+                long pending = 1;
+                const BPart parent = parts[i];
+                while (pending > 0) {
+                    pending--;
+                    int col = parent.mode - S.ngamma;
+                    if (col < 0) throw value_error("This particle should be deleted");
+                    auto sol = S.hist[col].get(rng.uniform());
+                    BPart c = parent;
+                    c.mode = sol.second + S.ngamma;
+                    int ns = (parent.sign * sol.first) < 0 ? -1 : 1;
+                    c.sign = ns;
+                    outs[tid].push_back(c);
+                    if (ns * parent.sign < 0) pending += 2;
+                }
+            }
+            for (auto& o : outs) out.insert(out.end(), o.begin(), o.end());
+            parts.swap(out);
+        }
+        // 7. histogram
+        std::vector<std::vector<int>> H(nb, std::vector<int>(m.n, 0));
+        for (auto& p : parts) H[p.box][p.mode] += p.sign;
+        // 8. cancellation (sorted by (box, mode, sign); keep the first |h|
+        // particles of sign(h) per (box, mode))
+        if (istep % S.pcancel_each == 0) {
+            std::stable_sort(parts.begin(), parts.end(), [](const BPart& a, const BPart& b) {
+                if (a.box != b.box) return a.box < b.box;
+                if (a.mode != b.mode) return a.mode < b.mode;
+                return a.sign < b.sign;
+            });
+            auto Hc = H;
+            std::vector<BPart> kept;
+            kept.reserve(parts.size());
+            for (auto& p : parts) {
+                if (sys[p.box].reservoir || sys[p.box].periodic) continue;
+                int& h = Hc[p.box][p.mode];
+                if (h == 0) continue;
+                if ((p.sign < 0) != (h < 0)) continue;
+                h -= p.sign;
+                kept.push_back(p);
+            }
+            parts.swap(kept);
+        }
+        istep++;
+        stepcount++;
+        time += S.dt;
+        calc_prop(H, istep);
+        npart[istep] = (int64_t)parts.size();
+        if (pending) {
+            long long pe = 0;
+            for (int b = 0; b < nb; b++)
+                for (int k = 0; k < 4; k++) pe += Eb[b][k];
+            pending[istep] = pe;
+        }
+        if (H_last)
+            for (int b = 0; b < nb; b++)
+                for (int k = 0; k < m.n; k++) H_last[(size_t)b * m.n + k] = H[b][k];
+    }
+    return istep;
+}
+
+}  // namespace orc

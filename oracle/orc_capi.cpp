@@ -1,0 +1,308 @@
+// Oracle C interface (test infrastructure only; loaded with ctypes by the
+// tests, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg).
+//
+// Parity pinning of each oracle function (what checks it in tests/):
+//   pcg32                       reference KAT external/pcg-cpp-0.98/test-high/
+//                               expected/check-pcg32.out (all 5 rounds)
+//   bose_einstein(_kernel),
+//   calc_cv, wall flux,
+//   calc_Eeff                   closed forms (numpy) + limits
+//   GridDist sampling           exact pmf on small N (chi^2) and brute force
+//   CubicSpline1D               scipy CubicSpline(bc_type='natural')
+//   expm                        scipy.linalg.expm golden fixtures, 2x2 closed
+//                               form, diagonal matrices
+//   arnoldi / build_P           m >= N exactness, textbook-vs-reference Krylov
+//                               relation A V = V H + f e_m^T, expm fixtures
+//   synthetic_B                 invariants 1^T B = 0, B C = 0, B offdiag >= 0
+//   get_inter_side / contacts   analytic slab intersections on rectangles
+//   propagator_H                exact enumeration of R on a grid
+//   rta_run                     ballistic closed form (no scattering), flux
+//                               antisymmetry, energy bookkeeping
+//   bert_run                    conservation of the signed count (P >= 0,
+//                               adiabatic walls), E[H_{n+1}] = P H_n with v = 0
+//   Lambert bounce, corner
+//   resolution, nudges          parity unpinned (needs a run of the reference)
+#include "orc_geometry.hpp"
+#include <omp.h>
+
+namespace orc {
+void expm(int n, const double* A, double* E);
+long double error_bound(int N, const double* A, int n, double t);
+int krylov_size(int N, const double* A, double t, double eps);
+void arnoldi(int N, const double* A, const double* b, int n, double t, int mode, double* V, double* H);
+void build_P(int N, const double* A, double t, int m, int mode, double* P);
+void project_columns(int N, double* B);
+void synthetic_B(int N, const double* C, const double* w, double density, uint64_t seed,
+                 int sinkhorn_iters, double* B);
+}  // namespace orc
+
+// full definitions are needed for the setup structs
+#include "orc_rta.cpp"
+#include "orc_bert.cpp"
+
+using namespace orc;
+
+static thread_local std::string g_err;
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+// ---------------- PCG32 ----------------
+void orc_pcg32_init(uint64_t initstate, uint64_t initseq, uint64_t* st) {
+    pcg32 r(initstate, initseq);
+    st[0] = r.state;
+    st[1] = r.inc;
+}
+void orc_pcg32_next(uint64_t* st, int n, uint32_t* out) {
+    pcg32 r;
+    r.state = st[0];
+    r.inc = st[1];
+    for (int i = 0; i < n; i++) out[i] = r.next();
+    st[0] = r.state;
+}
+void orc_pcg32_bounded(uint64_t* st, uint32_t bound, int n, uint32_t* out) {
+    pcg32 r;
+    r.state = st[0];
+    r.inc = st[1];
+    for (int i = 0; i < n; i++) out[i] = r.bounded(bound);
+    st[0] = r.state;
+}
+void orc_pcg32_uniform(uint64_t* st, int n, double* out) {
+    pcg32 r;
+    r.state = st[0];
+    r.inc = st[1];
+    for (int i = 0; i < n; i++) out[i] = r.uniform();
+    st[0] = r.state;
+}
+
+// ---------------- synthetic mode table ----------------
+// f ~ U(fmin,fmax) THz, omega = 2 pi f; |v| ~ U(vmin,vmax) nm/ps with angle
+// ~ U(0, 2pi); tau = 10^U(log10 tmin, log10 tmax) ps, w = 1/tau. With
+// mirror != 0 the second half repeats the first with vy -> -vy.
+// Draw order per mode: f, |v|, angle, tau from PCG32(seed, stream 0x6d6f6465).
+void orc_synthetic_modes(int n, uint64_t seed, double fmin, double fmax, double vmin,
+                         double vmax, double tmin, double tmax, int mirror, double* omega,
+                         double* vx, double* vy, double* w) {
+    pcg32 r(seed, 0x6d6f6465u);
+    int nbase = mirror ? n / 2 : n;
+    double l0 = std::log10(tmin), l1 = std::log10(tmax);
+    for (int i = 0; i < nbase; i++) {
+        double f = fmin + (fmax - fmin) * r.uniform();
+        double sp = vmin + (vmax - vmin) * r.uniform();
+        double th = 2.0 * PI * r.uniform();
+        double tau = std::pow(10.0, l0 + (l1 - l0) * r.uniform());
+        omega[i] = 2.0 * PI * f;
+        vx[i] = sp * std::cos(th);
+        vy[i] = sp * std::sin(th);
+        w[i] = 1.0 / tau;
+    }
+    if (mirror)
+        for (int i = 0; i < nbase; i++) {
+            omega[nbase + i] = omega[i];
+            vx[nbase + i] = vx[i];
+            vy[nbase + i] = -vy[i];
+            w[nbase + i] = w[i];
+        }
+}
+
+// ---------------- closed-form physics ----------------
+void orc_bose_einstein(int n, const double* omega, double T, double* out) {
+    for (int i = 0; i < n; i++) out[i] = bose_einstein(omega[i], T);
+}
+void orc_bose_einstein_kernel(int n, const double* omega, double T, double* out) {
+    for (int i = 0; i < n; i++) out[i] = bose_einstein_kernel(omega[i], T);
+}
+static Modes mk_modes(int n, const double* omega, const double* vx, const double* vy,
+                      const double* w, double V, double lattz, int nq) {
+    Modes m;
+    m.n = n;
+    m.nq = nq;
+    m.omega.assign(omega, omega + n);
+    m.vx.assign(vx, vx + n);
+    m.vy.assign(vy, vy + n);
+    m.w.assign(w, w + n);
+    m.V = V;
+    m.lattz = lattz;
+    return m;
+}
+double orc_calc_cv(int n, const double* omega, double V, int nq, double T) {
+    Modes m;
+    m.n = n;
+    m.nq = nq;
+    m.V = V;
+    m.omega.assign(omega, omega + n);
+    return calc_cv(m, T);
+}
+// wall distribution: cumulative (n) and flux for vuc; returns cumulsum
+double orc_wall_dist(int n, const double* omega, const double* vx, const double* vy, int nq,
+                     double Twall, double Teq, double nx, double ny, double vuc,
+                     double* cumulative, double* flux) {
+    Modes m;
+    m.n = n;
+    m.nq = nq;
+    m.omega.assign(omega, omega + n);
+    m.vx.assign(vx, vx + n);
+    m.vy.assign(vy, vy + n);
+    m.w.assign(n, 1.0);
+    GridDist d = isothermal_wall(m, Twall, Teq, nx, ny);
+    if (cumulative) std::memcpy(cumulative, d.cumulative.data(), n * sizeof(double));
+    if (flux) *flux = wall_flux(d, vuc);
+    return d.cumulsum;
+}
+void orc_spline(int n, const double* X, const double* Y, int nq, const double* q, double* out) {
+    CubicSpline1D s(std::vector<double>(X, X + n), std::vector<double>(Y, Y + n));
+    for (int i = 0; i < nq; i++) out[i] = s.interpolate(q[i]);
+}
+// GridDist sampling: draw ns samples from the pmf p (n) with PCG32(seed, stream)
+void orc_grid_sample(int n, const double* p, uint64_t seed, uint64_t stream, int ns, int32_t* out) {
+    GridDist d;
+    d.fill_cumulative(std::vector<double>(p, p + n));
+    pcg32 r(seed, stream);
+    for (int i = 0; i < ns; i++) out[i] = (int32_t)d.sample(r);
+}
+// propagator_H::get for a column b at the given R values
+void orc_propH_get(int n, const double* b, int nr, const double* R, int32_t* sign, int32_t* idx) {
+    PropH h(b, n);
+    for (int i = 0; i < nr; i++) {
+        auto s = h.get(R[i]);
+        sign[i] = s.first;
+        idx[i] = s.second;
+    }
+}
+
+// ---------------- geometry ----------------
+static std::vector<Box> mk_sys(int nbox, const double* rect, const int32_t* kind,
+                               const double* Teq, const double* Tinit) {
+    std::vector<Box> sys(nbox);
+    for (int i = 0; i < nbox; i++) {
+        sys[i].id = i;
+        sys[i].make(rect[4 * i + 0], rect[4 * i + 1], rect[4 * i + 2], rect[4 * i + 3]);
+        sys[i].reservoir = kind[i] == 1;
+        sys[i].periodic = kind[i] == 2;
+        sys[i].Teq = Teq[i];
+        sys[i].Treal = Tinit ? Tinit[i] : Teq[i];
+    }
+    calculate_contacts(sys);
+    return sys;
+}
+// get_inter_side on box ibox: returns 0 ok, 1 geometry_error; out[0]=time,
+// out[1..2]=point, bids (up to 4) with nbids
+int orc_inter_side(int nbox, const double* rect, int ibox, double x0, double y0, double vx,
+                   double vy, double dt, double* out, int32_t* bids, int32_t* nbids) {
+    std::vector<int32_t> kind(nbox, 0);
+    std::vector<double> T(nbox, 300.);
+    try {
+        auto sys = mk_sys(nbox, rect, kind.data(), T.data(), nullptr);
+        auto h = sys[ibox].get_inter_side(x0, y0, vx, vy, dt);
+        out[0] = h.time;
+        out[1] = h.rx;
+        out[2] = h.ry;
+        *nbids = (int32_t)h.bids.size();
+        for (size_t k = 0; k < h.bids.size() && k < 4; k++) bids[k] = h.bids[k];
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+// contacts: for each box, list (neighbour, npx, npy, p1x, p1y, p2x, p2y)
+int orc_contacts(int nbox, const double* rect, int maxc, int32_t* ncont, int32_t* nbr, double* geo) {
+    std::vector<int32_t> kind(nbox, 0);
+    std::vector<double> T(nbox, 300.);
+    try {
+        auto sys = mk_sys(nbox, rect, kind.data(), T.data(), nullptr);
+        for (int i = 0; i < nbox; i++) {
+            int k = 0;
+            for (auto& [c, b] : sys[i].contacts) {
+                if (k >= maxc) break;
+                nbr[i * maxc + k] = c;
+                double* g = geo + (size_t)(i * maxc + k) * 6;
+                g[0] = b.npx; g[1] = b.npy; g[2] = b.p1x; g[3] = b.p1y; g[4] = b.p2x; g[5] = b.p2y;
+                k++;
+            }
+            ncont[i] = k;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// ---------------- RTAMC2D ----------------
+// kind: 0 bin, 1 reservoir, 2 periodic. Outputs [ntime-1][nbox].
+int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, const double* w,
+                double V, double lattz, int nq, int nbox, const double* rect, const int32_t* kind,
+                const double* Teq, double thickness, double T0, double dt, double maxtime,
+                uint64_t nparticles, uint64_t seed, int ballistic, double sigma, int nthreads,
+                double* T_out, double* jx_out, double* jy_out, uint64_t* counters, double* raw_gz,
+                double* Eeff_out) {
+    try {
+        RtaSetup S;
+        S.m = mk_modes(n, omega, vx, vy, w, V, lattz, nq);
+        S.sys = mk_sys(nbox, rect, kind, Teq, nullptr);
+        S.thickness = thickness;
+        S.T0 = T0;
+        S.dt = dt;
+        S.maxtime = maxtime;
+        S.ballistic = ballistic != 0;
+        S.sigma = sigma;
+        rta_run(S, nparticles, seed, nthreads, T_out, jx_out, jy_out, counters, raw_gz);
+        if (Eeff_out) *Eeff_out = S.Eeff;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// ---------------- beRTAMC2D ----------------
+int orc_bert_run(int n, const double* omega, const double* vx, const double* vy, const double* w,
+                 double V, double lattz, int nq, int nbox, const double* rect, const int32_t* kind,
+                 const double* Teq, const double* Tinit, double thickness, double dt,
+                 double maxtime, int ngamma, const double* P, int rta, double Ed,
+                 uint64_t Efactor, double Tmax, double Tmin, uint64_t seed, int nthreads,
+                 int max_steps, double* props, int64_t* npart, int32_t* H_last, double* Eeff_out,
+                 int32_t* nsteps_out, int64_t* pending) {
+    try {
+        BertSetup S;
+        S.m = mk_modes(n, omega, vx, vy, w, V, lattz, nq);
+        S.sys = mk_sys(nbox, rect, kind, Teq, Tinit);
+        S.thickness = thickness;
+        S.dt = dt;
+        S.maxtime = maxtime;
+        S.ngamma = ngamma;
+        S.rta = rta != 0;
+        if (!S.rta) S.P.assign(P, P + (size_t)(n - ngamma) * (n - ngamma));
+        S.Ed = Ed;
+        S.Efactor = Efactor;
+        S.Tmax = Tmax;
+        S.Tmin = Tmin;
+        *nsteps_out = bert_run(S, seed, nthreads, max_steps, props, npart, H_last, Eeff_out, pending);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// ---------------- Krylov ----------------
+void orc_expm(int n, const double* A, double* E) { expm(n, A, E); }
+double orc_error_bound(int N, const double* A, int n, double t) { return (double)error_bound(N, A, n, t); }
+int orc_krylov_size(int N, const double* A, double t, double eps) { return krylov_size(N, A, t, eps); }
+void orc_arnoldi(int N, const double* A, const double* b, int n, double t, int mode, double* V,
+                 double* H) {
+    arnoldi(N, A, b, n, t, mode, V, H);
+}
+void orc_build_P(int N, const double* A, double t, int m, int mode, int nthreads, double* P) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    build_P(N, A, t, m, mode, P);
+}
+void orc_project_columns(int N, double* B) { project_columns(N, B); }
+void orc_synthetic_B(int N, const double* C, const double* w, double density, uint64_t seed,
+                     int iters, double* B) {
+    synthetic_B(N, C, w, density, seed, iters, B);
+}
+
+}  // extern "C"

@@ -1,0 +1,251 @@
+"""ctypes wrapper of the oracle (oracle/_build/liboracle.so). Test
+infrastructure only: the oracle is the checker, never the thing measured."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+dp = C.POINTER(C.c_double)
+ip32 = C.POINTER(C.c_int32)
+u64 = C.c_uint64
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run __graft_entry__.build()")
+        _L = C.CDLL(ORACLE_SO)
+        _L.orc_last_error.restype = C.c_char_p
+        _L.orc_calc_cv.restype = C.c_double
+        _L.orc_wall_dist.restype = C.c_double
+        _L.orc_error_bound.restype = C.c_double
+    return _L
+
+
+def _dp(a):
+    return a.ctypes.data_as(dp)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Pcg32:
+    def __init__(self, state, stream):
+        self.st = (C.c_uint64 * 2)()
+        lib().orc_pcg32_init(u64(state), u64(stream), self.st)
+
+    def next(self, n=1):
+        out = (C.c_uint32 * n)()
+        lib().orc_pcg32_next(self.st, n, out)
+        return list(out)
+
+    def bounded(self, bound, n=1):
+        out = (C.c_uint32 * n)()
+        lib().orc_pcg32_bounded(self.st, C.c_uint32(bound), n, out)
+        return list(out)
+
+    def uniform(self, n=1):
+        out = np.zeros(n)
+        lib().orc_pcg32_uniform(self.st, n, _dp(out))
+        return out
+
+
+def synthetic_modes(n, seed, fmin=0.1, fmax=15.0, vmin=0.5, vmax=8.0, tmin=1.0, tmax=1000.0, mirror=False):
+    o = {k: np.zeros(n) for k in ("omega", "vx", "vy", "w")}
+    lib().orc_synthetic_modes(n, u64(seed), C.c_double(fmin), C.c_double(fmax), C.c_double(vmin),
+                              C.c_double(vmax), C.c_double(tmin), C.c_double(tmax), int(mirror),
+                              _dp(o["omega"]), _dp(o["vx"]), _dp(o["vy"]), _dp(o["w"]))
+    return o
+
+
+def bose_einstein(omega, T):
+    omega = _f(omega)
+    out = np.zeros_like(omega)
+    lib().orc_bose_einstein(len(omega), _dp(omega), C.c_double(T), _dp(out))
+    return out
+
+
+def bose_einstein_kernel(omega, T):
+    omega = _f(omega)
+    out = np.zeros_like(omega)
+    lib().orc_bose_einstein_kernel(len(omega), _dp(omega), C.c_double(T), _dp(out))
+    return out
+
+
+def calc_cv(omega, V, T, nq=1):
+    omega = _f(omega)
+    return lib().orc_calc_cv(len(omega), _dp(omega), C.c_double(V), nq, C.c_double(T))
+
+
+def wall_dist(omega, vx, vy, Twall, Teq, nx, ny, vuc, nq=1):
+    omega, vx, vy = _f(omega), _f(vx), _f(vy)
+    cum = np.zeros(len(omega))
+    flux = C.c_double()
+    cs = lib().orc_wall_dist(len(omega), _dp(omega), _dp(vx), _dp(vy), nq, C.c_double(Twall), C.c_double(Teq),
+                             C.c_double(nx), C.c_double(ny), C.c_double(vuc), _dp(cum), C.byref(flux))
+    return cs, cum, flux.value
+
+
+def spline(X, Y, q):
+    X, Y, q = _f(X), _f(Y), _f(q)
+    out = np.zeros(len(q))
+    lib().orc_spline(len(X), _dp(X), _dp(Y), len(q), _dp(q), _dp(out))
+    return out
+
+
+def grid_sample(p, seed, stream, ns):
+    p = _f(p)
+    out = np.zeros(ns, np.int32)
+    lib().orc_grid_sample(len(p), _dp(p), u64(seed), u64(stream), ns, out.ctypes.data_as(ip32))
+    return out
+
+
+def propH_get(b, R):
+    b, R = _f(b), _f(R)
+    sg = np.zeros(len(R), np.int32)
+    idx = np.zeros(len(R), np.int32)
+    lib().orc_propH_get(len(b), _dp(b), len(R), _dp(R), sg.ctypes.data_as(ip32), idx.ctypes.data_as(ip32))
+    return sg, idx
+
+
+def inter_side(rect, ibox, x0, y0, vx, vy, dt):
+    rect = _f(rect)
+    out = np.zeros(3)
+    bids = np.zeros(4, np.int32)
+    nb = C.c_int32()
+    rc = lib().orc_inter_side(len(rect) // 4 if rect.ndim == 1 else len(rect), _dp(rect), ibox, C.c_double(x0),
+                              C.c_double(y0), C.c_double(vx), C.c_double(vy), C.c_double(dt), _dp(out),
+                              bids.ctypes.data_as(ip32), C.byref(nb))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out[0], out[1], out[2], list(bids[:nb.value])
+
+
+def contacts(rect, maxc=8):
+    rect = _f(rect)
+    nbox = len(rect)
+    nc = np.zeros(nbox, np.int32)
+    nbr = np.zeros((nbox, maxc), np.int32)
+    geo = np.zeros((nbox, maxc, 6))
+    rc = lib().orc_contacts(nbox, _dp(rect), maxc, nc.ctypes.data_as(ip32), nbr.ctypes.data_as(ip32), _dp(geo))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return nc, nbr, geo
+
+
+def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, ballistic=False, sigma=0.25,
+            nthreads=0):
+    rect, Teq = _f(rect), _f(Teq)
+    kind = np.ascontiguousarray(kind, np.int32)
+    nbox = len(kind)
+    ntime = int(np.floor(maxtime / dt + 1e-6)) + 1
+    nt = ntime - 1
+    T = np.zeros((nt, nbox))
+    jx = np.zeros((nt, nbox))
+    jy = np.zeros((nt, nbox))
+    raw = np.zeros((nbox, nt))
+    cnt = (C.c_uint64 * 3)()
+    eeff = C.c_double()
+    om, vx, vy, w = (_f(mat[k]) for k in ("omega", "vx", "vy", "w"))
+    rc = lib().orc_rta_run(len(om), _dp(om), _dp(vx), _dp(vy), _dp(w), C.c_double(mat["V"]),
+                           C.c_double(mat.get("lattz", 1.0)), 1, nbox, _dp(rect), kind.ctypes.data_as(ip32),
+                           _dp(Teq), C.c_double(thickness), C.c_double(T0), C.c_double(dt), C.c_double(maxtime),
+                           u64(nparticles), u64(seed), int(ballistic), C.c_double(sigma), nthreads, _dp(T),
+                           _dp(jx), _dp(jy), cnt, _dp(raw), C.byref(eeff))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return {"T": T, "jx": jx, "jy": jy, "events": cnt[0], "crossings": cnt[1], "nudges": cnt[2], "raw_gz": raw,
+            "Eeff": eeff.value}
+
+
+def bert_run(mat, rect, kind, Teq, Tinit, thickness, dt, maxtime, P=None, ngamma=0, rta=False, Ed=-1.0,
+             Efactor=0, Tmax=0.0, Tmin=0.0, seed=1, max_steps=0, nthreads=0):
+    rect, Teq, Tinit = _f(rect), _f(Teq), _f(Tinit)
+    kind = np.ascontiguousarray(kind, np.int32)
+    nbox = len(kind)
+    n = len(mat["omega"])
+    nst = 0
+    t = 0.0
+    while t <= maxtime:
+        nst += 1
+        t += dt
+    if max_steps > 0:
+        nst = min(nst, max_steps)
+    props = np.zeros((nst + 1, nbox, 3))
+    npart = np.zeros(nst + 1, np.int64)
+    pending = np.zeros(nst + 1, np.int64)
+    H = np.zeros((nbox, n), np.int32)
+    eeff = C.c_double()
+    ns = C.c_int32()
+    Pm = np.asfortranarray(P, dtype=np.float64) if P is not None else np.zeros((1, 1))
+    om, vx, vy, w = (_f(mat[k]) for k in ("omega", "vx", "vy", "w"))
+    rc = lib().orc_bert_run(n, _dp(om), _dp(vx), _dp(vy), _dp(w), C.c_double(mat["V"]),
+                            C.c_double(mat.get("lattz", 1.0)), 1, nbox, _dp(rect), kind.ctypes.data_as(ip32),
+                            _dp(Teq), _dp(Tinit), C.c_double(thickness), C.c_double(dt), C.c_double(maxtime),
+                            ngamma, Pm.ctypes.data_as(dp), int(rta), C.c_double(Ed), u64(Efactor),
+                            C.c_double(Tmax), C.c_double(Tmin), u64(seed), nthreads, nst, _dp(props),
+                            npart.ctypes.data_as(C.POINTER(C.c_int64)), H.ctypes.data_as(ip32), C.byref(eeff),
+                            C.byref(ns), pending.ctypes.data_as(C.POINTER(C.c_int64)))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    k = ns.value
+    return {"props": props[:k + 1], "npart": npart[:k + 1], "pending": pending[:k + 1], "H_last": H,
+            "Eeff": eeff.value, "nsteps": k}
+
+
+def expm(A):
+    A = np.asfortranarray(A, dtype=np.float64)
+    n = A.shape[0]
+    E = np.zeros((n, n), order="F")
+    lib().orc_expm(n, A.ctypes.data_as(dp), E.ctypes.data_as(dp))
+    return E
+
+
+def arnoldi(A, b, n, t, mode):
+    A = np.asfortranarray(A, dtype=np.float64)
+    N = A.shape[0]
+    b = _f(b)
+    V = np.zeros((N, n), order="F")
+    H = np.zeros((n, n), order="F")
+    lib().orc_arnoldi(N, A.ctypes.data_as(dp), _dp(b), n, C.c_double(t), mode, V.ctypes.data_as(dp),
+                      H.ctypes.data_as(dp))
+    return V, H
+
+
+def build_P(A, t, m, mode, nthreads=0):
+    A = np.asfortranarray(A, dtype=np.float64)
+    N = A.shape[0]
+    P = np.zeros((N, N), order="F")
+    lib().orc_build_P(N, A.ctypes.data_as(dp), C.c_double(t), m, mode, nthreads, P.ctypes.data_as(dp))
+    return P
+
+
+def krylov_size(A, t, eps=1e-8):
+    A = np.asfortranarray(A, dtype=np.float64)
+    return lib().orc_krylov_size(A.shape[0], A.ctypes.data_as(dp), C.c_double(t), C.c_double(eps))
+
+
+def error_bound(A, n, t):
+    A = np.asfortranarray(A, dtype=np.float64)
+    return lib().orc_error_bound(A.shape[0], A.ctypes.data_as(dp), n, C.c_double(t))
+
+
+def synthetic_B(Cm, w, density, seed, iters=200):
+    Cm, w = _f(Cm), _f(w)
+    n = len(Cm)
+    B = np.zeros((n, n), order="F")
+    lib().orc_synthetic_B(n, _dp(Cm), _dp(w), C.c_double(density), u64(seed), iters, B.ctypes.data_as(dp))
+    return B
+
+
+def project_columns(B):
+    B = np.asfortranarray(B, dtype=np.float64).copy(order="F")
+    lib().orc_project_columns(B.shape[0], B.ctypes.data_as(dp))
+    return B

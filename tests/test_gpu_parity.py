@@ -1,0 +1,91 @@
+"""GPU parity of the HIP path against the oracle on the same seeded inputs.
+
+Monte Carlo results are compared statistically: R independent replicas on
+each side give per-cell means and standard errors; every compared cell must
+satisfy |mean_gpu - mean_oracle| < 5 sigma_diff and the mean of z^2 over the
+cells must stay below 2 (a correct pair gives ~1)."""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+
+def zstats(a, b):
+    """a, b: (R, ...) replicas -> z-scores of the difference of means"""
+    ma, mb = a.mean(0), b.mean(0)
+    sa = a.std(0, ddof=1) / np.sqrt(a.shape[0])
+    sb = b.std(0, ddof=1) / np.sqrt(b.shape[0])
+    s = np.sqrt(sa ** 2 + sb ** 2)
+    s = np.where(s > 0, s, np.inf)
+    return (ma - mb) / s
+
+
+def _rta_setup(pkg, ctx, nbins=20, nmodes=600):
+    W = pkg.workloads
+    mat = W.material(nmodes, seed=3)
+    rect, kind, teq, _ = W.ribbon(200.0, 100.0, nbins, 300.0, reservoirs=(300.5, 299.5))
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    return mat, rect, kind, teq, modes, geom
+
+
+def test_rta_matches_oracle(pkg, ctx):
+    mat, rect, kind, teq, modes, geom = _rta_setup(pkg, ctx)
+    kw = dict(thickness=1.0, T0=300.0, dt=20.0, maxtime=400.0)
+    R, N = 8, 20000
+    g = [pkg.rtamc2d(ctx, modes, geom, nparticles=N, seed=100 + r, sigma_bounce=0.25, **kw) for r in range(R)]
+    o = [orc.rta_run(mat, rect, kind, teq, nparticles=N, seed=200 + r, sigma=0.25, **kw) for r in range(R)]
+    for key in ("T", "jx"):
+        ga = np.array([x[key][[4, -1], :20] for x in g])
+        oa = np.array([x[key][[4, -1], :20] for x in o])
+        z = zstats(ga, oa)
+        assert np.abs(z).max() < 5.0, (key, z)
+        assert (z ** 2).mean() < 2.0, (key, (z ** 2).mean())
+    ev_g = np.mean([x["events"] for x in g])
+    ev_o = np.mean([x["events"] for x in o])
+    assert abs(ev_g / ev_o - 1) < 0.02
+
+
+def test_bert_matches_oracle(pkg, ctx):
+    W = pkg.workloads
+    mat, B = W.config1_inputs(nmodes=300, seed=5)
+    import scipy.linalg
+    P = scipy.linalg.expm(5.0 * B)
+    rect, kind, teq, tin = W.ribbon(100.0, 50.0, 10, 300.0, Tinit_first=301.0)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq, tin)
+    prop = pkg.Propagator(ctx, P)
+    kw = dict(thickness=1.0, dt=5.0, maxtime=5.0 * 19, Efactor=20000, Tmax=301.0, Tmin=300.0)
+    R = 8
+    g = [pkg.bertamc2d(ctx, modes, geom, prop, seed=10 + r, **kw) for r in range(R)]
+    o = [orc.bert_run(mat, rect, kind, teq, tin, P=P, seed=20 + r, **kw) for r in range(R)]
+    assert g[0]["nsteps"] == o[0]["nsteps"] == 20
+    for q in (0, 1):
+        ga = np.array([x["props"][[3, 10, 20], :, q] for x in g])
+        oa = np.array([x["props"][[3, 10, 20], :, q] for x in o])
+        z = zstats(ga, oa)
+        assert np.abs(z).max() < 5.0, (q, z)
+        assert (z ** 2).mean() < 2.0, (q, (z ** 2).mean())
+    # P >= 0 and adiabatic walls conserve particles + pending wall counts
+    # exactly (every particle here is positive)
+    for x in g + o:
+        tot = x["npart"] + x["pending"]
+        assert (tot == tot[0]).all(), tot
+
+
+def test_krylov_matches_oracle(pkg, ctx):
+    W = pkg.workloads
+    mat, B = W.config1_inputs(nmodes=256, seed=7)
+    for mode, mcode in (("reference", 0), ("textbook", 1)):
+        Po = orc.build_P(B, 5.0, 30, mcode)
+        Pg, rep = pkg.build_P(ctx, B, 5.0, m=30, precision="fp64", arnoldi=mode)
+        assert np.abs(Pg - Po).max() < 1e-10, (mode, np.abs(Pg - Po).max())
+    import scipy.linalg
+    Pe = scipy.linalg.expm(5.0 * B)
+    for prec, tol in (("fp64", 1e-11), ("fp32", 2e-5), ("bf16", 2e-3)):
+        Pg, rep = pkg.build_P(ctx, B, 5.0, m=30, precision=prec, arnoldi="textbook")
+        l1 = np.abs(Pg - Pe).sum(0).max()
+        assert l1 < tol, (prec, l1)
+        assert np.abs(Pg.sum(0) - 1).max() < 1e-12
