@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X deviational MC engine.
+
+Workload (one "step" of this benchmark = one full beRTAMC2D transient run):
+2000-mode synthetic material, synthetic energy-conserving collision operator
+(5% dense), propagator P = exp(5 ps * B) built on the GPU by the block-Krylov
+path (FP64, m = 30), 500 nm x 100 nm ribbon in 50 x-bins, 1 K pulse in bin
+0 giving ~1e6 deviational particles, 200 time steps of 5 ps with adiabatic
+diffuse walls and cancellation every 4 steps. Each timed step is an
+independent replica (seed = base + step), as used for sigma(T).
+
+value = particle-events/s over the timed steps, where a particle-event of
+beRTAMC2D is one particle advected and scattered for one time step.
+
+Extras (not the headline value): RTAMC2D at the 3000-mode, 1e6-particle
+steady-state configuration (events/s) and the Krylov propagator build at
+N = 8192, m = 30 in FP32 / BF16 against FP64.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PARTICLE_STEP = 72  # 32 B state in + 32 B out + 8 B propagator alias entry
+
+
+def load_pkg():
+    import importlib.util
+    name = "bte_barna_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    path = os.path.join(ROOT, "bte-barna_amd", "__init__.py")
+    spec = importlib.util.spec_from_file_location(name, path, submodule_search_locations=[os.path.dirname(path)])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per step-kernel launch from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", "bert_step_kernel_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(pkg, mat, P, cfg, nparticles, nsteps):
+    """The oracle (C++ restatement of the reference loop, OpenMP) on a bounded
+    sample of the same workload: fewer initial particles and steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    nthreads = max(1, min(nthreads, 16))
+    t0 = time.perf_counter()
+    o = orc.bert_run(mat, cfg["rect"], cfg["kind"], cfg["Teq"], cfg["Tinit"], thickness=cfg["thickness"],
+                     dt=cfg["dt"], maxtime=cfg["dt"] * (nsteps - 1), P=P, Efactor=nparticles,
+                     Tmax=cfg["Tmax"], Tmin=cfg["Tmin"], seed=12345, max_steps=nsteps, nthreads=nthreads)
+    el = time.perf_counter() - t0
+    ps = int(o["npart"][:-1].sum())  # particles entering each step
+    return {"value": ps / el, "unit": "particle-events/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle beRTAMC2D, same material/propagator/geometry, {nparticles} initial particles x "
+                      f"{nsteps} steps ({ps} particle-events in {el:.2f} s), {platform.processor() or platform.machine()}"}
+
+
+def rta_extra(pkg, ctx):
+    W = pkg.workloads
+    c = W.config0()
+    m = c["material"]
+    modes = pkg.ModeTable(ctx, m["omega"], m["vx"], m["vy"], m["w"], m["V"], m["lattz"])
+    geom = pkg.Geometry(ctx, c["rect"], c["kind"], c["Teq"])
+    r = pkg.RtaRun(ctx, modes, geom, thickness=c["thickness"], T0=c["T0"], dt=c["dt"], maxtime=c["maxtime"],
+                   nparticles=c["nparticles"], seed=1, sigma_bounce=c["sigma_bounce"])
+    r.execute()  # warm-up
+    ctx.sync()
+    t0 = time.perf_counter()
+    out = r.execute()
+    el = time.perf_counter() - t0
+    r.close()
+    T = out["T"][-1, :100]
+    return {"workload": "RTAMC2D steady state: 3000 modes, 1e6 particles, 1 um x 200 nm, +-0.5 K walls, 100 x-bins, "
+                        "dt 25 ps, maxtime 5 ns",
+            "events": int(out["events"]), "crossings": int(out["crossings"]), "seconds": el,
+            "events_per_s": out["events"] / el, "kernel_ms": out["kernel_ms"],
+            "kernel_events_per_s": out["events"] / (out["kernel_ms"] * 1e-3),
+            "T_left_right": [float(T[0]), float(T[-1])]}
+
+
+def krylov_extra(pkg, ctx, N=8192, m=30):
+    W = pkg.workloads
+    mat, B = W.config1_inputs(nmodes=N, seed=1)
+    res = {}
+    P64, rep64 = pkg.build_P(ctx, B, 5.0, m=m, precision="fp64", arnoldi="textbook")
+    res["fp64"] = {"build_ms": rep64["build_ms"], "gemm_tflops": rep64["gemm_tflops"],
+                   "col_sum_drift": rep64["max_col_sum_err"]}
+    for prec in ("fp32", "bf16"):
+        P, rep = pkg.build_P(ctx, B, 5.0, m=m, precision=prec, arnoldi="textbook")
+        res[prec] = {"build_ms": rep["build_ms"], "gemm_ms": rep["gemm_ms"], "gemm_tflops": rep["gemm_tflops"],
+                     "max_col_L1_vs_fp64": float(np.abs(P - P64).sum(0).max()),
+                     "col_sum_drift_before_projection": rep["max_col_sum_err"],
+                     "col_sum_drift_after": float(np.abs(P.sum(0) - 1).max())}
+        del P
+    res["workload"] = f"PropagatorBuilder Krylov N={N}, m={m}, dt=5 ps, all columns as block-Krylov GEMMs"
+    res["flops_2mN3"] = 2.0 * m * N ** 3
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--cpu-particles", type=int, default=200_000)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is described in "
+              f"DESIGN.md and is not run here.", file=sys.stderr)
+        sys.exit(2)
+
+    pkg = load_pkg()
+    W = pkg.workloads
+    ctx = pkg.Context(0)
+    log("setup: material, collision operator, propagator")
+    mat, B = W.config1_inputs()
+    P, krep = pkg.build_P(ctx, B, 5.0, m=30, precision="fp64", arnoldi="textbook")
+    cfg = W.config1()
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, cfg["rect"], cfg["kind"], cfg["Teq"], cfg["Tinit"])
+    prop = pkg.Propagator(ctx, P)
+    runs = []
+    for s in range(args.warmup + args.steps):
+        runs.append(pkg.BertRun(ctx, modes, geom, prop, thickness=cfg["thickness"], dt=cfg["dt"],
+                                maxtime=cfg["maxtime"], max_steps=cfg["max_steps"], Efactor=cfg["Efactor"],
+                                Tmax=cfg["Tmax"], Tmin=cfg["Tmin"], seed=1000 + s))
+    for s in range(args.warmup):
+        runs[s].execute()
+        log(f"warmup {s} done")
+    ctx.sync()
+    t0 = time.perf_counter()
+    outs = []
+    for s in range(args.warmup, args.warmup + args.steps):
+        outs.append(runs[s].execute())
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    log(f"timed {args.steps} runs in {elapsed:.3f} s")
+    events = sum(o["particle_steps"] for o in outs)
+    value = events / elapsed
+    k_ms = sum(o["step_kernel_ms"] for o in outs)
+    k_launch = sum(o["step_kernel_launches"] for o in outs)
+    k_units = sum(o["step_kernel_particle_steps"] for o in outs)
+    avg_ms = k_ms / k_launch
+    bytes_per_launch = BYTES_PER_PARTICLE_STEP * k_units / k_launch
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # sigma(T) at the last step across the replicas (T = T0 + ed / Cv)
+    cv = 1.6e6
+    Tlast = np.array([300.0 + o["props"][-1, :, 0] / cv for o in outs])
+    sigmaT = float((Tlast.std(0, ddof=1) / np.sqrt(len(outs))).max()) if len(outs) > 1 else None
+    conserved = all(bool(((o["npart"] + o["pending"]) == o["npart"][0]).all()) for o in outs)
+    line = {
+        "metric": "deviational-MC particle-events/s at fixed σ(T); Krylov propagator build time",
+        "value": value,
+        "unit": "particle-events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "beRTAMC2D transient: 2000-mode synthetic material, 5%-dense synthetic Omega, "
+                               "P=exp(-Omega*5ps) (GPU Krylov FP64 m=30), 500x100 nm ribbon, 50 x-bins, 1 K pulse "
+                               "in bin 0, ~1e6 particles, 200 steps of 5 ps; one bench step = one full run",
+                   "initial_particles": int(outs[0]["npart"][0]), "time_steps": int(outs[0]["nsteps"]),
+                   "particle_events_per_run": int(outs[0]["particle_steps"]),
+                   "max_sigma_T_K_last_step": sigmaT, "count_conserved": conserved,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": "bert_step_kernel", "avg_launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline(pkg, mat, P, cfg, args.cpu_particles, args.cpu_steps)
+        log("cpu baseline done")
+    if not args.no_extras:
+        log("extra: RTAMC2D config ...")
+        ex = {"rta_steady": rta_extra(pkg, ctx)}
+        log("extra: Krylov N=8192 ...")
+        ex["krylov"] = krylov_extra(pkg, ctx)
+        ex["propagator_setup_ms"] = krep["build_ms"]
+        line["extras"] = ex
+    for r in runs:
+        r.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -201,8 +201,9 @@ class BertRun(_Handle):
         n = rep.nsteps
         return {"props": props[:n + 1], "npart": npart[:n + 1], "pending": pending[:n + 1], "nsteps": n,
                 "Eeff": rep.Eeff,
-                "particle_steps": rep.particle_steps, "max_particles": rep.max_particles,
-                "loop_ms": rep.loop_ms, "step_kernel_ms": rep.step_kernel_ms}
+                "particle_steps": rep.particle_steps, "step_kernel_particle_steps": rep.step_kernel_particle_steps,
+                "max_particles": rep.max_particles, "loop_ms": rep.loop_ms,
+                "step_kernel_ms": rep.step_kernel_ms, "step_kernel_launches": rep.step_kernel_launches}
 
 
 def bertamc2d(ctx, modes, geom, prop, **kw):

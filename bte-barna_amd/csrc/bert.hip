@@ -55,7 +55,7 @@ struct BertParams {
     int64_t* npart;          // [steps+1]
     int64_t* pending;        // [steps+1] signed wall counts awaiting re-emission
     uint32_t* flags;         // [0] overflow, [1] bad mode, [2] stuck
-    unsigned long long* stats;  // [0] particle-steps
+    unsigned long long* stats;  // [0] step-kernel particle-steps, [1] emitted
     uint64_t cap;
     uint64_t seed;
     double dt, Eeff, thickness;
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void bert_walls_kernel(BertParams P, uint32_t 
         done += alive ? 1 : 0;
     }
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
-    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
+    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
 
 // Reservoir injection (Isothermal_walls, beyondRTA_MC2d.cpp:446-585): one
@@ -402,7 +402,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
         done += alive ? 1 : 0;
     }
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
-    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
+    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
 
 // calculate_prop (beyondRTA_MC2d.cpp:1535-1602) from the step histogram, the
@@ -524,9 +524,11 @@ struct Bert {
     int nsrc = 0, ninit = 0;
     std::vector<int32_t> hH;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<hipEvent_t> kev;  // begin/end event per step-kernel launch
     ~Bert() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
+        for (auto e : kev) (void)hipEventDestroy(e);
     }
 };
 
@@ -816,6 +818,8 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         b->d_src_tab.upload(stab, s);
     }
     b->hH.resize((size_t)nbox * n);
+    b->kev.resize((size_t)2 * b->max_steps);
+    for (auto& e : b->kev) HIPCHECK(hipEventCreate(&e));
     HIPCHECK(hipEventCreate(&b->ev0));
     HIPCHECK(hipEventCreate(&b->ev1));
     HIPCHECK(hipStreamSynchronize(s));
@@ -872,8 +876,10 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         P.pout = buf[cur ^ 1];
         const int cancel_prev = istep > 0 && ((istep - 1) % pc == 0);
         const int cancel_now = (istep % pc == 0);
+        HIPCHECK(hipEventRecord(b->kev[2 * istep], s));
         hipLaunchKernelGGL(bert_step_kernel, dim3(step_blocks), dim3(256), 0, s, P, (uint32_t)istep, cur,
                            cancel_prev);
+        HIPCHECK(hipEventRecord(b->kev[2 * istep + 1], s));
         if (b->nsrc)
             hipLaunchKernelGGL(bert_source_kernel, dim3(b->nsrc), dim3(256), 0, s, P, b->d_src.p,
                                b->d_src_tab.p, (uint32_t)istep, cur);
@@ -900,6 +906,12 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     auto t1 = std::chrono::steady_clock::now();
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    double kms = 0;
+    for (int i = 0; i < istep; i++) {
+        float e = 0;
+        HIPCHECK(hipEventElapsedTime(&e, b->kev[2 * i], b->kev[2 * i + 1]));
+        kms += e;
+    }
     uint32_t flags[4];
     unsigned long long stats[2];
     HIPCHECK(hipMemcpy(flags, b->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost));
@@ -914,12 +926,14 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     if (rep) {
         rep->nsteps = istep;
         rep->Eeff = b->Eeff;
-        rep->particle_steps = stats[0];
+        rep->particle_steps = stats[0] + stats[1];
+        rep->step_kernel_particle_steps = stats[0];
+        rep->step_kernel_launches = istep;
         int64_t mx = 0;
         for (int i = 0; i <= istep; i++) mx = std::max(mx, npart[i]);
         rep->max_particles = mx;
         rep->loop_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-        rep->step_kernel_ms = ms;
+        rep->step_kernel_ms = kms;
     }
     BTE_GUARD_END
 }

@@ -80,16 +80,25 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
     bool alive = false, need_flight = true;
     unsigned long long events = 0, crossings = 0, stuck = 0;
     unsigned long long budget = 0;
+    int still = 0;        // consecutive events without time advance
+    double tlast = -1.0;
     const double maxtime = P.maxtime;
     for (;;) {
-        // every history ends: a particle that exceeds the iteration budget
-        // is dropped and counted, so each lane reaches the exit
-        if (alive && ++budget > P.max_iters) {
-            stuck++;
-            alive = false;
+        // every history ends: a particle that exceeds the iteration budget or
+        // makes 64 events in a row without advancing in time is dropped and
+        // counted, so each lane reaches the exit
+        if (alive) {
+            still = (t == tlast) ? still + 1 : 0;
+            tlast = t;
+            if (++budget > P.max_iters || still > 64) {
+                stuck++;
+                alive = false;
+            }
         }
         if (!alive) {
             budget = 0;
+            still = 0;
+            tlast = -1.0;
             unsigned long long pid = atomicAdd(P.next_pid, 1ULL);
             if (pid >= P.nparticles) break;
             rng.seed(P.seed, pid);
@@ -273,9 +282,10 @@ static void build_lambert(const Modes& m, double sigma, std::vector<int4>& info,
                 pmf[k] = ok ? vn * std::exp(-0.5 * d * d / (sig * sig)) : 0.;
             }
             if (!build_alias(pmf.data(), len, th.data(), al.data())) {
-                // no partner moving away from this side: the row is never used
-                // by a particle arriving on it unless the mode table is
-                // degenerate; keep it pointing at the mode itself
+                // a mode that can reach this side needs a partner leaving it
+                if (nx * m.vx[i] + ny * m.vy[i] > 0. && !almost_equal(0., m.omega[i]))
+                    throw Error(BTE_EINVAL, "mode " + std::to_string(i) +
+                                                " has no diffuse partner within the Lambert window; raise sigma_bounce");
                 for (int k = 0; k < len; k++) { th[k] = 0u; al[k] = (uint32_t)(pos[i] - lo); }
             }
             for (int k = 0; k < len; k++) tab.push_back(make_uint2(th[k], al[k]));

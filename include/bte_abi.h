@@ -136,10 +136,12 @@ typedef struct {
 typedef struct {
     int nsteps;
     double Eeff;
-    uint64_t particle_steps; /* sum over steps of particles scattered */
+    uint64_t particle_steps;  /* particles advected and scattered, all kernels */
+    uint64_t step_kernel_particle_steps; /* the part done by the main step kernel */
     int64_t max_particles;
-    double loop_ms;          /* wall time of the step loop (device-synchronised) */
-    double step_kernel_ms;   /* device time of the main step kernel summed */
+    double loop_ms;           /* wall time of the step loop (device-synchronised) */
+    double step_kernel_ms;    /* device time of the main step kernel, summed */
+    int step_kernel_launches;
 } bte_bert_report;
 
 /* Frame callback: called after each step with the (box, mode) histogram

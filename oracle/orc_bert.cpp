@@ -271,14 +271,21 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
         {
             int nth = omp_get_max_threads();
             std::vector<std::vector<std::vector<int>>> Ebt(nth, std::vector<std::vector<int>>(nb, std::vector<int>(4, 0)));
+            std::string err;
 #pragma omp parallel for schedule(static)
             for (long long i = 0; i < (long long)parts.size(); i++) {
                 int tid = omp_get_thread_num();
                 pcg32 rng(seed ^ 0x9e3779b97f4a7c15ULL, (stepcount << 32) ^ (uint64_t)i);
                 double vx, vy;
                 get_v(m, parts[i].mode, vx, vy);
-                bert_evol(S, parts[i], vx, vy, S.dt, rng, Ebt[tid]);
+                try {
+                    bert_evol(S, parts[i], vx, vy, S.dt, rng, Ebt[tid]);
+                } catch (const std::exception& e) {
+#pragma omp critical
+                    err = e.what();
+                }
             }
+            if (!err.empty()) throw geometry_error(err);
             for (auto& e : Ebt)
                 for (int b = 0; b < nb; b++)
                     for (int k = 0; k < 4; k++) Eb[b][k] += e[b][k];
@@ -371,16 +378,20 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
             out.reserve(parts.size() * 2);
             int nth = omp_get_max_threads();
             std::vector<std::vector<BPart>> outs(nth);
+            bool badmode = false;
 #pragma omp parallel for schedule(static)
             for (long long i = 0; i < (long long)parts.size(); i++) {
                 int tid = omp_get_thread_num();
                 pcg32 rng(seed ^ 0x5ca7ULL, (stepcount << 32) ^ (uint64_t)i);
                 long pending = 1;
                 const BPart parent = parts[i];
+                if (parent.mode - S.ngamma < 0) {
+                    badmode = true;
+                    continue;
+                }
                 while (pending > 0) {
                     pending--;
                     int col = parent.mode - S.ngamma;
-                    if (col < 0) throw value_error("This particle should be deleted");
                     auto sol = S.hist[col].get(rng.uniform());
                     BPart c = parent;
                     c.mode = sol.second + S.ngamma;
@@ -390,6 +401,7 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
                     if (ns * parent.sign < 0) pending += 2;
                 }
             }
+            if (badmode) throw value_error("This particle should be deleted");
             for (auto& o : outs) out.insert(out.end(), o.begin(), o.end());
             parts.swap(out);
         }

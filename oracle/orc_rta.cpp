@@ -305,9 +305,11 @@ int rta_run(RtaSetup& S, uint64_t nparticles, uint64_t seed, int nthreads,
     std::vector<Tally> tallies(nth, Tally(nb, nt));
     std::vector<Counters> cnt(nth);
     const double maxtime = S.maxtime;
+    std::string err;
 #pragma omp parallel for schedule(dynamic, 64)
     for (long long i = 0; i < (long long)nparticles; i++) {
         int tid = omp_get_thread_num();
+        try {
         pcg32 rng(seed, (uint64_t)i);
         Part p = init_particle(S, rng);
         double time = 0.;
@@ -324,7 +326,12 @@ int rta_run(RtaSetup& S, uint64_t nparticles, uint64_t seed, int nthreads,
             if (p.dead) break;
             time = p.t;
         }
+        } catch (const std::exception& e) {
+#pragma omp critical
+            err = e.what();
+        }
     }
+    if (!err.empty()) throw geometry_error(err);
     std::vector<double> gz(nb * nt, 0.), jx(nb * nt, 0.), jy(nb * nt, 0.);
     Counters tot;
     for (int t = 0; t < nth; t++) {

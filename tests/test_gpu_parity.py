@@ -35,8 +35,8 @@ def test_rta_matches_oracle(pkg, ctx):
     mat, rect, kind, teq, modes, geom = _rta_setup(pkg, ctx)
     kw = dict(thickness=1.0, T0=300.0, dt=20.0, maxtime=400.0)
     R, N = 8, 20000
-    g = [pkg.rtamc2d(ctx, modes, geom, nparticles=N, seed=100 + r, sigma_bounce=0.25, **kw) for r in range(R)]
-    o = [orc.rta_run(mat, rect, kind, teq, nparticles=N, seed=200 + r, sigma=0.25, **kw) for r in range(R)]
+    g = [pkg.rtamc2d(ctx, modes, geom, nparticles=N, seed=100 + r, sigma_bounce=1.0, **kw) for r in range(R)]
+    o = [orc.rta_run(mat, rect, kind, teq, nparticles=N, seed=200 + r, sigma=1.0, **kw) for r in range(R)]
     for key in ("T", "jx"):
         ga = np.array([x[key][[4, -1], :20] for x in g])
         oa = np.array([x[key][[4, -1], :20] for x in o])
