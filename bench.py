@@ -130,7 +130,18 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--cpu-particles", type=int, default=200_000)
     ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": x.numel() * 4, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is described in "
               f"DESIGN.md and is not run here.", file=sys.stderr)

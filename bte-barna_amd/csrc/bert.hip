@@ -8,10 +8,12 @@
 //   bert_source_kernel Isothermal_walls (446-585), emitted particles
 //                      advected U*dt, scattered, histogrammed (only launched
 //                      when reservoirs inject)
-//   bert_walls_kernel  adiabatic_diffuse_walls (281-435): one workgroup per
-//                      box side; border temperature from the J(T) spline,
-//                      |Eborder| particles re-emitted, advected U*dt,
-//                      scattered, histogrammed
+//   bert_wallcdf_kernel adiabatic_diffuse_walls (281-435), part 1: per box
+//                      side the border temperature from the J(T) spline and
+//                      the emission cumulative distribution
+//   bert_emit_kernel   part 2: all |Eborder| re-emissions as one balanced
+//                      index space: placed, advected U*dt, scattered,
+//                      histogrammed
 //   bert_props_kernel  calculate_prop (1535-1602) for every box, post-cancel
 //                      particle count, buffer recycling
 // Particle state is 32 bytes (x, y f64; mode; box|sign; 64-bit id). The
@@ -47,33 +49,66 @@ struct BertParams {
     const int32_t* side_spl; // per (box, side) spline table index, -1 none
     BPart* pin;
     BPart* pout;
-    uint32_t* count;         // [2] live counts
+    uint32_t* seg;           // [2][nsegs] live count per buffer segment
     int32_t* H;              // [2][nbox][n]
     uint32_t* tickets;       // [nbox][n]
     int32_t* Eb;             // [2][nbox*4]
     double* props;           // [steps+1][nbox][3]
     int64_t* npart;          // [steps+1]
     int64_t* pending;        // [steps+1] signed wall counts awaiting re-emission
+    double* wcdf;            // [nbox*4][n] emission cumulative per box side
+    uint32_t* wcount;        // [nbox*4] particles to emit this step
+    int32_t* wsign;          // [nbox*4] their sign
     uint32_t* flags;         // [0] overflow, [1] bad mode, [2] stuck
     unsigned long long* stats;  // [0] step-kernel particle-steps, [1] emitted
     uint64_t cap;
+    uint32_t segcap;         // capacity of one segment (cap / nsegs)
+    int32_t nsegs;           // step + emit + source workgroups
+    int32_t seg_emit0, seg_src0;  // first emit and first source segment
+    int32_t ncts;
     uint64_t seed;
     double dt, Eeff, thickness;
     int32_t n, ngamma, ncol, nbox, nseg;
     int32_t rta, max_children;
 };
 
-__device__ inline void seed_stream(Pcg32& r, uint64_t seed, uint64_t pid, uint32_t step, uint32_t salt) {
-    r.seed(mix64(seed ^ mix64(pid + 0x632be59bd9b4e019ULL * ((uint64_t)step + 1) + salt)), pid);
+// The stream of particle pid in a given step and kernel: PCG32 with
+// increment 2*pid+1 started from pid*K ^ key, where key mixes the run seed,
+// the step and the kernel (a uniform value, computed once per wave).
+__device__ inline uint64_t stream_key(uint64_t seed, uint32_t step, uint32_t salt) {
+    return mix64(seed ^ mix64(((uint64_t)step << 8) + salt + 1));
+}
+__device__ inline void seed_stream(Pcg32& r, uint64_t key, uint64_t pid) {
+    r.seed((pid * 0x9e3779b97f4a7c15ULL) ^ key, pid);
+}
+
+// Geometry tables small enough for LDS are staged there by every workgroup.
+#define BERT_LDS_BOXES 128
+#define BERT_LDS_CTS 512
+struct GeoLds {
+    DevBox boxes[BERT_LDS_BOXES];
+    DevContact cts[BERT_LDS_CTS];
+};
+__device__ inline void geo_stage(const BertParams& P, GeoLds& g, const DevBox*& boxes, const DevContact*& cts) {
+    if (P.nbox <= BERT_LDS_BOXES && P.ncts <= BERT_LDS_CTS) {
+        for (int i = threadIdx.x; i < P.nbox; i += blockDim.x) g.boxes[i] = P.boxes[i];
+        for (int i = threadIdx.x; i < P.ncts; i += blockDim.x) g.cts[i] = P.cts[i];
+        boxes = g.boxes;
+        cts = g.cts;
+    } else {
+        boxes = P.boxes;
+        cts = P.cts;
+    }
 }
 
 // evolparticle (beyondRTA_MC2d.cpp:708-970): ballistic advection by dt
 // through box crossings; a void side adds the particle sign to that side's
 // Eborder and ends the particle. Returns false if the particle ended.
-__device__ inline bool advect(const BertParams& P, double& x, double& y, int& box, int sign,
-                              double vx, double vy, double dt, Pcg32& rng, int32_t* Eb) {
+__device__ inline bool advect(const BertParams& P, const DevBox* boxes, const DevContact* cts, double& x,
+                              double& y, int& box, int sign, double vx, double vy, double dt, Pcg32& rng,
+                              int32_t* Eb) {
     for (int it = 0; it < 4096; it++) {
-        const DevBox B = P.boxes[box];
+        const DevBox B = boxes[box];
         if (B.kind == 1) return false;  // reservoir: deleted
         double nx = x + dt * vx, ny = y + dt * vy;
         if (box_inside(B, nx, ny)) {
@@ -87,10 +122,10 @@ __device__ inline bool advect(const BertParams& P, double& x, double& y, int& bo
         double left = dt - tf;
         if (almost_equal(left, 0.)) left = 0.;
         int pb[4];
-        int npb = find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
+        int npb = find_pboxes(boxes, cts, B, ex, ey, vx, vy, pb);
         if (npb == 0) {
             int s = __popc(mask) == 1 ? __ffs(mask) - 1
-                                      : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next());
+                                      : pick_void_side(boxes, cts, B, mask, ex, ey, rng.next());
             atomicAdd(Eb + box * 4 + s, sign);
             return false;
         }
@@ -103,7 +138,7 @@ __device__ inline bool advect(const BertParams& P, double& x, double& y, int& bo
             double tau = 1.0e-6;
             while (nc == 0 && tau >= 1.0e-15) {
                 for (int k = 0; k < npb && k < 4; k++)
-                    if (box_inside(P.boxes[pb[k]], ex + tau * vx, ey + tau * vy)) cand[nc++] = pb[k];
+                    if (box_inside(boxes[pb[k]], ex + tau * vx, ey + tau * vy)) cand[nc++] = pb[k];
                 tau /= 10.0;
             }
             if (nc == 0) {
@@ -123,13 +158,14 @@ __device__ inline bool advect(const BertParams& P, double& x, double& y, int& bo
 }
 
 // Scattering with the propagator (beyondRTA_MC2d.cpp:1133-1192) or
-// ScatteringRTA (1040-1055). Replays from a copy of the stream: pass 0
-// counts children, pass 1 emits. Children of a sign flip re-queue the
+// ScatteringRTA (1040-1055). Pass 0 (EMIT false) counts the children and
+// returns the first one in nm0/ns0; pass 1 replays the same stream to emit
+// them when there is more than one. Children of a sign flip re-queue the
 // parent twice (1189-1192).
 template <bool EMIT>
 __device__ inline int scatter(const BertParams& P, Pcg32 rng, int mode, int sign, int box,
-                              BPart* out, uint64_t slot, double x, double y, uint64_t pid,
-                              uint32_t step) {
+                              BPart* out, uint64_t slot, uint64_t limit, double x, double y, uint64_t pid,
+                              uint32_t step, int& nm0, int& ns0) {
     if (P.rta) {
         double R = rng.uniform();
         int nm = mode;
@@ -140,11 +176,13 @@ __device__ inline int scatter(const BertParams& P, Pcg32 rng, int mode, int sign
         }
         if (EMIT) {
             atomicAdd(P.H + (size_t)box * P.n + nm, sign);
-            if (slot < P.cap) {
+            if (slot < limit) {
                 BPart q{x, y, (uint32_t)nm, ((uint32_t)box << 1) | (sign < 0 ? 1u : 0u), pid};
                 out[slot] = q;
             }
         }
+        nm0 = nm;
+        ns0 = sign;
         return 1;
     }
     int col = mode - P.ngamma;
@@ -160,10 +198,14 @@ __device__ inline int scatter(const BertParams& P, Pcg32 rng, int mode, int sign
         int k = (int)alias_pick(tab, (uint32_t)P.ncol, rng, neg);
         int nm = k + P.ngamma;
         int ns = neg ? -sign : sign;
+        if (nchild == 0) {
+            nm0 = nm;
+            ns0 = ns;
+        }
         if (EMIT) {
             atomicAdd(P.H + (size_t)box * P.n + nm, ns);
             uint64_t s = slot + nchild;
-            if (s < P.cap) {
+            if (s < limit) {
                 uint64_t cpid = nchild == 0 ? pid : mix64(pid ^ ((uint64_t)step << 40) ^ (uint64_t)nchild);
                 BPart q{x, y, (uint32_t)nm, ((uint32_t)box << 1) | (ns < 0 ? 1u : 0u), cpid};
                 out[s] = q;
@@ -176,43 +218,158 @@ __device__ inline int scatter(const BertParams& P, Pcg32 rng, int mode, int sign
     return nchild;
 }
 
-// wave-wide exclusive prefix sum of c; returns the base slot for this lane
-__device__ inline uint64_t wave_reserve(uint32_t* counter, int c, uint64_t cap, uint32_t* flags) {
-    const int lane = threadIdx.x & 63;
-    int incl = c;
-    for (int d = 1; d < 64; d <<= 1) {
-        int v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
+// The particle buffers are split into segments, one per writing workgroup
+// (step, emission and source kernels). A workgroup appends to its own
+// segment through an LDS counter and stores the final count once, so
+// compaction needs no global atomics. Readers see the segments as one list
+// through the prefix sum of their counts.
+#define BERT_MAXSEG 2048
+
+// inclusive prefix of the segment counts into pre[nsegs] (blockDim.x 256)
+__device__ inline uint32_t seg_prefix(const uint32_t* segc, int nsegs, uint32_t segcap, uint32_t* pre,
+                                      uint32_t* tsum) {
+    const int t = threadIdx.x;
+    const int per = (nsegs + 255) / 256;
+    const int i0 = t * per, i1 = min(nsegs, i0 + per);
+    uint32_t acc = 0;
+    for (int i = i0; i < i1; i++) {
+        acc += min(segc[i], segcap);
+        pre[i] = acc;
     }
-    int total = __shfl(incl, 63, 64);
-    uint32_t base = 0;
-    if (lane == 63 && total > 0) base = atomicAdd(counter, (uint32_t)total);
-    base = __shfl(base, 63, 64);
-    uint64_t b = (uint64_t)base + (uint64_t)(incl - c);
-    if (c > 0 && b + c > cap) atomicOr(flags, 1u);
-    return b;
+    tsum[t] = acc;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        uint32_t v = t >= d ? tsum[t - d] : 0u;
+        __syncthreads();
+        tsum[t] += v;
+        __syncthreads();
+    }
+    const uint32_t off = t > 0 ? tsum[t - 1] : 0u;
+    for (int i = i0; i < i1; i++) pre[i] += off;
+    __syncthreads();
+    return tsum[255];
+}
+// first segment s with pre[s] > i
+__device__ inline int seg_search(const uint32_t* pre, int nsegs, uint32_t i) {
+    int l = 0, h = nsegs - 1;
+    while (l < h) {
+        int mid = (l + h) >> 1;
+        if (pre[mid] <= i) l = mid + 1; else h = mid;
+    }
+    return l;
 }
 
+// wave-wide exclusive prefix sum of c; one LDS atomic on the workgroup's
+// segment counter gives the wave's base. Returns this lane's slot.
+__device__ inline uint64_t wave_reserve(uint32_t* lds_count, int myseg, uint32_t segcap, int c,
+                                        uint32_t* flags, uint64_t& limit) {
+    const int lane = threadIdx.x & 63;
+    int incl, total;
+    if (__ballot(c > 1) == 0ULL) {
+        // every lane has 0 or 1 child: prefix from the ballot mask
+        const unsigned long long m = __ballot(c > 0);
+        incl = __popcll(m & ((2ULL << lane) - 1ULL));
+        total = __popcll(m);
+    } else {
+        incl = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        total = __shfl(incl, 63, 64);
+    }
+    uint32_t base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(lds_count, (uint32_t)total);
+    base = __shfl(base, 63, 64);
+    uint64_t b = (uint64_t)base + (uint64_t)(incl - c);
+    if (c > 0 && b + c > segcap) atomicOr(flags, 1u);
+    limit = (uint64_t)(myseg + 1) * segcap;
+    return (uint64_t)myseg * segcap + b;
+}
+// stores the workgroup's segment count (after all its waves appended)
+__device__ inline void seg_close(const BertParams& P, uint32_t* segout, int myseg, const uint32_t* lds_count) {
+    __syncthreads();
+    if (threadIdx.x == 0) segout[myseg] = min(*lds_count, P.segcap);
+}
+
+// count the children, reserve their slots in the workgroup's segment and
+// write them; a single child (the common case) is written from pass 0's result
+__device__ inline void scatter_out(const BertParams& P, Pcg32& rng, bool alive, int mode, int sign, int box,
+                                   double x, double y, uint64_t pid, uint32_t step, uint32_t* lds_count,
+                                   int myseg) {
+    int nm0 = 0, ns0 = 1;
+    int c = alive ? scatter<false>(P, rng, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
+    uint64_t limit;
+    uint64_t slot = wave_reserve(lds_count, myseg, P.segcap, c, P.flags, limit);
+    if (c == 1) {
+        atomicAdd(P.H + (size_t)box * P.n + nm0, ns0);
+        if (slot < limit) {
+            BPart q{x, y, (uint32_t)nm0, ((uint32_t)box << 1) | (ns0 < 0 ? 1u : 0u), pid};
+            P.pout[slot] = q;
+        }
+    } else if (c > 1) {
+        scatter<true>(P, rng, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
+    }
+}
+
+// Wall counters are accumulated per workgroup in LDS and flushed with one
+// global atomic per touched side (the walls of a few boxes take most hits).
+#define BERT_EB_LDS 1024
+__device__ inline int32_t* eb_lds_begin(int32_t* sEb, int32_t* Eb, int nb4) {
+    if (nb4 > BERT_EB_LDS) return Eb;
+    for (int i = threadIdx.x; i < nb4; i += blockDim.x) sEb[i] = 0;
+    return sEb;
+}
+__device__ inline void eb_lds_flush(const int32_t* sEb, int32_t* Eb, int nb4) {
+    if (nb4 > BERT_EB_LDS) return;
+    __syncthreads();
+    for (int i = threadIdx.x; i < nb4; i += blockDim.x) {
+        int v = sEb[i];
+        if (v) atomicAdd(Eb + i, v);
+    }
+}
+
+// One workgroup per segment of the step's output. Workgroup w takes the
+// w-th equal slice of the live particle list (all input segments).
 __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t step, int cur,
                                                         int cancel_prev) {
-    const uint32_t nin = min(P.count[cur], (uint32_t)P.cap);
+    __shared__ int32_t sEb[BERT_EB_LDS];
+    __shared__ uint32_t pre[BERT_MAXSEG];
+    __shared__ uint32_t tsum[256];
+    __shared__ uint32_t lcount;
+    __shared__ GeoLds geo;
+    const DevBox* boxes;
+    const DevContact* cts;
+    geo_stage(P, geo, boxes, cts);
+    const int nb4 = P.nbox * 4;
+    int32_t* Eg = P.Eb + (size_t)(step & 1) * nb4;
+    int32_t* Eb = eb_lds_begin(sEb, Eg, nb4);
+    if (threadIdx.x == 0) lcount = 0;
+    const uint32_t* segin = P.seg + (size_t)cur * P.nsegs;
+    uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
+    const uint32_t nin = seg_prefix(segin, P.nsegs, P.segcap, pre, tsum);  // ends with a barrier
+    const uint32_t lo = (uint32_t)((uint64_t)nin * blockIdx.x / gridDim.x);
+    const uint32_t hi = (uint32_t)((uint64_t)nin * (blockIdx.x + 1) / gridDim.x);
+    const int myseg = blockIdx.x;
     int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
     const int32_t* Hprev = P.H + (size_t)(cur ^ 1) * P.nbox * P.n;
-    int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
     BertParams Q = P;
     Q.H = H;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    // whole waves iterate together so wave_reserve sees every lane
-    const uint64_t nround = ((uint64_t)nin + 63) & ~63ULL;
+    const uint64_t skey = stream_key(P.seed, step, 0);
     unsigned long long done = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i - (threadIdx.x & 63) < nround; i += stride) {
-        bool valid = i < nin;
+    for (uint32_t base = lo; base < hi; base += blockDim.x) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < hi;
+        // segment of the wave's first index, then a short walk per lane
+        int sg = seg_search(pre, P.nsegs, base + (threadIdx.x & ~63u));
         BPart p{};
         int box = 0, sign = 1, mode = 0;
         bool alive = false;
         Pcg32 rng;
         if (valid) {
-            p = P.pin[i];
+            while (sg < P.nsegs - 1 && i >= pre[sg]) sg++;
+            const uint32_t off = i - (sg > 0 ? pre[sg - 1] : 0u);
+            p = P.pin[(uint64_t)sg * P.segcap + off];
             box = (int)(p.boxsign >> 1);
             sign = (p.boxsign & 1u) ? -1 : 1;
             mode = (int)p.mode;
@@ -229,46 +386,44 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
                 }
             }
             if (alive) {
-                seed_stream(rng, P.seed, p.pid, step, 0);
+                seed_stream(rng, skey, p.pid);
                 const double2 v = P.vel[mode];
-                alive = advect(P, p.x, p.y, box, sign, v.x, v.y, P.dt, rng, Eb);
+                alive = advect(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, P.dt, rng, Eb);
             }
         }
-        int c = alive ? scatter<false>(Q, rng, mode, sign, box, nullptr, 0, 0, 0, 0, 0) : 0;
-        uint64_t slot = wave_reserve(P.count + (cur ^ 1), c, P.cap, P.flags);
-        if (c > 0) scatter<true>(Q, rng, mode, sign, box, P.pout, slot, p.x, p.y, p.pid, step);
+        scatter_out(Q, rng, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
+    seg_close(P, segout, myseg, &lcount);
+    eb_lds_flush(sEb, Eg, nb4);
     // one atomic per wave for the particle-step counter
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
 }
 
-// One workgroup per (box, side): adiabatic diffuse wall re-emission
-// (beyondRTA_MC2d.cpp:281-435). Needs n <= BERT_WALL_MAXN.
-#define BERT_WALL_MAXN 7680
-__global__ __launch_bounds__(256) void bert_walls_kernel(BertParams P, uint32_t step, int cur) {
-    __shared__ double cdf[BERT_WALL_MAXN];
+// Adiabatic diffuse walls (beyondRTA_MC2d.cpp:281-435), phase 1: one
+// workgroup per (box, side) with a nonzero Eborder computes the border
+// temperature from the J(T) spline and the emission cumulative of
+// Isothermal_wall_distribution(Tborder, Teq, -n) (sampling.cpp:189-269).
+__global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_t step) {
     __shared__ double wsum[256];
     const int b = blockIdx.x;  // box*4 + side
     const int box = b >> 2, side = b & 3;
     int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
-    int32_t* Ebn = P.Eb + (size_t)((step + 1) & 1) * P.nbox * 4;
     const int E = Eb[b];
-    if (E == 0) return;
     const int si = P.side_spl[b];
+    if (E == 0 || si < 0) {
+        if (threadIdx.x == 0) {
+            P.wcount[b] = 0;
+            Eb[b] = 0;
+        }
+        return;
+    }
     __syncthreads();
     if (threadIdx.x == 0) Eb[b] = 0;
-    if (si < 0) return;
     const DevBox B = P.boxes[box];
     const double Teq = P.box_Teq[box];
-    double p1x, p1y, ndx, ndy;
-    if (side == 0) { p1x = B.xmin; p1y = B.ymin; ndx = 0; ndy = B.ymax - B.ymin; }
-    else if (side == 1) { p1x = B.xmin; p1y = B.ymin; ndx = B.xmax - B.xmin; ndy = 0; }
-    else if (side == 2) { p1x = B.xmin; p1y = B.ymax; ndx = B.xmax - B.xmin; ndy = 0; }
-    else { p1x = B.xmax; p1y = B.ymax; ndx = 0; ndy = B.ymin - B.ymax; }
-    const double L = sqrt(ndx * ndx + ndy * ndy);
-    // border temperature from the J(T) spline
+    const double L = (side == 0 || side == 3) ? (B.ymax - B.ymin) : (B.xmax - B.xmin);
     const double J = P.Eeff * (double)E / P.dt / (L * P.thickness);
     const double* S = P.spl + (size_t)si * 6 * P.nseg;  // a b c d x0 xe
     int lo = 0, hi = P.nseg;
@@ -283,8 +438,8 @@ __global__ __launch_bounds__(256) void bert_walls_kernel(BertParams P, uint32_t 
     side_normal(side, nx, ny);
     nx = -nx;  // emission normal points into the box
     ny = -ny;
-    // wall pmf (sampling.cpp:227-240) and its inclusive scan in LDS
     const int n = P.n;
+    double* cdf = P.wcdf + (size_t)b * n;
     const int per = (n + blockDim.x - 1) / blockDim.x;
     const int i0 = threadIdx.x * per, i1 = min(n, i0 + per);
     double acc = 0;
@@ -308,43 +463,82 @@ __global__ __launch_bounds__(256) void bert_walls_kernel(BertParams P, uint32_t 
     }
     const double off = threadIdx.x > 0 ? wsum[threadIdx.x - 1] : 0.;
     for (int i = i0; i < i1; i++) cdf[i] += off;
-    __syncthreads();
-    const double total = cdf[n - 1];
-    const int sign = (Tb - Teq) >= 0 ? 1 : -1;
-    const int ng = (int)(fabs((double)E) + 1.0e-6);
+    if (threadIdx.x == 0) {
+        P.wcount[b] = wsum[255] > 0. ? (uint32_t)(fabs((double)E) + 1.0e-6) : 0u;
+        P.wsign[b] = (Tb - Teq) >= 0 ? 1 : -1;
+    }
+}
+
+// Phase 2: the |Eborder| particles of every side form one index space split
+// in equal slices over the workgroups; each is placed uniformly on its side,
+// advected U*dt, scattered and histogrammed.
+#define BERT_EMIT_MAXB 4096
+__global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t step, int cur) {
+    __shared__ uint32_t pre[BERT_EMIT_MAXB];
+    __shared__ uint32_t tsum[256];
+    __shared__ int32_t sEb[BERT_EB_LDS];
+    __shared__ uint32_t lcount;
+    __shared__ GeoLds geo;
+    const int nb4 = P.nbox * 4;
+    const int myseg = P.seg_emit0 + blockIdx.x;
+    uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
+    if (threadIdx.x == 0) lcount = 0;
+    const DevBox* boxes;
+    const DevContact* cts;
+    geo_stage(P, geo, boxes, cts);
+    int32_t* Eg = P.Eb + (size_t)((step + 1) & 1) * nb4;
+    int32_t* Ebn = eb_lds_begin(sEb, Eg, nb4);
+    const uint32_t total = seg_prefix(P.wcount, nb4, 0xffffffffu, pre, tsum);  // ends with a barrier
+    const uint32_t lo = (uint32_t)((uint64_t)total * blockIdx.x / gridDim.x);
+    const uint32_t hi = (uint32_t)((uint64_t)total * (blockIdx.x + 1) / gridDim.x);
     int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
     BertParams Q = P;
     Q.H = H;
-    const int nloop = (ng + 63) & ~63;
+    const uint64_t skey = stream_key(P.seed, step, 1);
     unsigned long long done = 0;
-    for (int k = threadIdx.x; k - (threadIdx.x & 63) < nloop; k += blockDim.x) {
-        bool valid = k < ng && total > 0.;
+    for (uint32_t base = lo; base < hi; base += blockDim.x) {
+        const uint32_t k = base + threadIdx.x;
+        const bool valid = k < hi;
         Pcg32 rng;
-        int mode = 0, bx = box;
+        int mode = 0, bx = 0, sign = 1;
         double x = 0, y = 0;
         bool alive = false;
-        uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)b << 20) ^ (uint64_t)k ^ 0xad1aULL);
+        uint64_t pid = 0;
         if (valid) {
-            seed_stream(rng, P.seed, pid, step, 1);
-            double u = rng.uniform() * total;
-            int l = 0, h = n - 1;
+            // side b holds indices [pre[b-1], pre[b])
+            const int b = seg_search(pre, nb4, k);
+            const uint32_t j = k - (b > 0 ? pre[b - 1] : 0u);
+            bx = b >> 2;
+            const int side = b & 3;
+            sign = P.wsign[b];
+            pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)b << 20) ^ (uint64_t)j ^ 0xad1aULL);
+            seed_stream(rng, skey, pid);
+            const double* cdf = P.wcdf + (size_t)b * P.n;
+            double u = rng.uniform() * cdf[P.n - 1];
+            int l = 0, h = P.n - 1;
             while (l < h) {
                 int mid = (l + h) >> 1;
                 if (cdf[mid] < u) l = mid + 1; else h = mid;
             }
             mode = l;
+            const DevBox B = boxes[bx];
+            double p1x, p1y, ndx, ndy;
+            if (side == 0) { p1x = B.xmin; p1y = B.ymin; ndx = 0; ndy = B.ymax - B.ymin; }
+            else if (side == 1) { p1x = B.xmin; p1y = B.ymin; ndx = B.xmax - B.xmin; ndy = 0; }
+            else if (side == 2) { p1x = B.xmin; p1y = B.ymax; ndx = B.xmax - B.xmin; ndy = 0; }
+            else { p1x = B.xmax; p1y = B.ymax; ndx = 0; ndy = B.ymin - B.ymax; }
             double t = rng.uniform();
             x = p1x + t * ndx;
             y = p1y + t * ndy;
             const double2 v = P.vel[mode];
             double rdt = P.dt * rng.uniform();
-            alive = advect(P, x, y, bx, sign, v.x, v.y, rdt, rng, Ebn);
+            alive = advect(P, boxes, cts, x, y, bx, sign, v.x, v.y, rdt, rng, Ebn);
         }
-        int c = alive ? scatter<false>(Q, rng, mode, sign, bx, nullptr, 0, 0, 0, 0, 0) : 0;
-        uint64_t slot = wave_reserve(P.count + (cur ^ 1), c, P.cap, P.flags);
-        if (c > 0) scatter<true>(Q, rng, mode, sign, bx, P.pout, slot, x, y, pid, step);
+        scatter_out(Q, rng, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
+    seg_close(P, segout, myseg, &lcount);
+    eb_lds_flush(sEb, Eg, nb4);
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
@@ -360,10 +554,16 @@ struct BertSource {
 };
 __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src,
                                                           const uint2* stab, uint32_t step, int cur) {
+    __shared__ uint32_t lcount;
     const BertSource S = src[blockIdx.x];
+    const int myseg = P.seg_src0 + blockIdx.x;
+    uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
+    if (threadIdx.x == 0) lcount = 0;
+    __syncthreads();
     int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
     Pcg32 cr;
-    seed_stream(cr, P.seed, 0x5ce0000000000000ULL + blockIdx.x, step, 2);
+    seed_stream(cr, stream_key(P.seed, step, 2), 0x5ce0000000000000ULL + blockIdx.x);
+    const uint64_t skey = stream_key(P.seed, step, 3);
     const double fl = floor(S.expected);
     const int ng = (int)fl + (cr.uniform() <= S.expected - fl ? 1 : 0);
     int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
@@ -371,7 +571,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     Q.H = H;
     const int nloop = (ng + 63) & ~63;
     unsigned long long done = 0;
-    for (int k = threadIdx.x; k - (threadIdx.x & 63) < nloop; k += blockDim.x) {
+    for (int k = threadIdx.x; k - (int)(threadIdx.x & 63) < nloop; k += blockDim.x) {
         bool valid = k < ng;
         Pcg32 rng;
         int mode = 0, bx = S.box;
@@ -379,7 +579,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
         bool alive = false;
         uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)blockIdx.x << 20) ^ (uint64_t)k ^ 0x15a0ULL);
         if (valid) {
-            seed_stream(rng, P.seed, pid, step, 3);
+            seed_stream(rng, skey, pid);
             double t = rng.uniform();
             x = S.p1x + t * S.ndx;
             y = S.p1y + t * S.ndy;
@@ -394,13 +594,12 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
             mode = (int)alias_pick(stab + S.tab, (uint32_t)P.n, rng, ngf);
             const double2 v = P.vel[mode];
             double rdt = P.dt * rng.uniform();
-            alive = advect(P, x, y, bx, S.sign, v.x, v.y, rdt, rng, Eb);
+            alive = advect(P, P.boxes, P.cts, x, y, bx, S.sign, v.x, v.y, rdt, rng, Eb);
         }
-        int c = alive ? scatter<false>(Q, rng, mode, S.sign, bx, nullptr, 0, 0, 0, 0, 0) : 0;
-        uint64_t slot = wave_reserve(P.count + (cur ^ 1), c, P.cap, P.flags);
-        if (c > 0) scatter<true>(Q, rng, mode, S.sign, bx, P.pout, slot, x, y, pid, step);
+        scatter_out(Q, rng, alive, mode, S.sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
+    seg_close(P, segout, myseg, &lcount);
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
@@ -453,25 +652,42 @@ __global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, 
         o[2] = skip ? 0. : P.Eeff / vol * 1.0e30 * red[2][0];
         // particles left after this step: sum |h| over regular boxes when
         // the step cancels, else every particle written this step
-        if (cancel_now) {
-            if (!skip) atomicAdd((unsigned long long*)(P.npart + row), (unsigned long long)redc[1][0]);
-        } else if (box == 0) {
-            P.npart[row] = (int64_t)P.count[cur ^ 1];
-        }
-        if (box == 0) P.count[cur] = 0;
+        if (cancel_now && !skip)
+            atomicAdd((unsigned long long*)(P.npart + row), (unsigned long long)redc[1][0]);
         const int32_t* E = P.Eb + (size_t)ebn * P.nbox * 4 + (size_t)box * 4;
         long long pend = (long long)E[0] + E[1] + E[2] + E[3];
         if (pend) atomicAdd((unsigned long long*)(P.pending + row), (unsigned long long)pend);
+    }
+    if (box == 0) {
+        // live particles written this step; recycle the input counters
+        __syncthreads();
+        uint32_t* segin = P.seg + (size_t)cur * P.nsegs;
+        const uint32_t* segw = P.seg + (size_t)(cur ^ 1) * P.nsegs;
+        long long c = 0;
+        for (int k = threadIdx.x; k < P.nsegs; k += blockDim.x) {
+            c += min(segw[k], P.segcap);
+            segin[k] = 0;
+        }
+        redc[0][threadIdx.x] = c;
+        __syncthreads();
+        for (int d = 128; d > 0; d >>= 1) {
+            if (threadIdx.x < d) redc[0][threadIdx.x] += redc[0][threadIdx.x + d];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && !cancel_now) P.npart[row] = (int64_t)redc[0][0];
     }
 }
 
 // initial particles: histogram into H[cur] (properties row 0)
 __global__ __launch_bounds__(256) void bert_hist_kernel(BertParams P, int cur) {
-    const uint32_t nin = min(P.count[cur], (uint32_t)P.cap);
+    __shared__ uint32_t pre[BERT_MAXSEG];
+    __shared__ uint32_t tsum[256];
+    const uint32_t nin = seg_prefix(P.seg + (size_t)cur * P.nsegs, P.nsegs, P.segcap, pre, tsum);
     int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nin;
          i += (uint64_t)gridDim.x * blockDim.x) {
-        const BPart p = P.pin[i];
+        const int sg = seg_search(pre, P.nsegs, (uint32_t)i);
+        const BPart p = P.pin[(uint64_t)sg * P.segcap + ((uint32_t)i - (sg > 0 ? pre[sg - 1] : 0u))];
         int box = (int)(p.boxsign >> 1);
         int sign = (p.boxsign & 1u) ? -1 : 1;
         atomicAdd(H + (size_t)box * P.n + p.mode, sign);
@@ -479,7 +695,8 @@ __global__ __launch_bounds__(256) void bert_hist_kernel(BertParams P, int cur) {
 }
 
 // initial particles from outTref distributions (beyondRTA_MC2d.cpp:211-269):
-// one workgroup per box with its count and alias table
+// one workgroup per box with its count and alias table; particle pid goes to
+// segment pid % nsegs
 struct BertInit {
     double xmin, xmax, ymin, ymax;
     uint64_t first, count;
@@ -491,13 +708,13 @@ __global__ __launch_bounds__(256) void bert_init_kernel(BertParams P, const Bert
     for (uint64_t k = threadIdx.x; k < I.count; k += blockDim.x) {
         uint64_t pid = I.first + k;
         Pcg32 rng;
-        seed_stream(rng, P.seed, pid, 0xffffffffu, 4);
+        seed_stream(rng, stream_key(P.seed, 0xffffffu, 4), pid);
         double x = I.xmin + rng.uniform() * (I.xmax - I.xmin);
         double y = I.ymin + rng.uniform() * (I.ymax - I.ymin);
         bool ng;
         int mode = (int)alias_pick(itab + I.tab, (uint32_t)P.n, rng, ng);
         BPart q{x, y, (uint32_t)mode, ((uint32_t)I.box << 1) | (I.sign < 0 ? 1u : 0u), pid};
-        P.pin[pid] = q;
+        P.pin[(pid % P.nsegs) * P.segcap + pid / P.nsegs] = q;
     }
 }
 
@@ -513,15 +730,19 @@ struct Bert {
     int nseg = 99;
     DevBuf<double2> d_vel;
     DevBuf<BPart> d_parts;
-    DevBuf<uint32_t> d_count, d_tickets, d_flags;
+    DevBuf<uint32_t> d_seg, d_tickets, d_flags;
     DevBuf<int32_t> d_H, d_Eb, d_side_spl, d_rta_slot;
     DevBuf<double> d_props, d_spl, d_Teq;
     DevBuf<int64_t> d_npart, d_pending;
+    DevBuf<double> d_wcdf;
+    DevBuf<uint32_t> d_wcount;
+    DevBuf<int32_t> d_wsign;
     DevBuf<unsigned long long> d_stats;
     DevBuf<uint2> d_rta_alias, d_src_tab, d_init_tab;
     DevBuf<BertSource> d_src;
     DevBuf<BertInit> d_init;
     int nsrc = 0, ninit = 0;
+    int step_blocks = 0, emit_blocks = 0, nsegs = 0;
     std::vector<int32_t> hH;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;  // begin/end event per step-kernel launch
@@ -555,16 +776,24 @@ static BertParams make_params(Bert* b) {
     P.box_Teq = b->d_Teq.p;
     P.spl = b->d_spl.p;
     P.side_spl = b->d_side_spl.p;
-    P.count = b->d_count.p;
+    P.seg = b->d_seg.p;
     P.H = b->d_H.p;
     P.tickets = b->d_tickets.p;
     P.Eb = b->d_Eb.p;
     P.props = b->d_props.p;
     P.npart = b->d_npart.p;
     P.pending = b->d_pending.p;
+    P.wcdf = b->d_wcdf.p;
+    P.wcount = b->d_wcount.p;
+    P.wsign = b->d_wsign.p;
     P.flags = b->d_flags.p;
     P.stats = b->d_stats.p;
     P.cap = b->cap;
+    P.nsegs = b->nsegs;
+    P.segcap = (uint32_t)(b->cap / b->nsegs);
+    P.seg_emit0 = b->step_blocks;
+    P.seg_src0 = b->step_blocks + b->emit_blocks;
+    P.ncts = (int)b->g->dcontacts.size();
     P.seed = b->cfg.seed;
     P.dt = b->cfg.dt;
     P.Eeff = b->Eeff;
@@ -643,7 +872,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     if (!cfg->rta && !p) throw Error(BTE_EINVAL, "a propagator is required unless rta is set");
     if (p && p->ngamma + p->ncol != m->n) throw Error(BTE_EINVAL, "propagator size does not match the mode table");
     if (!(cfg->dt > 0) || !(cfg->thickness > 0)) throw Error(BTE_EINVAL, "bad beRTA configuration");
-    if (m->n > BERT_WALL_MAXN) throw Error(BTE_EINVAL, "mode count above the wall-emission limit");
+    if ((size_t)g->boxes.size() * 4 > BERT_EMIT_MAXB) throw Error(BTE_EINVAL, "more than 1024 boxes");
     HIPCHECK(hipSetDevice(c->device));
     auto* b = new Bert();
     std::unique_ptr<Bert> guard(b);
@@ -718,14 +947,31 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     }
     b->n0 = total;
     b->ninit = (int)inits.size();
-    b->cap = cfg->capacity ? cfg->capacity : std::max<uint64_t>(1u << 20, 2 * total + (total >> 1));
+    // one buffer segment per writing workgroup: step, emission, sources
+    b->nsrc = 0;
+    for (int sid = 0; sid < nbox; sid++) {
+        if (g->boxes[sid].kind != 1) continue;
+        for (auto& hc : g->contacts[sid]) {
+            const RectBox& cb = g->boxes[hc.box];
+            if (cb.kind != 0 || almost_equal(hc.length(), 0.) || almost_equal(g->boxes[sid].Teq, cb.Teq)) continue;
+            b->nsrc++;
+        }
+    }
+    b->step_blocks = std::min(4 * c->cu_count, 2048);
+    b->emit_blocks = std::min(c->cu_count, 1024);
+    b->nsegs = b->step_blocks + b->emit_blocks + b->nsrc;
+    if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
+    b->cap = cfg->capacity ? cfg->capacity : std::max<uint64_t>(1u << 21, 3 * total);
+    b->cap = (b->cap + b->nsegs - 1) / b->nsegs * b->nsegs;
     if (b->cap >= 0xffffffffULL) throw Error(BTE_EINVAL, "capacity above 2^32 particles");
+    if ((total + b->nsegs - 1) / b->nsegs > b->cap / b->nsegs)
+        throw Error(BTE_ECAPACITY, "initial particles exceed the capacity");
     // device buffers
     std::vector<double2> vel(n);
     for (int k = 0; k < n; k++) vel[k] = make_double2(m->vx[k], m->vy[k]);
     b->d_vel.upload(vel, s);
     b->d_parts.alloc(2 * b->cap);
-    b->d_count.alloc(2);
+    b->d_seg.alloc(2 * (size_t)b->nsegs);
     b->d_flags.alloc(4);
     b->d_stats.alloc(2);
     b->d_H.alloc((size_t)2 * nbox * n);
@@ -734,6 +980,9 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     b->d_props.alloc((size_t)(b->max_steps + 1) * nbox * 3);
     b->d_npart.alloc((size_t)(b->max_steps + 1));
     b->d_pending.alloc((size_t)(b->max_steps + 1));
+    b->d_wcdf.alloc((size_t)nbox * 4 * n);
+    b->d_wcount.alloc((size_t)nbox * 4);
+    b->d_wsign.alloc((size_t)nbox * 4);
     std::vector<double> Teq(nbox);
     for (int i = 0; i < nbox; i++) Teq[i] = g->boxes[i].Teq;
     b->d_Teq.upload(Teq, s);
@@ -812,7 +1061,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
             srcs.push_back(S);
         }
     }
-    b->nsrc = (int)srcs.size();
+    if ((int)srcs.size() != b->nsrc) throw Error(BTE_EINVAL, "source count mismatch");
     if (b->nsrc) {
         b->d_src.upload(srcs, s);
         b->d_src_tab.upload(stab, s);
@@ -838,7 +1087,6 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     const int n = b->m->n, nbox = (int)b->g->boxes.size();
     BertParams P = make_params(b);
     BPart* buf[2] = {b->d_parts.p, b->d_parts.p + b->cap};
-    if (b->n0 > b->cap) throw Error(BTE_ECAPACITY, "initial particles exceed the capacity");
     HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)2 * nbox * n * sizeof(int32_t), s));
     HIPCHECK(hipMemsetAsync(b->d_tickets.p, 0, (size_t)nbox * n * sizeof(uint32_t), s));
     HIPCHECK(hipMemsetAsync(b->d_Eb.p, 0, (size_t)2 * nbox * 4 * sizeof(int32_t), s));
@@ -846,8 +1094,10 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     HIPCHECK(hipMemsetAsync(b->d_stats.p, 0, 2 * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(b->d_npart.p, 0, (size_t)(b->max_steps + 1) * sizeof(int64_t), s));
     HIPCHECK(hipMemsetAsync(b->d_pending.p, 0, (size_t)(b->max_steps + 1) * sizeof(int64_t), s));
-    uint32_t c0[2] = {(uint32_t)b->n0, 0u};
-    HIPCHECK(hipMemcpyAsync(b->d_count.p, c0, sizeof(c0), hipMemcpyHostToDevice, s));
+    std::vector<uint32_t> c0(2 * (size_t)b->nsegs, 0u);
+    for (int k = 0; k < b->nsegs; k++)
+        c0[k] = (uint32_t)(b->n0 / b->nsegs + ((uint64_t)k < b->n0 % b->nsegs ? 1 : 0));
+    HIPCHECK(hipMemcpyAsync(b->d_seg.p, c0.data(), c0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     P.pin = buf[0];
     P.pout = buf[1];
     if (b->ninit) {
@@ -857,15 +1107,17 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     // row 0: properties of the initial particles
     hipLaunchKernelGGL(bert_hist_kernel, dim3(c->cu_count * 4), dim3(256), 0, s, P, 0);
     HIPCHECK(hipGetLastError());
-    // props of H[0] into row 0; the kernel clears H[1] and count[0]; restore
-    // count[0] afterwards because here the initial particles are the input
+    // props of H[0] into row 0; the kernel clears H[1] and the input segment
+    // counters; restore them afterwards because the initial particles are
+    // the first step's input
     hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, 0, 0, 0, 0, 0);
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipMemcpyAsync(b->d_count.p, c0, sizeof(c0), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(b->d_seg.p, c0.data(), c0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)nbox * n * sizeof(int32_t), s));
     HIPCHECK(hipMemcpyAsync(b->d_npart.p, &b->n0, sizeof(int64_t), hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
-    const int step_blocks = c->cu_count * 8;
+    const int step_blocks = b->step_blocks;
+    const int emit_blocks = b->emit_blocks;
     const int pc = b->cfg.pcancel_each;
     auto t0 = std::chrono::steady_clock::now();
     HIPCHECK(hipEventRecord(b->ev0, s));
@@ -883,7 +1135,8 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         if (b->nsrc)
             hipLaunchKernelGGL(bert_source_kernel, dim3(b->nsrc), dim3(256), 0, s, P, b->d_src.p,
                                b->d_src_tab.p, (uint32_t)istep, cur);
-        hipLaunchKernelGGL(bert_walls_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep, cur);
+        hipLaunchKernelGGL(bert_wallcdf_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep);
+        hipLaunchKernelGGL(bert_emit_kernel, dim3(emit_blocks), dim3(256), 0, s, P, (uint32_t)istep, cur);
         // H of this step lives in H[cur]; H[cur^1] (previous step) is cleared
         // after the step kernel used it for cancellation
         hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, istep + 1, cur, cancel_now,

@@ -83,6 +83,9 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
     int still = 0;        // consecutive events without time advance
     double tlast = -1.0;
     const double maxtime = P.maxtime;
+    // particle ids come from a per-wave pool refilled 64 at a time, so the
+    // global id counter sees one atomic per 64 histories
+    unsigned long long pool_lo = 0, pool_hi = 0;
     for (;;) {
         // every history ends: a particle that exceeds the iteration budget or
         // makes 64 events in a row without advancing in time is dropped and
@@ -95,11 +98,29 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
                 alive = false;
             }
         }
+        unsigned long long pid = ~0ULL;
+        unsigned long long want = __ballot(!alive);
+        while (want) {
+            if (pool_lo == pool_hi) {
+                unsigned long long base = 0;
+                const int leader = __ffsll((long long)want) - 1;
+                if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(P.next_pid, 64ULL);
+                base = __shfl(base, leader, 64);
+                pool_lo = base;
+                pool_hi = base + 64;
+            }
+            const unsigned long long lt = want & ((1ULL << (threadIdx.x & 63)) - 1ULL);
+            const unsigned long long rank = (unsigned long long)__popcll(lt);
+            const unsigned long long take = min((unsigned long long)__popcll(want), pool_hi - pool_lo);
+            const bool mine = (want >> (threadIdx.x & 63)) & 1ULL;
+            if (mine && rank < take) pid = pool_lo + rank;
+            pool_lo += take;
+            want = __ballot(!alive && pid == ~0ULL);
+        }
         if (!alive) {
             budget = 0;
             still = 0;
             tlast = -1.0;
-            unsigned long long pid = atomicAdd(P.next_pid, 1ULL);
             if (pid >= P.nparticles) break;
             rng.seed(P.seed, pid);
             // Init_generator (RTA_MC2D.cpp:436-484)

@@ -1,9 +1,11 @@
 """GPU parity of the HIP path against the oracle on the same seeded inputs.
 
 Monte Carlo results are compared statistically: R independent replicas on
-each side give per-cell means and standard errors; every compared cell must
-satisfy |mean_gpu - mean_oracle| < 5 sigma_diff and the mean of z^2 over the
-cells must stay below 2 (a correct pair gives ~1)."""
+each side give per-cell means and standard errors. Every compared cell must
+satisfy |mean_gpu - mean_oracle| < 5 sigma_diff, and a few scalar summaries
+of each run (spatial means, gradients, event counts) must agree within
+4 sigma_diff. Cells of one run are correlated (a replica with more flux has
+more flux everywhere), so no statistic pools z^2 over cells."""
 import numpy as np
 import pytest
 
@@ -34,7 +36,7 @@ def _rta_setup(pkg, ctx, nbins=20, nmodes=600):
 def test_rta_matches_oracle(pkg, ctx):
     mat, rect, kind, teq, modes, geom = _rta_setup(pkg, ctx)
     kw = dict(thickness=1.0, T0=300.0, dt=20.0, maxtime=400.0)
-    R, N = 8, 20000
+    R, N = 16, 20000
     g = [pkg.rtamc2d(ctx, modes, geom, nparticles=N, seed=100 + r, sigma_bounce=1.0, **kw) for r in range(R)]
     o = [orc.rta_run(mat, rect, kind, teq, nparticles=N, seed=200 + r, sigma=1.0, **kw) for r in range(R)]
     for key in ("T", "jx"):
@@ -42,10 +44,16 @@ def test_rta_matches_oracle(pkg, ctx):
         oa = np.array([x[key][[4, -1], :20] for x in o])
         z = zstats(ga, oa)
         assert np.abs(z).max() < 5.0, (key, z)
-        assert (z ** 2).mean() < 2.0, (key, (z ** 2).mean())
-    ev_g = np.mean([x["events"] for x in g])
-    ev_o = np.mean([x["events"] for x in o])
-    assert abs(ev_g / ev_o - 1) < 0.02
+    summaries = {
+        "jx_mean": lambda x: x["jx"][-1, :20].mean(),
+        "T_gradient": lambda x: x["T"][-1, :10].mean() - x["T"][-1, 10:20].mean(),
+        "T_early": lambda x: x["T"][4, :20].mean(),
+        "events": lambda x: float(x["events"]),
+        "crossings": lambda x: float(x["crossings"]),
+    }
+    for name, f in summaries.items():
+        z = zstats(np.array([f(x) for x in g]), np.array([f(x) for x in o]))
+        assert abs(z) < 4.0, (name, z)
 
 
 def test_bert_matches_oracle(pkg, ctx):
@@ -58,7 +66,7 @@ def test_bert_matches_oracle(pkg, ctx):
     geom = pkg.Geometry(ctx, rect, kind, teq, tin)
     prop = pkg.Propagator(ctx, P)
     kw = dict(thickness=1.0, dt=5.0, maxtime=5.0 * 19, Efactor=20000, Tmax=301.0, Tmin=300.0)
-    R = 8
+    R = 12
     g = [pkg.bertamc2d(ctx, modes, geom, prop, seed=10 + r, **kw) for r in range(R)]
     o = [orc.bert_run(mat, rect, kind, teq, tin, P=P, seed=20 + r, **kw) for r in range(R)]
     assert g[0]["nsteps"] == o[0]["nsteps"] == 20
@@ -67,7 +75,13 @@ def test_bert_matches_oracle(pkg, ctx):
         oa = np.array([x["props"][[3, 10, 20], :, q] for x in o])
         z = zstats(ga, oa)
         assert np.abs(z).max() < 5.0, (q, z)
-        assert (z ** 2).mean() < 2.0, (q, (z ** 2).mean())
+    # centre of energy and mean flux at the last step
+    xc = 0.5 * (rect[:, 0] + rect[:, 1])
+    for name, f in {"centre": lambda x: (x["props"][-1, :, 0] * xc).sum() / x["props"][-1, :, 0].sum(),
+                    "jx_mean": lambda x: x["props"][-1, :, 1].mean(),
+                    "npart_last": lambda x: float(x["npart"][-1])}.items():
+        z = zstats(np.array([f(x) for x in g]), np.array([f(x) for x in o]))
+        assert abs(z) < 4.0, (name, z)
     # P >= 0 and adiabatic walls conserve particles + pending wall counts
     # exactly (every particle here is positive)
     for x in g + o:
