@@ -1,0 +1,128 @@
+"""Whole-solver properties of the oracle's RTAMC2D and beRTAMC2D loops."""
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+
+def horizontal_material(n=200, seed=5):
+    """modes moving along x only (vy = 0): no contact with top/bottom walls"""
+    m = orc.synthetic_modes(n, seed)
+    sp = np.hypot(m["vx"], m["vy"])
+    m["vx"] = np.where(m["vx"] >= 0, sp, -sp)
+    m["vy"] = np.zeros(n)
+    m["V"] = 20.0
+    m["lattz"] = 1.0
+    return m
+
+
+def bar(nbins=10, dx=10.0, H=40.0, TL=300.5, TR=299.5):
+    rect = [[k * dx, (k + 1) * dx, 0.0, H] for k in range(nbins)]
+    rect += [[-dx, 0.0, 0.0, H], [nbins * dx, (nbins + 1) * dx, 0.0, H]]
+    kind = [0] * nbins + [1, 1]
+    teq = [300.0] * nbins + [TL, TR]
+    return np.array(rect), np.array(kind, np.int32), np.array(teq)
+
+
+def test_rta_ballistic_flux_is_exact():
+    """Ballistic horizontal particles cross every box once with displacement
+    dx, so the steady cumulative flux of every box equals the total emitted
+    wall power per unit cross-section: (|J_L| + |J_R|) * 1e30 W/m^2 (the
+    particle energy Eeff = einfo/N of RTA_MC2D.cpp:2177)."""
+    m = horizontal_material()
+    rect, kind, teq = bar()
+    out = orc.rta_run(m, rect, kind, teq, thickness=1.0, T0=300.0, dt=50.0, maxtime=1000.0, nparticles=20000,
+                      seed=3, ballistic=True, sigma=5.0)
+    vuc = m["V"] * 1.0 / m["lattz"]
+    JL = orc.wall_dist(m["omega"], m["vx"], m["vy"], 300.5, 300.0, 1.0, 0.0, vuc)[2]
+    JR = orc.wall_dist(m["omega"], m["vx"], m["vy"], 299.5, 300.0, -1.0, 0.0, vuc)[2]
+    np.testing.assert_allclose(out["jx"][-1, :10], (abs(JL) + abs(JR)) * 1e30, rtol=1e-9)
+    assert np.abs(out["jy"]).max() == 0.0
+    # each particle: one flight segment per box, absorbed once
+    assert out["crossings"] == 20000 * 10
+
+
+def test_rta_flux_conserved_across_bins_at_steady_state():
+    m = orc.synthetic_modes(600, 3)
+    C = orc.bose_einstein_kernel(m["omega"], 300.0)
+    m["V"] = 1.3806488e-23 * C.sum() / 1.6e-21
+    m["lattz"] = 1.0
+    rect, kind, teq = bar(nbins=10, H=50.0)
+    out = orc.rta_run(m, rect, kind, teq, thickness=1.0, T0=300.0, dt=100.0, maxtime=6000.0, nparticles=30000,
+                      seed=9, sigma=1.0)
+    jx = out["jx"][-1, :10]
+    assert jx.std() / jx.mean() < 0.03
+    T = out["T"][-1, :10]
+    assert T[0] > 300.0 > T[-1]          # hot left, cold right
+    assert abs((T[0] - 300) + (T[-1] - 300)) < 0.1 * (T[0] - T[-1])   # antisymmetric about T0
+
+
+def _bert_inputs(n=40, seed=1, vzero=False):
+    m = orc.synthetic_modes(n, seed)
+    if vzero:
+        m["vx"] = np.zeros(n)
+        m["vy"] = np.zeros(n)
+    C = orc.bose_einstein_kernel(m["omega"], 300.0)
+    m["V"] = 1.3806488e-23 * C.sum() / 1.6e-21
+    m["lattz"] = 1.0
+    B = orc.synthetic_B(C, m["w"], 0.3, seed)
+    P = orc.expm(5.0 * B)
+    return m, B, P
+
+
+def test_bert_conserves_particles_plus_wall_counts():
+    m, B, P = _bert_inputs(60)
+    rect = np.array([[k * 10.0, (k + 1) * 10.0, 0.0, 30.0] for k in range(5)])
+    kind = np.zeros(5, np.int32)
+    teq = np.full(5, 300.0)
+    tin = teq.copy()
+    tin[0] = 301.0
+    o = orc.bert_run(m, rect, kind, teq, tin, thickness=1.0, dt=5.0, maxtime=95.0, P=P, Efactor=20000,
+                     Tmax=301.0, Tmin=300.0, seed=2)
+    tot = o["npart"] + o["pending"]
+    assert (tot == tot[0]).all()
+    assert 19000 < o["npart"][0] < 21000
+
+
+def test_bert_mode_distribution_follows_P():
+    """With v = 0 nothing moves, so E[H_k] = P^k H_0 for the (single) box."""
+    m, B, P = _bert_inputs(30, seed=4, vzero=True)
+    rect = np.array([[0.0, 10.0, 0.0, 10.0], [10.0, 20.0, 0.0, 10.0]])
+    kind = np.zeros(2, np.int32)
+    teq = np.full(2, 300.0)
+    tin = np.array([310.0, 300.0])
+    hs = []
+    for seed in range(4):
+        o = orc.bert_run(m, rect, kind, teq, tin, thickness=1.0, dt=5.0, maxtime=10.0, P=P, Efactor=200000,
+                         Tmax=310.0, Tmin=300.0, seed=seed, max_steps=3)
+        hs.append(o["H_last"][0].astype(float))
+    h = np.mean(hs, 0)
+    # initial distribution: outTref pmf ~ omega |n(310)-n(300)|
+    n310 = orc.bose_einstein(m["omega"], 310.0)
+    n300 = orc.bose_einstein(m["omega"], 300.0)
+    h0 = m["omega"] * np.abs(n310 - n300)
+    expect = np.linalg.matrix_power(P, 3) @ (h0 / h0.sum()) * h.sum()
+    sig = np.sqrt(expect / 4 + 1)
+    z = (h - expect) / sig
+    assert np.abs(z).max() < 5, z
+
+
+def test_bert_cancellation_keeps_net_counts():
+    """A propagator with negative entries creates particles of both signs;
+    after the first (cancelling) step the survivors per (box, mode) are
+    exactly |h| (beyondRTA_MC2d.cpp:2342-2359)."""
+    m, B, P = _bert_inputs(20, seed=6, vzero=True)
+    Pn = P.copy()
+    Pn[0, :] += Pn[1, :] + 0.1   # columns still sum to one ...
+    Pn[1, :] = -0.1              # ... with a negative row 1
+    rect = np.array([[0.0, 10.0, 0.0, 10.0], [10.0, 20.0, 0.0, 10.0]])
+    kind = np.zeros(2, np.int32)
+    teq = np.full(2, 300.0)
+    tin = np.array([305.0, 300.0])
+    o = orc.bert_run(m, rect, kind, teq, tin, thickness=1.0, dt=5.0, maxtime=10.0, P=Pn, Efactor=50000,
+                     Tmax=305.0, Tmin=300.0, seed=1, max_steps=1)
+    H = o["H_last"]
+    assert (H < 0).any()
+    assert o["npart"][1] == np.abs(H).sum()
+    # the signed total is unbiased: E = sum over columns of (P_n h0) = N0
+    assert abs(H.sum() / o["npart"][0] - 1) < 0.02
