@@ -1,0 +1,136 @@
+"""Size-independent properties of the HIP path, checked at the benchmark
+sizes and on closed-form cases."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+import oracle_lib as orc
+from test_oracle_mc import _bert_inputs, bar, horizontal_material
+
+pytestmark = pytest.mark.gpu
+
+
+def test_rta_ballistic_flux_is_exact(pkg, ctx):
+    m = horizontal_material()
+    rect, kind, teq = bar()
+    modes = pkg.ModeTable(ctx, m["omega"], m["vx"], m["vy"], m["w"], m["V"], m["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    out = pkg.rtamc2d(ctx, modes, geom, thickness=1.0, T0=300.0, dt=50.0, maxtime=1000.0, nparticles=100000,
+                      seed=5, ballistic=True, sigma_bounce=5.0)
+    vuc = m["V"] / m["lattz"]
+    JL = orc.wall_dist(m["omega"], m["vx"], m["vy"], 300.5, 300.0, 1.0, 0.0, vuc)[2]
+    JR = orc.wall_dist(m["omega"], m["vx"], m["vy"], 299.5, 300.0, -1.0, 0.0, vuc)[2]
+    np.testing.assert_allclose(out["jx"][-1, :10], (abs(JL) + abs(JR)) * 1e30, rtol=1e-9)
+    assert out["crossings"] == 100000 * 10
+    assert np.abs(out["jy"]).max() == 0.0
+
+
+@pytest.fixture(scope="module")
+def config1(pkg, ctx):
+    W = pkg.workloads
+    mat, B = W.config1_inputs()
+    P, rep = pkg.build_P(ctx, B, 5.0, m=30, precision="fp64", arnoldi="textbook")
+    return mat, B, P, rep
+
+
+def test_config1_propagator_matches_scipy(config1):
+    mat, B, P, rep = config1
+    E = scipy.linalg.expm(5.0 * B)
+    assert np.abs(P - E).sum(0).max() < 1e-10
+    assert rep["max_col_sum_err"] < 1e-12 and rep["breakdowns"] == 0
+
+
+def test_config1_conserves_particles_at_full_size(pkg, ctx, config1):
+    mat, B, P, _ = config1
+    W = pkg.workloads
+    c = W.config1()
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, c["rect"], c["kind"], c["Teq"], c["Tinit"])
+    prop = pkg.Propagator(ctx, P)
+    o = pkg.bertamc2d(ctx, modes, geom, prop, thickness=1.0, dt=5.0, maxtime=c["maxtime"], max_steps=200,
+                      Efactor=c["Efactor"], Tmax=301.0, Tmin=300.0, seed=77)
+    assert o["nsteps"] == 200
+    assert abs(o["npart"][0] / 1e6 - 1) < 0.005
+    tot = o["npart"] + o["pending"]
+    assert (tot == tot[0]).all()
+    # energy density sums to the particle count (all particles positive)
+    vol = (c["rect"][:, 1] - c["rect"][:, 0]) * (c["rect"][:, 3] - c["rect"][:, 2])
+    count = (o["props"][:, :, 0] * vol[None, :] / (o["Eeff"] * 1e27)).sum(1)
+    np.testing.assert_allclose(count, o["npart"], rtol=1e-9)
+    # the pulse spreads: energy leaves bin 0 and reaches the far end
+    ed = o["props"][:, :, 0]
+    assert ed[-1, 0] < 0.2 * ed[0, 0] and ed[-1, -1] > 0
+
+
+def test_bert_mode_distribution_follows_P(pkg, ctx):
+    m, B, P = _bert_inputs(30, seed=4, vzero=True)
+    rect = np.array([[0.0, 10.0, 0.0, 10.0], [10.0, 20.0, 0.0, 10.0]])
+    kind = np.zeros(2, np.int32)
+    teq = np.full(2, 300.0)
+    tin = np.array([310.0, 300.0])
+    modes = pkg.ModeTable(ctx, m["omega"], m["vx"], m["vy"], m["w"], m["V"], m["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq, tin)
+    prop = pkg.Propagator(ctx, P)
+    hs = []
+    for seed in range(4):
+        frames = []
+        b = pkg.BertRun(ctx, modes, geom, prop, thickness=1.0, dt=5.0, maxtime=10.0, max_steps=3, Efactor=200000,
+                        Tmax=310.0, Tmin=300.0, seed=seed)
+        b.execute(frame_cb=lambda s, t, e, H: frames.append(H))
+        b.close()
+        hs.append(frames[-1][0].astype(float))
+    h = np.mean(hs, 0)
+    n310 = orc.bose_einstein(m["omega"], 310.0)
+    n300 = orc.bose_einstein(m["omega"], 300.0)
+    h0 = m["omega"] * np.abs(n310 - n300)
+    expect = np.linalg.matrix_power(P, 3) @ (h0 / h0.sum()) * h.sum()
+    z = (h - expect) / np.sqrt(expect / 4 + 1)
+    assert np.abs(z).max() < 5, z
+
+
+def test_bert_cancellation_keeps_net_counts(pkg, ctx):
+    m, B, P = _bert_inputs(20, seed=6, vzero=True)
+    Pn = P.copy()
+    Pn[0, :] += Pn[1, :] + 0.1
+    Pn[1, :] = -0.1
+    rect = np.array([[0.0, 10.0, 0.0, 10.0], [10.0, 20.0, 0.0, 10.0]])
+    kind = np.zeros(2, np.int32)
+    teq = np.full(2, 300.0)
+    tin = np.array([305.0, 300.0])
+    modes = pkg.ModeTable(ctx, m["omega"], m["vx"], m["vy"], m["w"], m["V"], m["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq, tin)
+    prop = pkg.Propagator(ctx, Pn)
+    frames = []
+    b = pkg.BertRun(ctx, modes, geom, prop, thickness=1.0, dt=5.0, maxtime=20.0, max_steps=3, Efactor=50000,
+                    Tmax=305.0, Tmin=300.0, seed=1)
+    o = b.execute(frame_cb=lambda s, t, e, H: frames.append(H))
+    b.close()
+    H0 = frames[0]
+    assert (H0 < 0).any()
+    # step 0 cancels: exactly |h| particles survive per (box, mode)
+    assert o["npart"][1] == np.abs(H0).sum()
+    assert abs(H0.sum() / o["npart"][0] - 1) < 0.02
+
+
+def test_bert_capacity_overflow_is_an_error(pkg, ctx):
+    m, B, P = _bert_inputs(20, seed=6, vzero=True)
+    Pn = P.copy()
+    Pn[0, :] += Pn[1, :] + 0.4
+    Pn[1, :] = -0.4    # strong branching: the population grows every step
+    rect = np.array([[0.0, 10.0, 0.0, 10.0], [10.0, 20.0, 0.0, 10.0]])
+    modes = pkg.ModeTable(ctx, m["omega"], m["vx"], m["vy"], m["w"], m["V"], m["lattz"])
+    geom = pkg.Geometry(ctx, rect, np.zeros(2, np.int32), np.full(2, 300.0), np.array([305.0, 300.0]))
+    prop = pkg.Propagator(ctx, Pn)
+    with pytest.raises(pkg.BteError, match="ECAPACITY"):
+        pkg.bertamc2d(ctx, modes, geom, prop, thickness=1.0, dt=5.0, maxtime=100.0, max_steps=20, Efactor=50000,
+                      Tmax=305.0, Tmin=300.0, seed=1, pcancel_each=1000, capacity=1 << 21)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 5e-5), ("bf16", 5e-3)])
+def test_krylov_precisions_on_golden_operator(pkg, ctx, prec, tol):
+    import os
+    fix = np.load(os.path.join(os.path.dirname(__file__), "golden", "expm_fixtures.npz"))
+    A, E = fix["ecop_n128_A"], fix["ecop_n128_E"]
+    P, rep = pkg.build_P(ctx, A, 1.0, m=40, precision=prec, arnoldi="textbook")
+    assert np.abs(P - E).sum(0).max() < tol
+    np.testing.assert_allclose(P.sum(0), 1.0, atol=1e-12)
