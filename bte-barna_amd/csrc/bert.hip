@@ -37,6 +37,7 @@ static_assert(sizeof(BPart) == 32, "particle record is 32 bytes");
 
 struct BertParams {
     const double2* vel;      // (vx, vy) per mode
+    const double2* ivel;     // (1/vx, 1/vy) per mode (0 where the component is 0)
     const double* omega;
     const double* w;
     const DevBox* boxes;
@@ -105,8 +106,8 @@ __device__ inline void geo_stage(const BertParams& P, GeoLds& g, const DevBox*& 
 // through box crossings; a void side adds the particle sign to that side's
 // Eborder and ends the particle. Returns false if the particle ended.
 __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const DevContact* cts, double& x,
-                              double& y, int& box, int sign, double vx, double vy, double dt, Pcg32& rng,
-                              int32_t* Eb) {
+                              double& y, int& box, int sign, double vx, double vy, double ivx, double ivy,
+                              double dt, Pcg32& rng, int32_t* Eb) {
     for (int it = 0; it < 4096; it++) {
         const DevBox B = boxes[box];
         if (B.kind == 1) return false;  // reservoir: deleted
@@ -118,10 +119,24 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
         }
         int mask;
         double ex, ey;
-        double tf = box_exit(B, x, y, vx, vy, mask, ex, ey);
+        double tf = box_exit(B, x, y, vx, vy, ivx, ivy, mask, ex, ey);
         double left = dt - tf;
         if (almost_equal(left, 0.)) left = 0.;
-        int pb[4];
+        int side = 0;
+        const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
+        if (fast == SIDE_VOID) {
+            atomicAdd(Eb + box * 4 + side, sign);
+            return false;
+        }
+        if (fast >= 0) {
+            x = ex;
+            y = ey;
+            box = fast;
+            dt = left;
+            if (dt == 0.) return true;
+            continue;
+        }
+        Ids4 pb{};
         int npb = find_pboxes(boxes, cts, B, ex, ey, vx, vy, pb);
         if (npb == 0) {
             int s = __popc(mask) == 1 ? __ffs(mask) - 1
@@ -132,13 +147,14 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
         x = ex;
         y = ey;
         if (npb == 1) {
-            box = pb[0];
+            box = pb.a;
         } else {
-            int cand[4], nc = 0;
+            Ids4 cand{};
+            int nc = 0;
             double tau = 1.0e-6;
             while (nc == 0 && tau >= 1.0e-15) {
                 for (int k = 0; k < npb && k < 4; k++)
-                    if (box_inside(boxes[pb[k]], ex + tau * vx, ey + tau * vy)) cand[nc++] = pb[k];
+                    if (box_inside(boxes[pb.get(k)], ex + tau * vx, ey + tau * vy)) cand.set(nc++, pb.get(k));
                 tau /= 10.0;
             }
             if (nc == 0) {
@@ -147,7 +163,7 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
                 x = ex + 1.0e-6 * dx / dn;
                 y = ey + 1.0e-6 * dy / dn;
             } else {
-                box = cand[rng.bounded((uint32_t)nc)];
+                box = cand.get((int)rng.bounded((uint32_t)nc));
             }
         }
         dt = left;
@@ -163,7 +179,7 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
 // them when there is more than one. Children of a sign flip re-queue the
 // parent twice (1189-1192).
 template <bool EMIT>
-__device__ inline int scatter(const BertParams& P, Pcg32 rng, int mode, int sign, int box,
+__device__ inline int scatter(const BertParams& P, int32_t* H, Pcg32 rng, int mode, int sign, int box,
                               BPart* out, uint64_t slot, uint64_t limit, double x, double y, uint64_t pid,
                               uint32_t step, int& nm0, int& ns0) {
     if (P.rta) {
@@ -175,7 +191,7 @@ __device__ inline int scatter(const BertParams& P, Pcg32 rng, int mode, int sign
             nm = (int)alias_pick(P.rta_alias + (size_t)sl * P.n, (uint32_t)P.n, rng, ng);
         }
         if (EMIT) {
-            atomicAdd(P.H + (size_t)box * P.n + nm, sign);
+            atomicAdd(H + (size_t)box * P.n + nm, sign);
             if (slot < limit) {
                 BPart q{x, y, (uint32_t)nm, ((uint32_t)box << 1) | (sign < 0 ? 1u : 0u), pid};
                 out[slot] = q;
@@ -203,7 +219,7 @@ __device__ inline int scatter(const BertParams& P, Pcg32 rng, int mode, int sign
             ns0 = ns;
         }
         if (EMIT) {
-            atomicAdd(P.H + (size_t)box * P.n + nm, ns);
+            atomicAdd(H + (size_t)box * P.n + nm, ns);
             uint64_t s = slot + nchild;
             if (s < limit) {
                 uint64_t cpid = nchild == 0 ? pid : mix64(pid ^ ((uint64_t)step << 40) ^ (uint64_t)nchild);
@@ -294,21 +310,21 @@ __device__ inline void seg_close(const BertParams& P, uint32_t* segout, int myse
 
 // count the children, reserve their slots in the workgroup's segment and
 // write them; a single child (the common case) is written from pass 0's result
-__device__ inline void scatter_out(const BertParams& P, Pcg32& rng, bool alive, int mode, int sign, int box,
-                                   double x, double y, uint64_t pid, uint32_t step, uint32_t* lds_count,
+__device__ inline void scatter_out(const BertParams& P, int32_t* H, Pcg32& rng, bool alive, int mode, int sign,
+                                   int box, double x, double y, uint64_t pid, uint32_t step, uint32_t* lds_count,
                                    int myseg) {
     int nm0 = 0, ns0 = 1;
-    int c = alive ? scatter<false>(P, rng, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
+    int c = alive ? scatter<false>(P, H, rng, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
     uint64_t limit;
     uint64_t slot = wave_reserve(lds_count, myseg, P.segcap, c, P.flags, limit);
     if (c == 1) {
-        atomicAdd(P.H + (size_t)box * P.n + nm0, ns0);
+        atomicAdd(H + (size_t)box * P.n + nm0, ns0);
         if (slot < limit) {
             BPart q{x, y, (uint32_t)nm0, ((uint32_t)box << 1) | (ns0 < 0 ? 1u : 0u), pid};
             P.pout[slot] = q;
         }
     } else if (c > 1) {
-        scatter<true>(P, rng, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
+        scatter<true>(P, H, rng, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
     }
 }
 
@@ -353,8 +369,6 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
     const int myseg = blockIdx.x;
     int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
     const int32_t* Hprev = P.H + (size_t)(cur ^ 1) * P.nbox * P.n;
-    BertParams Q = P;
-    Q.H = H;
     const uint64_t skey = stream_key(P.seed, step, 0);
     unsigned long long done = 0;
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
@@ -388,10 +402,11 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
             if (alive) {
                 seed_stream(rng, skey, p.pid);
                 const double2 v = P.vel[mode];
-                alive = advect(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, P.dt, rng, Eb);
+                const double2 iv = P.ivel[mode];
+                alive = advect(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
             }
         }
-        scatter_out(Q, rng, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
+        scatter_out(P, H, rng, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -492,8 +507,6 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
     const uint32_t lo = (uint32_t)((uint64_t)total * blockIdx.x / gridDim.x);
     const uint32_t hi = (uint32_t)((uint64_t)total * (blockIdx.x + 1) / gridDim.x);
     int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
-    BertParams Q = P;
-    Q.H = H;
     const uint64_t skey = stream_key(P.seed, step, 1);
     unsigned long long done = 0;
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
@@ -531,10 +544,11 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
             x = p1x + t * ndx;
             y = p1y + t * ndy;
             const double2 v = P.vel[mode];
+            const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
-            alive = advect(P, boxes, cts, x, y, bx, sign, v.x, v.y, rdt, rng, Ebn);
+            alive = advect(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
         }
-        scatter_out(Q, rng, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out(P, H, rng, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -567,8 +581,6 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     const double fl = floor(S.expected);
     const int ng = (int)fl + (cr.uniform() <= S.expected - fl ? 1 : 0);
     int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
-    BertParams Q = P;
-    Q.H = H;
     const int nloop = (ng + 63) & ~63;
     unsigned long long done = 0;
     for (int k = threadIdx.x; k - (int)(threadIdx.x & 63) < nloop; k += blockDim.x) {
@@ -593,10 +605,11 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
             bool ngf;
             mode = (int)alias_pick(stab + S.tab, (uint32_t)P.n, rng, ngf);
             const double2 v = P.vel[mode];
+            const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
-            alive = advect(P, P.boxes, P.cts, x, y, bx, S.sign, v.x, v.y, rdt, rng, Eb);
+            alive = advect(P, P.boxes, P.cts, x, y, bx, S.sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
         }
-        scatter_out(Q, rng, alive, mode, S.sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out(P, H, rng, alive, mode, S.sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -728,7 +741,7 @@ struct Bert {
     int max_steps = 0;
     uint64_t n0 = 0, cap = 0;
     int nseg = 99;
-    DevBuf<double2> d_vel;
+    DevBuf<double2> d_vel, d_ivel;
     DevBuf<BPart> d_parts;
     DevBuf<uint32_t> d_seg, d_tickets, d_flags;
     DevBuf<int32_t> d_H, d_Eb, d_side_spl, d_rta_slot;
@@ -766,6 +779,7 @@ static double calc_Eeff_host(const Modes& m, double Th, double Tc, double Tr, do
 static BertParams make_params(Bert* b) {
     BertParams P{};
     P.vel = b->d_vel.p;
+    P.ivel = b->d_ivel.p;
     P.omega = b->m->d_omega.p;
     P.w = b->m->d_w.p;
     P.boxes = b->g->d_boxes.p;
@@ -970,6 +984,10 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     std::vector<double2> vel(n);
     for (int k = 0; k < n; k++) vel[k] = make_double2(m->vx[k], m->vy[k]);
     b->d_vel.upload(vel, s);
+    std::vector<double2> ivel(n);
+    for (int k = 0; k < n; k++)
+        ivel[k] = make_double2(m->vx[k] != 0. ? 1.0 / m->vx[k] : 0., m->vy[k] != 0. ? 1.0 / m->vy[k] : 0.);
+    b->d_ivel.upload(ivel, s);
     b->d_parts.alloc(2 * b->cap);
     b->d_seg.alloc(2 * (size_t)b->nsegs);
     b->d_flags.alloc(4);

@@ -81,7 +81,12 @@ struct DevBox {
     int32_t kind;      // 0 regular, 1 reservoir
     int32_t c0, c1;    // contacts [c0, c1)
     int32_t pad;
+    // per side: the one box sharing the whole side (>= 0), SIDE_VOID when no
+    // box shares any part of it, SIDE_MIXED otherwise
+    int32_t side_nbr[4];
 };
+#define SIDE_VOID (-1)
+#define SIDE_MIXED (-2)
 struct DevContact {
     int32_t box;
     int32_t pad;
@@ -98,15 +103,17 @@ __device__ __host__ inline void side_normal(int s, double& nx, double& ny) {
 
 // Exit of the ray r + t v from box b: time of the first side crossed with
 // n.v > 0 (geometry_2d.cpp:160-293 for rectangles). side_mask gets bit s for
-// every side reached at that time (corner ties set two bits).
+// every side reached at that time (corner ties set two bits). ivx, ivy are
+// the reciprocal velocity components (unused where the component is 0).
 __device__ __host__ inline double box_exit(const DevBox& b, double x, double y, double vx,
-                                           double vy, int& side_mask, double& ex, double& ey) {
+                                           double vy, double ivx, double ivy, int& side_mask,
+                                           double& ex, double& ey) {
     double tx = 1e300, ty = 1e300;
     int sx = -1, sy = -1;
-    if (vx > 0.) { tx = (b.xmax - x) / vx; sx = 3; }
-    else if (vx < 0.) { tx = (b.xmin - x) / vx; sx = 0; }
-    if (vy > 0.) { ty = (b.ymax - y) / vy; sy = 2; }
-    else if (vy < 0.) { ty = (b.ymin - y) / vy; sy = 1; }
+    if (vx > 0.) { tx = (b.xmax - x) * ivx; sx = 3; }
+    else if (vx < 0.) { tx = (b.xmin - x) * ivx; sx = 0; }
+    if (vy > 0.) { ty = (b.ymax - y) * ivy; sy = 2; }
+    else if (vy < 0.) { ty = (b.ymin - y) * ivy; sy = 1; }
     if (tx < 0.) tx = 0.;
     if (ty < 0.) ty = 0.;
     double t;
@@ -130,12 +137,41 @@ __device__ __host__ inline double box_exit(const DevBox& b, double x, double y, 
     return t;
 }
 
+// Fast path of the neighbour search for an exit through a single side at a
+// point strictly inside that side: the side's precomputed neighbour decides
+// (the general search below would find exactly that box, or none for a void
+// side). Returns the neighbour, SIDE_VOID, or SIDE_MIXED when the general
+// search must run.
+__device__ __host__ inline int side_fast(const DevBox& B, int mask, double ex, double ey, double vx,
+                                         double vy, int& side) {
+    if (__builtin_popcount(mask) != 1) return SIDE_MIXED;
+    side = __builtin_ctz(mask);
+    const bool xside = (side == 0 || side == 3);
+    const bool interior = xside ? (ey > B.ymin && ey < B.ymax) : (ex > B.xmin && ex < B.xmax);
+    const double vn = xside ? vx : vy;
+    if (!interior || almost_equal(vn, 0.)) return SIDE_MIXED;
+    // select chain instead of an indexed load: keeps the box in registers
+    return side == 0 ? B.side_nbr[0]
+                     : (side == 1 ? B.side_nbr[1] : (side == 2 ? B.side_nbr[2] : B.side_nbr[3]));
+}
+
 // Neighbour search at an exit point (RTA_MC2D.cpp:1130-1163): contacts c
 // whose closed box holds rnew or rnew + 1e-6 v/|v| with n_c.v > 0 and not
-// parallel. Returns the count (up to 4 ids stored).
+// parallel. Returns the count; the first four ids are packed in ids (kept in
+// registers, no indexed local array).
+struct Ids4 {
+    int a, b, c, d;
+    __device__ __host__ inline int get(int k) const { return k == 0 ? a : (k == 1 ? b : (k == 2 ? c : d)); }
+    __device__ __host__ inline void set(int k, int v) {
+        if (k == 0) a = v;
+        else if (k == 1) b = v;
+        else if (k == 2) c = v;
+        else if (k == 3) d = v;
+    }
+};
 __device__ __host__ inline int find_pboxes(const DevBox* boxes, const DevContact* cts,
                                            const DevBox& cur, double rx, double ry, double vx,
-                                           double vy, int* out) {
+                                           double vy, Ids4& out) {
     double vn = sqrt(vx * vx + vy * vy);
     double ex = rx + 1.0e-6 * vx / vn, ey = ry + 1.0e-6 * vy / vn;
     int n = 0;
@@ -145,7 +181,7 @@ __device__ __host__ inline int find_pboxes(const DevBox* boxes, const DevContact
         if (!(d > 0.) || almost_equal(d, 0.)) continue;
         const DevBox nb = boxes[ct.box];
         if (box_inside(nb, rx, ry) || box_inside(nb, ex, ey)) {
-            if (n < 4) out[n] = ct.box;
+            out.set(n, ct.box);
             n++;
         }
     }
@@ -157,7 +193,7 @@ __device__ __host__ inline int find_pboxes(const DevBox* boxes, const DevContact
 __device__ __host__ inline int pick_void_side(const DevBox* boxes, const DevContact* cts,
                                               const DevBox& cur, int mask, double rx, double ry,
                                               uint32_t rnd) {
-    int cand[2], nc = 0, first = -1;
+    int c0 = -1, c1 = -1, first = -1;
     for (int s = 0; s < 4; s++) {
         if (!((mask >> s) & 1)) continue;
         if (first < 0) first = s;
@@ -167,10 +203,13 @@ __device__ __host__ inline int pick_void_side(const DevBox* boxes, const DevCont
         bool isvoid = true;
         for (int c = cur.c0; c < cur.c1; c++)
             if (box_inside(boxes[cts[c].box], cx, cy)) { isvoid = false; break; }
-        if (isvoid && nc < 2) cand[nc++] = s;
+        if (isvoid) {
+            if (c0 < 0) c0 = s;
+            else if (c1 < 0) c1 = s;
+        }
     }
-    if (nc == 0) return first;
-    return cand[nc == 1 ? 0 : (rnd & 1u)];
+    if (c0 < 0) return first;
+    return (c1 < 0 || (rnd & 1u) == 0) ? c0 : c1;
 }
 
 }  // namespace bte

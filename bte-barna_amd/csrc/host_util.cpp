@@ -357,6 +357,24 @@ int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int3
         for (auto& hc : g->contacts[i]) g->dcontacts.push_back(DevContact{hc.box, 0, hc.npx, hc.npy});
         d.c1 = (int32_t)g->dcontacts.size();
         d.pad = 0;
+        // side classification for the neighbour fast path
+        for (int sd = 0; sd < 4; sd++) {
+            const bool xside = (sd == 0 || sd == 3);
+            const double sv = sd == 0 ? b.xmin : sd == 3 ? b.xmax : sd == 1 ? b.ymin : b.ymax;
+            const double lo = xside ? b.ymin : b.xmin, hi = xside ? b.ymax : b.xmax;
+            int full = -1, touching = 0;
+            for (auto& hc : g->contacts[i]) {
+                // a positive-length shared segment lying on this side
+                bool on = xside ? (hc.p1x == sv && hc.p2x == sv && hc.p1y != hc.p2y)
+                                : (hc.p1y == sv && hc.p2y == sv && hc.p1x != hc.p2x);
+                if (!on) continue;
+                touching++;
+                double a = xside ? std::min(hc.p1y, hc.p2y) : std::min(hc.p1x, hc.p2x);
+                double c = xside ? std::max(hc.p1y, hc.p2y) : std::max(hc.p1x, hc.p2x);
+                if (a == lo && c == hi) full = hc.box;
+            }
+            d.side_nbr[sd] = touching == 0 ? SIDE_VOID : (touching == 1 && full >= 0 ? full : SIDE_MIXED);
+        }
     }
     HIPCHECK(hipSetDevice(c->device));
     g->d_boxes.upload(g->dboxes, c->stream);

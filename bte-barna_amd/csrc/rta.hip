@@ -23,6 +23,7 @@ struct RtaGen {
 
 struct RtaParams {
     const double4* modes;  // (vx, vy, w, omega)
+    const double2* ivel;   // (1/vx, 1/vy), 0 where the component is 0
     const uint2* scat;     // BE_derivative alias (n)
     const DevBox* boxes;
     const DevContact* cts;
@@ -177,17 +178,27 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
         }
         int mask;
         double ex, ey;
-        double tf = box_exit(B, x, y, vx, vy, mask, ex, ey);
+        const double2 iv = P.ivel[mode];
+        double tf = box_exit(B, x, y, vx, vy, iv.x, iv.y, mask, ex, ey);
         double left = dtrem - tf;
         if (almost_equal(left, 0.)) left = 0.;
         tally_segment(P, box, t, t + tf, sign, vx, vy);
         t += tf;
-        int pb[4];
-        int npb = find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
+        int side = 0;
+        const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
+        Ids4 pb{};
+        int npb;
+        if (fast >= 0) {
+            pb.a = fast;
+            npb = 1;
+        } else {
+            npb = fast == SIDE_VOID ? 0 : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
+        }
         if (npb == 0) {
             // void side: Lambert re-emission, remaining flight discarded (Q8)
-            int s = __popc(mask) == 1 ? __ffs(mask) - 1
-                                      : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next());
+            int s = fast == SIDE_VOID ? side
+                  : (__popc(mask) == 1 ? __ffs(mask) - 1
+                                       : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next()));
             const int4 li = P.lam_info[mode];
             bool ng;
             int k = (int)alias_pick(P.lam_tab + li.z + (size_t)s * li.y, (uint32_t)li.y, rng, ng);
@@ -201,17 +212,18 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
         x = ex;
         y = ey;
         if (npb == 1) {
-            box = pb[0];
+            box = pb.a;
         } else {
             // several candidate boxes at a corner (RTA_MC2D.cpp:1316-1380)
-            int cand[4], nc = 0;
+            Ids4 cand{};
+            int nc = 0;
             double cx = ex + 1.0e-6 * vx, cy = ey + 1.0e-6 * vy;
             for (int k = 0; k < npb && k < 4; k++)
-                if (box_inside(P.boxes[pb[k]], cx, cy)) cand[nc++] = pb[k];
+                if (box_inside(P.boxes[pb.get(k)], cx, cy)) cand.set(nc++, pb.get(k));
             if (nc > 0) {
-                box = cand[rng.bounded((uint32_t)nc)];
+                box = cand.get((int)rng.bounded((uint32_t)nc));
             } else {
-                box = pb[rng.bounded((uint32_t)(npb < 4 ? npb : 4))];
+                box = pb.get((int)rng.bounded((uint32_t)(npb < 4 ? npb : 4)));
                 const DevBox nb = P.boxes[box];
                 double dx = 0.5 * (nb.xmin + nb.xmax) - x, dy = 0.5 * (nb.ymin + nb.ymax) - y;
                 double dn = sqrt(dx * dx + dy * dy);
@@ -237,6 +249,7 @@ struct Rta {
     double maxtime = 0, Eeff = 0, cv = 0, einfo1 = 0;
     std::vector<double> timegrid;
     DevBuf<double4> d_modes;
+    DevBuf<double2> d_ivel;
     DevBuf<uint2> d_scat, d_gen_sel, d_gen_tab, d_lam_tab;
     DevBuf<RtaGen> d_gens;
     DevBuf<int4> d_lam_info;
@@ -355,6 +368,10 @@ int bte_rta_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const 
     std::vector<double4> md(n);
     for (int i = 0; i < n; i++) md[i] = make_double4(m->vx[i], m->vy[i], m->w[i], m->omega[i]);
     r->d_modes.upload(md, s);
+    std::vector<double2> iv(n);
+    for (int i = 0; i < n; i++)
+        iv[i] = make_double2(m->vx[i] != 0. ? 1.0 / m->vx[i] : 0., m->vy[i] != 0. ? 1.0 / m->vy[i] : 0.);
+    r->d_ivel.upload(iv, s);
     // BE_derivative sampler (sampling.cpp:47-68)
     std::vector<double> pmf(n);
     for (int i = 0; i < n; i++) pmf[i] = m->omega[i] <= 0. ? 0. : h_be_kernel(m->omega[i], cfg->T0) * m->w[i];
@@ -424,6 +441,7 @@ int bte_rta_execute(bte_rta* rr, double* T_out, double* jx_out, double* jy_out, 
     HIPCHECK(hipMemsetAsync(r->d_counters.p, 0, 4 * sizeof(unsigned long long), s));
     RtaParams P{};
     P.modes = r->d_modes.p;
+    P.ivel = r->d_ivel.p;
     P.scat = r->d_scat.p;
     P.boxes = r->g->d_boxes.p;
     P.cts = r->g->d_contacts.p;
