@@ -52,6 +52,7 @@ struct BertParams {
     BPart* pout;
     uint32_t* seg;           // [2][nsegs] live count per buffer segment
     int32_t* H;              // [2][nbox][n]
+    double* pbox;            // [nsegs][nbox][3] per-workgroup (sum h, sum h vx, sum h vy)
     uint32_t* tickets;       // [nbox][n]
     int32_t* Eb;             // [2][nbox*4]
     double* props;           // [steps+1][nbox][3]
@@ -71,6 +72,7 @@ struct BertParams {
     double dt, Eeff, thickness;
     int32_t n, ngamma, ncol, nbox, nseg;
     int32_t rta, max_children;
+    int32_t lds_props;       // properties from per-workgroup box sums (nbox <= BERT_LDS_BOXES)
 };
 
 // The stream of particle pid in a given step and kernel: PCG32 with
@@ -173,14 +175,27 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
     return false;
 }
 
+// A child written in this step enters the (box, mode) histogram when that
+// step feeds a cancellation or a frame (H not null) and the box sums behind
+// calculate_prop (sbox in LDS, not null when the boxes fit there).
+__device__ inline void tally_child(const BertParams& P, int32_t* H, double* sbox, int box, int nm, int ns) {
+    if (H) atomicAdd(H + (size_t)box * P.n + nm, ns);
+    if (sbox) {
+        const double2 v = P.vel[nm];
+        atomicAdd(sbox + box * 3 + 0, (double)ns);
+        atomicAdd(sbox + box * 3 + 1, ns * v.x);
+        atomicAdd(sbox + box * 3 + 2, ns * v.y);
+    }
+}
+
 // Scattering with the propagator (beyondRTA_MC2d.cpp:1133-1192) or
 // ScatteringRTA (1040-1055). Pass 0 (EMIT false) counts the children and
 // returns the first one in nm0/ns0; pass 1 replays the same stream to emit
 // them when there is more than one. Children of a sign flip re-queue the
 // parent twice (1189-1192).
 template <bool EMIT>
-__device__ inline int scatter(const BertParams& P, int32_t* H, Pcg32 rng, int mode, int sign, int box,
-                              BPart* out, uint64_t slot, uint64_t limit, double x, double y, uint64_t pid,
+__device__ inline int scatter(const BertParams& P, int32_t* H, double* sbox, Pcg32 rng, int mode, int sign,
+                              int box, BPart* out, uint64_t slot, uint64_t limit, double x, double y, uint64_t pid,
                               uint32_t step, int& nm0, int& ns0) {
     if (P.rta) {
         double R = rng.uniform();
@@ -191,7 +206,7 @@ __device__ inline int scatter(const BertParams& P, int32_t* H, Pcg32 rng, int mo
             nm = (int)alias_pick(P.rta_alias + (size_t)sl * P.n, (uint32_t)P.n, rng, ng);
         }
         if (EMIT) {
-            atomicAdd(H + (size_t)box * P.n + nm, sign);
+            tally_child(P, H, sbox, box, nm, sign);
             if (slot < limit) {
                 BPart q{x, y, (uint32_t)nm, ((uint32_t)box << 1) | (sign < 0 ? 1u : 0u), pid};
                 out[slot] = q;
@@ -219,7 +234,7 @@ __device__ inline int scatter(const BertParams& P, int32_t* H, Pcg32 rng, int mo
             ns0 = ns;
         }
         if (EMIT) {
-            atomicAdd(H + (size_t)box * P.n + nm, ns);
+            tally_child(P, H, sbox, box, nm, ns);
             uint64_t s = slot + nchild;
             if (s < limit) {
                 uint64_t cpid = nchild == 0 ? pid : mix64(pid ^ ((uint64_t)step << 40) ^ (uint64_t)nchild);
@@ -310,21 +325,21 @@ __device__ inline void seg_close(const BertParams& P, uint32_t* segout, int myse
 
 // count the children, reserve their slots in the workgroup's segment and
 // write them; a single child (the common case) is written from pass 0's result
-__device__ inline void scatter_out(const BertParams& P, int32_t* H, Pcg32& rng, bool alive, int mode, int sign,
-                                   int box, double x, double y, uint64_t pid, uint32_t step, uint32_t* lds_count,
-                                   int myseg) {
+__device__ inline void scatter_out(const BertParams& P, int32_t* H, double* sbox, Pcg32& rng, bool alive, int mode,
+                                   int sign, int box, double x, double y, uint64_t pid, uint32_t step,
+                                   uint32_t* lds_count, int myseg) {
     int nm0 = 0, ns0 = 1;
-    int c = alive ? scatter<false>(P, H, rng, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
+    int c = alive ? scatter<false>(P, H, sbox, rng, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
     uint64_t limit;
     uint64_t slot = wave_reserve(lds_count, myseg, P.segcap, c, P.flags, limit);
     if (c == 1) {
-        atomicAdd(H + (size_t)box * P.n + nm0, ns0);
+        tally_child(P, H, sbox, box, nm0, ns0);
         if (slot < limit) {
             BPart q{x, y, (uint32_t)nm0, ((uint32_t)box << 1) | (ns0 < 0 ? 1u : 0u), pid};
             P.pout[slot] = q;
         }
     } else if (c > 1) {
-        scatter<true>(P, H, rng, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
+        scatter<true>(P, H, sbox, rng, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
     }
 }
 
@@ -345,11 +360,26 @@ __device__ inline void eb_lds_flush(const int32_t* sEb, int32_t* Eb, int nb4) {
     }
 }
 
+// Per-workgroup box sums for calculate_prop: cleared at kernel start, stored
+// to the workgroup's row of pbox at the end (no global atomics).
+__device__ inline double* box_lds_begin(const BertParams& P, double* s) {
+    if (!P.lds_props) return nullptr;
+    for (int i = threadIdx.x; i < P.nbox * 3; i += blockDim.x) s[i] = 0.;
+    return s;
+}
+__device__ inline void box_lds_flush(const BertParams& P, const double* s, int myseg) {
+    if (!P.lds_props) return;
+    __syncthreads();
+    double* o = P.pbox + (size_t)myseg * P.nbox * 3;
+    for (int i = threadIdx.x; i < P.nbox * 3; i += blockDim.x) o[i] = s[i];
+}
+
 // One workgroup per segment of the step's output. Workgroup w takes the
 // w-th equal slice of the live particle list (all input segments).
 __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t step, int cur,
-                                                        int cancel_prev) {
+                                                        int cancel_prev, int hist) {
     __shared__ int32_t sEb[BERT_EB_LDS];
+    __shared__ double sbox_[BERT_LDS_BOXES * 3];
     __shared__ uint32_t pre[BERT_MAXSEG];
     __shared__ uint32_t tsum[256];
     __shared__ uint32_t lcount;
@@ -361,13 +391,14 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
     int32_t* Eg = P.Eb + (size_t)(step & 1) * nb4;
     int32_t* Eb = eb_lds_begin(sEb, Eg, nb4);
     if (threadIdx.x == 0) lcount = 0;
+    double* sbox = box_lds_begin(P, sbox_);
     const uint32_t* segin = P.seg + (size_t)cur * P.nsegs;
     uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
     const uint32_t nin = seg_prefix(segin, P.nsegs, P.segcap, pre, tsum);  // ends with a barrier
     const uint32_t lo = (uint32_t)((uint64_t)nin * blockIdx.x / gridDim.x);
     const uint32_t hi = (uint32_t)((uint64_t)nin * (blockIdx.x + 1) / gridDim.x);
     const int myseg = blockIdx.x;
-    int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
+    int32_t* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
     const int32_t* Hprev = P.H + (size_t)(cur ^ 1) * P.nbox * P.n;
     const uint64_t skey = stream_key(P.seed, step, 0);
     unsigned long long done = 0;
@@ -406,11 +437,12 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
                 alive = advect(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
             }
         }
-        scatter_out(P, H, rng, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
+        scatter_out(P, H, sbox, rng, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
     eb_lds_flush(sEb, Eg, nb4);
+    box_lds_flush(P, sbox, myseg);
     // one atomic per wave for the particle-step counter
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
@@ -488,7 +520,8 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
 // in equal slices over the workgroups; each is placed uniformly on its side,
 // advected U*dt, scattered and histogrammed.
 #define BERT_EMIT_MAXB 4096
-__global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t step, int cur) {
+__global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
+    __shared__ double sbox_[BERT_LDS_BOXES * 3];
     __shared__ uint32_t pre[BERT_EMIT_MAXB];
     __shared__ uint32_t tsum[256];
     __shared__ int32_t sEb[BERT_EB_LDS];
@@ -503,10 +536,11 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
     geo_stage(P, geo, boxes, cts);
     int32_t* Eg = P.Eb + (size_t)((step + 1) & 1) * nb4;
     int32_t* Ebn = eb_lds_begin(sEb, Eg, nb4);
+    double* sbox = box_lds_begin(P, sbox_);
     const uint32_t total = seg_prefix(P.wcount, nb4, 0xffffffffu, pre, tsum);  // ends with a barrier
     const uint32_t lo = (uint32_t)((uint64_t)total * blockIdx.x / gridDim.x);
     const uint32_t hi = (uint32_t)((uint64_t)total * (blockIdx.x + 1) / gridDim.x);
-    int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
+    int32_t* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
     const uint64_t skey = stream_key(P.seed, step, 1);
     unsigned long long done = 0;
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
@@ -548,11 +582,12 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
             double rdt = P.dt * rng.uniform();
             alive = advect(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
         }
-        scatter_out(P, H, rng, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out(P, H, sbox, rng, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
     eb_lds_flush(sEb, Eg, nb4);
+    box_lds_flush(P, sbox, myseg);
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
@@ -567,12 +602,14 @@ struct BertSource {
     int32_t pad;
 };
 __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src,
-                                                          const uint2* stab, uint32_t step, int cur) {
+                                                          const uint2* stab, uint32_t step, int cur, int hist) {
     __shared__ uint32_t lcount;
+    __shared__ double sbox_[BERT_LDS_BOXES * 3];
     const BertSource S = src[blockIdx.x];
     const int myseg = P.seg_src0 + blockIdx.x;
     uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
     if (threadIdx.x == 0) lcount = 0;
+    double* sbox = box_lds_begin(P, sbox_);
     __syncthreads();
     int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
     Pcg32 cr;
@@ -580,7 +617,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     const uint64_t skey = stream_key(P.seed, step, 3);
     const double fl = floor(S.expected);
     const int ng = (int)fl + (cr.uniform() <= S.expected - fl ? 1 : 0);
-    int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
+    int32_t* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
     const int nloop = (ng + 63) & ~63;
     unsigned long long done = 0;
     for (int k = threadIdx.x; k - (int)(threadIdx.x & 63) < nloop; k += blockDim.x) {
@@ -609,18 +646,22 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
             double rdt = P.dt * rng.uniform();
             alive = advect(P, P.boxes, P.cts, x, y, bx, S.sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
         }
-        scatter_out(P, H, rng, alive, mode, S.sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out(P, H, sbox, rng, alive, mode, S.sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
+    box_lds_flush(P, sbox, myseg);
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
 
-// calculate_prop (beyondRTA_MC2d.cpp:1535-1602) from the step histogram, the
-// post-cancellation particle count, and buffer recycling for the next step.
-__global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, int cur,
-                                                         int cancel_now, int clear_tickets, int ebn) {
+// calculate_prop (beyondRTA_MC2d.cpp:1535-1602) from the step's box sums
+// (from_boxes) or its histogram, the post-cancellation particle count, and
+// buffer recycling for the next step (clear_old: the previous step wrote its
+// histogram, which the step kernel has now consumed).
+__global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, int cur, int cancel_now,
+                                                         int clear_tickets, int ebn, int from_boxes,
+                                                         int clear_old) {
     const int box = blockIdx.x;
     const int32_t* H = P.H + (size_t)cur * P.nbox * P.n + (size_t)box * P.n;
     int32_t* Hold = P.H + (size_t)(cur ^ 1) * P.nbox * P.n + (size_t)box * P.n;
@@ -628,18 +669,32 @@ __global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, 
     __shared__ long long redc[2][256];
     double s = 0, fx = 0, fy = 0;
     long long cnt = 0, acnt = 0;
-    for (int k = threadIdx.x; k < P.n; k += blockDim.x) {
-        int h = H[k];
-        if (h != 0) {
-            const double2 v = P.vel[k];
-            s += h;
-            fx += h * v.x;
-            fy += h * v.y;
-            cnt += h;
-            acnt += h < 0 ? -h : h;
+    if (!from_boxes || cancel_now || clear_old || clear_tickets) {
+        for (int k = threadIdx.x; k < P.n; k += blockDim.x) {
+            if (!from_boxes || cancel_now) {
+                int h = H[k];
+                if (h != 0) {
+                    const double2 v = P.vel[k];
+                    s += h;
+                    fx += h * v.x;
+                    fy += h * v.y;
+                    cnt += h;
+                    acnt += h < 0 ? -h : h;
+                }
+            }
+            if (clear_old) Hold[k] = 0;
+            if (clear_tickets) P.tickets[(size_t)box * P.n + k] = 0;
         }
-        Hold[k] = 0;
-        if (clear_tickets) P.tickets[(size_t)box * P.n + k] = 0;
+    }
+    if (from_boxes) {
+        // fixed summation order over the writing workgroups
+        s = fx = fy = 0.;
+        for (int g = threadIdx.x; g < P.nsegs; g += blockDim.x) {
+            const double* q = P.pbox + ((size_t)g * P.nbox + box) * 3;
+            s += q[0];
+            fx += q[1];
+            fy += q[2];
+        }
     }
     red[0][threadIdx.x] = s;
     red[1][threadIdx.x] = fx;
@@ -718,7 +773,8 @@ struct BertInit {
 __global__ __launch_bounds__(256) void bert_init_kernel(BertParams P, const BertInit* ini,
                                                         const uint2* itab) {
     const BertInit I = ini[blockIdx.x];
-    for (uint64_t k = threadIdx.x; k < I.count; k += blockDim.x) {
+    for (uint64_t k = (uint64_t)blockIdx.y * blockDim.x + threadIdx.x; k < I.count;
+         k += (uint64_t)gridDim.y * blockDim.x) {
         uint64_t pid = I.first + k;
         Pcg32 rng;
         seed_stream(rng, stream_key(P.seed, 0xffffffu, 4), pid);
@@ -745,7 +801,7 @@ struct Bert {
     DevBuf<BPart> d_parts;
     DevBuf<uint32_t> d_seg, d_tickets, d_flags;
     DevBuf<int32_t> d_H, d_Eb, d_side_spl, d_rta_slot;
-    DevBuf<double> d_props, d_spl, d_Teq;
+    DevBuf<double> d_props, d_spl, d_Teq, d_pbox;
     DevBuf<int64_t> d_npart, d_pending;
     DevBuf<double> d_wcdf;
     DevBuf<uint32_t> d_wcount;
@@ -819,6 +875,8 @@ static BertParams make_params(Bert* b) {
     P.nseg = b->nseg;
     P.rta = b->cfg.rta;
     P.max_children = 1 << 20;
+    P.pbox = b->d_pbox.p;
+    P.lds_props = (int)b->g->boxes.size() <= BERT_LDS_BOXES ? 1 : 0;
     return P;
 }
 
@@ -994,6 +1052,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     b->d_stats.alloc(2);
     b->d_H.alloc((size_t)2 * nbox * n);
     b->d_tickets.alloc((size_t)nbox * n);
+    if (nbox <= BERT_LDS_BOXES) b->d_pbox.alloc((size_t)b->nsegs * nbox * 3);
     b->d_Eb.alloc((size_t)2 * nbox * 4);
     b->d_props.alloc((size_t)(b->max_steps + 1) * nbox * 3);
     b->d_npart.alloc((size_t)(b->max_steps + 1));
@@ -1119,7 +1178,8 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     P.pin = buf[0];
     P.pout = buf[1];
     if (b->ninit) {
-        hipLaunchKernelGGL(bert_init_kernel, dim3(b->ninit), dim3(256), 0, s, P, b->d_init.p, b->d_init_tab.p);
+        hipLaunchKernelGGL(bert_init_kernel, dim3(b->ninit, 1024), dim3(256), 0, s, P, b->d_init.p,
+                           b->d_init_tab.p);
         HIPCHECK(hipGetLastError());
     }
     // row 0: properties of the initial particles
@@ -1128,7 +1188,7 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     // props of H[0] into row 0; the kernel clears H[1] and the input segment
     // counters; restore them afterwards because the initial particles are
     // the first step's input
-    hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, 0, 0, 0, 0, 0);
+    hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, 0, 0, 0, 0, 0, 0, 1);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(b->d_seg.p, c0.data(), c0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)nbox * n * sizeof(int32_t), s));
@@ -1140,25 +1200,30 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     auto t0 = std::chrono::steady_clock::now();
     HIPCHECK(hipEventRecord(b->ev0, s));
     int istep = 0;
+    int prev_hist = 0;
     for (; istep < b->max_steps; istep++) {
         const int cur = istep & 1;
         P.pin = buf[cur];
         P.pout = buf[cur ^ 1];
         const int cancel_prev = istep > 0 && ((istep - 1) % pc == 0);
         const int cancel_now = (istep % pc == 0);
+        // the (box, mode) histogram is built only where a cancellation or a
+        // frame reads it; properties come from the box sums otherwise
+        const int hist = cancel_now || cb != nullptr || !P.lds_props;
         HIPCHECK(hipEventRecord(b->kev[2 * istep], s));
         hipLaunchKernelGGL(bert_step_kernel, dim3(step_blocks), dim3(256), 0, s, P, (uint32_t)istep, cur,
-                           cancel_prev);
+                           cancel_prev, hist);
         HIPCHECK(hipEventRecord(b->kev[2 * istep + 1], s));
         if (b->nsrc)
             hipLaunchKernelGGL(bert_source_kernel, dim3(b->nsrc), dim3(256), 0, s, P, b->d_src.p,
-                               b->d_src_tab.p, (uint32_t)istep, cur);
+                               b->d_src_tab.p, (uint32_t)istep, cur, hist);
         hipLaunchKernelGGL(bert_wallcdf_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep);
-        hipLaunchKernelGGL(bert_emit_kernel, dim3(emit_blocks), dim3(256), 0, s, P, (uint32_t)istep, cur);
+        hipLaunchKernelGGL(bert_emit_kernel, dim3(emit_blocks), dim3(256), 0, s, P, (uint32_t)istep, cur, hist);
         // H of this step lives in H[cur]; H[cur^1] (previous step) is cleared
         // after the step kernel used it for cancellation
         hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, istep + 1, cur, cancel_now,
-                           cancel_now, (istep + 1) & 1);
+                           cancel_now, (istep + 1) & 1, P.lds_props, prev_hist);
+        prev_hist = hist;
         HIPCHECK(hipGetLastError());
         if (cb) {
             // the histogram of this step is in H[cur] until the next step's
