@@ -73,6 +73,9 @@ struct BertParams {
     int32_t n, ngamma, ncol, nbox, nseg;
     int32_t rta, max_children;
     int32_t lds_props;       // properties from per-workgroup box sums (nbox <= BERT_LDS_BOXES)
+    // dynamic LDS layout of the particle kernels (byte offsets)
+    int32_t lds_pre, lds_eb, lds_sbox, lds_boxes, lds_cts;
+    int32_t eb_lds;          // wall counters in LDS (nbox*4 <= BERT_EB_LDS)
 };
 
 // The stream of particle pid in a given step and kernel: PCG32 with
@@ -85,19 +88,21 @@ __device__ inline void seed_stream(Pcg32& r, uint64_t key, uint64_t pid) {
     r.seed((pid * 0x9e3779b97f4a7c15ULL) ^ key, pid);
 }
 
-// Geometry tables small enough for LDS are staged there by every workgroup.
+// The particle kernels share one dynamic LDS block: segment prefix sums,
+// wall counters, box sums and, when small enough (GL instantiations), the
+// geometry tables. Offsets are set on the host.
 #define BERT_LDS_BOXES 128
-#define BERT_LDS_CTS 512
-struct GeoLds {
-    DevBox boxes[BERT_LDS_BOXES];
-    DevContact cts[BERT_LDS_CTS];
-};
-__device__ inline void geo_stage(const BertParams& P, GeoLds& g, const DevBox*& boxes, const DevContact*& cts) {
-    if (P.nbox <= BERT_LDS_BOXES && P.ncts <= BERT_LDS_CTS) {
-        for (int i = threadIdx.x; i < P.nbox; i += blockDim.x) g.boxes[i] = P.boxes[i];
-        for (int i = threadIdx.x; i < P.ncts; i += blockDim.x) g.cts[i] = P.cts[i];
-        boxes = g.boxes;
-        cts = g.cts;
+#define BERT_GEO_LDS_BYTES 24576
+extern __shared__ __align__(16) unsigned char bert_dyn[];
+template <bool GL>
+__device__ inline void geo_stage(const BertParams& P, const DevBox*& boxes, const DevContact*& cts) {
+    if (GL) {
+        DevBox* gb = (DevBox*)(bert_dyn + P.lds_boxes);
+        DevContact* gc = (DevContact*)(bert_dyn + P.lds_cts);
+        for (int i = threadIdx.x; i < P.nbox; i += blockDim.x) gb[i] = P.boxes[i];
+        for (int i = threadIdx.x; i < P.ncts; i += blockDim.x) gc[i] = P.cts[i];
+        boxes = gb;
+        cts = gc;
     } else {
         boxes = P.boxes;
         cts = P.cts;
@@ -346,13 +351,15 @@ __device__ inline void scatter_out(const BertParams& P, int32_t* H, double* sbox
 // Wall counters are accumulated per workgroup in LDS and flushed with one
 // global atomic per touched side (the walls of a few boxes take most hits).
 #define BERT_EB_LDS 1024
-__device__ inline int32_t* eb_lds_begin(int32_t* sEb, int32_t* Eb, int nb4) {
-    if (nb4 > BERT_EB_LDS) return Eb;
+__device__ inline int32_t* eb_lds_begin(const BertParams& P, int32_t* Eb, int nb4) {
+    if (!P.eb_lds) return Eb;
+    int32_t* sEb = (int32_t*)(bert_dyn + P.lds_eb);
     for (int i = threadIdx.x; i < nb4; i += blockDim.x) sEb[i] = 0;
     return sEb;
 }
-__device__ inline void eb_lds_flush(const int32_t* sEb, int32_t* Eb, int nb4) {
-    if (nb4 > BERT_EB_LDS) return;
+__device__ inline void eb_lds_flush(const BertParams& P, int32_t* Eb, int nb4) {
+    if (!P.eb_lds) return;
+    const int32_t* sEb = (const int32_t*)(bert_dyn + P.lds_eb);
     __syncthreads();
     for (int i = threadIdx.x; i < nb4; i += blockDim.x) {
         int v = sEb[i];
@@ -362,13 +369,15 @@ __device__ inline void eb_lds_flush(const int32_t* sEb, int32_t* Eb, int nb4) {
 
 // Per-workgroup box sums for calculate_prop: cleared at kernel start, stored
 // to the workgroup's row of pbox at the end (no global atomics).
-__device__ inline double* box_lds_begin(const BertParams& P, double* s) {
+__device__ inline double* box_lds_begin(const BertParams& P) {
     if (!P.lds_props) return nullptr;
+    double* s = (double*)(bert_dyn + P.lds_sbox);
     for (int i = threadIdx.x; i < P.nbox * 3; i += blockDim.x) s[i] = 0.;
     return s;
 }
-__device__ inline void box_lds_flush(const BertParams& P, const double* s, int myseg) {
+__device__ inline void box_lds_flush(const BertParams& P, int myseg) {
     if (!P.lds_props) return;
+    const double* s = (const double*)(bert_dyn + P.lds_sbox);
     __syncthreads();
     double* o = P.pbox + (size_t)myseg * P.nbox * 3;
     for (int i = threadIdx.x; i < P.nbox * 3; i += blockDim.x) o[i] = s[i];
@@ -376,22 +385,20 @@ __device__ inline void box_lds_flush(const BertParams& P, const double* s, int m
 
 // One workgroup per segment of the step's output. Workgroup w takes the
 // w-th equal slice of the live particle list (all input segments).
+template <bool GL>
 __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t step, int cur,
                                                         int cancel_prev, int hist) {
-    __shared__ int32_t sEb[BERT_EB_LDS];
-    __shared__ double sbox_[BERT_LDS_BOXES * 3];
-    __shared__ uint32_t pre[BERT_MAXSEG];
     __shared__ uint32_t tsum[256];
     __shared__ uint32_t lcount;
-    __shared__ GeoLds geo;
+    uint32_t* pre = (uint32_t*)(bert_dyn + P.lds_pre);
     const DevBox* boxes;
     const DevContact* cts;
-    geo_stage(P, geo, boxes, cts);
+    geo_stage<GL>(P, boxes, cts);
     const int nb4 = P.nbox * 4;
     int32_t* Eg = P.Eb + (size_t)(step & 1) * nb4;
-    int32_t* Eb = eb_lds_begin(sEb, Eg, nb4);
+    int32_t* Eb = eb_lds_begin(P, Eg, nb4);
     if (threadIdx.x == 0) lcount = 0;
-    double* sbox = box_lds_begin(P, sbox_);
+    double* sbox = box_lds_begin(P);
     const uint32_t* segin = P.seg + (size_t)cur * P.nsegs;
     uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
     const uint32_t nin = seg_prefix(segin, P.nsegs, P.segcap, pre, tsum);  // ends with a barrier
@@ -402,11 +409,14 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
     const int32_t* Hprev = P.H + (size_t)(cur ^ 1) * P.nbox * P.n;
     const uint64_t skey = stream_key(P.seed, step, 0);
     unsigned long long done = 0;
+    // segment of the wave's first index, advanced as the wave moves on
+    int sgw = lo < hi ? seg_search(pre, P.nsegs, lo) : 0;
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < hi;
-        // segment of the wave's first index, then a short walk per lane
-        int sg = seg_search(pre, P.nsegs, base + (threadIdx.x & ~63u));
+        const uint32_t wfirst = base + (threadIdx.x & ~63u);
+        while (sgw < P.nsegs - 1 && wfirst >= pre[sgw]) sgw++;
+        int sg = sgw;
         BPart p{};
         int box = 0, sign = 1, mode = 0;
         bool alive = false;
@@ -441,8 +451,8 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
-    eb_lds_flush(sEb, Eg, nb4);
-    box_lds_flush(P, sbox, myseg);
+    eb_lds_flush(P, Eg, nb4);
+    box_lds_flush(P, myseg);
     // one atomic per wave for the particle-step counter
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
@@ -520,23 +530,21 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
 // in equal slices over the workgroups; each is placed uniformly on its side,
 // advected U*dt, scattered and histogrammed.
 #define BERT_EMIT_MAXB 4096
+template <bool GL>
 __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
-    __shared__ double sbox_[BERT_LDS_BOXES * 3];
-    __shared__ uint32_t pre[BERT_EMIT_MAXB];
     __shared__ uint32_t tsum[256];
-    __shared__ int32_t sEb[BERT_EB_LDS];
     __shared__ uint32_t lcount;
-    __shared__ GeoLds geo;
+    uint32_t* pre = (uint32_t*)(bert_dyn + P.lds_pre);
     const int nb4 = P.nbox * 4;
     const int myseg = P.seg_emit0 + blockIdx.x;
     uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
     if (threadIdx.x == 0) lcount = 0;
     const DevBox* boxes;
     const DevContact* cts;
-    geo_stage(P, geo, boxes, cts);
+    geo_stage<GL>(P, boxes, cts);
     int32_t* Eg = P.Eb + (size_t)((step + 1) & 1) * nb4;
-    int32_t* Ebn = eb_lds_begin(sEb, Eg, nb4);
-    double* sbox = box_lds_begin(P, sbox_);
+    int32_t* Ebn = eb_lds_begin(P, Eg, nb4);
+    double* sbox = box_lds_begin(P);
     const uint32_t total = seg_prefix(P.wcount, nb4, 0xffffffffu, pre, tsum);  // ends with a barrier
     const uint32_t lo = (uint32_t)((uint64_t)total * blockIdx.x / gridDim.x);
     const uint32_t hi = (uint32_t)((uint64_t)total * (blockIdx.x + 1) / gridDim.x);
@@ -586,8 +594,8 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
-    eb_lds_flush(sEb, Eg, nb4);
-    box_lds_flush(P, sbox, myseg);
+    eb_lds_flush(P, Eg, nb4);
+    box_lds_flush(P, myseg);
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
@@ -604,12 +612,11 @@ struct BertSource {
 __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src,
                                                           const uint2* stab, uint32_t step, int cur, int hist) {
     __shared__ uint32_t lcount;
-    __shared__ double sbox_[BERT_LDS_BOXES * 3];
     const BertSource S = src[blockIdx.x];
     const int myseg = P.seg_src0 + blockIdx.x;
     uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
     if (threadIdx.x == 0) lcount = 0;
-    double* sbox = box_lds_begin(P, sbox_);
+    double* sbox = box_lds_begin(P);
     __syncthreads();
     int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
     Pcg32 cr;
@@ -650,7 +657,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
-    box_lds_flush(P, sbox, myseg);
+    box_lds_flush(P, myseg);
     for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
@@ -811,6 +818,8 @@ struct Bert {
     DevBuf<BertSource> d_src;
     DevBuf<BertInit> d_init;
     int nsrc = 0, ninit = 0;
+    bool geo_lds = false;
+    size_t lds_bytes = 0;
     int step_blocks = 0, emit_blocks = 0, nsegs = 0;
     std::vector<int32_t> hH;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -877,6 +886,23 @@ static BertParams make_params(Bert* b) {
     P.max_children = 1 << 20;
     P.pbox = b->d_pbox.p;
     P.lds_props = (int)b->g->boxes.size() <= BERT_LDS_BOXES ? 1 : 0;
+    // dynamic LDS layout
+    const int nb4 = P.nbox * 4;
+    auto up16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
+    size_t off = 0;
+    P.lds_pre = (int32_t)off;
+    off = up16(off + sizeof(uint32_t) * (size_t)std::max(P.nsegs, nb4));
+    P.eb_lds = nb4 <= BERT_EB_LDS ? 1 : 0;
+    P.lds_eb = (int32_t)off;
+    if (P.eb_lds) off = up16(off + sizeof(int32_t) * (size_t)nb4);
+    P.lds_sbox = (int32_t)off;
+    if (P.lds_props) off = up16(off + sizeof(double) * 3 * (size_t)P.nbox);
+    const size_t geo = sizeof(DevBox) * (size_t)P.nbox + sizeof(DevContact) * (size_t)P.ncts;
+    b->geo_lds = geo <= BERT_GEO_LDS_BYTES;
+    P.lds_boxes = (int32_t)off;
+    P.lds_cts = (int32_t)up16(off + sizeof(DevBox) * (size_t)P.nbox);
+    if (b->geo_lds) off = up16(P.lds_cts + sizeof(DevContact) * (size_t)P.ncts);
+    b->lds_bytes = off;
     return P;
 }
 
@@ -1211,14 +1237,23 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         // frame reads it; properties come from the box sums otherwise
         const int hist = cancel_now || cb != nullptr || !P.lds_props;
         HIPCHECK(hipEventRecord(b->kev[2 * istep], s));
-        hipLaunchKernelGGL(bert_step_kernel, dim3(step_blocks), dim3(256), 0, s, P, (uint32_t)istep, cur,
-                           cancel_prev, hist);
+        if (b->geo_lds)
+            hipLaunchKernelGGL(bert_step_kernel<true>, dim3(step_blocks), dim3(256), b->lds_bytes, s, P,
+                               (uint32_t)istep, cur, cancel_prev, hist);
+        else
+            hipLaunchKernelGGL(bert_step_kernel<false>, dim3(step_blocks), dim3(256), b->lds_bytes, s, P,
+                               (uint32_t)istep, cur, cancel_prev, hist);
         HIPCHECK(hipEventRecord(b->kev[2 * istep + 1], s));
         if (b->nsrc)
-            hipLaunchKernelGGL(bert_source_kernel, dim3(b->nsrc), dim3(256), 0, s, P, b->d_src.p,
+            hipLaunchKernelGGL(bert_source_kernel, dim3(b->nsrc), dim3(256), b->lds_bytes, s, P, b->d_src.p,
                                b->d_src_tab.p, (uint32_t)istep, cur, hist);
         hipLaunchKernelGGL(bert_wallcdf_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep);
-        hipLaunchKernelGGL(bert_emit_kernel, dim3(emit_blocks), dim3(256), 0, s, P, (uint32_t)istep, cur, hist);
+        if (b->geo_lds)
+            hipLaunchKernelGGL(bert_emit_kernel<true>, dim3(emit_blocks), dim3(256), b->lds_bytes, s, P,
+                               (uint32_t)istep, cur, hist);
+        else
+            hipLaunchKernelGGL(bert_emit_kernel<false>, dim3(emit_blocks), dim3(256), b->lds_bytes, s, P,
+                               (uint32_t)istep, cur, hist);
         // H of this step lives in H[cur]; H[cur^1] (previous step) is cleared
         // after the step kernel used it for cancellation
         hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, istep + 1, cur, cancel_now,
