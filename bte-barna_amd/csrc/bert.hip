@@ -59,6 +59,7 @@ struct BertParams {
     int64_t* npart;          // [steps+1]
     int64_t* pending;        // [steps+1] signed wall counts awaiting re-emission
     double* wcdf;            // [nbox*4][n] emission cumulative per box side
+    int32_t* wguide;         // [nbox*4][BERT_GUIDE] first cumulative index per key
     uint32_t* wcount;        // [nbox*4] particles to emit this step
     int32_t* wsign;          // [nbox*4] their sign
     uint32_t* flags;         // [0] overflow, [1] bad mode, [2] stuck
@@ -458,6 +459,14 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
 }
 
+// Emission sampling: first index i with cdf[i] >= u (the reference's
+// lower_bound on the cumulative), started from a guide table.
+#define BERT_GUIDE 1024
+__device__ inline int guide_key(double v, double scale) {
+    const double k = v * scale;
+    return k >= (double)(BERT_GUIDE - 1) ? BERT_GUIDE - 1 : (int)k;
+}
+
 // Adiabatic diffuse walls (beyondRTA_MC2d.cpp:281-435), phase 1: one
 // workgroup per (box, side) with a nonzero Eborder computes the border
 // temperature from the J(T) spline and the emission cumulative of
@@ -483,10 +492,11 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     const double L = (side == 0 || side == 3) ? (B.ymax - B.ymin) : (B.xmax - B.xmin);
     const double J = P.Eeff * (double)E / P.dt / (L * P.thickness);
     const double* S = P.spl + (size_t)si * 6 * P.nseg;  // a b c d x0 xe
-    int lo = 0, hi = P.nseg;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (S[5 * P.nseg + mid] < J) lo = mid + 1; else hi = mid;
+    // lower bound of J among the ascending knots: the count of knots below J
+    int lo = 0;
+    for (int k0 = 0; k0 < P.nseg; k0 += blockDim.x) {
+        const int k = k0 + threadIdx.x;
+        lo += __syncthreads_count(k < P.nseg && S[5 * P.nseg + k] < J);
     }
     if (lo >= P.nseg) lo = P.nseg - 1;
     const double dx = J - S[4 * P.nseg + lo];
@@ -519,11 +529,29 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
         __syncthreads();
     }
     const double off = threadIdx.x > 0 ? wsum[threadIdx.x - 1] : 0.;
-    for (int i = i0; i < i1; i++) cdf[i] += off;
+    __shared__ double tot;
+    for (int i = i0; i < i1; i++) {
+        cdf[i] += off;
+        if (i == n - 1) tot = cdf[i];
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        P.wcount[b] = wsum[255] > 0. ? (uint32_t)(fabs((double)E) + 1.0e-6) : 0u;
+        P.wcount[b] = tot > 0. ? (uint32_t)(fabs((double)E) + 1.0e-6) : 0u;
         P.wsign[b] = (Tb - Teq) >= 0 ? 1 : -1;
     }
+    if (!(tot > 0.)) return;
+    // guide table: G[j] = first index whose key reaches j, so the search for
+    // u starts at G[key(u)] <= answer (key is monotone)
+    const double scale = (double)BERT_GUIDE / tot;
+    int32_t* G = P.wguide + (size_t)b * BERT_GUIDE;
+    int kprev = i0 > 0 ? guide_key(cdf[i0 - 1], scale) : -1;
+    for (int i = i0; i < i1; i++) {
+        const int k = guide_key(cdf[i], scale);
+        for (int j = kprev + 1; j <= k; j++) G[j] = i;
+        kprev = k;
+    }
+    if (i1 == n && i0 < i1)
+        for (int j = kprev + 1; j < BERT_GUIDE; j++) G[j] = n - 1;
 }
 
 // Phase 2: the |Eborder| particles of every side form one index space split
@@ -551,9 +579,12 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
     int32_t* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
     const uint64_t skey = stream_key(P.seed, step, 1);
     unsigned long long done = 0;
+    int bw = lo < hi ? seg_search(pre, nb4, lo) : 0;
     for (uint32_t base = lo; base < hi; base += blockDim.x) {
         const uint32_t k = base + threadIdx.x;
         const bool valid = k < hi;
+        const uint32_t wfirst = base + (threadIdx.x & ~63u);
+        while (bw < nb4 - 1 && wfirst >= pre[bw]) bw++;
         Pcg32 rng;
         int mode = 0, bx = 0, sign = 1;
         double x = 0, y = 0;
@@ -561,7 +592,8 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
         uint64_t pid = 0;
         if (valid) {
             // side b holds indices [pre[b-1], pre[b])
-            const int b = seg_search(pre, nb4, k);
+            int b = bw;
+            while (b < nb4 - 1 && k >= pre[b]) b++;
             const uint32_t j = k - (b > 0 ? pre[b - 1] : 0u);
             bx = b >> 2;
             const int side = b & 3;
@@ -569,12 +601,10 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
             pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)b << 20) ^ (uint64_t)j ^ 0xad1aULL);
             seed_stream(rng, skey, pid);
             const double* cdf = P.wcdf + (size_t)b * P.n;
-            double u = rng.uniform() * cdf[P.n - 1];
-            int l = 0, h = P.n - 1;
-            while (l < h) {
-                int mid = (l + h) >> 1;
-                if (cdf[mid] < u) l = mid + 1; else h = mid;
-            }
+            const double tot = cdf[P.n - 1];
+            double u = rng.uniform() * tot;
+            int l = P.wguide[(size_t)b * BERT_GUIDE + guide_key(u, (double)BERT_GUIDE / tot)];
+            while (l < P.n - 1 && cdf[l] < u) l++;
             mode = l;
             const DevBox B = boxes[bx];
             double p1x, p1y, ndx, ndy;
@@ -811,6 +841,7 @@ struct Bert {
     DevBuf<double> d_props, d_spl, d_Teq, d_pbox;
     DevBuf<int64_t> d_npart, d_pending;
     DevBuf<double> d_wcdf;
+    DevBuf<int32_t> d_wguide;
     DevBuf<uint32_t> d_wcount;
     DevBuf<int32_t> d_wsign;
     DevBuf<unsigned long long> d_stats;
@@ -863,6 +894,7 @@ static BertParams make_params(Bert* b) {
     P.npart = b->d_npart.p;
     P.pending = b->d_pending.p;
     P.wcdf = b->d_wcdf.p;
+    P.wguide = b->d_wguide.p;
     P.wcount = b->d_wcount.p;
     P.wsign = b->d_wsign.p;
     P.flags = b->d_flags.p;
@@ -1084,6 +1116,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     b->d_npart.alloc((size_t)(b->max_steps + 1));
     b->d_pending.alloc((size_t)(b->max_steps + 1));
     b->d_wcdf.alloc((size_t)nbox * 4 * n);
+    b->d_wguide.alloc((size_t)nbox * 4 * BERT_GUIDE);
     b->d_wcount.alloc((size_t)nbox * 4);
     b->d_wsign.alloc((size_t)nbox * 4);
     std::vector<double> Teq(nbox);
