@@ -143,8 +143,8 @@ def main():
         print(json.dumps({"copy_bytes": x.numel() * 4, "copies": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU sharding is described in "
-              f"DESIGN.md and is not run here.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); the engine runs on a single GPU and "
+              f"multi-GPU sharding is not part of this build.", file=sys.stderr)
         sys.exit(2)
 
     pkg = load_pkg()

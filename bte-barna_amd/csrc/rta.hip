@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
             npb = fast == SIDE_VOID ? 0 : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
         }
         if (npb == 0) {
-            // void side: Lambert re-emission, remaining flight discarded (Q8)
+            // void side: Lambert re-emission, remaining flight discarded (RTA_MC2D.cpp:1259)
             int s = fast == SIDE_VOID ? side
                   : (__popc(mask) == 1 ? __ffs(mask) - 1
                                        : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next()));

@@ -167,7 +167,7 @@ int bte_bert_destroy(bte_bert* b);
 #define BTE_PREC_FP64 0
 #define BTE_PREC_FP32 1
 #define BTE_PREC_BF16 2
-#define BTE_ARNOLDI_REFERENCE 0 /* projections over j<k only (Q1) */
+#define BTE_ARNOLDI_REFERENCE 0 /* projections over j<k only, as collision_operator.cpp:831-850 */
 #define BTE_ARNOLDI_TEXTBOOK 1  /* modified Gram-Schmidt over j<=k */
 typedef struct {
     double build_ms;   /* device time of the whole build */

@@ -13,7 +13,8 @@
 //   calculate_prop (histogram) beyondRTA_MC2d.cpp:1535-1602
 //   step loop                  beyondRTA_MC2d.cpp:2138-2425
 // Deviations applied identically on the GPU path: per-particle PCG32 streams
-// instead of thread-duplicated pcg64 (Q2); explicit dead flag (Q6);
+// instead of default-seeded per-thread pcg64 (beyondRTA_MC2d.cpp:1992-1993);
+// an explicit dead flag instead of the mode (0,0) sentinel;
 // gradient sources are zero (uniform Teq) and are rejected otherwise.
 #include "orc_geometry.hpp"
 #include <omp.h>

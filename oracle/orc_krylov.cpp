@@ -2,7 +2,7 @@
 // Krylov path and of the synthetic collision operator used by the tests.
 //   error_bound          include/collision_operator.hpp:216-219
 //   Arnoldi_algorithm    src/collision_operator.cpp:815-851 (mode 0 keeps
-//                        the reference's j<k projection loop, Q1; mode 1 is
+//                        the reference's j<k projection loop; mode 1 is
 //                        textbook MGS over j<=k)
 //   build_P              src/collision_operator.cpp:1015-1064
 //   energy projection    src/collision_operator.cpp:786-807

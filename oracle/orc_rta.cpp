@@ -8,9 +8,10 @@
 //   post-processing         RTA_MC2D.cpp:2326-2435
 // Deviations, each forced by the test setting and applied identically to
 // the GPU path: one PCG32 stream per particle (the reference copies one
-// pcg64 into every thread, Q2); u in (0,1] for random_dt (Q11); the random
-// void-border choice uses [0,size) (Q3); the dead-particle sentinel is an
-// explicit flag instead of mode (0,0) (Q6); the Lambert broadening uses a
+// pcg64 into every thread, RTA_MC2D.cpp:2131-2134); u in (0,1] for random_dt
+// (sampling.hpp:44-45 allows u = 0); the random void-border choice uses
+// [0,size) (RTA_MC2D.cpp:1234-1235 is inclusive); the dead-particle sentinel
+// is an explicit flag instead of mode (0,0); the Lambert broadening uses a
 // constant sigma per mode because a synthetic table has no q-grid.
 #include "orc_geometry.hpp"
 #include <omp.h>
