@@ -6,6 +6,7 @@ Python here is a thin host mirror used by the tests and the benchmark; the
 C++ front-ends in apps/ are the drop-in executables.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -224,3 +225,23 @@ def build_P(ctx, B, dt, m=30, precision="fp64", arnoldi="textbook"):
                                  ARNOLDI[arnoldi], P.ctypes.data_as(C.POINTER(C.c_double)), C.byref(rep)))
     return P, {"build_ms": rep.build_ms, "gemm_ms": rep.gemm_ms, "gemm_tflops": rep.gemm_tflops,
                "max_col_sum_err": rep.max_col_sum_err, "breakdowns": rep.breakdowns}
+
+
+def krylov_size(B, dt, eps=1e-8):
+    """The reference's Krylov size rule for exp(dt*B) (host only)."""
+    B = np.asfortranarray(B, dtype=np.float64)
+    m = C.c_int(0)
+    check(lib().bte_krylov_size(B.shape[0], B.ctypes.data_as(C.POINTER(C.c_double)), dt, eps, C.byref(m)))
+    return m.value
+
+
+APPS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+
+def app_path(name):
+    """Path of a built front-end executable (RTAMC2D, beRTAMC2D, PropagatorBuilder)."""
+    p = os.path.join(APPS_DIR, name)
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: run __graft_entry__.build() first")
+    return p
+

@@ -87,6 +87,7 @@ SIGNATURES = {
     "bte_bert_destroy": (C.c_int, [vp]),
     "bte_krylov_build": (C.c_int, [vp, C.c_int, dp, C.c_double, C.c_int, C.c_int, C.c_int, dp,
                                    C.POINTER(KrylovReport)]),
+    "bte_krylov_size": (C.c_int, [C.c_int, dp, C.c_double, C.c_double, C.POINTER(C.c_int)]),
 }
 
 _lib = None

@@ -1,0 +1,262 @@
+// Shared pieces of the solver front-ends: geometry and material input,
+// propagator files, CSV and msgpack output, error reporting.
+#pragma once
+#include "bte_abi.h"
+#include "xml_lite.hpp"
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+namespace front {
+
+constexpr double kB = 1.3806488e-23;      // include/constants.hpp:37
+constexpr double hbar = 1.054571726e-34;  // include/constants.hpp:47
+
+inline void check(int rc, const char* what) {
+    if (rc == BTE_OK) return;
+    char msg[1024];
+    bte_last_error(msg, sizeof msg);
+    throw std::runtime_error(std::string(what) + ": " + msg);
+}
+
+// read_geometry_XML (src/geometry_2d.cpp:408-522) for axis-aligned
+// rectangles: boxes in file order, ids assigned in that order.
+struct Boxes {
+    std::vector<double> rect;  // xmin xmax ymin ymax per box
+    std::vector<int32_t> kind;
+    std::vector<double> Teq, Tinit;
+    std::vector<std::string> material;
+    int n() const { return (int)kind.size(); }
+};
+
+inline Boxes read_geometry(const std::string& fn) {
+    auto root = xml::parse_file(fn);
+    if (root->name != "Geometry") throw std::runtime_error(fn + ": root element is not <Geometry>");
+    Boxes g;
+    for (const xml::Node* b : root->children("Box")) {
+        const int id = g.n();
+        const xml::Node* v = b->child("Vertices");
+        if (!v) throw std::runtime_error("box " + std::to_string(id) + ": no <Vertices>");
+        const int dim = (int)v->num("dim"), np = (int)v->num("npoints");
+        if (dim != 2) throw std::runtime_error("box " + std::to_string(id) + ": Vertices dim must be 2");
+        std::stringstream ss(v->text);
+        std::vector<double> xy(2 * (size_t)np);
+        for (auto& z : xy)
+            if (!(ss >> z)) throw std::runtime_error("box " + std::to_string(id) + ": short vertex list");
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        for (int k = 0; k < np; k++) {
+            x0 = std::min(x0, xy[2 * k]);
+            x1 = std::max(x1, xy[2 * k]);
+            y0 = std::min(y0, xy[2 * k + 1]);
+            y1 = std::max(y1, xy[2 * k + 1]);
+        }
+        bool rect = np == 4 && x1 > x0 && y1 > y0;
+        for (int k = 0; rect && k < np; k++)
+            rect = (xy[2 * k] == x0 || xy[2 * k] == x1) && (xy[2 * k + 1] == y0 || xy[2 * k + 1] == y1);
+        if (!rect)
+            throw std::runtime_error("box " + std::to_string(id) +
+                                     ": only axis-aligned rectangles are supported by the GPU engine");
+        if (b->child("Translate_to"))
+            throw std::runtime_error("box " + std::to_string(id) + ": periodic boxes (<Translate_to>) are not supported");
+        if (const xml::Node* th = b->child("theta"))
+            if (th->num("angle") != 0.)
+                throw std::runtime_error("box " + std::to_string(id) + ": rotated materials are not supported");
+        const xml::Node* ic = b->child("initCnd");
+        if (!ic) throw std::runtime_error("box " + std::to_string(id) + ": no <initCnd>");
+        const double Teq = ic->num("Teq");
+        g.rect.insert(g.rect.end(), {x0, x1, y0, y1});
+        g.kind.push_back(b->child("Reservoir") ? 1 : 0);
+        g.Teq.push_back(Teq);
+        g.Tinit.push_back(ic->num("Tinit", Teq));
+        const xml::Node* mid = b->child("MaterialID");
+        g.material.push_back(mid ? mid->str("name") : "");
+    }
+    if (g.n() == 0) throw std::runtime_error(fn + ": no boxes");
+    return g;
+}
+
+// Bose-Einstein kernel (x/sinh x)^2, x = hbar omega / 2 kB T
+// (include/utilities.hpp:401-415)
+inline double be_kernel(double omega, double T) {
+    const double x = 0.5e12 * hbar * omega / (kB * T);
+    if (x < 1e-320) return 1.0;
+    if (x > 300.0) return 0.0;
+    const double s = x / std::sinh(x);
+    return s * s;
+}
+
+// Mode table. The reference loads an ab-initio HDF5 database
+// (bulk_hdf5.cpp); the GPU engine takes either a seeded synthetic table or
+// a text table:
+//   <material name="m" thickness="1" T0="300" modetable="synthetic" nmodes="3000" seed="1"
+//             [cv="1.6e6"] [sigma_bounce="0.25"]/>
+//   <material name="m" thickness="1" T0="300" modetable="file.modes"/>
+// A .modes file holds "n nq V lattz" then n lines "omega vx vy w"
+// (rad/ps, nm/ps, nm^3, nm, 1/ps).
+struct Material {
+    std::string name;
+    int n = 0, nq = 1;
+    double V = 0, lattz = 0, thickness = 0, T0 = 300.0, sigma_bounce = 0.25;
+    std::vector<double> omega, vx, vy, w;
+    double cv(double T) const {  // J/(K nm^3), calc_cv (bulk_properties.hpp:32-47)
+        double s = 0;
+        for (int i = 0; i < n; i++) s += be_kernel(omega[i], T);
+        return kB * s / (nq * V);
+    }
+};
+
+inline Material read_material(const xml::Node& m) {
+    Material M;
+    M.name = m.str("name", "");
+    M.thickness = m.num("thickness");
+    M.T0 = m.num("T0", 300.0);
+    M.sigma_bounce = m.num("sigma_bounce", 0.25);
+    if (!m.has("modetable")) {
+        throw std::runtime_error("material '" + M.name +
+                                 "': HDF5 databases are not read by the GPU engine; give modetable=\"synthetic\" "
+                                 "or a .modes file");
+    }
+    const std::string mt = m.str("modetable");
+    if (mt == "synthetic") {
+        M.n = (int)m.num("nmodes");
+        const uint64_t seed = (uint64_t)m.num("seed", 1);
+        M.omega.resize(M.n);
+        M.vx.resize(M.n);
+        M.vy.resize(M.n);
+        M.w.resize(M.n);
+        check(bte_synthetic_modes(M.n, seed, m.num("fmin", 0.1), m.num("fmax", 15.0), m.num("vmin", 0.5),
+                                  m.num("vmax", 8.0), m.num("taumin", 1.0), m.num("taumax", 1000.0),
+                                  (int)m.num("mirror", 0), M.omega.data(), M.vx.data(), M.vy.data(),
+                                  M.w.data()),
+              "synthetic modes");
+        // V such that the volumetric heat capacity at T0 is cv [J/(m^3 K)]
+        double s = 0;
+        for (int i = 0; i < M.n; i++) s += be_kernel(M.omega[i], M.T0);
+        M.V = kB * s / (m.num("cv", 1.6e6) * 1e-27);
+        M.lattz = M.thickness;
+    } else {
+        std::ifstream f(mt);
+        if (!f) throw std::runtime_error("cannot open mode table " + mt);
+        f >> M.n >> M.nq >> M.V >> M.lattz;
+        if (!f || M.n <= 0 || M.nq <= 0 || !(M.V > 0) || !(M.lattz > 0))
+            throw std::runtime_error(mt + ": bad header (n nq V lattz)");
+        M.omega.resize(M.n);
+        M.vx.resize(M.n);
+        M.vy.resize(M.n);
+        M.w.resize(M.n);
+        for (int i = 0; i < M.n; i++)
+            if (!(f >> M.omega[i] >> M.vx[i] >> M.vy[i] >> M.w[i]))
+                throw std::runtime_error(mt + ": short mode list");
+    }
+    return M;
+}
+
+// <stem>.eigen.bin (collision_operator.cpp:931-1013): int64 rows, then
+// rows x rows f64 column-major, native byte order; square by assumption.
+inline std::vector<double> load_eigen_bin(const std::string& fn, int64_t& rows) {
+    std::ifstream f(fn, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot open " + fn);
+    f.read(reinterpret_cast<char*>(&rows), sizeof rows);
+    if (!f || rows <= 0 || rows > 100000) throw std::runtime_error(fn + ": bad header");
+    std::vector<double> a((size_t)rows * rows);
+    f.read(reinterpret_cast<char*>(a.data()), (std::streamsize)(a.size() * sizeof(double)));
+    if (!f) throw std::runtime_error(fn + ": file shorter than rows*rows doubles");
+    return a;
+}
+inline void save_eigen_bin(const std::string& fn, int64_t rows, const double* a) {
+    std::ofstream f(fn, std::ios::binary);
+    if (!f) throw std::runtime_error("cannot write " + fn);
+    f.write(reinterpret_cast<const char*>(&rows), sizeof rows);
+    f.write(reinterpret_cast<const char*>(a), (std::streamsize)((size_t)rows * rows * sizeof(double)));
+    if (!f) throw std::runtime_error("write failed: " + fn);
+}
+
+// alma::write_to_csv (include/io_utils.hpp:38-105): default stream
+// formatting, one row per line
+inline void write_csv(const std::string& fn, int rows, int cols, const double* rowmajor) {
+    std::ofstream f(fn);
+    if (!f) throw std::runtime_error("cannot write " + fn);
+    for (int r = 0; r < rows; r++)
+        for (int c = 0; c < cols; c++) {
+            f << rowmajor[(size_t)r * cols + c];
+            if (c == cols - 1) f << std::endl;
+            else f << ',';
+        }
+}
+
+// msgpack encoder for the frame format of beyondRTA_MC2d.cpp:1612-1769
+// (smallest encodings, big-endian payloads, as the msgpack specification
+// and msgpack-c's packer produce them).
+class Pack {
+public:
+    std::string buf;
+    void u8(uint8_t v) { buf.push_back((char)v); }
+    void be(uint64_t v, int nbytes) {
+        for (int i = nbytes - 1; i >= 0; i--) u8((uint8_t)(v >> (8 * i)));
+    }
+    void array(uint32_t n) {
+        if (n < 16) u8(0x90 | n);
+        else if (n < 65536) { u8(0xdc); be(n, 2); }
+        else { u8(0xdd); be(n, 4); }
+    }
+    void f64(double d) {
+        uint64_t b;
+        std::memcpy(&b, &d, 8);
+        u8(0xcb);
+        be(b, 8);
+    }
+    void uint(uint64_t v) {
+        if (v < 128) u8((uint8_t)v);
+        else if (v < 256) { u8(0xcc); be(v, 1); }
+        else if (v < 65536) { u8(0xcd); be(v, 2); }
+        else if (v < 4294967296ULL) { u8(0xce); be(v, 4); }
+        else { u8(0xcf); be(v, 8); }
+    }
+    void sint(int64_t v) {
+        if (v >= 0) { uint((uint64_t)v); return; }
+        if (v >= -32) u8((uint8_t)(int8_t)v);
+        else if (v >= -128) { u8(0xd0); be((uint64_t)(uint8_t)(int8_t)v, 1); }
+        else if (v >= -32768) { u8(0xd1); be((uint64_t)(uint16_t)(int16_t)v, 2); }
+        else if (v >= INT32_MIN) { u8(0xd2); be((uint64_t)(uint32_t)(int32_t)v, 4); }
+        else { u8(0xd3); be((uint64_t)v, 8); }
+    }
+};
+
+// One frame: [time, Eeff, [T per box], then per box (2*nnz, [mode, count]...)
+// or (0, [42.0]) for an empty box]; written as "<size>#<bytes>", the file
+// ends with "0#".
+inline std::string pack_frame(double time, double Eeff, const std::vector<double>& temps, int nbox, int n,
+                              const int32_t* H) {
+    Pack p;
+    p.array(3 + 2 * (uint32_t)nbox);
+    p.f64(time);
+    p.f64(Eeff);
+    p.array((uint32_t)temps.size());
+    for (double t : temps) p.f64(t);
+    for (int b = 0; b < nbox; b++) {
+        const int32_t* h = H + (size_t)b * n;
+        uint64_t nnz = 0;
+        for (int k = 0; k < n; k++) nnz += h[k] != 0;
+        if (nnz == 0) {
+            p.uint(0);
+            p.array(1);
+            p.f64(42.0);
+            continue;
+        }
+        p.uint(2 * nnz);
+        p.array((uint32_t)(2 * nnz));
+        for (int k = 0; k < n; k++)
+            if (h[k] != 0) {
+                p.uint((uint64_t)k);
+                p.sint(h[k]);
+            }
+    }
+    return p.buf;
+}
+
+}  // namespace front

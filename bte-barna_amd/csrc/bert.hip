@@ -1003,6 +1003,17 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     if (p && p->ngamma + p->ncol != m->n) throw Error(BTE_EINVAL, "propagator size does not match the mode table");
     if (!(cfg->dt > 0) || !(cfg->thickness > 0)) throw Error(BTE_EINVAL, "bad beRTA configuration");
     if ((size_t)g->boxes.size() * 4 > BERT_EMIT_MAXB) throw Error(BTE_EINVAL, "more than 1024 boxes");
+    // gradient sources (calculate_gradientT, geometry_2d.cpp:524-576) vanish
+    // only when every regular box has the same Teq
+    {
+        double t0 = -1;
+        for (auto& bx : g->boxes) {
+            if (bx.kind != 0) continue;
+            if (t0 < 0) t0 = bx.Teq;
+            else if (bx.Teq != t0)
+                throw Error(BTE_EINVAL, "regular boxes with different Teq need gradient sources, which are not supported");
+        }
+    }
     HIPCHECK(hipSetDevice(c->device));
     auto* b = new Bert();
     std::unique_ptr<Bert> guard(b);

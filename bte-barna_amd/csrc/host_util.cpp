@@ -408,4 +408,24 @@ int bte_geom_contacts(const bte_geom* gg, int maxc, int32_t* ncont, int32_t* nbr
     BTE_GUARD_END
 }
 
+int bte_krylov_size(int N, const double* B, double dt, double eps, int* m) {
+    BTE_GUARD_BEGIN
+    if (N < 2 || !B || !m || !(eps > 0)) throw Error(BTE_EINVAL, "bad arguments to bte_krylov_size");
+    long double s = 0;
+    for (size_t i = 0; i < (size_t)N * N; i++) {
+        const long double v = (long double)dt * B[i];
+        s += v * v;
+    }
+    const long double nrm = std::sqrt(s);
+    const long double NN = (long double)N * N;
+    int k = 2;
+    for (; k < 100000; k++) {
+        // log of 2 nrm^k e^nrm / Gamma(k), compared in logs to avoid overflow
+        const long double lg = std::log(2.0L) + k * std::log(nrm) + nrm - std::lgamma((long double)k);
+        if (nrm == 0 || lg - std::log(NN) < std::log((long double)eps)) break;
+    }
+    *m = k;
+    BTE_GUARD_END
+}
+
 }  // extern "C"

@@ -180,6 +180,11 @@ typedef struct {
 int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt, int m, int precision,
                      int arnoldi_mode, double* P, bte_krylov_report* rep);
 
+/* Krylov size rule of build_P (collision_operator.cpp:1020-1030 with
+ * error_bound, collision_operator.hpp:216-219): the smallest m >= 2 with
+ * 2 |dt B|_F^m e^{|dt B|_F} / Gamma(m) / N^2 < eps. Host only. */
+int bte_krylov_size(int N, const double* B, double dt, double eps, int* m);
+
 #ifdef __cplusplus
 }
 #endif

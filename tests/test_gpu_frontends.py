@@ -1,0 +1,133 @@
+"""The C++ front-ends on the GPU: PropagatorBuilder against the oracle's
+Krylov build, beRTAMC2D stdout/frames against each other and against the
+library run with the same seed, RTAMC2D CSVs against the library run."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+GEOM = """<Geometry>
+  <number_of_boxes Ngeom="{n}"/>
+{boxes}
+</Geometry>"""
+BOX = """  <Box>
+    <BoxID id="{i}"/>
+    <MaterialID name="m"/>
+    <Vertices dim="2" npoints="4"> {x0} {y0} {x0} {y1} {x1} {y1} {x1} {y0} </Vertices>
+    <initCnd Teq="{teq}" Tinit="{tin}"/>{extra}
+  </Box>"""
+
+
+def write_ribbon(path, rect, kind, teq, tin):
+    boxes = [BOX.format(i=i, x0=r[0], x1=r[1], y0=r[2], y1=r[3], teq=teq[i], tin=tin[i],
+                        extra="\n    <Reservoir/>" if kind[i] == 1 else "") for i, r in enumerate(rect)]
+    path.write_text(GEOM.format(n=len(rect), boxes="\n".join(boxes)))
+
+
+def run(cmd, cwd):
+    print("front-end:", " ".join(str(c) for c in cmd), flush=True)
+    r = subprocess.run([str(c) for c in cmd], cwd=cwd, capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def eigen_bin(path):
+    b = path.read_bytes()
+    n = int(np.frombuffer(b[:8], np.int64)[0])
+    return np.frombuffer(b[8:], np.float64).reshape(n, n, order="F")
+
+
+def test_propagator_builder_matches_oracle(pkg, tmp_path):
+    run([pkg.app_path("PropagatorBuilder"), "synthetic:96:2", "300", "5", "1", "20", "fp64", "reference"], tmp_path)
+    B = eigen_bin(tmp_path / "synthetic_96_2_300K_5ps.B.eigen.bin")
+    P = eigen_bin(tmp_path / "synthetic_96_2_300K_5ps.P.eigen.bin")
+    mat, B2 = pkg.workloads.config1_inputs(nmodes=96, seed=2)
+    # same operator up to libm-vs-numpy rounding of the Bose-Einstein kernel
+    assert np.allclose(B, B2, rtol=1e-12, atol=1e-15 * np.abs(B2).max())
+    Po = orc.build_P(B, 5.0, 20, 0)
+    assert np.abs(P - Po).max() < 1e-10
+    assert np.abs(P.sum(0) - 1).max() < 1e-12
+
+
+def test_bertamc2d_outputs(pkg, ctx, tmp_path):
+    msgpack = pytest.importorskip("msgpack")
+    W = pkg.workloads
+    run([pkg.app_path("PropagatorBuilder"), "synthetic:96:2", "300", "5", "1", "20"], tmp_path)
+    rect, kind, teq, tin = W.ribbon(60.0, 30.0, 6, 300.0, Tinit_first=301.0)
+    write_ribbon(tmp_path / "g.xml", rect, kind, teq, tin)
+    (tmp_path / "in.xml").write_text(
+        '<beRTAMC2D>\n<geometry file="g.xml"/>\n<time dt="5" maxtime="45"/>\n'
+        '<material name="m" modetable="synthetic" nmodes="96" seed="2" thickness="1" T0="300">\n'
+        '  <propagator T="300" file="synthetic_96_2_300K_5ps.P.eigen.bin"/>\n</material>\n'
+        '<Eeff particles="5000" Tmax="301" Tmin="300"/>\n<seed value="7"/>\n</beRTAMC2D>\n')
+    r = run([pkg.app_path("beRTAMC2D"), "in.xml", "4"], tmp_path)
+    lines = [ln.split("\t") for ln in r.stdout.splitlines() if ln and not ln.startswith("#")]
+    nsteps = 10
+    assert len(lines) == nsteps + 1
+    rows = np.array([[float(x) for x in ln if x != ""] for ln in lines])
+    assert rows.shape == (nsteps + 1, 3 + 3 * 6)
+    assert np.allclose(rows[:, 1], 5.0 * np.arange(nsteps + 1))
+    # frames: one per step, time (k+1) dt, Teq per box, histogram per box
+    raw = (tmp_path / "properties.msgpack.bin").read_bytes()
+    frames, pos = [], 0
+    while True:
+        k = raw.index(b"#", pos)
+        size = int(raw[pos:k])
+        if size == 0:
+            break
+        frames.append(msgpack.unpackb(raw[k + 1:k + 1 + size]))
+        pos = k + 1 + size
+    assert len(frames) == nsteps
+    mat = W.material(96, seed=2)
+    vx = mat["vx"]
+    vol = (rect[:, 1] - rect[:, 0]) * (rect[:, 3] - rect[:, 2]) * 1.0
+    for s, f in enumerate(frames):
+        assert f[0] == pytest.approx(5.0 * (s + 1)) and f[2] == list(teq)
+        Eeff = f[1]
+        for b in range(6):
+            n2, pairs = f[3 + 2 * b], f[4 + 2 * b]
+            h = np.zeros(96)
+            if n2:
+                h[np.array(pairs[0::2])] = pairs[1::2]
+            ed = Eeff / vol[b] * 1e27 * h.sum()
+            jx = Eeff / vol[b] * 1e30 * (h * vx).sum()
+            assert rows[s + 1, 3 + 3 * b] == pytest.approx(ed, rel=2e-5, abs=1e-300)
+            assert rows[s + 1, 4 + 3 * b] == pytest.approx(jx, rel=2e-5, abs=1e-6 * abs(jx) + 1e-300)
+    # the library with the same inputs and seed gives the same particle counts
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq, tin)
+    P = eigen_bin(tmp_path / "synthetic_96_2_300K_5ps.P.eigen.bin")
+    prop = pkg.Propagator(ctx, P)
+    o = pkg.bertamc2d(ctx, modes, geom, prop, thickness=1.0, dt=5.0, maxtime=45.0, Efactor=5000, Tmax=301.0,
+                      Tmin=300.0, seed=7)
+    assert np.array_equal(o["npart"], rows[:, 2].astype(np.int64))
+    assert np.allclose(o["props"][:, :, 0], rows[:, 3::3], rtol=2e-5, atol=0)
+
+
+def test_rtamc2d_csv(pkg, ctx, tmp_path):
+    W = pkg.workloads
+    rect, kind, teq, _ = W.ribbon(60.0, 30.0, 6, 300.0, reservoirs=(300.5, 299.5))
+    write_ribbon(tmp_path / "g.xml", rect, kind, teq, teq)
+    (tmp_path / "in.xml").write_text(
+        '<RTA_MC2d>\n<geometry file="g.xml"/>\n<time dt="20" maxtime="200"/>\n'
+        '<material name="m" modetable="synthetic" nmodes="400" seed="2" thickness="1" T0="300" sigma_bounce="2.0"/>\n'
+        '<particles N="20000" seed="5"/>\n</RTA_MC2d>\n')
+    run([pkg.app_path("RTAMC2D"), "in.xml", "8", "2"], tmp_path)
+    mat = W.material(400, seed=2)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    for irun in range(2):
+        T = np.loadtxt(tmp_path / f"temperature_300K_run_{irun}.csv", delimiter=",")
+        jx = np.loadtxt(tmp_path / f"jx_300K_run_{irun}.csv", delimiter=",")
+        assert T.shape == (10, 1 + 8) and jx.shape == T.shape
+        assert np.allclose(T[:, 0], (np.arange(10) + 0.5) * 20.0)
+        o = pkg.rtamc2d(ctx, modes, geom, thickness=1.0, T0=300.0, dt=20.0, maxtime=200.0, nparticles=20000,
+                        seed=5 + irun, sigma_bounce=2.0)
+        assert np.allclose(T[:, 1:], o["T"], rtol=1e-5, atol=1e-9)
+        assert np.allclose(jx[:, 1:], o["jx"], rtol=1e-5, atol=1e-6 * np.abs(o["jx"]).max())
