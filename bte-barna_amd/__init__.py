@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from ._abi import (BertCfg, BertReport, BteError, FRAME_CB, KrylovReport, PREC, ARNOLDI, RtaCfg,
-                   RtaReport, check, lib, LIBPATH)
+                   RtaReport, RtaRunDesc, RtaBatchReport, check, lib, LIBPATH)
 from . import workloads  # noqa: F401
 
 __all__ = ["Context", "ModeTable", "Geometry", "Propagator", "rtamc2d", "bertamc2d", "build_P",
@@ -168,6 +168,61 @@ def rtamc2d(ctx, modes, geom, **kw):
         return r.execute()
     finally:
         r.close()
+
+
+class RtaBatch(_Handle):
+    """Independent RTAMC2D runs of one material in one persistent launch.
+
+    runs: list of dicts with geom, T0, nparticles, seed and optionally
+    ballistic, wscale (scattering-rate factor), specularity (4 sides: left,
+    bottom, top, right) and grad (dT/dp for sides with 0 < p < 1)."""
+    _destroy = "bte_rta_batch_destroy"
+
+    def __init__(self, ctx, modes, runs, thickness, dt, maxtime, sigma_bounce=0.25, use_graph=False):
+        super().__init__()
+        self.nruns = len(runs)
+        self.descs = (RtaRunDesc * self.nruns)()
+        for d, r in zip(self.descs, runs):
+            d.geom = r["geom"].h
+            d.T0 = r["T0"]
+            d.nparticles = int(r["nparticles"])
+            d.seed = int(r.get("seed", 1))
+            d.ballistic = int(r.get("ballistic", False))
+            d.wscale = float(r.get("wscale", 1.0))
+            sp = r.get("specularity", (0.0, 0.0, 0.0, 0.0))
+            for k in range(4):
+                d.specularity[k] = float(sp[k])
+            d.grad = int(r.get("grad", False))
+        self.nbox = [r["geom"].nbox for r in runs]
+        self.ntime = int(np.floor(maxtime / dt + 1e-6)) + 1
+        self._keep = (ctx, modes, [r["geom"] for r in runs])
+        check(lib().bte_rta_batch_create(ctx.h, modes.h, self.nruns, self.descs, thickness, dt, maxtime,
+                                         sigma_bounce, int(use_graph), C.byref(self.h)))
+
+    def tune(self, fraction=0.05):
+        blk, nb = C.c_int(0), C.c_int(0)
+        check(lib().bte_rta_batch_tune(self.h, fraction, C.byref(blk), C.byref(nb)))
+        return blk.value, nb.value
+
+    def execute(self, grad=False):
+        nt = self.ntime - 1
+        tot = nt * sum(self.nbox)
+        T, jx, jy = np.zeros(tot), np.zeros(tot), np.zeros(tot)
+        dT = np.zeros(4 * tot) if grad else None
+        E = np.zeros(self.nruns)
+        rep = RtaBatchReport()
+        check(lib().bte_rta_batch_execute(self.h, _dp(T), _dp(jx), _dp(jy), _dp(dT) if grad else None, _dp(E),
+                                          C.byref(rep)))
+        out, o = [], 0
+        for nb in self.nbox:
+            sl = slice(o, o + nt * nb)
+            r = {"T": T[sl].reshape(nt, nb), "jx": jx[sl].reshape(nt, nb), "jy": jy[sl].reshape(nt, nb)}
+            if grad:
+                r["dTdp"] = dT[4 * o:4 * (o + nt * nb)].reshape(4, nt, nb)
+            out.append(r)
+            o += nt * nb
+        return out, {"events": rep.events, "crossings": rep.crossings, "kernel_ms": rep.kernel_ms,
+                     "block": rep.block, "blocks": rep.blocks, "Eeff": E}
 
 
 class BertRun(_Handle):

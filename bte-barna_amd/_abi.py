@@ -32,6 +32,17 @@ class RtaReport(C.Structure):
                 ("kernel_ms", C.c_double), ("ntime", C.c_int)]
 
 
+class RtaRunDesc(C.Structure):
+    _fields_ = [("geom", C.c_void_p), ("T0", C.c_double), ("nparticles", C.c_uint64), ("seed", C.c_uint64),
+                ("ballistic", C.c_int), ("wscale", C.c_double), ("specularity", C.c_double * 4),
+                ("grad", C.c_int)]
+
+
+class RtaBatchReport(C.Structure):
+    _fields_ = [("events", C.c_uint64), ("crossings", C.c_uint64), ("kernel_ms", C.c_double),
+                ("ntime", C.c_int), ("block", C.c_int), ("blocks", C.c_int)]
+
+
 class BertCfg(C.Structure):
     _fields_ = [("thickness", C.c_double), ("dt", C.c_double), ("maxtime", C.c_double),
                 ("max_steps", C.c_int), ("Ed", C.c_double), ("Efactor", C.c_uint64),
@@ -80,6 +91,11 @@ SIGNATURES = {
     "bte_rta_execute": (C.c_int, [vp, dp, dp, dp, C.POINTER(RtaReport)]),
     "bte_rta_destroy": (C.c_int, [vp]),
     "bte_rta_run": (C.c_int, [vp, vp, vp, C.POINTER(RtaCfg), dp, dp, dp, C.POINTER(RtaReport)]),
+    "bte_rta_batch_create": (C.c_int, [vp, vp, C.c_int, C.POINTER(RtaRunDesc), C.c_double, C.c_double,
+                                       C.c_double, C.c_double, C.c_int, pvp]),
+    "bte_rta_batch_tune": (C.c_int, [vp, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "bte_rta_batch_execute": (C.c_int, [vp, dp, dp, dp, dp, dp, C.POINTER(RtaBatchReport)]),
+    "bte_rta_batch_destroy": (C.c_int, [vp]),
     "bte_prop_create": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, pvp]),
     "bte_prop_destroy": (C.c_int, [vp]),
     "bte_bert_create": (C.c_int, [vp, vp, vp, vp, C.POINTER(BertCfg), pvp]),

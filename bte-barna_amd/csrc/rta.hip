@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <tuple>
 
 namespace bte {
 
@@ -21,29 +22,42 @@ struct RtaGen {
     int32_t idi, sign, pad0, pad1;
 };
 
+// One run of a batch: its particle-id range, seed, tables and options. All
+// box, generator and table indices are absolute in the batch's arrays.
+struct RtaRunDev {
+    uint64_t first, count, seed;
+    int32_t gen0, ngen, gsel0, scat0;
+    int32_t ballistic, gmask, pad0, pad1;  // gmask: sides whose specularity gets a gradient
+    double wscale;                         // scattering-rate factor of the run
+    float spec[4];                         // specular probability per side
+};
+
 struct RtaParams {
     const double4* modes;  // (vx, vy, w, omega)
     const double2* ivel;   // (1/vx, 1/vy), 0 where the component is 0
-    const uint2* scat;     // BE_derivative alias (n)
+    const int32_t* mirror; // [2][n]: partner with vx -> -vx (sides 0, 3), vy -> -vy (sides 1, 2)
+    const uint2* scat;     // BE_derivative alias per run (n each)
     const DevBox* boxes;
     const DevContact* cts;
     const RtaGen* gens;
-    const uint2* gen_sel;  // alias over generators (ngen)
-    const uint2* gen_tab;  // ngen x n
+    const uint2* gen_sel;  // alias over each run's generators
+    const uint2* gen_tab;  // per generator, n entries
     const int4* lam_info;  // per mode: lo, len, base, 0
     const uint2* lam_tab;
     const int32_t* perm;   // omega-sorted position -> mode
     const double* timegrid;
+    const uint64_t* box_cell;  // first tally cell of each box (box-major, nt cells)
+    const RtaRunDev* runs;
     double* gz;
     double* jx;
     double* jy;
-    unsigned long long* counters;  // events, crossings
-    unsigned long long* next_pid;
-    uint64_t nparticles;
-    uint64_t seed;
+    double* dgz;           // [4][cells] score-weighted gz per side (gradient runs)
+    unsigned long long* counters;  // events, crossings, next id, stuck, bad mirror
+    uint64_t total;        // particles over all runs
+    uint64_t cells;
     unsigned long long max_iters;
     double maxtime;
-    int32_t n, nbox, ngen, nt, ballistic;
+    int32_t n, nt, nruns, pad;
 };
 
 __device__ inline int time_bin(const double* grid, int nt, double target) {
@@ -57,32 +71,54 @@ __device__ inline int time_bin(const double* grid, int nt, double target) {
     return b < 0 ? 0 : (b > nt - 1 ? nt - 1 : b);
 }
 
-// process_segment (RTA_MC2D.cpp:699-769)
-__device__ inline void tally_segment(const RtaParams& P, int box, double t0, double t1, int sign,
-                                     double vx, double vy) {
+// process_segment (RTA_MC2D.cpp:699-769); gradient runs also add the
+// segment weighted by the particle's running score for each active side
+template <bool GRAD>
+__device__ inline void tally_segment(const RtaParams& P, int box, double t0, double t1, int sign, double vx,
+                                     double vy, int gmask, const float* score) {
     int b0 = time_bin(P.timegrid, P.nt, t0), b1 = time_bin(P.timegrid, P.nt, t1);
     double delta = t1 - t0;
+    const uint64_t base = P.box_cell[box];
     for (int i = b0; i <= b1; i++) {
         double lo = fmax(P.timegrid[i] - t0, 0.);
         double hi = fmin(P.timegrid[i + 1] - t0, delta);
         double c = sign * fmax(hi - lo, 0.);
         if (c == 0.) continue;
-        size_t k = (size_t)box * P.nt + i;
+        const uint64_t k = base + i;
         atomicAdd(P.gz + k, c);
         atomicAdd(P.jx + k, vx * c);
         atomicAdd(P.jy + k, vy * c);
+        if (GRAD) {
+            for (int s = 0; s < 4; s++)
+                if (((gmask >> s) & 1) && score[s] != 0.f) atomicAdd(P.dgz + (size_t)s * P.cells + k, c * (double)score[s]);
+        }
     }
 }
 
-__global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
+// the run holding global particle id pid (first ids ascending)
+__device__ inline int find_run(const RtaParams& P, uint64_t pid) {
+    int lo = 0, hi = P.nruns - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (P.runs[mid].first <= pid) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One lane follows one particle history at a time and takes the next id
+// from a per-wave pool when it ends; ids of all runs form one range, so a
+// batch of runs of any sizes fills the GPU in one launch.
+template <bool GRAD>
+__global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
     Pcg32 rng;
-    double x = 0, y = 0, t = 0, dtrem = 0;
-    int mode = 0, box = 0, sign = 1;
+    double x = 0, y = 0, t = 0, dtrem = 0, wscale = 1.0;
+    int mode = 0, box = 0, sign = 1, run = 0;
     bool alive = false, need_flight = true;
-    unsigned long long events = 0, crossings = 0, stuck = 0;
+    unsigned long long events = 0, crossings = 0, stuck = 0, badmirror = 0;
     unsigned long long budget = 0;
     int still = 0;        // consecutive events without time advance
     double tlast = -1.0;
+    float score[4] = {0.f, 0.f, 0.f, 0.f};
     const double maxtime = P.maxtime;
     // particle ids come from a per-wave pool refilled 64 at a time, so the
     // global id counter sees one atomic per 64 histories
@@ -105,7 +141,7 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
             if (pool_lo == pool_hi) {
                 unsigned long long base = 0;
                 const int leader = __ffsll((long long)want) - 1;
-                if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(P.next_pid, 64ULL);
+                if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(P.counters + 2, 64ULL);
                 base = __shfl(base, leader, 64);
                 pool_lo = base;
                 pool_hi = base + 64;
@@ -122,11 +158,15 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
             budget = 0;
             still = 0;
             tlast = -1.0;
-            if (pid >= P.nparticles) break;
-            rng.seed(P.seed, pid);
+            if (pid >= P.total) break;
+            run = find_run(P, pid);
+            const RtaRunDev& R = P.runs[run];
+            rng.seed(R.seed, pid - R.first);
+            wscale = R.wscale;
+            if (GRAD) score[0] = score[1] = score[2] = score[3] = 0.f;
             // Init_generator (RTA_MC2D.cpp:436-484)
             bool ng;
-            int gi = (int)alias_pick(P.gen_sel, (uint32_t)P.ngen, rng, ng);
+            int gi = R.gen0 + (int)alias_pick(P.gen_sel + R.gsel0, (uint32_t)R.ngen, rng, ng);
             const RtaGen g = P.gens[gi];
             double u = rng.uniform();
             x = g.p1x + u * g.ndx;
@@ -150,7 +190,7 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
             // RTA_MC2D.cpp:2262-2288: exponential flight, truncated at maxtime
             if (!(t < maxtime)) { alive = false; continue; }
             double u = 1.0 - rng.uniform();
-            dtrem = -log(u) / md.z;
+            dtrem = -log(u) / (md.z * wscale);
             if (t + dtrem > maxtime) dtrem = maxtime - t + 1.0e-8;
             if (almost_equal(dtrem, 0.)) continue;
             need_flight = false;
@@ -161,16 +201,17 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
             alive = false;
             continue;
         }
+        const int gmask = GRAD ? P.runs[run].gmask : 0;
         const double vx = md.x, vy = md.y;
         double nx = x + dtrem * vx, ny = y + dtrem * vy;
         if (box_inside(B, nx, ny)) {
-            tally_segment(P, box, t, t + dtrem, sign, vx, vy);
+            tally_segment<GRAD>(P, box, t, t + dtrem, sign, vx, vy, gmask, score);
             x = nx;
             y = ny;
             t += dtrem;
-            if (!P.ballistic) {
+            if (!P.runs[run].ballistic) {
                 bool ng;
-                mode = (int)alias_pick(P.scat, (uint32_t)P.n, rng, ng);
+                mode = (int)alias_pick(P.scat + P.runs[run].scat0, (uint32_t)P.n, rng, ng);
             }
             events++;
             need_flight = true;
@@ -182,7 +223,7 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
         double tf = box_exit(B, x, y, vx, vy, iv.x, iv.y, mask, ex, ey);
         double left = dtrem - tf;
         if (almost_equal(left, 0.)) left = 0.;
-        tally_segment(P, box, t, t + tf, sign, vx, vy);
+        tally_segment<GRAD>(P, box, t, t + tf, sign, vx, vy, gmask, score);
         t += tf;
         int side = 0;
         const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
@@ -195,14 +236,40 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
             npb = fast == SIDE_VOID ? 0 : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
         }
         if (npb == 0) {
-            // void side: Lambert re-emission, remaining flight discarded (RTA_MC2D.cpp:1259)
             int s = fast == SIDE_VOID ? side
                   : (__popc(mask) == 1 ? __ffs(mask) - 1
                                        : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next()));
-            const int4 li = P.lam_info[mode];
-            bool ng;
-            int k = (int)alias_pick(P.lam_tab + li.z + (size_t)s * li.y, (uint32_t)li.y, rng, ng);
-            mode = P.perm[li.x + k];
+            // specular with probability p (mirror partner of the mode), else
+            // diffuse; a draw is made only when 0 < p < 1
+            const float p = P.runs[run].spec[s];
+            bool specular = p >= 1.f;
+            if (p > 0.f && p < 1.f) {
+                specular = rng.uniform() < (double)p;
+                if (GRAD && ((gmask >> s) & 1)) {
+                    // d/dp log P(outcome): 1/p for specular, -1/(1-p) for diffuse
+                    const float d = specular ? 1.f / p : -1.f / (1.f - p);
+                    if (s == 0) score[0] += d;
+                    else if (s == 1) score[1] += d;
+                    else if (s == 2) score[2] += d;
+                    else score[3] += d;
+                }
+            }
+            if (specular) {
+                const int partner = P.mirror[(s == 0 || s == 3 ? 0 : (size_t)P.n) + mode];
+                if (partner < 0) {
+                    badmirror++;
+                    alive = false;
+                    continue;
+                }
+                mode = partner;
+            } else {
+                // Lambert re-emission (RTA_MC2D.cpp:635-678)
+                const int4 li = P.lam_info[mode];
+                bool ng;
+                int k = (int)alias_pick(P.lam_tab + li.z + (size_t)s * li.y, (uint32_t)li.y, rng, ng);
+                mode = P.perm[li.x + k];
+            }
+            // the remaining flight is discarded (RTA_MC2D.cpp:1259)
             x = ex;
             y = ey;
             events++;
@@ -238,27 +305,51 @@ __global__ __launch_bounds__(256) void rta_history_kernel(RtaParams P) {
     atomicAdd(P.counters + 0, events);
     atomicAdd(P.counters + 1, crossings);
     if (stuck) atomicAdd(P.counters + 3, stuck);
+    if (badmirror) atomicAdd(P.counters + 4, badmirror);
 }
 
-struct Rta {
+// clears the tallies and counters (a kernel rather than a memset so that the
+// launch sequence can be captured in a graph)
+__global__ __launch_bounds__(256) void rta_clear_kernel(double* a, uint64_t n, unsigned long long* cnt) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        a[i] = 0.;
+    if (blockIdx.x == 0 && threadIdx.x < 8) cnt[threadIdx.x] = 0ULL;
+}
+
+struct RtaBatch {
     Ctx* ctx = nullptr;
     const Modes* m = nullptr;
-    const Geom* g = nullptr;
-    bte_rta_cfg cfg{};
-    int ntime = 0;
-    double maxtime = 0, Eeff = 0, cv = 0, einfo1 = 0;
+    std::vector<bte_rta_run_desc> desc;
+    std::vector<const Geom*> geoms;
+    double thickness = 1, dt = 1, maxtime = 1, sigma = 0.25;
+    int ntime = 0, nruns = 0;
     std::vector<double> timegrid;
+    std::vector<double> Eeff, cv;
+    std::vector<int> box0, nbox;
+    std::vector<uint64_t> cell0;   // first tally cell of each run
+    uint64_t cells = 0, total = 0;
+    bool any_grad = false;
+    std::vector<RtaRunDev> runs;
     DevBuf<double4> d_modes;
     DevBuf<double2> d_ivel;
+    DevBuf<int32_t> d_mirror, d_perm;
     DevBuf<uint2> d_scat, d_gen_sel, d_gen_tab, d_lam_tab;
     DevBuf<RtaGen> d_gens;
     DevBuf<int4> d_lam_info;
-    DevBuf<int32_t> d_perm;
-    DevBuf<double> d_timegrid, d_tally;
+    DevBuf<DevBox> d_boxes;
+    DevBuf<DevContact> d_cts;
+    DevBuf<double> d_timegrid, d_tally, d_dgz;
+    DevBuf<uint64_t> d_box_cell;
+    DevBuf<RtaRunDev> d_runs, d_trial_runs;
     DevBuf<unsigned long long> d_counters;
-    int ngen = 0;
+    int block = 256, blocks = 0;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    ~Rta() {
+    ~RtaBatch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
     }
@@ -327,197 +418,432 @@ static void build_lambert(const Modes& m, double sigma, std::vector<int4>& info,
     }
 }
 
+
+// mirror partners: same omega and |v| components with one velocity
+// component negated (exact float match), -1 where there is none
+static void build_mirrors(const Modes& m, std::vector<int32_t>& mir) {
+    const int n = m.n;
+    mir.assign(2 * (size_t)n, -1);
+    std::map<std::tuple<double, double, double>, int> idx;
+    for (int i = 0; i < n; i++) idx.emplace(std::make_tuple(m.omega[i], m.vx[i], m.vy[i]), i);
+    for (int i = 0; i < n; i++) {
+        auto a = idx.find(std::make_tuple(m.omega[i], -m.vx[i], m.vy[i]));
+        auto b = idx.find(std::make_tuple(m.omega[i], m.vx[i], -m.vy[i]));
+        if (a != idx.end()) mir[i] = a->second;
+        if (b != idx.end()) mir[(size_t)n + i] = b->second;
+    }
+}
+
+static RtaParams batch_params(RtaBatch* b, const RtaRunDev* runs, uint64_t total) {
+    RtaParams P{};
+    P.modes = b->d_modes.p;
+    P.ivel = b->d_ivel.p;
+    P.mirror = b->d_mirror.p;
+    P.scat = b->d_scat.p;
+    P.boxes = b->d_boxes.p;
+    P.cts = b->d_cts.p;
+    P.gens = b->d_gens.p;
+    P.gen_sel = b->d_gen_sel.p;
+    P.gen_tab = b->d_gen_tab.p;
+    P.lam_info = b->d_lam_info.p;
+    P.lam_tab = b->d_lam_tab.p;
+    P.perm = b->d_perm.p;
+    P.timegrid = b->d_timegrid.p;
+    P.box_cell = b->d_box_cell.p;
+    P.runs = runs;
+    P.gz = b->d_tally.p;
+    P.jx = b->d_tally.p + b->cells;
+    P.jy = b->d_tally.p + 2 * b->cells;
+    P.dgz = b->any_grad ? b->d_dgz.p : nullptr;
+    P.counters = b->d_counters.p;
+    P.total = total;
+    P.cells = b->cells;
+    P.max_iters = 1ULL << 26;
+    P.maxtime = b->maxtime;
+    P.n = b->m->n;
+    P.nt = b->ntime - 1;
+    P.nruns = b->nruns;
+    return P;
+}
+
+static void launch_batch(RtaBatch* b, const RtaParams& P, int block, int blocks, hipStream_t s) {
+    const uint64_t nclear = (b->any_grad ? 7 : 3) * b->cells;
+    // the tally and gradient arrays are contiguous (d_tally then d_dgz is
+    // cleared separately when present)
+    hipLaunchKernelGGL(rta_clear_kernel, dim3(256), dim3(256), 0, s, b->d_tally.p, 3 * b->cells, b->d_counters.p);
+    if (b->any_grad)
+        hipLaunchKernelGGL(rta_clear_kernel, dim3(256), dim3(256), 0, s, b->d_dgz.p, nclear - 3 * b->cells,
+                           b->d_counters.p);
+    if (b->any_grad)
+        hipLaunchKernelGGL(rta_history_kernel<true>, dim3(blocks), dim3(block), 0, s, P);
+    else
+        hipLaunchKernelGGL(rta_history_kernel<false>, dim3(blocks), dim3(block), 0, s, P);
+}
+
+static void batch_create(RtaBatch* b) {
+    Ctx* c = b->ctx;
+    const Modes* m = b->m;
+    const int n = m->n;
+    hipStream_t s = c->stream;
+    if (!(b->dt > 0) || !(b->maxtime > 0) || !(b->thickness > 0)) throw Error(BTE_EINVAL, "bad RTA configuration");
+    for (int i = 0; i < n; i++)
+        if (m->w[i] == 0.) throw Error(BTE_EINVAL, "invalid scattering rate");
+    // time grid (RTA_MC2D.cpp:2121-2124)
+    b->ntime = (int)std::floor(b->maxtime / b->dt + 1.0e-6) + 1;
+    if (b->ntime < 2) throw Error(BTE_EINVAL, "maxtime shorter than dt");
+    b->maxtime = (b->ntime - 1) * b->dt;
+    b->timegrid.resize(b->ntime);
+    for (int i = 0; i < b->ntime; i++) b->timegrid[i] = (double)i * b->maxtime / (b->ntime - 1);
+    const int nt = b->ntime - 1;
+    std::vector<double4> md(n);
+    for (int i = 0; i < n; i++) md[i] = make_double4(m->vx[i], m->vy[i], m->w[i], m->omega[i]);
+    b->d_modes.upload(md, s);
+    std::vector<double2> iv(n);
+    for (int i = 0; i < n; i++)
+        iv[i] = make_double2(m->vx[i] != 0. ? 1.0 / m->vx[i] : 0., m->vy[i] != 0. ? 1.0 / m->vy[i] : 0.);
+    b->d_ivel.upload(iv, s);
+    std::vector<int32_t> mir;
+    build_mirrors(*m, mir);
+    b->d_mirror.upload(mir, s);
+    const double vuc = m->V * b->thickness / m->lattz;
+    std::vector<DevBox> boxes;
+    std::vector<DevContact> cts;
+    std::vector<uint64_t> box_cell;
+    std::vector<RtaGen> gens;
+    std::vector<uint2> gsel, gtab, scat;
+    std::vector<double> pmf(n);
+    std::vector<uint32_t> th(n), al(n);
+    b->nruns = (int)b->desc.size();
+    b->total = 0;
+    b->cells = 0;
+    for (int r = 0; r < b->nruns; r++) {
+        const bte_rta_run_desc& D = b->desc[r];
+        const Geom* g = b->geoms[r];
+        if (!g || D.nparticles == 0) throw Error(BTE_EINVAL, "run " + std::to_string(r) + ": no geometry or particles");
+        if (!(D.wscale > 0)) throw Error(BTE_EINVAL, "run " + std::to_string(r) + ": wscale must be positive");
+        const int nb = (int)g->boxes.size();
+        for (auto& bx : g->boxes)
+            if (bx.kind == 0) {
+                if (bx.Teq != D.T0) throw Error(BTE_EINVAL, "Tref in xml is different from provided T0");
+                if (bx.Tinit != bx.Teq) throw Error(BTE_EINVAL, "Initial temperature out of reference is not allowed");
+            }
+        RtaRunDev R{};
+        R.first = b->total;
+        R.count = D.nparticles;
+        R.seed = D.seed;
+        R.ballistic = D.ballistic ? 1 : 0;
+        R.wscale = D.wscale;
+        for (int k = 0; k < 4; k++) {
+            if (!(D.specularity[k] >= 0. && D.specularity[k] <= 1.))
+                throw Error(BTE_EINVAL, "specularity must lie in [0, 1]");
+            R.spec[k] = (float)D.specularity[k];
+            if (D.grad && D.specularity[k] > 0. && D.specularity[k] < 1.) R.gmask |= 1 << k;
+        }
+        if (R.gmask) b->any_grad = true;
+        const int bo = (int)boxes.size(), co = (int)cts.size();
+        b->box0.push_back(bo);
+        b->nbox.push_back(nb);
+        b->cell0.push_back(b->cells);
+        for (int i = 0; i < nb; i++) {
+            DevBox d = g->dboxes[i];
+            d.c0 += co;
+            d.c1 += co;
+            for (int k = 0; k < 4; k++)
+                if (d.side_nbr[k] >= 0) d.side_nbr[k] += bo;
+            boxes.push_back(d);
+            box_cell.push_back(b->cells + (uint64_t)i * nt);
+        }
+        for (auto ct : g->dcontacts) {
+            ct.box += bo;
+            cts.push_back(ct);
+        }
+        b->cells += (uint64_t)nb * nt;
+        // BE_derivative sampler at the run temperature (sampling.cpp:47-68)
+        for (int i = 0; i < n; i++) pmf[i] = m->omega[i] <= 0. ? 0. : h_be_kernel(m->omega[i], D.T0) * m->w[i];
+        if (!build_alias(pmf.data(), n, th.data(), al.data())) throw Error(BTE_EINVAL, "empty scattering pmf");
+        R.scat0 = (int32_t)scat.size();
+        for (int i = 0; i < n; i++) scat.push_back(make_uint2(th[i], al[i]));
+        // isothermal wall generators (RTA_MC2D.cpp:296-347, 366-397)
+        std::vector<double> gw;
+        R.gen0 = (int32_t)gens.size();
+        for (int sid = 0; sid < nb; sid++) {
+            const RectBox& sb = g->boxes[sid];
+            if (sb.kind != 1) continue;
+            for (auto& hc : g->contacts[sid]) {
+                const RectBox& cb = g->boxes[hc.box];
+                if (cb.kind != 0) continue;
+                if (almost_equal(hc.length(), 0.)) continue;
+                if (almost_equal(sb.Teq, cb.Teq)) continue;
+                double cum = wall_pmf(*m, sb.Teq, cb.Teq, hc.npx, hc.npy, pmf.data());
+                int sg = (sb.Teq - cb.Teq) >= 0 ? 1 : -1;
+                double jd = sg * cum / (m->nq * vuc);
+                gw.push_back(std::fabs(jd) * hc.length() * b->thickness);
+                if (!build_alias(pmf.data(), n, th.data(), al.data())) throw Error(BTE_EINVAL, "empty wall pmf");
+                for (int i = 0; i < n; i++) gtab.push_back(make_uint2(th[i], al[i]));
+                gens.push_back(RtaGen{hc.p1x, hc.p1y, hc.p2x - hc.p1x, hc.p2y - hc.p1y, hc.box + bo, sg, 0, 0});
+            }
+        }
+        R.ngen = (int32_t)gens.size() - R.gen0;
+        if (R.ngen == 0) throw Error(BTE_EINVAL, "no isothermal generators (reservoirs at a different Teq)");
+        const double einfo1 = std::accumulate(gw.begin(), gw.end(), 0.);
+        b->Eeff.push_back(einfo1 / (double)D.nparticles);
+        b->cv.push_back(h_calc_cv(*m, D.T0));
+        std::vector<uint32_t> gth(gw.size()), gal(gw.size());
+        build_alias(gw.data(), (int)gw.size(), gth.data(), gal.data());
+        R.gsel0 = (int32_t)gsel.size();
+        for (size_t k = 0; k < gw.size(); k++) gsel.push_back(make_uint2(gth[k], gal[k]));
+        b->runs.push_back(R);
+        b->total += D.nparticles;
+    }
+    b->d_boxes.upload(boxes, s);
+    b->d_cts.upload(cts.empty() ? std::vector<DevContact>(1) : cts, s);
+    b->d_box_cell.upload(box_cell, s);
+    b->d_gens.upload(gens, s);
+    b->d_gen_sel.upload(gsel, s);
+    b->d_gen_tab.upload(gtab, s);
+    b->d_scat.upload(scat, s);
+    b->d_runs.upload(b->runs, s);
+    // Lambert tables (shared: they depend on the modes only)
+    std::vector<int4> info;
+    std::vector<uint2> ltab;
+    std::vector<int32_t> perm;
+    build_lambert(*m, b->sigma > 0 ? b->sigma : 0.25, info, ltab, perm);
+    b->d_lam_info.upload(info, s);
+    b->d_lam_tab.upload(ltab, s);
+    b->d_perm.upload(perm, s);
+    b->d_timegrid.upload(b->timegrid, s);
+    b->d_tally.alloc(3 * b->cells);
+    if (b->any_grad) b->d_dgz.alloc(4 * b->cells);
+    b->d_counters.alloc(8);
+    b->blocks = c->cu_count * 8;
+    HIPCHECK(hipEventCreate(&b->ev0));
+    HIPCHECK(hipEventCreate(&b->ev1));
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+static void batch_execute(RtaBatch* b, double* T_out, double* jx_out, double* jy_out, double* dTdp,
+                          double* Eeff_out, bte_rta_batch_report* rep) {
+    Ctx* c = b->ctx;
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int nt = b->ntime - 1;
+    const uint64_t cells = b->cells;
+    const RtaParams P = batch_params(b, b->d_runs.p, b->total);
+    if (b->use_graph) {
+        if (!b->gexec) {
+            HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            launch_batch(b, P, b->block, b->blocks, s);
+            HIPCHECK(hipStreamEndCapture(s, &b->graph));
+            HIPCHECK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+        }
+        HIPCHECK(hipEventRecord(b->ev0, s));
+        HIPCHECK(hipGraphLaunch(b->gexec, s));
+        HIPCHECK(hipEventRecord(b->ev1, s));
+    } else {
+        HIPCHECK(hipEventRecord(b->ev0, s));
+        launch_batch(b, P, b->block, b->blocks, s);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(b->ev1, s));
+    }
+    std::vector<double> tally(3 * cells), dg(b->any_grad ? 4 * cells : 0);
+    unsigned long long cnt[8];
+    HIPCHECK(hipMemcpyAsync(tally.data(), b->d_tally.p, 3 * cells * sizeof(double), hipMemcpyDeviceToHost, s));
+    if (b->any_grad)
+        HIPCHECK(hipMemcpyAsync(dg.data(), b->d_dgz.p, 4 * cells * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(cnt, b->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    if (cnt[3] != 0)
+        throw Error(BTE_EGEOM, std::to_string(cnt[3]) + " particle histories exceeded the iteration budget");
+    if (cnt[4] != 0)
+        throw Error(BTE_EINVAL, "specular reflection of a mode without a mirror partner in the mode table");
+    // post-processing per run (RTA_MC2D.cpp:2336-2435): scale, cumulative
+    // sum over time (step response), T = gz / (Cv / vcorr) + T0
+    const double th = b->thickness, vcorr = th / b->m->lattz;
+    uint64_t out0 = 0;
+    for (int r = 0; r < b->nruns; r++) {
+        const Geom* g = b->geoms[r];
+        const int nb = b->nbox[r];
+        const double E = b->Eeff[r], cvr = b->cv[r];
+        double* gz = tally.data() + b->cell0[r];
+        double* jx = gz + cells;
+        double* jy = jx + cells;
+        for (int bx = 0; bx < nb; bx++) {
+            const double vbox = g->boxes[bx].area() * th;
+            double agz = 0, ajx = 0, ajy = 0;
+            double ad[4] = {0, 0, 0, 0};
+            for (int k = 0; k < nt; k++) {
+                const size_t cidx = (size_t)bx * nt + k;
+                agz += gz[cidx] * E / vbox;
+                ajx += jx[cidx] * E * 1e30 / vbox;
+                ajy += jy[cidx] * E * 1e30 / vbox;
+                const size_t o = out0 + (size_t)k * nb + bx;
+                T_out[o] = agz / (cvr / vcorr) + b->desc[r].T0;
+                jx_out[o] = ajx;
+                jy_out[o] = ajy;
+                if (dTdp) {
+                    for (int q = 0; q < 4; q++) {
+                        if (b->any_grad) ad[q] += dg[(size_t)q * cells + b->cell0[r] + cidx] * E / vbox;
+                        dTdp[4 * out0 + (size_t)q * nt * nb + (size_t)k * nb + bx] = ad[q] / (cvr / vcorr);
+                    }
+                }
+            }
+        }
+        if (Eeff_out) Eeff_out[r] = E;
+        out0 += (uint64_t)nt * nb;
+    }
+    if (rep) {
+        rep->events = cnt[0];
+        rep->crossings = cnt[1];
+        rep->kernel_ms = ms;
+        rep->ntime = b->ntime;
+        rep->block = b->block;
+        rep->blocks = b->blocks;
+    }
+}
+
+// Launch-shape autotuner: times the batch with every run scaled to a
+// fraction of its particles for each candidate (block size, workgroups per
+// CU) and keeps the fastest.
+static void batch_tune(RtaBatch* b, double fraction) {
+    Ctx* c = b->ctx;
+    hipStream_t s = c->stream;
+    std::vector<RtaRunDev> tr = b->runs;
+    uint64_t tot = 0;
+    for (auto& R : tr) {
+        R.count = std::max<uint64_t>(1, (uint64_t)(R.count * fraction));
+        R.first = tot;
+        tot += R.count;
+    }
+    b->d_trial_runs.upload(tr, s);
+    const RtaParams P = batch_params(b, b->d_trial_runs.p, tot);
+    const int cand[][2] = {{256, 4}, {256, 8}, {256, 16}, {128, 8}, {128, 16}, {128, 32}, {512, 2}, {512, 4}};
+    double best = 1e300;
+    int bb = b->block, bn = b->blocks;
+    for (auto& cd : cand) {
+        const int blocks = c->cu_count * cd[1];
+        launch_batch(b, P, cd[0], blocks, s);  // warm-up
+        HIPCHECK(hipEventRecord(b->ev0, s));
+        launch_batch(b, P, cd[0], blocks, s);
+        HIPCHECK(hipEventRecord(b->ev1, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+        if (ms < best) {
+            best = ms;
+            bb = cd[0];
+            bn = blocks;
+        }
+    }
+    if (bb != b->block || bn != b->blocks) {
+        b->block = bb;
+        b->blocks = bn;
+        if (b->gexec) {
+            (void)hipGraphExecDestroy(b->gexec);
+            (void)hipGraphDestroy(b->graph);
+            b->gexec = nullptr;
+            b->graph = nullptr;
+        }
+    }
+}
+
 }  // namespace bte
 
 using namespace bte;
 
 extern "C" {
 
-int bte_rta_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const bte_rta_cfg* cfg,
-                   bte_rta** out) {
+int bte_rta_batch_create(bte_ctx* ctx, const bte_modes* mm, int nruns, const bte_rta_run_desc* runs,
+                         double thickness, double dt, double maxtime, double sigma_bounce, int use_graph,
+                         bte_rta_batch** out) {
     BTE_GUARD_BEGIN
     auto* c = reinterpret_cast<Ctx*>(ctx);
     auto* m = reinterpret_cast<const Modes*>(mm);
-    auto* g = reinterpret_cast<const Geom*>(gg);
-    if (!c || !m || !g || !cfg || !out) throw Error(BTE_EINVAL, "null argument");
+    if (!c || !m || !runs || !out || nruns <= 0) throw Error(BTE_EINVAL, "bad arguments to bte_rta_batch_create");
+    HIPCHECK(hipSetDevice(c->device));
+    auto* b = new RtaBatch();
+    std::unique_ptr<RtaBatch> guard(b);
+    b->ctx = c;
+    b->m = m;
+    b->desc.assign(runs, runs + nruns);
+    for (int r = 0; r < nruns; r++) b->geoms.push_back(reinterpret_cast<const Geom*>(runs[r].geom));
+    b->thickness = thickness;
+    b->dt = dt;
+    b->maxtime = maxtime;
+    b->sigma = sigma_bounce;
+    b->use_graph = use_graph != 0;
+    batch_create(b);
+    *out = reinterpret_cast<bte_rta_batch*>(guard.release());
+    BTE_GUARD_END
+}
+
+int bte_rta_batch_tune(bte_rta_batch* bb, double fraction, int* block, int* blocks) {
+    BTE_GUARD_BEGIN
+    auto* b = reinterpret_cast<RtaBatch*>(bb);
+    if (!b || !(fraction > 0) || fraction > 1) throw Error(BTE_EINVAL, "bad arguments to bte_rta_batch_tune");
+    HIPCHECK(hipSetDevice(b->ctx->device));
+    batch_tune(b, fraction);
+    if (block) *block = b->block;
+    if (blocks) *blocks = b->blocks;
+    BTE_GUARD_END
+}
+
+int bte_rta_batch_execute(bte_rta_batch* bb, double* T_out, double* jx_out, double* jy_out, double* dTdp,
+                          double* Eeff, bte_rta_batch_report* rep) {
+    BTE_GUARD_BEGIN
+    auto* b = reinterpret_cast<RtaBatch*>(bb);
+    if (!b || !T_out || !jx_out || !jy_out) throw Error(BTE_EINVAL, "null argument");
+    batch_execute(b, T_out, jx_out, jy_out, dTdp, Eeff, rep);
+    BTE_GUARD_END
+}
+
+int bte_rta_batch_destroy(bte_rta_batch* bb) {
+    BTE_GUARD_BEGIN
+    delete reinterpret_cast<RtaBatch*>(bb);
+    BTE_GUARD_END
+}
+
+// single runs: a batch of one, with the reference's diffuse walls
+int bte_rta_create(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_cfg* cfg, bte_rta** out) {
+    BTE_GUARD_BEGIN
+    if (!cfg || !out || !g) throw Error(BTE_EINVAL, "null argument");
     if (!(cfg->dt > 0) || !(cfg->maxtime > 0) || cfg->nparticles == 0 || !(cfg->thickness > 0))
         throw Error(BTE_EINVAL, "bad RTA configuration");
-    HIPCHECK(hipSetDevice(c->device));
-    auto* r = new Rta();
-    std::unique_ptr<Rta> guard(r);
-    r->ctx = c;
-    r->m = m;
-    r->g = g;
-    r->cfg = *cfg;
-    const int n = m->n, nbox = (int)g->boxes.size();
-    for (auto& b : g->boxes)
-        if (b.kind == 0) {
-            if (b.Teq != cfg->T0) throw Error(BTE_EINVAL, "Tref in xml is different from provided T0");
-            if (b.Tinit != b.Teq) throw Error(BTE_EINVAL, "Initial temperature out of reference is not allowed");
-        }
-    for (int i = 0; i < n; i++)
-        if (m->w[i] == 0.) throw Error(BTE_EINVAL, "invalid scattering rate");
-    // time grid (RTA_MC2D.cpp:2121-2124)
-    r->ntime = (int)std::floor(cfg->maxtime / cfg->dt + 1.0e-6) + 1;
-    if (r->ntime < 2) throw Error(BTE_EINVAL, "maxtime shorter than dt");
-    r->maxtime = (r->ntime - 1) * cfg->dt;
-    r->timegrid.resize(r->ntime);
-    for (int i = 0; i < r->ntime; i++) r->timegrid[i] = (double)i * r->maxtime / (r->ntime - 1);
-    hipStream_t s = c->stream;
-    // mode records
-    std::vector<double4> md(n);
-    for (int i = 0; i < n; i++) md[i] = make_double4(m->vx[i], m->vy[i], m->w[i], m->omega[i]);
-    r->d_modes.upload(md, s);
-    std::vector<double2> iv(n);
-    for (int i = 0; i < n; i++)
-        iv[i] = make_double2(m->vx[i] != 0. ? 1.0 / m->vx[i] : 0., m->vy[i] != 0. ? 1.0 / m->vy[i] : 0.);
-    r->d_ivel.upload(iv, s);
-    // BE_derivative sampler (sampling.cpp:47-68)
-    std::vector<double> pmf(n);
-    for (int i = 0; i < n; i++) pmf[i] = m->omega[i] <= 0. ? 0. : h_be_kernel(m->omega[i], cfg->T0) * m->w[i];
-    std::vector<uint32_t> th(n), al(n);
-    if (!build_alias(pmf.data(), n, th.data(), al.data())) throw Error(BTE_EINVAL, "empty scattering pmf");
-    upload_alias(r->d_scat, th, al, s);
-    // isothermal wall generators (RTA_MC2D.cpp:296-347, 366-397)
-    double vuc = m->V * cfg->thickness / m->lattz;
-    std::vector<RtaGen> gens;
-    std::vector<double> gw;
-    std::vector<uint2> gtab;
-    for (int sid = 0; sid < nbox; sid++) {
-        const RectBox& sb = g->boxes[sid];
-        if (sb.kind != 1) continue;
-        for (auto& hc : g->contacts[sid]) {
-            const RectBox& cb = g->boxes[hc.box];
-            if (cb.kind != 0) continue;
-            if (almost_equal(hc.length(), 0.)) continue;
-            if (almost_equal(sb.Teq, cb.Teq)) continue;
-            double cum = wall_pmf(*m, sb.Teq, cb.Teq, hc.npx, hc.npy, pmf.data());
-            int sg = (sb.Teq - cb.Teq) >= 0 ? 1 : -1;
-            double jd = sg * cum / (m->nq * vuc);
-            gw.push_back(std::fabs(jd) * hc.length() * cfg->thickness);
-            if (!build_alias(pmf.data(), n, th.data(), al.data())) throw Error(BTE_EINVAL, "empty wall pmf");
-            for (int i = 0; i < n; i++) gtab.push_back(make_uint2(th[i], al[i]));
-            gens.push_back(RtaGen{hc.p1x, hc.p1y, hc.p2x - hc.p1x, hc.p2y - hc.p1y, hc.box, sg, 0, 0});
-        }
-    }
-    if (gens.empty()) throw Error(BTE_EINVAL, "no isothermal generators (reservoirs at a different Teq)");
-    r->ngen = (int)gens.size();
-    r->einfo1 = std::accumulate(gw.begin(), gw.end(), 0.);
-    r->Eeff = r->einfo1 / (double)cfg->nparticles;
-    std::vector<uint32_t> gth(gens.size()), gal(gens.size());
-    build_alias(gw.data(), (int)gw.size(), gth.data(), gal.data());
-    upload_alias(r->d_gen_sel, gth, gal, s);
-    r->d_gens.upload(gens, s);
-    r->d_gen_tab.upload(gtab, s);
-    // Lambert tables
-    std::vector<int4> info;
-    std::vector<uint2> ltab;
-    std::vector<int32_t> perm;
-    build_lambert(*m, cfg->sigma_bounce > 0 ? cfg->sigma_bounce : 0.25, info, ltab, perm);
-    r->d_lam_info.upload(info, s);
-    r->d_lam_tab.upload(ltab, s);
-    r->d_perm.upload(perm, s);
-    r->d_timegrid.upload(r->timegrid, s);
-    r->d_tally.alloc((size_t)3 * nbox * (r->ntime - 1));
-    r->d_counters.alloc(4);
-    r->cv = h_calc_cv(*m, cfg->T0);
-    HIPCHECK(hipEventCreate(&r->ev0));
-    HIPCHECK(hipEventCreate(&r->ev1));
-    HIPCHECK(hipStreamSynchronize(s));
-    *out = reinterpret_cast<bte_rta*>(guard.release());
+    bte_rta_run_desc d{};
+    d.geom = g;
+    d.T0 = cfg->T0;
+    d.nparticles = cfg->nparticles;
+    d.seed = cfg->seed;
+    d.ballistic = cfg->ballistic;
+    d.wscale = 1.0;
+    bte_rta_batch* b = nullptr;
+    int rc = bte_rta_batch_create(ctx, m, 1, &d, cfg->thickness, cfg->dt, cfg->maxtime, cfg->sigma_bounce, 0, &b);
+    if (rc != BTE_OK) return rc;
+    *out = reinterpret_cast<bte_rta*>(b);
     BTE_GUARD_END
 }
 
 int bte_rta_execute(bte_rta* rr, double* T_out, double* jx_out, double* jy_out, bte_rta_report* rep) {
     BTE_GUARD_BEGIN
-    auto* r = reinterpret_cast<Rta*>(rr);
-    if (!r || !T_out || !jx_out || !jy_out) throw Error(BTE_EINVAL, "null argument");
-    Ctx* c = r->ctx;
-    HIPCHECK(hipSetDevice(c->device));
-    hipStream_t s = c->stream;
-    const int nbox = (int)r->g->boxes.size(), nt = r->ntime - 1;
-    const size_t cells = (size_t)nbox * nt;
-    HIPCHECK(hipMemsetAsync(r->d_tally.p, 0, 3 * cells * sizeof(double), s));
-    HIPCHECK(hipMemsetAsync(r->d_counters.p, 0, 4 * sizeof(unsigned long long), s));
-    RtaParams P{};
-    P.modes = r->d_modes.p;
-    P.ivel = r->d_ivel.p;
-    P.scat = r->d_scat.p;
-    P.boxes = r->g->d_boxes.p;
-    P.cts = r->g->d_contacts.p;
-    P.gens = r->d_gens.p;
-    P.gen_sel = r->d_gen_sel.p;
-    P.gen_tab = r->d_gen_tab.p;
-    P.lam_info = r->d_lam_info.p;
-    P.lam_tab = r->d_lam_tab.p;
-    P.perm = r->d_perm.p;
-    P.timegrid = r->d_timegrid.p;
-    P.gz = r->d_tally.p;
-    P.jx = r->d_tally.p + cells;
-    P.jy = r->d_tally.p + 2 * cells;
-    P.counters = r->d_counters.p;
-    P.next_pid = r->d_counters.p + 2;
-    P.nparticles = r->cfg.nparticles;
-    P.seed = r->cfg.seed;
-    P.maxtime = r->maxtime;
-    P.max_iters = 1ULL << 26;
-    P.n = r->m->n;
-    P.nbox = nbox;
-    P.ngen = r->ngen;
-    P.nt = nt;
-    P.ballistic = r->cfg.ballistic;
-    int blocks = c->cu_count * 8;
-    HIPCHECK(hipEventRecord(r->ev0, s));
-    hipLaunchKernelGGL(rta_history_kernel, dim3(blocks), dim3(256), 0, s, P);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(r->ev1, s));
-    std::vector<double> tally(3 * cells);
-    unsigned long long cnt[4];
-    HIPCHECK(hipMemcpyAsync(tally.data(), r->d_tally.p, 3 * cells * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(cnt, r->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
-    float ms = 0;
-    HIPCHECK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
-    // post-processing (RTA_MC2D.cpp:2336-2435)
-    double* gz = tally.data();
-    double* jx = gz + cells;
-    double* jy = jx + cells;
-    const double th = r->cfg.thickness;
-    for (int b = 0; b < nbox; b++) {
-        double vbox = r->g->boxes[b].area() * th;
-        for (int k = 0; k < nt; k++) {
-            gz[b * nt + k] *= r->Eeff / vbox;
-            jx[b * nt + k] *= r->Eeff * 1e30 / vbox;
-            jy[b * nt + k] *= r->Eeff * 1e30 / vbox;
-        }
-        for (int k = 1; k < nt; k++) {
-            gz[b * nt + k] += gz[b * nt + k - 1];
-            jx[b * nt + k] += jx[b * nt + k - 1];
-            jy[b * nt + k] += jy[b * nt + k - 1];
-        }
-    }
-    double vcorr = th / r->m->lattz;
-    for (int b = 0; b < nbox; b++)
-        for (int k = 0; k < nt; k++) {
-            T_out[(size_t)k * nbox + b] = gz[b * nt + k] / (r->cv / vcorr) + r->cfg.T0;
-            jx_out[(size_t)k * nbox + b] = jx[b * nt + k];
-            jy_out[(size_t)k * nbox + b] = jy[b * nt + k];
-        }
-    if (cnt[3] != 0)
-        throw Error(BTE_EGEOM, std::to_string(cnt[3]) + " particle histories exceeded the iteration budget");
+    auto* b = reinterpret_cast<RtaBatch*>(rr);
+    if (!b || !T_out || !jx_out || !jy_out) throw Error(BTE_EINVAL, "null argument");
+    bte_rta_batch_report br{};
+    double E = 0;
+    batch_execute(b, T_out, jx_out, jy_out, nullptr, &E, &br);
     if (rep) {
-        rep->events = cnt[0];
-        rep->crossings = cnt[1];
-        rep->Eeff = r->Eeff;
-        rep->kernel_ms = ms;
-        rep->ntime = r->ntime;
+        rep->events = br.events;
+        rep->crossings = br.crossings;
+        rep->Eeff = E;
+        rep->kernel_ms = br.kernel_ms;
+        rep->ntime = br.ntime;
     }
     BTE_GUARD_END
 }
 
 int bte_rta_destroy(bte_rta* rr) {
     BTE_GUARD_BEGIN
-    delete reinterpret_cast<Rta*>(rr);
+    delete reinterpret_cast<RtaBatch*>(rr);
     BTE_GUARD_END
 }
 

@@ -110,6 +110,45 @@ int bte_rta_destroy(bte_rta* r);
 int bte_rta_run(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_cfg* cfg,
                 double* T_out, double* jx_out, double* jy_out, bte_rta_report* rep);
 
+/* ---- batched RTAMC2D: independent runs of one material (geometry,
+ * temperature, particle count, seed and edge specularity per run) in one
+ * persistent launch. The reference runs its nruns one after another
+ * (RTA_MC2D.cpp:2188-2471); batching, specular edges and the specularity
+ * gradient have no counterpart there. ---- */
+typedef struct bte_rta_batch bte_rta_batch;
+typedef struct {
+    const bte_geom* geom;   /* the run's boxes; regular boxes at Teq = T0 */
+    double T0;              /* run temperature [K]: Cv(T0), BE_derivative at T0 */
+    uint64_t nparticles;
+    uint64_t seed;
+    int ballistic;
+    double wscale;          /* scattering rates used: w * wscale */
+    double specularity[4];  /* probability of a specular reflection at a void
+                               side (left, bottom, top, right); else diffuse */
+    int grad;               /* accumulate dT/dp for each side with 0 < p < 1 */
+} bte_rta_run_desc;
+
+typedef struct {
+    uint64_t events, crossings;
+    double kernel_ms;       /* device time of the batch */
+    int ntime;
+    int block, blocks;      /* launch shape in use */
+} bte_rta_batch_report;
+
+/* use_graph: capture the clear + history launches once in a HIP graph and
+ * replay it on every execute. */
+int bte_rta_batch_create(bte_ctx* ctx, const bte_modes* m, int nruns, const bte_rta_run_desc* runs,
+                         double thickness, double dt, double maxtime, double sigma_bounce, int use_graph,
+                         bte_rta_batch** out);
+/* Times the batch scaled to `fraction` of each run's particles over a set of
+ * launch shapes and keeps the fastest. */
+int bte_rta_batch_tune(bte_rta_batch* b, double fraction, int* block, int* blocks);
+/* T, jx, jy: per run r, [ntime-1][nbox_r] blocks one after another; dTdp (may
+ * be NULL): per run [4 sides][ntime-1][nbox_r]; Eeff (may be NULL): [nruns]. */
+int bte_rta_batch_execute(bte_rta_batch* b, double* T, double* jx, double* jy, double* dTdp, double* Eeff,
+                          bte_rta_batch_report* rep);
+int bte_rta_batch_destroy(bte_rta_batch* b);
+
 /* ---- propagator: dense P (ncol x ncol, column-major) for modes
  * [ngamma, ngamma+ncol) (replaces load_P + propagator_H per column,
  * src/collision_operator.cpp:854-1013, beyondRTA_MC2d.cpp:1356-1378) ---- */
