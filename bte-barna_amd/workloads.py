@@ -85,3 +85,38 @@ def config1(nsteps=200, dt=5.0, nparticles=1_000_000, nbins=50, seed=1):
     return dict(rect=rect, kind=kind, Teq=teq, Tinit=tin, thickness=1.0, dt=dt,
                 maxtime=dt * (nsteps - 1), max_steps=nsteps, Efactor=nparticles, Tmax=301.0,
                 Tmin=300.0, seed=seed)
+
+
+def config3(nruns=64, nmodes=3000, nbins=100, length=1000.0, maxtime=5000.0, dt=25.0, seed=1):
+    """Batched sweep of independent RTAMC2D runs: widths log-spaced
+    50 nm - 2 um, temperatures 100 - 500 K (walls at T +- 0.5 K), Cv and the
+    BE_derivative sampler at each run's T, scattering rates scaled as
+    tau(T) = tau(300 K) * 300 / T, particle counts log-uniform 1e5 - 1e7
+    (seeded), the 3000-mode table of config 0 shared by all runs."""
+    mat = material(nmodes, seed=1)
+    widths = np.logspace(np.log10(50.0), np.log10(2000.0), nruns)
+    temps = np.linspace(100.0, 500.0, nruns)
+    rng = np.random.default_rng(seed)
+    counts = np.round(10.0 ** rng.uniform(5.0, 7.0, nruns)).astype(np.int64)
+    runs = []
+    for r in range(nruns):
+        rect, kind, teq, _ = ribbon(length, widths[r], nbins, temps[r],
+                                    reservoirs=(temps[r] + 0.5, temps[r] - 0.5))
+        runs.append(dict(rect=rect, kind=kind, Teq=teq, T0=float(temps[r]), nparticles=int(counts[r]),
+                         seed=seed * 1000 + r, wscale=float(temps[r]) / 300.0, width=float(widths[r])))
+    return dict(material=mat, runs=runs, thickness=1.0, dt=dt, maxtime=maxtime, length=length,
+                sigma_bounce=0.25)
+
+
+def config4(nparticles=10_000_000, nmodes=5000, nbins=128, ntime=64, maxtime=5000.0, p_top=0.3,
+            p_bottom=0.7, seed=1):
+    """Differentiable transient RTAMC2D: 1 um x 100 nm ribbon in 128 x-bins,
+    64 time bins, walls at 300 +- 0.5 K, specular probability p_top on the
+    top edge and p_bottom on the bottom edge (diffuse otherwise), 5000 modes
+    built as 2500 mirror pairs in v_y."""
+    mat = material(nmodes, seed=seed, mirror=True)
+    rect, kind, teq, _ = ribbon(1000.0, 100.0, nbins, 300.0, reservoirs=(300.5, 299.5))
+    # side order: left, bottom, top, right
+    spec = (0.0, p_bottom, p_top, 0.0)
+    return dict(material=mat, rect=rect, kind=kind, Teq=teq, T0=300.0, thickness=1.0, dt=maxtime / ntime,
+                maxtime=maxtime, nparticles=nparticles, seed=seed, specularity=spec, sigma_bounce=0.25)
