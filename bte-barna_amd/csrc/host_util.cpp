@@ -157,6 +157,7 @@ int bte_ctx_destroy(bte_ctx* ctx) {
     if (!c) return BTE_OK;
     HIPCHECK(hipSetDevice(c->device));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    if (c->blas && c->blas_free) c->blas_free(c->blas);
     HIPCHECK(hipStreamDestroy(c->stream));
     delete c;
     BTE_GUARD_END

@@ -48,18 +48,6 @@ __global__ void kr_f32_to_f64(const float* __restrict__ a, double* __restrict__ 
         o[i] = (double)a[i];
 }
 
-__device__ inline double block_sum(double v, double* red) {
-    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    __syncthreads();
-    if (l == 0) red[w] = v;
-    __syncthreads();
-    double s = 0;
-    const int nw = blockDim.x >> 6;
-    for (int q = 0; q < nw; q++) s += red[q];
-    return s;
-}
-
 // Basis storage: FP64 (prec 0), FP32 (1) or BF16 hi/lo (2); column i of V_j
 // starts at (j*N + i)*N.
 struct Basis {
@@ -82,75 +70,6 @@ struct Basis {
         lo[idx] = __float2bfloat16(f - __bfloat162float(h));
     }
 };
-
-// Modified Gram-Schmidt step k for every column (Arnoldi_algorithm,
-// collision_operator.cpp:831-850). z comes from the GEMM (FP32 or FP64), the
-// projections run in FP64. V_0 = I is implicit.
-template <int E>
-__global__ __launch_bounds__(256) void kr_mgs(const float* __restrict__ Z32, const double* __restrict__ Z64,
-                                              Basis V, double* __restrict__ H, int* __restrict__ closed,
-                                              int N, int m, int k, int textbook) {
-    __shared__ double red[4];
-    const int i = blockIdx.x;
-    if (closed[i]) return;
-    double z[E];
-    const size_t zc = (size_t)i * N;
-#pragma unroll
-    for (int e = 0; e < E; e++) {
-        int r = threadIdx.x + 256 * e;
-        z[e] = r < N ? (Z64 ? Z64[zc + r] : (double)Z32[zc + r]) : 0.;
-    }
-    double* Hi = H + (size_t)i * m * m;  // column-major m x m
-    const int jmax = textbook ? k + 1 : k;
-    for (int j = 0; j < jmax; j++) {
-        double d = 0;
-        if (j == 0) {
-            // V_0(:,i) = e_i
-#pragma unroll
-            for (int e = 0; e < E; e++)
-                if (threadIdx.x + 256 * e == i) d = z[e];
-        } else {
-            const size_t vc = ((size_t)j * N + i) * N;
-#pragma unroll
-            for (int e = 0; e < E; e++) {
-                int r = threadIdx.x + 256 * e;
-                if (r < N) d += V.load(vc + r) * z[e];
-            }
-        }
-        double h = block_sum(d, red);
-        if (threadIdx.x == 0) Hi[j + (size_t)k * m] = h;
-        if (j == 0) {
-#pragma unroll
-            for (int e = 0; e < E; e++)
-                if (threadIdx.x + 256 * e == i) z[e] -= h;
-        } else {
-            const size_t vc = ((size_t)j * N + i) * N;
-#pragma unroll
-            for (int e = 0; e < E; e++) {
-                int r = threadIdx.x + 256 * e;
-                if (r < N) z[e] -= h * V.load(vc + r);
-            }
-        }
-    }
-    double nn = 0;
-#pragma unroll
-    for (int e = 0; e < E; e++) nn += z[e] * z[e];
-    double zn = sqrt(block_sum(nn, red));
-    if (almost_equal(zn, 0., 1.0e-12, 1.0e-9)) {
-        if (threadIdx.x == 0) closed[i] = 1;
-        return;
-    }
-    if (k + 1 < m) {
-        if (threadIdx.x == 0) Hi[(k + 1) + (size_t)k * m] = zn;
-        const size_t vc = ((size_t)(k + 1) * N + i) * N;
-        const double inv = 1.0 / zn;
-#pragma unroll
-        for (int e = 0; e < E; e++) {
-            int r = threadIdx.x + 256 * e;
-            if (r < N) V.store(vc + r, z[e] * inv);
-        }
-    }
-}
 
 // c = exp(H_i) e_1 for every column: one wave per column, H in LDS;
 // exp(H) e_1 = (exp(H/2^s))^(2^s) e_1 with each factor a Taylor polynomial
@@ -205,62 +124,176 @@ __global__ __launch_bounds__(64) void kr_expm_e1(const double* __restrict__ H, d
     if (l < m) C[(size_t)i * m + l] = v[l];
 }
 
-// P(:,i) = sum_k c_k V_k(:,i), then the column energy projection
-// P(:,i) += (1 - sum_r P(r,i)) / N. colerr gets |sum - 1| before projection.
-template <int E>
-__global__ __launch_bounds__(256) void kr_combine(Basis V, const double* __restrict__ C,
-                                                  double* __restrict__ P, double* __restrict__ colerr,
-                                                  int N, int m, int project) {
-    __shared__ double red[4];
+// Wide variants: one 1024-thread workgroup per column, element r of the
+// column held by thread r mod 1024 (E = ceil(N/1024) per thread), the basis
+// type fixed at compile time, V_{j+1} prefetched while V_j's projection is
+// reduced, one barrier per projection. Same operation order per column as
+// Arnoldi_algorithm: each projection uses the updated vector.
+template <int PREC>
+__device__ inline double bload(const Basis& V, size_t i) {
+    if constexpr (PREC == 0) return V.f64[i];
+    else if constexpr (PREC == 1) return (double)V.f32[i];
+    else return (double)__bfloat162float(V.hi[i]) + (double)__bfloat162float(V.lo[i]);
+}
+template <int PREC>
+__device__ inline void bstore(const Basis& V, size_t i, double v) {
+    if constexpr (PREC == 0) V.f64[i] = v;
+    else if constexpr (PREC == 1) V.f32[i] = (float)v;
+    else {
+        float f = (float)v;
+        __hip_bfloat16 h = __float2bfloat16(f);
+        V.hi[i] = h;
+        V.lo[i] = __float2bfloat16(f - __bfloat162float(h));
+    }
+}
+// sum over the 16 waves of a 1024-thread workgroup; buf holds 16 doubles
+// and must not be rewritten before every thread has read it
+__device__ inline double wide_sum(double v, double* buf) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) buf[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) s += buf[q];
+    return s;
+}
+
+template <int E, int PREC>
+__global__ __launch_bounds__(1024) void kr_mgs_wide(const float* __restrict__ Z32, const double* __restrict__ Z64,
+                                                    Basis V, double* __restrict__ H, int* __restrict__ closed,
+                                                    int N, int m, int k, int textbook) {
+    __shared__ double red[3][16];
+    __shared__ double hz;
     const int i = blockIdx.x;
+    if (closed[i]) return;
+    const int tid = threadIdx.x;
+    double z[E], v[E];
+    const size_t zc = (size_t)i * N;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int r = tid + 1024 * e;
+        z[e] = r < N ? (PREC == 0 ? Z64[zc + r] : (double)Z32[zc + r]) : 0.;
+    }
+    double* Hi = H + (size_t)i * m * m;  // column-major m x m
+    const int jmax = textbook ? k + 1 : k;
+    // j = 0: V_0(:,i) = e_i, so the projection is element i of z
+#pragma unroll
+    for (int e = 0; e < E; e++)
+        if (tid + 1024 * e == i) hz = z[e];
+    if (jmax > 1) {
+        const size_t vc = ((size_t)1 * N + i) * N;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int r = tid + 1024 * e;
+            v[e] = r < N ? bload<PREC>(V, vc + r) : 0.;
+        }
+    }
+    __syncthreads();
+    if (jmax > 0) {
+        const double h = hz;
+        if (tid == 0) Hi[(size_t)k * m] = h;
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            if (tid + 1024 * e == i) z[e] -= h;
+    }
+    for (int j = 1; j < jmax; j++) {
+        double d = 0;
+#pragma unroll
+        for (int e = 0; e < E; e++) d += v[e] * z[e];
+        double vn[E];
+        if (j + 1 < jmax) {
+            const size_t vc = ((size_t)(j + 1) * N + i) * N;
+#pragma unroll
+            for (int e = 0; e < E; e++) {
+                const int r = tid + 1024 * e;
+                vn[e] = r < N ? bload<PREC>(V, vc + r) : 0.;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; e++) vn[e] = 0.;
+        }
+        const double h = wide_sum(d, red[j & 1]);
+        if (tid == 0) Hi[j + (size_t)k * m] = h;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            z[e] -= h * v[e];
+            v[e] = vn[e];
+        }
+    }
+    double nn = 0;
+#pragma unroll
+    for (int e = 0; e < E; e++) nn += z[e] * z[e];
+    const double zn = sqrt(wide_sum(nn, red[2]));
+    if (almost_equal(zn, 0., 1.0e-12, 1.0e-9)) {
+        if (tid == 0) closed[i] = 1;
+        return;
+    }
+    if (k + 1 < m) {
+        if (tid == 0) Hi[(k + 1) + (size_t)k * m] = zn;
+        const size_t vc = ((size_t)(k + 1) * N + i) * N;
+        const double inv = 1.0 / zn;
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+            const int r = tid + 1024 * e;
+            if (r < N) bstore<PREC>(V, vc + r, z[e] * inv);
+        }
+    }
+}
+
+template <int E, int PREC>
+__global__ __launch_bounds__(1024) void kr_combine_wide(Basis V, const double* __restrict__ C,
+                                                        double* __restrict__ P, double* __restrict__ colerr, int N,
+                                                        int m, int project) {
+    __shared__ double red[16];
+    const int i = blockIdx.x, tid = threadIdx.x;
     const double* c = C + (size_t)i * m;
     double acc[E];
 #pragma unroll
-    for (int e = 0; e < E; e++) {
-        int r = threadIdx.x + 256 * e;
-        acc[e] = r == i ? c[0] : 0.;
-    }
+    for (int e = 0; e < E; e++) acc[e] = tid + 1024 * e == i ? c[0] : 0.;
     for (int k = 1; k < m; k++) {
         const double ck = c[k];
         if (ck == 0.) continue;
         const size_t vc = ((size_t)k * N + i) * N;
 #pragma unroll
         for (int e = 0; e < E; e++) {
-            int r = threadIdx.x + 256 * e;
-            if (r < N) acc[e] += ck * V.load(vc + r);
+            const int r = tid + 1024 * e;
+            if (r < N) acc[e] += ck * bload<PREC>(V, vc + r);
         }
     }
     double s = 0;
 #pragma unroll
     for (int e = 0; e < E; e++) s += acc[e];
-    double tot = block_sum(s, red);
-    if (threadIdx.x == 0) colerr[i] = fabs(tot - 1.0);
+    const double tot = wide_sum(s, red);
+    if (tid == 0) colerr[i] = fabs(tot - 1.0);
     const double corr = project ? (1.0 - tot) / N : 0.;
 #pragma unroll
     for (int e = 0; e < E; e++) {
-        int r = threadIdx.x + 256 * e;
+        const int r = tid + 1024 * e;
         if (r < N) P[(size_t)i * N + r] = acc[e] + corr;
     }
 }
 
 template <int E>
-static void launch_mgs(hipStream_t s, int N, const float* z32, const double* z64, Basis V, double* H,
-                       int* closed, int m, int k, int tb) {
-    hipLaunchKernelGGL((kr_mgs<E>), dim3(N), dim3(256), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+static void launch_mgs_wide(hipStream_t s, int N, const float* z32, const double* z64, Basis V, double* H,
+                            int* closed, int m, int k, int tb) {
+    if (V.prec == 0)
+        hipLaunchKernelGGL((kr_mgs_wide<E, 0>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+    else if (V.prec == 1)
+        hipLaunchKernelGGL((kr_mgs_wide<E, 1>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+    else
+        hipLaunchKernelGGL((kr_mgs_wide<E, 2>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, m, k, tb);
 }
 template <int E>
-static void launch_combine(hipStream_t s, int N, Basis V, const double* C, double* P, double* err, int m) {
-    hipLaunchKernelGGL((kr_combine<E>), dim3(N), dim3(256), 0, s, V, C, P, err, N, m, 1);
+static void launch_combine_wide(hipStream_t s, int N, Basis V, const double* C, double* P, double* err, int m) {
+    if (V.prec == 0)
+        hipLaunchKernelGGL((kr_combine_wide<E, 0>), dim3(N), dim3(1024), 0, s, V, C, P, err, N, m, 1);
+    else if (V.prec == 1)
+        hipLaunchKernelGGL((kr_combine_wide<E, 1>), dim3(N), dim3(1024), 0, s, V, C, P, err, N, m, 1);
+    else
+        hipLaunchKernelGGL((kr_combine_wide<E, 2>), dim3(N), dim3(1024), 0, s, V, C, P, err, N, m, 1);
 }
+static int wide_elems(int N) { return N <= 1024 ? 1 : (N <= 2048 ? 2 : (N <= 4096 ? 4 : (N <= 8192 ? 8 : 16))); }
 
-static int elems_for(int N) {
-    if (N <= 1024) return 4;
-    if (N <= 2048) return 8;
-    if (N <= 4096) return 16;
-    if (N <= 8192) return 32;
-    if (N <= 16384) return 64;
-    return -1;
-}
 
 }  // namespace bte
 
@@ -272,17 +305,19 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
     auto* c = reinterpret_cast<Ctx*>(ctx);
     if (!c || !B || !P || N < 2 || m < 2 || m > 64 || prec < 0 || prec > 2 || (mode != 0 && mode != 1))
         throw Error(BTE_EINVAL, "bad arguments to bte_krylov_build");
-    int E = elems_for(N);
-    if (E < 0) throw Error(BTE_EINVAL, "N above 16384 is not supported");
+    if (N > 16384) throw Error(BTE_EINVAL, "N above 16384 is not supported");
+    const int WE = wide_elems(N);
     HIPCHECK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const size_t NN = (size_t)N * N;
     rocblas_handle hb;
-    RBCHECK(rocblas_create_handle(&hb));
-    struct HandleGuard {
-        rocblas_handle h;
-        ~HandleGuard() { rocblas_destroy_handle(h); }
-    } hg{hb};
+    if (!c->blas) {
+        rocblas_handle h0;
+        RBCHECK(rocblas_create_handle(&h0));
+        c->blas = h0;
+        c->blas_free = [](void* h) { rocblas_destroy_handle((rocblas_handle)h); };
+    }
+    hb = (rocblas_handle)c->blas;
     RBCHECK(rocblas_set_stream(hb, s));
     DevBuf<double> dB, dZ64, dH, dC, dP, dErr;
     DevBuf<float> dA32, dZ32, dV32;
@@ -378,23 +413,23 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
         }
         const float* z32 = prec == 0 ? nullptr : dZ32.p;
         const double* z64 = prec == 0 ? dZ64.p : nullptr;
-        switch (E) {
-            case 4: launch_mgs<4>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
-            case 8: launch_mgs<8>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
-            case 16: launch_mgs<16>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
-            case 32: launch_mgs<32>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
-            default: launch_mgs<64>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+        switch (WE) {
+            case 1: launch_mgs_wide<1>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 2: launch_mgs_wide<2>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 4: launch_mgs_wide<4>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 8: launch_mgs_wide<8>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            default: launch_mgs_wide<16>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
         }
         HIPCHECK(hipGetLastError());
     }
     hipLaunchKernelGGL(kr_expm_e1, dim3(N), dim3(64), (size_t)(m * m + 2 * m) * sizeof(double), s, dH.p, dC.p, m);
     HIPCHECK(hipGetLastError());
-    switch (E) {
-        case 4: launch_combine<4>(s, N, V, dC.p, dP.p, dErr.p, m); break;
-        case 8: launch_combine<8>(s, N, V, dC.p, dP.p, dErr.p, m); break;
-        case 16: launch_combine<16>(s, N, V, dC.p, dP.p, dErr.p, m); break;
-        case 32: launch_combine<32>(s, N, V, dC.p, dP.p, dErr.p, m); break;
-        default: launch_combine<64>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+    switch (WE) {
+        case 1: launch_combine_wide<1>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+        case 2: launch_combine_wide<2>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+        case 4: launch_combine_wide<4>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+        case 8: launch_combine_wide<8>(s, N, V, dC.p, dP.p, dErr.p, m); break;
+        default: launch_combine_wide<16>(s, N, V, dC.p, dP.p, dErr.p, m); break;
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(e1, s));

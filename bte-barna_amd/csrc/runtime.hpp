@@ -79,6 +79,9 @@ struct Ctx {
     hipStream_t stream = nullptr;
     int cu_count = 0;
     size_t lds_per_block = 0;
+    // BLAS handle kept for the context's lifetime (created on first use)
+    void* blas = nullptr;
+    void (*blas_free)(void*) = nullptr;
 };
 
 struct Modes {
