@@ -1098,7 +1098,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
             b->nsrc++;
         }
     }
-    b->step_blocks = std::min(4 * c->cu_count, 2048);
+    b->step_blocks = std::min(2 * c->cu_count, 2048);
     b->emit_blocks = std::min(c->cu_count, 1024);
     b->nsegs = b->step_blocks + b->emit_blocks + b->nsrc;
     if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
