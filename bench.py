@@ -1,13 +1,16 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X deviational MC engine.
 
-Workload (one "step" of this benchmark = one full beRTAMC2D transient run):
-2000-mode synthetic material, synthetic energy-conserving collision operator
-(5% dense), propagator P = exp(5 ps * B) built on the GPU by the block-Krylov
-path (FP64, m = 30), 500 nm x 100 nm ribbon in 50 x-bins, 1 K pulse in bin
-0 giving ~1e6 deviational particles, 200 time steps of 5 ps with adiabatic
-diffuse walls and cancellation every 4 steps. Each timed step is an
-independent replica (seed = base + step), as used for sigma(T).
+Workload (one "step" of this benchmark = one full beRTAMC2D transient run
+of R independent replicas, R = 4 by default): 2000-mode synthetic material,
+synthetic energy-conserving collision operator (5% dense), propagator
+P = exp(5 ps * B) built on the GPU by the block-Krylov path (FP64, m = 30);
+per replica a 500 nm x 100 nm ribbon in 50 x-bins, 1 K pulse in bin 0 giving
+~1e6 deviational particles, 200 time steps of 5 ps with adiabatic diffuse
+walls and cancellation every 4 steps. The R replicas are disjoint copies of
+the ribbon in one geometry (they share no side), so one run advances R
+independent replicas together; sigma(T) comes from all replicas of all timed
+runs (seed = base + step).
 
 value = particle-events/s over the timed steps, where a particle-event of
 beRTAMC2D is one particle advected and scattered for one time step.
@@ -221,8 +224,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--cpu-particles", type=int, default=200_000)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-particles", type=int, default=1_000_000)
+    ap.add_argument("--cpu-steps", type=int, default=200)
+    ap.add_argument("--replicas", type=int, default=4, help="independent replicas advanced together per run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -246,7 +250,8 @@ def main():
     log("setup: material, collision operator, propagator")
     mat, B = W.config1_inputs()
     P, krep = pkg.build_P(ctx, B, 5.0, m=30, precision="fp64", arnoldi="textbook")
-    cfg = W.config1()
+    cfg = W.config1(replicas=args.replicas)
+    cfg1 = W.config1()
     modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
     geom = pkg.Geometry(ctx, cfg["rect"], cfg["kind"], cfg["Teq"], cfg["Tinit"])
     prop = pkg.Propagator(ctx, P)
@@ -276,8 +281,10 @@ def main():
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     # sigma(T) at the last step across the replicas (T = T0 + ed / Cv)
     cv = 1.6e6
-    Tlast = np.array([300.0 + o["props"][-1, :, 0] / cv for o in outs])
-    sigmaT = float((Tlast.std(0, ddof=1) / np.sqrt(len(outs))).max()) if len(outs) > 1 else None
+    nb = len(cfg1["rect"])
+    Tlast = np.array([300.0 + o["props"][-1, r * nb:(r + 1) * nb, 0] / cv for o in outs
+                      for r in range(args.replicas)])
+    sigmaT = float((Tlast.std(0, ddof=1) / np.sqrt(len(Tlast))).max()) if len(Tlast) > 1 else None
     conserved = all(bool(((o["npart"] + o["pending"]) == o["npart"][0]).all()) for o in outs)
     line = {
         "metric": "deviational-MC particle-events/s at fixed σ(T); Krylov propagator build time",
@@ -294,7 +301,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": "beRTAMC2D transient: 2000-mode synthetic material, 5%-dense synthetic Omega, "
                                "P=exp(-Omega*5ps) (GPU Krylov FP64 m=30), 500x100 nm ribbon, 50 x-bins, 1 K pulse "
-                               "in bin 0, ~1e6 particles, 200 steps of 5 ps; one bench step = one full run",
+                               "in bin 0, ~1e6 particles, 200 steps of 5 ps; one bench step = one full run of "
+                               f"{args.replicas} independent replicas advanced together",
+                   "replicas_per_run": args.replicas,
                    "initial_particles": int(outs[0]["npart"][0]), "time_steps": int(outs[0]["nsteps"]),
                    "particle_events_per_run": int(outs[0]["particle_steps"]),
                    "max_sigma_T_K_last_step": sigmaT, "count_conserved": conserved,
@@ -306,7 +315,7 @@ def main():
     }
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
-        line["cpu_baseline"] = cpu_baseline(pkg, mat, P, cfg, args.cpu_particles, args.cpu_steps)
+        line["cpu_baseline"] = cpu_baseline(pkg, mat, P, cfg1, args.cpu_particles, args.cpu_steps)
         log("cpu baseline done")
     if not args.no_extras:
         log("extra: RTAMC2D config ...")

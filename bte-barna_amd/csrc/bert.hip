@@ -92,8 +92,8 @@ __device__ inline void seed_stream(Pcg32& r, uint64_t key, uint64_t pid) {
 // The particle kernels share one dynamic LDS block: segment prefix sums,
 // wall counters, box sums and, when small enough (GL instantiations), the
 // geometry tables. Offsets are set on the host.
-#define BERT_LDS_BOXES 128
-#define BERT_GEO_LDS_BYTES 24576
+#define BERT_LDS_BOXES 512
+#define BERT_GEO_LDS_BYTES 49152
 extern __shared__ __align__(16) unsigned char bert_dyn[];
 template <bool GL>
 __device__ inline void geo_stage(const BertParams& P, const DevBox*& boxes, const DevContact*& cts) {
@@ -351,7 +351,7 @@ __device__ inline void scatter_out(const BertParams& P, int32_t* H, double* sbox
 
 // Wall counters are accumulated per workgroup in LDS and flushed with one
 // global atomic per touched side (the walls of a few boxes take most hits).
-#define BERT_EB_LDS 1024
+#define BERT_EB_LDS 2048
 __device__ inline int32_t* eb_lds_begin(const BertParams& P, int32_t* Eb, int nb4) {
     if (!P.eb_lds) return Eb;
     int32_t* sEb = (int32_t*)(bert_dyn + P.lds_eb);

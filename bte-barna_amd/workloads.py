@@ -77,11 +77,20 @@ def config1_inputs(nmodes=2000, seed=1, density=0.05):
     return mat, B
 
 
-def config1(nsteps=200, dt=5.0, nparticles=1_000_000, nbins=50, seed=1):
+def config1(nsteps=200, dt=5.0, nparticles=1_000_000, nbins=50, seed=1, replicas=1, gap=100.0):
     """beRTAMC2D transient: 500 nm x 100 nm ribbon, 50 x-bins, 1 K pulse in
     bin 0, all sides adiabatic-diffuse, Eeff set for ~nparticles initial
-    particles, dt = 5 ps, nsteps steps."""
-    rect, kind, teq, tin = ribbon(500.0, 100.0, nbins, 300.0, Tinit_first=301.0)
+    particles per ribbon, dt = 5 ps, nsteps steps. replicas > 1 places that
+    many copies of the ribbon side by side in y, gap nm apart: they share no
+    side, so one run simulates independent replicas (boxes r*nbins ...
+    (r+1)*nbins - 1 belong to replica r)."""
+    parts = [ribbon(500.0, 100.0, nbins, 300.0, Tinit_first=301.0) for _ in range(replicas)]
+    for r, pr in enumerate(parts):
+        pr[0][:, 2:] += r * (100.0 + gap)
+    rect = np.concatenate([p[0] for p in parts])
+    kind = np.concatenate([p[1] for p in parts])
+    teq = np.concatenate([p[2] for p in parts])
+    tin = np.concatenate([p[3] for p in parts])
     return dict(rect=rect, kind=kind, Teq=teq, Tinit=tin, thickness=1.0, dt=dt,
                 maxtime=dt * (nsteps - 1), max_steps=nsteps, Efactor=nparticles, Tmax=301.0,
                 Tmin=300.0, seed=seed)

@@ -134,3 +134,26 @@ def test_krylov_precisions_on_golden_operator(pkg, ctx, prec, tol):
     P, rep = pkg.build_P(ctx, A, 1.0, m=40, precision=prec, arnoldi="textbook")
     assert np.abs(P - E).sum(0).max() < tol
     np.testing.assert_allclose(P.sum(0), 1.0, atol=1e-12)
+
+
+def test_bert_replicas_in_one_geometry_are_isolated(pkg, ctx, config1):
+    """Ribbon copies side by side in y share no side: a copy without an
+    initial pulse stays empty while the pulsed copy evolves, so a run over R
+    copies is R independent replicas."""
+    mat, B, P, _ = config1
+    W = pkg.workloads
+    c = W.config1(replicas=2)
+    tin = c["Tinit"].copy()
+    tin[50:] = 300.0  # second copy: no pulse
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, c["rect"], c["kind"], c["Teq"], tin)
+    nc, nbr, _ = geom.contacts()
+    for b in range(100):  # no contact between the copies
+        assert all((n >= 50) == (b >= 50) for n in nbr[b, :nc[b]])
+    prop = pkg.Propagator(ctx, P)
+    o = pkg.bertamc2d(ctx, modes, geom, prop, thickness=1.0, dt=5.0, maxtime=c["maxtime"], max_steps=40,
+                      Efactor=200000, Tmax=301.0, Tmin=300.0, seed=5)
+    assert np.abs(o["props"][:, 50:, :]).max() == 0.0
+    assert np.abs(o["props"][-1, :50, 0]).sum() > 0
+    tot = o["npart"] + o["pending"]
+    assert (tot == tot[0]).all()
