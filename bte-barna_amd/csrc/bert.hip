@@ -181,6 +181,24 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
     return false;
 }
 
+// The first propagator draw of a particle depends only on its mode, which
+// advection does not change, so it is drawn and its alias entry loaded before
+// advection: the gather overlaps the advection arithmetic. Later draws of the
+// same particle (children of negative entries) follow in stream order.
+struct Pick0 {
+    uint32_t k, u;
+    uint2 e;
+};
+__device__ inline Pick0 first_pick(const BertParams& P, Pcg32& rng, int mode) {
+    Pick0 f{0u, 0u, make_uint2(0u, 0u)};
+    const int col = mode - P.ngamma;
+    if (P.rta || col < 0 || col >= P.ncol) return f;
+    f.k = rng.bounded((uint32_t)P.ncol);
+    f.u = rng.next();
+    f.e = P.palias[(size_t)col * P.ncol + f.k];
+    return f;
+}
+
 // A child written in this step enters the (box, mode) histogram when that
 // step feeds a cancellation or a frame (H not null) and the box sums behind
 // calculate_prop (sbox in LDS, not null when the boxes fit there).
@@ -200,9 +218,9 @@ __device__ inline void tally_child(const BertParams& P, int32_t* H, double* sbox
 // them when there is more than one. Children of a sign flip re-queue the
 // parent twice (1189-1192).
 template <bool EMIT>
-__device__ inline int scatter(const BertParams& P, int32_t* H, double* sbox, Pcg32 rng, int mode, int sign,
-                              int box, BPart* out, uint64_t slot, uint64_t limit, double x, double y, uint64_t pid,
-                              uint32_t step, int& nm0, int& ns0) {
+__device__ inline int scatter(const BertParams& P, int32_t* H, double* sbox, Pcg32 rng, const Pick0& f0, int mode,
+                              int sign, int box, BPart* out, uint64_t slot, uint64_t limit, double x, double y,
+                              uint64_t pid, uint32_t step, int& nm0, int& ns0) {
     if (P.rta) {
         double R = rng.uniform();
         int nm = mode;
@@ -232,7 +250,15 @@ __device__ inline int scatter(const BertParams& P, int32_t* H, double* sbox, Pcg
     while (pending > 0 && nchild < P.max_children) {
         pending--;
         bool neg;
-        int k = (int)alias_pick(tab, (uint32_t)P.ncol, rng, neg);
+        int k;
+        if (nchild == 0) {
+            // alias_pick with the draws and entry taken before advection
+            const bool own = f0.u < f0.e.x;
+            neg = own ? ((f0.e.y >> 31) & 1u) : ((f0.e.y >> 30) & 1u);
+            k = own ? (int)f0.k : (int)(f0.e.y & 0x3fffffffu);
+        } else {
+            k = (int)alias_pick(tab, (uint32_t)P.ncol, rng, neg);
+        }
         int nm = k + P.ngamma;
         int ns = neg ? -sign : sign;
         if (nchild == 0) {
@@ -331,11 +357,11 @@ __device__ inline void seg_close(const BertParams& P, uint32_t* segout, int myse
 
 // count the children, reserve their slots in the workgroup's segment and
 // write them; a single child (the common case) is written from pass 0's result
-__device__ inline void scatter_out(const BertParams& P, int32_t* H, double* sbox, Pcg32& rng, bool alive, int mode,
-                                   int sign, int box, double x, double y, uint64_t pid, uint32_t step,
-                                   uint32_t* lds_count, int myseg) {
+__device__ inline void scatter_out(const BertParams& P, int32_t* H, double* sbox, Pcg32& rng, const Pick0& f0,
+                                   bool alive, int mode, int sign, int box, double x, double y, uint64_t pid,
+                                   uint32_t step, uint32_t* lds_count, int myseg) {
     int nm0 = 0, ns0 = 1;
-    int c = alive ? scatter<false>(P, H, sbox, rng, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
+    int c = alive ? scatter<false>(P, H, sbox, rng, f0, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
     uint64_t limit;
     uint64_t slot = wave_reserve(lds_count, myseg, P.segcap, c, P.flags, limit);
     if (c == 1) {
@@ -345,7 +371,7 @@ __device__ inline void scatter_out(const BertParams& P, int32_t* H, double* sbox
             P.pout[slot] = q;
         }
     } else if (c > 1) {
-        scatter<true>(P, H, sbox, rng, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
+        scatter<true>(P, H, sbox, rng, f0, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
     }
 }
 
@@ -422,6 +448,7 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
         int box = 0, sign = 1, mode = 0;
         bool alive = false;
         Pcg32 rng;
+        Pick0 f0{0u, 0u, make_uint2(0u, 0u)};
         if (valid) {
             while (sg < P.nsegs - 1 && i >= pre[sg]) sg++;
             const uint32_t off = i - (sg > 0 ? pre[sg - 1] : 0u);
@@ -443,12 +470,13 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
             }
             if (alive) {
                 seed_stream(rng, skey, p.pid);
+                f0 = first_pick(P, rng, mode);
                 const double2 v = P.vel[mode];
                 const double2 iv = P.ivel[mode];
                 alive = advect(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
             }
         }
-        scatter_out(P, H, sbox, rng, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
+        scatter_out(P, H, sbox, rng, f0, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -586,6 +614,7 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
         const uint32_t wfirst = base + (threadIdx.x & ~63u);
         while (bw < nb4 - 1 && wfirst >= pre[bw]) bw++;
         Pcg32 rng;
+        Pick0 f0{0u, 0u, make_uint2(0u, 0u)};
         int mode = 0, bx = 0, sign = 1;
         double x = 0, y = 0;
         bool alive = false;
@@ -618,9 +647,10 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
             const double2 v = P.vel[mode];
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
+            f0 = first_pick(P, rng, mode);
             alive = advect(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
         }
-        scatter_out(P, H, sbox, rng, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -660,6 +690,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     for (int k = threadIdx.x; k - (int)(threadIdx.x & 63) < nloop; k += blockDim.x) {
         bool valid = k < ng;
         Pcg32 rng;
+        Pick0 f0{0u, 0u, make_uint2(0u, 0u)};
         int mode = 0, bx = S.box;
         double x = 0, y = 0;
         bool alive = false;
@@ -681,9 +712,10 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
             const double2 v = P.vel[mode];
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
+            f0 = first_pick(P, rng, mode);
             alive = advect(P, P.boxes, P.cts, x, y, bx, S.sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
         }
-        scatter_out(P, H, sbox, rng, alive, mode, S.sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out(P, H, sbox, rng, f0, alive, mode, S.sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
