@@ -101,10 +101,11 @@ class ModeTable(_Handle):
 
 class Geometry(_Handle):
     """Rectangular boxes: rect rows (xmin, xmax, ymin, ymax) [nm];
-    kind 0 box, 1 reservoir."""
+    kind 0 box, 1 reservoir, 2 periodic; periodic = {box: (target, tx, ty)}
+    gives each periodic box its translation."""
     _destroy = "bte_geom_destroy"
 
-    def __init__(self, ctx, rect, kind, Teq, Tinit=None):
+    def __init__(self, ctx, rect, kind, Teq, Tinit=None, periodic=None):
         super().__init__()
         self.ctx = ctx
         self.rect = _f64(rect).reshape(-1, 4)
@@ -115,6 +116,8 @@ class Geometry(_Handle):
         check(lib().bte_geom_create_rects(ctx.h, self.nbox, _dp(self.rect),
                                           self.kind.ctypes.data_as(C.POINTER(C.c_int32)), _dp(self.Teq),
                                           None if self.Tinit is None else _dp(self.Tinit), C.byref(self.h)))
+        for b, (t, tx, ty) in (periodic or {}).items():
+            check(lib().bte_geom_set_periodic(self.h, int(b), int(t), float(tx), float(ty)))
 
     def contacts(self, maxc=8):
         nc = np.zeros(self.nbox, np.int32)

@@ -86,6 +86,7 @@ SIGNATURES = {
     "bte_modes_destroy": (C.c_int, [vp]),
     "bte_geom_create_rects": (C.c_int, [vp, C.c_int, dp, ip32, dp, dp, pvp]),
     "bte_geom_destroy": (C.c_int, [vp]),
+    "bte_geom_set_periodic": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double]),
     "bte_geom_contacts": (C.c_int, [vp, C.c_int, ip32, ip32, dp]),
     "bte_rta_create": (C.c_int, [vp, vp, vp, C.POINTER(RtaCfg), pvp]),
     "bte_rta_execute": (C.c_int, [vp, dp, dp, dp, C.POINTER(RtaReport)]),

@@ -60,9 +60,7 @@ static int run(int argc, char** argv) {
     check(bte_modes_create(ctx, M.n, M.nq, M.omega.data(), M.vx.data(), M.vy.data(), M.w.data(), M.V, M.lattz,
                            &modes),
           "mode table");
-    bte_geom* geom = nullptr;
-    check(bte_geom_create_rects(ctx, B.n(), B.rect.data(), B.kind.data(), B.Teq.data(), B.Tinit.data(), &geom),
-          "geometry");
+    bte_geom* geom = make_geom(ctx, B);
     const double dt = tim->num("dt");
     bte_prop* prop = nullptr;
     if (!rta) {

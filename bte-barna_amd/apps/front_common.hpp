@@ -29,6 +29,8 @@ inline void check(int rc, const char* what) {
 struct Boxes {
     std::vector<double> rect;  // xmin xmax ymin ymax per box
     std::vector<int32_t> kind;
+    std::vector<int32_t> target;       // periodic boxes: <Translate_to id>
+    std::vector<double> tx, ty;        // and their translation
     std::vector<double> Teq, Tinit;
     std::vector<std::string> material;
     int n() const { return (int)kind.size(); }
@@ -61,8 +63,13 @@ inline Boxes read_geometry(const std::string& fn) {
         if (!rect)
             throw std::runtime_error("box " + std::to_string(id) +
                                      ": only axis-aligned rectangles are supported by the GPU engine");
-        if (b->child("Translate_to"))
-            throw std::runtime_error("box " + std::to_string(id) + ": periodic boxes (<Translate_to>) are not supported");
+        int tgt = -1;
+        double ptx = 0, pty = 0;
+        if (const xml::Node* tr = b->child("Translate_to")) {
+            tgt = (int)tr->num("id");
+            std::stringstream ts(tr->text);
+            if (!(ts >> ptx >> pty)) throw std::runtime_error("box " + std::to_string(id) + ": bad <Translate_to>");
+        }
         if (const xml::Node* th = b->child("theta"))
             if (th->num("angle") != 0.)
                 throw std::runtime_error("box " + std::to_string(id) + ": rotated materials are not supported");
@@ -70,7 +77,10 @@ inline Boxes read_geometry(const std::string& fn) {
         if (!ic) throw std::runtime_error("box " + std::to_string(id) + ": no <initCnd>");
         const double Teq = ic->num("Teq");
         g.rect.insert(g.rect.end(), {x0, x1, y0, y1});
-        g.kind.push_back(b->child("Reservoir") ? 1 : 0);
+        g.kind.push_back(tgt >= 0 ? 2 : (b->child("Reservoir") ? 1 : 0));
+        g.target.push_back(tgt);
+        g.tx.push_back(ptx);
+        g.ty.push_back(pty);
         g.Teq.push_back(Teq);
         g.Tinit.push_back(ic->num("Tinit", Teq));
         const xml::Node* mid = b->child("MaterialID");
@@ -109,6 +119,16 @@ struct Material {
         return kB * s / (nq * V);
     }
 };
+
+// the geometry handle with periodic translations applied
+inline bte_geom* make_geom(bte_ctx* ctx, const Boxes& B) {
+    bte_geom* geom = nullptr;
+    check(bte_geom_create_rects(ctx, B.n(), B.rect.data(), B.kind.data(), B.Teq.data(), B.Tinit.data(), &geom),
+          "geometry");
+    for (int i = 0; i < B.n(); i++)
+        if (B.kind[i] == 2) check(bte_geom_set_periodic(geom, i, B.target[i], B.tx[i], B.ty[i]), "periodic box");
+    return geom;
+}
 
 inline Material read_material(const xml::Node& m) {
     Material M;

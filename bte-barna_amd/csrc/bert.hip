@@ -42,6 +42,8 @@ struct BertParams {
     const double* w;
     const DevBox* boxes;
     const DevContact* cts;
+    const double4* cseg;     // contact segments (periodic translation)
+    const double2* ptrans;   // periodic translation per box
     const uint2* palias;     // propagator alias, ncol x ncol column-major
     const uint2* rta_alias;  // BE_derivative alias per box-temperature slot
     const int32_t* box_rta_slot;
@@ -62,7 +64,7 @@ struct BertParams {
     int32_t* wguide;         // [nbox*4][BERT_GUIDE] first cumulative index per key
     uint32_t* wcount;        // [nbox*4] particles to emit this step
     int32_t* wsign;          // [nbox*4] their sign
-    uint32_t* flags;         // [0] overflow, [1] bad mode, [2] stuck
+    uint32_t* flags;         // [0] overflow, [1] bad mode, [2] stuck, [3] failed translation
     unsigned long long* stats;  // [0] step-kernel particle-steps, [1] emitted
     uint64_t cap;
     uint32_t segcap;         // capacity of one segment (cap / nsegs)
@@ -119,6 +121,16 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
     for (int it = 0; it < 4096; it++) {
         const DevBox B = boxes[box];
         if (B.kind == 1) return false;  // reservoir: deleted
+        if (B.kind == 2) {
+            // periodic box: translate and continue the flight (beyondRTA_MC2d.cpp:729-737)
+            const int nb = translate_rect(boxes, cts, P.cseg, P.ptrans, box, x, y, vx, vy, rng);
+            if (nb < 0) {
+                atomicOr(P.flags + 3, 1u);
+                return false;
+            }
+            box = nb;
+            continue;
+        }
         double nx = x + dt * vx, ny = y + dt * vy;
         if (box_inside(B, nx, ny)) {
             x = nx;
@@ -912,6 +924,8 @@ static BertParams make_params(Bert* b) {
     P.w = b->m->d_w.p;
     P.boxes = b->g->d_boxes.p;
     P.cts = b->g->d_contacts.p;
+    P.cseg = b->g->d_cseg.p;
+    P.ptrans = b->g->d_ptrans.p;
     P.palias = b->p ? b->p->d_alias.p : nullptr;
     P.rta_alias = b->d_rta_alias.p;
     P.box_rta_slot = b->d_rta_slot.p;
@@ -1035,6 +1049,9 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     if (p && p->ngamma + p->ncol != m->n) throw Error(BTE_EINVAL, "propagator size does not match the mode table");
     if (!(cfg->dt > 0) || !(cfg->thickness > 0)) throw Error(BTE_EINVAL, "bad beRTA configuration");
     if ((size_t)g->boxes.size() * 4 > BERT_EMIT_MAXB) throw Error(BTE_EINVAL, "more than 1024 boxes");
+    for (size_t i = 0; i < g->boxes.size(); i++)
+        if (g->dboxes[i].kind == 2 && g->dboxes[i].tgt < 0)
+            throw Error(BTE_EGEOM, "periodic box " + std::to_string(i) + " has no translation");
     // gradient sources (calculate_gradientT, geometry_2d.cpp:524-576) vanish
     // only when every regular box has the same Teq
     {
@@ -1370,6 +1387,7 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     if (flags[0]) throw Error(BTE_ECAPACITY, "particle buffer capacity exceeded; raise capacity");
     if (flags[1]) throw Error(BTE_EINVAL, "a particle reached a mode outside the propagator");
     if (flags[2]) throw Error(BTE_EGEOM, "advection or branching exceeded its iteration bound");
+    if (flags[3]) throw Error(BTE_EGEOM, "Error in translation, no box found");
     if (rep) {
         rep->nsteps = istep;
         rep->Eeff = b->Eeff;

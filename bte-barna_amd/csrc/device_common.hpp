@@ -78,9 +78,9 @@ __device__ inline double bose_einstein(double omega, double T) {
 // 3 right; outward normals (-1,0),(0,-1),(0,1),(1,0).
 struct DevBox {
     double xmin, xmax, ymin, ymax;
-    int32_t kind;      // 0 regular, 1 reservoir
+    int32_t kind;      // 0 regular, 1 reservoir, 2 periodic
     int32_t c0, c1;    // contacts [c0, c1)
-    int32_t pad;
+    int32_t tgt;       // periodic: the box the translation leads to
     // per side: the one box sharing the whole side (>= 0), SIDE_VOID when no
     // box shares any part of it, SIDE_MIXED otherwise
     int32_t side_nbr[4];
@@ -210,6 +210,106 @@ __device__ __host__ inline int pick_void_side(const DevBox* boxes, const DevCont
     }
     if (c0 < 0) return first;
     return (c1 < 0 || (rnd & 1u) == 0) ? c0 : c1;
+}
+
+// closed-segment intersection of a0->a1 with b0->b1 (parallel segments do
+// not intersect); the coordinate on an axis-aligned b is snapped to it
+__device__ __host__ inline bool seg_cross(double a0x, double a0y, double a1x, double a1y, double b0x, double b0y,
+                                          double b1x, double b1y, double& px, double& py) {
+    const double dx = a1x - a0x, dy = a1y - a0y, ex = b1x - b0x, ey = b1y - b0y;
+    const double den = dx * ey - dy * ex;
+    if (den == 0.) return false;
+    const double wx = b0x - a0x, wy = b0y - a0y;
+    const double t = (wx * ey - wy * ex) / den;
+    const double u = (wx * dy - wy * dx) / den;
+    if (t < 0. || t > 1. || u < 0. || u > 1.) return false;
+    px = a0x + t * dx;
+    py = a0y + t * dy;
+    if (ex == 0.) px = b0x;
+    if (ey == 0.) py = b0y;
+    return true;
+}
+
+// geometry_2d::translate (include/geometry_2d.hpp:366-494) for rectangles.
+// A particle in periodic box id at (x, y) moves along 1.0005 T (T its
+// translation) to the first border (left, bottom, top, right, each extended
+// by 1e-6 of its length) of the target box that this segment crosses, then
+// into the regular or reservoir-free neighbour of the target box that holds
+// that point: through the shared border (cseg: contact segments extended by
+// 1e-6), the point itself, the point + 1e-6 v/|v| or the point - 1e-6 T/|T|.
+// Several candidates: those inside after + 1e-6 v, one chosen at random.
+// Returns the new box, or -1 if no box is found (a geometry error).
+template <class Rng>
+__device__ __host__ inline int translate_rect(const DevBox* boxes, const DevContact* cts, const double4* cseg,
+                                              const double2* ptrans, int id, double& x, double& y, double vx,
+                                              double vy, Rng& rng) {
+    const DevBox b = boxes[id];
+    const double2 T = ptrans[id];
+    const double Tx = 1.0005 * T.x, Ty = 1.0005 * T.y;
+    const int tib = b.tgt;
+    if (tib < 0) return -1;
+    const DevBox tb = boxes[tib];
+    const double s0x = x, s0y = y, s1x = x + Tx, s1y = y + Ty;
+    double px = 0., py = 0.;
+    bool found = false;
+    for (int k = 0; k < 4 && !found; k++) {
+        double p1x, p1y, p2x, p2y;
+        if (k == 0) { p1x = tb.xmin; p1y = tb.ymin; p2x = tb.xmin; p2y = tb.ymax; }
+        else if (k == 1) { p1x = tb.xmin; p1y = tb.ymin; p2x = tb.xmax; p2y = tb.ymin; }
+        else if (k == 2) { p1x = tb.xmin; p1y = tb.ymax; p2x = tb.xmax; p2y = tb.ymax; }
+        else { p1x = tb.xmax; p1y = tb.ymax; p2x = tb.xmax; p2y = tb.ymin; }
+        const double ndx = p2x - p1x, ndy = p2y - p1y;
+        found = seg_cross(s0x, s0y, s1x, s1y, p1x - 1.0e-6 * ndx, p1y - 1.0e-6 * ndy, p2x + 1.0e-6 * ndx,
+                          p2y + 1.0e-6 * ndy, px, py);
+    }
+    if (!found) return -1;
+    const double tn = sqrt(T.x * T.x + T.y * T.y), vn = sqrt(vx * vx + vy * vy);
+    const double pTx = px - 1.0e-6 * T.x / tn, pTy = py - 1.0e-6 * T.y / tn;
+    const double pVx = px + 1.0e-6 * vx / vn, pVy = py + 1.0e-6 * vy / vn;
+    int nb = 0, first = -1, chosen = -1, nkeep = 0;
+    double fx = 0., fy = 0., cx = 0., cy = 0.;
+    // two passes over the contacts: count the candidates that stay inside
+    // after + 1e-6 v, then take the r-th of them
+    uint32_t r = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        int seen = 0;
+        for (int c = tb.c0; c < tb.c1; c++) {
+            const DevContact ct = cts[c];
+            const int ibc = ct.box;
+            const DevBox nbx = boxes[ibc];
+            if (ibc == tib || nbx.kind != 0) continue;
+            const double4 sg = cseg[c];
+            double ix = 0., iy = 0.;
+            const bool here_i = seg_cross(s0x, s0y, s1x, s1y, sg.x, sg.y, sg.z, sg.w, ix, iy);
+            const bool here_T = box_inside(nbx, px, py);
+            const bool here_Tv = box_inside(nbx, pVx, pVy);
+            const bool here_TT = box_inside(nbx, pTx, pTy);
+            if (!(here_i || here_T || here_Tv || here_TT)) continue;
+            double rx, ry;
+            if (here_i) { rx = ix; ry = iy; }
+            else if (here_T) { rx = px; ry = py; }
+            else if (here_Tv) { rx = pVx; ry = pVy; }
+            else { rx = pTx; ry = pTy; }
+            if (pass == 0) {
+                if (nb == 0) { first = ibc; fx = rx; fy = ry; }
+                nb++;
+                if (box_inside(nbx, rx + 1.0e-6 * vx, ry + 1.0e-6 * vy)) nkeep++;
+            } else if (box_inside(nbx, rx + 1.0e-6 * vx, ry + 1.0e-6 * vy)) {
+                if ((uint32_t)seen == r) { chosen = ibc; cx = rx; cy = ry; break; }
+                seen++;
+            }
+        }
+        if (pass == 0) {
+            if (nb == 0) return -1;
+            if (nb == 1) { x = fx; y = fy; return first; }
+            if (nkeep == 0) return -1;
+            r = rng.bounded((uint32_t)nkeep);
+        }
+    }
+    if (chosen < 0) return -1;
+    x = cx;
+    y = cy;
+    return chosen;
 }
 
 }  // namespace bte

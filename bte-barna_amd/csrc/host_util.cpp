@@ -308,8 +308,7 @@ int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int3
         RectBox& b = g->boxes[i];
         b.xmin = rect[4 * i]; b.xmax = rect[4 * i + 1]; b.ymin = rect[4 * i + 2]; b.ymax = rect[4 * i + 3];
         if (!(b.xmax > b.xmin && b.ymax > b.ymin)) { delete g; throw Error(BTE_EGEOM, "degenerate box"); }
-        if (kind[i] == 2) { delete g; throw Error(BTE_EGEOM, "periodic boxes are not supported"); }
-        if (kind[i] != 0 && kind[i] != 1) { delete g; throw Error(BTE_EINVAL, "bad box kind"); }
+        if (kind[i] < 0 || kind[i] > 2) { delete g; throw Error(BTE_EINVAL, "bad box kind"); }
         b.kind = kind[i];
         b.Teq = Teq[i];
         b.Tinit = Tinit ? Tinit[i] : Teq[i];
@@ -355,9 +354,16 @@ int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int3
         d.xmin = b.xmin; d.xmax = b.xmax; d.ymin = b.ymin; d.ymax = b.ymax;
         d.kind = b.kind;
         d.c0 = (int32_t)g->dcontacts.size();
-        for (auto& hc : g->contacts[i]) g->dcontacts.push_back(DevContact{hc.box, 0, hc.npx, hc.npy});
+        for (auto& hc : g->contacts[i]) {
+            g->dcontacts.push_back(DevContact{hc.box, 0, hc.npx, hc.npy});
+            // the contact border as a segment extended by 1e-6 at both ends
+            const double L = hc.length();
+            const double ux = L > 0 ? (hc.p2x - hc.p1x) / L : 0., uy = L > 0 ? (hc.p2y - hc.p1y) / L : 0.;
+            g->cseg.push_back(make_double4(hc.p1x - 1.0e-6 * ux, hc.p1y - 1.0e-6 * uy, hc.p2x + 1.0e-6 * ux,
+                                           hc.p2y + 1.0e-6 * uy));
+        }
         d.c1 = (int32_t)g->dcontacts.size();
-        d.pad = 0;
+        d.tgt = -1;
         // side classification for the neighbour fast path
         for (int sd = 0; sd < 4; sd++) {
             const bool xside = (sd == 0 || sd == 3);
@@ -377,11 +383,31 @@ int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int3
             d.side_nbr[sd] = touching == 0 ? SIDE_VOID : (touching == 1 && full >= 0 ? full : SIDE_MIXED);
         }
     }
+    g->ptrans.assign(nbox, make_double2(0., 0.));
     HIPCHECK(hipSetDevice(c->device));
     g->d_boxes.upload(g->dboxes, c->stream);
-    g->d_contacts.upload(g->dcontacts, c->stream);
+    g->d_contacts.upload(g->dcontacts.empty() ? std::vector<DevContact>(1) : g->dcontacts, c->stream);
+    g->d_cseg.upload(g->cseg.empty() ? std::vector<double4>(1) : g->cseg, c->stream);
+    g->d_ptrans.upload(g->ptrans, c->stream);
     HIPCHECK(hipStreamSynchronize(c->stream));
     *out = reinterpret_cast<bte_geom*>(g);
+    BTE_GUARD_END
+}
+
+int bte_geom_set_periodic(bte_geom* gg, int box, int target, double tx, double ty) {
+    BTE_GUARD_BEGIN
+    auto* g = reinterpret_cast<Geom*>(gg);
+    if (!g || box < 0 || box >= (int)g->boxes.size()) throw Error(BTE_EINVAL, "bad box id");
+    if (g->boxes[box].kind != 2) throw Error(BTE_EINVAL, "only periodic boxes (kind 2) take a translation");
+    // translate() (include/geometry_2d.hpp:379-386)
+    if (target < 0 || target >= (int)g->boxes.size() || almost_equal(std::hypot(tx, ty), 0.))
+        throw Error(BTE_EGEOM, "Error: Bad id for mapping of periodic cells");
+    g->dboxes[box].tgt = target;
+    g->ptrans[box] = make_double2(tx, ty);
+    HIPCHECK(hipSetDevice(g->ctx->device));
+    g->d_boxes.upload(g->dboxes, g->ctx->stream);
+    g->d_ptrans.upload(g->ptrans, g->ctx->stream);
+    HIPCHECK(hipStreamSynchronize(g->ctx->stream));
     BTE_GUARD_END
 }
 

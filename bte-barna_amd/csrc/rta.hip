@@ -39,6 +39,8 @@ struct RtaParams {
     const uint2* scat;     // BE_derivative alias per run (n each)
     const DevBox* boxes;
     const DevContact* cts;
+    const double4* cseg;    // contact segments (periodic translation)
+    const double2* ptrans;  // periodic translation per box
     const RtaGen* gens;
     const uint2* gen_sel;  // alias over each run's generators
     const uint2* gen_tab;  // per generator, n entries
@@ -52,7 +54,7 @@ struct RtaParams {
     double* jx;
     double* jy;
     double* dgz;           // [4][cells] score-weighted gz per side (gradient runs)
-    unsigned long long* counters;  // events, crossings, next id, stuck, bad mirror
+    unsigned long long* counters;  // events, crossings, next id, stuck, bad mirror, bad translation
     uint64_t total;        // particles over all runs
     uint64_t cells;
     unsigned long long max_iters;
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
     double x = 0, y = 0, t = 0, dtrem = 0, wscale = 1.0;
     int mode = 0, box = 0, sign = 1, run = 0;
     bool alive = false, need_flight = true;
-    unsigned long long events = 0, crossings = 0, stuck = 0, badmirror = 0;
+    unsigned long long events = 0, crossings = 0, stuck = 0, badmirror = 0, badtrans = 0;
     unsigned long long budget = 0;
     int still = 0;        // consecutive events without time advance
     double tlast = -1.0;
@@ -196,6 +198,17 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
             need_flight = false;
         }
         const DevBox B = P.boxes[box];
+        if (B.kind == 2) {
+            // periodic box: translate, keep the remaining flight (RTA_MC2D.cpp:1034-1043)
+            const int nb = translate_rect(P.boxes, P.cts, P.cseg, P.ptrans, box, x, y, md.x, md.y, rng);
+            if (nb < 0) {
+                badtrans++;
+                alive = false;
+                continue;
+            }
+            box = nb;
+            continue;
+        }
         if (B.kind == 1) {  // reservoir: absorbed (RTA_MC2D.cpp:1046-1053)
             events++;
             alive = false;
@@ -306,6 +319,7 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
     atomicAdd(P.counters + 1, crossings);
     if (stuck) atomicAdd(P.counters + 3, stuck);
     if (badmirror) atomicAdd(P.counters + 4, badmirror);
+    if (badtrans) atomicAdd(P.counters + 5, badtrans);
 }
 
 // clears the tallies and counters (a kernel rather than a memset so that the
@@ -338,6 +352,8 @@ struct RtaBatch {
     DevBuf<int4> d_lam_info;
     DevBuf<DevBox> d_boxes;
     DevBuf<DevContact> d_cts;
+    DevBuf<double4> d_cseg;
+    DevBuf<double2> d_ptrans;
     DevBuf<double> d_timegrid, d_tally, d_dgz;
     DevBuf<uint64_t> d_box_cell;
     DevBuf<RtaRunDev> d_runs, d_trial_runs;
@@ -442,6 +458,8 @@ static RtaParams batch_params(RtaBatch* b, const RtaRunDev* runs, uint64_t total
     P.scat = b->d_scat.p;
     P.boxes = b->d_boxes.p;
     P.cts = b->d_cts.p;
+    P.cseg = b->d_cseg.p;
+    P.ptrans = b->d_ptrans.p;
     P.gens = b->d_gens.p;
     P.gen_sel = b->d_gen_sel.p;
     P.gen_tab = b->d_gen_tab.p;
@@ -508,6 +526,8 @@ static void batch_create(RtaBatch* b) {
     const double vuc = m->V * b->thickness / m->lattz;
     std::vector<DevBox> boxes;
     std::vector<DevContact> cts;
+    std::vector<double4> cseg;
+    std::vector<double2> ptrans;
     std::vector<uint64_t> box_cell;
     std::vector<RtaGen> gens;
     std::vector<uint2> gsel, gtab, scat;
@@ -546,6 +566,10 @@ static void batch_create(RtaBatch* b) {
         b->cell0.push_back(b->cells);
         for (int i = 0; i < nb; i++) {
             DevBox d = g->dboxes[i];
+            if (d.kind == 2 && d.tgt < 0)
+                throw Error(BTE_EGEOM, "periodic box " + std::to_string(i) + " has no translation");
+            if (d.tgt >= 0) d.tgt += bo;
+            ptrans.push_back(g->ptrans[i]);
             d.c0 += co;
             d.c1 += co;
             for (int k = 0; k < 4; k++)
@@ -557,6 +581,7 @@ static void batch_create(RtaBatch* b) {
             ct.box += bo;
             cts.push_back(ct);
         }
+        cseg.insert(cseg.end(), g->cseg.begin(), g->cseg.end());
         b->cells += (uint64_t)nb * nt;
         // BE_derivative sampler at the run temperature (sampling.cpp:47-68)
         for (int i = 0; i < n; i++) pmf[i] = m->omega[i] <= 0. ? 0. : h_be_kernel(m->omega[i], D.T0) * m->w[i];
@@ -597,6 +622,8 @@ static void batch_create(RtaBatch* b) {
     }
     b->d_boxes.upload(boxes, s);
     b->d_cts.upload(cts.empty() ? std::vector<DevContact>(1) : cts, s);
+    b->d_cseg.upload(cseg.empty() ? std::vector<double4>(1) : cseg, s);
+    b->d_ptrans.upload(ptrans, s);
     b->d_box_cell.upload(box_cell, s);
     b->d_gens.upload(gens, s);
     b->d_gen_sel.upload(gsel, s);
@@ -658,6 +685,7 @@ static void batch_execute(RtaBatch* b, double* T_out, double* jx_out, double* jy
         throw Error(BTE_EGEOM, std::to_string(cnt[3]) + " particle histories exceeded the iteration budget");
     if (cnt[4] != 0)
         throw Error(BTE_EINVAL, "specular reflection of a mode without a mirror partner in the mode table");
+    if (cnt[5] != 0) throw Error(BTE_EGEOM, "Error in translation, no box found");
     // post-processing per run (RTA_MC2D.cpp:2336-2435): scale, cumulative
     // sum over time (step response), T = gz / (Cv / vcorr) + T0
     const double th = b->thickness, vcorr = th / b->m->lattz;

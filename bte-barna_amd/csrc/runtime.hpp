@@ -112,8 +112,12 @@ struct Geom {
     std::vector<std::vector<HostContact>> contacts;  // ordered by neighbour id
     std::vector<DevBox> dboxes;
     std::vector<DevContact> dcontacts;
+    std::vector<double4> cseg;     // per contact: shared segment extended by 1e-6
+    std::vector<double2> ptrans;   // per box: periodic translation (0 otherwise)
     DevBuf<DevBox> d_boxes;
     DevBuf<DevContact> d_contacts;
+    DevBuf<double4> d_cseg;
+    DevBuf<double2> d_ptrans;
 };
 
 struct Prop {

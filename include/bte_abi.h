@@ -70,11 +70,16 @@ int bte_modes_destroy(bte_modes* m);
 /* ---- geometry: axis-aligned rectangular boxes (replaces
  * alma::read_geometry_XML -> std::vector<alma::geometry_2d>,
  * src/geometry_2d.cpp:408-522). rect: xmin,xmax,ymin,ymax per box.
- * kind: 0 regular box, 1 reservoir (<Reservoir/>), 2 periodic (rejected).
+ * kind: 0 regular box, 1 reservoir (<Reservoir/>), 2 periodic (then
+ * bte_geom_set_periodic gives its translation).
  * Tinit may be NULL (Tinit = Teq). ---- */
 int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int32_t* kind,
                           const double* Teq, const double* Tinit, bte_geom** out);
 int bte_geom_destroy(bte_geom* g);
+/* Periodic box (kind 2, <Translate_to id> tx ty </Translate_to>,
+ * geometry_2d.cpp:465-473): a particle entering it is moved by 1.0005 (tx, ty)
+ * into the regular neighbour of box `target` (geometry_2d.hpp:366-494). */
+int bte_geom_set_periodic(bte_geom* g, int box, int target, double tx, double ty);
 /* Contacts (geometry_2d::calculate_contacts, src/geometry_2d.cpp:295-381):
  * for each box up to maxc neighbours with (npx, npy, p1x, p1y, p2x, p2y). */
 int bte_geom_contacts(const bte_geom* g, int maxc, int32_t* ncont, int32_t* nbr, double* geo);

@@ -82,9 +82,10 @@ static void bert_evol(BertSetup& S, BPart& p, double vx, double vy, double dt,
                       pcg32& rng, std::vector<std::vector<int>>& Eb, int depth = 0) {
     auto& sys = S.sys;
     if (depth > 100000) throw geometry_error("runaway recursion");
+    if (sys[p.box].reservoir) { p.dead = true; return; }
+    // beyondRTA_MC2d.cpp:729-737: translate and continue the flight
+    if (sys[p.box].periodic) p.box = translate(sys, p.box, p.x, p.y, vx, vy, rng);
     Box& box = sys[p.box];
-    if (box.reservoir) { p.dead = true; return; }
-    if (box.periodic) throw geometry_error("periodic boxes are not supported");
     double nx = p.x + dt * vx, ny = p.y + dt * vy;
     if (box.inside(nx, ny)) { p.x = nx; p.y = ny; return; }
     Box::Hit hit;

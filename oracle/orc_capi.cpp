@@ -172,6 +172,11 @@ void orc_propH_get(int n, const double* b, int nr, const double* R, int32_t* sig
 }
 
 // ---------------- geometry ----------------
+// periodic maps for the next run (target box and translation per box),
+// set by orc_set_periodic and consumed by mk_sys
+static thread_local std::vector<int32_t> g_ptarget;
+static thread_local std::vector<double> g_ptrans;
+
 static std::vector<Box> mk_sys(int nbox, const double* rect, const int32_t* kind,
                                const double* Teq, const double* Tinit) {
     std::vector<Box> sys(nbox);
@@ -182,9 +187,21 @@ static std::vector<Box> mk_sys(int nbox, const double* rect, const int32_t* kind
         sys[i].periodic = kind[i] == 2;
         sys[i].Teq = Teq[i];
         sys[i].Treal = Tinit ? Tinit[i] : Teq[i];
+        if (sys[i].periodic) {
+            if ((int)g_ptarget.size() != nbox) throw geometry_error("periodic box without a translation");
+            sys[i].target = g_ptarget[i];
+            sys[i].tx = g_ptrans[2 * i];
+            sys[i].ty = g_ptrans[2 * i + 1];
+        }
     }
+    g_ptarget.clear();
+    g_ptrans.clear();
     calculate_contacts(sys);
     return sys;
+}
+void orc_set_periodic(int nbox, const int32_t* target, const double* trans) {
+    g_ptarget.assign(target, target + nbox);
+    g_ptrans.assign(trans, trans + 2 * (size_t)nbox);
 }
 // get_inter_side on box ibox: returns 0 ok, 1 geometry_error; out[0]=time,
 // out[1..2]=point, bids (up to 4) with nbids

@@ -58,6 +58,8 @@ struct Box {
     int id = 0;
     double xmin, xmax, ymin, ymax, cx, cy, area;
     bool reservoir = false, periodic = false;
+    int target = -1;             // periodic: box2translate
+    double tx = 0., ty = 0.;     // periodic: translation vector
     double Teq = -1, Treal = -1;
     std::vector<Border> borders;
     std::map<int, Border> contacts;  // contact box id -> shared border
@@ -214,6 +216,64 @@ inline std::pair<int, int> correct_corner_problem(double x, double y, double vx,
     int w = finalist[rng.bounded((uint32_t)finalist.size())];
     if (bids[w].empty()) throw geometry_error("corner without border");
     return {w, bids[w][0]};
+}
+
+
+// geometry_2d::translate (include/geometry_2d.hpp:366-494) on rectangles: a
+// particle in periodic box `id` at r0 moves along 1.0005 T to the first
+// border of the target box that the translation segment crosses, then into
+// the regular neighbour of the target box that holds that point (the shared
+// border, the point, or the point nudged 1e-6 along v or back along T).
+// Several candidates: those still inside after 1e-6 v, one chosen at random
+// (the point is taken from the chosen candidate). Returns the new box.
+template <class Rng>
+inline int translate(const std::vector<Box>& sys, int id, double& x, double& y, double vx, double vy, Rng& rng) {
+    const Box& b = sys[id];
+    if (!b.periodic) throw geometry_error("Error: this can only be used in periodic cells");
+    const double Tx = 1.0005 * b.tx, Ty = 1.0005 * b.ty;
+    const int tib = b.target;
+    if (tib < 0 || tib >= (int)sys.size() || almost_equal(std::hypot(Tx, Ty), 0.))
+        throw geometry_error("Error: Bad id for mapping of periodic cells");
+    const double s0x = x, s0y = y, s1x = x + Tx, s1y = y + Ty;
+    double px = 0, py = 0;
+    bool found = false;
+    for (const Border& br : sys[tib].borders)
+        if (seg_intersect(s0x, s0y, s1x, s1y, br.ax, br.ay, br.bx, br.by, px, py)) {
+            found = true;
+            break;
+        }
+    if (!found) throw geometry_error("Error in translation, no box found");
+    const double tn = std::hypot(b.tx, b.ty), vn = std::hypot(vx, vy);
+    const double pTx = px - 1.0e-6 * b.tx / tn, pTy = py - 1.0e-6 * b.ty / tn;
+    const double pVx = px + 1.0e-6 * vx / vn, pVy = py + 1.0e-6 * vy / vn;
+    std::vector<int> pb;
+    std::vector<std::pair<double, double>> rf;
+    for (const auto& [ibc, cb] : sys[tib].contacts) {
+        double ix, iy;
+        const bool here_i = seg_intersect(s0x, s0y, s1x, s1y, cb.ax, cb.ay, cb.bx, cb.by, ix, iy);
+        const bool here_T = sys[ibc].inside(px, py);
+        const bool here_Tv = sys[ibc].inside(pVx, pVy);
+        const bool here_TT = sys[ibc].inside(pTx, pTy);
+        if (ibc == tib || !(here_i || here_T || here_Tv || here_TT) || sys[ibc].periodic || sys[ibc].reservoir)
+            continue;
+        pb.push_back(ibc);
+        if (here_i) rf.emplace_back(ix, iy);
+        else if (here_T) rf.emplace_back(px, py);
+        else if (here_Tv) rf.emplace_back(pVx, pVy);
+        else rf.emplace_back(pTx, pTy);
+    }
+    if (pb.empty()) throw geometry_error("Error in Translation");
+    int k = 0;
+    if (pb.size() > 1) {
+        std::vector<int> keep;
+        for (size_t i = 0; i < pb.size(); i++)
+            if (sys[pb[i]].inside(rf[i].first + 1.0e-6 * vx, rf[i].second + 1.0e-6 * vy)) keep.push_back((int)i);
+        if (keep.empty()) throw geometry_error("Error in Translation");
+        k = keep[rng.bounded((uint32_t)keep.size())];
+    }
+    x = rf[k].first;
+    y = rf[k].second;
+    return pb[k];
 }
 
 }  // namespace orc

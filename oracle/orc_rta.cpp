@@ -178,7 +178,11 @@ static void evolparticle(const RtaSetup& S, Part& p, double vx, double vy,
                          double& dt, pcg32& rng, Tally& T, Counters& C) {
     const auto& sys = S.sys;
     const Box& box = sys[p.box];
-    if (box.periodic) throw geometry_error("periodic boxes are not supported");
+    if (box.periodic) {
+        // RTA_MC2D.cpp:1034-1043: translate and return with dt unchanged
+        p.box = translate(sys, p.box, p.x, p.y, vx, vy, rng);
+        return;
+    }
     if (box.reservoir) {
         p.dead = true;
         dt = 0.;

@@ -140,11 +140,24 @@ def contacts(rect, maxc=8):
     return nc, nbr, geo
 
 
+def set_periodic(nbox, periodic):
+    """periodic: {box: (target, tx, ty)} for the next oracle run"""
+    if not periodic:
+        return
+    tgt = np.full(nbox, -1, np.int32)
+    tr = np.zeros(2 * nbox)
+    for b, (t, tx, ty) in periodic.items():
+        tgt[b] = t
+        tr[2 * b:2 * b + 2] = (tx, ty)
+    lib().orc_set_periodic(nbox, tgt.ctypes.data_as(ip32), _dp(tr))
+
+
 def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, ballistic=False, sigma=0.25,
-            nthreads=0):
+            nthreads=0, periodic=None):
     rect, Teq = _f(rect), _f(Teq)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
+    set_periodic(nbox, periodic)
     ntime = int(np.floor(maxtime / dt + 1e-6)) + 1
     nt = ntime - 1
     T = np.zeros((nt, nbox))
@@ -166,10 +179,11 @@ def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, 
 
 
 def bert_run(mat, rect, kind, Teq, Tinit, thickness, dt, maxtime, P=None, ngamma=0, rta=False, Ed=-1.0,
-             Efactor=0, Tmax=0.0, Tmin=0.0, seed=1, max_steps=0, nthreads=0):
+             Efactor=0, Tmax=0.0, Tmin=0.0, seed=1, max_steps=0, nthreads=0, periodic=None):
     rect, Teq, Tinit = _f(rect), _f(Teq), _f(Tinit)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
+    set_periodic(nbox, periodic)
     n = len(mat["omega"])
     nst = 0
     t = 0.0

@@ -46,7 +46,7 @@ def test_rtamc2d_rejects_hdf5_material(pkg, tmp_path):
 
 
 @pytest.mark.parametrize("extra,msg", [
-    ('<Translate_to id="0"> 1.0 0.0 </Translate_to>', "periodic"),
+    ('<Translate_to id="0"> 1.0 </Translate_to>', "bad <Translate_to>"),
     ('<theta angle="0.3"/>', "rotated"),
 ])
 def test_geometry_features_rejected(pkg, tmp_path, extra, msg):

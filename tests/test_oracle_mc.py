@@ -24,6 +24,63 @@ def bar(nbins=10, dx=10.0, H=40.0, TL=300.5, TR=299.5):
     return np.array(rect), np.array(kind, np.int32), np.array(teq)
 
 
+def periodic_strip(width=20.0, nrows=5, dy=10.0, TB=300.5, TT=299.5):
+    """One column of nrows boxes, periodic in x through ghost boxes on both
+    sides (each translates onto the other, as the reference's beam.xml), with
+    reservoirs below and above. Returns rect, kind, teq, periodic."""
+    H = nrows * dy
+    rect = [[0.0, width, k * dy, (k + 1) * dy] for k in range(nrows)]
+    rect += [[-2.0, 0.0, 0.0, H], [width, width + 2.0, 0.0, H]]       # ghosts
+    rect += [[-2.0, width + 2.0, -2.0, 0.0], [-2.0, width + 2.0, H, H + 2.0]]  # reservoirs
+    kind = [0] * nrows + [2, 2, 1, 1]
+    teq = [300.0] * (nrows + 2) + [TB, TT]
+    left, right = nrows, nrows + 1
+    periodic = {left: (right, width, 0.0), right: (left, -width, 0.0)}
+    return np.array(rect), np.array(kind, np.int32), np.array(teq), periodic
+
+
+def tilted_y_material(n=60, seed=13):
+    """modes within 60 degrees of the y axis (|vy| >= |v|/2), |v| 2-8 nm/ps"""
+    rng = np.random.default_rng(seed)
+    om = 2 * np.pi * rng.uniform(0.5, 15.0, n)
+    sp = rng.uniform(2.0, 8.0, n)
+    ang = np.pi / 2 + rng.uniform(-np.pi / 3, np.pi / 3, n) + np.where(rng.random(n) < 0.5, 0.0, np.pi)
+    return {"omega": om, "vx": sp * np.cos(ang), "vy": sp * np.sin(ang), "w": np.full(n, 0.01), "V": 20.0,
+            "lattz": 1.0}
+
+
+def test_rta_periodic_ballistic_flux_is_exact():
+    """Ballistic particles from the bottom reservoir wrap around the periodic
+    x direction until they reach the top: each crosses every row once with
+    displacement dy, so the steady cumulative y flux of every row is
+    (|J_B| + |J_T|) * 1e30 whatever the translations did to x."""
+    m = tilted_y_material()
+    rect, kind, teq, per = periodic_strip()
+    out = orc.rta_run(m, rect, kind, teq, thickness=1.0, T0=300.0, dt=50.0, maxtime=1000.0, nparticles=20000,
+                      seed=3, ballistic=True, sigma=5.0, periodic=per)
+    vuc = m["V"] / m["lattz"]
+    JB = orc.wall_dist(m["omega"], m["vx"], m["vy"], 300.5, 300.0, 0.0, 1.0, vuc)[2]
+    JT = orc.wall_dist(m["omega"], m["vx"], m["vy"], 299.5, 300.0, 0.0, -1.0, vuc)[2]
+    np.testing.assert_allclose(out["jy"][-1, :5], (abs(JB) + abs(JT)) * 1e30, rtol=1e-9)
+    assert np.abs(out["jy"][-1, 5:7]).max() == 0.0  # no time is spent in the ghost boxes
+
+
+def test_bert_periodic_strip_conserves():
+    """beRTAMC2D with periodic ghost boxes: the signed count (particles +
+    pending wall counts) is conserved with adiabatic walls and P >= 0."""
+    m, B, P = _bert_inputs(24, seed=6)
+    rect, kind, teq, per = periodic_strip(TB=300.0, TT=300.0)
+    kind = kind.copy()
+    # drop the reservoirs: a closed strip, adiabatic walls below and above
+    rect, kind, teq = rect[:7], kind[:7], teq[:7]
+    tin = teq.copy()
+    tin[0] = 310.0
+    o = orc.bert_run(m, rect, kind, teq, tin, thickness=1.0, dt=2.0, maxtime=40.0, P=P, Efactor=3000, Tmax=310.0,
+                     Tmin=300.0, seed=4, periodic=per)
+    tot = o["npart"] + o["pending"]
+    assert (tot == tot[0]).all() and tot[0] > 0
+
+
 def test_rta_ballistic_flux_is_exact():
     """Ballistic horizontal particles cross every box once with displacement
     dx, so the steady cumulative flux of every box equals the total emitted
