@@ -679,7 +679,8 @@ struct BertSource {
     double expected;  // particles per step
     int32_t box, sign;
     int32_t tab;      // alias table offset (in entries)
-    int32_t pad;
+    int32_t src;      // source index (keys its random streams)
+    int32_t part, nparts;  // this workgroup's share of the source's particles
 };
 __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src,
                                                           const uint2* stab, uint32_t step, int cur, int hist) {
@@ -691,22 +692,26 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     double* sbox = box_lds_begin(P);
     __syncthreads();
     int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
+    // every part of a source draws the same count (Ntogenerate,
+    // sampling.cpp:230-245) and takes its slice of the particle indices
     Pcg32 cr;
-    seed_stream(cr, stream_key(P.seed, step, 2), 0x5ce0000000000000ULL + blockIdx.x);
+    seed_stream(cr, stream_key(P.seed, step, 2), 0x5ce0000000000000ULL + (uint64_t)S.src);
     const uint64_t skey = stream_key(P.seed, step, 3);
     const double fl = floor(S.expected);
     const int ng = (int)fl + (cr.uniform() <= S.expected - fl ? 1 : 0);
+    const int klo = (int)((int64_t)ng * S.part / S.nparts), khi = (int)((int64_t)ng * (S.part + 1) / S.nparts);
     int32_t* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
-    const int nloop = (ng + 63) & ~63;
+    const int nloop = (khi - klo + 63) & ~63;
     unsigned long long done = 0;
-    for (int k = threadIdx.x; k - (int)(threadIdx.x & 63) < nloop; k += blockDim.x) {
-        bool valid = k < ng;
+    for (int q = threadIdx.x; q - (int)(threadIdx.x & 63) < nloop; q += blockDim.x) {
+        const int k = klo + q;
+        bool valid = k < khi;
         Pcg32 rng;
         Pick0 f0{0u, 0u, make_uint2(0u, 0u)};
         int mode = 0, bx = S.box;
         double x = 0, y = 0;
         bool alive = false;
-        uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)blockIdx.x << 20) ^ (uint64_t)k ^ 0x15a0ULL);
+        uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)S.src << 20) ^ (uint64_t)k ^ 0x15a0ULL);
         if (valid) {
             seed_stream(rng, skey, pid);
             double t = rng.uniform();
@@ -1138,20 +1143,40 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     b->n0 = total;
     b->ninit = (int)inits.size();
     // one buffer segment per writing workgroup: step, emission, sources
-    b->nsrc = 0;
+    // reservoir sources (Isothermal_walls): expected particles per step,
+    // split over workgroups of at most ~1024 particles each
+    std::vector<BertSource> srcs;
+    std::vector<uint2> stab;
+    double src_total = 0;
     for (int sid = 0; sid < nbox; sid++) {
-        if (g->boxes[sid].kind != 1) continue;
+        const RectBox& sb = g->boxes[sid];
+        if (sb.kind != 1) continue;
         for (auto& hc : g->contacts[sid]) {
             const RectBox& cb = g->boxes[hc.box];
-            if (cb.kind != 0 || almost_equal(hc.length(), 0.) || almost_equal(g->boxes[sid].Teq, cb.Teq)) continue;
-            b->nsrc++;
+            if (cb.kind != 0 || almost_equal(hc.length(), 0.) || almost_equal(sb.Teq, cb.Teq)) continue;
+            double cum = wall_pmf(*m, sb.Teq, cb.Teq, hc.npx, hc.npy, pmf.data());
+            double spf = hc.length() * cfg->thickness / vuc;
+            build_alias(pmf.data(), n, th.data(), al.data());
+            const double expected = cfg->dt * spf * cum / (b->Eeff * m->nq);
+            const int nparts = (int)std::min(256.0, std::max(1.0, std::ceil(expected / 1024.0)));
+            const int src = (int)(stab.size() / n);
+            for (int part = 0; part < nparts; part++)
+                srcs.push_back(BertSource{hc.p1x, hc.p1y, hc.p2x - hc.p1x, hc.p2y - hc.p1y, expected, hc.box,
+                                          (sb.Teq - cb.Teq) >= 0 ? 1 : -1, (int32_t)stab.size(), src, part, nparts});
+            for (int k = 0; k < n; k++) stab.push_back(make_uint2(th[k], al[k]));
+            src_total += expected;
         }
     }
+    b->nsrc = (int)srcs.size();
     b->step_blocks = std::min(2 * c->cu_count, 2048);
     b->emit_blocks = std::min(c->cu_count, 1024);
     b->nsegs = b->step_blocks + b->emit_blocks + b->nsrc;
     if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
-    b->cap = cfg->capacity ? cfg->capacity : std::max<uint64_t>(1u << 21, 3 * total);
+    // generous by default (32 B per particle and buffer): reservoir-driven
+    // runs grow to a steady population, and every segment keeps >= 4096 slots
+    b->cap = cfg->capacity ? cfg->capacity
+                           : std::max<uint64_t>({1ull << 22, 3 * total, src_total > 0 ? (1ull << 25) : 0ull,
+                                                 4096ull * (uint64_t)b->nsegs});
     b->cap = (b->cap + b->nsegs - 1) / b->nsegs * b->nsegs;
     if (b->cap >= 0xffffffffULL) throw Error(BTE_EINVAL, "capacity above 2^32 particles");
     if ((total + b->nsegs - 1) / b->nsegs > b->cap / b->nsegs)
@@ -1238,26 +1263,6 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     }
     b->d_spl.upload(spl.empty() ? std::vector<double>(1, 0.) : spl, s);
     b->d_side_spl.upload(side_spl, s);
-    // reservoir sources (Isothermal_walls)
-    std::vector<BertSource> srcs;
-    std::vector<uint2> stab;
-    for (int sid = 0; sid < nbox; sid++) {
-        const RectBox& sb = g->boxes[sid];
-        if (sb.kind != 1) continue;
-        for (auto& hc : g->contacts[sid]) {
-            const RectBox& cb = g->boxes[hc.box];
-            if (cb.kind != 0 || almost_equal(hc.length(), 0.) || almost_equal(sb.Teq, cb.Teq)) continue;
-            double cum = wall_pmf(*m, sb.Teq, cb.Teq, hc.npx, hc.npy, pmf.data());
-            double spf = hc.length() * cfg->thickness / vuc;
-            build_alias(pmf.data(), n, th.data(), al.data());
-            BertSource S{hc.p1x, hc.p1y, hc.p2x - hc.p1x, hc.p2y - hc.p1y,
-                         cfg->dt * spf * cum / (b->Eeff * m->nq), hc.box,
-                         (sb.Teq - cb.Teq) >= 0 ? 1 : -1, (int32_t)stab.size(), 0};
-            for (int k = 0; k < n; k++) stab.push_back(make_uint2(th[k], al[k]));
-            srcs.push_back(S);
-        }
-    }
-    if ((int)srcs.size() != b->nsrc) throw Error(BTE_EINVAL, "source count mismatch");
     if (b->nsrc) {
         b->d_src.upload(srcs, s);
         b->d_src_tab.upload(stab, s);

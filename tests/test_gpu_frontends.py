@@ -131,3 +131,52 @@ def test_rtamc2d_csv(pkg, ctx, tmp_path):
                         seed=5 + irun, sigma_bounce=2.0)
         assert np.allclose(T[:, 1:], o["T"], rtol=1e-5, atol=1e-9)
         assert np.allclose(jx[:, 1:], o["jx"], rtol=1e-5, atol=1e-6 * np.abs(o["jx"]).max())
+
+
+def beam_xml(nrows, path):
+    """The layout of the reference's examples/example_bar/beam.xml with nrows
+    rows: boxes of 50 x 2 nm stacked in y from y = 0.5, ghost boxes on both
+    sides translating by -+50 nm in x, a 302 K reservoir below and a 300 K
+    reservoir above."""
+    H = 0.5 + 2.0 * nrows
+    boxes = []
+    for k in range(nrows):
+        y0 = 0.5 + 2.0 * k
+        boxes.append(BOX.format(i=k, x0=9.0, x1=59.0, y0=y0, y1=y0 + 2.0, teq=300.0, tin=300.0, extra=""))
+    g = nrows
+    boxes.append(BOX.format(i=g, x0=59.0, x1=60.5, y0=0.5, y1=H, teq=300.0, tin=300.0,
+                            extra=f'\n    <Translate_to id="{g + 1}"> -50. 0.0 0.0 </Translate_to>'))
+    boxes.append(BOX.format(i=g + 1, x0=7.0, x1=9.0, y0=0.5, y1=H, teq=300.0, tin=300.0,
+                            extra=f'\n    <Translate_to id="{g}"> 50.0 0.0 0.0 </Translate_to>'))
+    boxes.append(BOX.format(i=g + 2, x0=7.0, x1=60.5, y0=-1.5, y1=0.5, teq=302.0, tin=302.0,
+                            extra="\n    <Reservoir/>"))
+    boxes.append(BOX.format(i=g + 3, x0=7.0, x1=60.5, y0=H, y1=H + 2.0, teq=300.0, tin=300.0,
+                            extra="\n    <Reservoir/>"))
+    path.write_text(GEOM.format(n=nrows + 4, boxes="\n".join(boxes)))
+
+
+def test_beam_layout_runs_in_both_solvers(pkg, tmp_path):
+    beam_xml(40, tmp_path / "beam.xml")
+    (tmp_path / "rta.xml").write_text(
+        '<RTA_MC2d>\n<geometry file="beam.xml"/>\n<time dt="2" maxtime="200"/>\n'
+        '<material name="m" modetable="synthetic" nmodes="400" seed="4" thickness="0.533" T0="300"'
+        ' sigma_bounce="2.0"/>\n<particles N="100000" seed="2"/>\n</RTA_MC2d>\n')
+    run([pkg.app_path("RTAMC2D"), "rta.xml", "8"], tmp_path)
+    T = np.loadtxt(tmp_path / "temperature_300K_run_0.csv", delimiter=",")
+    jy = np.loadtxt(tmp_path / "jy_300K_run_0.csv", delimiter=",")
+    assert T.shape == (100, 1 + 44)
+    # heat flows up from the 302 K reservoir: positive flux, T falling with y
+    assert (jy[-1, 1:41] > 0).all()
+    assert T[-1, 1:6].mean() > T[-1, 36:41].mean()
+    run([pkg.app_path("PropagatorBuilder"), "synthetic:400:4", "300", "0.5", "1", "20"], tmp_path)
+    (tmp_path / "berta.xml").write_text(
+        '<beRTAMC2D>\n<geometry file="beam.xml"/>\n<time dt="0.5" maxtime="20"/>\n'
+        '<material name="m" modetable="synthetic" nmodes="400" seed="4" thickness="0.533">\n'
+        '  <propagator T="300" file="synthetic_400_4_300K_0.5ps.P.eigen.bin"/>\n</material>\n'
+        '<Eeff Ed="5.39836e-25"/>\n</beRTAMC2D>\n')
+    r = run([pkg.app_path("beRTAMC2D"), "berta.xml", "4"], tmp_path)
+    lines = [ln.split("\t") for ln in r.stdout.splitlines() if ln and not ln.startswith("#")]
+    rows = np.array([[float(x) for x in ln if x != ""] for ln in lines])
+    assert rows.shape[1] == 3 + 3 * 44
+    # particles enter from the hot reservoir and energy appears in the first rows
+    assert rows[-1, 2] > 0 and rows[-1, 3] > 0

@@ -103,3 +103,27 @@ def test_krylov_matches_oracle(pkg, ctx):
         l1 = np.abs(Pg - Pe).sum(0).max()
         assert l1 < tol, (prec, l1)
         assert np.abs(Pg.sum(0) - 1).max() < 1e-12
+
+
+def test_bert_reservoir_sources_match_oracle(pkg, ctx):
+    """beRTAMC2D driven by reservoirs (Isothermal_walls, beyondRTA_MC2d.cpp:
+    446-585): thousands of particles per step and source, so each source is
+    split over several workgroups on the GPU."""
+    W = pkg.workloads
+    mat, B = W.config1_inputs(nmodes=200, seed=9)
+    import scipy.linalg
+    P = scipy.linalg.expm(2.0 * B)
+    rect, kind, teq, _ = W.ribbon(60.0, 40.0, 6, 300.0, reservoirs=(301.0, 299.0))
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    prop = pkg.Propagator(ctx, P)
+    kw = dict(thickness=1.0, dt=2.0, maxtime=2.0 * 24, Efactor=200000, Tmax=301.0, Tmin=299.0)
+    R = 10
+    g = [pkg.bertamc2d(ctx, modes, geom, prop, seed=70 + r, **kw) for r in range(R)]
+    o = [orc.bert_run(mat, rect, kind, teq, teq, P=P, seed=90 + r, **kw) for r in range(R)]
+    assert g[0]["npart"][-1] > 20000  # several thousand injected per step
+    for q in (0, 1):
+        z = zstats(np.array([x["props"][[5, -1], :6, q] for x in g]), np.array([x["props"][[5, -1], :6, q] for x in o]))
+        assert np.abs(z).max() < 5.0, (q, z)
+    z = zstats(np.array([float(x["npart"][-1]) for x in g]), np.array([float(x["npart"][-1]) for x in o]))
+    assert abs(z) < 4.0, z
