@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from ._abi import (BertCfg, BertReport, BteError, FRAME_CB, KrylovReport, PREC, ARNOLDI, RtaCfg,
-                   RtaReport, RtaRunDesc, RtaBatchReport, check, lib, LIBPATH)
+                   RtaReport, RtaRunDesc, RtaBatchReport, RtaSteadyCfg, RtaSteadyReport, check, lib, LIBPATH)
 from . import workloads  # noqa: F401
 
 __all__ = ["Context", "ModeTable", "Geometry", "Propagator", "rtamc2d", "bertamc2d", "build_P",
@@ -171,6 +171,18 @@ def rtamc2d(ctx, modes, geom, **kw):
         return r.execute()
     finally:
         r.close()
+
+
+def rta_steady(ctx, modes, geom, thickness, T0, gradient, nparticles, seed=1, sigma_bounce=0.25, eps_energy=-42.0,
+               eps_flux=-42.0, max_generations=0):
+    """RTAMC2D steady branch (nonzero gradient [K/nm]): per box T, jx, jy."""
+    cfg = RtaSteadyCfg(thickness, T0, (C.c_double * 2)(*gradient), nparticles, seed, sigma_bounce, eps_energy,
+                       eps_flux, max_generations)
+    T, jx, jy = np.zeros(geom.nbox), np.zeros(geom.nbox), np.zeros(geom.nbox)
+    rep = RtaSteadyReport()
+    check(lib().bte_rta_steady_run(ctx.h, modes.h, geom.h, C.byref(cfg), _dp(T), _dp(jx), _dp(jy), C.byref(rep)))
+    return {"T": T, "jx": jx, "jy": jy, "generations": rep.generations, "converged": bool(rep.converged),
+            "flights": rep.flights, "Eeff": rep.Eeff, "kernel_ms": rep.kernel_ms}
 
 
 class RtaBatch(_Handle):

@@ -32,6 +32,17 @@ class RtaReport(C.Structure):
                 ("kernel_ms", C.c_double), ("ntime", C.c_int)]
 
 
+class RtaSteadyCfg(C.Structure):
+    _fields_ = [("thickness", C.c_double), ("T0", C.c_double), ("gradient", C.c_double * 2),
+                ("nparticles", C.c_uint64), ("seed", C.c_uint64), ("sigma_bounce", C.c_double),
+                ("eps_energy", C.c_double), ("eps_flux", C.c_double), ("max_generations", C.c_int)]
+
+
+class RtaSteadyReport(C.Structure):
+    _fields_ = [("generations", C.c_int), ("converged", C.c_int), ("flights", C.c_uint64), ("Eeff", C.c_double),
+                ("kernel_ms", C.c_double)]
+
+
 class RtaRunDesc(C.Structure):
     _fields_ = [("geom", C.c_void_p), ("T0", C.c_double), ("nparticles", C.c_uint64), ("seed", C.c_uint64),
                 ("ballistic", C.c_int), ("wscale", C.c_double), ("specularity", C.c_double * 4),
@@ -92,6 +103,7 @@ SIGNATURES = {
     "bte_rta_execute": (C.c_int, [vp, dp, dp, dp, C.POINTER(RtaReport)]),
     "bte_rta_destroy": (C.c_int, [vp]),
     "bte_rta_run": (C.c_int, [vp, vp, vp, C.POINTER(RtaCfg), dp, dp, dp, C.POINTER(RtaReport)]),
+    "bte_rta_steady_run": (C.c_int, [vp, vp, vp, C.POINTER(RtaSteadyCfg), dp, dp, dp, C.POINTER(RtaSteadyReport)]),
     "bte_rta_batch_create": (C.c_int, [vp, vp, C.c_int, C.POINTER(RtaRunDesc), C.c_double, C.c_double,
                                        C.c_double, C.c_double, C.c_int, pvp]),
     "bte_rta_batch_tune": (C.c_int, [vp, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]),

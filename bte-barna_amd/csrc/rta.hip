@@ -886,3 +886,515 @@ int bte_rta_run(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_r
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// RTAMC2D steady branch (nonzero <gradient>, RTA_MC2D.cpp:2544-2782): a
+// generation iteration. Each generation every particle makes one flight:
+// a relaxation inside a box adds its sign to phi(box) (process_segment with
+// evalphi, RTA_MC2D.cpp:771-826), a diffuse wall bounce carries it into the
+// next generation, a reservoir absorbs it; then |phi(box)| particles of sign
+// phi are re-emitted uniformly in each box from BE_derivative. gz, jx, jy
+// accumulate over generations until the relative change falls below the
+// <convergence> thresholds or no particle is left.
+namespace bte {
+
+struct RtaSPart {
+    double x, y;
+    uint32_t mode, boxsign;  // box << 1 | (sign < 0)
+    uint64_t id;
+};
+
+struct RtaSteadyParams {
+    const double4* modes;
+    const double2* ivel;
+    const uint2* scat;
+    const DevBox* boxes;
+    const DevContact* cts;
+    const double4* cseg;
+    const double2* ptrans;
+    const int4* lam_info;
+    const uint2* lam_tab;
+    const int32_t* perm;
+    const RtaGen* gens;
+    const uint2* gen_sel;
+    const uint2* gen_tab;
+    const int32_t* gbox;     // boxes with a gradient generator
+    const uint2* gbox_sel;   // alias over their energies
+    const uint2* gtab;       // per gradient box: [2][n] alias (0: sign +, 1: sign -)
+    const int32_t* gclass;   // per gradient box: bit 0 has +, bit 1 has -
+    const uint64_t* regen;   // prefix of re-emission counts [nbox + 1]
+    double* acc;             // [3][nbox] gz, jx, jy over all generations
+    double* phi;             // [nbox] this generation
+    const RtaSPart* pin;
+    RtaSPart* pout;
+    unsigned long long* counters;  // [0] appended, [1] flights, [2] stuck, [3] bad translation
+    uint64_t nin, seed, key;
+    double pwall;            // share of wall generators in the initial energy
+    unsigned long long max_iters;
+    int32_t n, nbox, ngen, ngbox, lds_tally;
+};
+
+__device__ inline void steady_append(const RtaSteadyParams& P, bool want, const RtaSPart& q) {
+    const unsigned long long m = __ballot(want);
+    if (m == 0ULL) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(P.counters + 0, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (want) P.pout[base + __popcll(m & ((1ULL << lane) - 1ULL))] = q;
+}
+
+// initial particles (Init_generator, RTA_MC2D.cpp:436-533)
+__global__ __launch_bounds__(256) void rta_steady_init_kernel(RtaSteadyParams P, uint64_t np) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (uint64_t)gridDim.x * blockDim.x) {
+        Pcg32 rng;
+        rng.seed(P.seed, i);
+        RtaSPart q{0., 0., 0u, 0u, i};
+        const double r = rng.uniform();
+        bool ng;
+        if (P.ngen > 0 && r <= P.pwall) {
+            const int gi = (int)alias_pick(P.gen_sel, (uint32_t)P.ngen, rng, ng);
+            const RtaGen g = P.gens[gi];
+            const double u = rng.uniform();
+            double x = g.p1x + u * g.ndx, y = g.p1y + u * g.ndy;
+            const DevBox bi = P.boxes[g.idi];
+            if (!box_inside(bi, x, y)) {
+                double dx = 0.5 * (bi.xmin + bi.xmax) - x, dy = 0.5 * (bi.ymin + bi.ymax) - y;
+                double dn = sqrt(dx * dx + dy * dy);
+                x += 1.0e-6 * dx / dn;
+                y += 1.0e-6 * dy / dn;
+            }
+            q.x = x;
+            q.y = y;
+            q.mode = alias_pick(P.gen_tab + (size_t)gi * P.n, (uint32_t)P.n, rng, ng);
+            q.boxsign = ((uint32_t)g.idi << 1) | (g.sign < 0 ? 1u : 0u);
+        } else {
+            const int k = (int)alias_pick(P.gbox_sel, (uint32_t)P.ngbox, rng, ng);
+            const int b = P.gbox[k];
+            const DevBox B = P.boxes[b];
+            q.x = B.xmin + rng.uniform() * (B.xmax - B.xmin);
+            q.y = B.ymin + rng.uniform() * (B.ymax - B.ymin);
+            // signs alternate with the id; a class without modes gives way
+            const int cls = P.gclass[k];
+            int neg = (i & 1) ? 0 : 1;
+            if (neg && !(cls & 2)) neg = 0;
+            if (!neg && !(cls & 1)) neg = 1;
+            q.mode = alias_pick(P.gtab + ((size_t)k * 2 + neg) * P.n, (uint32_t)P.n, rng, ng);
+            q.boxsign = ((uint32_t)b << 1) | (uint32_t)neg;
+        }
+        P.pout[i] = q;
+    }
+}
+
+// one flight per particle (steady evolparticle, RTA_MC2D.cpp:1406-1799)
+__global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams P) {
+    extern __shared__ __align__(16) unsigned char sdyn[];
+    double* sacc = (double*)sdyn;  // [4][nbox]: gz, jx, jy, phi
+    const bool lds = P.lds_tally != 0;
+    if (lds)
+        for (int i = threadIdx.x; i < 4 * P.nbox; i += blockDim.x) sacc[i] = 0.;
+    __syncthreads();
+    double* tgz = lds ? sacc : P.acc;
+    double* tjx = tgz + P.nbox;
+    double* tjy = tjx + P.nbox;
+    double* tphi = lds ? sacc + 3 * P.nbox : P.phi;
+    unsigned long long flights = 0, stuck = 0, badtrans = 0;
+    const uint64_t nloop = (P.nin + 63) & ~63ULL;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i - (threadIdx.x & 63) < nloop;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const bool valid = i < P.nin;
+        RtaSPart p{};
+        bool bounced = false;
+        if (valid) {
+            p = P.pin[i];
+            Pcg32 rng;
+            rng.seed(P.key, i);
+            int mode = (int)p.mode, box = (int)(p.boxsign >> 1);
+            const int sign = (p.boxsign & 1u) ? -1 : 1;
+            double4 md = P.modes[mode];
+            double dt = -log(1.0 - rng.uniform()) / md.z;
+            flights++;
+            unsigned long long it = 0;
+            int still = 0;  // consecutive crossings without advance
+            double x = p.x, y = p.y;
+            while (!almost_equal(dt, 0.)) {
+                if (++it > P.max_iters || still > 64) { stuck++; break; }
+                const DevBox B = P.boxes[box];
+                if (B.kind == 2) {
+                    const int nb = translate_rect(P.boxes, P.cts, P.cseg, P.ptrans, box, x, y, md.x, md.y, rng);
+                    if (nb < 0) { badtrans++; break; }
+                    box = nb;
+                    continue;
+                }
+                if (B.kind == 1) break;  // absorbed
+                const double vx = md.x, vy = md.y;
+                const double nx = x + dt * vx, ny = y + dt * vy;
+                if (box_inside(B, nx, ny)) {
+                    atomicAdd(tgz + box, sign * dt);
+                    atomicAdd(tjx + box, vx * sign * dt);
+                    atomicAdd(tjy + box, vy * sign * dt);
+                    atomicAdd(tphi + box, (double)sign);
+                    break;
+                }
+                int mask;
+                double ex, ey;
+                const double2 iv = P.ivel[mode];
+                const double tf = box_exit(B, x, y, vx, vy, iv.x, iv.y, mask, ex, ey);
+                double left = dt - tf;
+                if (almost_equal(left, 0.)) left = 0.;
+                still = tf == 0. ? still + 1 : 0;
+                atomicAdd(tgz + box, sign * tf);
+                atomicAdd(tjx + box, vx * sign * tf);
+                atomicAdd(tjy + box, vy * sign * tf);
+                int side = 0;
+                const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
+                Ids4 pb{};
+                int npb;
+                if (fast >= 0) {
+                    pb.a = fast;
+                    npb = 1;
+                } else {
+                    npb = fast == SIDE_VOID ? 0 : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
+                }
+                if (npb == 0) {
+                    // diffuse wall: Lambert bounce, carried to the next generation
+                    const int s = fast == SIDE_VOID ? side
+                                : (__popc(mask) == 1 ? __ffs(mask) - 1
+                                                     : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next()));
+                    const int4 li = P.lam_info[mode];
+                    bool ng;
+                    const int k = (int)alias_pick(P.lam_tab + li.z + (size_t)s * li.y, (uint32_t)li.y, rng, ng);
+                    mode = P.perm[li.x + k];
+                    x = ex;
+                    y = ey;
+                    bounced = true;
+                    break;
+                }
+                x = ex;
+                y = ey;
+                if (npb == 1) {
+                    box = pb.a;
+                } else {
+                    Ids4 cand{};
+                    int nc = 0;
+                    const double cx = ex + 1.0e-6 * vx, cy = ey + 1.0e-6 * vy;
+                    for (int k = 0; k < npb && k < 4; k++)
+                        if (box_inside(P.boxes[pb.get(k)], cx, cy)) cand.set(nc++, pb.get(k));
+                    if (nc > 0) {
+                        box = cand.get((int)rng.bounded((uint32_t)nc));
+                    } else {
+                        box = pb.get((int)rng.bounded((uint32_t)(npb < 4 ? npb : 4)));
+                        const DevBox nb = P.boxes[box];
+                        if (nb.kind != 2) {
+                            double dx = 0.5 * (nb.xmin + nb.xmax) - x, dy = 0.5 * (nb.ymin + nb.ymax) - y;
+                            double dn = sqrt(dx * dx + dy * dy);
+                            x += 1.0e-6 * dx / dn;
+                            y += 1.0e-6 * dy / dn;
+                        }
+                    }
+                }
+                dt = left;  // a flight ending on a crossing leaves the particle out (as the reference's loop)
+            }
+            p.x = x;
+            p.y = y;
+            p.mode = (uint32_t)mode;
+            p.boxsign = ((uint32_t)box << 1) | (sign < 0 ? 1u : 0u);
+        }
+        steady_append(P, bounced, p);
+    }
+    if (lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < 4 * P.nbox; i += blockDim.x) {
+            const double v = sacc[i];
+            if (v != 0.) atomicAdd(i < 3 * P.nbox ? P.acc + i : P.phi + (i - 3 * P.nbox), v);
+        }
+    }
+    if (flights) atomicAdd(P.counters + 1, flights);
+    if (stuck) atomicAdd(P.counters + 2, stuck);
+    if (badtrans) atomicAdd(P.counters + 3, badtrans);
+}
+
+// re-emission of |phi(box)| particles per box (RTA_MC2D.cpp:2637-2654)
+__global__ __launch_bounds__(256) void rta_steady_regen_kernel(RtaSteadyParams P, uint64_t out0, uint64_t total) {
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (uint64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = P.nbox - 1;  // box b holds [regen[b], regen[b+1])
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (P.regen[mid] <= k) lo = mid; else hi = mid - 1;
+        }
+        const int b = lo;
+        Pcg32 rng;
+        rng.seed(P.key ^ 0x5eed5eed5eed5eedULL, k);
+        const DevBox B = P.boxes[b];
+        RtaSPart q;
+        q.x = B.xmin + rng.uniform() * (B.xmax - B.xmin);
+        q.y = B.ymin + rng.uniform() * (B.ymax - B.ymin);
+        bool ng;
+        q.mode = alias_pick(P.scat, (uint32_t)P.n, rng, ng);
+        q.boxsign = ((uint32_t)b << 1) | (P.phi[b] < 0. ? 1u : 0u);
+        q.id = out0 + k;
+        P.pout[out0 + k] = q;
+    }
+}
+
+}  // namespace bte
+
+extern "C" int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const bte_rta_steady_cfg* cfg,
+                                  double* T_out, double* jx_out, double* jy_out, bte_rta_steady_report* rep) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    auto* m = reinterpret_cast<const Modes*>(mm);
+    auto* g = reinterpret_cast<const Geom*>(gg);
+    if (!c || !m || !g || !cfg || !T_out || !jx_out || !jy_out) throw Error(BTE_EINVAL, "null argument");
+    if (cfg->nparticles == 0 || !(cfg->thickness > 0)) throw Error(BTE_EINVAL, "bad steady configuration");
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const int n = m->n, nbox = (int)g->boxes.size();
+    for (auto& b : g->boxes)
+        if (b.kind == 0) {
+            if (b.Teq != cfg->T0) throw Error(BTE_EINVAL, "Tref in xml is different from provided T0");
+            if (b.Tinit != b.Teq) throw Error(BTE_EINVAL, "Initial temperature out of reference is not allowed");
+        }
+    for (int i = 0; i < n; i++)
+        if (m->w[i] == 0.) throw Error(BTE_EINVAL, "invalid scattering rate");
+    for (int i = 0; i < nbox; i++)
+        if (g->dboxes[i].kind == 2 && g->dboxes[i].tgt < 0)
+            throw Error(BTE_EGEOM, "periodic box " + std::to_string(i) + " has no translation");
+    const double vuc = m->V * cfg->thickness / m->lattz;
+    std::vector<double> pmf(n);
+    std::vector<uint32_t> th(n), al(n);
+    DevBuf<double4> d_modes;
+    DevBuf<double2> d_ivel;
+    std::vector<double4> md(n);
+    std::vector<double2> iv(n);
+    for (int i = 0; i < n; i++) {
+        md[i] = make_double4(m->vx[i], m->vy[i], m->w[i], m->omega[i]);
+        iv[i] = make_double2(m->vx[i] != 0. ? 1.0 / m->vx[i] : 0., m->vy[i] != 0. ? 1.0 / m->vy[i] : 0.);
+    }
+    d_modes.upload(md, s);
+    d_ivel.upload(iv, s);
+    // BE_derivative at T0 (re-emission)
+    for (int i = 0; i < n; i++) pmf[i] = m->omega[i] <= 0. ? 0. : h_be_kernel(m->omega[i], cfg->T0) * m->w[i];
+    if (!build_alias(pmf.data(), n, th.data(), al.data())) throw Error(BTE_EINVAL, "empty scattering pmf");
+    DevBuf<uint2> d_scat, d_gen_sel, d_gen_tab, d_gbox_sel, d_gtab, d_lam_tab;
+    upload_alias(d_scat, th, al, s);
+    // wall generators (RTA_MC2D.cpp:296-347, 366-397)
+    std::vector<RtaGen> gens;
+    std::vector<double> gw;
+    std::vector<uint2> gtab;
+    for (int sid = 0; sid < nbox; sid++) {
+        const RectBox& sb = g->boxes[sid];
+        if (sb.kind != 1) continue;
+        for (auto& hc : g->contacts[sid]) {
+            const RectBox& cb = g->boxes[hc.box];
+            if (cb.kind != 0 || almost_equal(hc.length(), 0.) || almost_equal(sb.Teq, cb.Teq)) continue;
+            double cum = wall_pmf(*m, sb.Teq, cb.Teq, hc.npx, hc.npy, pmf.data());
+            int sg = (sb.Teq - cb.Teq) >= 0 ? 1 : -1;
+            gw.push_back(std::fabs(sg * cum / (m->nq * vuc)) * hc.length() * cfg->thickness);
+            if (!build_alias(pmf.data(), n, th.data(), al.data())) throw Error(BTE_EINVAL, "empty wall pmf");
+            for (int i = 0; i < n; i++) gtab.push_back(make_uint2(th[i], al[i]));
+            gens.push_back(RtaGen{hc.p1x, hc.p1y, hc.p2x - hc.p1x, hc.p2y - hc.p1y, hc.box, sg, 0, 0});
+        }
+    }
+    double Ewall = 0;
+    for (double w : gw) Ewall += w;
+    // gradient generators: Nabla_T_distribution per regular box
+    // (sampling.cpp:118-187; energy vol/vuc * cumulsum / nq, RTA_MC2D.cpp:402-427)
+    const double gx = cfg->gradient[0], gy = cfg->gradient[1];
+    std::vector<int32_t> gbox, gclass;
+    std::vector<double> gE;
+    std::vector<uint2> gt;
+    if (!almost_equal(std::hypot(gx, gy), 0.)) {
+        std::vector<double> pp(n), pm(n);
+        for (int b = 0; b < nbox; b++) {
+            const RectBox& B = g->boxes[b];
+            if (B.kind != 0) continue;
+            double acc = 0;
+            for (int i = 0; i < n; i++) {
+                pp[i] = pm[i] = 0.;
+                if (m->omega[i] == 0.) continue;
+                const double vgt = -(gx * m->vx[i] + gy * m->vy[i]);
+                const double p = std::fabs(vgt * h_be_kernel(m->omega[i], B.Teq));
+                acc += p;
+                (vgt >= 0. ? pp : pm)[i] = p;
+            }
+            int cls = 0;
+            for (int sgn = 0; sgn < 2; sgn++) {
+                if (build_alias((sgn == 0 ? pp : pm).data(), n, th.data(), al.data())) cls |= 1 << sgn;
+                for (int i = 0; i < n; i++) gt.push_back(make_uint2(th[i], al[i]));
+            }
+            gbox.push_back(b);
+            gclass.push_back(cls);
+            gE.push_back(B.area() * cfg->thickness / vuc * kB * acc / m->nq);
+        }
+    }
+    double Egrad = 0;
+    for (double w : gE) Egrad += w;
+    const double Etot = Ewall + Egrad;
+    if (!(Etot > 0)) throw Error(BTE_EINVAL, "no isothermal walls or gradient sources");
+    const double Eeff = Etot / (double)cfg->nparticles;
+    std::vector<uint32_t> sth, sal;
+    DevBuf<RtaGen> d_gens;
+    if (!gens.empty()) {
+        sth.resize(gw.size());
+        sal.resize(gw.size());
+        build_alias(gw.data(), (int)gw.size(), sth.data(), sal.data());
+        upload_alias(d_gen_sel, sth, sal, s);
+        d_gens.upload(gens, s);
+        d_gen_tab.upload(gtab, s);
+    }
+    DevBuf<int32_t> d_gbox, d_gclass, d_perm, d_mirror;
+    if (!gbox.empty()) {
+        sth.resize(gE.size());
+        sal.resize(gE.size());
+        build_alias(gE.data(), (int)gE.size(), sth.data(), sal.data());
+        upload_alias(d_gbox_sel, sth, sal, s);
+        d_gbox.upload(gbox, s);
+        d_gclass.upload(gclass, s);
+        d_gtab.upload(gt, s);
+    }
+    std::vector<int4> info;
+    std::vector<uint2> ltab;
+    std::vector<int32_t> perm;
+    build_lambert(*m, cfg->sigma_bounce > 0 ? cfg->sigma_bounce : 0.25, info, ltab, perm);
+    DevBuf<int4> d_lam_info;
+    d_lam_info.upload(info, s);
+    d_lam_tab.upload(ltab, s);
+    d_perm.upload(perm, s);
+    const uint64_t cap = cfg->nparticles;
+    DevBuf<RtaSPart> d_parts;
+    d_parts.alloc(2 * cap);
+    DevBuf<double> d_acc, d_phi;
+    d_acc.alloc(3 * (size_t)nbox);
+    d_phi.alloc(nbox);
+    DevBuf<uint64_t> d_regen;
+    d_regen.alloc(nbox + 1);
+    DevBuf<unsigned long long> d_cnt;
+    d_cnt.alloc(4);
+    HIPCHECK(hipMemsetAsync(d_acc.p, 0, 3 * (size_t)nbox * sizeof(double), s));
+    RtaSteadyParams P{};
+    P.modes = d_modes.p;
+    P.ivel = d_ivel.p;
+    P.scat = d_scat.p;
+    P.boxes = g->d_boxes.p;
+    P.cts = g->d_contacts.p;
+    P.cseg = g->d_cseg.p;
+    P.ptrans = g->d_ptrans.p;
+    P.lam_info = d_lam_info.p;
+    P.lam_tab = d_lam_tab.p;
+    P.perm = d_perm.p;
+    P.gens = d_gens.p;
+    P.gen_sel = d_gen_sel.p;
+    P.gen_tab = d_gen_tab.p;
+    P.gbox = d_gbox.p;
+    P.gbox_sel = d_gbox_sel.p;
+    P.gtab = d_gtab.p;
+    P.gclass = d_gclass.p;
+    P.regen = d_regen.p;
+    P.acc = d_acc.p;
+    P.phi = d_phi.p;
+    P.counters = d_cnt.p;
+    P.seed = cfg->seed;
+    P.pwall = Ewall / Etot;
+    P.max_iters = 1ULL << 26;
+    P.n = n;
+    P.nbox = nbox;
+    P.ngen = (int)gens.size();
+    P.ngbox = (int)gbox.size();
+    P.lds_tally = nbox <= 1024 ? 1 : 0;
+    const size_t lds = P.lds_tally ? 4 * (size_t)nbox * sizeof(double) : 0;
+    const int blocks = c->cu_count * 8;
+    RtaSPart* buf[2] = {d_parts.p, d_parts.p + cap};
+    int cur = 0;
+    P.pout = buf[cur];
+    hipLaunchKernelGGL(rta_steady_init_kernel, dim3(blocks), dim3(256), 0, s, P, cap);
+    HIPCHECK(hipGetLastError());
+    uint64_t nin = cap;
+    std::vector<double> acc(3 * (size_t)nbox, 0.), old(3 * (size_t)nbox), phi(nbox);
+    std::vector<uint64_t> pre(nbox + 1, 0);
+    int gen = 0;
+    bool converged = false;
+    unsigned long long flights = 0;
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    struct EvG {
+        hipEvent_t a, b;
+        ~EvG() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+    } evg{e0, e1};
+    double kms = 0;
+    const int maxgen = cfg->max_generations > 0 ? cfg->max_generations : 1000000;
+    while (nin > 0 && !converged && gen < maxgen) {
+        P.pin = buf[cur];
+        P.pout = buf[cur ^ 1];
+        P.nin = nin;
+        uint64_t z = cfg->seed + 0x9e3779b97f4a7c15ULL * (uint64_t)(gen + 1);
+        P.key = mix64(z ^ 0x57ead7ULL);
+        HIPCHECK(hipMemsetAsync(d_phi.p, 0, nbox * sizeof(double), s));
+        HIPCHECK(hipMemsetAsync(d_cnt.p, 0, 4 * sizeof(unsigned long long), s));
+        HIPCHECK(hipEventRecord(e0, s));
+        hipLaunchKernelGGL(rta_steady_flight_kernel, dim3(blocks), dim3(256), lds, s, P);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(e1, s));
+        unsigned long long cnt[4];
+        old = acc;
+        HIPCHECK(hipMemcpyAsync(acc.data(), d_acc.p, acc.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(phi.data(), d_phi.p, nbox * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(cnt, d_cnt.p, sizeof(cnt), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+        kms += ms;
+        if (cnt[2]) throw Error(BTE_EGEOM, "a flight exceeded the iteration budget");
+        if (cnt[3]) throw Error(BTE_EGEOM, "Error in translation, no box found");
+        flights += cnt[1];
+        // re-emission counts (RTA_MC2D.cpp:2637-2654); pre outlives the copy
+        pre[0] = 0;
+        for (int b = 0; b < nbox; b++) {
+            const double a = std::fabs(phi[b]);
+            pre[b + 1] = pre[b] + (almost_equal(phi[b], 0.) ? 0 : (uint64_t)(a + 1.0e-9));
+        }
+        const uint64_t nb = cnt[0], nre = pre[nbox];
+        if (nb + nre > cap) throw Error(BTE_ECAPACITY, "steady generation larger than the first");
+        if (nre) {
+            d_regen.upload(pre.data(), nbox + 1, s);
+            hipLaunchKernelGGL(rta_steady_regen_kernel, dim3(blocks), dim3(256), 0, s, P, nb, nre);
+            HIPCHECK(hipGetLastError());
+        }
+        // convergence (RTA_MC2D.cpp:2660-2692)
+        auto rel = [&](int q) {
+            double d = 0, sq = 0;
+            for (int b = 0; b < nbox; b++) {
+                const double o = old[(size_t)q * nbox + b], v = acc[(size_t)q * nbox + b];
+                d += (o - v) * (o - v);
+                sq += v * v;
+            }
+            return std::sqrt(d) / std::sqrt(sq);
+        };
+        const double rgz = rel(0), rjx = rel(1), rjy = rel(2);
+        if (almost_equal(gx, 0.)) converged = rgz < cfg->eps_energy && rjy < cfg->eps_flux;
+        else if (almost_equal(gy, 0.)) converged = rgz < cfg->eps_energy && rjx < cfg->eps_flux;
+        else converged = rgz < cfg->eps_energy && rjx < cfg->eps_flux && rjy < cfg->eps_flux;
+        nin = nb + nre;
+        cur ^= 1;
+        gen++;
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    // post-processing (RTA_MC2D.cpp:2696-2728)
+    const double vcorr = cfg->thickness / m->lattz, cv = h_calc_cv(*m, cfg->T0);
+    for (int b = 0; b < nbox; b++) {
+        const double vbox = g->boxes[b].area() * cfg->thickness;
+        T_out[b] = acc[b] / vbox * Eeff / (cv / vcorr) + cfg->T0;
+        jx_out[b] = acc[nbox + b] / vbox * Eeff * 1e30;
+        jy_out[b] = acc[2 * nbox + b] / vbox * Eeff * 1e30;
+    }
+    if (rep) {
+        rep->generations = gen;
+        rep->flights = flights;
+        rep->Eeff = Eeff;
+        rep->kernel_ms = kms;
+        rep->converged = converged ? 1 : 0;
+    }
+    BTE_GUARD_END
+}

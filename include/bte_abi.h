@@ -115,6 +115,30 @@ int bte_rta_destroy(bte_rta* r);
 int bte_rta_run(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_cfg* cfg,
                 double* T_out, double* jx_out, double* jy_out, bte_rta_report* rep);
 
+/* ---- RTAMC2D steady branch (a nonzero <gradient>; replaces the
+ * generation loop of src/RTA_MC2D.cpp:2544-2782 with Nabla_T sources,
+ * RTA_MC2D.cpp:347-533, and the steady evolparticle, 1406-1799) ---- */
+typedef struct {
+    double thickness, T0;
+    double gradient[2];      /* <gradient x y> [K/nm] */
+    uint64_t nparticles, seed;
+    double sigma_bounce;
+    double eps_energy, eps_flux; /* <convergence energy flux>; <= 0: until no particle is left */
+    int max_generations;     /* 0: no cap */
+} bte_rta_steady_cfg;
+
+typedef struct {
+    int generations;
+    int converged;
+    uint64_t flights;
+    double Eeff;
+    double kernel_ms;        /* device time of the flight kernels */
+} bte_rta_steady_report;
+
+/* Outputs per box (the rows of steady_<T0>K_run_<i>.csv): T [K], jx, jy [W/m^2]. */
+int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_steady_cfg* cfg,
+                       double* T, double* jx, double* jy, bte_rta_steady_report* rep);
+
 /* ---- batched RTAMC2D: independent runs of one material (geometry,
  * temperature, particle count, seed and edge specularity per run) in one
  * persistent launch. The reference runs its nruns one after another

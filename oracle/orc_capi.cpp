@@ -39,6 +39,7 @@ void synthetic_B(int N, const double* C, const double* w, double density, uint64
 // full definitions are needed for the setup structs
 #include "orc_rta.cpp"
 #include "orc_bert.cpp"
+#include "orc_rta_steady.cpp"
 
 using namespace orc;
 
@@ -275,6 +276,37 @@ int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, 
 }
 
 // ---------------- beRTAMC2D ----------------
+// steady branch (nonzero gradient): per box T, jx, jy
+int orc_rta_steady_run(int n, const double* omega, const double* vx, const double* vy, const double* w,
+                       double V, double lattz, int nq, int nbox, const double* rect, const int32_t* kind,
+                       const double* Teq, double thickness, double T0, double gx, double gy, double eps_energy,
+                       double eps_flux, int max_generations, uint64_t nparticles, uint64_t seed, double sigma,
+                       int nthreads, double* T_out, double* jx_out, double* jy_out, int32_t* generations,
+                       double* Eeff_out) {
+    try {
+        SteadySetup SS;
+        RtaSetup& S = SS.base;
+        S.m = mk_modes(n, omega, vx, vy, w, V, lattz, nq);
+        S.sys = mk_sys(nbox, rect, kind, Teq, nullptr);
+        S.thickness = thickness;
+        S.T0 = T0;
+        S.dt = 1.0;
+        S.maxtime = 1.0;
+        S.sigma = sigma;
+        SS.gx = gx;
+        SS.gy = gy;
+        SS.eps_energy = eps_energy;
+        SS.eps_flux = eps_flux;
+        if (max_generations > 0) SS.max_generations = max_generations;
+        int gens = 0;
+        rta_steady_run(SS, nparticles, seed, nthreads, T_out, jx_out, jy_out, &gens, Eeff_out);
+        *generations = gens;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
 int orc_bert_run(int n, const double* omega, const double* vx, const double* vy, const double* w,
                  double V, double lattz, int nq, int nbox, const double* rect, const int32_t* kind,
                  const double* Teq, const double* Tinit, double thickness, double dt,

@@ -178,6 +178,28 @@ def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, 
             "Eeff": eeff.value}
 
 
+def rta_steady_run(mat, rect, kind, Teq, thickness, T0, gradient, nparticles, seed, sigma=0.25, eps_energy=-42.0,
+                   eps_flux=-42.0, max_generations=0, nthreads=0, periodic=None):
+    """steady branch: per box T, jx, jy and the number of generations"""
+    rect, Teq = _f(rect), _f(Teq)
+    kind = np.ascontiguousarray(kind, np.int32)
+    nbox = len(kind)
+    set_periodic(nbox, periodic)
+    T, jx, jy = np.zeros(nbox), np.zeros(nbox), np.zeros(nbox)
+    gens = C.c_int32()
+    eeff = C.c_double()
+    om, vx, vy, w = (_f(mat[k]) for k in ("omega", "vx", "vy", "w"))
+    rc = lib().orc_rta_steady_run(len(om), _dp(om), _dp(vx), _dp(vy), _dp(w), C.c_double(mat["V"]),
+                                  C.c_double(mat.get("lattz", 1.0)), 1, nbox, _dp(rect), kind.ctypes.data_as(ip32),
+                                  _dp(Teq), C.c_double(thickness), C.c_double(T0), C.c_double(gradient[0]),
+                                  C.c_double(gradient[1]), C.c_double(eps_energy), C.c_double(eps_flux),
+                                  int(max_generations), u64(nparticles), u64(seed), C.c_double(sigma), nthreads,
+                                  _dp(T), _dp(jx), _dp(jy), C.byref(gens), C.byref(eeff))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return {"T": T, "jx": jx, "jy": jy, "generations": gens.value, "Eeff": eeff.value}
+
+
 def bert_run(mat, rect, kind, Teq, Tinit, thickness, dt, maxtime, P=None, ngamma=0, rta=False, Ed=-1.0,
              Efactor=0, Tmax=0.0, Tmin=0.0, seed=1, max_steps=0, nthreads=0, periodic=None):
     rect, Teq, Tinit = _f(rect), _f(Teq), _f(Tinit)
