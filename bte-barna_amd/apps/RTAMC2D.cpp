@@ -7,8 +7,10 @@
 //
 // Input (<RTA_MC2d>): <geometry file/>, <time dt maxtime/>, <material .../>
 // (see front_common.hpp for the mode-table attributes), <particles N
-// [seed]/>, optional <ballistic/>. Output per run: temperature_<T0>K_run_<i>.csv,
-// jx_<T0>K_run_<i>.csv, jy_<T0>K_run_<i>.csv with rows t = (i + 0.5) dt.
+// [seed]/>, optional <ballistic/>, <gradient x y/>, <convergence energy flux/>.
+// Output per run: temperature_<T0>K_run_<i>.csv, jx_<T0>K_run_<i>.csv,
+// jy_<T0>K_run_<i>.csv with rows t = (i + 0.5) dt; with a nonzero gradient
+// the steady branch writes steady_<T0>K_run_<i>.csv (rows T, jx, jy).
 #include "front_common.hpp"
 #include <chrono>
 
@@ -30,9 +32,18 @@ static int run(int argc, char** argv) {
     if (!geo || !tim || !mat || !par) throw std::runtime_error("<RTA_MC2d> needs geometry, time, material and particles");
     if (root->children("material").size() > 1 || root->child("layer") || root->child("layers_connection"))
         throw std::runtime_error("multi-material inputs (DMM interfaces) are not supported");
-    if (const xml::Node* g = root->child("gradient"))
-        if (g->num("x", 0.) != 0. || g->num("y", 0.) != 0.)
-            throw std::runtime_error("the steady-state gradient branch is not supported");
+    double grad[2] = {0., 0.};
+    if (const xml::Node* g = root->child("gradient")) {
+        grad[0] = g->num("x", 0.);
+        grad[1] = g->num("y", 0.);
+    }
+    // a nonzero gradient selects the steady branch (RTA_MC2D.cpp:2212, 2544)
+    const bool steady = !(std::fabs(std::hypot(grad[0], grad[1])) < 1e-8);  // almost_equal(norm, 0)
+    double eps_energy = -42., eps_flux = -42.;
+    if (const xml::Node* cv = root->child("convergence")) {
+        eps_energy = cv->num("energy");
+        eps_flux = cv->num("flux");
+    }
     if (root->child("spectral")) std::cout << "# <spectral> tallies are not produced by the GPU engine" << std::endl;
     Boxes B = read_geometry(geo->str("file"));
     Material M = read_material(*mat);
@@ -61,7 +72,34 @@ static int run(int argc, char** argv) {
     const uint64_t seed0 = (uint64_t)par->num("seed", 1);
     std::cout << "# RTAMC2D on the GPU: " << B.n() << " boxes, " << M.n << " modes, " << cfg.nparticles
               << " particles, dt " << cfg.dt << " ps, maxtime " << cfg.maxtime << " ps" << std::endl;
-    for (int irun = 0; irun < nruns; irun++) {
+    for (int irun = 0; irun < nruns && steady; irun++) {
+        bte_rta_steady_cfg sc{};
+        sc.thickness = M.thickness;
+        sc.T0 = M.T0;
+        sc.gradient[0] = grad[0];
+        sc.gradient[1] = grad[1];
+        sc.nparticles = cfg.nparticles;
+        sc.seed = seed0 + (uint64_t)irun;
+        sc.sigma_bounce = M.sigma_bounce;
+        sc.eps_energy = eps_energy;
+        sc.eps_flux = eps_flux;
+        std::vector<double> T(B.n()), jx(B.n()), jy(B.n());
+        bte_rta_steady_report rep{};
+        check(bte_rta_steady_run(ctx, modes, geom, &sc, T.data(), jx.data(), jy.data(), &rep), "RTAMC2D steady run");
+        std::cout << "# run " << irun << ": " << rep.generations << " generations, " << rep.flights << " flights"
+                  << (rep.converged ? ", converged" : "") << std::endl;
+        char name[96];
+        std::snprintf(name, sizeof name, "steady_%gK_run_%i.csv", M.T0, irun);
+        std::vector<double> out(3 * (size_t)B.n());
+        for (int b = 0; b < B.n(); b++) {
+            out[b] = T[b];
+            out[B.n() + b] = jx[b];
+            out[2 * B.n() + b] = jy[b];
+        }
+        std::cout << "# Writing steady state" << std::endl;
+        write_csv(name, 3, B.n(), out.data());
+    }
+    for (int irun = 0; irun < nruns && !steady; irun++) {
         cfg.seed = seed0 + (uint64_t)irun;
         // rows: (i + 0.5) dt for i < ntime - 1 with ntime = floor(maxtime/dt) + 1
         const int nt = (int)std::floor(cfg.maxtime / cfg.dt + 1.0e-6);

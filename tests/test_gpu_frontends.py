@@ -180,3 +180,22 @@ def test_beam_layout_runs_in_both_solvers(pkg, tmp_path):
     assert rows.shape[1] == 3 + 3 * 44
     # particles enter from the hot reservoir and energy appears in the first rows
     assert rows[-1, 2] > 0 and rows[-1, 3] > 0
+
+
+def test_rtamc2d_steady_csv(pkg, tmp_path):
+    """<gradient> selects the steady branch: steady_<T0>K_run_<i>.csv holds
+    the rows T, jx, jy over the boxes."""
+    W = pkg.workloads
+    rect, kind, teq, _ = W.ribbon(80.0, 30.0, 8, 300.0, reservoirs=(300.0 + 0.08, 300.0 - 0.08))
+    write_ribbon(tmp_path / "g.xml", rect, kind, teq, teq)
+    (tmp_path / "in.xml").write_text(
+        '<RTA_MC2d>\n<geometry file="g.xml"/>\n<time dt="20" maxtime="200"/>\n<gradient x="-2.0e-3" y="0.0"/>\n'
+        '<convergence energy="1e-3" flux="1e-3"/>\n'
+        '<material name="m" modetable="synthetic" nmodes="300" seed="5" thickness="1" T0="300" sigma_bounce="2.0"/>\n'
+        '<particles N="50000" seed="3"/>\n</RTA_MC2d>\n')
+    r = run([pkg.app_path("RTAMC2D"), "in.xml", "8"], tmp_path)
+    assert "generations" in r.stdout
+    out = np.loadtxt(tmp_path / "steady_300K_run_0.csv", delimiter=",")
+    assert out.shape == (3, 10)
+    assert (out[1, :8] > 0).all()  # heat flows down the imposed gradient
+    assert np.abs(out[0, :8] - 300.0).max() < 1.0
