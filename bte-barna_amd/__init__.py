@@ -127,17 +127,37 @@ class Geometry(_Handle):
                                       nbr.ctypes.data_as(C.POINTER(C.c_int32)), _dp(geo)))
         return nc, nbr, geo
 
+    def gradient_T(self):
+        """calculate_gradientT: (nbox, 2) gradients [K/nm] used by the
+        gradient sources"""
+        g = np.zeros((self.nbox, 2))
+        check(lib().bte_geom_gradient_T(self.h, _dp(g)))
+        return g
+
 
 class Propagator(_Handle):
-    """Dense propagator P (ncol x ncol) acting on modes [ngamma, ngamma+ncol)."""
+    """Dense propagator P (ncol x ncol) acting on modes [ngamma, ngamma+ncol).
+    P may be a list of matrices tabulated at temperatures T (a list); a
+    single matrix with T = 0 applies at every box temperature."""
     _destroy = "bte_prop_destroy"
 
     def __init__(self, ctx, P, ngamma=0, T=0.0):
         super().__init__()
-        P = np.asfortranarray(P, dtype=np.float64)
-        self.ncol = P.shape[0]
-        check(lib().bte_prop_create(ctx.h, self.ncol, P.ctypes.data_as(C.POINTER(C.c_double)), ngamma, T,
-                                    C.byref(self.h)))
+        Ps = list(P) if isinstance(P, (list, tuple)) else [P]
+        Ts = np.atleast_1d(np.asarray(T, dtype=np.float64))
+        if len(Ts) != len(Ps):
+            raise ValueError("one temperature per propagator")
+        self.ncol = np.asarray(Ps[0]).shape[0]
+        flat = np.concatenate([np.asfortranarray(x, dtype=np.float64).ravel(order="F") for x in Ps])
+        check(lib().bte_prop_create_multi(ctx.h, self.ncol, len(Ps), _dp(flat), _dp(Ts), ngamma, C.byref(self.h)))
+
+
+def prop_pick_pmf(signed_cumm):
+    """Probabilities of propagator_H::get over a signed cumulative (host only)."""
+    sc = np.ascontiguousarray(signed_cumm, dtype=np.float64)
+    pmf = np.zeros(len(sc))
+    check(lib().bte_prop_pick_pmf(len(sc), _dp(sc), _dp(pmf)))
+    return pmf
 
 
 class RtaRun(_Handle):
@@ -245,10 +265,15 @@ class BertRun(_Handle):
     _destroy = "bte_bert_destroy"
 
     def __init__(self, ctx, modes, geom, prop, thickness, dt, maxtime, max_steps=0, Ed=-1.0,
-                 Efactor=0, Tmax=0.0, Tmin=0.0, rta=False, pcancel_each=4, seed=1, capacity=0):
+                 Efactor=0, Tmax=0.0, Tmin=0.0, rta=False, pcancel_each=4, seed=1, capacity=0,
+                 gradient=None, prop_interp=0):
+        """gradient: (gx, gy) [K/nm] in every regular box (<gradient x y>);
+        None derives per-box gradients from the box temperatures"""
         super().__init__()
+        gx, gy = gradient if gradient is not None else (0.0, 0.0)
         self.cfg = BertCfg(thickness, dt, maxtime, max_steps, Ed, Efactor, Tmax, Tmin, int(rta),
-                           pcancel_each, seed, capacity)
+                           pcancel_each, seed, capacity, int(gradient is not None), (C.c_double * 2)(gx, gy),
+                           int(prop_interp))
         self.nbox = geom.nbox
         nst = 0
         t = 0.0

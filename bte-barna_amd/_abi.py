@@ -58,7 +58,8 @@ class BertCfg(C.Structure):
     _fields_ = [("thickness", C.c_double), ("dt", C.c_double), ("maxtime", C.c_double),
                 ("max_steps", C.c_int), ("Ed", C.c_double), ("Efactor", C.c_uint64),
                 ("Tmax", C.c_double), ("Tmin", C.c_double), ("rta", C.c_int),
-                ("pcancel_each", C.c_int), ("seed", C.c_uint64), ("capacity", C.c_uint64)]
+                ("pcancel_each", C.c_int), ("seed", C.c_uint64), ("capacity", C.c_uint64),
+                ("fixed_gradient", C.c_int), ("gradient", C.c_double * 2), ("prop_interp", C.c_int)]
 
 
 class BertReport(C.Structure):
@@ -99,6 +100,7 @@ SIGNATURES = {
     "bte_geom_destroy": (C.c_int, [vp]),
     "bte_geom_set_periodic": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double]),
     "bte_geom_contacts": (C.c_int, [vp, C.c_int, ip32, ip32, dp]),
+    "bte_geom_gradient_T": (C.c_int, [vp, dp]),
     "bte_rta_create": (C.c_int, [vp, vp, vp, C.POINTER(RtaCfg), pvp]),
     "bte_rta_execute": (C.c_int, [vp, dp, dp, dp, C.POINTER(RtaReport)]),
     "bte_rta_destroy": (C.c_int, [vp]),
@@ -110,6 +112,8 @@ SIGNATURES = {
     "bte_rta_batch_execute": (C.c_int, [vp, dp, dp, dp, dp, dp, C.POINTER(RtaBatchReport)]),
     "bte_rta_batch_destroy": (C.c_int, [vp]),
     "bte_prop_create": (C.c_int, [vp, C.c_int, dp, C.c_int, C.c_double, pvp]),
+    "bte_prop_create_multi": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, C.c_int, pvp]),
+    "bte_prop_pick_pmf": (C.c_int, [C.c_int, dp, dp]),
     "bte_prop_destroy": (C.c_int, [vp]),
     "bte_bert_create": (C.c_int, [vp, vp, vp, vp, C.POINTER(BertCfg), pvp]),
     "bte_bert_execute": (C.c_int, [vp, dp, ip64, ip64, FRAME_CB, vp, C.POINTER(BertReport)]),

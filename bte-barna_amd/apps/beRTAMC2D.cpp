@@ -1,14 +1,16 @@
 // beRTAMC2D front-end on the GPU engine: src/beyondRTA_MC2d.cpp
 // (process_input 1299-1462, main 1902-2434) for rectangular single-material
-// geometries with a uniform equilibrium temperature.
+// geometries.
 //
 // usage: beRTAMC2D <inputfile.xml> nthreads
 //
 // Input (<beRTAMC2D>): <geometry file/>, <time dt maxtime/>, <material ...>
-// with a <propagator T file/> child (a .P.eigen.bin over the modes after
-// the Gamma modes, or file="synthetic" with density, seed, m and precision
-// attributes to build exp(dt B) on the GPU from the synthetic operator),
-// <Eeff particles|Ed [Tmax Tmin]/>, optional <RTA/>, optional <seed value/>.
+// with one or more <propagator T file/> children (a .P.eigen.bin over the
+// modes after the Gamma modes, or file="synthetic" with density, seed, m and
+// precision attributes to build exp(dt B) on the GPU from the synthetic
+// operator at T), <Eeff particles|Ed [Tmax Tmin]/>, optional <RTA/>,
+// optional <gradient x y/> (else gradients from the box temperatures),
+// optional <seed value/>, optional <interpolation rule="bracket"/>.
 // Output: one line per step on stdout, "istep time Npart (ed jx jy) per box",
 // and properties.msgpack.bin with one frame per step.
 #include "front_common.hpp"
@@ -64,36 +66,46 @@ static int run(int argc, char** argv) {
     const double dt = tim->num("dt");
     bte_prop* prop = nullptr;
     if (!rta) {
-        const xml::Node* pn = mat->child("propagator");
-        if (!pn) throw std::runtime_error("material needs a <propagator T file/> unless <RTA/> is given");
-        const double T = pn->num("T");
-        const std::string file = pn->str("file");
-        std::vector<double> P;
-        int64_t rows = 0;
-        if (file == "synthetic") {
-            // synthetic energy-conserving operator at T, P = exp(dt B) on the GPU
-            std::vector<double> C(M.n), Bm((size_t)M.n * M.n);
-            for (int i = 0; i < M.n; i++) C[i] = kB * be_kernel(M.omega[i], T);
-            check(bte_synthetic_B(M.n, C.data(), M.w.data(), pn->num("density", 0.05), (uint64_t)pn->num("seed", 1),
-                                  200, Bm.data()),
-                  "synthetic collision operator");
-            int m = (int)pn->num("m", 0);
-            if (m <= 0) check(bte_krylov_size(M.n, Bm.data(), dt, 1e-8, &m), "Krylov size");
-            if (m > 64) throw std::runtime_error("Krylov size " + std::to_string(m) + " above 64; give m");
-            const std::string pr = pn->str("precision", "fp64");
-            const int prec = pr == "fp32" ? BTE_PREC_FP32 : (pr == "bf16" ? BTE_PREC_BF16 : BTE_PREC_FP64);
-            P.resize((size_t)M.n * M.n);
-            bte_krylov_report kr{};
-            check(bte_krylov_build(ctx, M.n, Bm.data(), dt, m, prec, BTE_ARNOLDI_TEXTBOOK, P.data(), &kr),
-                  "Krylov propagator");
-            rows = M.n;
-            std::cout << "# propagator: synthetic operator, m = " << m << ", " << pr << ", " << kr.build_ms
-                      << " ms on the GPU" << std::endl;
-        } else {
-            P = load_eigen_bin(file, rows);
+        // every <propagator T file/> child (beyondRTA_MC2d.cpp:1355-1376)
+        const auto pns = mat->children("propagator");
+        if (pns.empty()) throw std::runtime_error("material needs a <propagator T file/> unless <RTA/> is given");
+        std::vector<double> Ps, Ts;
+        int64_t rows = -1;
+        for (const xml::Node* pn : pns) {
+            const double T = pn->num("T");
+            const std::string file = pn->str("file");
+            std::vector<double> P;
+            int64_t r = 0;
+            if (file == "synthetic") {
+                // synthetic energy-conserving operator at T, P = exp(dt B) on the GPU
+                std::vector<double> C(M.n), Bm((size_t)M.n * M.n);
+                for (int i = 0; i < M.n; i++) C[i] = kB * be_kernel(M.omega[i], T);
+                check(bte_synthetic_B(M.n, C.data(), M.w.data(), pn->num("density", 0.05),
+                                      (uint64_t)pn->num("seed", 1), 200, Bm.data()),
+                      "synthetic collision operator");
+                int m = (int)pn->num("m", 0);
+                if (m <= 0) check(bte_krylov_size(M.n, Bm.data(), dt, 1e-8, &m), "Krylov size");
+                if (m > 64) throw std::runtime_error("Krylov size " + std::to_string(m) + " above 64; give m");
+                const std::string pr = pn->str("precision", "fp64");
+                const int prec = pr == "fp32" ? BTE_PREC_FP32 : (pr == "bf16" ? BTE_PREC_BF16 : BTE_PREC_FP64);
+                P.resize((size_t)M.n * M.n);
+                bte_krylov_report kr{};
+                check(bte_krylov_build(ctx, M.n, Bm.data(), dt, m, prec, BTE_ARNOLDI_TEXTBOOK, P.data(), &kr),
+                      "Krylov propagator");
+                r = M.n;
+                std::cout << "# propagator at " << T << " K: synthetic operator, m = " << m << ", " << pr << ", "
+                          << kr.build_ms << " ms on the GPU" << std::endl;
+            } else {
+                P = load_eigen_bin(file, r);
+            }
+            if (r > M.n) throw std::runtime_error("propagator larger than the mode table");
+            if (rows >= 0 && r != rows) throw std::runtime_error("propagators of different sizes");
+            rows = r;
+            Ps.insert(Ps.end(), P.begin(), P.end());
+            Ts.push_back(T);
         }
-        if (rows > M.n) throw std::runtime_error("propagator larger than the mode table");
-        check(bte_prop_create(ctx, (int)rows, P.data(), M.n - (int)rows, T, &prop), "propagator");
+        check(bte_prop_create_multi(ctx, (int)rows, (int)Ts.size(), Ps.data(), Ts.data(), M.n - (int)rows, &prop),
+              "propagator");
     }
     bte_bert_cfg cfg{};
     cfg.thickness = M.thickness;
@@ -106,6 +118,17 @@ static int run(int argc, char** argv) {
     cfg.rta = rta ? 1 : 0;
     cfg.pcancel_each = 4;
     cfg.seed = root->child("seed") ? (uint64_t)root->child("seed")->num("value") : 1;
+    // <gradient x y>: the same gradient in every regular box instead of
+    // calculate_gradientT (beyondRTA_MC2d.cpp:1328-1333, 1441-1460)
+    if (const xml::Node* gn = root->child("gradient")) {
+        cfg.fixed_gradient = 1;
+        cfg.gradient[0] = gn->num("x");
+        cfg.gradient[1] = gn->num("y");
+    }
+    // <interpolation rule="bracket"/>: interpolate untabulated box
+    // temperatures between the adjacent propagators (engine option)
+    if (const xml::Node* in = root->child("interpolation"))
+        cfg.prop_interp = in->str("rule", "reference") == "bracket" ? 1 : 0;
     bte_bert* run = nullptr;
     check(bte_bert_create(ctx, modes, geom, prop, &cfg, &run), "beRTAMC2D setup");
     // steps while time <= maxtime (beyondRTA_MC2d.cpp:2138)

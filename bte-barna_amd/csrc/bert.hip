@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 namespace bte {
 
@@ -44,7 +45,8 @@ struct BertParams {
     const DevContact* cts;
     const double4* cseg;     // contact segments (periodic translation)
     const double2* ptrans;   // periodic translation per box
-    const uint2* palias;     // propagator alias, ncol x ncol column-major
+    const uint2* palias;     // propagator alias per slot, ncol x ncol column-major
+    const int32_t* box_pslot; // propagator slot per box (nslots > 1)
     const uint2* rta_alias;  // BE_derivative alias per box-temperature slot
     const int32_t* box_rta_slot;
     const double* box_Teq;
@@ -53,7 +55,7 @@ struct BertParams {
     BPart* pin;
     BPart* pout;
     uint32_t* seg;           // [2][nsegs] live count per buffer segment
-    int32_t* H;              // [2][nbox][n]
+    unsigned long long* H;   // [2][nbox][n] particle counts: positive | negative << 32
     double* pbox;            // [nsegs][nbox][3] per-workgroup (sum h, sum h vx, sum h vy)
     uint32_t* tickets;       // [nbox][n]
     int32_t* Eb;             // [2][nbox*4]
@@ -75,6 +77,7 @@ struct BertParams {
     double dt, Eeff, thickness;
     int32_t n, ngamma, ncol, nbox, nseg;
     int32_t rta, max_children;
+    int32_t nslots;          // propagator slots (box temperatures)
     int32_t lds_props;       // properties from per-workgroup box sums (nbox <= BERT_LDS_BOXES)
     // dynamic LDS layout of the particle kernels (byte offsets)
     int32_t lds_pre, lds_eb, lds_sbox, lds_boxes, lds_cts;
@@ -193,29 +196,43 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
     return false;
 }
 
-// The first propagator draw of a particle depends only on its mode, which
-// advection does not change, so it is drawn and its alias entry loaded before
-// advection: the gather overlaps the advection arithmetic. Later draws of the
-// same particle (children of negative entries) follow in stream order.
+// The propagator of the box a particle scatters in (one slot per box
+// temperature, Scattering beyondRTA_MC2d.cpp:1131-1166).
+__device__ inline const uint2* slot_table(const BertParams& P, int box) {
+    const size_t nn = (size_t)P.ncol * P.ncol;
+    return P.nslots > 1 ? P.palias + (size_t)P.box_pslot[box] * nn : P.palias;
+}
+
+// The first propagator draw of a particle depends only on its mode and, with
+// several box temperatures, on the box it scatters in. With one propagator it
+// is drawn and its alias entry loaded before advection, so the gather
+// overlaps the advection arithmetic; otherwise after advection. Later draws
+// of the same particle (children of negative entries) follow in stream order.
 struct Pick0 {
     uint32_t k, u;
     uint2 e;
 };
-__device__ inline Pick0 first_pick(const BertParams& P, Pcg32& rng, int mode) {
+__device__ inline Pick0 first_pick(const BertParams& P, Pcg32& rng, int mode, int box) {
     Pick0 f{0u, 0u, make_uint2(0u, 0u)};
     const int col = mode - P.ngamma;
     if (P.rta || col < 0 || col >= P.ncol) return f;
     f.k = rng.bounded((uint32_t)P.ncol);
     f.u = rng.next();
-    f.e = P.palias[(size_t)col * P.ncol + f.k];
+    f.e = slot_table(P, box)[(size_t)col * P.ncol + f.k];
     return f;
 }
+
+// The (box, mode) histogram keeps the positive and the negative particle
+// counts in one 64-bit word, so that one atomic tallies a particle and the
+// cancellation knows how many candidates of each sign a bin holds.
+__device__ inline int h_net(unsigned long long v) { return (int)(uint32_t)v - (int)(uint32_t)(v >> 32); }
+__device__ inline unsigned long long h_unit(int sign) { return sign > 0 ? 1ull : (1ull << 32); }
 
 // A child written in this step enters the (box, mode) histogram when that
 // step feeds a cancellation or a frame (H not null) and the box sums behind
 // calculate_prop (sbox in LDS, not null when the boxes fit there).
-__device__ inline void tally_child(const BertParams& P, int32_t* H, double* sbox, int box, int nm, int ns) {
-    if (H) atomicAdd(H + (size_t)box * P.n + nm, ns);
+__device__ inline void tally_child(const BertParams& P, unsigned long long* H, double* sbox, int box, int nm, int ns) {
+    if (H) atomicAdd(H + (size_t)box * P.n + nm, h_unit(ns));
     if (sbox) {
         const double2 v = P.vel[nm];
         atomicAdd(sbox + box * 3 + 0, (double)ns);
@@ -230,7 +247,7 @@ __device__ inline void tally_child(const BertParams& P, int32_t* H, double* sbox
 // them when there is more than one. Children of a sign flip re-queue the
 // parent twice (1189-1192).
 template <bool EMIT>
-__device__ inline int scatter(const BertParams& P, int32_t* H, double* sbox, Pcg32 rng, const Pick0& f0, int mode,
+__device__ inline int scatter(const BertParams& P, unsigned long long* H, double* sbox, Pcg32 rng, const Pick0& f0, int mode,
                               int sign, int box, BPart* out, uint64_t slot, uint64_t limit, double x, double y,
                               uint64_t pid, uint32_t step, int& nm0, int& ns0) {
     if (P.rta) {
@@ -257,7 +274,7 @@ __device__ inline int scatter(const BertParams& P, int32_t* H, double* sbox, Pcg
         atomicOr(P.flags + 1, 1u);
         return 0;
     }
-    const uint2* tab = P.palias + (size_t)col * P.ncol;
+    const uint2* tab = slot_table(P, box) + (size_t)col * P.ncol;
     int pending = 1, nchild = 0;
     while (pending > 0 && nchild < P.max_children) {
         pending--;
@@ -369,7 +386,7 @@ __device__ inline void seg_close(const BertParams& P, uint32_t* segout, int myse
 
 // count the children, reserve their slots in the workgroup's segment and
 // write them; a single child (the common case) is written from pass 0's result
-__device__ inline void scatter_out(const BertParams& P, int32_t* H, double* sbox, Pcg32& rng, const Pick0& f0,
+__device__ inline void scatter_out(const BertParams& P, unsigned long long* H, double* sbox, Pcg32& rng, const Pick0& f0,
                                    bool alive, int mode, int sign, int box, double x, double y, uint64_t pid,
                                    uint32_t step, uint32_t* lds_count, int myseg) {
     int nm0 = 0, ns0 = 1;
@@ -385,6 +402,49 @@ __device__ inline void scatter_out(const BertParams& P, int32_t* H, double* sbox
     } else if (c > 1) {
         scatter<true>(P, H, sbox, rng, f0, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
     }
+}
+
+// A keyed pseudo-random bijection of [0, c), so that the tickets below |h|
+// form a uniformly spread subset. Up to 64 candidates: the rank of the
+// hashed ticket among all hashed tickets (a uniform permutation up to the
+// hash). Above: a 6-round Feistel network on 2*half >= log2(c) bits with
+// cycle-walking (each step is a bijection; fewer than four steps on average).
+__device__ inline uint32_t hash32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ inline uint32_t permute_index(uint32_t t, uint32_t c, uint64_t key) {
+    if (c <= 1) return 0u;
+    const uint32_t k0 = (uint32_t)key, k1 = (uint32_t)(key >> 32);
+    if (c <= 64) {
+        const uint32_t ht = hash32(hash32(t ^ k0) ^ k1);
+        uint32_t r = 0;
+        for (uint32_t u = 0; u < c; u++) {
+            const uint32_t hu = hash32(hash32(u ^ k0) ^ k1);
+            r += (hu < ht || (hu == ht && u < t)) ? 1u : 0u;
+        }
+        return r;
+    }
+    int b = 32 - __clz(c - 1u);
+    b += b & 1;
+    const int half = b >> 1;
+    const uint32_t hm = (1u << half) - 1u;
+    uint32_t x = t;
+    do {
+        uint32_t L = x >> half, R = x & hm;
+        for (int r = 0; r < 6; r++) {
+            const uint32_t F = hash32(R ^ ((r & 1) ? k0 : k1) ^ ((uint32_t)r * 0x9e3779b9u)) & hm;
+            const uint32_t nl = R;
+            R = L ^ F;
+            L = nl;
+        }
+        x = (L << half) | R;
+    } while (x >= c);
+    return x;
 }
 
 // Wall counters are accumulated per workgroup in LDS and flushed with one
@@ -444,9 +504,10 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
     const uint32_t lo = (uint32_t)((uint64_t)nin * blockIdx.x / gridDim.x);
     const uint32_t hi = (uint32_t)((uint64_t)nin * (blockIdx.x + 1) / gridDim.x);
     const int myseg = blockIdx.x;
-    int32_t* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
-    const int32_t* Hprev = P.H + (size_t)(cur ^ 1) * P.nbox * P.n;
+    unsigned long long* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
+    const unsigned long long* Hprev = P.H + (size_t)(cur ^ 1) * P.nbox * P.n;
     const uint64_t skey = stream_key(P.seed, step, 0);
+    const uint64_t ckey = stream_key(P.seed, step, 5);
     unsigned long long done = 0;
     // segment of the wave's first index, advanced as the wave moves on
     int sgw = lo < hi ? seg_search(pre, P.nsegs, lo) : 0;
@@ -471,21 +532,30 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
             alive = true;
             if (cancel_prev) {
                 // cancellation (beyondRTA_MC2d.cpp:2342-2359): keep |h| particles
-                // of sign(h) per (box, mode); tickets order the survivors
-                int h = Hprev[(size_t)box * P.n + mode];
+                // of sign(h) per (box, mode). Which ones is unspecified in the
+                // reference (an unstable sort orders them): the c candidates
+                // take tickets 0..c-1 and a keyed permutation of the tickets
+                // picks the |h| survivors, a subset that does not depend on
+                // the order in which the candidates arrive.
+                const size_t hb = (size_t)box * P.n + mode;
+                const unsigned long long hv = Hprev[hb];
+                const int h = h_net(hv);
                 if (h == 0 || ((sign < 0) != (h < 0))) {
                     alive = false;
                 } else {
-                    uint32_t tk = atomicAdd(P.tickets + (size_t)box * P.n + mode, 1u);
-                    if (tk >= (uint32_t)(h < 0 ? -h : h)) alive = false;
+                    const uint32_t c = h < 0 ? (uint32_t)(hv >> 32) : (uint32_t)hv;
+                    const uint32_t tk = atomicAdd(P.tickets + hb, 1u);
+                    if (tk >= c || permute_index(tk, c, mix64(ckey ^ (uint64_t)hb)) >= (uint32_t)(h < 0 ? -h : h))
+                        alive = false;
                 }
             }
             if (alive) {
                 seed_stream(rng, skey, p.pid);
-                f0 = first_pick(P, rng, mode);
+                if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
                 const double2 v = P.vel[mode];
                 const double2 iv = P.ivel[mode];
                 alive = advect(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
+                if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, box);
             }
         }
         scatter_out(P, H, sbox, rng, f0, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
@@ -616,7 +686,7 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
     const uint32_t total = seg_prefix(P.wcount, nb4, 0xffffffffu, pre, tsum);  // ends with a barrier
     const uint32_t lo = (uint32_t)((uint64_t)total * blockIdx.x / gridDim.x);
     const uint32_t hi = (uint32_t)((uint64_t)total * (blockIdx.x + 1) / gridDim.x);
-    int32_t* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
+    unsigned long long* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
     const uint64_t skey = stream_key(P.seed, step, 1);
     unsigned long long done = 0;
     int bw = lo < hi ? seg_search(pre, nb4, lo) : 0;
@@ -659,8 +729,9 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
             const double2 v = P.vel[mode];
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
-            f0 = first_pick(P, rng, mode);
+            if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
             alive = advect(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
+            if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, bx);
         }
         scatter_out(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
@@ -672,8 +743,11 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
     if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
 }
 
-// Reservoir injection (Isothermal_walls, beyondRTA_MC2d.cpp:446-585): one
-// workgroup per (reservoir, contact) source with a fixed alias table.
+// Fixed-rate sources, one or more workgroups each with a fixed alias table:
+// reservoir injection (Isothermal_walls, beyondRTA_MC2d.cpp:446-585) on a
+// contact segment p1 + t nd with a fixed sign, and gradient sources
+// (source_term_gradient, 597-676) at uniform points of a box (area: p1 the
+// lower corner, nd the box size) with the sign in the alias entry.
 struct BertSource {
     double p1x, p1y, ndx, ndy;
     double expected;  // particles per step
@@ -681,6 +755,7 @@ struct BertSource {
     int32_t tab;      // alias table offset (in entries)
     int32_t src;      // source index (keys its random streams)
     int32_t part, nparts;  // this workgroup's share of the source's particles
+    int32_t area, pad;
 };
 __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src,
                                                           const uint2* stab, uint32_t step, int cur, int hist) {
@@ -700,7 +775,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     const double fl = floor(S.expected);
     const int ng = (int)fl + (cr.uniform() <= S.expected - fl ? 1 : 0);
     const int klo = (int)((int64_t)ng * S.part / S.nparts), khi = (int)((int64_t)ng * (S.part + 1) / S.nparts);
-    int32_t* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
+    unsigned long long* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
     const int nloop = (khi - klo + 63) & ~63;
     unsigned long long done = 0;
     for (int q = threadIdx.x; q - (int)(threadIdx.x & 63) < nloop; q += blockDim.x) {
@@ -708,31 +783,39 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
         bool valid = k < khi;
         Pcg32 rng;
         Pick0 f0{0u, 0u, make_uint2(0u, 0u)};
-        int mode = 0, bx = S.box;
+        int mode = 0, bx = S.box, sign = S.sign;
         double x = 0, y = 0;
         bool alive = false;
         uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)S.src << 20) ^ (uint64_t)k ^ 0x15a0ULL);
         if (valid) {
             seed_stream(rng, skey, pid);
             double t = rng.uniform();
-            x = S.p1x + t * S.ndx;
-            y = S.p1y + t * S.ndy;
-            const DevBox B = P.boxes[S.box];
-            if (!box_inside(B, x, y)) {
-                double dx = 0.5 * (B.xmin + B.xmax) - x, dy = 0.5 * (B.ymin + B.ymax) - y;
-                double dn = sqrt(dx * dx + dy * dy);
-                x += 1.0e-6 * dx / dn;
-                y += 1.0e-6 * dy / dn;
+            if (S.area) {
+                // get_random_point of a rectangle: x, then y
+                x = S.p1x + t * S.ndx;
+                y = S.p1y + rng.uniform() * S.ndy;
+            } else {
+                x = S.p1x + t * S.ndx;
+                y = S.p1y + t * S.ndy;
+                const DevBox B = P.boxes[S.box];
+                if (!box_inside(B, x, y)) {
+                    double dx = 0.5 * (B.xmin + B.xmax) - x, dy = 0.5 * (B.ymin + B.ymax) - y;
+                    double dn = sqrt(dx * dx + dy * dy);
+                    x += 1.0e-6 * dx / dn;
+                    y += 1.0e-6 * dy / dn;
+                }
             }
             bool ngf;
             mode = (int)alias_pick(stab + S.tab, (uint32_t)P.n, rng, ngf);
+            if (S.area) sign = ngf ? -1 : 1;
             const double2 v = P.vel[mode];
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
-            f0 = first_pick(P, rng, mode);
-            alive = advect(P, P.boxes, P.cts, x, y, bx, S.sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
+            if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
+            alive = advect(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
+            if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, bx);
         }
-        scatter_out(P, H, sbox, rng, f0, alive, mode, S.sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -749,8 +832,8 @@ __global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, 
                                                          int clear_tickets, int ebn, int from_boxes,
                                                          int clear_old) {
     const int box = blockIdx.x;
-    const int32_t* H = P.H + (size_t)cur * P.nbox * P.n + (size_t)box * P.n;
-    int32_t* Hold = P.H + (size_t)(cur ^ 1) * P.nbox * P.n + (size_t)box * P.n;
+    const unsigned long long* H = P.H + (size_t)cur * P.nbox * P.n + (size_t)box * P.n;
+    unsigned long long* Hold = P.H + (size_t)(cur ^ 1) * P.nbox * P.n + (size_t)box * P.n;
     __shared__ double red[3][256];
     __shared__ long long redc[2][256];
     double s = 0, fx = 0, fy = 0;
@@ -758,7 +841,7 @@ __global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, 
     if (!from_boxes || cancel_now || clear_old || clear_tickets) {
         for (int k = threadIdx.x; k < P.n; k += blockDim.x) {
             if (!from_boxes || cancel_now) {
-                int h = H[k];
+                const int h = h_net(H[k]);
                 if (h != 0) {
                     const double2 v = P.vel[k];
                     s += h;
@@ -837,14 +920,14 @@ __global__ __launch_bounds__(256) void bert_hist_kernel(BertParams P, int cur) {
     __shared__ uint32_t pre[BERT_MAXSEG];
     __shared__ uint32_t tsum[256];
     const uint32_t nin = seg_prefix(P.seg + (size_t)cur * P.nsegs, P.nsegs, P.segcap, pre, tsum);
-    int32_t* H = P.H + (size_t)cur * P.nbox * P.n;
+    unsigned long long* H = P.H + (size_t)cur * P.nbox * P.n;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nin;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const int sg = seg_search(pre, P.nsegs, (uint32_t)i);
         const BPart p = P.pin[(uint64_t)sg * P.segcap + ((uint32_t)i - (sg > 0 ? pre[sg - 1] : 0u))];
         int box = (int)(p.boxsign >> 1);
         int sign = (p.boxsign & 1u) ? -1 : 1;
-        atomicAdd(H + (size_t)box * P.n + p.mode, sign);
+        atomicAdd(H + (size_t)box * P.n + p.mode, h_unit(sign));
     }
 }
 
@@ -886,7 +969,8 @@ struct Bert {
     DevBuf<double2> d_vel, d_ivel;
     DevBuf<BPart> d_parts;
     DevBuf<uint32_t> d_seg, d_tickets, d_flags;
-    DevBuf<int32_t> d_H, d_Eb, d_side_spl, d_rta_slot;
+    DevBuf<int32_t> d_Eb, d_side_spl, d_rta_slot;
+    DevBuf<unsigned long long> d_H;
     DevBuf<double> d_props, d_spl, d_Teq, d_pbox;
     DevBuf<int64_t> d_npart, d_pending;
     DevBuf<double> d_wcdf;
@@ -897,11 +981,17 @@ struct Bert {
     DevBuf<uint2> d_rta_alias, d_src_tab, d_init_tab;
     DevBuf<BertSource> d_src;
     DevBuf<BertInit> d_init;
+    // propagator slots: one table per regular box temperature
+    DevBuf<uint2> d_palias;
+    DevBuf<int32_t> d_pslot;
+    const uint2* palias = nullptr;
+    int nslots = 1;
     int nsrc = 0, ninit = 0;
     bool geo_lds = false;
     size_t lds_bytes = 0;
     int step_blocks = 0, emit_blocks = 0, nsegs = 0;
     std::vector<int32_t> hH;
+    std::vector<unsigned long long> hH64;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> kev;  // begin/end event per step-kernel launch
     ~Bert() {
@@ -921,6 +1011,68 @@ static double calc_Eeff_host(const Modes& m, double Th, double Tc, double Tr, do
     return (double)(vol * (dc + dh) / m.nq / np);
 }
 
+// The pair of tabulated temperatures for a box temperature Tx that is not
+// tabulated. rule 0 is Scattering's choice (beyondRTA_MC2d.cpp:1139-1164):
+// T1 = the first key not below Tx, T2 = the key after it, allowed when T2
+// exists and Tx >= the lowest key (the line through T1, T2 is evaluated at
+// Tx < T1). rule 1 brackets Tx between adjacent keys.
+static void prop_pair(const std::vector<double>& keys, double Tx, int rule, int& t1, int& t2) {
+    const int i = (int)(std::lower_bound(keys.begin(), keys.end(), Tx) - keys.begin());
+    const int nk = (int)keys.size();
+    if (rule == 0) {
+        if (i < nk && i + 1 < nk && Tx >= keys.front()) {
+            t1 = i;
+            t2 = i + 1;
+            return;
+        }
+    } else if (i > 0 && i < nk) {
+        t1 = i - 1;
+        t2 = i;
+        return;
+    }
+    throw Error(BTE_EINVAL, "Extrapolation is not allowed: box temperature " + std::to_string(Tx) +
+                                " must lie between the min and max temperatures used in Propagator calculation");
+}
+
+// Alias tables (sign bits in the alias word) of lirp(H(T1), H(T2), Tx) for
+// every column: the interpolated signed cumulative, sampled as
+// propagator_H::get samples it (pick_pmf), sign = (sc > 0 ? + : -).
+static void interp_table(const Prop& p, int t1, int t2, double Tx, std::vector<uint2>& tab) {
+    const int n = p.ncol;
+    tab.resize((size_t)n * n);
+    const double T1 = p.T[t1], T2 = p.T[t2];
+    const double f = (Tx - T1) / (T2 - T1);
+    auto work = [&](int j0, int j1) {
+        std::vector<double> sc1(n), sc2(n), sc(n), pmf(n);
+        std::vector<uint32_t> th(n), al(n);
+        auto signed_cumm = [n](const double* col, double* out) {
+            double tot = 0;
+            for (int k = 0; k < n; k++) tot += std::fabs(col[k]);
+            double acc = 0;
+            for (int k = 0; k < n; k++) {
+                acc += std::fabs(col[k]);
+                out[k] = col[k] < 0. ? -acc / tot : acc / tot;
+            }
+        };
+        for (int j = j0; j < j1; j++) {
+            signed_cumm(p.P[t1].data() + (size_t)j * n, sc1.data());
+            signed_cumm(p.P[t2].data() + (size_t)j * n, sc2.data());
+            for (int k = 0; k < n; k++) sc[k] = sc1[k] + f * (sc2[k] - sc1[k]);
+            pick_pmf(sc.data(), n, pmf.data());
+            build_alias(pmf.data(), n, th.data(), al.data());
+            for (int k = 0; k < n; k++) {
+                const uint32_t a = al[k];
+                const uint32_t word = a | (sc[k] > 0. ? 0u : 0x80000000u) | (sc[a] > 0. ? 0u : 0x40000000u);
+                tab[(size_t)j * n + k] = make_uint2(th[k], word);
+            }
+        }
+    };
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> ths;
+    for (int w = 0; w < nt; w++) ths.emplace_back(work, (int)((int64_t)n * w / nt), (int)((int64_t)n * (w + 1) / nt));
+    for (auto& t : ths) t.join();
+}
+
 static BertParams make_params(Bert* b) {
     BertParams P{};
     P.vel = b->d_vel.p;
@@ -931,7 +1083,9 @@ static BertParams make_params(Bert* b) {
     P.cts = b->g->d_contacts.p;
     P.cseg = b->g->d_cseg.p;
     P.ptrans = b->g->d_ptrans.p;
-    P.palias = b->p ? b->p->d_alias.p : nullptr;
+    P.palias = b->palias;
+    P.box_pslot = b->d_pslot.p;
+    P.nslots = b->nslots;
     P.rta_alias = b->d_rta_alias.p;
     P.box_rta_slot = b->d_rta_slot.p;
     P.box_Teq = b->d_Teq.p;
@@ -995,44 +1149,76 @@ using namespace bte;
 
 extern "C" {
 
-int bte_prop_create(bte_ctx* ctx, int ncol, const double* P, int ngamma, double T, bte_prop** out) {
+int bte_prop_create_multi(bte_ctx* ctx, int ncol, int ntemp, const double* Ps, const double* Ts, int ngamma,
+                          bte_prop** out) {
     BTE_GUARD_BEGIN
     auto* c = reinterpret_cast<Ctx*>(ctx);
-    if (!c || !P || !out || ncol <= 0 || ngamma < 0 || ncol >= (1 << 30))
-        throw Error(BTE_EINVAL, "bad arguments to bte_prop_create");
+    if (!c || !Ps || !Ts || !out || ncol <= 0 || ntemp <= 0 || ngamma < 0 || ncol >= (1 << 30))
+        throw Error(BTE_EINVAL, "bad arguments to bte_prop_create_multi");
     HIPCHECK(hipSetDevice(c->device));
     auto* p = new Prop();
     std::unique_ptr<Prop> guard(p);
     p->ctx = c;
     p->ncol = ncol;
     p->ngamma = ngamma;
-    p->T = T;
-    p->P.assign(P, P + (size_t)ncol * ncol);
+    // std::map<double, hist_type> semantics (beyondRTA_MC2d.cpp:1352-1378):
+    // ascending temperatures, a repeated temperature keeps the last matrix
+    const size_t nn = (size_t)ncol * ncol;
+    std::map<double, int> last;
+    for (int t = 0; t < ntemp; t++) {
+        if (!std::isfinite(Ts[t])) throw Error(BTE_EINVAL, "propagator temperature is not finite");
+        last[Ts[t]] = t;
+    }
+    if (last.size() > 1 && last.begin()->first <= 0.)
+        throw Error(BTE_EINVAL, "propagator temperatures must be positive");
+    for (auto& [T, t] : last) {
+        p->T.push_back(T);
+        p->P.emplace_back(Ps + (size_t)t * nn, Ps + (size_t)(t + 1) * nn);
+    }
     // propagator_H per column (collision_operator.cpp:854-870) as an alias
     // table over |P_kj| with the entry signs in the alias word
-    std::vector<uint2> tab((size_t)ncol * ncol);
+    std::vector<uint2> tab(p->T.size() * nn);
     std::vector<double> pm(ncol);
     std::vector<uint32_t> th(ncol), al(ncol);
-    p->abssum.resize(ncol);
-    for (int j = 0; j < ncol; j++) {
-        const double* col = P + (size_t)j * ncol;
-        double s = 0;
-        for (int k = 0; k < ncol; k++) {
-            pm[k] = std::fabs(col[k]);
-            s += pm[k];
-        }
-        p->abssum[j] = s;
-        if (!build_alias(pm.data(), ncol, th.data(), al.data()))
-            throw Error(BTE_EINVAL, "propagator column " + std::to_string(j) + " is zero");
-        for (int k = 0; k < ncol; k++) {
-            uint32_t a = al[k];
-            uint32_t word = a | (col[k] < 0. ? 0x80000000u : 0u) | (col[a] < 0. ? 0x40000000u : 0u);
-            tab[(size_t)j * ncol + k] = make_uint2(th[k], word);
+    for (size_t t = 0; t < p->T.size(); t++) {
+        for (int j = 0; j < ncol; j++) {
+            const double* col = p->P[t].data() + (size_t)j * ncol;
+            for (int k = 0; k < ncol; k++) pm[k] = std::fabs(col[k]);
+            if (!build_alias(pm.data(), ncol, th.data(), al.data()))
+                throw Error(BTE_EINVAL, "propagator column " + std::to_string(j) + " is zero");
+            for (int k = 0; k < ncol; k++) {
+                uint32_t a = al[k];
+                uint32_t word = a | (col[k] < 0. ? 0x80000000u : 0u) | (col[a] < 0. ? 0x40000000u : 0u);
+                tab[t * nn + (size_t)j * ncol + k] = make_uint2(th[k], word);
+            }
         }
     }
     p->d_alias.upload(tab, c->stream);
     HIPCHECK(hipStreamSynchronize(c->stream));
     *out = reinterpret_cast<bte_prop*>(guard.release());
+    BTE_GUARD_END
+}
+
+int bte_prop_create(bte_ctx* ctx, int ncol, const double* P, int ngamma, double T, bte_prop** out) {
+    return bte_prop_create_multi(ctx, ncol, 1, P, &T, ngamma, out);
+}
+
+int bte_prop_pick_pmf(int n, const double* signed_cumm, double* pmf) {
+    BTE_GUARD_BEGIN
+    if (n <= 0 || !signed_cumm || !pmf) throw Error(BTE_EINVAL, "bad arguments to bte_prop_pick_pmf");
+    pick_pmf(signed_cumm, n, pmf);
+    BTE_GUARD_END
+}
+
+int bte_geom_gradient_T(const bte_geom* gg, double* grad) {
+    BTE_GUARD_BEGIN
+    auto* g = reinterpret_cast<const Geom*>(gg);
+    if (!g || !grad) throw Error(BTE_EINVAL, "null argument");
+    auto gr = gradient_T(*g);
+    for (size_t i = 0; i < gr.size(); i++) {
+        grad[2 * i] = gr[i].x;
+        grad[2 * i + 1] = gr[i].y;
+    }
     BTE_GUARD_END
 }
 
@@ -1057,17 +1243,6 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     for (size_t i = 0; i < g->boxes.size(); i++)
         if (g->dboxes[i].kind == 2 && g->dboxes[i].tgt < 0)
             throw Error(BTE_EGEOM, "periodic box " + std::to_string(i) + " has no translation");
-    // gradient sources (calculate_gradientT, geometry_2d.cpp:524-576) vanish
-    // only when every regular box has the same Teq
-    {
-        double t0 = -1;
-        for (auto& bx : g->boxes) {
-            if (bx.kind != 0) continue;
-            if (t0 < 0) t0 = bx.Teq;
-            else if (bx.Teq != t0)
-                throw Error(BTE_EINVAL, "regular boxes with different Teq need gradient sources, which are not supported");
-        }
-    }
     HIPCHECK(hipSetDevice(c->device));
     auto* b = new Bert();
     std::unique_ptr<Bert> guard(b);
@@ -1092,6 +1267,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         double maxT = 0, minT = 1.416784e+32, meanT = 0;
         int nb_ = 0;
         for (auto& bx : g->boxes) {
+            if (bx.kind == 2) continue;
             meanT += bx.Teq;
             nb_++;
             minT = std::min(minT, bx.Teq);
@@ -1162,8 +1338,53 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
             const int src = (int)(stab.size() / n);
             for (int part = 0; part < nparts; part++)
                 srcs.push_back(BertSource{hc.p1x, hc.p1y, hc.p2x - hc.p1x, hc.p2y - hc.p1y, expected, hc.box,
-                                          (sb.Teq - cb.Teq) >= 0 ? 1 : -1, (int32_t)stab.size(), src, part, nparts});
+                                          (sb.Teq - cb.Teq) >= 0 ? 1 : -1, (int32_t)stab.size(), src, part, nparts,
+                                          0, 0});
             for (int k = 0; k < n; k++) stab.push_back(make_uint2(th[k], al[k]));
+            src_total += expected;
+        }
+    }
+    // gradient sources (source_term_gradient, beyondRTA_MC2d.cpp:597-676):
+    // per regular box with a gradient that is not almost zero, the
+    // Nabla_T_distribution (sampling.cpp:118-164) at the box Teq, pmf
+    // |(-g.v) kernel(omega, Teq)| with the sign of -g.v in the alias word,
+    // Ntogenerate(dt, vol, Eeff) (sampling.cpp:166-183) particles per step
+    {
+        std::vector<double2> grad;
+        if (cfg->fixed_gradient) {
+            grad.assign(nbox, make_double2(0., 0.));
+            for (int i = 0; i < nbox; i++)
+                if (g->boxes[i].kind == 0) grad[i] = make_double2(cfg->gradient[0], cfg->gradient[1]);
+        } else {
+            grad = gradient_T(*g);
+        }
+        std::vector<int8_t> neg(n);
+        for (int i = 0; i < nbox; i++) {
+            const RectBox& bx = g->boxes[i];
+            if (bx.kind != 0 || almost_equal(std::hypot(grad[i].x, grad[i].y), 0.)) continue;
+            double acc = 0;
+            for (int k = 0; k < n; k++) {
+                pmf[k] = 0.;
+                neg[k] = 0;
+                if (m->omega[k] == 0.) continue;
+                const double vgt = -(grad[i].x * m->vx[k] + grad[i].y * m->vy[k]);
+                pmf[k] = std::fabs(vgt * h_be_kernel(m->omega[k], bx.Teq));
+                neg[k] = vgt >= 0. ? 0 : 1;
+                acc += pmf[k];
+            }
+            if (!(acc > 0.)) continue;  // no mode carries this gradient
+            const double cumulsum = kB * acc;
+            build_alias(pmf.data(), n, th.data(), al.data());
+            const double expected = cfg->dt * (bx.area() * cfg->thickness / vuc) * cumulsum / (b->Eeff * m->nq);
+            const int nparts = (int)std::min(256.0, std::max(1.0, std::ceil(expected / 1024.0)));
+            const int src = (int)(stab.size() / n);
+            for (int part = 0; part < nparts; part++)
+                srcs.push_back(BertSource{bx.xmin, bx.ymin, bx.xmax - bx.xmin, bx.ymax - bx.ymin, expected, i, 1,
+                                          (int32_t)stab.size(), src, part, nparts, 1, 0});
+            for (int k = 0; k < n; k++) {
+                const uint32_t a = al[k];
+                stab.push_back(make_uint2(th[k], a | (neg[k] ? 0x80000000u : 0u) | (neg[a] ? 0x40000000u : 0u)));
+            }
             src_total += expected;
         }
     }
@@ -1232,6 +1453,57 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         b->d_rta_alias.alloc(1);
     }
     b->d_rta_slot.upload(slot, s);
+    // propagator per regular box temperature (Scattering,
+    // beyondRTA_MC2d.cpp:1131-1166): the tabulated one, or the lirp
+    // (collision_operator.cpp:918-929) of the pair prop_pair picks; every
+    // regular box is checked here, before the first step
+    std::vector<int32_t> pslot(nbox, 0);
+    b->nslots = 1;
+    if (b->p) {
+        const Prop* pp = b->p;
+        const size_t nn = (size_t)pp->ncol * pp->ncol;
+        if (pp->T.size() == 1 && pp->T[0] <= 0.) {
+            b->palias = pp->d_alias.p;
+        } else {
+            struct SlotSrc { int exact, t1, t2; double Tx; };
+            std::map<double, int> slot_of;
+            std::vector<SlotSrc> ss;
+            for (int i = 0; i < nbox; i++) {
+                const double T = g->boxes[i].Teq;
+                if (g->boxes[i].kind != 0) continue;
+                if (!slot_of.count(T)) {
+                    SlotSrc q{-1, 0, 0, T};
+                    for (size_t t = 0; t < pp->T.size(); t++)
+                        if (pp->T[t] == T) q.exact = (int)t;
+                    if (q.exact < 0) prop_pair(pp->T, T, cfg->prop_interp, q.t1, q.t2);
+                    slot_of[T] = (int)ss.size();
+                    ss.push_back(q);
+                }
+                pslot[i] = slot_of[T];
+            }
+            if (ss.empty()) ss.push_back(SlotSrc{0, 0, 0, pp->T[0]});
+            b->nslots = (int)ss.size();
+            if (b->nslots == 1 && ss[0].exact >= 0) {
+                b->palias = pp->d_alias.p + (size_t)ss[0].exact * nn;
+            } else {
+                b->d_palias.alloc(ss.size() * nn);
+                std::vector<uint2> tab;
+                for (size_t q = 0; q < ss.size(); q++) {
+                    if (ss[q].exact >= 0) {
+                        HIPCHECK(hipMemcpyAsync(b->d_palias.p + q * nn, pp->d_alias.p + (size_t)ss[q].exact * nn,
+                                                nn * sizeof(uint2), hipMemcpyDeviceToDevice, s));
+                    } else {
+                        interp_table(*pp, ss[q].t1, ss[q].t2, ss[q].Tx, tab);
+                        HIPCHECK(hipMemcpyAsync(b->d_palias.p + q * nn, tab.data(), nn * sizeof(uint2),
+                                                hipMemcpyHostToDevice, s));
+                        HIPCHECK(hipStreamSynchronize(s));  // tab is reused
+                    }
+                }
+                b->palias = b->d_palias.p;
+            }
+        }
+    }
+    b->d_pslot.upload(pslot, s);
     // adiabatic wall splines J(T), T = 10..1000 K (beyondRTA_MC2d.cpp:294-350)
     std::vector<double> Ts(100);
     for (int i = 0; i < 100; i++) Ts[i] = 10.0 + 10.0 * i;
@@ -1268,6 +1540,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         b->d_src_tab.upload(stab, s);
     }
     b->hH.resize((size_t)nbox * n);
+    b->hH64.resize((size_t)nbox * n);
     b->kev.resize((size_t)2 * b->max_steps);
     for (auto& e : b->kev) HIPCHECK(hipEventCreate(&e));
     HIPCHECK(hipEventCreate(&b->ev0));
@@ -1288,7 +1561,7 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     const int n = b->m->n, nbox = (int)b->g->boxes.size();
     BertParams P = make_params(b);
     BPart* buf[2] = {b->d_parts.p, b->d_parts.p + b->cap};
-    HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)2 * nbox * n * sizeof(int32_t), s));
+    HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)2 * nbox * n * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(b->d_tickets.p, 0, (size_t)nbox * n * sizeof(uint32_t), s));
     HIPCHECK(hipMemsetAsync(b->d_Eb.p, 0, (size_t)2 * nbox * 4 * sizeof(int32_t), s));
     HIPCHECK(hipMemsetAsync(b->d_flags.p, 0, 4 * sizeof(uint32_t), s));
@@ -1315,7 +1588,7 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, 0, 0, 0, 0, 0, 0, 1);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipMemcpyAsync(b->d_seg.p, c0.data(), c0.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)nbox * n * sizeof(int32_t), s));
+    HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)nbox * n * sizeof(unsigned long long), s));
     HIPCHECK(hipMemcpyAsync(b->d_npart.p, &b->n0, sizeof(int64_t), hipMemcpyHostToDevice, s));
     HIPCHECK(hipStreamSynchronize(s));
     const int step_blocks = b->step_blocks;
@@ -1361,9 +1634,11 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         if (cb) {
             // the histogram of this step is in H[cur] until the next step's
             // props kernel clears it
-            HIPCHECK(hipMemcpyAsync(b->hH.data(), b->d_H.p + (size_t)cur * nbox * n,
-                                    (size_t)nbox * n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipMemcpyAsync(b->hH64.data(), b->d_H.p + (size_t)cur * nbox * n,
+                                    (size_t)nbox * n * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
             HIPCHECK(hipStreamSynchronize(s));
+            for (size_t k = 0; k < b->hH.size(); k++)
+                b->hH[k] = (int32_t)(uint32_t)b->hH64[k] - (int32_t)(uint32_t)(b->hH64[k] >> 32);
             if (cb(user, istep, (istep + 1) * b->cfg.dt, b->Eeff, nbox, n, b->hH.data())) {
                 istep++;
                 break;

@@ -69,6 +69,92 @@ bool build_alias(const double* pmf, int n, uint32_t* thresh, uint32_t* alias) {
     return true;
 }
 
+// R maps to the first index (in std::lower_bound's search) with |sc| >= R,
+// a non-decreasing step function of R that only changes at the values |sc|.
+// A monotone |sc| gives the increments directly; otherwise every interval
+// between consecutive distinct values is mapped through std::lower_bound
+// itself, so a non-monotone cumulative (interpolated propagators) is sampled
+// exactly as the search would.
+void pick_pmf(const double* sc, int n, double* pmf) {
+    for (int k = 0; k < n; k++) pmf[k] = 0.;
+    if (n <= 0) return;
+    bool mono = true;
+    for (int k = 1; k < n && mono; k++) mono = std::fabs(sc[k]) >= std::fabs(sc[k - 1]);
+    double lo = 0.;
+    if (mono) {
+        for (int k = 0; k < n; k++) {
+            const double hi = std::min(std::fabs(sc[k]), 1.0);
+            if (hi > lo) {
+                pmf[k] = hi - lo;
+                lo = hi;
+            }
+        }
+    } else {
+        std::vector<double> v(n);
+        for (int k = 0; k < n; k++) v[k] = std::min(std::fabs(sc[k]), 1.0);
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        auto abs_less = [](double a, double b) { return std::fabs(a) < std::fabs(b); };
+        for (double hi : v) {
+            if (!(hi > lo)) continue;
+            // every R in (lo, hi] gives the same search result as R = hi
+            size_t r = (size_t)(std::lower_bound(sc, sc + n, hi, abs_less) - sc);
+            if (r >= (size_t)n) r = (size_t)n - 1;
+            pmf[r] += hi - lo;
+            lo = hi;
+        }
+    }
+    if (lo < 1.0) pmf[n - 1] += 1.0 - lo;  // R above every |sc|
+}
+
+// Minimum-norm least squares h_i . g = Teq_i - Teq_0 over the regular
+// contact boxes i of a regular box (h_i: centre differences), through the
+// singular value decomposition of the 2-column system; singular values below
+// 2 eps sigma_max count as zero.
+std::vector<double2> gradient_T(const Geom& g) {
+    const int nbox = (int)g.boxes.size();
+    std::vector<double2> out(nbox, make_double2(0., 0.));
+    for (int i = 0; i < nbox; i++) {
+        const RectBox& b0 = g.boxes[i];
+        if (b0.kind != 0) continue;
+        std::vector<double> hx, hy, df;
+        for (const HostContact& hc : g.contacts[i]) {
+            const RectBox& bc = g.boxes[hc.box];
+            if (bc.kind != 0) continue;
+            df.push_back(bc.Teq - b0.Teq);
+            hx.push_back(bc.cx() - b0.cx());
+            hy.push_back(bc.cy() - b0.cy());
+        }
+        const size_t m = df.size();
+        if (m == 0) continue;
+        double a = 0, bb = 0, c = 0;
+        for (size_t k = 0; k < m; k++) {
+            a += hx[k] * hx[k];
+            bb += hy[k] * hy[k];
+            c += hx[k] * hy[k];
+        }
+        // right singular vectors (cs, sn) and (-sn, cs)
+        const double th = 0.5 * std::atan2(2. * c, a - bb);
+        const double cs = std::cos(th), sn = std::sin(th);
+        double s1 = 0, s2 = 0, p1 = 0, p2 = 0;
+        for (size_t k = 0; k < m; k++) {
+            const double u1 = cs * hx[k] + sn * hy[k];
+            const double u2 = -sn * hx[k] + cs * hy[k];
+            s1 += u1 * u1;
+            s2 += u2 * u2;
+            p1 += u1 * df[k];
+            p2 += u2 * df[k];
+        }
+        const double smax = std::sqrt(std::max(s1, s2));
+        const double tol = 2. * 2.220446049250313e-16 * (double)std::min<size_t>(m, 2) * smax;
+        double gx = 0, gy = 0;
+        if (std::sqrt(s1) > tol) { gx += cs * p1 / s1; gy += sn * p1 / s1; }
+        if (std::sqrt(s2) > tol) { gx += -sn * p2 / s2; gy += cs * p2 / s2; }
+        out[i] = make_double2(gx, gy);
+    }
+    return out;
+}
+
 // Isothermal_wall_distribution pmf (sampling.cpp:189-269); normal points into
 // the box the particles are emitted into.
 double wall_pmf(const Modes& m, double Twall, double Teq, double nx, double ny, double* pmf) {

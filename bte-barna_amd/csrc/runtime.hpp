@@ -120,13 +120,14 @@ struct Geom {
     DevBuf<double2> d_ptrans;
 };
 
+// Propagators of one material at ascending temperatures; a single entry with
+// T <= 0 is used at every box temperature.
 struct Prop {
     Ctx* ctx = nullptr;
     int ncol = 0, ngamma = 0;
-    double T = 0;
-    std::vector<double> P;  // host copy (column-major)
-    DevBuf<uint2> d_alias;  // ncol x ncol entries, column-major
-    std::vector<double> abssum;
+    std::vector<double> T;
+    std::vector<std::vector<double>> P;  // host copies (column-major), per T
+    DevBuf<uint2> d_alias;  // per T: ncol x ncol entries, column-major
 };
 
 // ---- host helpers (host_util.cpp) ----
@@ -135,6 +136,12 @@ double h_be_kernel(double omega, double T);
 double h_calc_cv(const Modes& m, double T);
 // Vose alias construction; pmf >= 0, returns false if the pmf sums to 0
 bool build_alias(const double* pmf, int n, uint32_t* thresh, uint32_t* alias);
+// The distribution of propagator_H::get(R) (collision_operator.cpp:892-904)
+// for R uniform in [0, 1) over a signed cumulative sc[n]: pick k with
+// probability pmf[k]; a lower_bound past the end picks n-1.
+void pick_pmf(const double* sc, int n, double* pmf);
+// calculate_gradientT (geometry_2d.cpp:524-576) per box [K/nm]
+std::vector<double2> gradient_T(const struct Geom& g);
 // isothermal wall pmf (sampling.cpp:189-269) into pmf; returns cumulsum
 double wall_pmf(const Modes& m, double Twall, double Teq, double nx, double ny, double* pmf);
 // natural cubic spline coefficients (src/interpolation.cpp:29-100):

@@ -83,6 +83,11 @@ int bte_geom_set_periodic(bte_geom* g, int box, int target, double tx, double ty
 /* Contacts (geometry_2d::calculate_contacts, src/geometry_2d.cpp:295-381):
  * for each box up to maxc neighbours with (npx, npy, p1x, p1y, p2x, p2y). */
 int bte_geom_contacts(const bte_geom* g, int maxc, int32_t* ncont, int32_t* nbr, double* geo);
+/* Temperature gradients (calculate_gradientT, src/geometry_2d.cpp:524-576):
+ * per box (gx, gy) [K/nm] into grad[2*nbox], the minimum-norm least-squares
+ * fit of the Teq differences to the regular contact boxes; 0 for reservoir
+ * and periodic boxes. Host only. */
+int bte_geom_gradient_T(const bte_geom* g, double* grad);
 
 /* ---- RTAMC2D transient solver (replaces the particle loop of
  * src/RTA_MC2D.cpp:2188-2435: Init_generator, evolparticle, process_segment,
@@ -180,10 +185,24 @@ int bte_rta_batch_destroy(bte_rta_batch* b);
 
 /* ---- propagator: dense P (ncol x ncol, column-major) for modes
  * [ngamma, ngamma+ncol) (replaces load_P + propagator_H per column,
- * src/collision_operator.cpp:854-1013, beyondRTA_MC2d.cpp:1356-1378) ---- */
+ * src/collision_operator.cpp:854-1013, beyondRTA_MC2d.cpp:1356-1378).
+ * A box scatters with the propagator tabulated at its Teq; T <= 0 makes a
+ * single propagator apply at every box temperature. ---- */
 int bte_prop_create(bte_ctx* ctx, int ncol, const double* P, int ngamma, double T,
                     bte_prop** out);
+/* Propagators of one material at ntemp temperatures Ts (the <propagator T
+ * file/> children of <material>, beyondRTA_MC2d.cpp:1352-1378): Ps holds
+ * ntemp column-major ncol x ncol matrices. A box whose Teq is not tabulated
+ * scatters with the lirp of two of them (collision_operator.cpp:918-929),
+ * chosen by bte_bert_cfg.prop_interp. */
+int bte_prop_create_multi(bte_ctx* ctx, int ncol, int ntemp, const double* Ps, const double* Ts,
+                          int ngamma, bte_prop** out);
 int bte_prop_destroy(bte_prop* p);
+/* The distribution of propagator_H::get(R) (collision_operator.cpp:892-904)
+ * for R uniform in [0, 1) over a signed cumulative (propagator_H's
+ * signed_cumm, possibly interpolated and then not monotone in |.|): pmf[k]
+ * = probability of picking k; a search past the end picks n-1. Host only. */
+int bte_prop_pick_pmf(int n, const double* signed_cumm, double* pmf);
 
 /* ---- beRTAMC2D (replaces the step loop of src/beyondRTA_MC2d.cpp:
  * 2138-2425: advection, adiabatic/isothermal wall generators, Scattering or
@@ -199,6 +218,16 @@ typedef struct {
     int pcancel_each;  /* cancellation period (4 in the reference) */
     uint64_t seed;
     uint64_t capacity; /* particle buffer capacity; 0 = automatic */
+    /* gradient sources (source_term_gradient, beyondRTA_MC2d.cpp:597-676):
+     * 0 = per-box gradients from calculate_gradientT (the default),
+     * 1 = <gradient x y> [K/nm] in every regular box */
+    int fixed_gradient;
+    double gradient[2];
+    /* pair of tabulated propagators for an untabulated box Teq:
+     * 0 = Scattering's choice (beyondRTA_MC2d.cpp:1139-1164: the first
+     *     temperature not below Teq and the next one, extrapolated down),
+     * 1 = the adjacent temperatures bracketing Teq */
+    int prop_interp;
 } bte_bert_cfg;
 
 typedef struct {

@@ -8,17 +8,25 @@
 //   ScatteringRTA              beyondRTA_MC2d.cpp:983-1062
 //   Scattering                 beyondRTA_MC2d.cpp:1073-1210
 //   propagator_H               collision_operator.hpp:241-323, .cpp:854-916
+//   lirp, propagator selection collision_operator.cpp:918-929,
+//                              beyondRTA_MC2d.cpp:1131-1166
+//   calculate_gradientT        geometry_2d.cpp:524-576
+//   source_term_gradient       beyondRTA_MC2d.cpp:597-676, advected 2169-2180
 //   delete_particles           beyondRTA_MC2d.cpp:1214-1220
 //   histogram / cancellation   beyondRTA_MC2d.cpp:2277-2370
 //   calculate_prop (histogram) beyondRTA_MC2d.cpp:1535-1602
 //   step loop                  beyondRTA_MC2d.cpp:2138-2425
 // Deviations applied identically on the GPU path: per-particle PCG32 streams
 // instead of default-seeded per-thread pcg64 (beyondRTA_MC2d.cpp:1992-1993);
-// an explicit dead flag instead of the mode (0,0) sentinel;
-// gradient sources are zero (uniform Teq) and are rejected otherwise.
+// an explicit dead flag instead of the mode (0,0) sentinel; gradient
+// generators are visited in box order (the reference iterates an
+// unordered_map); a lower_bound past the last cumulative entry picks the last
+// mode (the reference reads one past the end there).
 #include "orc_geometry.hpp"
 #include <omp.h>
 #include <map>
+#include <array>
+#include <limits>
 
 namespace orc {
 
@@ -26,6 +34,7 @@ namespace orc {
 struct PropH {
     std::vector<double> signed_cumm;
     double abssum = 0;
+    PropH(std::vector<double> sc, double abs_) : signed_cumm(std::move(sc)), abssum(abs_) {}
     explicit PropH(const double* b, int n) : signed_cumm(n) {
         abssum = 0;
         for (int i = 0; i < n; i++) abssum += std::fabs(b[i]);
@@ -45,19 +54,52 @@ struct PropH {
     }
 };
 
+// lirp (collision_operator.cpp:918-929): linear interpolation of the signed
+// cumulatives and of the column sums
+static PropH lirp(const PropH& H1, const PropH& H2, double T1, double T2, double Tx) {
+    std::vector<double> sc(H1.signed_cumm.size());
+    for (size_t i = 0; i < sc.size(); i++)
+        sc[i] = H1.signed_cumm[i] + (Tx - T1) * (H2.signed_cumm[i] - H1.signed_cumm[i]) / (T2 - T1);
+    return PropH(std::move(sc), H1.abssum + (Tx - T1) * (H2.abssum - H1.abssum) / (T2 - T1));
+}
+
+// The pair of tabulated temperatures used for a box temperature Tx that is
+// not tabulated. rule 0 is Scattering's choice (beyondRTA_MC2d.cpp:1139-1164):
+// T1 = the first key not below Tx (map lower_bound), T2 = the key after it,
+// allowed when T2 exists and Tx >= the lowest key; the line through (T1, T2)
+// is then evaluated at Tx < T1. rule 1 brackets Tx: T1 < Tx < T2 adjacent.
+static std::pair<double, double> prop_pair(const std::vector<double>& keys, double Tx, int rule) {
+    auto it = std::lower_bound(keys.begin(), keys.end(), Tx);
+    if (rule == 0) {
+        if (it != keys.end() && it + 1 != keys.end() && Tx >= keys.front()) return {*it, *(it + 1)};
+    } else if (it != keys.begin() && it != keys.end()) {
+        return {*(it - 1), *it};
+    }
+    throw value_error("Extrapolation is not allowed: limit Tref between the min and max temperatures "
+                      "used in Propagator calculation");
+}
+
 struct BertSetup {
     Modes m;
     std::vector<Box> sys;
     double thickness = 1.0, dt = 1.0, maxtime = 10.0;
     int ngamma = 0;              // leading modes not in P (3 in the reference)
-    std::vector<double> P;       // (n-ngamma)^2 column-major
+    std::vector<double> P;       // (n-ngamma)^2 column-major, used at every Teq
+    // <propagator T file> children (beyondRTA_MC2d.cpp:1355-1376): when not
+    // empty, replaces P; ascending temperatures
+    std::vector<double> PT;
+    std::vector<std::vector<double>> Ps;
+    int prop_rule = 0;           // prop_pair rule
+    // <gradient x y> (beyondRTA_MC2d.cpp:1328-1333, 1441-1460): the same
+    // gradient [K/nm] in every regular box instead of calculate_gradientT
+    bool fixed_gradient = false;
+    double gx = 0., gy = 0.;
     bool rta = false;
     double Ed = -1.;             // fixed particle energy, or
     uint64_t Efactor = 0;        // number of particles for calc_Eeff
     double Tmax = 0, Tmin = 0;
     int pcancel_each = 4;
 
-    std::vector<PropH> hist;
     double Eeff = 0;
 };
 
@@ -76,6 +118,74 @@ struct BPart {
     int mode, box, sign;
     bool dead;
 };
+
+// calculate_gradientT (geometry_2d.cpp:524-576): for every regular box the
+// minimum-norm least-squares solution g of h_i . g = Teq_i - Teq_0 over its
+// regular contact boxes i (h_i: centre differences). The 2-column system is
+// solved through its singular value decomposition (one rotation of the
+// columns); singular values below 2 eps sigma_max count as zero.
+std::vector<std::array<double, 2>> gradient_T(const std::vector<Box>& sys) {
+    std::vector<std::array<double, 2>> g(sys.size(), {0., 0.});
+    for (auto& s : sys) {
+        if (s.reservoir || s.periodic) continue;
+        std::vector<double> hx, hy, df;
+        for (auto& [c, b] : s.contacts) {
+            if (sys[c].reservoir || sys[c].periodic) continue;
+            df.push_back(sys[c].Teq - s.Teq);
+            hx.push_back(sys[c].cx - s.cx);
+            hy.push_back(sys[c].cy - s.cy);
+        }
+        const size_t m = df.size();
+        if (m == 0) continue;
+        double a = 0, b = 0, c = 0;  // A^T A = [[a, c], [c, b]]
+        for (size_t i = 0; i < m; i++) {
+            a += hx[i] * hx[i];
+            b += hy[i] * hy[i];
+            c += hx[i] * hy[i];
+        }
+        // right singular vectors v1 = (cs, sn), v2 = (-sn, cs): the
+        // eigenvectors of A^T A, at angle atan2(2c, a - b) / 2
+        const double th = 0.5 * std::atan2(2. * c, a - b);
+        const double cs = std::cos(th), sn = std::sin(th);
+        // rotated columns u1 = A v1, u2 = A v2
+        double s1 = 0, s2 = 0, p1 = 0, p2 = 0;
+        for (size_t i = 0; i < m; i++) {
+            const double u1 = cs * hx[i] + sn * hy[i];
+            const double u2 = -sn * hx[i] + cs * hy[i];
+            s1 += u1 * u1;
+            s2 += u2 * u2;
+            p1 += u1 * df[i];
+            p2 += u2 * df[i];
+        }
+        const double smax = std::sqrt(std::max(s1, s2));
+        const double tol = 2. * std::numeric_limits<double>::epsilon() * (double)std::min<size_t>(m, 2) * smax;
+        double gxv = 0, gyv = 0;
+        if (std::sqrt(s1) > tol) { gxv += cs * p1 / s1; gyv += sn * p1 / s1; }
+        if (std::sqrt(s2) > tol) { gxv += -sn * p2 / s2; gyv += cs * p2 / s2; }
+        g[s.id] = {gxv, gyv};
+    }
+    return g;
+}
+
+// Nabla_T_distribution (sampling.cpp:118-164) for an in-plane gradient at T:
+// pmf |(-g.v) kernel(omega, T)|, sign of -g.v (>= 0 is +), cumulsum kB sum
+static GridDist nabla_T_signed(const Modes& m, double gx, double gy, double T) {
+    GridDist d;
+    d.nq = m.nq;
+    std::vector<double> pmf(m.n, 0.);
+    d.signs.assign(m.n, 1);
+    double acc = 0;
+    for (int i = 0; i < m.n; i++) {
+        if (m.omega[i] == 0.) continue;
+        const double vgt = -(gx * m.vx[i] + gy * m.vy[i]);
+        pmf[i] = std::fabs(vgt * bose_einstein_kernel(m.omega[i], T));
+        d.signs[i] = vgt >= 0. ? 1 : -1;
+        acc += pmf[i];
+    }
+    d.cumulsum = kB * acc;
+    d.fill_cumulative(pmf);
+    return d;
+}
 
 // evolparticle (beyondRTA_MC2d.cpp:708-970)
 static void bert_evol(BertSetup& S, BPart& p, double vx, double vy, double dt,
@@ -207,9 +317,60 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
     }
     const double Eeff = S.Eeff;
     if (Eeff_out) *Eeff_out = Eeff;
+    // propagator_H per column for every regular box temperature: the
+    // tabulated one, or lirp of the pair prop_pair picks. The reference makes
+    // this choice when a particle scatters; here every regular box is
+    // checked once before the first step.
+    std::map<double, std::vector<PropH>> hist_by_T;
+    std::vector<const std::vector<PropH>*> box_hist(nb, nullptr);
     if (!S.rta) {
-        S.hist.clear();
-        for (int c = 0; c < ncol; c++) S.hist.emplace_back(&S.P[(size_t)c * ncol], ncol);
+        auto columns = [&](const std::vector<double>& P) {
+            std::vector<PropH> h;
+            for (int c = 0; c < ncol; c++) h.emplace_back(&P[(size_t)c * ncol], ncol);
+            return h;
+        };
+        if (S.PT.empty()) {
+            hist_by_T[0.] = columns(S.P);
+            for (int b = 0; b < nb; b++) box_hist[b] = &hist_by_T[0.];
+        } else {
+            std::map<double, std::vector<PropH>> tab;
+            for (size_t t = 0; t < S.PT.size(); t++) tab[S.PT[t]] = columns(S.Ps[t]);
+            for (auto& s : sys) {
+                if (s.reservoir || s.periodic) continue;
+                if (!hist_by_T.count(s.Teq)) {
+                    if (tab.count(s.Teq)) {
+                        hist_by_T[s.Teq] = tab[s.Teq];
+                    } else {
+                        const auto pr = prop_pair(S.PT, s.Teq, S.prop_rule);
+                        const auto& H1 = tab[pr.first];
+                        const auto& H2 = tab[pr.second];
+                        std::vector<PropH> h;
+                        for (int c = 0; c < ncol; c++) h.push_back(lirp(H1[c], H2[c], pr.first, pr.second, s.Teq));
+                        hist_by_T[s.Teq] = std::move(h);
+                    }
+                }
+                box_hist[s.id] = &hist_by_T[s.Teq];
+            }
+        }
+    }
+    // gradient generators (source_term_gradient): boxes whose gradient is
+    // not almost zero, each with its Nabla_T distribution at the box Teq
+    std::vector<int> gbox;
+    std::vector<GridDist> ggen;
+    {
+        std::vector<std::array<double, 2>> grad;
+        if (S.fixed_gradient) {
+            grad.assign(nb, {0., 0.});
+            for (auto& s : sys)
+                if (!s.reservoir && !s.periodic) grad[s.id] = {S.gx, S.gy};
+        } else {
+            grad = gradient_T(sys);
+        }
+        for (int b = 0; b < nb; b++) {
+            if (almost_equal(std::hypot(grad[b][0], grad[b][1]), 0.)) continue;
+            gbox.push_back(b);
+            ggen.push_back(nabla_T_signed(m, grad[b][0], grad[b][1], sys[b].Teq));
+        }
     }
     std::vector<GridDist> rta_sampler;
     std::map<double, GridDist> rta_by_T;
@@ -302,7 +463,24 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
             }
             parts.insert(parts.end(), np_.begin(), np_.end());
         };
-        // 2. gradient sources: zero for uniform Teq (calculate_gradientT)
+        // 2. source_term_gradient: Ntogenerate(dt, vol, Eeff) particles per
+        // generator box at uniform points of the box, mode and sign from its
+        // Nabla_T distribution, advected U*dt
+        {
+            std::vector<BPart> np_;
+            for (size_t k = 0; k < gbox.size(); k++) {
+                const Box& s = sys[gbox[k]];
+                const GridDist& g = ggen[k];
+                size_t ng = n_to_generate(S.dt * volsf[s.id] * g.cumulsum / (Eeff * m.nq), hostrng);
+                for (size_t i = 0; i < ng; i++) {
+                    double x = s.xmin + hostrng.uniform() * (s.xmax - s.xmin);
+                    double y = s.ymin + hostrng.uniform() * (s.ymax - s.ymin);
+                    size_t pos = g.sample(hostrng);
+                    np_.push_back({x, y, (int)pos, s.id, g.signs[pos], false});
+                }
+            }
+            advect_new(np_, 0x9ad5ULL);
+        }
         // 3. Isothermal walls (reservoir injection)
         {
             std::vector<BPart> np_;
@@ -394,7 +572,7 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
                 while (pending > 0) {
                     pending--;
                     int col = parent.mode - S.ngamma;
-                    auto sol = S.hist[col].get(rng.uniform());
+                    auto sol = (*box_hist[parent.box])[col].get(rng.uniform());
                     BPart c = parent;
                     c.mode = sol.second + S.ngamma;
                     int ns = (parent.sign * sol.first) < 0 ? -1 : 1;
@@ -411,8 +589,18 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
         std::vector<std::vector<int>> H(nb, std::vector<int>(m.n, 0));
         for (auto& p : parts) H[p.box][p.mode] += p.sign;
         // 8. cancellation (sorted by (box, mode, sign); keep the first |h|
-        // particles of sign(h) per (box, mode))
+        // particles of sign(h) per (box, mode)). The reference sorts with
+        // tbb::parallel_sort, which is not stable: the order within one
+        // (box, mode, sign) group, and so which particles survive, is
+        // unspecified. The oracle takes a uniformly random subset (a shuffle
+        // before a stable sort).
         if (istep % S.pcancel_each == 0) {
+            pcg32 shuf(seed ^ 0xca9ce1ULL, stepcount);
+            for (size_t i = parts.size(); i > 1; i--) {
+                const uint64_t r = ((uint64_t)shuf.next() << 32) | shuf.next();
+                const size_t j = (size_t)(r % i);
+                std::swap(parts[i - 1], parts[j]);
+            }
             std::stable_sort(parts.begin(), parts.end(), [](const BPart& a, const BPart& b) {
                 if (a.box != b.box) return a.box < b.box;
                 if (a.mode != b.mode) return a.mode < b.mode;

@@ -16,6 +16,13 @@
 //   synthetic_B                 invariants 1^T B = 0, B C = 0, B offdiag >= 0
 //   get_inter_side / contacts   analytic slab intersections on rectangles
 //   propagator_H                exact enumeration of R on a grid
+//   lirp / propagator selection closed forms of the interpolation line and
+//                               the key pair Scattering picks
+//   calculate_gradientT         exact gradients of linear Teq fields,
+//                               minimum-norm rank-1 case, pseudo-inverse
+//                               (numpy.linalg.lstsq) on random layouts
+//   source_term_gradient        mean generated energy per step = the
+//                               Nabla_T energy dt vol cumulsum/nq (Eeff units)
 //   rta_run                     ballistic closed form (no scattering), flux
 //                               antisymmetry, energy bookkeeping
 //   bert_run                    conservation of the signed count (P >= 0,
@@ -172,6 +179,44 @@ void orc_propH_get(int n, const double* b, int nr, const double* R, int32_t* sig
     }
 }
 
+// lirp of the propagators at Ts (ntemp column-major ncol x ncol matrices in
+// Ps, ascending Ts) for temperature Tx with the given pair rule: the signed
+// cumulative of every column into sc (column-major). Returns 0, or 1 when Tx
+// cannot be interpolated.
+int orc_prop_interp(int ncol, int ntemp, const double* Ts, const double* Ps, double Tx, int rule, double* sc) {
+    try {
+        std::vector<double> keys(Ts, Ts + ntemp);
+        auto col = [&](int t, int c) { return PropH(Ps + ((size_t)t * ncol + c) * ncol, ncol); };
+        int exact = -1;
+        for (int t = 0; t < ntemp; t++)
+            if (Ts[t] == Tx) exact = t;
+        std::pair<double, double> pr{0., 0.};
+        int t1 = 0, t2 = 0;
+        if (exact < 0) {
+            pr = prop_pair(keys, Tx, rule);
+            t1 = (int)(std::find(keys.begin(), keys.end(), pr.first) - keys.begin());
+            t2 = (int)(std::find(keys.begin(), keys.end(), pr.second) - keys.begin());
+        }
+        for (int c = 0; c < ncol; c++) {
+            PropH h = exact >= 0 ? col(exact, c) : lirp(col(t1, c), col(t2, c), pr.first, pr.second, Tx);
+            std::copy(h.signed_cumm.begin(), h.signed_cumm.end(), sc + (size_t)c * ncol);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+// propagator_H::get on a given signed cumulative (e.g. from lirp)
+void orc_propH_get_sc(int n, const double* sc, int nr, const double* R, int32_t* sign, int32_t* idx) {
+    PropH h(std::vector<double>(sc, sc + n), 1.0);
+    for (int i = 0; i < nr; i++) {
+        auto s = h.get(R[i]);
+        sign[i] = s.first;
+        idx[i] = s.second;
+    }
+}
+
 // ---------------- geometry ----------------
 // periodic maps for the next run (target box and translation per box),
 // set by orc_set_periodic and consumed by mk_sys
@@ -203,6 +248,21 @@ static std::vector<Box> mk_sys(int nbox, const double* rect, const int32_t* kind
 void orc_set_periodic(int nbox, const int32_t* target, const double* trans) {
     g_ptarget.assign(target, target + nbox);
     g_ptrans.assign(trans, trans + 2 * (size_t)nbox);
+}
+// calculate_gradientT: (gx, gy) per box into grad[2*nbox]
+int orc_gradient_T(int nbox, const double* rect, const int32_t* kind, const double* Teq, double* grad) {
+    try {
+        auto sys = mk_sys(nbox, rect, kind, Teq, nullptr);
+        auto g = gradient_T(sys);
+        for (int i = 0; i < nbox; i++) {
+            grad[2 * i] = g[i][0];
+            grad[2 * i + 1] = g[i][1];
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
 }
 // get_inter_side on box ibox: returns 0 ok, 1 geometry_error; out[0]=time,
 // out[1..2]=point, bids (up to 4) with nbids
@@ -307,6 +367,29 @@ int orc_rta_steady_run(int n, const double* omega, const double* vx, const doubl
         return 1;
     }
 }
+// propagators at several temperatures, the pair rule and a fixed gradient
+// for the next orc_bert_run (Ps: ntemp column-major matrices, any T order)
+static thread_local std::vector<double> g_bT;
+static thread_local std::vector<std::vector<double>> g_bP;
+static thread_local int g_brule = 0, g_bfixed = 0, g_bcancel = 4;
+static thread_local double g_bgx = 0., g_bgy = 0.;
+void orc_bert_options(int ncol, int ntemp, const double* Ts, const double* Ps, int rule, int fixed_gradient,
+                      double gx, double gy, int pcancel_each) {
+    std::vector<int> ord(ntemp);
+    for (int t = 0; t < ntemp; t++) ord[t] = t;
+    std::sort(ord.begin(), ord.end(), [&](int a, int b) { return Ts[a] < Ts[b]; });
+    g_bT.clear();
+    g_bP.clear();
+    for (int t : ord) {
+        g_bT.push_back(Ts[t]);
+        g_bP.emplace_back(Ps + (size_t)t * ncol * ncol, Ps + (size_t)(t + 1) * ncol * ncol);
+    }
+    g_brule = rule;
+    g_bfixed = fixed_gradient;
+    g_bgx = gx;
+    g_bgy = gy;
+    g_bcancel = pcancel_each > 0 ? pcancel_each : 4;
+}
 int orc_bert_run(int n, const double* omega, const double* vx, const double* vy, const double* w,
                  double V, double lattz, int nq, int nbox, const double* rect, const int32_t* kind,
                  const double* Teq, const double* Tinit, double thickness, double dt,
@@ -328,6 +411,18 @@ int orc_bert_run(int n, const double* omega, const double* vx, const double* vy,
         S.Efactor = Efactor;
         S.Tmax = Tmax;
         S.Tmin = Tmin;
+        S.PT.swap(g_bT);
+        S.Ps.swap(g_bP);
+        S.prop_rule = g_brule;
+        S.fixed_gradient = g_bfixed != 0;
+        S.gx = g_bgx;
+        S.gy = g_bgy;
+        S.pcancel_each = g_bcancel;
+        g_bT.clear();
+        g_bP.clear();
+        g_brule = g_bfixed = 0;
+        g_bcancel = 4;
+        g_bgx = g_bgy = 0.;
         *nsteps_out = bert_run(S, seed, nthreads, max_steps, props, npart, H_last, Eeff_out, pending);
         return 0;
     } catch (const std::exception& e) {

@@ -115,6 +115,40 @@ def propH_get(b, R):
     return sg, idx
 
 
+def propH_get_sc(sc, R):
+    """propagator_H::get on a signed cumulative (e.g. an interpolated one)"""
+    sc, R = _f(sc), _f(R)
+    sg = np.zeros(len(R), np.int32)
+    idx = np.zeros(len(R), np.int32)
+    lib().orc_propH_get_sc(len(sc), _dp(sc), len(R), _dp(R), sg.ctypes.data_as(ip32), idx.ctypes.data_as(ip32))
+    return sg, idx
+
+
+def prop_interp(Ps, Ts, Tx, rule=0):
+    """signed cumulatives (ncol x ncol, column j = propagator_H of column j)
+    used at temperature Tx; raises ValueError when Tx cannot be interpolated"""
+    order = np.argsort(Ts)
+    Ts = _f(np.asarray(Ts, np.float64)[order])
+    Pa = np.concatenate([np.asfortranarray(Ps[i], dtype=np.float64).ravel(order="F") for i in order])
+    ncol = Ps[0].shape[0]
+    sc = np.zeros((ncol, ncol), order="F")
+    rc = lib().orc_prop_interp(ncol, len(Ts), _dp(Ts), _dp(Pa), C.c_double(Tx), rule, sc.ctypes.data_as(dp))
+    if rc != 0:
+        raise ValueError(lib().orc_last_error().decode())
+    return sc
+
+
+def gradient_T(rect, kind, Teq):
+    """calculate_gradientT: (nbox, 2) gradients [K/nm]"""
+    rect, Teq = _f(rect), _f(Teq)
+    kind = np.ascontiguousarray(kind, np.int32)
+    g = np.zeros((len(kind), 2))
+    rc = lib().orc_gradient_T(len(kind), _dp(rect), kind.ctypes.data_as(ip32), _dp(Teq), g.ctypes.data_as(dp))
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return g
+
+
 def inter_side(rect, ibox, x0, y0, vx, vy, dt):
     rect = _f(rect)
     out = np.zeros(3)
@@ -201,10 +235,24 @@ def rta_steady_run(mat, rect, kind, Teq, thickness, T0, gradient, nparticles, se
 
 
 def bert_run(mat, rect, kind, Teq, Tinit, thickness, dt, maxtime, P=None, ngamma=0, rta=False, Ed=-1.0,
-             Efactor=0, Tmax=0.0, Tmin=0.0, seed=1, max_steps=0, nthreads=0, periodic=None):
+             Efactor=0, Tmax=0.0, Tmin=0.0, seed=1, max_steps=0, nthreads=0, periodic=None, Ps=None, Ts=None,
+             prop_rule=0, gradient=None, pcancel_each=4):
+    """Ps/Ts: propagators at several temperatures (replace P); gradient:
+    (gx, gy) in every regular box instead of calculate_gradientT"""
     rect, Teq, Tinit = _f(rect), _f(Teq), _f(Tinit)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
+    if Ps is not None:
+        ncol = Ps[0].shape[0]
+        Pa = np.concatenate([np.asfortranarray(x, dtype=np.float64).ravel(order="F") for x in Ps])
+        P = Ps[0]
+    else:
+        ncol, Pa = 1, np.zeros(1)
+    gx, gy = gradient if gradient is not None else (0.0, 0.0)
+    lib().orc_bert_options(ncol, 0 if Ps is None else len(Ps), _dp(_f(np.asarray(Ts if Ts is not None else [0.0],
+                                                                              np.float64))),
+                           _dp(Pa), prop_rule, int(gradient is not None), C.c_double(gx), C.c_double(gy),
+                           int(pcancel_each))
     set_periodic(nbox, periodic)
     n = len(mat["omega"])
     nst = 0
