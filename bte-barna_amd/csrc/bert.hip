@@ -544,9 +544,12 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
                     alive = false;
                 } else {
                     const uint32_t c = h < 0 ? (uint32_t)(hv >> 32) : (uint32_t)hv;
-                    const uint32_t tk = atomicAdd(P.tickets + hb, 1u);
-                    if (tk >= c || permute_index(tk, c, mix64(ckey ^ (uint64_t)hb)) >= (uint32_t)(h < 0 ? -h : h))
-                        alive = false;
+                    const uint32_t keep = (uint32_t)(h < 0 ? -h : h);
+                    // every candidate survives when the bin holds one sign
+                    if (keep < c) {
+                        const uint32_t tk = atomicAdd(P.tickets + hb, 1u);
+                        if (tk >= c || permute_index(tk, c, mix64(ckey ^ (uint64_t)hb)) >= keep) alive = false;
+                    }
                 }
             }
             if (alive) {
