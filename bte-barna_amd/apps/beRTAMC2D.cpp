@@ -49,9 +49,6 @@ static int run(int argc, char** argv) {
     if (!geo || !tim || !mat || !eef) throw std::runtime_error("<beRTAMC2D> needs geometry, time, material and Eeff");
     if (root->children("material").size() > 1) throw std::runtime_error("This is currently not supported (several materials)");
     if (root->child("init_distribution")) throw std::runtime_error("<init_distribution> is not supported");
-    if (const xml::Node* g = root->child("gradient"))
-        if (g->num("x", 0.) != 0. || g->num("y", 0.) != 0.)
-            throw std::runtime_error("gradient sources are not supported");
     if (root->child("spectral")) std::cout << "# <spectral> is read by dist_reader, not by the solver" << std::endl;
     Boxes B = read_geometry(geo->str("file"));
     Material M = read_material(*mat);

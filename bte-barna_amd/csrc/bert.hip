@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <exception>
 #include <thread>
 
 namespace bte {
@@ -1037,10 +1038,15 @@ static void prop_pair(const std::vector<double>& keys, double Tx, int rule, int&
                                 " must lie between the min and max temperatures used in Propagator calculation");
 }
 
-// Alias tables (sign bits in the alias word) of lirp(H(T1), H(T2), Tx) for
-// every column: the interpolated signed cumulative, sampled as
-// propagator_H::get samples it (pick_pmf), sign = (sc > 0 ? + : -).
-static void interp_table(const Prop& p, int t1, int t2, double Tx, std::vector<uint2>& tab) {
+// Alias tables (sign bits in the alias word) for an untabulated temperature.
+// rule 0: lirp(H(T1), H(T2), Tx) for every column, the interpolated signed
+// cumulative sampled as propagator_H::get samples it (pick_pmf), sign =
+// (sc > 0 ? + : -). rule 1: the column of (1 - f) P(T1) + f P(T2), sampled by
+// its absolute values with its signs. Interpolating signed cumulatives is
+// unstable where an entry's sign differs between the two tables (rounding
+// noise on negligible entries does that): the cumulative then dips and the
+// next entry takes a large, possibly negative, probability.
+static void interp_table(const Prop& p, int t1, int t2, double Tx, int rule, std::vector<uint2>& tab) {
     const int n = p.ncol;
     tab.resize((size_t)n * n);
     const double T1 = p.T[t1], T2 = p.T[t2];
@@ -1058,11 +1064,22 @@ static void interp_table(const Prop& p, int t1, int t2, double Tx, std::vector<u
             }
         };
         for (int j = j0; j < j1; j++) {
-            signed_cumm(p.P[t1].data() + (size_t)j * n, sc1.data());
-            signed_cumm(p.P[t2].data() + (size_t)j * n, sc2.data());
-            for (int k = 0; k < n; k++) sc[k] = sc1[k] + f * (sc2[k] - sc1[k]);
-            pick_pmf(sc.data(), n, pmf.data());
-            build_alias(pmf.data(), n, th.data(), al.data());
+            const double* c1 = p.P[t1].data() + (size_t)j * n;
+            const double* c2 = p.P[t2].data() + (size_t)j * n;
+            if (rule == 1) {
+                // sc holds the interpolated column itself: only its signs are used
+                for (int k = 0; k < n; k++) {
+                    sc[k] = c1[k] + f * (c2[k] - c1[k]);
+                    pmf[k] = std::fabs(sc[k]);
+                }
+            } else {
+                signed_cumm(c1, sc1.data());
+                signed_cumm(c2, sc2.data());
+                for (int k = 0; k < n; k++) sc[k] = sc1[k] + f * (sc2[k] - sc1[k]);
+                pick_pmf(sc.data(), n, pmf.data());
+            }
+            if (!build_alias(pmf.data(), n, th.data(), al.data()))
+                throw Error(BTE_EINVAL, "interpolated propagator column " + std::to_string(j) + " is zero");
             for (int k = 0; k < n; k++) {
                 const uint32_t a = al[k];
                 const uint32_t word = a | (sc[k] > 0. ? 0u : 0x80000000u) | (sc[a] > 0. ? 0u : 0x40000000u);
@@ -1072,8 +1089,18 @@ static void interp_table(const Prop& p, int t1, int t2, double Tx, std::vector<u
     };
     const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> ths;
-    for (int w = 0; w < nt; w++) ths.emplace_back(work, (int)((int64_t)n * w / nt), (int)((int64_t)n * (w + 1) / nt));
+    std::vector<std::exception_ptr> errs(nt);
+    for (int w = 0; w < nt; w++)
+        ths.emplace_back([&, w] {
+            try {
+                work((int)((int64_t)n * w / nt), (int)((int64_t)n * (w + 1) / nt));
+            } catch (...) {
+                errs[w] = std::current_exception();
+            }
+        });
     for (auto& t : ths) t.join();
+    for (auto& e : errs)
+        if (e) std::rethrow_exception(e);
 }
 
 static BertParams make_params(Bert* b) {
@@ -1123,7 +1150,9 @@ static BertParams make_params(Bert* b) {
     P.nbox = (int)b->g->boxes.size();
     P.nseg = b->nseg;
     P.rta = b->cfg.rta;
-    P.max_children = 1 << 20;
+    // a particle's sign-flip branching stops with an error after this many
+    // children (a table with negative probability >= 1/2 never ends)
+    P.max_children = 1 << 14;
     P.pbox = b->d_pbox.p;
     P.lds_props = (int)b->g->boxes.size() <= BERT_LDS_BOXES ? 1 : 0;
     // dynamic LDS layout
@@ -1496,7 +1525,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
                         HIPCHECK(hipMemcpyAsync(b->d_palias.p + q * nn, pp->d_alias.p + (size_t)ss[q].exact * nn,
                                                 nn * sizeof(uint2), hipMemcpyDeviceToDevice, s));
                     } else {
-                        interp_table(*pp, ss[q].t1, ss[q].t2, ss[q].Tx, tab);
+                        interp_table(*pp, ss[q].t1, ss[q].t2, ss[q].Tx, cfg->prop_interp, tab);
                         HIPCHECK(hipMemcpyAsync(b->d_palias.p + q * nn, tab.data(), nn * sizeof(uint2),
                                                 hipMemcpyHostToDevice, s));
                         HIPCHECK(hipStreamSynchronize(s));  // tab is reused

@@ -67,7 +67,8 @@ static PropH lirp(const PropH& H1, const PropH& H2, double T1, double T2, double
 // not tabulated. rule 0 is Scattering's choice (beyondRTA_MC2d.cpp:1139-1164):
 // T1 = the first key not below Tx (map lower_bound), T2 = the key after it,
 // allowed when T2 exists and Tx >= the lowest key; the line through (T1, T2)
-// is then evaluated at Tx < T1. rule 1 brackets Tx: T1 < Tx < T2 adjacent.
+// is then evaluated at Tx < T1, on the signed cumulatives (lirp). rule 1
+// brackets Tx (T1 < Tx < T2 adjacent) and interpolates the matrices.
 static std::pair<double, double> prop_pair(const std::vector<double>& keys, double Tx, int rule) {
     auto it = std::lower_bound(keys.begin(), keys.end(), Tx);
     if (rule == 0) {
@@ -77,6 +78,13 @@ static std::pair<double, double> prop_pair(const std::vector<double>& keys, doub
     }
     throw value_error("Extrapolation is not allowed: limit Tref between the min and max temperatures "
                       "used in Propagator calculation");
+}
+
+// (1 - f) P1 + f P2 entry by entry
+static std::vector<double> interp_matrix(const std::vector<double>& P1, const std::vector<double>& P2, double f) {
+    std::vector<double> P(P1.size());
+    for (size_t i = 0; i < P.size(); i++) P[i] = P1[i] + f * (P2[i] - P1[i]);
+    return P;
 }
 
 struct BertSetup {
@@ -342,10 +350,16 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
                         hist_by_T[s.Teq] = tab[s.Teq];
                     } else {
                         const auto pr = prop_pair(S.PT, s.Teq, S.prop_rule);
-                        const auto& H1 = tab[pr.first];
-                        const auto& H2 = tab[pr.second];
                         std::vector<PropH> h;
-                        for (int c = 0; c < ncol; c++) h.push_back(lirp(H1[c], H2[c], pr.first, pr.second, s.Teq));
+                        if (S.prop_rule == 1) {
+                            const auto& P1 = S.Ps[std::find(S.PT.begin(), S.PT.end(), pr.first) - S.PT.begin()];
+                            const auto& P2 = S.Ps[std::find(S.PT.begin(), S.PT.end(), pr.second) - S.PT.begin()];
+                            h = columns(interp_matrix(P1, P2, (s.Teq - pr.first) / (pr.second - pr.first)));
+                        } else {
+                            const auto& H1 = tab[pr.first];
+                            const auto& H2 = tab[pr.second];
+                            for (int c = 0; c < ncol; c++) h.push_back(lirp(H1[c], H2[c], pr.first, pr.second, s.Teq));
+                        }
                         hist_by_T[s.Teq] = std::move(h);
                     }
                 }

@@ -179,10 +179,11 @@ void orc_propH_get(int n, const double* b, int nr, const double* R, int32_t* sig
     }
 }
 
-// lirp of the propagators at Ts (ntemp column-major ncol x ncol matrices in
-// Ps, ascending Ts) for temperature Tx with the given pair rule: the signed
-// cumulative of every column into sc (column-major). Returns 0, or 1 when Tx
-// cannot be interpolated.
+// The propagator used at temperature Tx from those at Ts (ntemp column-major
+// ncol x ncol matrices in Ps, ascending Ts): the tabulated one, lirp of the
+// signed cumulatives (rule 0) or the interpolated matrix (rule 1); the
+// signed cumulative of every column into sc (column-major). Returns 0, or 1
+// when Tx cannot be interpolated.
 int orc_prop_interp(int ncol, int ntemp, const double* Ts, const double* Ps, double Tx, int rule, double* sc) {
     try {
         std::vector<double> keys(Ts, Ts + ntemp);
@@ -197,8 +198,16 @@ int orc_prop_interp(int ncol, int ntemp, const double* Ts, const double* Ps, dou
             t1 = (int)(std::find(keys.begin(), keys.end(), pr.first) - keys.begin());
             t2 = (int)(std::find(keys.begin(), keys.end(), pr.second) - keys.begin());
         }
+        std::vector<double> Pm;
+        if (exact < 0 && rule == 1) {
+            std::vector<double> P1(Ps + (size_t)t1 * ncol * ncol, Ps + (size_t)(t1 + 1) * ncol * ncol);
+            std::vector<double> P2(Ps + (size_t)t2 * ncol * ncol, Ps + (size_t)(t2 + 1) * ncol * ncol);
+            Pm = interp_matrix(P1, P2, (Tx - pr.first) / (pr.second - pr.first));
+        }
         for (int c = 0; c < ncol; c++) {
-            PropH h = exact >= 0 ? col(exact, c) : lirp(col(t1, c), col(t2, c), pr.first, pr.second, Tx);
+            PropH h = exact >= 0 ? col(exact, c)
+                      : (rule == 1 ? PropH(Pm.data() + (size_t)c * ncol, ncol)
+                                   : lirp(col(t1, c), col(t2, c), pr.first, pr.second, Tx));
             std::copy(h.signed_cumm.begin(), h.signed_cumm.end(), sc + (size_t)c * ncol);
         }
         return 0;

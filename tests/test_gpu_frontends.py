@@ -110,6 +110,47 @@ def test_bertamc2d_outputs(pkg, ctx, tmp_path):
     assert np.allclose(o["props"][:, :, 0], rows[:, 3::3], rtol=2e-5, atol=0)
 
 
+def test_bertamc2d_temperature_profile(pkg, ctx, tmp_path):
+    """Box temperatures 300-305 K: two <propagator> children (300 K and
+    306 K, interpolated between), gradient sources from the temperatures or
+    from <gradient>; the executable and the library agree on the same seed."""
+    W = pkg.workloads
+    for T in (300, 306):
+        run([pkg.app_path("PropagatorBuilder"), "synthetic:96:2", str(T), "5", "1", "20"], tmp_path)
+    rect, kind, _, _ = W.ribbon(60.0, 30.0, 6, 300.0)
+    teq = 300.0 + np.arange(6, dtype=float)
+    write_ribbon(tmp_path / "g.xml", rect, kind, teq, teq)
+    base = ('<beRTAMC2D>\n<geometry file="g.xml"/>\n<time dt="5" maxtime="45"/>\n'
+            '<material name="m" modetable="synthetic" nmodes="96" seed="2" thickness="1" T0="300">\n'
+            '  <propagator T="300" file="synthetic_96_2_300K_5ps.P.eigen.bin"/>\n'
+            '  <propagator T="306" file="synthetic_96_2_306K_5ps.P.eigen.bin"/>\n</material>\n'
+            '<Eeff particles="20000" Tmax="305" Tmin="300"/>\n<seed value="3"/>\n{extra}</beRTAMC2D>\n')
+    mat = W.material(96, seed=2)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    Ps = [eigen_bin(tmp_path / f"synthetic_96_2_{T}K_5ps.P.eigen.bin") for T in (300, 306)]
+    prop = pkg.Propagator(ctx, Ps, T=[300.0, 306.0])
+    kw = dict(thickness=1.0, dt=5.0, maxtime=45.0, Efactor=20000, Tmax=305.0, Tmin=300.0, seed=3, prop_interp=1)
+    for extra, grad in (('<interpolation rule="bracket"/>\n', None),
+                        ('<interpolation rule="bracket"/>\n<gradient x="0.02" y="0.0"/>\n', (0.02, 0.0))):
+        (tmp_path / "in.xml").write_text(base.format(extra=extra))
+        r = run([pkg.app_path("beRTAMC2D"), "in.xml", "4"], tmp_path)
+        lines = [ln.split("\t") for ln in r.stdout.splitlines() if ln and not ln.startswith("#")]
+        rows = np.array([[float(x) for x in ln if x != ""] for ln in lines])
+        o = pkg.bertamc2d(ctx, modes, geom, prop, gradient=grad, **kw)
+        # identical up to the first cancellation of bins holding both signs
+        # (which particles survive then depends on their arrival order)
+        assert np.array_equal(o["npart"][:2], rows[:2, 2].astype(np.int64))
+        assert np.allclose(o["props"][:2, :, 0], rows[:2, 3::3], rtol=2e-5, atol=0)
+        assert np.allclose(o["npart"][2:], rows[2:, 2], rtol=0.03)
+        assert rows[-1, 2] > 1000  # the gradient sources fill the ribbon
+    # Scattering's own pair rule has no pair for 301-305 K with two tables
+    (tmp_path / "in.xml").write_text(base.format(extra=""))
+    r = subprocess.run([str(pkg.app_path("beRTAMC2D")), "in.xml", "4"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=180)
+    assert r.returncode == 1 and "Extrapolation" in r.stderr
+
+
 def test_rtamc2d_csv(pkg, ctx, tmp_path):
     W = pkg.workloads
     rect, kind, teq, _ = W.ribbon(60.0, 30.0, 6, 300.0, reservoirs=(300.5, 299.5))

@@ -83,19 +83,42 @@ def test_propagator_pair_rules():
     for rule in (0, 1):
         assert np.allclose(orc.prop_interp(Ps, Ts, 300.0, rule), sc[1], atol=1e-15)
     # 295: Scattering takes the first key not below 295 (300) and the next
-    # (310) and evaluates their line at 295; the bracketing rule takes 290, 300
+    # (310) and evaluates the line of their signed cumulatives at 295; the
+    # bracketing rule interpolates the matrices at 290 and 300
     assert np.allclose(orc.prop_interp(Ps, Ts, 295.0, 0), sc[1] - 0.5 * (sc[2] - sc[1]), atol=1e-14)
-    assert np.allclose(orc.prop_interp(Ps, Ts, 295.0, 1), sc[0] + 0.5 * (sc[1] - sc[0]), atol=1e-14)
+    assert np.allclose(orc.prop_interp(Ps, Ts, 295.0, 1), _signed_cumm(0.5 * (Ps[0] + Ps[1])), atol=1e-14)
     # 305 lies between the last two keys: no key after 310 for rule 0
     with pytest.raises(ValueError):
         orc.prop_interp(Ps, Ts, 305.0, 0)
-    assert np.allclose(orc.prop_interp(Ps, Ts, 305.0, 1), sc[1] + 0.5 * (sc[2] - sc[1]), atol=1e-14)
+    assert np.allclose(orc.prop_interp(Ps, Ts, 302.5, 1), _signed_cumm(0.75 * Ps[1] + 0.25 * Ps[2]), atol=1e-14)
     for Tx in (280.0, 320.0):
         for rule in (0, 1):
             with pytest.raises(ValueError):
                 orc.prop_interp(Ps, Ts, Tx, rule)
     # order of the input does not matter
-    assert np.allclose(orc.prop_interp(Ps[::-1], Ts[::-1], 295.0, 1), sc[0] + 0.5 * (sc[1] - sc[0]), atol=1e-14)
+    assert np.allclose(orc.prop_interp(Ps[::-1], Ts[::-1], 295.0, 1), _signed_cumm(0.5 * (Ps[0] + Ps[1])),
+                       atol=1e-14)
+
+
+def test_signed_cumulative_interpolation_is_unstable_under_sign_noise(pkg):
+    """Why the bracketing rule interpolates matrices: Krylov propagators of
+    the synthetic operator at 300 K and 306 K hold entries far below rounding
+    level, whose computed signs are noise (modelled by adding 1e-17). Where
+    an entry's sign differs between the two tables, lirp's cumulative dips
+    and the next entry takes a large probability, negative when that entry
+    is; the interpolated matrix keeps the negative weight at noise level."""
+    mat = pkg.workloads.material(96, seed=2)
+    rng = np.random.default_rng(0)
+    Ps = []
+    for T in (300.0, 306.0):
+        B = orc.synthetic_B(KB * orc.bose_einstein_kernel(mat["omega"], T), mat["w"], 0.05, 2)
+        Ps.append(orc.build_P(B, 5.0, 20, 0) + rng.normal(0, 1e-17, (96, 96)))
+    a, b = _signed_cumm(Ps[0]), _signed_cumm(Ps[1])
+    lirp = a + 0.5 * (b - a)
+    mat_interp = orc.prop_interp(Ps, [300.0, 306.0], 303.0, 1)
+    neg = lambda sc: max(pkg.prop_pick_pmf(sc[:, j])[sc[:, j] <= 0].sum() for j in range(96))
+    assert neg(lirp) > 0.5        # supercritical sign-flip branching
+    assert neg(mat_interp) < 1e-12
 
 
 def test_get_on_a_signed_cumulative():
