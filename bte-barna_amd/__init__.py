@@ -173,6 +173,15 @@ class RtaRun(_Handle):
         self.ntime = int(np.floor(maxtime / dt + 1e-6)) + 1
         self._keep = (ctx, modes, geom)
         check(lib().bte_rta_create(ctx.h, modes.h, geom.h, C.byref(self.cfg), C.byref(self.h)))
+        self.spectral_boxes, self.nomega, self.nq = [], 0, modes.nq
+
+    def spectral(self, boxes, nomega, sigma=None):
+        """<spectral>: per-mode tallies in these boxes, broadened on nomega
+        ticks after each run (sigma: per-mode broadening or None)"""
+        b = np.ascontiguousarray(boxes, np.int32)
+        sg = None if sigma is None else _dp(_f64(sigma))
+        check(lib().bte_rta_spectral(self.h, len(b), b.ctypes.data_as(C.POINTER(C.c_int32)), int(nomega), sg))
+        self.spectral_boxes, self.nomega = list(b), int(nomega)
 
     def execute(self):
         nt = self.ntime - 1
@@ -181,13 +190,25 @@ class RtaRun(_Handle):
         jy = np.zeros((nt, self.nbox))
         rep = RtaReport()
         check(lib().bte_rta_execute(self.h, _dp(T), _dp(jx), _dp(jy), C.byref(rep)))
-        return {"T": T, "jx": jx, "jy": jy, "events": rep.events, "crossings": rep.crossings,
-                "Eeff": rep.Eeff, "kernel_ms": rep.kernel_ms}
+        out = {"T": T, "jx": jx, "jy": jy, "events": rep.events, "crossings": rep.crossings,
+               "Eeff": rep.Eeff, "kernel_ms": rep.kernel_ms}
+        if self.spectral_boxes:
+            out["spectral"] = []
+            for k in range(len(self.spectral_boxes)):
+                d = {"omega": np.zeros(self.nomega), "dT": np.zeros((nt, self.nomega)),
+                     "jx": np.zeros((nt, self.nomega)), "jy": np.zeros((nt, self.nomega)),
+                     "fd": np.zeros((nt, self.nq))}
+                check(lib().bte_rta_spectral_get(self.h, k, *(_dp(d[x]) for x in ("omega", "dT", "jx", "jy", "fd"))))
+                out["spectral"].append(d)
+        return out
 
 
-def rtamc2d(ctx, modes, geom, **kw):
+def rtamc2d(ctx, modes, geom, spectral=None, **kw):
+    """spectral: {"boxes": [...], "nomega": K, "sigma": None or per mode}"""
     r = RtaRun(ctx, modes, geom, **kw)
     try:
+        if spectral:
+            r.spectral(spectral["boxes"], spectral["nomega"], spectral.get("sigma"))
         return r.execute()
     finally:
         r.close()

@@ -7,10 +7,13 @@
 //
 // Input (<RTA_MC2d>): <geometry file/>, <time dt maxtime/>, <material .../>
 // (see front_common.hpp for the mode-table attributes), <particles N
-// [seed]/>, optional <ballistic/>, <gradient x y/>, <convergence energy flux/>.
+// [seed]/>, optional <ballistic/>, <gradient x y/>, <convergence energy flux/>,
+// <spectral><resolution ticks/><location bin/>...</spectral>.
 // Output per run: temperature_<T0>K_run_<i>.csv, jx_<T0>K_run_<i>.csv,
-// jy_<T0>K_run_<i>.csv with rows t = (i + 0.5) dt; with a nonzero gradient
-// the steady branch writes steady_<T0>K_run_<i>.csv (rows T, jx, jy).
+// jy_<T0>K_run_<i>.csv with rows t = (i + 0.5) dt; with <spectral>, per
+// location deltaT_omega_, jx_omega_, jy_omega_ and fd_q_<loc>_<T0>K_run_<i>.csv
+// (RTA_MC2D.cpp:2473-2541); with a nonzero gradient the steady branch writes
+// steady_<T0>K_run_<i>.csv (rows T, jx, jy).
 #include "front_common.hpp"
 #include <chrono>
 
@@ -44,7 +47,16 @@ static int run(int argc, char** argv) {
         eps_energy = cv->num("energy");
         eps_flux = cv->num("flux");
     }
-    if (root->child("spectral")) std::cout << "# <spectral> tallies are not produced by the GPU engine" << std::endl;
+    // <spectral> (RTA_MC2D.cpp:1930-1945): ticks and locations
+    int nomega = 0;
+    std::vector<int32_t> locs;
+    if (const xml::Node* sp = root->child("spectral")) {
+        if (const xml::Node* rs = sp->child("resolution")) nomega = (int)rs->num("ticks");
+        for (const xml::Node* l : sp->children("location")) locs.push_back((int32_t)l->num("bin"));
+        if (!locs.empty() && nomega < 1) throw std::runtime_error("<spectral> needs <resolution ticks/>");
+        if (steady && !locs.empty())
+            std::cout << "# <spectral> is produced by the transient branch only" << std::endl;
+    }
     Boxes B = read_geometry(geo->str("file"));
     Material M = read_material(*mat);
     for (auto& m : B.material)
@@ -106,7 +118,11 @@ static int run(int argc, char** argv) {
         std::vector<double> T((size_t)nt * B.n()), jx(T.size()), jy(T.size());
         bte_rta_report rep{};
         auto t0 = std::chrono::steady_clock::now();
-        check(bte_rta_run(ctx, modes, geom, &cfg, T.data(), jx.data(), jy.data(), &rep), "RTAMC2D run");
+        bte_rta* rr = nullptr;
+        check(bte_rta_create(ctx, modes, geom, &cfg, &rr), "RTAMC2D setup");
+        if (!locs.empty())
+            check(bte_rta_spectral(rr, (int)locs.size(), locs.data(), nomega, nullptr), "spectral decomposition");
+        check(bte_rta_execute(rr, T.data(), jx.data(), jy.data(), &rep), "RTAMC2D run");
         double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         const int rows = rep.ntime - 1;
         std::cout << "# run " << irun << ": " << rep.events << " events, " << rep.crossings << " crossings, Eeff "
@@ -124,6 +140,42 @@ static int run(int argc, char** argv) {
             std::cout << "# Writing " << names[q] << tag << std::endl;
             write_csv(std::string(names[q]) + tag, rows, B.n() + 1, out.data());
         }
+        if (!locs.empty()) {
+            // RTA_MC2D.cpp:2473-2541: deltaT_omega, jx_omega, jy_omega as
+            // (ntime) x (nomega + 1) with the tick row first (time -1) and
+            // data row i at time (i + 0.5) dt; fd_q without a time column
+            // under a header of the q coordinates (the mode table has none:
+            // zeros)
+            std::cout << "# Writing spectral decompositions" << std::endl;
+            std::vector<double> om(nomega), dT((size_t)rows * nomega), sx(dT.size()), sy(dT.size()),
+                fd((size_t)rows * M.nq);
+            for (size_t k = 0; k < locs.size(); k++) {
+                check(bte_rta_spectral_get(rr, (int)k, om.data(), dT.data(), sx.data(), sy.data(), fd.data()),
+                      "spectral decomposition");
+                const char* sn[3] = {"deltaT_omega_", "jx_omega_", "jy_omega_"};
+                const std::vector<double>* sd[3] = {&dT, &sx, &sy};
+                char stag[96];
+                std::snprintf(stag, sizeof stag, "%i_%gK_run_%i.csv", (int)locs[k], cfg.T0, irun);
+                for (int q = 0; q < 3; q++) {
+                    std::vector<double> out((size_t)(rows + 1) * (nomega + 1));
+                    out[0] = -1.0;
+                    for (int j = 0; j < nomega; j++) out[1 + j] = om[j];
+                    for (int i = 1; i <= rows; i++) {
+                        double* r = out.data() + (size_t)i * (nomega + 1);
+                        r[0] = (i + 0.5) * cfg.dt;
+                        for (int j = 0; j < nomega; j++) r[1 + j] = (*sd[q])[(size_t)(i - 1) * nomega + j];
+                    }
+                    write_csv(std::string(sn[q]) + stag, rows + 1, nomega + 1, out.data());
+                }
+                std::string head = "# ";
+                for (int q = 0; q < M.nq; q++) head += std::to_string(0.0) + (q + 1 < M.nq ? "," : "");
+                head += "\n# ";
+                for (int q = 0; q < M.nq; q++) head += std::to_string(0.0) + (q + 1 < M.nq ? "," : "");
+                head += "\n# ";
+                write_csv(std::string("fd_q_") + stag, rows, M.nq, fd.data(), head);
+            }
+        }
+        check(bte_rta_destroy(rr), "RTAMC2D teardown");
     }
     bte_geom_destroy(geom);
     bte_modes_destroy(modes);

@@ -198,9 +198,11 @@ inline void save_eigen_bin(const std::string& fn, int64_t rows, const double* a)
 
 // alma::write_to_csv (include/io_utils.hpp:38-105): default stream
 // formatting, one row per line
-inline void write_csv(const std::string& fn, int rows, int cols, const double* rowmajor) {
+inline void write_csv(const std::string& fn, int rows, int cols, const double* rowmajor,
+                      const std::string& header = "") {
     std::ofstream f(fn);
     if (!f) throw std::runtime_error("cannot write " + fn);
+    f << header;
     for (int r = 0; r < rows; r++)
         for (int c = 0; c < cols; c++) {
             f << rowmajor[(size_t)r * cols + c];

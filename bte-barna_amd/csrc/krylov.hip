@@ -299,6 +299,20 @@ static int wide_elems(int N) { return N <= 1024 ? 1 : (N <= 2048 ? 2 : (N <= 409
 
 using namespace bte;
 
+namespace bte {
+// rocBLAS handle of a context, created on first use and bound to its stream
+void* blas_handle(Ctx* c) {
+    if (!c->blas) {
+        rocblas_handle h0;
+        RBCHECK(rocblas_create_handle(&h0));
+        c->blas = h0;
+        c->blas_free = [](void* h) { rocblas_destroy_handle((rocblas_handle)h); };
+    }
+    RBCHECK(rocblas_set_stream((rocblas_handle)c->blas, c->stream));
+    return c->blas;
+}
+}  // namespace bte
+
 extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt, int m, int prec,
                                 int mode, double* P, bte_krylov_report* rep) {
     BTE_GUARD_BEGIN
@@ -310,15 +324,7 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
     HIPCHECK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const size_t NN = (size_t)N * N;
-    rocblas_handle hb;
-    if (!c->blas) {
-        rocblas_handle h0;
-        RBCHECK(rocblas_create_handle(&h0));
-        c->blas = h0;
-        c->blas_free = [](void* h) { rocblas_destroy_handle((rocblas_handle)h); };
-    }
-    hb = (rocblas_handle)c->blas;
-    RBCHECK(rocblas_set_stream(hb, s));
+    rocblas_handle hb = (rocblas_handle)blas_handle(c);
     DevBuf<double> dB, dZ64, dH, dC, dP, dErr;
     DevBuf<float> dA32, dZ32, dV32;
     DevBuf<double> dV64;

@@ -14,6 +14,8 @@
 #include <cstring>
 #include <numeric>
 #include <tuple>
+#include <limits>
+#include <rocblas/rocblas.h>
 
 namespace bte {
 
@@ -54,6 +56,8 @@ struct RtaParams {
     double* jx;
     double* jy;
     double* dgz;           // [4][cells] score-weighted gz per side (gradient runs)
+    const int32_t* box_loc; // spectral location of each box, -1 none
+    double* spec;          // [nloc][nt][n] per-mode gz of the spectral locations (null: none)
     unsigned long long* counters;  // events, crossings, next id, stuck, bad mirror, bad translation
     uint64_t total;        // particles over all runs
     uint64_t cells;
@@ -77,10 +81,18 @@ __device__ inline int time_bin(const double* grid, int nt, double target) {
 // segment weighted by the particle's running score for each active side
 template <bool GRAD>
 __device__ inline void tally_segment(const RtaParams& P, int box, double t0, double t1, int sign, double vx,
-                                     double vy, int gmask, const float* score) {
+                                     double vy, int gmask, const float* score, int mode) {
     int b0 = time_bin(P.timegrid, P.nt, t0), b1 = time_bin(P.timegrid, P.nt, t1);
     double delta = t1 - t0;
     const uint64_t base = P.box_cell[box];
+    // spectral locations tally per mode; the Gamma-broadened spectra are one
+    // matrix product after the run (the reference adds the broadened weights
+    // of every segment, RTA_MC2D.cpp:718-766)
+    double* sp = nullptr;
+    if (P.spec) {
+        const int l = P.box_loc[box];
+        if (l >= 0) sp = P.spec + (size_t)l * P.nt * P.n + mode;
+    }
     for (int i = b0; i <= b1; i++) {
         double lo = fmax(P.timegrid[i] - t0, 0.);
         double hi = fmin(P.timegrid[i + 1] - t0, delta);
@@ -90,6 +102,7 @@ __device__ inline void tally_segment(const RtaParams& P, int box, double t0, dou
         atomicAdd(P.gz + k, c);
         atomicAdd(P.jx + k, vx * c);
         atomicAdd(P.jy + k, vy * c);
+        if (sp) atomicAdd(sp + (size_t)i * P.n, c);
         if (GRAD) {
             for (int s = 0; s < 4; s++)
                 if (((gmask >> s) & 1) && score[s] != 0.f) atomicAdd(P.dgz + (size_t)s * P.cells + k, c * (double)score[s]);
@@ -218,7 +231,7 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
         const double vx = md.x, vy = md.y;
         double nx = x + dtrem * vx, ny = y + dtrem * vy;
         if (box_inside(B, nx, ny)) {
-            tally_segment<GRAD>(P, box, t, t + dtrem, sign, vx, vy, gmask, score);
+            tally_segment<GRAD>(P, box, t, t + dtrem, sign, vx, vy, gmask, score, mode);
             x = nx;
             y = ny;
             t += dtrem;
@@ -236,7 +249,7 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
         double tf = box_exit(B, x, y, vx, vy, iv.x, iv.y, mask, ex, ey);
         double left = dtrem - tf;
         if (almost_equal(left, 0.)) left = 0.;
-        tally_segment<GRAD>(P, box, t, t + tf, sign, vx, vy, gmask, score);
+        tally_segment<GRAD>(P, box, t, t + tf, sign, vx, vy, gmask, score, mode);
         t += tf;
         int side = 0;
         const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
@@ -330,6 +343,19 @@ __global__ __launch_bounds__(256) void rta_clear_kernel(double* a, uint64_t n, u
     if (blockIdx.x == 0 && threadIdx.x < 8) cnt[threadIdx.x] = 0ULL;
 }
 
+// running sum over the time bins of every (location, mode) column: the step
+// response, taken before the spectral products (they commute)
+__global__ __launch_bounds__(256) void rta_spec_cumsum_kernel(double* S, int nloc, int nt, int n) {
+    const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (uint64_t)nloc * n) return;
+    double* p = S + (col / n) * (uint64_t)nt * n + col % n;
+    double acc = 0.;
+    for (int t = 0; t < nt; t++) {
+        acc += p[(size_t)t * n];
+        p[(size_t)t * n] = acc;
+    }
+}
+
 struct RtaBatch {
     Ctx* ctx = nullptr;
     const Modes* m = nullptr;
@@ -358,6 +384,13 @@ struct RtaBatch {
     DevBuf<uint64_t> d_box_cell;
     DevBuf<RtaRunDev> d_runs, d_trial_runs;
     DevBuf<unsigned long long> d_counters;
+    // spectral decomposition (one run): locations, per-mode tallies, the
+    // weight matrix [n][3 nomega + nq] and the products [nloc][nt][ncols]
+    int nloc = 0, nomega = 0, ncols = 0;
+    std::vector<int32_t> spec_boxes;
+    std::vector<double> omegagrid, spec_host;
+    DevBuf<int32_t> d_box_loc;
+    DevBuf<double> d_spec, d_specw, d_specout;
     int block = 256, blocks = 0;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -473,6 +506,8 @@ static RtaParams batch_params(RtaBatch* b, const RtaRunDev* runs, uint64_t total
     P.jx = b->d_tally.p + b->cells;
     P.jy = b->d_tally.p + 2 * b->cells;
     P.dgz = b->any_grad ? b->d_dgz.p : nullptr;
+    P.box_loc = b->nloc ? b->d_box_loc.p : nullptr;
+    P.spec = b->nloc ? b->d_spec.p : nullptr;
     P.counters = b->d_counters.p;
     P.total = total;
     P.cells = b->cells;
@@ -492,6 +527,9 @@ static void launch_batch(RtaBatch* b, const RtaParams& P, int block, int blocks,
     if (b->any_grad)
         hipLaunchKernelGGL(rta_clear_kernel, dim3(256), dim3(256), 0, s, b->d_dgz.p, nclear - 3 * b->cells,
                            b->d_counters.p);
+    if (b->nloc)
+        hipLaunchKernelGGL(rta_clear_kernel, dim3(1024), dim3(256), 0, s, b->d_spec.p,
+                           (uint64_t)b->nloc * (b->ntime - 1) * b->m->n, b->d_counters.p);
     if (b->any_grad)
         hipLaunchKernelGGL(rta_history_kernel<true>, dim3(blocks), dim3(block), 0, s, P);
     else
@@ -648,6 +686,29 @@ static void batch_create(RtaBatch* b) {
     HIPCHECK(hipStreamSynchronize(s));
 }
 
+// The spectral arrays of every location as one strided-batched FP64 GEMM:
+// out_l [nt][ncols] = cumsum_t(S_l) [nt][n] x W [n][ncols] (row-major)
+static void spectral_products(RtaBatch* b) {
+    Ctx* c = b->ctx;
+    hipStream_t s = c->stream;
+    const int nt = b->ntime - 1, n = b->m->n;
+    const uint64_t ncolsum = (uint64_t)b->nloc * n;
+    hipLaunchKernelGGL(rta_spec_cumsum_kernel, dim3((unsigned)((ncolsum + 255) / 256)), dim3(256), 0, s, b->d_spec.p,
+                       b->nloc, nt, n);
+    HIPCHECK(hipGetLastError());
+    rocblas_handle hb = (rocblas_handle)blas_handle(c);
+    const double one = 1.0, zero = 0.0;
+    const rocblas_status st = rocblas_dgemm_strided_batched(
+        hb, rocblas_operation_none, rocblas_operation_none, b->ncols, nt, n, &one, b->d_specw.p, b->ncols, 0,
+        b->d_spec.p, n, (rocblas_stride)nt * n, &zero, b->d_specout.p, b->ncols, (rocblas_stride)nt * b->ncols,
+        b->nloc);
+    if (st != rocblas_status_success) throw Error(BTE_EHIP, "rocblas_dgemm_strided_batched failed");
+    b->spec_host.resize((size_t)b->nloc * nt * b->ncols);
+    HIPCHECK(hipMemcpyAsync(b->spec_host.data(), b->d_specout.p, b->spec_host.size() * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
 static void batch_execute(RtaBatch* b, double* T_out, double* jx_out, double* jy_out, double* dTdp,
                           double* Eeff_out, bte_rta_batch_report* rep) {
     Ctx* c = b->ctx;
@@ -686,6 +747,7 @@ static void batch_execute(RtaBatch* b, double* T_out, double* jx_out, double* jy
     if (cnt[4] != 0)
         throw Error(BTE_EINVAL, "specular reflection of a mode without a mirror partner in the mode table");
     if (cnt[5] != 0) throw Error(BTE_EGEOM, "Error in translation, no box found");
+    if (b->nloc) spectral_products(b);
     // post-processing per run (RTA_MC2D.cpp:2336-2435): scale, cumulative
     // sum over time (step response), T = gz / (Cv / vcorr) + T0
     const double th = b->thickness, vcorr = th / b->m->lattz;
@@ -865,6 +927,107 @@ int bte_rta_execute(bte_rta* rr, double* T_out, double* jx_out, double* jy_out, 
         rep->Eeff = E;
         rep->kernel_ms = br.kernel_ms;
         rep->ntime = br.ntime;
+    }
+    BTE_GUARD_END
+}
+
+int bte_rta_spectral(bte_rta* rr, int nloc, const int32_t* boxes, int nomega, const double* sigma) {
+    BTE_GUARD_BEGIN
+    auto* b = reinterpret_cast<RtaBatch*>(rr);
+    if (!b || nloc < 0 || (nloc > 0 && (!boxes || nomega < 1))) throw Error(BTE_EINVAL, "bad arguments to bte_rta_spectral");
+    if (b->nruns != 1) throw Error(BTE_EINVAL, "spectral decomposition needs a single-run handle");
+    Ctx* c = b->ctx;
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    const Modes* m = b->m;
+    const int n = m->n, nbox = b->nbox[0], nq = m->nq, nb = n / nq;
+    b->nloc = 0;
+    if (nloc == 0) return BTE_OK;
+    std::vector<int32_t> loc((size_t)b->box0[0] + nbox, -1);
+    for (int l = 0; l < nloc; l++) {
+        if (boxes[l] < 0 || boxes[l] >= nbox) throw Error(BTE_EINVAL, "spectral location out of range");
+        loc[(size_t)b->box0[0] + boxes[l]] = l;
+    }
+    // omega ticks: LinSpaced(nomega, 0, maxomega) + delta/2, delta =
+    // maxomega / nomega (RTA_MC2D.cpp:2040-2045)
+    double maxomega = 0;
+    for (int i = 0; i < n; i++) maxomega = std::max(maxomega, m->omega[i]);
+    const double delta = maxomega / nomega;
+    b->omegagrid.resize(nomega);
+    for (int i = 0; i < nomega; i++)
+        b->omegagrid[i] = (nomega > 1 ? maxomega * i / (nomega - 1) : maxomega) + 0.5 * delta;
+    // broadening per mode with the low-outlier floor exp(p25 - 1.5 (p75 -
+    // p25)) of log sigma (RTA_MC2D.cpp:2034-2037, isotopic_scattering.cpp:31-50)
+    std::vector<double> sg(n), ls(n);
+    for (int i = 0; i < n; i++) {
+        sg[i] = sigma ? sigma[i] : b->sigma;
+        ls[i] = sg[i] == 0. ? std::numeric_limits<double>::lowest() : std::log(sg[i]);
+    }
+    std::sort(ls.begin(), ls.end());
+    const double p0 = ls[25 * (size_t)n / 100], p1 = ls[75 * (size_t)n / 100];
+    const double lb = std::exp(p0 - 1.5 * (p1 - p0));
+    // W: Gamma weights gamma_pdf(omega^2/sigma, sigma/omega, tick) (RTA_MC2D.cpp:
+    // 61-63, 723-731), times vx and vy, and 1/omega on the mode's q column
+    const int ncols = 3 * nomega + nq;
+    std::vector<double> W((size_t)n * ncols, 0.);
+    for (int i = 0; i < n; i++) {
+        const double om = m->omega[i];
+        if (!(om > 0.)) continue;  // never sampled
+        const double sgi = std::max(sg[i], lb);
+        const double k = om * om / sgi, th = sgi / om, lgk = std::lgamma(k);
+        double* row = W.data() + (size_t)i * ncols;
+        for (int j = 0; j < nomega; j++) {
+            const double z = b->omegagrid[j] / th;
+            const double w = z > 0. ? std::exp((k - 1.) * std::log(z) - z - lgk) / th : 0.;
+            row[j] = w;
+            row[nomega + j] = m->vx[i] * w;
+            row[2 * nomega + j] = m->vy[i] * w;
+        }
+        row[3 * nomega + i / nb] = 1.0 / om;
+    }
+    const int nt = b->ntime - 1;
+    b->d_box_loc.upload(loc, s);
+    b->d_specw.upload(W, s);
+    b->d_spec.alloc((size_t)nloc * nt * n);
+    b->d_specout.alloc((size_t)nloc * nt * ncols);
+    HIPCHECK(hipStreamSynchronize(s));
+    b->spec_boxes.assign(boxes, boxes + nloc);
+    b->nloc = nloc;
+    b->nomega = nomega;
+    b->ncols = ncols;
+    b->spec_host.clear();
+    // a captured launch would miss the spectral tally
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    BTE_GUARD_END
+}
+
+int bte_rta_spectral_get(const bte_rta* rr, int k, double* omega, double* dT, double* jx, double* jy, double* fd_q) {
+    BTE_GUARD_BEGIN
+    auto* b = reinterpret_cast<const RtaBatch*>(rr);
+    if (!b || k < 0 || k >= b->nloc) throw Error(BTE_EINVAL, "no such spectral location");
+    if (b->spec_host.empty()) throw Error(BTE_EINVAL, "bte_rta_spectral_get before bte_rta_execute");
+    const int nt = b->ntime - 1, nw = b->nomega, nq = b->m->nq, nc = b->ncols;
+    const RectBox& bx = b->geoms[0]->boxes[b->spec_boxes[k]];
+    const double th = b->thickness, vbox = bx.area() * th, vcorr = th / b->m->lattz, E = b->Eeff[0];
+    // post-processing (RTA_MC2D.cpp:2349-2405): gz_omega * Eeff / vbox /
+    // (Cv / vcorr), fluxes * Eeff 1e30 / vbox, fd * Eeff vcorr NQ /
+    // (vbox 1e12 hbar); the time sums are already taken
+    const double sT = E / vbox / (b->cv[0] / vcorr), sJ = E * 1e30 / vbox, sF = E * vcorr * nq / (vbox * 1.0e12 * hbar);
+    if (omega) std::copy(b->omegagrid.begin(), b->omegagrid.end(), omega);
+    for (int t = 0; t < nt; t++) {
+        const double* r = b->spec_host.data() + ((size_t)k * nt + t) * nc;
+        for (int j = 0; j < nw; j++) {
+            if (dT) dT[(size_t)t * nw + j] = r[j] * sT;
+            if (jx) jx[(size_t)t * nw + j] = r[nw + j] * sJ;
+            if (jy) jy[(size_t)t * nw + j] = r[2 * nw + j] * sJ;
+        }
+        if (fd_q)
+            for (int q = 0; q < nq; q++) fd_q[(size_t)t * nq + q] = r[3 * nw + q] * sF;
     }
     BTE_GUARD_END
 }

@@ -130,6 +130,10 @@ struct Prop {
     DevBuf<uint2> d_alias;  // per T: ncol x ncol entries, column-major
 };
 
+// rocBLAS handle of a context (krylov.hip), created on first use and bound
+// to the context's stream
+void* blas_handle(Ctx* c);
+
 // ---- host helpers (host_util.cpp) ----
 double h_bose_einstein(double omega, double T);
 double h_be_kernel(double omega, double T);

@@ -116,6 +116,22 @@ int bte_rta_create(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bt
                    bte_rta** out);
 int bte_rta_execute(bte_rta* r, double* T_out, double* jx_out, double* jy_out,
                     bte_rta_report* rep);
+/* Spectral decomposition (<spectral><resolution ticks/><location bin/>,
+ * RTA_MC2D.cpp:1930-1945, 1978-2052; process_segment 718-766; outputs
+ * 2349-2405, 2473-2541), set before bte_rta_execute: the segments in the nloc
+ * boxes are also tallied per mode and broadened afterwards with Gamma
+ * densities of shape omega^2/sigma and scale sigma/omega on nomega
+ * frequency ticks. sigma: per-mode broadening [rad/ps]
+ * (Gamma_grid::base_sigma, qpoint_grid.hpp:179-181), or NULL for the run's
+ * sigma_bounce; low outliers are raised to exp(p25 - 1.5 (p75 - p25)) of
+ * log sigma. nloc = 0 switches it off. */
+int bte_rta_spectral(bte_rta* r, int nloc, const int32_t* boxes, int nomega, const double* sigma);
+/* After bte_rta_execute, location k: omega[nomega] tick centres,
+ * dT[(ntime-1) x nomega] (deltaT_omega), jx, jy [(ntime-1) x nomega]
+ * (jx_omega, jy_omega), fd_q [(ntime-1) x nq] (fd_q); rows are the
+ * cumulative time bins like the temperature output. Outputs may be NULL. */
+int bte_rta_spectral_get(const bte_rta* r, int k, double* omega, double* dT, double* jx, double* jy,
+                         double* fd_q);
 int bte_rta_destroy(bte_rta* r);
 int bte_rta_run(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_cfg* cfg,
                 double* T_out, double* jx_out, double* jy_out, bte_rta_report* rep);

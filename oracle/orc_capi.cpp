@@ -319,6 +319,29 @@ int orc_contacts(int nbox, const double* rect, int maxc, int32_t* ncont, int32_t
 
 // ---------------- RTAMC2D ----------------
 // kind: 0 bin, 1 reservoir, 2 periodic. Outputs [ntime-1][nbox].
+// spectral decomposition of the next orc_rta_run (boxes, ticks, per-mode
+// broadening or NULL), and its results after the run
+static thread_local std::vector<int> g_spec_boxes;
+static thread_local std::vector<double> g_spec_sigma, g_spec_grid, g_spec_out[4];
+static thread_local int g_spec_nomega = 0, g_spec_nt = 0, g_spec_nq = 0, g_spec_nw = 0;
+void orc_rta_spectral(int nloc, const int32_t* boxes, int nomega, int n, const double* sigma) {
+    g_spec_boxes.assign(boxes, boxes + nloc);
+    g_spec_nomega = nomega;
+    if (sigma) g_spec_sigma.assign(sigma, sigma + n);
+    else g_spec_sigma.clear();
+}
+// location k of the last run: omega[nw], dT/jx/jy [nt][nw], fd [nt][nq]
+int orc_rta_spectral_get(int k, double* omega, double* dT, double* jx, double* jy, double* fd) {
+    if (g_spec_out[0].empty() || k < 0 || (size_t)k * g_spec_nt * g_spec_nw >= g_spec_out[0].size()) return 1;
+    const size_t nw = (size_t)g_spec_nt * g_spec_nw, nq = (size_t)g_spec_nt * g_spec_nq;
+    std::copy(g_spec_grid.begin(), g_spec_grid.end(), omega);
+    std::copy(g_spec_out[0].begin() + k * nw, g_spec_out[0].begin() + (k + 1) * nw, dT);
+    std::copy(g_spec_out[1].begin() + k * nw, g_spec_out[1].begin() + (k + 1) * nw, jx);
+    std::copy(g_spec_out[2].begin() + k * nw, g_spec_out[2].begin() + (k + 1) * nw, jy);
+    std::copy(g_spec_out[3].begin() + k * nq, g_spec_out[3].begin() + (k + 1) * nq, fd);
+    return 0;
+}
+double orc_gamma_pdf(double k, double theta, double x) { return gamma_pdf(k, theta, x); }
 int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, const double* w,
                 double V, double lattz, int nq, int nbox, const double* rect, const int32_t* kind,
                 const double* Teq, double thickness, double T0, double dt, double maxtime,
@@ -335,8 +358,19 @@ int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, 
         S.maxtime = maxtime;
         S.ballistic = ballistic != 0;
         S.sigma = sigma;
+        S.spec_boxes.swap(g_spec_boxes);
+        S.spec_sigma.swap(g_spec_sigma);
+        S.nomega = g_spec_nomega;
+        g_spec_boxes.clear();
+        g_spec_sigma.clear();
+        g_spec_nomega = 0;
         rta_run(S, nparticles, seed, nthreads, T_out, jx_out, jy_out, counters, raw_gz);
         if (Eeff_out) *Eeff_out = S.Eeff;
+        g_spec_grid = S.omegagrid;
+        for (int k = 0; k < 4; k++) g_spec_out[k].swap(S.spec_out[k]);
+        g_spec_nt = S.ntime - 1;
+        g_spec_nq = S.m.nq;
+        g_spec_nw = S.nomega;
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

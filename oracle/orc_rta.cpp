@@ -3,6 +3,8 @@
 //   Init_generator          RTA_MC2D.cpp:272-533 (isothermal-wall part)
 //   diffusive_bouncing      RTA_MC2D.cpp:547-679
 //   process_segment         RTA_MC2D.cpp:699-769, get_bin_index 140-148
+//   spectral decomposition  RTA_MC2D.cpp:61-63 (gamma_pdf), 718-766,
+//                           1930-1945, 1978-2052, 2349-2405
 //   evolparticle            RTA_MC2D.cpp:990-1386 (DMM and periodic parts out)
 //   history loop            RTA_MC2D.cpp:2228-2317
 //   post-processing         RTA_MC2D.cpp:2326-2435
@@ -14,6 +16,7 @@
 // is an explicit flag instead of mode (0,0); the Lambert broadening uses a
 // constant sigma per mode because a synthetic table has no q-grid.
 #include "orc_geometry.hpp"
+#include <limits>
 #include <omp.h>
 
 namespace orc {
@@ -24,6 +27,16 @@ struct RtaSetup {
     double thickness = 1.0, T0 = 300.0, dt = 1.0, maxtime = 10.0;
     bool ballistic = false;
     double sigma = 0.25;  // Lambert broadening per mode [rad/ps]
+    // <spectral> (RTA_MC2D.cpp:1930-1945, 1978-2052): boxes with a spectral
+    // decomposition, nomega ticks, per-mode broadening (base_sigma; empty:
+    // sigma for every mode)
+    std::vector<int> spec_boxes;
+    int nomega = 0;
+    std::vector<double> spec_sigma;
+    std::vector<int> spec_loc;        // per box: location index or -1
+    std::vector<double> omegagrid;    // tick centres
+    std::vector<double> spec_out[4];  // results: dT, jx, jy [loc][tbin][omega]; fd [loc][tbin][q]
+    std::vector<double> bsig;         // broadening per mode after the outlier floor
 
     // derived
     int ntime = 0;
@@ -37,7 +50,38 @@ struct RtaSetup {
     // Lambert table: candidates within nsigma*sqrt(2)*sigma, with pdf values
     std::vector<std::vector<std::pair<int, double>>> lambert;
 
+    void prepare_spectral() {
+        spec_loc.assign(sys.size(), -1);
+        if (spec_boxes.empty()) return;
+        if (nomega < 1) throw input_error("spectral resolution must be at least one tick");
+        for (size_t l = 0; l < spec_boxes.size(); l++) {
+            const int b = spec_boxes[l];
+            if (b < 0 || b >= (int)sys.size()) throw input_error("spectral location out of range");
+            spec_loc[b] = (int)l;
+        }
+        // omega grid: LinSpaced(nomega, 0, maxomega) + delta/2, delta =
+        // maxomega / nomega (RTA_MC2D.cpp:2040-2045)
+        double maxomega = 0;
+        for (int i = 0; i < m.n; i++) maxomega = std::max(maxomega, m.omega[i]);
+        const double delta = maxomega / nomega;
+        omegagrid.resize(nomega);
+        for (int i = 0; i < nomega; i++)
+            omegagrid[i] = (nomega > 1 ? maxomega * i / (nomega - 1) : maxomega) + 0.5 * delta;
+        bsig.resize(m.n);
+        for (int i = 0; i < m.n; i++) bsig[i] = spec_sigma.empty() ? sigma : spec_sigma[i];
+        // outlier floor exp(p25 - 1.5 (p75 - p25)) of log sigma
+        // (calc_percentiles_log, isotopic_scattering.cpp:31-50)
+        std::vector<double> ls(m.n);
+        for (int i = 0; i < m.n; i++)
+            ls[i] = bsig[i] == 0. ? std::numeric_limits<double>::lowest() : std::log(bsig[i]);
+        std::sort(ls.begin(), ls.end());
+        const double p0 = ls[25 * (size_t)m.n / 100], p1 = ls[75 * (size_t)m.n / 100];
+        const double lb = std::exp(p0 - 1.5 * (p1 - p0));
+        for (auto& v : bsig) v = std::max(v, lb);
+    }
+
     void prepare(uint64_t nparticles) {
+        prepare_spectral();
         ntime = (int)std::floor(maxtime / dt + 1.0e-6) + 1;
         maxtime = (ntime - 1) * dt;
         timegrid.resize(ntime);
@@ -127,8 +171,24 @@ struct Part {
 struct Tally {
     int nbox, nt;
     std::vector<double> gz, jx, jy;  // [box][tbin]
+    // spectral: [loc][tbin][omega] and [loc][tbin][q]
+    std::vector<double> gzw, jxw, jyw, fdq;
     Tally(int nb, int nt_) : nbox(nb), nt(nt_), gz(nb * nt_, 0.), jx(nb * nt_, 0.), jy(nb * nt_, 0.) {}
+    void spectral(size_t nloc, int nomega, int nq) {
+        gzw.assign(nloc * nt * nomega, 0.);
+        jxw.assign(nloc * nt * nomega, 0.);
+        jyw.assign(nloc * nt * nomega, 0.);
+        fdq.assign(nloc * nt * nq, 0.);
+    }
 };
+
+// gamma_pdf (RTA_MC2D.cpp:61-63): density of the Gamma distribution with
+// shape k and scale theta at x (boost gamma_p_derivative(k, x/theta)/theta)
+inline double gamma_pdf(double k, double theta, double x) {
+    if (x <= 0.) return 0.;
+    const double z = x / theta;
+    return std::exp((k - 1.) * std::log(z) - z - std::lgamma(k)) / theta;
+}
 
 static inline int bin_index(const std::vector<double>& grid, double target) {
     long idx = (long)(std::lower_bound(grid.begin(), grid.end(), target) - grid.begin()) - 1;
@@ -136,11 +196,20 @@ static inline int bin_index(const std::vector<double>& grid, double target) {
     return (int)std::clamp(idx, 0L, hi);
 }
 
-// process_segment, transient overload (RTA_MC2D.cpp:699-769)
+// process_segment, transient overload (RTA_MC2D.cpp:699-769), with the
+// Gamma-broadened spectral tallies of a spectral location (718-766)
 static void process_segment(const RtaSetup& S, Tally& T, int box, double t0,
-                            double t1, int sign, double vx, double vy) {
+                            double t1, int sign, double vx, double vy, int mode) {
     int b0 = bin_index(S.timegrid, t0), b1 = bin_index(S.timegrid, t1);
     double delta = t1 - t0;
+    const int loc = S.spec_loc.empty() ? -1 : S.spec_loc[box];
+    std::vector<double> gw;
+    if (loc >= 0) {
+        const double om = S.m.omega[mode], sg = S.bsig[mode];
+        const double k = om * om / sg, theta = sg / om;
+        gw.resize(S.nomega);
+        for (int i = 0; i < S.nomega; i++) gw[i] = gamma_pdf(k, theta, S.omegagrid[i]);
+    }
     for (int i = b0; i <= b1; i++) {
         double lo = std::max(S.timegrid[i] - t0, 0.);
         double hi = std::min(S.timegrid[i + 1] - t0, delta);
@@ -148,6 +217,16 @@ static void process_segment(const RtaSetup& S, Tally& T, int box, double t0,
         T.gz[box * T.nt + i] += contr;
         T.jx[box * T.nt + i] += vx * contr;
         T.jy[box * T.nt + i] += vy * contr;
+        if (loc >= 0) {
+            const size_t o = ((size_t)loc * T.nt + i) * S.nomega;
+            for (int w = 0; w < S.nomega; w++) {
+                T.gzw[o + w] += contr * gw[w];
+                T.jxw[o + w] += vx * contr * gw[w];
+                T.jyw[o + w] += vy * contr * gw[w];
+            }
+            const int nb = S.m.n / S.m.nq;
+            T.fdq[((size_t)loc * T.nt + i) * S.m.nq + mode / nb] += contr / S.m.omega[mode];
+        }
     }
 }
 
@@ -193,7 +272,7 @@ static void evolparticle(const RtaSetup& S, Part& p, double vx, double vy,
     if (box.inside(nx, ny)) {
         p.x = nx;
         p.y = ny;
-        process_segment(S, T, p.box, p.t, p.t + dt, p.sign, vx, vy);
+        process_segment(S, T, p.box, p.t, p.t + dt, p.sign, vx, vy, p.mode);
         p.t += dt;
         if (!S.ballistic) p.mode = (int)S.sampler.sample(rng);
         C.events++;
@@ -227,7 +306,7 @@ static void evolparticle(const RtaSetup& S, Part& p, double vx, double vy,
             pboxes.push_back(c);
     }
     if (pboxes.empty()) {
-        process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy);
+        process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode);
         p.t += tf;
         int bid = p.box;
         std::vector<int> cids;
@@ -269,13 +348,13 @@ static void evolparticle(const RtaSetup& S, Part& p, double vx, double vy,
     if (pboxes.size() == 1) {
         p.x = rx;
         p.y = ry;
-        process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy);
+        process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode);
         p.t += tf;
         p.box = pboxes[0];
         C.crossings++;
         return;
     }
-    process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy);
+    process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode);
     p.t += tf;
     p.x = rx;
     p.y = ry;
@@ -308,6 +387,9 @@ int rta_run(RtaSetup& S, uint64_t nparticles, uint64_t seed, int nthreads,
     if (nthreads > 0) omp_set_num_threads(nthreads);
     int nth = omp_get_max_threads();
     std::vector<Tally> tallies(nth, Tally(nb, nt));
+    const size_t nloc = S.spec_boxes.size();
+    if (nloc)
+        for (auto& t : tallies) t.spectral(nloc, S.nomega, S.m.nq);
     std::vector<Counters> cnt(nth);
     const double maxtime = S.maxtime;
     std::string err;
@@ -351,6 +433,48 @@ int rta_run(RtaSetup& S, uint64_t nparticles, uint64_t seed, int nthreads,
     }
     if (raw_gz)
         for (int k = 0; k < nb * nt; k++) raw_gz[k] = gz[k];
+    // spectral post-processing (RTA_MC2D.cpp:2349-2405): scale, cumulative
+    // sum over time, deltaT = gz_omega / (Cv / vcorr)
+    if (nloc) {
+        const int nw = S.nomega, nq = S.m.nq;
+        Tally& A = tallies[0];
+        for (int t = 1; t < nth; t++)
+            for (size_t k = 0; k < A.gzw.size(); k++) {
+                A.gzw[k] += tallies[t].gzw[k];
+                A.jxw[k] += tallies[t].jxw[k];
+                A.jyw[k] += tallies[t].jyw[k];
+            }
+        for (int t = 1; t < nth; t++)
+            for (size_t k = 0; k < A.fdq.size(); k++) A.fdq[k] += tallies[t].fdq[k];
+        const double vcorr = S.thickness / S.m.lattz;
+        for (size_t l = 0; l < nloc; l++) {
+            const double vbox = S.sys[S.spec_boxes[l]].area * S.thickness;
+            for (int i = 0; i < nt; i++) {
+                for (int w = 0; w < nw; w++) {
+                    const size_t o = (l * nt + i) * nw + w, p = (l * nt + (i > 0 ? i - 1 : 0)) * nw + w;
+                    A.gzw[o] *= S.Eeff / vbox;
+                    A.jxw[o] *= S.Eeff * 1e30 / vbox;
+                    A.jyw[o] *= S.Eeff * 1e30 / vbox;
+                    if (i > 0) {
+                        A.gzw[o] += A.gzw[p];
+                        A.jxw[o] += A.jxw[p];
+                        A.jyw[o] += A.jyw[p];
+                    }
+                }
+                for (int q = 0; q < nq; q++) {
+                    const size_t o = (l * nt + i) * nq + q;
+                    A.fdq[o] *= S.Eeff * vcorr * nq / (vbox * 1.0e12 * hbar);
+                    if (i > 0) A.fdq[o] += A.fdq[o - nq];
+                }
+            }
+            for (int i = 0; i < nt; i++)
+                for (int w = 0; w < nw; w++) A.gzw[(l * nt + i) * nw + w] /= S.cv / vcorr;
+        }
+        S.spec_out[0].swap(A.gzw);
+        S.spec_out[1].swap(A.jxw);
+        S.spec_out[2].swap(A.jyw);
+        S.spec_out[3].swap(A.fdq);
+    }
     for (int b = 0; b < nb; b++) {
         double vbox = S.sys[b].area * S.thickness;
         for (int k = 0; k < nt; k++) {

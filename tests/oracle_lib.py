@@ -25,6 +25,7 @@ def lib():
         _L.orc_calc_cv.restype = C.c_double
         _L.orc_wall_dist.restype = C.c_double
         _L.orc_error_bound.restype = C.c_double
+        _L.orc_gamma_pdf.restype = C.c_double
     return _L
 
 
@@ -187,11 +188,20 @@ def set_periodic(nbox, periodic):
 
 
 def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, ballistic=False, sigma=0.25,
-            nthreads=0, periodic=None):
+            nthreads=0, periodic=None, spectral=None):
+    """spectral: {"boxes": [...], "nomega": K, "sigma": per-mode array or
+    None}; the result then holds "spectral": one dict per location with
+    omega, dT, jx, jy [nt][K] and fd [nt][nq]"""
     rect, Teq = _f(rect), _f(Teq)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
     set_periodic(nbox, periodic)
+    nq = int(mat.get("nq", 1))
+    if spectral:
+        sb = np.ascontiguousarray(spectral["boxes"], np.int32)
+        sg = spectral.get("sigma")
+        lib().orc_rta_spectral(len(sb), sb.ctypes.data_as(ip32), int(spectral["nomega"]), len(mat["omega"]),
+                               None if sg is None else _dp(_f(sg)))
     ntime = int(np.floor(maxtime / dt + 1e-6)) + 1
     nt = ntime - 1
     T = np.zeros((nt, nbox))
@@ -202,14 +212,27 @@ def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, 
     eeff = C.c_double()
     om, vx, vy, w = (_f(mat[k]) for k in ("omega", "vx", "vy", "w"))
     rc = lib().orc_rta_run(len(om), _dp(om), _dp(vx), _dp(vy), _dp(w), C.c_double(mat["V"]),
-                           C.c_double(mat.get("lattz", 1.0)), 1, nbox, _dp(rect), kind.ctypes.data_as(ip32),
+                           C.c_double(mat.get("lattz", 1.0)), nq, nbox, _dp(rect), kind.ctypes.data_as(ip32),
                            _dp(Teq), C.c_double(thickness), C.c_double(T0), C.c_double(dt), C.c_double(maxtime),
                            u64(nparticles), u64(seed), int(ballistic), C.c_double(sigma), nthreads, _dp(T),
                            _dp(jx), _dp(jy), cnt, _dp(raw), C.byref(eeff))
     if rc != 0:
         raise RuntimeError(lib().orc_last_error().decode())
-    return {"T": T, "jx": jx, "jy": jy, "events": cnt[0], "crossings": cnt[1], "nudges": cnt[2], "raw_gz": raw,
-            "Eeff": eeff.value}
+    out = {"T": T, "jx": jx, "jy": jy, "events": cnt[0], "crossings": cnt[1], "nudges": cnt[2], "raw_gz": raw,
+           "Eeff": eeff.value}
+    if spectral:
+        K = int(spectral["nomega"])
+        out["spectral"] = []
+        for k in range(len(spectral["boxes"])):
+            d = {"omega": np.zeros(K), "dT": np.zeros((nt, K)), "jx": np.zeros((nt, K)), "jy": np.zeros((nt, K)),
+                 "fd": np.zeros((nt, nq))}
+            lib().orc_rta_spectral_get(k, *(_dp(d[x]) for x in ("omega", "dT", "jx", "jy", "fd")))
+            out["spectral"].append(d)
+    return out
+
+
+def gamma_pdf(k, theta, x):
+    return lib().orc_gamma_pdf(C.c_double(k), C.c_double(theta), C.c_double(x))
 
 
 def rta_steady_run(mat, rect, kind, Teq, thickness, T0, gradient, nparticles, seed, sigma=0.25, eps_energy=-42.0,
