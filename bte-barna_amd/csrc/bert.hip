@@ -581,12 +581,30 @@ __device__ inline int guide_key(double v, double scale) {
     return k >= (double)(BERT_GUIDE - 1) ? BERT_GUIDE - 1 : (int)k;
 }
 
+// inclusive prefix sum of v over the 256 threads of a workgroup (wave scans
+// by shuffles, then the four wave totals through LDS); total in *tot
+__device__ inline double block_scan_256(double v, double* wtot, double& tot) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int d = 1; d < 64; d <<= 1) {
+        const double u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
+    }
+    if (lane == 63) wtot[w] = v;
+    __syncthreads();
+    double off = 0.;
+    for (int k = 0; k < w; k++) off += wtot[k];
+    tot = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+    __syncthreads();  // wtot is reused by the next call
+    return v + off;
+}
+
 // Adiabatic diffuse walls (beyondRTA_MC2d.cpp:281-435), phase 1: one
 // workgroup per (box, side) with a nonzero Eborder computes the border
 // temperature from the J(T) spline and the emission cumulative of
 // Isothermal_wall_distribution(Tborder, Teq, -n) (sampling.cpp:189-269).
 __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_t step) {
-    __shared__ double wsum[256];
+    __shared__ double wtot[4];
+    __shared__ int kcarry[4];
     const int b = blockIdx.x;  // box*4 + side
     const int box = b >> 2, side = b & 3;
     int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
@@ -621,51 +639,54 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     ny = -ny;
     const int n = P.n;
     double* cdf = P.wcdf + (size_t)b * n;
-    const int per = (n + blockDim.x - 1) / blockDim.x;
-    const int i0 = threadIdx.x * per, i1 = min(n, i0 + per);
-    double acc = 0;
-    for (int i = i0; i < i1; i++) {
-        const double2 v = P.vel[i];
-        const double om = P.omega[i];
-        double vn = nx * v.x + ny * v.y;
+    // the cumulative in chunks of 256 consecutive modes, one mode per thread
+    // (coalesced), carrying the running total from chunk to chunk
+    double carry = 0.;
+    for (int c0 = 0; c0 < n; c0 += 256) {
+        const int i = c0 + threadIdx.x;
         double pm = 0.;
-        if (vn > 0. && !almost_equal(0., om))
-            pm = vn * om * fabs(bose_einstein(om, Tb) - bose_einstein(om, Teq));
-        acc += pm;
-        cdf[i] = acc;
+        if (i < n) {
+            const double2 v = P.vel[i];
+            const double om = P.omega[i];
+            const double vn = nx * v.x + ny * v.y;
+            if (vn > 0. && !almost_equal(0., om)) pm = vn * om * fabs(bose_einstein(om, Tb) - bose_einstein(om, Teq));
+        }
+        double ctot;
+        const double incl = block_scan_256(pm, wtot, ctot);
+        if (i < n) cdf[i] = carry + incl;
+        carry += ctot;
     }
-    wsum[threadIdx.x] = acc;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        double v = threadIdx.x >= d ? wsum[threadIdx.x - d] : 0.;
-        __syncthreads();
-        wsum[threadIdx.x] += v;
-        __syncthreads();
-    }
-    const double off = threadIdx.x > 0 ? wsum[threadIdx.x - 1] : 0.;
-    __shared__ double tot;
-    for (int i = i0; i < i1; i++) {
-        cdf[i] += off;
-        if (i == n - 1) tot = cdf[i];
-    }
-    __syncthreads();
+    const double tot = carry;
     if (threadIdx.x == 0) {
         P.wcount[b] = tot > 0. ? (uint32_t)(fabs((double)E) + 1.0e-6) : 0u;
         P.wsign[b] = (Tb - Teq) >= 0 ? 1 : -1;
     }
     if (!(tot > 0.)) return;
     // guide table: G[j] = first index whose key reaches j, so the search for
-    // u starts at G[key(u)] <= answer (key is monotone)
+    // u starts at G[key(u)] <= answer (key is monotone); each thread fills
+    // the keys between its predecessor's and its own
     const double scale = (double)BERT_GUIDE / tot;
     int32_t* G = P.wguide + (size_t)b * BERT_GUIDE;
-    int kprev = i0 > 0 ? guide_key(cdf[i0 - 1], scale) : -1;
-    for (int i = i0; i < i1; i++) {
-        const int k = guide_key(cdf[i], scale);
-        for (int j = kprev + 1; j <= k; j++) G[j] = i;
-        kprev = k;
+    int kprev_chunk = -1;  // key of the last mode of the previous chunk
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int c0 = 0; c0 < n; c0 += 256) {
+        const int i = c0 + threadIdx.x;
+        // own cdf values were written by this thread: no barrier needed
+        const int k = i < n ? guide_key(cdf[i], scale) : -1;
+        int kp = __shfl_up(k, 1, 64);
+        if (lane == 63) kcarry[w] = k;
+        __syncthreads();
+        if (lane == 0) kp = w > 0 ? kcarry[w - 1] : kprev_chunk;
+        const int klast = kcarry[3];
+        __syncthreads();
+        if (i < n)
+            for (int j = kp + 1; j <= k; j++) G[j] = i;
+        kprev_chunk = klast;
     }
-    if (i1 == n && i0 < i1)
-        for (int j = kprev + 1; j < BERT_GUIDE; j++) G[j] = n - 1;
+    // keys above the last cumulative value (rounding) point at the last mode
+    // (cdf[n-1] was written before the barriers above)
+    const int klast = guide_key(cdf[n - 1], scale);
+    for (int j = klast + 1 + threadIdx.x; j < BERT_GUIDE; j += blockDim.x) G[j] = n - 1;
 }
 
 // Phase 2: the |Eborder| particles of every side form one index space split
@@ -1421,8 +1442,10 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         }
     }
     b->nsrc = (int)srcs.size();
-    b->step_blocks = std::min(2 * c->cu_count, 2048);
-    b->emit_blocks = std::min(c->cu_count, 1024);
+    // three step workgroups per CU: as many as its registers (166 VGPRs, 3
+    // waves per SIMD) keep resident, for latency hiding; two for emission
+    b->step_blocks = std::min(3 * c->cu_count, 2048);
+    b->emit_blocks = std::min(2 * c->cu_count, 1024);
     b->nsegs = b->step_blocks + b->emit_blocks + b->nsrc;
     if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
     // generous by default (32 B per particle and buffer): reservoir-driven
