@@ -249,6 +249,49 @@ public:
     }
 };
 
+// msgpack reader for the frames written by pack_frame (arrays, integers,
+// floats; the subset msgpack-c's packer uses for them)
+class Unpack {
+public:
+    Unpack(const char* p, size_t n) : p_((const uint8_t*)p), end_((const uint8_t*)p + n) {}
+    uint32_t array() {
+        const uint8_t t = u8();
+        if ((t & 0xf0) == 0x90) return t & 0x0f;
+        if (t == 0xdc) return (uint32_t)be(2);
+        if (t == 0xdd) return (uint32_t)be(4);
+        throw std::runtime_error("msgpack: array expected");
+    }
+    double number() {
+        const uint8_t t = u8();
+        if (t < 0x80) return (double)t;
+        if (t >= 0xe0) return (double)(int8_t)t;
+        switch (t) {
+            case 0xcb: { uint64_t b = be(8); double d; std::memcpy(&d, &b, 8); return d; }
+            case 0xca: { uint32_t b = (uint32_t)be(4); float f; std::memcpy(&f, &b, 4); return f; }
+            case 0xcc: return (double)be(1);
+            case 0xcd: return (double)be(2);
+            case 0xce: return (double)be(4);
+            case 0xcf: return (double)be(8);
+            case 0xd0: return (double)(int8_t)be(1);
+            case 0xd1: return (double)(int16_t)be(2);
+            case 0xd2: return (double)(int32_t)be(4);
+            case 0xd3: return (double)(int64_t)be(8);
+        }
+        throw std::runtime_error("msgpack: number expected");
+    }
+private:
+    const uint8_t *p_, *end_;
+    uint8_t u8() {
+        if (p_ >= end_) throw std::runtime_error("msgpack: truncated frame");
+        return *p_++;
+    }
+    uint64_t be(int nbytes) {
+        uint64_t v = 0;
+        for (int i = 0; i < nbytes; i++) v = (v << 8) | u8();
+        return v;
+    }
+};
+
 // One frame: [time, Eeff, [T per box], then per box (2*nnz, [mode, count]...)
 // or (0, [42.0]) for an empty box]; written as "<size>#<bytes>", the file
 // ends with "0#".

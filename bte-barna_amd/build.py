@@ -20,7 +20,7 @@ CFLAGS = [
     "-I", os.path.join(ROOT, "include"), "-Wno-unused-command-line-argument",
 ]
 SOURCES = ["host_util.cpp", "rta.hip", "bert.hip", "krylov.hip"]
-APP_SOURCES = ["RTAMC2D.cpp", "beRTAMC2D.cpp", "PropagatorBuilder.cpp"]
+APP_SOURCES = ["RTAMC2D.cpp", "beRTAMC2D.cpp", "PropagatorBuilder.cpp", "dist_reader.cpp"]
 
 
 def _run(cmd):

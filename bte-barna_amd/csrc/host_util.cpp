@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstring>
 #include <numeric>
+#include <limits>
 
 namespace bte {
 
@@ -153,6 +154,39 @@ std::vector<double2> gradient_T(const Geom& g) {
         out[i] = make_double2(gx, gy);
     }
     return out;
+}
+
+void spectral_weights(const Modes& m, const double* sigma, double sigma_default, int nomega, const double* grid,
+                      std::vector<double>& W) {
+    const int n = m.n, nq = m.nq, nb = n / nq, ncols = 3 * nomega + nq;
+    // broadening per mode with the low-outlier floor exp(p25 - 1.5 (p75 -
+    // p25)) of log sigma (RTA_MC2D.cpp:2034-2037, dist_reader.cpp:327-331,
+    // isotopic_scattering.cpp:31-50)
+    std::vector<double> sg(n), ls(n);
+    for (int i = 0; i < n; i++) {
+        sg[i] = sigma ? sigma[i] : sigma_default;
+        ls[i] = sg[i] == 0. ? std::numeric_limits<double>::lowest() : std::log(sg[i]);
+    }
+    std::sort(ls.begin(), ls.end());
+    const double p0 = ls[25 * (size_t)n / 100], p1 = ls[75 * (size_t)n / 100];
+    const double lb = std::exp(p0 - 1.5 * (p1 - p0));
+    W.assign((size_t)n * ncols, 0.);
+    for (int i = 0; i < n; i++) {
+        const double om = m.omega[i];
+        if (almost_equal(om, 0.)) continue;  // Gamma modes carry no particles
+        double* row = W.data() + (size_t)i * ncols;
+        row[3 * nomega + i / nb] = 1.0 / om;
+        const double sgi = std::max(sg[i], lb);
+        if (almost_equal(sgi, 0.)) continue;  // skipped as in dist_reader.cpp:357-358
+        const double k = om * om / sgi, th = sgi / om, lgk = std::lgamma(k);
+        for (int j = 0; j < nomega; j++) {
+            const double z = grid[j] / th;
+            const double w = z > 0. ? std::exp((k - 1.) * std::log(z) - z - lgk) / th : 0.;
+            row[j] = w;
+            row[nomega + j] = m.vx[i] * w;
+            row[2 * nomega + j] = m.vy[i] * w;
+        }
+    }
 }
 
 // Isothermal_wall_distribution pmf (sampling.cpp:189-269); normal points into

@@ -940,7 +940,7 @@ int bte_rta_spectral(bte_rta* rr, int nloc, const int32_t* boxes, int nomega, co
     HIPCHECK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const Modes* m = b->m;
-    const int n = m->n, nbox = b->nbox[0], nq = m->nq, nb = n / nq;
+    const int n = m->n, nbox = b->nbox[0], nq = m->nq;
     b->nloc = 0;
     if (nloc == 0) return BTE_OK;
     std::vector<int32_t> loc((size_t)b->box0[0] + nbox, -1);
@@ -956,35 +956,9 @@ int bte_rta_spectral(bte_rta* rr, int nloc, const int32_t* boxes, int nomega, co
     b->omegagrid.resize(nomega);
     for (int i = 0; i < nomega; i++)
         b->omegagrid[i] = (nomega > 1 ? maxomega * i / (nomega - 1) : maxomega) + 0.5 * delta;
-    // broadening per mode with the low-outlier floor exp(p25 - 1.5 (p75 -
-    // p25)) of log sigma (RTA_MC2D.cpp:2034-2037, isotopic_scattering.cpp:31-50)
-    std::vector<double> sg(n), ls(n);
-    for (int i = 0; i < n; i++) {
-        sg[i] = sigma ? sigma[i] : b->sigma;
-        ls[i] = sg[i] == 0. ? std::numeric_limits<double>::lowest() : std::log(sg[i]);
-    }
-    std::sort(ls.begin(), ls.end());
-    const double p0 = ls[25 * (size_t)n / 100], p1 = ls[75 * (size_t)n / 100];
-    const double lb = std::exp(p0 - 1.5 * (p1 - p0));
-    // W: Gamma weights gamma_pdf(omega^2/sigma, sigma/omega, tick) (RTA_MC2D.cpp:
-    // 61-63, 723-731), times vx and vy, and 1/omega on the mode's q column
     const int ncols = 3 * nomega + nq;
-    std::vector<double> W((size_t)n * ncols, 0.);
-    for (int i = 0; i < n; i++) {
-        const double om = m->omega[i];
-        if (!(om > 0.)) continue;  // never sampled
-        const double sgi = std::max(sg[i], lb);
-        const double k = om * om / sgi, th = sgi / om, lgk = std::lgamma(k);
-        double* row = W.data() + (size_t)i * ncols;
-        for (int j = 0; j < nomega; j++) {
-            const double z = b->omegagrid[j] / th;
-            const double w = z > 0. ? std::exp((k - 1.) * std::log(z) - z - lgk) / th : 0.;
-            row[j] = w;
-            row[nomega + j] = m->vx[i] * w;
-            row[2 * nomega + j] = m->vy[i] * w;
-        }
-        row[3 * nomega + i / nb] = 1.0 / om;
-    }
+    std::vector<double> W;
+    spectral_weights(*m, sigma, b->sigma, nomega, b->omegagrid.data(), W);
     const int nt = b->ntime - 1;
     b->d_box_loc.upload(loc, s);
     b->d_specw.upload(W, s);
@@ -1029,6 +1003,33 @@ int bte_rta_spectral_get(const bte_rta* rr, int k, double* omega, double* dT, do
         if (fd_q)
             for (int q = 0; q < nq; q++) fd_q[(size_t)t * nq + q] = r[3 * nw + q] * sF;
     }
+    BTE_GUARD_END
+}
+
+int bte_spectra(bte_ctx* ctx, const bte_modes* mm, const double* sigma, double sigma_default, int nomega,
+                const double* grid, int nrows, const double* X, double* out) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    auto* m = reinterpret_cast<const Modes*>(mm);
+    if (!c || !m || !grid || !X || !out || nomega < 1 || nrows < 0) throw Error(BTE_EINVAL, "bad arguments to bte_spectra");
+    if (nrows == 0) return BTE_OK;
+    HIPCHECK(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    std::vector<double> W;
+    spectral_weights(*m, sigma, sigma_default, nomega, grid, W);
+    const int n = m->n, ncols = 3 * nomega + m->nq;
+    DevBuf<double> dW, dX, dO;
+    dW.upload(W, s);
+    dX.upload(X, (size_t)nrows * n, s);
+    dO.alloc((size_t)nrows * ncols);
+    // out [nrows][ncols] = X [nrows][n] x W [n][ncols], row-major
+    rocblas_handle hb = (rocblas_handle)blas_handle(c);
+    const double one = 1.0, zero = 0.0;
+    if (rocblas_dgemm(hb, rocblas_operation_none, rocblas_operation_none, ncols, nrows, n, &one, dW.p, ncols, dX.p, n,
+                      &zero, dO.p, ncols) != rocblas_status_success)
+        throw Error(BTE_EHIP, "rocblas_dgemm failed");
+    HIPCHECK(hipMemcpyAsync(out, dO.p, (size_t)nrows * ncols * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
     BTE_GUARD_END
 }
 

@@ -144,6 +144,14 @@ bool build_alias(const double* pmf, int n, uint32_t* thresh, uint32_t* alias);
 // for R uniform in [0, 1) over a signed cumulative sc[n]: pick k with
 // probability pmf[k]; a lower_bound past the end picks n-1.
 void pick_pmf(const double* sc, int n, double* pmf);
+// Spectral weight matrix W [n][3 nomega + nq] (row-major) on the frequency
+// ticks grid: Gamma densities gamma_pdf(omega^2/sigma, sigma/omega, tick)
+// of every mode (RTA_MC2D.cpp:61-63, 723-731; dist_reader.cpp:349-366), the
+// same times vx and vy, and 1/omega in the mode's q column. sigma per mode,
+// or sigma_default for all, raised to exp(p25 - 1.5 (p75 - p25)) of log
+// sigma; modes with omega ~ 0 get no weights.
+void spectral_weights(const Modes& m, const double* sigma, double sigma_default, int nomega, const double* grid,
+                      std::vector<double>& W);
 // calculate_gradientT (geometry_2d.cpp:524-576) per box [K/nm]
 std::vector<double2> gradient_T(const struct Geom& g);
 // isothermal wall pmf (sampling.cpp:189-269) into pmf; returns cumulsum

@@ -133,6 +133,15 @@ int bte_rta_spectral(bte_rta* r, int nloc, const int32_t* boxes, int nomega, con
 int bte_rta_spectral_get(const bte_rta* r, int k, double* omega, double* dT, double* jx, double* jy,
                          double* fd_q);
 int bte_rta_destroy(bte_rta* r);
+/* Gamma-broadened spectra of per-mode weights (dist_reader's
+ * print_distribution, dist_reader.cpp:290-380, on a frame's histogram; the
+ * same broadening as the RTAMC2D spectra): out[nrows][3 nomega + nq] =
+ * X[nrows][n] x W with W = (gamma_pdf(omega^2/sigma, sigma/omega, grid_j),
+ * vx times it, vy times it, 1/omega on the mode's q column). sigma: per
+ * mode, or NULL for sigma_default; low outliers raised as in
+ * bte_rta_spectral. One matrix product on the GPU. */
+int bte_spectra(bte_ctx* ctx, const bte_modes* m, const double* sigma, double sigma_default, int nomega,
+                const double* grid, int nrows, const double* X, double* out);
 int bte_rta_run(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_cfg* cfg,
                 double* T_out, double* jx_out, double* jy_out, bte_rta_report* rep);
 

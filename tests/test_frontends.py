@@ -34,6 +34,8 @@ def test_usage_errors(pkg, tmp_path):
     for name in ("RTAMC2D", "beRTAMC2D", "PropagatorBuilder"):
         r = run([app(pkg, name)], tmp_path)
         assert r.returncode == 1 and "Usage" in r.stderr
+    r = run([app(pkg, "dist_reader")], tmp_path)  # dist_reader.cpp:547-551
+    assert r.returncode == 1 and "USAGE: dist_reader" in r.stdout
 
 
 def test_rtamc2d_rejects_hdf5_material(pkg, tmp_path):
