@@ -119,6 +119,10 @@ __device__ inline void geo_stage(const BertParams& P, const DevBox*& boxes, cons
 // evolparticle (beyondRTA_MC2d.cpp:708-970): ballistic advection by dt
 // through box crossings; a void side adds the particle sign to that side's
 // Eborder and ends the particle. Returns false if the particle ended.
+// PER compiles in the periodic translation; without periodic boxes the
+// kernels use the instantiation without it, which needs about 50 fewer
+// registers per lane (4 resident waves per SIMD instead of 3).
+template <bool PER>
 __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const DevContact* cts, double& x,
                               double& y, int& box, int sign, double vx, double vy, double ivx, double ivy,
                               double dt, Pcg32& rng, int32_t* Eb) {
@@ -126,6 +130,10 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
         const DevBox B = boxes[box];
         if (B.kind == 1) return false;  // reservoir: deleted
         if (B.kind == 2) {
+            if (!PER) {  // the host launches PER whenever a box is periodic
+                atomicOr(P.flags + 3, 1u);
+                return false;
+            }
             // periodic box: translate and continue the flight (beyondRTA_MC2d.cpp:729-737)
             const int nb = translate_rect(boxes, cts, P.cseg, P.ptrans, box, x, y, vx, vy, rng);
             if (nb < 0) {
@@ -485,7 +493,7 @@ __device__ inline void box_lds_flush(const BertParams& P, int myseg) {
 
 // One workgroup per segment of the step's output. Workgroup w takes the
 // w-th equal slice of the live particle list (all input segments).
-template <bool GL>
+template <bool GL, bool PER>
 __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t step, int cur,
                                                         int cancel_prev, int hist) {
     __shared__ uint32_t tsum[256];
@@ -558,7 +566,7 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
                 if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
                 const double2 v = P.vel[mode];
                 const double2 iv = P.ivel[mode];
-                alive = advect(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
+                alive = advect<PER>(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
                 if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, box);
             }
         }
@@ -693,7 +701,7 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
 // in equal slices over the workgroups; each is placed uniformly on its side,
 // advected U*dt, scattered and histogrammed.
 #define BERT_EMIT_MAXB 4096
-template <bool GL>
+template <bool GL, bool PER>
 __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
     __shared__ uint32_t tsum[256];
     __shared__ uint32_t lcount;
@@ -755,7 +763,7 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
             if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
-            alive = advect(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
+            alive = advect<PER>(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
             if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, bx);
         }
         scatter_out(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
@@ -782,6 +790,7 @@ struct BertSource {
     int32_t part, nparts;  // this workgroup's share of the source's particles
     int32_t area, pad;
 };
+template <bool PER>
 __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src,
                                                           const uint2* stab, uint32_t step, int cur, int hist) {
     __shared__ uint32_t lcount;
@@ -837,7 +846,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
             if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
-            alive = advect(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
+            alive = advect<PER>(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
             if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, bx);
         }
         scatter_out(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
@@ -1013,6 +1022,7 @@ struct Bert {
     int nslots = 1;
     int nsrc = 0, ninit = 0;
     bool geo_lds = false;
+    bool periodic = false;  // some box is periodic: launch the PER kernels
     size_t lds_bytes = 0;
     int step_blocks = 0, emit_blocks = 0, nsegs = 0;
     std::vector<int32_t> hH;
@@ -1442,9 +1452,10 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         }
     }
     b->nsrc = (int)srcs.size();
-    // three step workgroups per CU: as many as its registers (166 VGPRs, 3
-    // waves per SIMD) keep resident, for latency hiding; two for emission
-    b->step_blocks = std::min(3 * c->cu_count, 2048);
+    // step workgroups per CU: as many as its registers keep resident (3 with
+    // the periodic translation compiled in, 4 without), for latency hiding
+    for (auto& bx : g->dboxes) b->periodic = b->periodic || bx.kind == 2;
+    b->step_blocks = std::min((b->periodic ? 3 : 4) * c->cu_count, 2048);
     b->emit_blocks = std::min(2 * c->cu_count, 1024);
     b->nsegs = b->step_blocks + b->emit_blocks + b->nsrc;
     if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
@@ -1663,23 +1674,22 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         // frame reads it; properties come from the box sums otherwise
         const int hist = cancel_now || cb != nullptr || !P.lds_props;
         HIPCHECK(hipEventRecord(b->kev[2 * istep], s));
-        if (b->geo_lds)
-            hipLaunchKernelGGL(bert_step_kernel<true>, dim3(step_blocks), dim3(256), b->lds_bytes, s, P,
-                               (uint32_t)istep, cur, cancel_prev, hist);
-        else
-            hipLaunchKernelGGL(bert_step_kernel<false>, dim3(step_blocks), dim3(256), b->lds_bytes, s, P,
-                               (uint32_t)istep, cur, cancel_prev, hist);
+        {
+            auto k = b->geo_lds ? (b->periodic ? bert_step_kernel<true, true> : bert_step_kernel<true, false>)
+                                : (b->periodic ? bert_step_kernel<false, true> : bert_step_kernel<false, false>);
+            hipLaunchKernelGGL(k, dim3(step_blocks), dim3(256), b->lds_bytes, s, P, (uint32_t)istep, cur, cancel_prev,
+                               hist);
+        }
         HIPCHECK(hipEventRecord(b->kev[2 * istep + 1], s));
         if (b->nsrc)
-            hipLaunchKernelGGL(bert_source_kernel, dim3(b->nsrc), dim3(256), b->lds_bytes, s, P, b->d_src.p,
-                               b->d_src_tab.p, (uint32_t)istep, cur, hist);
+            hipLaunchKernelGGL(b->periodic ? bert_source_kernel<true> : bert_source_kernel<false>, dim3(b->nsrc),
+                               dim3(256), b->lds_bytes, s, P, b->d_src.p, b->d_src_tab.p, (uint32_t)istep, cur, hist);
         hipLaunchKernelGGL(bert_wallcdf_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep);
-        if (b->geo_lds)
-            hipLaunchKernelGGL(bert_emit_kernel<true>, dim3(emit_blocks), dim3(256), b->lds_bytes, s, P,
-                               (uint32_t)istep, cur, hist);
-        else
-            hipLaunchKernelGGL(bert_emit_kernel<false>, dim3(emit_blocks), dim3(256), b->lds_bytes, s, P,
-                               (uint32_t)istep, cur, hist);
+        {
+            auto k = b->geo_lds ? (b->periodic ? bert_emit_kernel<true, true> : bert_emit_kernel<true, false>)
+                                : (b->periodic ? bert_emit_kernel<false, true> : bert_emit_kernel<false, false>);
+            hipLaunchKernelGGL(k, dim3(emit_blocks), dim3(256), b->lds_bytes, s, P, (uint32_t)istep, cur, hist);
+        }
         // H of this step lives in H[cur]; H[cur^1] (previous step) is cleared
         // after the step kernel used it for cancellation
         hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, istep + 1, cur, cancel_now,
