@@ -240,3 +240,43 @@ def test_rtamc2d_steady_csv(pkg, tmp_path):
     assert out.shape == (3, 10)
     assert (out[1, :8] > 0).all()  # heat flows down the imposed gradient
     assert np.abs(out[0, :8] - 300.0).max() < 1.0
+
+
+def test_reference_example_bar(pkg, tmp_path):
+    """The reference's examples/example_bar inputs (tests/golden/
+    example_bar_beam.xml is its beam.xml: 200 boxes of 50 x 2 nm, periodic
+    ghost boxes on both sides, a 302 K reservoir below and a 300 K one
+    above) through RTAMC2D with <spectral>, beRTAMC2D and dist_reader, with
+    a synthetic mode table in place of the HDF5 database and shorter runs."""
+    import shutil
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "example_bar_beam.xml")
+    shutil.copy(golden, tmp_path / "beam.xml")
+    mat = '<material name="black_P" modetable="synthetic" nmodes="400" seed="4" thickness="0.533" T0="300.0"'
+    (tmp_path / "rta.xml").write_text(
+        '<RTA_MC2d>\n<geometry file="beam.xml"/>\n<time dt="0.25" maxtime="100"/>\n' + mat +
+        ' sigma_bounce="2.0"/>\n<particles N="200000"/>\n<spectral>\n  <resolution ticks="50"/>\n'
+        '  <location bin="5"/>\n  <location bin="99"/>\n</spectral>\n</RTA_MC2d>\n')
+    run([pkg.app_path("RTAMC2D"), "rta.xml", "8"], tmp_path)
+    T = np.loadtxt(tmp_path / "temperature_300K_run_0.csv", delimiter=",")
+    jy = np.loadtxt(tmp_path / "jy_300K_run_0.csv", delimiter=",")
+    assert T.shape == (400, 1 + 204)
+    # heat enters from the 302 K reservoir (box 202) below: the lowest rows
+    # warm first and the flux points up
+    assert T[-1, 1:11].mean() > T[-1, 191:201].mean()
+    assert jy[-40:, 1:11].mean() > 0
+    sp = np.loadtxt(tmp_path / "deltaT_omega_5_300K_run_0.csv", delimiter=",")
+    assert sp.shape == (401, 51)
+    run([pkg.app_path("PropagatorBuilder"), "synthetic:400:4", "300", "0.25", "1", "20"], tmp_path)
+    (tmp_path / "berta.xml").write_text(
+        '<beRTAMC2D>\n<geometry file="beam.xml"/>\n<time dt="0.25" maxtime="10"/>\n' + mat + '>\n'
+        '  <propagator T="300" file="synthetic_400_4_300K_0.25ps.P.eigen.bin"/>\n</material>\n'
+        '<Eeff Ed="5.39836e-25"/>\n<spectral>\n  <resolution ticks="50"/>\n  <time t="2."/>\n  <time t="5."/>\n'
+        '  <location bin="5"/>\n  <location bin="6"/>\n</spectral>\n</beRTAMC2D>\n')
+    r = run([pkg.app_path("beRTAMC2D"), "berta.xml", "4"], tmp_path)
+    lines = [ln.split("\t") for ln in r.stdout.splitlines() if ln and not ln.startswith("#")]
+    rows = np.array([[float(x) for x in ln if x != ""] for ln in lines])
+    assert rows.shape[1] == 3 + 3 * 204 and rows.shape[0] >= 41
+    assert rows[-1, 2] > 0 and rows[-1, 3] > 0  # particles from the hot reservoir in box 0
+    run([pkg.app_path("dist_reader"), "berta.xml", "properties.msgpack.bin"], tmp_path)
+    d = np.loadtxt(tmp_path / "deltaT_omega_5.csv", delimiter=",")
+    assert d.shape == (3, 51) and d[1, 0] == 2.0 and d[2, 0] == 5.0
