@@ -97,12 +97,29 @@ def rta_extra(pkg, ctx):
     el = time.perf_counter() - t0
     r.close()
     T = out["T"][-1, :100]
+    # sigma(T) at the final time over R = 16 independent replicas (disjoint
+    # seeds), run as one batch: the particle count that meets a 2 mK target
+    R = 16
+    b = pkg.RtaBatch(ctx, modes, [dict(geom=geom, T0=c["T0"], nparticles=c["nparticles"], seed=100 + k)
+                                  for k in range(R)],
+                     thickness=c["thickness"], dt=c["dt"], maxtime=c["maxtime"], sigma_bounce=c["sigma_bounce"])
+    b.execute()
+    ctx.sync()
+    t0 = time.perf_counter()
+    res, rep = b.execute()
+    t_rep = time.perf_counter() - t0
+    b.close()
+    Tl = np.array([o["T"][-1, :100] for o in res])
+    sig = float((Tl.std(0, ddof=1) / np.sqrt(R)).max())
     return {"workload": "RTAMC2D steady state: 3000 modes, 1e6 particles, 1 um x 200 nm, +-0.5 K walls, 100 x-bins, "
                         "dt 25 ps, maxtime 5 ns",
             "events": int(out["events"]), "crossings": int(out["crossings"]), "seconds": el,
             "events_per_s": out["events"] / el, "kernel_ms": out["kernel_ms"],
             "kernel_events_per_s": out["events"] / (out["kernel_ms"] * 1e-3),
-            "T_left_right": [float(T[0]), float(T[-1])]}
+            "T_left_right": [float(T[0]), float(T[-1])],
+            "sigma_T": {"replicas": R, "particles_per_replica": c["nparticles"], "max_bin_sigma_K": sig,
+                        "target_K": 0.002, "met": sig <= 0.002, "seconds": t_rep,
+                        "events_per_s": rep["events"] / t_rep}}
 
 
 def batch_extra(pkg, ctx):
@@ -201,17 +218,29 @@ def krylov_extra(pkg, ctx, N=8192, m=30):
     for prec in ("fp64", "fp32", "bf16"):
         _, r0 = pkg.build_P(ctx, B, 5.0, m=2, precision=prec, arnoldi="textbook")
         first[prec] = r0["build_ms"]
+    # FP64 Pade exponential on the GPU as the exact reference
+    Ex, ex_ms = pkg.expm(ctx, B, 5.0)
+    res["pade_expm_fp64_ms"] = ex_ms
     P64, rep64 = pkg.build_P(ctx, B, 5.0, m=m, precision="fp64", arnoldi="textbook")
     res["fp64"] = {"build_ms": rep64["build_ms"], "gemm_ms": rep64["gemm_ms"], "gemm_tflops": rep64["gemm_tflops"],
-                   "col_sum_drift": rep64["max_col_sum_err"]}
+                   "col_sum_drift": rep64["max_col_sum_err"],
+                   "max_col_L1_vs_pade": float(np.abs(P64 - Ex).sum(0).max())}
+    Pr, repr_ = pkg.build_P(ctx, B, 5.0, m=m, precision="fp64", arnoldi="reference")
+    res["fp64_reference_arnoldi"] = {"build_ms": repr_["build_ms"],
+                                     "max_col_L1_vs_pade": float(np.abs(Pr - Ex).sum(0).max()),
+                                     "max_col_L1_vs_fp64": float(np.abs(Pr - P64).sum(0).max()),
+                                     "col_sum_drift": float(np.abs(Pr.sum(0) - 1).max())}
+    del Pr
     res["first_call_ms_m2"] = first
     for prec in ("fp32", "bf16"):
         P, rep = pkg.build_P(ctx, B, 5.0, m=m, precision=prec, arnoldi="textbook")
         res[prec] = {"build_ms": rep["build_ms"], "gemm_ms": rep["gemm_ms"], "gemm_tflops": rep["gemm_tflops"],
                      "max_col_L1_vs_fp64": float(np.abs(P - P64).sum(0).max()),
+                     "max_col_L1_vs_pade": float(np.abs(P - Ex).sum(0).max()),
                      "col_sum_drift_before_projection": rep["max_col_sum_err"],
                      "col_sum_drift_after": float(np.abs(P.sum(0) - 1).max())}
         del P
+    del Ex
     res["workload"] = f"PropagatorBuilder Krylov N={N}, m={m}, dt=5 ps, all columns as block-Krylov GEMMs"
     res["flops_2mN3"] = 2.0 * m * N ** 3
     return res

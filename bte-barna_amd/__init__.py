@@ -343,6 +343,17 @@ def build_P(ctx, B, dt, m=30, precision="fp64", arnoldi="textbook"):
                "max_col_sum_err": rep.max_col_sum_err, "breakdowns": rep.breakdowns}
 
 
+def expm(ctx, A, t=1.0):
+    """exp(t A) in FP64 on the GPU (Pade(13) scaling and squaring); returns
+    (E, device ms)"""
+    A = np.asfortranarray(A, dtype=np.float64)
+    E = np.zeros(A.shape, order="F")
+    ms = C.c_double(0.0)
+    check(lib().bte_expm(ctx.h, A.shape[0], A.ctypes.data_as(C.POINTER(C.c_double)), float(t),
+                         E.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ms)))
+    return E, ms.value
+
+
 def krylov_size(B, dt, eps=1e-8):
     """The reference's Krylov size rule for exp(dt*B) (host only)."""
     B = np.asfortranarray(B, dtype=np.float64)

@@ -107,6 +107,7 @@ SIGNATURES = {
     "bte_rta_spectral": (C.c_int, [vp, C.c_int, ip32, C.c_int, dp]),
     "bte_rta_spectral_get": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp]),
     "bte_spectra": (C.c_int, [vp, vp, dp, C.c_double, C.c_int, dp, C.c_int, dp, dp]),
+    "bte_expm": (C.c_int, [vp, C.c_int, dp, C.c_double, dp, dp]),
     "bte_rta_run": (C.c_int, [vp, vp, vp, C.POINTER(RtaCfg), dp, dp, dp, C.POINTER(RtaReport)]),
     "bte_rta_steady_run": (C.c_int, [vp, vp, vp, C.POINTER(RtaSteadyCfg), dp, dp, dp, C.POINTER(RtaSteadyReport)]),
     "bte_rta_batch_create": (C.c_int, [vp, vp, C.c_int, C.POINTER(RtaRunDesc), C.c_double, C.c_double,

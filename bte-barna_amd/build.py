@@ -43,7 +43,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     so = os.path.join(LIB, "libbte_amd.so")
-    _run([HIPCC, *CFLAGS, "-shared", "-o", so + ".tmp", *objs, "-lrocblas"])
+    _run([HIPCC, *CFLAGS, "-shared", "-o", so + ".tmp", *objs, "-lrocblas", "-lrocsolver"])
     os.replace(so + ".tmp", so)
     if os.path.isdir(APPS):
         for app in APP_SOURCES:

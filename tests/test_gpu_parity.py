@@ -105,6 +105,21 @@ def test_krylov_matches_oracle(pkg, ctx):
         assert np.abs(Pg.sum(0) - 1).max() < 1e-12
 
 
+def test_gpu_expm_matches_the_oracle(pkg, ctx):
+    """bte_expm (Pade(13) with GPU GEMMs and LU) against the oracle's
+    expm (pinned to scipy) on the synthetic operator, with squarings
+    (norm above theta_13) and without, and on a diagonal matrix"""
+    W = pkg.workloads
+    mat, B = W.config1_inputs(nmodes=200, seed=3)
+    for t in (0.1, 5.0):
+        E, ms = pkg.expm(ctx, B, t)
+        Eo = orc.expm(t * B)
+        assert np.abs(E - Eo).max() < 1e-12 * max(1.0, np.abs(Eo).max()), (t, np.abs(E - Eo).max())
+    d = np.linspace(-3.0, 2.0, 50)
+    E, _ = pkg.expm(ctx, np.diag(d), 1.0)
+    assert np.allclose(E, np.diag(np.exp(d)), rtol=1e-13, atol=1e-15)
+
+
 def test_bert_reservoir_sources_match_oracle(pkg, ctx):
     """beRTAMC2D driven by reservoirs (Isothermal_walls, beyondRTA_MC2d.cpp:
     446-585): thousands of particles per step and source, so each source is
