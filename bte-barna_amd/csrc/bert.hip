@@ -576,9 +576,8 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
     seg_close(P, segout, myseg, &lcount);
     eb_lds_flush(P, Eg, nb4);
     box_lds_flush(P, myseg);
-    // one atomic per wave for the particle-step counter
-    for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
-    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats, done);
+    // one atomic per workgroup for the particle-step counter
+    block_add_u64(P.stats, done);
 }
 
 // Emission sampling: first index i with cdf[i] >= u (the reference's
@@ -772,8 +771,7 @@ __global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t s
     seg_close(P, segout, myseg, &lcount);
     eb_lds_flush(P, Eg, nb4);
     box_lds_flush(P, myseg);
-    for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
-    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
+    block_add_u64(P.stats + 1, done);
 }
 
 // Fixed-rate sources, one or more workgroups each with a fixed alias table:
@@ -854,8 +852,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     }
     seg_close(P, segout, myseg, &lcount);
     box_lds_flush(P, myseg);
-    for (int d = 32; d > 0; d >>= 1) done += __shfl_down(done, d, 64);
-    if ((threadIdx.x & 63) == 0 && done) atomicAdd(P.stats + 1, done);
+    block_add_u64(P.stats + 1, done);
 }
 
 // calculate_prop (beyondRTA_MC2d.cpp:1535-1602) from the step's box sums

@@ -43,6 +43,28 @@ struct Pcg32 {
     }
 };
 
+// Hardware id of the XCD (accelerator complex die) the wave runs on
+// (HW_REG_XCC_ID, bits 3:0), for tallies kept as one copy per XCD.
+__device__ inline uint32_t xcd_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xfu; }
+
+// Adds v of every thread of the workgroup into *dst with one global atomic.
+// Counters that every wave bumped at its end cost the particle kernels tens
+// of microseconds per launch: thousands of same-address atomics queue at the
+// memory-side atomic unit after the last particle. blockDim.x is a multiple
+// of 64 (at most 1024) and every thread of the workgroup calls this.
+__device__ inline void block_add_u64(unsigned long long* dst, unsigned long long v) {
+    __shared__ unsigned long long part[16];
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w];
+        if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();  // part is reused by the next call
+}
+
 // splitmix64 finaliser: derives per-(particle, step) seeds
 __device__ __host__ inline uint64_t mix64(uint64_t z) {
     z += 0x9e3779b97f4a7c15ULL;

@@ -71,7 +71,6 @@ struct RtaParams {
 // per XCD they run about 25 % faster.
 #define RTA_NCOPY 8
 // the XCD (XCC) this wave runs on, from the hardware register HW_REG_XCC_ID
-__device__ inline uint32_t xcd_id() { return __builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xfu; }
 
 __device__ inline int time_bin(const double* grid, int nt, double target) {
     // get_bin_index (RTA_MC2D.cpp:140-148): lower_bound - 1 clamped to [0, nt-1]
@@ -341,11 +340,11 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
         dtrem = left;
         if (dtrem == 0.) need_flight = true;
     }
-    atomicAdd(P.counters + 0, events);
-    atomicAdd(P.counters + 1, crossings);
-    if (stuck) atomicAdd(P.counters + 3, stuck);
-    if (badmirror) atomicAdd(P.counters + 4, badmirror);
-    if (badtrans) atomicAdd(P.counters + 5, badtrans);
+    block_add_u64(P.counters + 0, events);
+    block_add_u64(P.counters + 1, crossings);
+    block_add_u64(P.counters + 3, stuck);
+    block_add_u64(P.counters + 4, badmirror);
+    block_add_u64(P.counters + 5, badtrans);
 }
 
 // clears the tallies and counters (a kernel rather than a memset so that the
@@ -1298,9 +1297,9 @@ __global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams 
             if (v != 0.) atomicAdd(i < 3 * P.nbox ? P.acc + i : P.phi + (i - 3 * P.nbox), v);
         }
     }
-    if (flights) atomicAdd(P.counters + 1, flights);
-    if (stuck) atomicAdd(P.counters + 2, stuck);
-    if (badtrans) atomicAdd(P.counters + 3, badtrans);
+    block_add_u64(P.counters + 1, flights);
+    block_add_u64(P.counters + 2, stuck);
+    block_add_u64(P.counters + 3, badtrans);
 }
 
 // re-emission of |phi(box)| particles per box (RTA_MC2D.cpp:2637-2654)

@@ -55,6 +55,17 @@ def test_propagator_builder_matches_oracle(pkg, tmp_path):
     assert np.abs(P.sum(0) - 1).max() < 1e-12
 
 
+def test_propagator_builder_pade(pkg, tmp_path):
+    """The "pade" method writes exp(dt B) to rounding: the oracle's Pade
+    exponential and, within the Krylov error, the oracle's Krylov build"""
+    run([pkg.app_path("PropagatorBuilder"), "synthetic:96:2", "300", "5", "1", "0", "pade"], tmp_path)
+    B = eigen_bin(tmp_path / "synthetic_96_2_300K_5ps.B.eigen.bin")
+    P = eigen_bin(tmp_path / "synthetic_96_2_300K_5ps.P.eigen.bin")
+    assert np.abs(P - orc.expm(5.0 * B)).max() < 1e-13
+    assert np.abs(P - orc.build_P(B, 5.0, 20, 0)).max() < 1e-10
+    assert np.abs(P.sum(0) - 1).max() < 1e-12
+
+
 def test_bertamc2d_outputs(pkg, ctx, tmp_path):
     msgpack = pytest.importorskip("msgpack")
     W = pkg.workloads
