@@ -326,11 +326,12 @@ __device__ inline int scatter(const BertParams& P, unsigned long long* H, double
 // through the prefix sum of their counts.
 #define BERT_MAXSEG 2048
 
-// inclusive prefix of the segment counts into pre[nsegs] (blockDim.x 256)
+// inclusive prefix of the segment counts into pre[nsegs]; tsum holds
+// blockDim.x entries (256 or 512 threads)
 __device__ inline uint32_t seg_prefix(const uint32_t* segc, int nsegs, uint32_t segcap, uint32_t* pre,
                                       uint32_t* tsum) {
-    const int t = threadIdx.x;
-    const int per = (nsegs + 255) / 256;
+    const int t = threadIdx.x, nt = blockDim.x;
+    const int per = (nsegs + nt - 1) / nt;
     const int i0 = t * per, i1 = min(nsegs, i0 + per);
     uint32_t acc = 0;
     for (int i = i0; i < i1; i++) {
@@ -339,7 +340,7 @@ __device__ inline uint32_t seg_prefix(const uint32_t* segc, int nsegs, uint32_t 
     }
     tsum[t] = acc;
     __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
+    for (int d = 1; d < nt; d <<= 1) {
         uint32_t v = t >= d ? tsum[t - d] : 0u;
         __syncthreads();
         tsum[t] += v;
@@ -348,7 +349,7 @@ __device__ inline uint32_t seg_prefix(const uint32_t* segc, int nsegs, uint32_t 
     const uint32_t off = t > 0 ? tsum[t - 1] : 0u;
     for (int i = i0; i < i1; i++) pre[i] += off;
     __syncthreads();
-    return tsum[255];
+    return tsum[nt - 1];
 }
 // first segment s with pre[s] > i
 __device__ inline int seg_search(const uint32_t* pre, int nsegs, uint32_t i) {
@@ -698,11 +699,15 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
 
 // Phase 2: the |Eborder| particles of every side form one index space split
 // in equal slices over the workgroups; each is placed uniformly on its side,
-// advected U*dt, scattered and histogrammed.
+// advected U*dt, scattered and histogrammed. 512-thread workgroups (two per
+// CU) keep four waves per SIMD resident with one output segment per
+// workgroup: 256-thread ones left two (and more segments slow the step
+// kernel's segment walk).
 #define BERT_EMIT_MAXB 4096
+#define BERT_EMIT_THREADS 512
 template <bool GL, bool PER>
-__global__ __launch_bounds__(256) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
-    __shared__ uint32_t tsum[256];
+__global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
+    __shared__ uint32_t tsum[BERT_EMIT_THREADS];
     __shared__ uint32_t lcount;
     uint32_t* pre = (uint32_t*)(bert_dyn + P.lds_pre);
     const int nb4 = P.nbox * 4;
@@ -1685,7 +1690,8 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         {
             auto k = b->geo_lds ? (b->periodic ? bert_emit_kernel<true, true> : bert_emit_kernel<true, false>)
                                 : (b->periodic ? bert_emit_kernel<false, true> : bert_emit_kernel<false, false>);
-            hipLaunchKernelGGL(k, dim3(emit_blocks), dim3(256), b->lds_bytes, s, P, (uint32_t)istep, cur, hist);
+            hipLaunchKernelGGL(k, dim3(emit_blocks), dim3(BERT_EMIT_THREADS), b->lds_bytes, s, P, (uint32_t)istep, cur,
+                               hist);
         }
         // H of this step lives in H[cur]; H[cur^1] (previous step) is cleared
         // after the step kernel used it for cancellation
