@@ -307,10 +307,11 @@ int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt, int m, int
  * 2 |dt B|_F^m e^{|dt B|_F} / Gamma(m) / N^2 < eps. Host only. */
 int bte_krylov_size(int N, const double* B, double dt, double eps, int* m);
 
-/* FP64 reference exponential E = exp(t A) (N x N, column-major) for
- * checking propagators: Pade(13) scaling and squaring (Higham 2005; the
- * expm that validates build_P, collision_operator.cpp:1015-1064) with FP64
- * GEMMs and an LU solve on the GPU. ms (may be NULL): device time. */
+/* FP64 exponential E = exp(t A) of the whole matrix (N x N, column-major):
+ * Pade(13) scaling and squaring (Higham 2005) with FP64 GEMMs and an LU
+ * solve on the GPU. No counterpart in the reference: it checks propagators
+ * and replaces build_P (collision_operator.cpp:1015-1064) when every column
+ * is wanted. ms (may be NULL): device time. */
 int bte_expm(bte_ctx* ctx, int N, const double* A, double t, double* E, double* ms);
 
 #ifdef __cplusplus
