@@ -860,6 +860,15 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     block_add_u64(P.stats + 1, done);
 }
 
+// The i-th of n workgroups, dispatched round-robin over 8 XCDs, takes item
+// start(x) + i / 8 on XCD x = i % 8: each XCD gets a contiguous range.
+__device__ inline int xcd_major_index(int i, int n) {
+    const int x = i & 7, j = i >> 3;
+    int start = 0;
+    for (int y = 0; y < x; y++) start += (n - y + 7) >> 3;
+    return start + j;
+}
+
 // calculate_prop (beyondRTA_MC2d.cpp:1535-1602) from the step's box sums
 // (from_boxes) or its histogram, the post-cancellation particle count, and
 // buffer recycling for the next step (clear_old: the previous step wrote its
@@ -867,7 +876,11 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
 __global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, int cur, int cancel_now,
                                                          int clear_tickets, int ebn, int from_boxes,
                                                          int clear_old) {
-    const int box = blockIdx.x;
+    // Workgroups go to the 8 XCDs round-robin; consecutive boxes are given
+    // to workgroups on the same XCD, so the cache lines of pbox rows (a
+    // 128-B line holds the sums of five neighbouring boxes) are fetched
+    // once into that XCD's L2 instead of once per box.
+    const int box = xcd_major_index((int)blockIdx.x, (int)gridDim.x);
     const unsigned long long* H = P.H + (size_t)cur * P.nbox * P.n + (size_t)box * P.n;
     unsigned long long* Hold = P.H + (size_t)(cur ^ 1) * P.nbox * P.n + (size_t)box * P.n;
     __shared__ double red[3][256];
