@@ -320,7 +320,8 @@ class BertRun(_Handle):
                 "Eeff": rep.Eeff,
                 "particle_steps": rep.particle_steps, "step_kernel_particle_steps": rep.step_kernel_particle_steps,
                 "max_particles": rep.max_particles, "loop_ms": rep.loop_ms,
-                "step_kernel_ms": rep.step_kernel_ms, "step_kernel_launches": rep.step_kernel_launches}
+                "step_kernel_ms": rep.step_kernel_ms, "step_kernel_launches": rep.step_kernel_launches,
+                "step_kernel_timed": rep.step_kernel_timed}
 
 
 def bertamc2d(ctx, modes, geom, prop, **kw):

@@ -65,7 +65,8 @@ class BertCfg(C.Structure):
 class BertReport(C.Structure):
     _fields_ = [("nsteps", C.c_int), ("Eeff", C.c_double), ("particle_steps", C.c_uint64),
                 ("step_kernel_particle_steps", C.c_uint64), ("max_particles", C.c_int64),
-                ("loop_ms", C.c_double), ("step_kernel_ms", C.c_double), ("step_kernel_launches", C.c_int)]
+                ("loop_ms", C.c_double), ("step_kernel_ms", C.c_double), ("step_kernel_launches", C.c_int),
+                ("step_kernel_timed", C.c_int)]
 
 
 class KrylovReport(C.Structure):

@@ -262,8 +262,12 @@ typedef struct {
     uint64_t step_kernel_particle_steps; /* the part done by the main step kernel */
     int64_t max_particles;
     double loop_ms;           /* wall time of the step loop (device-synchronised) */
-    double step_kernel_ms;    /* device time of the main step kernel, summed */
+    double step_kernel_ms;    /* device time of the timed step-kernel launches, summed */
     int step_kernel_launches;
+    /* launches bracketed by events: one in pcancel_each, rotating through
+     * the cancellation cycle so every step type is sampled equally (an
+     * event pair costs about 11 us of idle device time per step) */
+    int step_kernel_timed;
 } bte_bert_report;
 
 /* Frame callback: called after each step with the (box, mode) histogram
