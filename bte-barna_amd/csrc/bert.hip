@@ -326,8 +326,10 @@ __device__ inline int scatter(const BertParams& P, unsigned long long* H, double
 // through the prefix sum of their counts.
 #define BERT_MAXSEG 2048
 
-// inclusive prefix of the segment counts into pre[nsegs]; tsum holds
-// blockDim.x entries (256 or 512 threads)
+// inclusive prefix of the segment counts into pre[nsegs] (blockDim.x a
+// multiple of 64, at most 1024; tsum holds one entry per wave): each thread
+// sums a run of segments, shuffles scan the run totals within a wave and
+// the wave totals go through LDS (two barriers)
 __device__ inline uint32_t seg_prefix(const uint32_t* segc, int nsegs, uint32_t segcap, uint32_t* pre,
                                       uint32_t* tsum) {
     const int t = threadIdx.x, nt = blockDim.x;
@@ -338,18 +340,23 @@ __device__ inline uint32_t seg_prefix(const uint32_t* segc, int nsegs, uint32_t 
         acc += min(segc[i], segcap);
         pre[i] = acc;
     }
-    tsum[t] = acc;
-    __syncthreads();
-    for (int d = 1; d < nt; d <<= 1) {
-        uint32_t v = t >= d ? tsum[t - d] : 0u;
-        __syncthreads();
-        tsum[t] += v;
-        __syncthreads();
+    const int lane = t & 63, w = t >> 6;
+    uint32_t v = acc;
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = __shfl_up(v, d, 64);
+        if (lane >= d) v += u;
     }
-    const uint32_t off = t > 0 ? tsum[t - 1] : 0u;
+    if (lane == 63) tsum[w] = v;
+    __syncthreads();
+    uint32_t off = v - acc, total = 0;
+    for (int k = 0; k < (nt >> 6); k++) {
+        const uint32_t x = tsum[k];
+        if (k < w) off += x;
+        total += x;
+    }
     for (int i = i0; i < i1; i++) pre[i] += off;
     __syncthreads();
-    return tsum[nt - 1];
+    return total;
 }
 // first segment s with pre[s] > i
 __device__ inline int seg_search(const uint32_t* pre, int nsegs, uint32_t i) {
@@ -497,7 +504,7 @@ __device__ inline void box_lds_flush(const BertParams& P, int myseg) {
 template <bool GL, bool PER>
 __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t step, int cur,
                                                         int cancel_prev, int hist) {
-    __shared__ uint32_t tsum[256];
+    __shared__ uint32_t tsum[16];
     __shared__ uint32_t lcount;
     uint32_t* pre = (uint32_t*)(bert_dyn + P.lds_pre);
     const DevBox* boxes;
@@ -707,7 +714,7 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
 #define BERT_EMIT_THREADS 512
 template <bool GL, bool PER>
 __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
-    __shared__ uint32_t tsum[BERT_EMIT_THREADS];
+    __shared__ uint32_t tsum[16];
     __shared__ uint32_t lcount;
     uint32_t* pre = (uint32_t*)(bert_dyn + P.lds_pre);
     const int nb4 = P.nbox * 4;
@@ -967,7 +974,7 @@ __global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, 
 // initial particles: histogram into H[cur] (properties row 0)
 __global__ __launch_bounds__(256) void bert_hist_kernel(BertParams P, int cur) {
     __shared__ uint32_t pre[BERT_MAXSEG];
-    __shared__ uint32_t tsum[256];
+    __shared__ uint32_t tsum[16];
     const uint32_t nin = seg_prefix(P.seg + (size_t)cur * P.nsegs, P.nsegs, P.segcap, pre, tsum);
     unsigned long long* H = P.H + (size_t)cur * P.nbox * P.n;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nin;
