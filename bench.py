@@ -164,8 +164,10 @@ def batch_extra(pkg, ctx):
 
 def grad_extra(pkg, ctx, fd_dp=0.01):
     """Differentiable transient RTAMC2D: T(x, t) and dT/dp_top, dT/dp_bottom
-    from one run with score-function tallies, against the plain run and a
-    central-difference estimate with common random numbers."""
+    from one run with score-function tallies, against plain runs and a
+    central-difference estimate with common random numbers. Plain and
+    gradient runs are timed with both tally modes: FP64 device atomics, and
+    FP32 per-workgroup partial sums in LDS with an FP64 final sum."""
     W = pkg.workloads
     c = W.config4()
     m = c["material"]
@@ -174,17 +176,17 @@ def grad_extra(pkg, ctx, fd_dp=0.01):
     kw = dict(thickness=c["thickness"], dt=c["dt"], maxtime=c["maxtime"], sigma_bounce=c["sigma_bounce"])
     base = dict(geom=geom, T0=c["T0"], nparticles=c["nparticles"], seed=c["seed"])
     p = np.array(c["specularity"])
-    g = pkg.RtaBatch(ctx, modes, [dict(base, specularity=tuple(p), grad=True)], **kw)
-    g.execute(grad=True)  # warm-up
-    t0 = time.perf_counter()
-    (go,), grep = g.execute(grad=True)
-    t_grad = time.perf_counter() - t0
-    g.close()
-    pl = pkg.RtaBatch(ctx, modes, [dict(base, specularity=tuple(p))], **kw)
-    t0 = time.perf_counter()
-    pl.execute()
-    t_plain = time.perf_counter() - t0
-    pl.close()
+    times, outs, events = {}, {}, 0
+    for tally in ("fp64", "fp32-lds"):
+        for grad in (False, True):
+            b = pkg.RtaBatch(ctx, modes, [dict(base, specularity=tuple(p), grad=grad)], tally=tally, **kw)
+            b.execute(grad=grad)  # warm-up
+            t0 = time.perf_counter()
+            (o,), rep = b.execute(grad=grad)
+            times[(tally, grad)] = time.perf_counter() - t0
+            outs[(tally, grad)] = o
+            events = int(rep["events"])
+            b.close()
     fd_runs = []
     for side in (1, 2):
         for sgn in (+1, -1):
@@ -194,17 +196,27 @@ def grad_extra(pkg, ctx, fd_dp=0.01):
     f = pkg.RtaBatch(ctx, modes, fd_runs, **kw)
     fo, _ = f.execute()
     f.close()
+    go = outs[("fp32-lds", True)]
     obs = lambda T: float(T[-1, :64].mean())  # left half, last time
     out = {}
     for i, (side, name) in enumerate(((2, "top"), (1, "bottom"))):
         k = 0 if side == 1 else 2
         fd = (obs(fo[k]["T"]) - obs(fo[k + 1]["T"])) / (2 * fd_dp)
         out[name] = {"score_dTdp_left_half_K": obs(go["dTdp"][side]), "central_fd_K": fd}
+    g64 = outs[("fp64", True)]
+    t_plain = min(times[("fp64", False)], times[("fp32-lds", False)])
+    t_grad = min(times[("fp64", True)], times[("fp32-lds", True)])
     return {"workload": "1 um x 100 nm, 128 bins x 64 time bins, 5000 mirror-pair modes, 1e7 particles, "
                         "p_top 0.3, p_bottom 0.7, walls 300 +- 0.5 K",
-            "grad_run_s": t_grad, "plain_run_s": t_plain, "events": int(grep["events"]),
+            "runs_s": {f"{t}_{'grad' if g else 'plain'}": v for (t, g), v in times.items()},
+            "grad_run_s": t_grad, "plain_run_s": t_plain, "events": events,
             "useful_run_speedup": 5 * t_plain / t_grad,
-            "note": "one gradient run replaces the base run and four central-difference runs",
+            "useful_run_speedup_vs_fp64_plain": 5 * times[("fp64", False)] / t_grad,
+            "fp32_vs_fp64_tallies": {
+                "max_abs_T_diff_K": float(np.abs(go["T"] - g64["T"]).max()),
+                "max_abs_dTdp_diff_K": float(np.abs(go["dTdp"] - g64["dTdp"]).max())},
+            "note": "one gradient run replaces the base run and four central-difference runs; "
+                    "speedups use the faster tally mode of each form",
             "derivatives": out, "fd_dp": fd_dp}
 
 

@@ -231,10 +231,15 @@ class RtaBatch(_Handle):
 
     runs: list of dicts with geom, T0, nparticles, seed and optionally
     ballistic, wscale (scattering-rate factor), specularity (4 sides: left,
-    bottom, top, right) and grad (dT/dp for sides with 0 < p < 1)."""
+    bottom, top, right) and grad (dT/dp for sides with 0 < p < 1).
+
+    tally: "fp64" (device atomics, the default) or "fp32-lds" (per-workgroup
+    FP32 partial sums in LDS added into the FP64 tallies at the end; raises
+    when the regular boxes' cells do not fit in LDS)."""
     _destroy = "bte_rta_batch_destroy"
 
-    def __init__(self, ctx, modes, runs, thickness, dt, maxtime, sigma_bounce=0.25, use_graph=False):
+    def __init__(self, ctx, modes, runs, thickness, dt, maxtime, sigma_bounce=0.25, use_graph=False,
+                 tally="fp64"):
         super().__init__()
         self.nruns = len(runs)
         self.descs = (RtaRunDesc * self.nruns)()
@@ -254,6 +259,12 @@ class RtaBatch(_Handle):
         self._keep = (ctx, modes, [r["geom"] for r in runs])
         check(lib().bte_rta_batch_create(ctx.h, modes.h, self.nruns, self.descs, thickness, dt, maxtime,
                                          sigma_bounce, int(use_graph), C.byref(self.h)))
+        modes_ = {"fp64": 0, "fp32-lds": 1}
+        if tally not in modes_:
+            raise ValueError(f"tally must be one of {sorted(modes_)}")
+        if tally != "fp64":
+            check(lib().bte_rta_batch_set_tally(self.h, modes_[tally]))
+        self.tally = tally
 
     def tune(self, fraction=0.05):
         blk, nb = C.c_int(0), C.c_int(0)

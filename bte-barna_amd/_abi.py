@@ -113,6 +113,7 @@ SIGNATURES = {
     "bte_rta_steady_run": (C.c_int, [vp, vp, vp, C.POINTER(RtaSteadyCfg), dp, dp, dp, C.POINTER(RtaSteadyReport)]),
     "bte_rta_batch_create": (C.c_int, [vp, vp, C.c_int, C.POINTER(RtaRunDesc), C.c_double, C.c_double,
                                        C.c_double, C.c_double, C.c_int, pvp]),
+    "bte_rta_batch_set_tally": (C.c_int, [vp, C.c_int]),
     "bte_rta_batch_tune": (C.c_int, [vp, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "bte_rta_batch_execute": (C.c_int, [vp, dp, dp, dp, dp, dp, C.POINTER(RtaBatchReport)]),
     "bte_rta_batch_destroy": (C.c_int, [vp]),

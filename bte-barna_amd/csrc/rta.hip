@@ -64,6 +64,15 @@ struct RtaParams {
     unsigned long long max_iters;
     double maxtime;
     int32_t n, nt, nruns, pad;
+    // FP32 tallies in LDS (LT kernels): the LDS row (nt cells) of each box,
+    // ~0u for boxes that never tally; the global cell of each LDS row; per
+    // side the float offset of its score-weighted array, -1 for none. The
+    // LDS block holds 3 arrays (gz, jx, jy) of lrows * nt floats, then the
+    // gradient arrays.
+    const uint32_t* box_lrow;
+    const uint64_t* lrow_cell;
+    int32_t lrows, pad1;
+    int32_t lt_goff[4];
 };
 
 // Per-XCD copies of the f64 tallies. Device-wide f64 atomics to a few
@@ -84,12 +93,34 @@ __device__ inline int time_bin(const double* grid, int nt, double target) {
 }
 
 // process_segment (RTA_MC2D.cpp:699-769); gradient runs also add the
-// segment weighted by the particle's running score for each active side
-template <bool GRAD>
-__device__ inline void tally_segment(const RtaParams& P, int box, double t0, double t1, int sign, double vx,
-                                     double vy, int gmask, const float* score, int mode) {
+// segment weighted by the particle's running score for each active side.
+// LT: the workgroup's FP32 partial sums in LDS (lt), added into the FP64
+// tallies once at the end of the kernel (lt_flush).
+template <bool GRAD, bool LT>
+__device__ inline void tally_segment(const RtaParams& P, float* lt, int box, double t0, double t1, int sign,
+                                     double vx, double vy, int gmask, const float* score, int mode) {
     int b0 = time_bin(P.timegrid, P.nt, t0), b1 = time_bin(P.timegrid, P.nt, t1);
     double delta = t1 - t0;
+    if constexpr (LT) {
+        const uint32_t lc = (uint32_t)P.lrows * (uint32_t)P.nt;
+        float* L = lt + P.box_lrow[box] * (uint32_t)P.nt;
+        for (int i = b0; i <= b1; i++) {
+            double lo = fmax(P.timegrid[i] - t0, 0.);
+            double hi = fmin(P.timegrid[i + 1] - t0, delta);
+            double c = sign * fmax(hi - lo, 0.);
+            if (c == 0.) continue;
+            atomicAdd(L + i, (float)c);
+            atomicAdd(L + lc + i, (float)(vx * c));
+            atomicAdd(L + 2 * lc + i, (float)(vy * c));
+            if (GRAD) {
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    if (P.lt_goff[s] >= 0 && ((gmask >> s) & 1) && score[s] != 0.f)
+                        atomicAdd(L + P.lt_goff[s] + i, (float)(c * (double)score[s]));
+            }
+        }
+        return;
+    }
     // each XCD adds into its own copy of the tallies (RTA_NCOPY copies,
     // summed after the kernel): the copy is picked by the hardware XCD id,
     // so no two XCDs ever update one copy
@@ -135,8 +166,62 @@ __device__ inline int find_run(const RtaParams& P, uint64_t pid) {
 // One lane follows one particle history at a time and takes the next id
 // from a per-wave pool when it ends; ids of all runs form one range, so a
 // batch of runs of any sizes fills the GPU in one launch.
+// The workgroup's LDS partial sums into the FP64 tallies (the XCD's copy).
 template <bool GRAD>
-__global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
+__device__ inline void lt_flush(const RtaParams& P, const float* lt) {
+    const uint32_t nt = (uint32_t)P.nt, lc = (uint32_t)P.lrows * nt;
+    const uint64_t xc = xcd_id() % RTA_NCOPY;
+    double* tz = P.gz + xc * 3 * P.cells;
+    for (uint32_t j = threadIdx.x; j < lc; j += blockDim.x) {
+        const uint32_t row = j / nt;
+        const uint64_t g = P.lrow_cell[row] + (j - row * nt);
+        for (int a = 0; a < 3; a++) {
+            const float v = lt[(size_t)a * lc + j];
+            if (v != 0.f) atomicAdd(tz + (uint64_t)a * P.cells + g, (double)v);
+        }
+        if (GRAD) {
+            for (int s = 0; s < 4; s++) {
+                if (P.lt_goff[s] < 0) continue;
+                const float v = lt[(size_t)P.lt_goff[s] + j];
+                if (v != 0.f) atomicAdd(P.dgz + xc * 4 * P.cells + (uint64_t)s * P.cells + g, (double)v);
+            }
+        }
+    }
+}
+
+// The LT kernels give the whole LDS to the tallies (up to 160 KB), so their
+// counters are summed through that block after the flush instead of
+// block_add_u64's static array.
+extern __shared__ __align__(16) float rta_lt[];
+__device__ inline void lt_counters(unsigned long long* cnt, unsigned long long v0, unsigned long long v1,
+                                   unsigned long long v3, unsigned long long v4, unsigned long long v5) {
+    unsigned long long* part = (unsigned long long*)rta_lt;
+    unsigned long long v[5] = {v0, v1, v3, v4, v5};
+    for (int k = 0; k < 5; k++)
+        for (int d = 32; d > 0; d >>= 1) v[k] += __shfl_down(v[k], d, 64);
+    __syncthreads();  // the flush has read every partial sum
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; k++) part[(threadIdx.x >> 6) * 5 + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        unsigned long long t = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += part[w * 5 + threadIdx.x];
+        const int dst[5] = {0, 1, 3, 4, 5};
+        if (t) atomicAdd(cnt + dst[threadIdx.x], t);
+    }
+}
+
+template <bool GRAD, bool LT>
+__global__ __launch_bounds__(LT ? 768 : 512) void rta_history_kernel(RtaParams P) {
+    float* lt = nullptr;
+    if constexpr (LT) {
+        lt = rta_lt;
+        int narr = 3;
+        for (int s = 0; s < 4; s++) narr += P.lt_goff[s] >= 0 ? 1 : 0;
+        const uint32_t nf = (uint32_t)narr * (uint32_t)P.lrows * (uint32_t)P.nt;
+        for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) lt[j] = 0.f;
+        __syncthreads();
+    }
     Pcg32 rng;
     double x = 0, y = 0, t = 0, dtrem = 0, wscale = 1.0;
     int mode = 0, box = 0, sign = 1, run = 0;
@@ -243,7 +328,7 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
         const double vx = md.x, vy = md.y;
         double nx = x + dtrem * vx, ny = y + dtrem * vy;
         if (box_inside(B, nx, ny)) {
-            tally_segment<GRAD>(P, box, t, t + dtrem, sign, vx, vy, gmask, score, mode);
+            tally_segment<GRAD, LT>(P, lt, box, t, t + dtrem, sign, vx, vy, gmask, score, mode);
             x = nx;
             y = ny;
             t += dtrem;
@@ -261,7 +346,7 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
         double tf = box_exit(B, x, y, vx, vy, iv.x, iv.y, mask, ex, ey);
         double left = dtrem - tf;
         if (almost_equal(left, 0.)) left = 0.;
-        tally_segment<GRAD>(P, box, t, t + tf, sign, vx, vy, gmask, score, mode);
+        tally_segment<GRAD, LT>(P, lt, box, t, t + tf, sign, vx, vy, gmask, score, mode);
         t += tf;
         int side = 0;
         const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
@@ -340,11 +425,17 @@ __global__ __launch_bounds__(512) void rta_history_kernel(RtaParams P) {
         dtrem = left;
         if (dtrem == 0.) need_flight = true;
     }
-    block_add_u64(P.counters + 0, events);
-    block_add_u64(P.counters + 1, crossings);
-    block_add_u64(P.counters + 3, stuck);
-    block_add_u64(P.counters + 4, badmirror);
-    block_add_u64(P.counters + 5, badtrans);
+    if constexpr (LT) {
+        __syncthreads();  // every history of the workgroup has ended
+        lt_flush<GRAD>(P, lt);
+        lt_counters(P.counters, events, crossings, stuck, badmirror, badtrans);
+    } else {
+        block_add_u64(P.counters + 0, events);
+        block_add_u64(P.counters + 1, crossings);
+        block_add_u64(P.counters + 3, stuck);
+        block_add_u64(P.counters + 4, badmirror);
+        block_add_u64(P.counters + 5, badtrans);
+    }
 }
 
 // clears the tallies and counters (a kernel rather than a memset so that the
@@ -413,6 +504,13 @@ struct RtaBatch {
     DevBuf<int32_t> d_box_loc;
     DevBuf<double> d_spec, d_specw, d_specout;
     int block = 256, blocks = 0;
+    // tally: 0 FP64 device atomics into per-XCD copies; 1 FP32 per-workgroup
+    // partial sums in LDS (one 768-thread workgroup per CU), added in FP64
+    int tally = 0, lt_goff[4] = {-1, -1, -1, -1};
+    uint32_t lrows = 0;
+    size_t lt_bytes = 0;
+    DevBuf<uint32_t> d_box_lrow;
+    DevBuf<uint64_t> d_lrow_cell;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -537,6 +635,10 @@ static RtaParams batch_params(RtaBatch* b, const RtaRunDev* runs, uint64_t total
     P.n = b->m->n;
     P.nt = b->ntime - 1;
     P.nruns = b->nruns;
+    P.box_lrow = b->d_box_lrow.p;
+    P.lrow_cell = b->d_lrow_cell.p;
+    P.lrows = (int32_t)b->lrows;
+    for (int k = 0; k < 4; k++) P.lt_goff[k] = b->lt_goff[k];
     return P;
 }
 
@@ -549,10 +651,16 @@ static void launch_batch(RtaBatch* b, const RtaParams& P, int block, int blocks,
     if (b->nloc)
         hipLaunchKernelGGL(rta_clear_kernel, dim3(1024), dim3(256), 0, s, b->d_spec.p,
                            (uint64_t)b->nloc * (b->ntime - 1) * b->m->n, b->d_counters.p);
-    if (b->any_grad)
-        hipLaunchKernelGGL(rta_history_kernel<true>, dim3(blocks), dim3(block), 0, s, P);
-    else
-        hipLaunchKernelGGL(rta_history_kernel<false>, dim3(blocks), dim3(block), 0, s, P);
+    if (b->tally == 1) {
+        if (b->any_grad)
+            hipLaunchKernelGGL((rta_history_kernel<true, true>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
+        else
+            hipLaunchKernelGGL((rta_history_kernel<false, true>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
+    } else if (b->any_grad) {
+        hipLaunchKernelGGL((rta_history_kernel<true, false>), dim3(blocks), dim3(block), 0, s, P);
+    } else {
+        hipLaunchKernelGGL((rta_history_kernel<false, false>), dim3(blocks), dim3(block), 0, s, P);
+    }
     // the per-XCD copies into copy 0
     hipLaunchKernelGGL(rta_sum_copies_kernel, dim3(1024), dim3(256), 0, s, b->d_tally.p, 3 * b->cells);
     if (b->any_grad)
@@ -589,7 +697,8 @@ static void batch_create(RtaBatch* b) {
     std::vector<DevContact> cts;
     std::vector<double4> cseg;
     std::vector<double2> ptrans;
-    std::vector<uint64_t> box_cell;
+    std::vector<uint64_t> box_cell, lrow_cell;
+    std::vector<uint32_t> box_lrow;
     std::vector<RtaGen> gens;
     std::vector<uint2> gsel, gtab, scat;
     std::vector<double> pmf(n);
@@ -637,6 +746,9 @@ static void batch_create(RtaBatch* b) {
                 if (d.side_nbr[k] >= 0) d.side_nbr[k] += bo;
             boxes.push_back(d);
             box_cell.push_back(b->cells + (uint64_t)i * nt);
+            // only regular boxes tally (reservoirs absorb, periodic boxes translate)
+            box_lrow.push_back(d.kind == 0 ? b->lrows++ : ~0u);
+            if (d.kind == 0) lrow_cell.push_back(b->cells + (uint64_t)i * nt);
         }
         for (auto ct : g->dcontacts) {
             ct.box += bo;
@@ -686,6 +798,16 @@ static void batch_create(RtaBatch* b) {
     b->d_cseg.upload(cseg.empty() ? std::vector<double4>(1) : cseg, s);
     b->d_ptrans.upload(ptrans, s);
     b->d_box_cell.upload(box_cell, s);
+    b->d_box_lrow.upload(box_lrow, s);
+    b->d_lrow_cell.upload(lrow_cell.empty() ? std::vector<uint64_t>(1) : lrow_cell, s);
+    // LDS block of the FP32 tallies: gz, jx, jy, then one array per side that
+    // some run differentiates; at least the counters' reduction space
+    int gunion = 0;
+    for (auto& R : b->runs) gunion |= R.gmask;
+    int narr = 3;
+    for (int k = 0; k < 4; k++)
+        if ((gunion >> k) & 1) b->lt_goff[k] = narr++ * (int)(b->lrows * (uint32_t)nt);
+    b->lt_bytes = std::max<size_t>((size_t)narr * b->lrows * nt * sizeof(float), 16 * 5 * sizeof(uint64_t));
     b->d_gens.upload(gens, s);
     b->d_gen_sel.upload(gsel, s);
     b->d_gen_tab.upload(gtab, s);
@@ -819,7 +941,34 @@ static void batch_execute(RtaBatch* b, double* T_out, double* jx_out, double* jy
 // Launch-shape autotuner: times the batch with every run scaled to a
 // fraction of its particles for each candidate (block size, workgroups per
 // CU) and keeps the fastest.
+// FP32 LDS tallies on (mode 1) or off (mode 0); see bte_rta_batch_set_tally.
+static void batch_set_tally(RtaBatch* b, int mode) {
+    if (mode != 0 && mode != 1) throw Error(BTE_EINVAL, "tally mode must be 0 (FP64) or 1 (FP32 LDS)");
+    if (mode == 1) {
+        if (b->nloc) throw Error(BTE_EINVAL, "FP32 LDS tallies do not cover spectral locations");
+        if (b->lt_bytes > b->ctx->lds_per_block)
+            throw Error(BTE_EINVAL, "FP32 LDS tallies need " + std::to_string(b->lt_bytes) + " bytes of LDS, the device has " +
+                                        std::to_string(b->ctx->lds_per_block) + " per workgroup");
+        const void* k = b->any_grad ? reinterpret_cast<const void*>(&rta_history_kernel<true, true>)
+                                    : reinterpret_cast<const void*>(&rta_history_kernel<false, true>);
+        HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lt_bytes));
+        b->block = 768;
+        b->blocks = b->ctx->cu_count;
+    } else if (b->tally == 1) {
+        b->block = 256;
+        b->blocks = b->ctx->cu_count * 8;
+    }
+    if (mode != b->tally && b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        (void)hipGraphDestroy(b->graph);
+        b->gexec = nullptr;
+        b->graph = nullptr;
+    }
+    b->tally = mode;
+}
+
 static void batch_tune(RtaBatch* b, double fraction) {
+    if (b->tally == 1) return;  // one 768-thread workgroup per CU: the LDS holds one tally block
     Ctx* c = b->ctx;
     hipStream_t s = c->stream;
     std::vector<RtaRunDev> tr = b->runs;
@@ -888,6 +1037,15 @@ int bte_rta_batch_create(bte_ctx* ctx, const bte_modes* mm, int nruns, const bte
     b->use_graph = use_graph != 0;
     batch_create(b);
     *out = reinterpret_cast<bte_rta_batch*>(guard.release());
+    BTE_GUARD_END
+}
+
+int bte_rta_batch_set_tally(bte_rta_batch* bb, int mode) {
+    BTE_GUARD_BEGIN
+    auto* b = reinterpret_cast<RtaBatch*>(bb);
+    if (!b) throw Error(BTE_EINVAL, "null batch");
+    HIPCHECK(hipSetDevice(b->ctx->device));
+    batch_set_tally(b, mode);
     BTE_GUARD_END
 }
 

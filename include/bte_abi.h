@@ -199,8 +199,16 @@ typedef struct {
 int bte_rta_batch_create(bte_ctx* ctx, const bte_modes* m, int nruns, const bte_rta_run_desc* runs,
                          double thickness, double dt, double maxtime, double sigma_bounce, int use_graph,
                          bte_rta_batch** out);
+/* Tally arithmetic (no counterpart in the reference, which sums FP64
+ * thread-local arrays, RTA_MC2D.cpp:699-769 and 2326-2335). mode 0 (default):
+ * FP64 device atomics. mode 1: every workgroup sums its segments in FP32
+ * arrays in LDS (T, jx, jy and each differentiated side over the regular
+ * boxes' cells) and adds them into the FP64 tallies once at the end; one
+ * 768-thread workgroup per CU. BTE_EINVAL when those arrays exceed the
+ * device's LDS per workgroup. */
+int bte_rta_batch_set_tally(bte_rta_batch* b, int mode);
 /* Times the batch scaled to `fraction` of each run's particles over a set of
- * launch shapes and keeps the fastest. */
+ * launch shapes and keeps the fastest (no-op under tally mode 1). */
 int bte_rta_batch_tune(bte_rta_batch* b, double fraction, int* block, int* blocks);
 /* T, jx, jy: per run r, [ntime-1][nbox_r] blocks one after another; dTdp (may
  * be NULL): per run [4 sides][ntime-1][nbox_r]; Eeff (may be NULL): [nruns]. */

@@ -136,3 +136,57 @@ def test_specularity_gradient_matches_finite_differences(pkg, ctx):
               f"FD {fd.mean():.4e} +- {fd.std(ddof=1)/np.sqrt(R):.1e}", flush=True)
         assert abs(sf.mean() - fd.mean()) < 4 * se
         assert abs(sf.mean()) > 4 * sf.std(ddof=1) / np.sqrt(R)
+
+
+def test_fp32_lds_tallies_match_fp64(pkg, ctx):
+    """Tally mode 1 (FP32 partial sums per workgroup in LDS, FP64 final sum)
+    changes only the rounding: the particles draw the same streams, so the
+    event counts are equal and T, jx, jy and dT/dp agree with the FP64
+    device-atomic tallies to 1e-4 of each output's largest magnitude, also
+    on graph replays with eager work between them."""
+    m = tilted_mirror_material(n=80, seed=12)
+    W = pkg.workloads
+    rect, kind, teq, _ = W.ribbon(200.0, 20.0, 8, 300.0, reservoirs=(301.0, 299.0))
+    modes = pkg.ModeTable(ctx, m["omega"], m["vx"], m["vy"], m["w"], m["V"], m["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    runs = [dict(geom=geom, T0=300.0, nparticles=100000, seed=7, specularity=(0.0, 0.7, 0.3, 0.0), grad=True),
+            dict(geom=geom, T0=300.0, nparticles=60000, seed=8, specularity=(0.0, 0.5, 0.5, 0.0))]
+    kw = dict(thickness=1.0, dt=20.0, maxtime=400.0, sigma_bounce=5.0)
+    ref_b = pkg.RtaBatch(ctx, modes, runs, **kw)
+    ref, rrep = ref_b.execute(grad=True)
+    lds = pkg.RtaBatch(ctx, modes, runs, use_graph=True, tally="fp32-lds", **kw)
+    for replay in range(2):
+        got, grep = lds.execute(grad=True)
+        assert (grep["events"], grep["crossings"]) == (rrep["events"], rrep["crossings"])
+        assert grep["block"] == 768 and grep["blocks"] == rrep["blocks"] // 8
+        for a, b in zip(got, ref):
+            for key, off in (("T", 300.0), ("jx", 0.0), ("jy", 0.0)):
+                scale = np.abs(b[key] - off).max()
+                assert scale > 0
+                np.testing.assert_allclose(a[key] - off, b[key] - off, rtol=0, atol=1e-4 * scale)
+            scale = np.abs(b["dTdp"]).max()
+            np.testing.assert_allclose(a["dTdp"], b["dTdp"], rtol=0, atol=1e-4 * max(scale, 1e-30))
+        assert np.abs(got[0]["dTdp"][1]).max() > 0 and np.abs(got[1]["dTdp"]).max() == 0
+        ref_b.execute(grad=True)  # eager work between replays
+        ctx.sync()
+    ref_b.close()
+    lds.close()
+
+
+def test_fp32_lds_tallies_reject_what_does_not_fit(pkg, ctx):
+    """600 regular boxes x 100 time bins x 3 arrays of FP32 exceed the LDS of
+    a workgroup: the mode is refused before any launch and the batch keeps
+    working with FP64 tallies."""
+    W = pkg.workloads
+    mat = W.material(64, seed=3)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    rect, kind, teq, _ = W.ribbon(6000.0, 20.0, 600, 300.0, reservoirs=(300.5, 299.5))
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    runs = [dict(geom=geom, T0=300.0, nparticles=2000, seed=1)]
+    kw = dict(thickness=1.0, dt=10.0, maxtime=1000.0, sigma_bounce=2.0)
+    with pytest.raises(Exception, match="LDS"):
+        pkg.RtaBatch(ctx, modes, runs, tally="fp32-lds", **kw)
+    b = pkg.RtaBatch(ctx, modes, runs, **kw)
+    res, rep = b.execute()
+    b.close()
+    assert rep["events"] > 0 and np.isfinite(res[0]["T"]).all()
