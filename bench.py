@@ -268,6 +268,8 @@ def main():
     ap.add_argument("--cpu-particles", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=200)
     ap.add_argument("--replicas", type=int, default=4, help="independent replicas advanced together per run")
+    ap.add_argument("--extra", choices=("rta_steady", "rta_batch", "rta_grad", "krylov"),
+                    help="run only this extra measurement and print its JSON (for profiling one workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -288,6 +290,11 @@ def main():
     pkg = load_pkg()
     W = pkg.workloads
     ctx = pkg.Context(0)
+    if args.extra:
+        fn = {"rta_steady": rta_extra, "rta_batch": batch_extra, "rta_grad": grad_extra,
+              "krylov": krylov_extra}[args.extra]
+        print(json.dumps({"extra": args.extra, **fn(pkg, ctx)}), flush=True)
+        return
     log("setup: material, collision operator, propagator")
     mat, B = W.config1_inputs()
     P, krep = pkg.build_P(ctx, B, 5.0, m=30, precision="fp64", arnoldi="textbook")

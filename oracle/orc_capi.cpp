@@ -27,8 +27,13 @@
 //                               antisymmetry, energy bookkeeping
 //   bert_run                    conservation of the signed count (P >= 0,
 //                               adiabatic walls), E[H_{n+1}] = P H_n with v = 0
-//   Lambert bounce, corner
-//   resolution, nudges          parity unpinned (needs a run of the reference)
+//   lambert_table / bounce      Lambert cosine law per side (chi-square),
+//                               energy window, the odds exp(-d^2/(4 sigma^2))
+//                               of a partner at frequency distance d, omega = 0
+//                               modes never reached (test_oracle_bounce.py)
+//   corner resolution, nudges   ballistic flights through shared corners and
+//                               edges end at the exact endpoint (1e-5 nm);
+//                               symmetric corners split 1/2 : 1/2
 #include "orc_geometry.hpp"
 #include <omp.h>
 
@@ -371,6 +376,69 @@ int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, 
         g_spec_nt = S.ntime - 1;
         g_spec_nq = S.m.nq;
         g_spec_nw = S.nomega;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// diffusive_bouncing alone: ns bounces of `mode` at a side with outward
+// normal (npx, npy), stream k of `seed` for bounce k; out: the new modes
+int orc_lambert_sample(int n, const double* omega, const double* vx, const double* vy, double sigma, double npx,
+                       double npy, int mode, uint64_t seed, int ns, int32_t* out) {
+    try {
+        std::vector<double> w(n, 1.0);
+        Modes m = mk_modes(n, omega, vx, vy, w.data(), 1.0, 1.0, 1);
+        const LambertTable L = lambert_table(m, sigma);
+        for (int k = 0; k < ns; k++) {
+            pcg32 rng(seed, (uint64_t)k);
+            out[k] = lambert_bounce(m, L, npx, npy, mode, rng);
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+// One ballistic evolparticle event (RTA_MC2D.cpp:990-1386) of a particle at
+// (x, y) in box ibox with `mode` and flight time dt, repeated ns times with
+// stream k of `seed`: out_box, out_xy (2 per repeat), out_mode, out_dt (the
+// flight left) and out_kind (0 flight ended inside, 1 wall bounce, 2 box
+// crossing, 3 nudge, 4 absorbed, 5 periodic translation).
+int orc_rta_event(int n, const double* omega, const double* vx, const double* vy, double sigma, int nbox,
+                  const double* rect, const int32_t* kind, int ibox, double x, double y, int mode, double dt,
+                  uint64_t seed, int ns, int32_t* out_box, double* out_xy, int32_t* out_mode, double* out_dt,
+                  int32_t* out_kind) {
+    try {
+        std::vector<double> w(n, 1.0), Teq(nbox, 300.0);
+        RtaSetup S;
+        S.m = mk_modes(n, omega, vx, vy, w.data(), 1.0, 1.0, 1);
+        S.sys = mk_sys(nbox, rect, kind, Teq.data(), nullptr);
+        S.ballistic = true;
+        S.sigma = sigma;
+        S.lambert = lambert_table(S.m, sigma);
+        S.timegrid = {0.0, 1.0e30};
+        S.ntime = 2;
+        if (ibox < 0 || ibox >= nbox || mode < 0 || mode >= n) throw input_error("bad particle");
+        for (int k = 0; k < ns; k++) {
+            pcg32 rng(seed, (uint64_t)k);
+            Tally T(nbox, 1);
+            Counters C;
+            Part p{x, y, 0., mode, ibox, 1, false};
+            double v_x, v_y;
+            get_v(S.m, p.mode, v_x, v_y);
+            double d = dt;
+            evolparticle(S, p, v_x, v_y, d, rng, T, C);
+            out_box[k] = p.box;
+            out_xy[2 * k] = p.x;
+            out_xy[2 * k + 1] = p.y;
+            out_mode[k] = p.mode;
+            out_dt[k] = d;
+            const bool inside = S.sys[ibox].inside(x + dt * v_x, y + dt * v_y);
+            out_kind[k] = p.dead ? 4 : C.nudges ? 3 : C.crossings ? 2 : C.events ? (inside ? 0 : 1) : 5;
+        }
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

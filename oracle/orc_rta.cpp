@@ -21,6 +21,52 @@
 
 namespace orc {
 
+using LambertTable = std::vector<std::vector<std::pair<int, double>>>;
+
+// diffusive_bouncing table (RTA_MC2D.cpp:556-626): for mode i every mode f
+// with nonzero omega within nsigma * hypot(sigma_i, sigma_f) in frequency,
+// weighted by the normal pdf of |omega_i - omega_f|; sigma_i = sigma for all
+// modes (a synthetic table has no q-grid for base_sigma).
+static LambertTable lambert_table(const Modes& m, double sigma) {
+    LambertTable tab(m.n);
+    double sig = std::hypot(sigma, sigma);
+    double norm = 1.0 / (sig * std::sqrt(2 * PI));
+    for (int i = 0; i < m.n; i++) {
+        if (almost_equal(0., m.omega[i])) continue;
+        for (int f = 0; f < m.n; f++) {
+            if (almost_equal(0., m.omega[f])) continue;
+            double d = std::fabs(m.omega[i] - m.omega[f]);
+            if (d <= nsigma * sig) tab[i].push_back({f, norm * std::exp(-0.5 * d * d / (sig * sig))});
+        }
+    }
+    return tab;
+}
+
+// diffusive_bouncing::bounce (RTA_MC2D.cpp:635-678): f with probability
+// proportional to max(-n.v_f, 0) * table(i, f), n the side's outward normal
+static int lambert_bounce(const Modes& m, const LambertTable& L, double npx, double npy, int mode, pcg32& rng) {
+    const auto& tm = L[mode];
+    double cum = 0;
+    std::vector<std::pair<int, double>> states;
+    states.reserve(tm.size());
+    for (auto& [f, econ] : tm) {
+        double vx = m.vx[f], vy = m.vy[f];
+        if (almost_equal(vx, 0.)) vx = 0.;
+        if (almost_equal(vy, 0.)) vy = 0.;
+        double vn = -(npx * vx + npy * vy);
+        if (vn < 0. || almost_equal(vn, 0.)) continue;
+        cum += vn * econ;
+        states.push_back({f, vn * econ});
+    }
+    if (states.empty()) throw geometry_error("no diffuse partner");
+    double s = rng.uniform(), c2 = 0;
+    for (auto& [f, tr] : states) {
+        c2 += tr / cum;
+        if (c2 >= s) return f;
+    }
+    return states.back().first;
+}
+
 struct RtaSetup {
     Modes m;
     std::vector<Box> sys;
@@ -123,42 +169,11 @@ struct RtaSetup {
         }
         Eeff = einfo1 / (double)nparticles;
         // Lambert table (RTA_MC2D.cpp:556-626) with sigma_i = sigma for all modes
-        lambert.assign(m.n, {});
-        double sig = std::hypot(sigma, sigma);
-        double norm = 1.0 / (sig * std::sqrt(2 * PI));
-        for (int i = 0; i < m.n; i++) {
-            if (almost_equal(0., m.omega[i])) continue;
-            for (int f = 0; f < m.n; f++) {
-                if (almost_equal(0., m.omega[f])) continue;
-                double d = std::fabs(m.omega[i] - m.omega[f]);
-                if (d <= nsigma * sig)
-                    lambert[i].push_back({f, norm * std::exp(-0.5 * d * d / (sig * sig))});
-            }
-        }
+        lambert = lambert_table(m, sigma);
     }
 
-    // diffusive_bouncing::bounce (RTA_MC2D.cpp:635-678)
     int bounce(double npx, double npy, int mode, pcg32& rng) const {
-        const auto& tm = lambert[mode];
-        double cum = 0;
-        std::vector<std::pair<int, double>> states;
-        states.reserve(tm.size());
-        for (auto& [f, econ] : tm) {
-            double vx = m.vx[f], vy = m.vy[f];
-            if (almost_equal(vx, 0.)) vx = 0.;
-            if (almost_equal(vy, 0.)) vy = 0.;
-            double vn = -(npx * vx + npy * vy);
-            if (vn < 0. || almost_equal(vn, 0.)) continue;
-            cum += vn * econ;
-            states.push_back({f, vn * econ});
-        }
-        if (states.empty()) throw geometry_error("no diffuse partner");
-        double s = rng.uniform(), c2 = 0;
-        for (auto& [f, tr] : states) {
-            c2 += tr / cum;
-            if (c2 >= s) return f;
-        }
-        return states.back().first;
+        return lambert_bounce(m, lambert, npx, npy, mode, rng);
     }
 };
 

@@ -356,3 +356,35 @@ def project_columns(B):
     B = np.asfortranarray(B, dtype=np.float64).copy(order="F")
     lib().orc_project_columns(B.shape[0], B.ctypes.data_as(dp))
     return B
+
+
+def lambert_sample(omega, vx, vy, sigma, npx, npy, mode, seed, ns):
+    """ns diffuse bounces of `mode` at a side with outward normal (npx, npy)."""
+    om, ux, uy = _f(omega), _f(vx), _f(vy)
+    out = np.zeros(ns, np.int32)
+    if lib().orc_lambert_sample(len(om), _dp(om), _dp(ux), _dp(uy), C.c_double(sigma), C.c_double(npx),
+                                C.c_double(npy), int(mode), u64(seed), int(ns),
+                                out.ctypes.data_as(ip32)) != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out
+
+
+EVENT_KINDS = {0: "inside", 1: "bounce", 2: "crossing", 3: "nudge", 4: "absorbed", 5: "translated"}
+
+
+def rta_event(mat, rect, kind, ibox, x, y, mode, dt, seed, ns, sigma=0.25):
+    """ns repeats (streams 0..ns-1) of one ballistic evolparticle event."""
+    om, ux, uy = _f(mat["omega"]), _f(mat["vx"]), _f(mat["vy"])
+    rc = _f(rect).reshape(-1, 4)
+    kd = np.ascontiguousarray(kind, np.int32)
+    box = np.zeros(ns, np.int32)
+    xy = np.zeros((ns, 2))
+    md = np.zeros(ns, np.int32)
+    dtl = np.zeros(ns)
+    kk = np.zeros(ns, np.int32)
+    if lib().orc_rta_event(len(om), _dp(om), _dp(ux), _dp(uy), C.c_double(sigma), len(rc), _dp(rc),
+                           kd.ctypes.data_as(ip32), int(ibox), C.c_double(x), C.c_double(y), int(mode),
+                           C.c_double(dt), u64(seed), int(ns), box.ctypes.data_as(ip32), _dp(xy),
+                           md.ctypes.data_as(ip32), _dp(dtl), kk.ctypes.data_as(ip32)) != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return {"box": box, "xy": xy, "mode": md, "dt": dtl, "kind": kk}
