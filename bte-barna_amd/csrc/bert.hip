@@ -63,7 +63,7 @@ struct BertParams {
     double* props;           // [steps+1][nbox][3]
     int64_t* npart;          // [steps+1]
     int64_t* pending;        // [steps+1] signed wall counts awaiting re-emission
-    double* wcdf;            // [nbox*4][n] emission cumulative per box side
+    double* wcdf;            // [nbox*4][n] emission cumulative per box side, normalised to 1
     int32_t* wguide;         // [nbox*4][BERT_GUIDE] first cumulative index per key
     uint32_t* wcount;        // [nbox*4] particles to emit this step
     int32_t* wsign;          // [nbox*4] their sign
@@ -613,6 +613,7 @@ __device__ inline double block_scan_256(double v, double* wtot, double& tot) {
     return v + off;
 }
 
+#define BERT_WCDF_PER 8
 // Adiabatic diffuse walls (beyondRTA_MC2d.cpp:281-435), phase 1: one
 // workgroup per (box, side) with a nonzero Eborder computes the border
 // temperature from the J(T) spline and the emission cumulative of
@@ -654,7 +655,67 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     ny = -ny;
     const int n = P.n;
     double* cdf = P.wcdf + (size_t)b * n;
-    // the cumulative in chunks of 256 consecutive modes, one mode per thread
+    int32_t* G = P.wguide + (size_t)b * BERT_GUIDE;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (n <= 256 * BERT_WCDF_PER) {
+        // each thread owns up to BERT_WCDF_PER consecutive modes in
+        // registers: one load phase, one block scan, the guide table from
+        // the same values
+        const int per = (n + 255) >> 8;
+        const int i0 = threadIdx.x * per;
+        double c[BERT_WCDF_PER];
+        double run = 0.;
+#pragma unroll
+        for (int k = 0; k < BERT_WCDF_PER; k++) {
+            const int i = i0 + k;
+            double pm = 0.;
+            if (k < per && i < n) {
+                const double2 v = P.vel[i];
+                const double om = P.omega[i];
+                const double vn = nx * v.x + ny * v.y;
+                if (vn > 0. && !almost_equal(0., om))
+                    pm = vn * om * fabs(bose_einstein(om, Tb) - bose_einstein(om, Teq));
+            }
+            run += pm;
+            c[k] = run;
+        }
+        double tot;
+        const double excl = block_scan_256(run, wtot, tot) - run;
+        int mylast = -1;  // key of this thread's last mode (-1: it has none)
+        const double inv = tot > 0. ? 1.0 / tot : 0.;
+#pragma unroll
+        for (int k = 0; k < BERT_WCDF_PER; k++) {
+            if (k < per && i0 + k < n) {
+                c[k] = (c[k] + excl) * inv;
+                cdf[i0 + k] = c[k];
+                mylast = guide_key(c[k], (double)BERT_GUIDE);
+            }
+        }
+        if (threadIdx.x == 0) {
+            P.wcount[b] = tot > 0. ? (uint32_t)(fabs((double)E) + 1.0e-6) : 0u;
+            P.wsign[b] = (Tb - Teq) >= 0 ? 1 : -1;
+        }
+        if (!(tot > 0.)) return;  // uniform over the workgroup
+        // G[j] = first mode whose key reaches j: each mode fills the keys
+        // above its predecessor's (threads without modes come last)
+        int kp = __shfl_up(mylast, 1, 64);
+        if (lane == 63) kcarry[w] = mylast;
+        __syncthreads();
+        if (lane == 0) kp = w > 0 ? kcarry[w - 1] : -1;
+#pragma unroll
+        for (int k = 0; k < BERT_WCDF_PER; k++) {
+            if (k < per && i0 + k < n) {
+                const int kk = guide_key(c[k], (double)BERT_GUIDE);
+                for (int j = kp + 1; j <= kk; j++) G[j] = i0 + k;
+                kp = kk;
+                // keys above the last cumulative value (rounding) point at the last mode
+                if (i0 + k == n - 1)
+                    for (int j = kk + 1; j < BERT_GUIDE; j++) G[j] = n - 1;
+            }
+        }
+        return;
+    }
+    // larger tables: the cumulative in chunks of 256 consecutive modes, one mode per thread
     // (coalesced), carrying the running total from chunk to chunk
     double carry = 0.;
     for (int c0 = 0; c0 < n; c0 += 256) {
@@ -680,14 +741,18 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     // guide table: G[j] = first index whose key reaches j, so the search for
     // u starts at G[key(u)] <= answer (key is monotone); each thread fills
     // the keys between its predecessor's and its own
-    const double scale = (double)BERT_GUIDE / tot;
-    int32_t* G = P.wguide + (size_t)b * BERT_GUIDE;
+    const double inv = 1.0 / tot;
     int kprev_chunk = -1;  // key of the last mode of the previous chunk
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int c0 = 0; c0 < n; c0 += 256) {
         const int i = c0 + threadIdx.x;
-        // own cdf values were written by this thread: no barrier needed
-        const int k = i < n ? guide_key(cdf[i], scale) : -1;
+        // own cdf values were written by this thread: no barrier needed;
+        // normalised to a last value of 1
+        double cn = 0.;
+        if (i < n) {
+            cn = cdf[i] * inv;
+            cdf[i] = cn;
+        }
+        const int k = i < n ? guide_key(cn, (double)BERT_GUIDE) : -1;
         int kp = __shfl_up(k, 1, 64);
         if (lane == 63) kcarry[w] = k;
         __syncthreads();
@@ -700,7 +765,7 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     }
     // keys above the last cumulative value (rounding) point at the last mode
     // (cdf[n-1] was written before the barriers above)
-    const int klast = guide_key(cdf[n - 1], scale);
+    const int klast = guide_key(cdf[n - 1], (double)BERT_GUIDE);
     for (int j = klast + 1 + threadIdx.x; j < BERT_GUIDE; j += blockDim.x) G[j] = n - 1;
 }
 
@@ -755,10 +820,10 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
             sign = P.wsign[b];
             pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)b << 20) ^ (uint64_t)j ^ 0xad1aULL);
             seed_stream(rng, skey, pid);
+            // normalised cumulative: no load of the side's total before the guide
             const double* cdf = P.wcdf + (size_t)b * P.n;
-            const double tot = cdf[P.n - 1];
-            double u = rng.uniform() * tot;
-            int l = P.wguide[(size_t)b * BERT_GUIDE + guide_key(u, (double)BERT_GUIDE / tot)];
+            const double u = rng.uniform();
+            int l = P.wguide[(size_t)b * BERT_GUIDE + guide_key(u, (double)BERT_GUIDE)];
             while (l < P.n - 1 && cdf[l] < u) l++;
             mode = l;
             const DevBox B = boxes[bx];
