@@ -24,6 +24,7 @@
 #include <chrono>
 #include <cstring>
 #include <numeric>
+#include <set>
 #include <exception>
 #include <thread>
 
@@ -82,6 +83,7 @@ struct BertParams {
     int32_t lds_props;       // properties from per-workgroup box sums (nbox <= BERT_LDS_BOXES)
     // dynamic LDS layout of the particle kernels (byte offsets)
     int32_t lds_pre, lds_eb, lds_sbox, lds_boxes, lds_cts;
+    int32_t cts_lds;         // contacts staged in LDS too (GL instantiations)
     int32_t eb_lds;          // wall counters in LDS (nbox*4 <= BERT_EB_LDS)
 };
 
@@ -107,9 +109,12 @@ __device__ inline void geo_stage(const BertParams& P, const DevBox*& boxes, cons
         DevBox* gb = (DevBox*)(bert_dyn + P.lds_boxes);
         DevContact* gc = (DevContact*)(bert_dyn + P.lds_cts);
         for (int i = threadIdx.x; i < P.nbox; i += blockDim.x) gb[i] = P.boxes[i];
-        for (int i = threadIdx.x; i < P.ncts; i += blockDim.x) gc[i] = P.cts[i];
         boxes = gb;
-        cts = gc;
+        cts = P.cts;
+        if (P.cts_lds) {
+            for (int i = threadIdx.x; i < P.ncts; i += blockDim.x) gc[i] = P.cts[i];
+            cts = gc;
+        }
     } else {
         boxes = P.boxes;
         cts = P.cts;
@@ -207,9 +212,17 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
 
 // The propagator of the box a particle scatters in (one slot per box
 // temperature, Scattering beyondRTA_MC2d.cpp:1131-1166).
+// P1 instantiations are launched when there is one propagator table and no
+// RTA scattering: both tests and the slot and RTA tables drop out of the
+// particle loop (fewer live scalar registers, which the generic loop spills).
+template <bool P1>
+__device__ inline bool rta_on(const BertParams& P) { return P1 ? false : P.rta != 0; }
+template <bool P1>
+__device__ inline int nslots_of(const BertParams& P) { return P1 ? 1 : P.nslots; }
+template <bool P1>
 __device__ inline const uint2* slot_table(const BertParams& P, int box) {
     const size_t nn = (size_t)P.ncol * P.ncol;
-    return P.nslots > 1 ? P.palias + (size_t)P.box_pslot[box] * nn : P.palias;
+    return nslots_of<P1>(P) > 1 ? P.palias + (size_t)P.box_pslot[box] * nn : P.palias;
 }
 
 // The first propagator draw of a particle depends only on its mode and, with
@@ -221,13 +234,14 @@ struct Pick0 {
     uint32_t k, u;
     uint2 e;
 };
+template <bool P1>
 __device__ inline Pick0 first_pick(const BertParams& P, Pcg32& rng, int mode, int box) {
     Pick0 f{0u, 0u, make_uint2(0u, 0u)};
     const int col = mode - P.ngamma;
-    if (P.rta || col < 0 || col >= P.ncol) return f;
+    if (rta_on<P1>(P) || col < 0 || col >= P.ncol) return f;
     f.k = rng.bounded((uint32_t)P.ncol);
     f.u = rng.next();
-    f.e = slot_table(P, box)[(size_t)col * P.ncol + f.k];
+    f.e = slot_table<P1>(P, box)[(size_t)col * P.ncol + f.k];
     return f;
 }
 
@@ -255,11 +269,11 @@ __device__ inline void tally_child(const BertParams& P, unsigned long long* H, d
 // returns the first one in nm0/ns0; pass 1 replays the same stream to emit
 // them when there is more than one. Children of a sign flip re-queue the
 // parent twice (1189-1192).
-template <bool EMIT>
+template <bool EMIT, bool P1>
 __device__ inline int scatter(const BertParams& P, unsigned long long* H, double* sbox, Pcg32 rng, const Pick0& f0, int mode,
                               int sign, int box, BPart* out, uint64_t slot, uint64_t limit, double x, double y,
                               uint64_t pid, uint32_t step, int& nm0, int& ns0) {
-    if (P.rta) {
+    if (rta_on<P1>(P)) {
         double R = rng.uniform();
         int nm = mode;
         if (R < 1.0 - exp(-P.w[mode] * P.dt)) {
@@ -283,7 +297,7 @@ __device__ inline int scatter(const BertParams& P, unsigned long long* H, double
         atomicOr(P.flags + 1, 1u);
         return 0;
     }
-    const uint2* tab = slot_table(P, box) + (size_t)col * P.ncol;
+    const uint2* tab = slot_table<P1>(P, box) + (size_t)col * P.ncol;
     int pending = 1, nchild = 0;
     while (pending > 0 && nchild < P.max_children) {
         pending--;
@@ -403,11 +417,12 @@ __device__ inline void seg_close(const BertParams& P, uint32_t* segout, int myse
 
 // count the children, reserve their slots in the workgroup's segment and
 // write them; a single child (the common case) is written from pass 0's result
+template <bool P1>
 __device__ inline void scatter_out(const BertParams& P, unsigned long long* H, double* sbox, Pcg32& rng, const Pick0& f0,
                                    bool alive, int mode, int sign, int box, double x, double y, uint64_t pid,
                                    uint32_t step, uint32_t* lds_count, int myseg) {
     int nm0 = 0, ns0 = 1;
-    int c = alive ? scatter<false>(P, H, sbox, rng, f0, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
+    int c = alive ? scatter<false, P1>(P, H, sbox, rng, f0, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
     uint64_t limit;
     uint64_t slot = wave_reserve(lds_count, myseg, P.segcap, c, P.flags, limit);
     if (c == 1) {
@@ -417,7 +432,7 @@ __device__ inline void scatter_out(const BertParams& P, unsigned long long* H, d
             P.pout[slot] = q;
         }
     } else if (c > 1) {
-        scatter<true>(P, H, sbox, rng, f0, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
+        scatter<true, P1>(P, H, sbox, rng, f0, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
     }
 }
 
@@ -501,8 +516,10 @@ __device__ inline void box_lds_flush(const BertParams& P, int myseg) {
 
 // One workgroup per segment of the step's output. Workgroup w takes the
 // w-th equal slice of the live particle list (all input segments).
-template <bool GL, bool PER>
-__global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t step, int cur,
+// P1 without periodic boxes fits 96 VGPRs: five workgroups (waves per SIMD)
+// stay resident per CU
+template <bool GL, bool PER, bool P1>
+__global__ __launch_bounds__(256, (P1 && !PER) ? 5 : 1) void bert_step_kernel(BertParams P, uint32_t step, int cur,
                                                         int cancel_prev, int hist) {
     __shared__ uint32_t tsum[16];
     __shared__ uint32_t lcount;
@@ -571,14 +588,14 @@ __global__ __launch_bounds__(256) void bert_step_kernel(BertParams P, uint32_t s
             }
             if (alive) {
                 seed_stream(rng, skey, p.pid);
-                if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
+                if (nslots_of<P1>(P) == 1) f0 = first_pick<P1>(P, rng, mode, 0);
                 const double2 v = P.vel[mode];
                 const double2 iv = P.ivel[mode];
                 alive = advect<PER>(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
-                if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, box);
+                if (nslots_of<P1>(P) > 1 && alive) f0 = first_pick<P1>(P, rng, mode, box);
             }
         }
-        scatter_out(P, H, sbox, rng, f0, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
+        scatter_out<P1>(P, H, sbox, rng, f0, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -777,7 +794,7 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
 // kernel's segment walk).
 #define BERT_EMIT_MAXB 4096
 #define BERT_EMIT_THREADS 512
-template <bool GL, bool PER>
+template <bool GL, bool PER, bool P1>
 __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
     __shared__ uint32_t tsum[16];
     __shared__ uint32_t lcount;
@@ -838,11 +855,11 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
             const double2 v = P.vel[mode];
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
-            if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
+            if (nslots_of<P1>(P) == 1) f0 = first_pick<P1>(P, rng, mode, 0);
             alive = advect<PER>(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
-            if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, bx);
+            if (nslots_of<P1>(P) > 1 && alive) f0 = first_pick<P1>(P, rng, mode, bx);
         }
-        scatter_out(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out<P1>(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -920,11 +937,11 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
             const double2 v = P.vel[mode];
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
-            if (P.nslots == 1) f0 = first_pick(P, rng, mode, 0);
+            if (nslots_of<false>(P) == 1) f0 = first_pick<false>(P, rng, mode, 0);
             alive = advect<PER>(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
-            if (P.nslots > 1 && alive) f0 = first_pick(P, rng, mode, bx);
+            if (nslots_of<false>(P) > 1 && alive) f0 = first_pick<false>(P, rng, mode, bx);
         }
-        scatter_out(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out<false>(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -1077,6 +1094,20 @@ __global__ __launch_bounds__(256) void bert_init_kernel(BertParams P, const Bert
     }
 }
 
+// particle kernels by [geometry in LDS][periodic boxes][P1]
+using StepKernel = void (*)(BertParams, uint32_t, int, int, int);
+using EmitKernel = void (*)(BertParams, uint32_t, int, int);
+static const StepKernel step_kernel_tab[2][2][2] = {
+    {{bert_step_kernel<false, false, false>, bert_step_kernel<false, false, true>},
+     {bert_step_kernel<false, true, false>, bert_step_kernel<false, true, true>}},
+    {{bert_step_kernel<true, false, false>, bert_step_kernel<true, false, true>},
+     {bert_step_kernel<true, true, false>, bert_step_kernel<true, true, true>}}};
+static const EmitKernel emit_kernel_tab[2][2][2] = {
+    {{bert_emit_kernel<false, false, false>, bert_emit_kernel<false, false, true>},
+     {bert_emit_kernel<false, true, false>, bert_emit_kernel<false, true, true>}},
+    {{bert_emit_kernel<true, false, false>, bert_emit_kernel<true, false, true>},
+     {bert_emit_kernel<true, true, false>, bert_emit_kernel<true, true, true>}}};
+
 struct Bert {
     Ctx* ctx = nullptr;
     const Modes* m = nullptr;
@@ -1110,6 +1141,7 @@ struct Bert {
     int nsrc = 0, ninit = 0;
     bool geo_lds = false;
     bool periodic = false;  // some box is periodic: launch the PER kernels
+    bool p1 = false;        // expected to launch the P1 kernels (sizes the step grid)
     size_t lds_bytes = 0;
     int step_blocks = 0, emit_blocks = 0, nsegs = 0;
     std::vector<int32_t> hH;
@@ -1288,7 +1320,18 @@ static BertParams make_params(Bert* b) {
     b->geo_lds = geo <= BERT_GEO_LDS_BYTES;
     P.lds_boxes = (int32_t)off;
     P.lds_cts = (int32_t)up16(off + sizeof(DevBox) * (size_t)P.nbox);
-    if (b->geo_lds) off = up16(P.lds_cts + sizeof(DevContact) * (size_t)P.ncts);
+    P.cts_lds = 1;
+    if (b->geo_lds) {
+        off = up16(P.lds_cts + sizeof(DevContact) * (size_t)P.ncts);
+        // five step workgroups per CU share its LDS: when the layout does
+        // not fit a fifth, the contacts (read only for sides that several
+        // boxes share and for periodic translations) stay in global memory
+        const size_t per_wg = b->ctx->lds_per_block / 5;
+        if (b->p1 && !b->periodic && off + 256 > per_wg && (size_t)P.lds_cts + 256 <= per_wg) {
+            P.cts_lds = 0;
+            off = (size_t)P.lds_cts;
+        }
+    }
     b->lds_bytes = off;
     return P;
 }
@@ -1540,9 +1583,19 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     }
     b->nsrc = (int)srcs.size();
     // step workgroups per CU: as many as its registers keep resident (3 with
-    // the periodic translation compiled in, 4 without), for latency hiding
+    // the periodic translation compiled in, 5 for the P1 kernels without it,
+    // 4 otherwise), for latency hiding. P1: no RTA and one propagator table,
+    // i.e. a single tabulated temperature or one Teq over the regular boxes
     for (auto& bx : g->dboxes) b->periodic = b->periodic || bx.kind == 2;
-    b->step_blocks = std::min((b->periodic ? 3 : 4) * c->cu_count, 2048);
+    bool one_table = true;
+    if (b->p && !(b->p->T.size() == 1 && b->p->T[0] <= 0.)) {
+        std::set<double> temps;
+        for (auto& bx : g->boxes)
+            if (bx.kind == 0) temps.insert(bx.Teq);
+        one_table = temps.size() <= 1;
+    }
+    b->p1 = !cfg->rta && b->p != nullptr && one_table;
+    b->step_blocks = std::min((b->periodic ? 3 : (b->p1 ? 5 : 4)) * c->cu_count, 2048);
     b->emit_blocks = std::min(2 * c->cu_count, 1024);
     b->nsegs = b->step_blocks + b->emit_blocks + b->nsrc;
     if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
@@ -1746,6 +1799,8 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     HIPCHECK(hipStreamSynchronize(s));
     const int step_blocks = b->step_blocks;
     const int emit_blocks = b->emit_blocks;
+    // one propagator table and no RTA scattering: the P1 particle kernels
+    const int p1 = (!b->cfg.rta && P.nslots == 1) ? 1 : 0;
     const int pc = b->cfg.pcancel_each;
     auto t0 = std::chrono::steady_clock::now();
     HIPCHECK(hipEventRecord(b->ev0, s));
@@ -1766,8 +1821,7 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         const bool timed = (istep % pc) == ((istep / pc) % pc);
         if (timed) HIPCHECK(hipEventRecord(b->kev[2 * istep], s));
         {
-            auto k = b->geo_lds ? (b->periodic ? bert_step_kernel<true, true> : bert_step_kernel<true, false>)
-                                : (b->periodic ? bert_step_kernel<false, true> : bert_step_kernel<false, false>);
+            auto k = step_kernel_tab[b->geo_lds][b->periodic][p1];
             hipLaunchKernelGGL(k, dim3(step_blocks), dim3(256), b->lds_bytes, s, P, (uint32_t)istep, cur, cancel_prev,
                                hist);
         }
@@ -1777,8 +1831,7 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
                                dim3(256), b->lds_bytes, s, P, b->d_src.p, b->d_src_tab.p, (uint32_t)istep, cur, hist);
         hipLaunchKernelGGL(bert_wallcdf_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep);
         {
-            auto k = b->geo_lds ? (b->periodic ? bert_emit_kernel<true, true> : bert_emit_kernel<true, false>)
-                                : (b->periodic ? bert_emit_kernel<false, true> : bert_emit_kernel<false, false>);
+            auto k = emit_kernel_tab[b->geo_lds][b->periodic][p1];
             hipLaunchKernelGGL(k, dim3(emit_blocks), dim3(BERT_EMIT_THREADS), b->lds_bytes, s, P, (uint32_t)istep, cur,
                                hist);
         }
