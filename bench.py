@@ -220,7 +220,7 @@ def grad_extra(pkg, ctx, fd_dp=0.01):
             "derivatives": out, "fd_dp": fd_dp}
 
 
-def krylov_extra(pkg, ctx, N=8192, m=30):
+def krylov_extra(pkg, ctx, N=8192, m=30, cpu_cols=0):
     W = pkg.workloads
     mat, B = W.config1_inputs(nmodes=N, seed=1)
     res = {}
@@ -255,7 +255,28 @@ def krylov_extra(pkg, ctx, N=8192, m=30):
     del Ex
     res["workload"] = f"PropagatorBuilder Krylov N={N}, m={m}, dt=5 ps, all columns as block-Krylov GEMMs"
     res["flops_2mN3"] = 2.0 * m * N ** 3
+    if cpu_cols:
+        res["cpu_baseline"] = krylov_cpu_baseline(B, m, cpu_cols, res["fp64_reference_arnoldi"]["build_ms"])
     return res
+
+
+def krylov_cpu_baseline(B, m, ncols, gpu_ms):
+    """The oracle's per-column FP64 Arnoldi + Pade (OpenMP over columns, the
+    reference's build_P loop) on a bounded sample of columns, extrapolated to
+    all N columns; compared with the GPU build of the same Arnoldi."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    nthreads = max(1, min(nthreads, 16))
+    N = B.shape[0]
+    A = np.asfortranarray(B)
+    t0 = time.perf_counter()
+    orc.build_P_cols(A, 5.0, m, 0, 0, ncols, nthreads)
+    el = time.perf_counter() - t0
+    est = el * N / ncols
+    return {"value": est, "unit": "s per full propagator (extrapolated)", "cores": nthreads, "kind": "port",
+            "sample": f"oracle build_P, reference Arnoldi, FP64, columns 0..{ncols - 1} of {N} in {el:.2f} s",
+            "gpu_fp64_reference_arnoldi_s": gpu_ms * 1e-3, "gpu_speedup": est / (gpu_ms * 1e-3)}
 
 
 def main():
@@ -377,7 +398,7 @@ def main():
         log("extra: specularity gradient ...")
         ex["rta_grad"] = grad_extra(pkg, ctx)
         log("extra: Krylov N=8192 ...")
-        ex["krylov"] = krylov_extra(pkg, ctx)
+        ex["krylov"] = krylov_extra(pkg, ctx, cpu_cols=0 if args.no_cpu else 32)
         ex["propagator_setup_ms"] = krep["build_ms"]
         line["extras"] = ex
     for r in runs:

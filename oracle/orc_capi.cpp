@@ -43,6 +43,7 @@ long double error_bound(int N, const double* A, int n, double t);
 int krylov_size(int N, const double* A, double t, double eps);
 void arnoldi(int N, const double* A, const double* b, int n, double t, int mode, double* V, double* H);
 void build_P(int N, const double* A, double t, int m, int mode, double* P);
+void build_P_cols(int N, const double* A, double t, int m, int mode, int c0, int c1, double* P);
 void project_columns(int N, double* B);
 void synthetic_B(int N, const double* C, const double* w, double density, uint64_t seed,
                  int sinkhorn_iters, double* B);
@@ -553,6 +554,10 @@ void orc_arnoldi(int N, const double* A, const double* b, int n, double t, int m
 void orc_build_P(int N, const double* A, double t, int m, int mode, int nthreads, double* P) {
     if (nthreads > 0) omp_set_num_threads(nthreads);
     build_P(N, A, t, m, mode, P);
+}
+void orc_build_P_cols(int N, const double* A, double t, int m, int mode, int c0, int c1, int nthreads, double* P) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    build_P_cols(N, A, t, m, mode, c0, c1, P);
 }
 void orc_project_columns(int N, double* B) { project_columns(N, B); }
 void orc_synthetic_B(int N, const double* C, const double* w, double density, uint64_t seed,

@@ -187,12 +187,13 @@ void arnoldi(int N, const double* A, const double* b, int n, double t, int mode,
 }
 
 // build_P (collision_operator.cpp:1015-1064): P(:,i) = (V exp(H))(:,0)
-void build_P(int N, const double* A, double t, int m, int mode, double* P) {
+// columns [c0, c1) of P into P[:, 0 : c1 - c0] (OpenMP over columns)
+void build_P_cols(int N, const double* A, double t, int m, int mode, int c0, int c1, double* P) {
 #pragma omp parallel
     {
         std::vector<double> V((size_t)N * m), H((size_t)m * m), E((size_t)m * m), b(N);
 #pragma omp for schedule(dynamic, 1)
-        for (int i = 0; i < N; i++) {
+        for (int i = c0; i < c1; i++) {
             std::fill(b.begin(), b.end(), 0.);
             b[i] = 1.0;
             arnoldi(N, A, b.data(), m, t, mode, V.data(), H.data());
@@ -200,10 +201,14 @@ void build_P(int N, const double* A, double t, int m, int mode, double* P) {
             for (int r = 0; r < N; r++) {
                 double s = 0;
                 for (int k = 0; k < m; k++) s += V[r + (size_t)k * N] * E[k];
-                P[r + (size_t)i * N] = s;
+                P[r + (size_t)(i - c0) * N] = s;
             }
         }
     }
+}
+
+void build_P(int N, const double* A, double t, int m, int mode, double* P) {
+    build_P_cols(N, A, t, m, mode, 0, N, P);
 }
 
 // Column energy projection (collision_operator.cpp:786-807): subtract the

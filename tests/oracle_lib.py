@@ -334,6 +334,16 @@ def build_P(A, t, m, mode, nthreads=0):
     return P
 
 
+def build_P_cols(A, t, m, mode, c0, c1, nthreads=0):
+    """columns [c0, c1) of the per-column Arnoldi build"""
+    A = np.asfortranarray(A, dtype=np.float64)
+    N = A.shape[0]
+    P = np.zeros((N, c1 - c0), order="F")
+    lib().orc_build_P_cols(N, A.ctypes.data_as(dp), C.c_double(t), m, mode, c0, c1, nthreads,
+                           P.ctypes.data_as(dp))
+    return P
+
+
 def krylov_size(A, t, eps=1e-8):
     A = np.asfortranarray(A, dtype=np.float64)
     return lib().orc_krylov_size(A.shape[0], A.ctypes.data_as(dp), C.c_double(t), C.c_double(eps))
