@@ -82,7 +82,24 @@ def cpu_baseline(pkg, mat, P, cfg, nparticles, nsteps):
                       f"{nsteps} steps ({ps} particle-events in {el:.2f} s), {platform.processor() or platform.machine()}"}
 
 
-def rta_extra(pkg, ctx):
+def rta_cpu_baseline(c, nparticles):
+    """The oracle's RTAMC2D history loop (OpenMP over particles) on a bounded
+    particle sample of configuration 0."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as orc
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    nthreads = max(1, min(nthreads, 16))
+    t0 = time.perf_counter()
+    o = orc.rta_run(c["material"], c["rect"], c["kind"], c["Teq"], thickness=c["thickness"], T0=c["T0"], dt=c["dt"],
+                    maxtime=c["maxtime"], nparticles=nparticles, seed=12345, sigma=c["sigma_bounce"],
+                    nthreads=nthreads)
+    el = time.perf_counter() - t0
+    return {"value": o["events"] / el, "unit": "particle-events/s", "cores": nthreads, "kind": "port",
+            "sample": f"oracle RTAMC2D, configuration 0 with {nparticles} particles ({o['events']} events in "
+                      f"{el:.2f} s)"}
+
+
+def rta_extra(pkg, ctx, cpu_particles=0):
     W = pkg.workloads
     c = W.config0()
     m = c["material"]
@@ -119,7 +136,8 @@ def rta_extra(pkg, ctx):
             "T_left_right": [float(T[0]), float(T[-1])],
             "sigma_T": {"replicas": R, "particles_per_replica": c["nparticles"], "max_bin_sigma_K": sig,
                         "target_K": 0.002, "met": sig <= 0.002, "seconds": t_rep,
-                        "events_per_s": rep["events"] / t_rep}}
+                        "events_per_s": rep["events"] / t_rep},
+            **({"cpu_baseline": rta_cpu_baseline(c, cpu_particles)} if cpu_particles else {})}
 
 
 def batch_extra(pkg, ctx):
@@ -392,7 +410,7 @@ def main():
         log("cpu baseline done")
     if not args.no_extras:
         log("extra: RTAMC2D config ...")
-        ex = {"rta_steady": rta_extra(pkg, ctx)}
+        ex = {"rta_steady": rta_extra(pkg, ctx, cpu_particles=0 if args.no_cpu else 200000)}
         log("extra: batched RTAMC2D sweep ...")
         ex["rta_batch"] = batch_extra(pkg, ctx)
         log("extra: specularity gradient ...")
