@@ -274,6 +274,7 @@ def krylov_extra(pkg, ctx, N=8192, m=30, cpu_cols=0):
     res["workload"] = f"PropagatorBuilder Krylov N={N}, m={m}, dt=5 ps, all columns as block-Krylov GEMMs"
     res["flops_2mN3"] = 2.0 * m * N ** 3
     if cpu_cols:
+        log(f"extra: Krylov CPU sample ({cpu_cols} columns) ...")
         res["cpu_baseline"] = krylov_cpu_baseline(B, m, cpu_cols, res["fp64_reference_arnoldi"]["build_ms"])
     return res
 

@@ -160,11 +160,14 @@ void arnoldi(int N, const double* A, const double* b, int n, double t, int mode,
     for (int i = 0; i < N; i++) q[i] = b[i] / bn;
     for (int i = 0; i < N; i++) V[i] = q[i];
     for (int k = 0; k < n; k++) {
-        for (int i = 0; i < N; i++) {
-            double s = 0;
-            for (int j = 0; j < N; j++) s += A[i + (size_t)j * N] * q[j];
-            z[i] = t * s;
+        // z = t A q, streaming the column-major A column by column
+        std::fill(z.begin(), z.end(), 0.);
+        for (int j = 0; j < N; j++) {
+            const double qj = q[j];
+            const double* a = A + (size_t)j * N;
+            for (int i = 0; i < N; i++) z[i] += a[i] * qj;
         }
+        for (int i = 0; i < N; i++) z[i] *= t;
         int jmax = mode == 0 ? k : k + 1;
         for (int j = 0; j < jmax; j++) {
             double h = 0;
