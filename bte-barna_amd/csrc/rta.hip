@@ -94,14 +94,16 @@ __device__ inline int time_bin(const double* grid, int nt, double target) {
 
 // process_segment (RTA_MC2D.cpp:699-769); gradient runs also add the
 // segment weighted by the particle's running score for each active side.
-// LT: the workgroup's FP32 partial sums in LDS (lt), added into the FP64
-// tallies once at the end of the kernel (lt_flush).
-template <bool GRAD, bool LT>
-__device__ inline void tally_segment(const RtaParams& P, float* lt, int box, double t0, double t1, int sign,
-                                     double vx, double vy, int gmask, const float* score, int mode) {
+// TM (tally mode) 0: FP64 device atomics. 1: the workgroup's FP32 partial
+// sums in LDS (lds), added into the FP64 tallies once at the end of the
+// kernel (lt_flush).
+template <bool GRAD, int TM>
+__device__ inline void tally_segment(const RtaParams& P, void* lds, int box, double t0, double t1,
+                                     int sign, double vx, double vy, int gmask, const float* score, int mode) {
     int b0 = time_bin(P.timegrid, P.nt, t0), b1 = time_bin(P.timegrid, P.nt, t1);
     double delta = t1 - t0;
-    if constexpr (LT) {
+    if constexpr (TM == 1) {
+        float* lt = (float*)lds;
         const uint32_t lc = (uint32_t)P.lrows * (uint32_t)P.nt;
         float* L = lt + P.box_lrow[box] * (uint32_t)P.nt;
         for (int i = b0; i <= b1; i++) {
@@ -163,9 +165,6 @@ __device__ inline int find_run(const RtaParams& P, uint64_t pid) {
     return lo;
 }
 
-// One lane follows one particle history at a time and takes the next id
-// from a per-wave pool when it ends; ids of all runs form one range, so a
-// batch of runs of any sizes fills the GPU in one launch.
 // The workgroup's LDS partial sums into the FP64 tallies (the XCD's copy).
 template <bool GRAD>
 __device__ inline void lt_flush(const RtaParams& P, const float* lt) {
@@ -211,6 +210,205 @@ __device__ inline void lt_counters(unsigned long long* cnt, unsigned long long v
     }
 }
 
+// The state of one particle history between events.
+struct RtaState {
+    Pcg32 rng;
+    double x, y, t, dtrem, wscale, tlast;
+    unsigned long long budget;
+    int mode, box, sign, run, still, alive, need_flight, pad;
+    float score[4];
+};
+
+struct RtaCounts {
+    unsigned long long events = 0, crossings = 0, stuck = 0, badmirror = 0, badtrans = 0;
+};
+
+// Init_generator (RTA_MC2D.cpp:436-484): particle pid of its run at t = 0
+template <bool GRAD>
+__device__ inline void rta_start(const RtaParams& P, RtaState& S, uint64_t pid) {
+    S.budget = 0;
+    S.still = 0;
+    S.tlast = -1.0;
+    S.run = find_run(P, pid);
+    const RtaRunDev& R = P.runs[S.run];
+    S.rng.seed(R.seed, pid - R.first);
+    S.wscale = R.wscale;
+    if (GRAD) S.score[0] = S.score[1] = S.score[2] = S.score[3] = 0.f;
+    bool ng;
+    int gi = R.gen0 + (int)alias_pick(P.gen_sel + R.gsel0, (uint32_t)R.ngen, S.rng, ng);
+    const RtaGen g = P.gens[gi];
+    double u = S.rng.uniform();
+    S.x = g.p1x + u * g.ndx;
+    S.y = g.p1y + u * g.ndy;
+    const DevBox bi = P.boxes[g.idi];
+    if (!box_inside(bi, S.x, S.y)) {
+        double dx = 0.5 * (bi.xmin + bi.xmax) - S.x, dy = 0.5 * (bi.ymin + bi.ymax) - S.y;
+        double dn = sqrt(dx * dx + dy * dy);
+        S.x += 1.0e-6 * dx / dn;
+        S.y += 1.0e-6 * dy / dn;
+    }
+    S.mode = (int)alias_pick(P.gen_tab + (size_t)gi * P.n, (uint32_t)P.n, S.rng, ng);
+    S.sign = g.sign;
+    S.box = g.idi;
+    S.t = 0.;
+    S.alive = 1;
+    S.need_flight = 1;
+}
+
+// Every history ends: a particle that exceeds the iteration budget or makes
+// 64 events in a row without advancing in time is dropped and counted.
+__device__ inline void rta_watch(const RtaParams& P, RtaState& S, RtaCounts& C) {
+    S.still = (S.t == S.tlast) ? S.still + 1 : 0;
+    S.tlast = S.t;
+    if (++S.budget > P.max_iters || S.still > 64) {
+        C.stuck++;
+        S.alive = 0;
+    }
+}
+
+// One step of evolparticle (RTA_MC2D.cpp:990-1386) after the flight draw of
+// the history loop (2262-2288).
+template <bool GRAD, int TM>
+__device__ inline void rta_event(const RtaParams& P, void* lds, RtaState& S, RtaCounts& C) {
+    const double4 md = P.modes[S.mode];
+    if (S.need_flight) {
+        // exponential flight, truncated at maxtime
+        if (!(S.t < P.maxtime)) {
+            S.alive = 0;
+            return;
+        }
+        double u = 1.0 - S.rng.uniform();
+        S.dtrem = -log(u) / (md.z * S.wscale);
+        if (S.t + S.dtrem > P.maxtime) S.dtrem = P.maxtime - S.t + 1.0e-8;
+        if (almost_equal(S.dtrem, 0.)) return;
+        S.need_flight = 0;
+    }
+    const DevBox B = P.boxes[S.box];
+    if (B.kind == 2) {
+        // periodic box: translate, keep the remaining flight (RTA_MC2D.cpp:1034-1043)
+        const int nb = translate_rect(P.boxes, P.cts, P.cseg, P.ptrans, S.box, S.x, S.y, md.x, md.y, S.rng);
+        if (nb < 0) {
+            C.badtrans++;
+            S.alive = 0;
+            return;
+        }
+        S.box = nb;
+        return;
+    }
+    if (B.kind == 1) {  // reservoir: absorbed (RTA_MC2D.cpp:1046-1053)
+        C.events++;
+        S.alive = 0;
+        return;
+    }
+    const int gmask = GRAD ? P.runs[S.run].gmask : 0;
+    const double vx = md.x, vy = md.y;
+    double nx = S.x + S.dtrem * vx, ny = S.y + S.dtrem * vy;
+    const bool inside = box_inside(B, nx, ny);
+    int mask = 0;
+    double ex = 0., ey = 0., tf = 0.;
+    if (!inside) {
+        const double2 iv = P.ivel[S.mode];
+        tf = box_exit(B, S.x, S.y, vx, vy, iv.x, iv.y, mask, ex, ey);
+    }
+    if (inside) {
+        tally_segment<GRAD, TM>(P, lds, S.box, S.t, S.t + S.dtrem, S.sign, vx, vy, gmask, S.score, S.mode);
+        S.x = nx;
+        S.y = ny;
+        S.t += S.dtrem;
+        if (!P.runs[S.run].ballistic) {
+            bool ng;
+            S.mode = (int)alias_pick(P.scat + P.runs[S.run].scat0, (uint32_t)P.n, S.rng, ng);
+        }
+        C.events++;
+        S.need_flight = 1;
+        return;
+    }
+    double left = S.dtrem - tf;
+    if (almost_equal(left, 0.)) left = 0.;
+    tally_segment<GRAD, TM>(P, lds, S.box, S.t, S.t + tf, S.sign, vx, vy, gmask, S.score, S.mode);
+    S.t += tf;
+    int side = 0;
+    const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
+    Ids4 pb{};
+    int npb;
+    if (fast >= 0) {
+        pb.a = fast;
+        npb = 1;
+    } else {
+        npb = fast == SIDE_VOID ? 0 : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
+    }
+    if (npb == 0) {
+        int s = fast == SIDE_VOID ? side
+              : (__popc(mask) == 1 ? __ffs(mask) - 1
+                                   : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, S.rng.next()));
+        // specular with probability p (mirror partner of the mode), else
+        // diffuse; a draw is made only when 0 < p < 1
+        const float p = P.runs[S.run].spec[s];
+        bool specular = p >= 1.f;
+        if (p > 0.f && p < 1.f) {
+            specular = S.rng.uniform() < (double)p;
+            if (GRAD && ((gmask >> s) & 1)) {
+                // d/dp log P(outcome): 1/p for specular, -1/(1-p) for diffuse
+                const float d = specular ? 1.f / p : -1.f / (1.f - p);
+                if (s == 0) S.score[0] += d;
+                else if (s == 1) S.score[1] += d;
+                else if (s == 2) S.score[2] += d;
+                else S.score[3] += d;
+            }
+        }
+        if (specular) {
+            const int partner = P.mirror[(s == 0 || s == 3 ? 0 : (size_t)P.n) + S.mode];
+            if (partner < 0) {
+                C.badmirror++;
+                S.alive = 0;
+                return;
+            }
+            S.mode = partner;
+        } else {
+            // Lambert re-emission (RTA_MC2D.cpp:635-678)
+            const int4 li = P.lam_info[S.mode];
+            bool ng;
+            int k = (int)alias_pick(P.lam_tab + li.z + (size_t)s * li.y, (uint32_t)li.y, S.rng, ng);
+            S.mode = P.perm[li.x + k];
+        }
+        // the remaining flight is discarded (RTA_MC2D.cpp:1259)
+        S.x = ex;
+        S.y = ey;
+        C.events++;
+        S.need_flight = 1;
+        return;
+    }
+    S.x = ex;
+    S.y = ey;
+    if (npb == 1) {
+        S.box = pb.a;
+    } else {
+        // several candidate boxes at a corner (RTA_MC2D.cpp:1316-1380)
+        Ids4 cand{};
+        int nc = 0;
+        double cx = ex + 1.0e-6 * vx, cy = ey + 1.0e-6 * vy;
+        for (int k = 0; k < npb && k < 4; k++)
+            if (box_inside(P.boxes[pb.get(k)], cx, cy)) cand.set(nc++, pb.get(k));
+        if (nc > 0) {
+            S.box = cand.get((int)S.rng.bounded((uint32_t)nc));
+        } else {
+            S.box = pb.get((int)S.rng.bounded((uint32_t)(npb < 4 ? npb : 4)));
+            const DevBox nb = P.boxes[S.box];
+            double dx = 0.5 * (nb.xmin + nb.xmax) - S.x, dy = 0.5 * (nb.ymin + nb.ymax) - S.y;
+            double dn = sqrt(dx * dx + dy * dy);
+            S.x += 1.0e-6 * dx / dn;
+            S.y += 1.0e-6 * dy / dn;
+        }
+    }
+    C.crossings++;
+    S.dtrem = left;
+    if (S.dtrem == 0.) S.need_flight = 1;
+    return;
+}
+
+// One lane follows one particle history at a time and takes the next id
+// from a per-wave pool when it ends; ids of all runs form one range, so a
+// batch of runs of any sizes fills the GPU in one launch.
 template <bool GRAD, bool LT>
 __global__ __launch_bounds__(LT ? 768 : 512) void rta_history_kernel(RtaParams P) {
     float* lt = nullptr;
@@ -222,33 +420,16 @@ __global__ __launch_bounds__(LT ? 768 : 512) void rta_history_kernel(RtaParams P
         for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) lt[j] = 0.f;
         __syncthreads();
     }
-    Pcg32 rng;
-    double x = 0, y = 0, t = 0, dtrem = 0, wscale = 1.0;
-    int mode = 0, box = 0, sign = 1, run = 0;
-    bool alive = false, need_flight = true;
-    unsigned long long events = 0, crossings = 0, stuck = 0, badmirror = 0, badtrans = 0;
-    unsigned long long budget = 0;
-    int still = 0;        // consecutive events without time advance
-    double tlast = -1.0;
-    float score[4] = {0.f, 0.f, 0.f, 0.f};
-    const double maxtime = P.maxtime;
+    RtaState S;
+    S.alive = 0;
+    RtaCounts C;
     // particle ids come from a per-wave pool refilled 64 at a time, so the
     // global id counter sees one atomic per 64 histories
     unsigned long long pool_lo = 0, pool_hi = 0;
     for (;;) {
-        // every history ends: a particle that exceeds the iteration budget or
-        // makes 64 events in a row without advancing in time is dropped and
-        // counted, so each lane reaches the exit
-        if (alive) {
-            still = (t == tlast) ? still + 1 : 0;
-            tlast = t;
-            if (++budget > P.max_iters || still > 64) {
-                stuck++;
-                alive = false;
-            }
-        }
+        if (S.alive) rta_watch(P, S, C);
         unsigned long long pid = ~0ULL;
-        unsigned long long want = __ballot(!alive);
+        unsigned long long want = __ballot(!S.alive);
         while (want) {
             if (pool_lo == pool_hi) {
                 unsigned long long base = 0;
@@ -258,183 +439,30 @@ __global__ __launch_bounds__(LT ? 768 : 512) void rta_history_kernel(RtaParams P
                 pool_lo = base;
                 pool_hi = base + 64;
             }
-            const unsigned long long lt = want & ((1ULL << (threadIdx.x & 63)) - 1ULL);
-            const unsigned long long rank = (unsigned long long)__popcll(lt);
+            const unsigned long long below = want & ((1ULL << (threadIdx.x & 63)) - 1ULL);
+            const unsigned long long rank = (unsigned long long)__popcll(below);
             const unsigned long long take = min((unsigned long long)__popcll(want), pool_hi - pool_lo);
             const bool mine = (want >> (threadIdx.x & 63)) & 1ULL;
             if (mine && rank < take) pid = pool_lo + rank;
             pool_lo += take;
-            want = __ballot(!alive && pid == ~0ULL);
+            want = __ballot(!S.alive && pid == ~0ULL);
         }
-        if (!alive) {
-            budget = 0;
-            still = 0;
-            tlast = -1.0;
+        if (!S.alive) {
             if (pid >= P.total) break;
-            run = find_run(P, pid);
-            const RtaRunDev& R = P.runs[run];
-            rng.seed(R.seed, pid - R.first);
-            wscale = R.wscale;
-            if (GRAD) score[0] = score[1] = score[2] = score[3] = 0.f;
-            // Init_generator (RTA_MC2D.cpp:436-484)
-            bool ng;
-            int gi = R.gen0 + (int)alias_pick(P.gen_sel + R.gsel0, (uint32_t)R.ngen, rng, ng);
-            const RtaGen g = P.gens[gi];
-            double u = rng.uniform();
-            x = g.p1x + u * g.ndx;
-            y = g.p1y + u * g.ndy;
-            const DevBox bi = P.boxes[g.idi];
-            if (!box_inside(bi, x, y)) {
-                double dx = 0.5 * (bi.xmin + bi.xmax) - x, dy = 0.5 * (bi.ymin + bi.ymax) - y;
-                double dn = sqrt(dx * dx + dy * dy);
-                x += 1.0e-6 * dx / dn;
-                y += 1.0e-6 * dy / dn;
-            }
-            mode = (int)alias_pick(P.gen_tab + (size_t)gi * P.n, (uint32_t)P.n, rng, ng);
-            sign = g.sign;
-            box = g.idi;
-            t = 0.;
-            alive = true;
-            need_flight = true;
+            rta_start<GRAD>(P, S, pid);
         }
-        const double4 md = P.modes[mode];
-        if (need_flight) {
-            // RTA_MC2D.cpp:2262-2288: exponential flight, truncated at maxtime
-            if (!(t < maxtime)) { alive = false; continue; }
-            double u = 1.0 - rng.uniform();
-            dtrem = -log(u) / (md.z * wscale);
-            if (t + dtrem > maxtime) dtrem = maxtime - t + 1.0e-8;
-            if (almost_equal(dtrem, 0.)) continue;
-            need_flight = false;
-        }
-        const DevBox B = P.boxes[box];
-        if (B.kind == 2) {
-            // periodic box: translate, keep the remaining flight (RTA_MC2D.cpp:1034-1043)
-            const int nb = translate_rect(P.boxes, P.cts, P.cseg, P.ptrans, box, x, y, md.x, md.y, rng);
-            if (nb < 0) {
-                badtrans++;
-                alive = false;
-                continue;
-            }
-            box = nb;
-            continue;
-        }
-        if (B.kind == 1) {  // reservoir: absorbed (RTA_MC2D.cpp:1046-1053)
-            events++;
-            alive = false;
-            continue;
-        }
-        const int gmask = GRAD ? P.runs[run].gmask : 0;
-        const double vx = md.x, vy = md.y;
-        double nx = x + dtrem * vx, ny = y + dtrem * vy;
-        if (box_inside(B, nx, ny)) {
-            tally_segment<GRAD, LT>(P, lt, box, t, t + dtrem, sign, vx, vy, gmask, score, mode);
-            x = nx;
-            y = ny;
-            t += dtrem;
-            if (!P.runs[run].ballistic) {
-                bool ng;
-                mode = (int)alias_pick(P.scat + P.runs[run].scat0, (uint32_t)P.n, rng, ng);
-            }
-            events++;
-            need_flight = true;
-            continue;
-        }
-        int mask;
-        double ex, ey;
-        const double2 iv = P.ivel[mode];
-        double tf = box_exit(B, x, y, vx, vy, iv.x, iv.y, mask, ex, ey);
-        double left = dtrem - tf;
-        if (almost_equal(left, 0.)) left = 0.;
-        tally_segment<GRAD, LT>(P, lt, box, t, t + tf, sign, vx, vy, gmask, score, mode);
-        t += tf;
-        int side = 0;
-        const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
-        Ids4 pb{};
-        int npb;
-        if (fast >= 0) {
-            pb.a = fast;
-            npb = 1;
-        } else {
-            npb = fast == SIDE_VOID ? 0 : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
-        }
-        if (npb == 0) {
-            int s = fast == SIDE_VOID ? side
-                  : (__popc(mask) == 1 ? __ffs(mask) - 1
-                                       : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next()));
-            // specular with probability p (mirror partner of the mode), else
-            // diffuse; a draw is made only when 0 < p < 1
-            const float p = P.runs[run].spec[s];
-            bool specular = p >= 1.f;
-            if (p > 0.f && p < 1.f) {
-                specular = rng.uniform() < (double)p;
-                if (GRAD && ((gmask >> s) & 1)) {
-                    // d/dp log P(outcome): 1/p for specular, -1/(1-p) for diffuse
-                    const float d = specular ? 1.f / p : -1.f / (1.f - p);
-                    if (s == 0) score[0] += d;
-                    else if (s == 1) score[1] += d;
-                    else if (s == 2) score[2] += d;
-                    else score[3] += d;
-                }
-            }
-            if (specular) {
-                const int partner = P.mirror[(s == 0 || s == 3 ? 0 : (size_t)P.n) + mode];
-                if (partner < 0) {
-                    badmirror++;
-                    alive = false;
-                    continue;
-                }
-                mode = partner;
-            } else {
-                // Lambert re-emission (RTA_MC2D.cpp:635-678)
-                const int4 li = P.lam_info[mode];
-                bool ng;
-                int k = (int)alias_pick(P.lam_tab + li.z + (size_t)s * li.y, (uint32_t)li.y, rng, ng);
-                mode = P.perm[li.x + k];
-            }
-            // the remaining flight is discarded (RTA_MC2D.cpp:1259)
-            x = ex;
-            y = ey;
-            events++;
-            need_flight = true;
-            continue;
-        }
-        x = ex;
-        y = ey;
-        if (npb == 1) {
-            box = pb.a;
-        } else {
-            // several candidate boxes at a corner (RTA_MC2D.cpp:1316-1380)
-            Ids4 cand{};
-            int nc = 0;
-            double cx = ex + 1.0e-6 * vx, cy = ey + 1.0e-6 * vy;
-            for (int k = 0; k < npb && k < 4; k++)
-                if (box_inside(P.boxes[pb.get(k)], cx, cy)) cand.set(nc++, pb.get(k));
-            if (nc > 0) {
-                box = cand.get((int)rng.bounded((uint32_t)nc));
-            } else {
-                box = pb.get((int)rng.bounded((uint32_t)(npb < 4 ? npb : 4)));
-                const DevBox nb = P.boxes[box];
-                double dx = 0.5 * (nb.xmin + nb.xmax) - x, dy = 0.5 * (nb.ymin + nb.ymax) - y;
-                double dn = sqrt(dx * dx + dy * dy);
-                x += 1.0e-6 * dx / dn;
-                y += 1.0e-6 * dy / dn;
-            }
-        }
-        crossings++;
-        dtrem = left;
-        if (dtrem == 0.) need_flight = true;
+        rta_event<GRAD, LT ? 1 : 0>(P, lt, S, C);
     }
     if constexpr (LT) {
         __syncthreads();  // every history of the workgroup has ended
         lt_flush<GRAD>(P, lt);
-        lt_counters(P.counters, events, crossings, stuck, badmirror, badtrans);
+        lt_counters(P.counters, C.events, C.crossings, C.stuck, C.badmirror, C.badtrans);
     } else {
-        block_add_u64(P.counters + 0, events);
-        block_add_u64(P.counters + 1, crossings);
-        block_add_u64(P.counters + 3, stuck);
-        block_add_u64(P.counters + 4, badmirror);
-        block_add_u64(P.counters + 5, badtrans);
+        block_add_u64(P.counters + 0, C.events);
+        block_add_u64(P.counters + 1, C.crossings);
+        block_add_u64(P.counters + 3, C.stuck);
+        block_add_u64(P.counters + 4, C.badmirror);
+        block_add_u64(P.counters + 5, C.badtrans);
     }
 }
 
@@ -968,7 +996,7 @@ static void batch_set_tally(RtaBatch* b, int mode) {
 }
 
 static void batch_tune(RtaBatch* b, double fraction) {
-    if (b->tally == 1) return;  // one 768-thread workgroup per CU: the LDS holds one tally block
+    if (b->tally != 0) return;  // one 768-thread workgroup per CU: the LDS holds one tally block
     Ctx* c = b->ctx;
     hipStream_t s = c->stream;
     std::vector<RtaRunDev> tr = b->runs;

@@ -190,3 +190,4 @@ def test_fp32_lds_tallies_reject_what_does_not_fit(pkg, ctx):
     res, rep = b.execute()
     b.close()
     assert rep["events"] > 0 and np.isfinite(res[0]["T"]).all()
+
