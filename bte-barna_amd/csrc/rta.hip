@@ -96,12 +96,27 @@ __device__ inline int time_bin(const double* grid, int nt, double target) {
 // segment weighted by the particle's running score for each active side.
 // TM (tally mode) 0: FP64 device atomics. 1: the workgroup's FP32 partial
 // sums in LDS (lds), added into the FP64 tallies once at the end of the
-// kernel (lt_flush).
+// kernel (lt_flush). 2: the workgroup's FP64 partial sums of gz in LDS,
+// jx and jy by device atomics.
 template <bool GRAD, int TM>
 __device__ inline void tally_segment(const RtaParams& P, void* lds, int box, double t0, double t1,
                                      int sign, double vx, double vy, int gmask, const float* score, int mode) {
     int b0 = time_bin(P.timegrid, P.nt, t0), b1 = time_bin(P.timegrid, P.nt, t1);
     double delta = t1 - t0;
+    if constexpr (TM == 2) {
+        double* L = (double*)lds + P.box_lrow[box] * (uint32_t)P.nt;
+        const uint64_t k0 = P.box_cell[box] + (xcd_id() % RTA_NCOPY) * 3 * P.cells;
+        for (int i = b0; i <= b1; i++) {
+            double lo = fmax(P.timegrid[i] - t0, 0.);
+            double hi = fmin(P.timegrid[i + 1] - t0, delta);
+            double c = sign * fmax(hi - lo, 0.);
+            if (c == 0.) continue;
+            atomicAdd(L + i, c);
+            atomicAdd(P.jx + k0 + i, vx * c);
+            atomicAdd(P.jy + k0 + i, vy * c);
+        }
+        return;
+    }
     if constexpr (TM == 1) {
         float* lt = (float*)lds;
         const uint32_t lc = (uint32_t)P.lrows * (uint32_t)P.nt;
@@ -409,15 +424,21 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, RtaState& S, Rta
 // One lane follows one particle history at a time and takes the next id
 // from a per-wave pool when it ends; ids of all runs form one range, so a
 // batch of runs of any sizes fills the GPU in one launch.
-template <bool GRAD, bool LT>
-__global__ __launch_bounds__(LT ? 768 : 512) void rta_history_kernel(RtaParams P) {
+template <bool GRAD, int TMK>
+__global__ __launch_bounds__(TMK ? 768 : 512) void rta_history_kernel(RtaParams P) {
     float* lt = nullptr;
-    if constexpr (LT) {
+    if constexpr (TMK == 1) {
         lt = rta_lt;
         int narr = 3;
         for (int s = 0; s < 4; s++) narr += P.lt_goff[s] >= 0 ? 1 : 0;
         const uint32_t nf = (uint32_t)narr * (uint32_t)P.lrows * (uint32_t)P.nt;
         for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) lt[j] = 0.f;
+        __syncthreads();
+    } else if constexpr (TMK == 2) {
+        lt = rta_lt;
+        double* g = (double*)rta_lt;
+        const uint32_t nf = (uint32_t)P.lrows * (uint32_t)P.nt;
+        for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) g[j] = 0.;
         __syncthreads();
     }
     RtaState S;
@@ -451,11 +472,22 @@ __global__ __launch_bounds__(LT ? 768 : 512) void rta_history_kernel(RtaParams P
             if (pid >= P.total) break;
             rta_start<GRAD>(P, S, pid);
         }
-        rta_event<GRAD, LT ? 1 : 0>(P, lt, S, C);
+        rta_event<GRAD, TMK>(P, lt, S, C);
     }
-    if constexpr (LT) {
+    if constexpr (TMK == 1) {
         __syncthreads();  // every history of the workgroup has ended
         lt_flush<GRAD>(P, lt);
+        lt_counters(P.counters, C.events, C.crossings, C.stuck, C.badmirror, C.badtrans);
+    } else if constexpr (TMK == 2) {
+        __syncthreads();
+        const uint32_t nt = (uint32_t)P.nt, lc = (uint32_t)P.lrows * nt;
+        double* tz = P.gz + (xcd_id() % RTA_NCOPY) * 3 * P.cells;
+        const double* g = (const double*)lt;
+        for (uint32_t j = threadIdx.x; j < lc; j += blockDim.x) {
+            const uint32_t row = j / nt;
+            const double v = g[j];
+            if (v != 0.) atomicAdd(tz + P.lrow_cell[row] + (j - row * nt), v);
+        }
         lt_counters(P.counters, C.events, C.crossings, C.stuck, C.badmirror, C.badtrans);
     } else {
         block_add_u64(P.counters + 0, C.events);
@@ -536,7 +568,7 @@ struct RtaBatch {
     // partial sums in LDS (one 768-thread workgroup per CU), added in FP64
     int tally = 0, lt_goff[4] = {-1, -1, -1, -1};
     uint32_t lrows = 0;
-    size_t lt_bytes = 0;
+    size_t lt_bytes = 0, lg_bytes = 0;  // LDS of modes 1 and 2
     DevBuf<uint32_t> d_box_lrow;
     DevBuf<uint64_t> d_lrow_cell;
     bool use_graph = false;
@@ -679,15 +711,17 @@ static void launch_batch(RtaBatch* b, const RtaParams& P, int block, int blocks,
     if (b->nloc)
         hipLaunchKernelGGL(rta_clear_kernel, dim3(1024), dim3(256), 0, s, b->d_spec.p,
                            (uint64_t)b->nloc * (b->ntime - 1) * b->m->n, b->d_counters.p);
-    if (b->tally == 1) {
+    if (b->tally == 2) {
+        hipLaunchKernelGGL((rta_history_kernel<false, 2>), dim3(blocks), dim3(block), b->lg_bytes, s, P);
+    } else if (b->tally == 1) {
         if (b->any_grad)
-            hipLaunchKernelGGL((rta_history_kernel<true, true>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
+            hipLaunchKernelGGL((rta_history_kernel<true, 1>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
         else
-            hipLaunchKernelGGL((rta_history_kernel<false, true>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
+            hipLaunchKernelGGL((rta_history_kernel<false, 1>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
     } else if (b->any_grad) {
-        hipLaunchKernelGGL((rta_history_kernel<true, false>), dim3(blocks), dim3(block), 0, s, P);
+        hipLaunchKernelGGL((rta_history_kernel<true, 0>), dim3(blocks), dim3(block), 0, s, P);
     } else {
-        hipLaunchKernelGGL((rta_history_kernel<false, false>), dim3(blocks), dim3(block), 0, s, P);
+        hipLaunchKernelGGL((rta_history_kernel<false, 0>), dim3(blocks), dim3(block), 0, s, P);
     }
     // the per-XCD copies into copy 0
     hipLaunchKernelGGL(rta_sum_copies_kernel, dim3(1024), dim3(256), 0, s, b->d_tally.p, 3 * b->cells);
@@ -836,6 +870,7 @@ static void batch_create(RtaBatch* b) {
     for (int k = 0; k < 4; k++)
         if ((gunion >> k) & 1) b->lt_goff[k] = narr++ * (int)(b->lrows * (uint32_t)nt);
     b->lt_bytes = std::max<size_t>((size_t)narr * b->lrows * nt * sizeof(float), 16 * 5 * sizeof(uint64_t));
+    b->lg_bytes = std::max<size_t>((size_t)b->lrows * nt * sizeof(double), 16 * 5 * sizeof(uint64_t));
     b->d_gens.upload(gens, s);
     b->d_gen_sel.upload(gsel, s);
     b->d_gen_tab.upload(gtab, s);
@@ -971,18 +1006,30 @@ static void batch_execute(RtaBatch* b, double* T_out, double* jx_out, double* jy
 // CU) and keeps the fastest.
 // FP32 LDS tallies on (mode 1) or off (mode 0); see bte_rta_batch_set_tally.
 static void batch_set_tally(RtaBatch* b, int mode) {
-    if (mode != 0 && mode != 1) throw Error(BTE_EINVAL, "tally mode must be 0 (FP64) or 1 (FP32 LDS)");
-    if (mode == 1) {
+    if (mode < 0 || mode > 2)
+        throw Error(BTE_EINVAL, "tally mode must be 0 (FP64), 1 (FP32 LDS) or 2 (FP64 T in LDS)");
+    if (mode == 2) {
+        if (b->nloc) throw Error(BTE_EINVAL, "LDS tallies do not cover spectral locations");
+        if (b->any_grad) throw Error(BTE_EINVAL, "tally mode 2 does not carry dT/dp; use mode 1 or 0");
+        if (b->lg_bytes > b->ctx->lds_per_block)
+            throw Error(BTE_EINVAL, "FP64 LDS tallies of T need " + std::to_string(b->lg_bytes) +
+                                        " bytes of LDS, the device has " + std::to_string(b->ctx->lds_per_block) +
+                                        " per workgroup");
+        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&rta_history_kernel<false, 2>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lg_bytes));
+        b->block = 768;
+        b->blocks = b->ctx->cu_count;
+    } else if (mode == 1) {
         if (b->nloc) throw Error(BTE_EINVAL, "FP32 LDS tallies do not cover spectral locations");
         if (b->lt_bytes > b->ctx->lds_per_block)
             throw Error(BTE_EINVAL, "FP32 LDS tallies need " + std::to_string(b->lt_bytes) + " bytes of LDS, the device has " +
                                         std::to_string(b->ctx->lds_per_block) + " per workgroup");
-        const void* k = b->any_grad ? reinterpret_cast<const void*>(&rta_history_kernel<true, true>)
-                                    : reinterpret_cast<const void*>(&rta_history_kernel<false, true>);
+        const void* k = b->any_grad ? reinterpret_cast<const void*>(&rta_history_kernel<true, 1>)
+                                    : reinterpret_cast<const void*>(&rta_history_kernel<false, 1>);
         HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lt_bytes));
         b->block = 768;
         b->blocks = b->ctx->cu_count;
-    } else if (b->tally == 1) {
+    } else if (b->tally != 0) {
         b->block = 256;
         b->blocks = b->ctx->cu_count * 8;
     }
@@ -1127,6 +1174,12 @@ int bte_rta_execute(bte_rta* rr, double* T_out, double* jx_out, double* jy_out, 
     BTE_GUARD_BEGIN
     auto* b = reinterpret_cast<RtaBatch*>(rr);
     if (!b || !T_out || !jx_out || !jy_out) throw Error(BTE_EINVAL, "null argument");
+    HIPCHECK(hipSetDevice(b->ctx->device));
+    // T in FP64 LDS partial sums when the run's cells fit (tally mode 2:
+    // the same histories and FP64 sums, a third of the device atomics);
+    // device atomics otherwise and with spectral locations
+    const int want = (!b->nloc && !b->any_grad && b->lg_bytes <= b->ctx->lds_per_block) ? 2 : 0;
+    if (b->tally != want) batch_set_tally(b, want);
     bte_rta_batch_report br{};
     double E = 0;
     batch_execute(b, T_out, jx_out, jy_out, nullptr, &E, &br);

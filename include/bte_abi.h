@@ -205,7 +205,10 @@ int bte_rta_batch_create(bte_ctx* ctx, const bte_modes* m, int nruns, const bte_
  * arrays in LDS (T, jx, jy and each differentiated side over the regular
  * boxes' cells) and adds them into the FP64 tallies once at the end; one
  * 768-thread workgroup per CU. BTE_EINVAL when those arrays exceed the
- * device's LDS per workgroup. */
+ * device's LDS per workgroup. mode 2: the same for T alone in FP64
+ * (regular boxes x (ntime - 1) x 8 bytes of LDS), jx and jy by FP64 device
+ * atomics: FP64 sums throughout, a third of the device atomics of mode 0.
+ * Not with dT/dp (BTE_EINVAL). */
 int bte_rta_batch_set_tally(bte_rta_batch* b, int mode);
 /* Times the batch scaled to `fraction` of each run's particles over a set of
  * launch shapes and keeps the fastest (no-op under tally mode 1). */

@@ -191,3 +191,37 @@ def test_fp32_lds_tallies_reject_what_does_not_fit(pkg, ctx):
     b.close()
     assert rep["events"] > 0 and np.isfinite(res[0]["T"]).all()
 
+
+
+def test_fp64_lds_T_tallies_follow_the_same_histories(pkg, ctx):
+    """Tally mode 2 (FP64 partial sums of T per workgroup in LDS, jx and jy by
+    device atomics) runs the same histories as mode 0: equal event and
+    crossing counts, and T, jx, jy equal up to the order of the FP64 sums,
+    also on graph replays; it refuses dT/dp before any launch."""
+    W = pkg.workloads
+    mat = W.material(300, seed=3, mirror=True)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    rect, kind, teq, _ = W.ribbon(400.0, 40.0, 40, 300.0, reservoirs=(300.5, 299.5))
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    runs = [dict(geom=geom, T0=300.0, nparticles=60000, seed=21),
+            dict(geom=geom, T0=300.0, nparticles=30000, seed=22, specularity=(0.0, 1.0, 1.0, 0.0))]
+    kw = dict(thickness=1.0, dt=5.0, maxtime=1000.0, sigma_bounce=2.0)
+    ref_b = pkg.RtaBatch(ctx, modes, runs, **kw)
+    ref, rrep = ref_b.execute()
+    lds = pkg.RtaBatch(ctx, modes, runs, use_graph=True, tally="fp64-lds-T", **kw)
+    for replay in range(2):
+        got, grep = lds.execute()
+        assert (grep["events"], grep["crossings"]) == (rrep["events"], rrep["crossings"])
+        assert grep["block"] == 768
+        for a, b in zip(got, ref):
+            for key, off in (("T", 300.0), ("jx", 0.0), ("jy", 0.0)):
+                scale = np.abs(b[key] - off).max()
+                assert scale > 0
+                np.testing.assert_allclose(a[key] - off, b[key] - off, rtol=0, atol=1e-10 * scale)
+        ref_b.execute()  # eager work between replays
+        ctx.sync()
+    ref_b.close()
+    lds.close()
+    with pytest.raises(Exception, match="dT/dp"):
+        pkg.RtaBatch(ctx, modes, [dict(runs[0], specularity=(0.0, 0.5, 0.5, 0.0), grad=True)],
+                     tally="fp64-lds-T", **kw)
