@@ -348,6 +348,23 @@ int orc_rta_spectral_get(int k, double* omega, double* dT, double* jx, double* j
     return 0;
 }
 double orc_gamma_pdf(double k, double theta, double x) { return gamma_pdf(k, theta, x); }
+// extensions of the next orc_rta_run (no reference counterpart): scattering
+// rate factor, specular probability per side (left, bottom, top, right) and
+// the sides whose dT/dp is tallied; the derivative after the run
+static thread_local double g_ext_wscale = 1.0, g_ext_spec[4] = {0., 0., 0., 0.};
+static thread_local int g_ext_gmask = 0;
+static thread_local std::vector<double> g_grad_out;
+void orc_rta_ext(double wscale, const double* spec4, int gmask) {
+    g_ext_wscale = wscale;
+    for (int k = 0; k < 4; k++) g_ext_spec[k] = spec4 ? spec4[k] : 0.;
+    g_ext_gmask = gmask;
+}
+// dT/dp of the last run: [4][nt][nbox]
+int orc_rta_grad_get(double* out) {
+    if (g_grad_out.empty()) return 1;
+    std::copy(g_grad_out.begin(), g_grad_out.end(), out);
+    return 0;
+}
 int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, const double* w,
                 double V, double lattz, int nq, int nbox, const double* rect, const int32_t* kind,
                 const double* Teq, double thickness, double T0, double dt, double maxtime,
@@ -364,6 +381,13 @@ int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, 
         S.maxtime = maxtime;
         S.ballistic = ballistic != 0;
         S.sigma = sigma;
+        S.wscale = g_ext_wscale;
+        for (int k = 0; k < 4; k++) S.spec[k] = g_ext_spec[k];
+        S.gmask = g_ext_gmask;
+        g_ext_wscale = 1.0;
+        for (int k = 0; k < 4; k++) g_ext_spec[k] = 0.;
+        g_ext_gmask = 0;
+        if (!(S.wscale > 0.)) throw input_error("wscale must be positive");
         S.spec_boxes.swap(g_spec_boxes);
         S.spec_sigma.swap(g_spec_sigma);
         S.nomega = g_spec_nomega;
@@ -377,6 +401,7 @@ int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, 
         g_spec_nt = S.ntime - 1;
         g_spec_nq = S.m.nq;
         g_spec_nw = S.nomega;
+        g_grad_out.swap(S.grad_out);
         return 0;
     } catch (const std::exception& e) {
         g_err = e.what();

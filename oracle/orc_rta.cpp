@@ -15,6 +15,15 @@
 // [0,size) (RTA_MC2D.cpp:1234-1235 is inclusive); the dead-particle sentinel
 // is an explicit flag instead of mode (0,0); the Lambert broadening uses a
 // constant sigma per mode because a synthetic table has no q-grid.
+// Three extensions have no reference counterpart (the reference's void sides
+// are diffuse only, RTA_MC2D.cpp:1203-1260); they follow their definitions:
+// a scattering-rate factor wscale on every w; at side s a specular
+// reflection with probability spec[s] (the mode goes to its mirror partner,
+// v_x negated on the left and right sides, v_y on the bottom and top),
+// diffuse otherwise; and the score-function derivative of every tally with
+// respect to spec[s]: each particle carries sum over its wall events on s of
+// d/dp log P(outcome) (1/p specular, -1/(1-p) diffuse), and every segment
+// adds contr * score[s] to a per-side tally.
 #include "orc_geometry.hpp"
 #include <limits>
 #include <omp.h>
@@ -73,6 +82,11 @@ struct RtaSetup {
     double thickness = 1.0, T0 = 300.0, dt = 1.0, maxtime = 10.0;
     bool ballistic = false;
     double sigma = 0.25;  // Lambert broadening per mode [rad/ps]
+    double wscale = 1.0;  // scattering-rate factor
+    double spec[4] = {0., 0., 0., 0.};  // specular probability: left, bottom, top, right
+    int gmask = 0;        // sides whose specularity is differentiated (0 < p < 1)
+    std::vector<int> mirror;  // [2][n]: v_x negated, v_y negated; -1 none
+    std::vector<double> grad_out;  // dT/dp [side][tbin][box] of the last run
     // <spectral> (RTA_MC2D.cpp:1930-1945, 1978-2052): boxes with a spectral
     // decomposition, nomega ticks, per-mode broadening (base_sigma; empty:
     // sigma for every mode)
@@ -170,6 +184,45 @@ struct RtaSetup {
         Eeff = einfo1 / (double)nparticles;
         // Lambert table (RTA_MC2D.cpp:556-626) with sigma_i = sigma for all modes
         lambert = lambert_table(m, sigma);
+        // mirror partners: same omega, one velocity component negated, exact match
+        mirror.assign(2 * (size_t)m.n, -1);
+        bool anyspec = false;
+        for (int k = 0; k < 4; k++) {
+            if (!(spec[k] >= 0. && spec[k] <= 1.)) throw input_error("specularity must lie in [0, 1]");
+            anyspec = anyspec || spec[k] > 0.;
+            if (!(spec[k] > 0. && spec[k] < 1.)) gmask &= ~(1 << k);
+        }
+        if (anyspec) {
+            std::map<std::tuple<double, double, double>, int> idx;
+            for (int i = 0; i < m.n; i++) idx.emplace(std::make_tuple(m.omega[i], m.vx[i], m.vy[i]), i);
+            for (int i = 0; i < m.n; i++) {
+                auto a = idx.find(std::make_tuple(m.omega[i], -m.vx[i], m.vy[i]));
+                auto b = idx.find(std::make_tuple(m.omega[i], m.vx[i], -m.vy[i]));
+                if (a != idx.end()) mirror[i] = a->second;
+                if (b != idx.end()) mirror[(size_t)m.n + i] = b->second;
+            }
+        }
+    }
+
+    // side index of a rectangle side from its outward normal
+    static int side_of(double npx, double npy) {
+        return npx < 0. ? 0 : (npy < 0. ? 1 : (npy > 0. ? 2 : 3));
+    }
+
+    // a void side: specular with probability spec[s], else the diffuse bounce;
+    // the score of a differentiated side gets d/dp log P(outcome)
+    int reflect(double npx, double npy, int mode, pcg32& rng, double* score) const {
+        const int s = side_of(npx, npy);
+        const double p = spec[s];
+        bool specular = p >= 1.;
+        if (p > 0. && p < 1.) {
+            specular = rng.uniform() < p;
+            if (score && ((gmask >> s) & 1)) score[s] += specular ? 1. / p : -1. / (1. - p);
+        }
+        if (!specular) return bounce(npx, npy, mode, rng);
+        const int f = mirror[(s == 0 || s == 3 ? 0 : (size_t)m.n) + mode];
+        if (f < 0) throw input_error("specular reflection of a mode without a mirror partner");
+        return f;
     }
 
     int bounce(double npx, double npy, int mode, pcg32& rng) const {
@@ -181,11 +234,13 @@ struct Part {
     double x, y, t;
     int mode, box, sign;
     bool dead;
+    double score[4] = {0., 0., 0., 0.};
 };
 
 struct Tally {
     int nbox, nt;
     std::vector<double> gz, jx, jy;  // [box][tbin]
+    std::vector<double> dgz;         // [side][box][tbin], score-weighted gz
     // spectral: [loc][tbin][omega] and [loc][tbin][q]
     std::vector<double> gzw, jxw, jyw, fdq;
     Tally(int nb, int nt_) : nbox(nb), nt(nt_), gz(nb * nt_, 0.), jx(nb * nt_, 0.), jy(nb * nt_, 0.) {}
@@ -214,7 +269,8 @@ static inline int bin_index(const std::vector<double>& grid, double target) {
 // process_segment, transient overload (RTA_MC2D.cpp:699-769), with the
 // Gamma-broadened spectral tallies of a spectral location (718-766)
 static void process_segment(const RtaSetup& S, Tally& T, int box, double t0,
-                            double t1, int sign, double vx, double vy, int mode) {
+                            double t1, int sign, double vx, double vy, int mode,
+                            const double* score = nullptr) {
     int b0 = bin_index(S.timegrid, t0), b1 = bin_index(S.timegrid, t1);
     double delta = t1 - t0;
     const int loc = S.spec_loc.empty() ? -1 : S.spec_loc[box];
@@ -232,6 +288,9 @@ static void process_segment(const RtaSetup& S, Tally& T, int box, double t0,
         T.gz[box * T.nt + i] += contr;
         T.jx[box * T.nt + i] += vx * contr;
         T.jy[box * T.nt + i] += vy * contr;
+        if (S.gmask && score)
+            for (int q = 0; q < 4; q++)
+                if ((S.gmask >> q) & 1) T.dgz[((size_t)q * T.nbox + box) * T.nt + i] += contr * score[q];
         if (loc >= 0) {
             const size_t o = ((size_t)loc * T.nt + i) * S.nomega;
             for (int w = 0; w < S.nomega; w++) {
@@ -287,7 +346,7 @@ static void evolparticle(const RtaSetup& S, Part& p, double vx, double vy,
     if (box.inside(nx, ny)) {
         p.x = nx;
         p.y = ny;
-        process_segment(S, T, p.box, p.t, p.t + dt, p.sign, vx, vy, p.mode);
+        process_segment(S, T, p.box, p.t, p.t + dt, p.sign, vx, vy, p.mode, p.score);
         p.t += dt;
         if (!S.ballistic) p.mode = (int)S.sampler.sample(rng);
         C.events++;
@@ -321,7 +380,7 @@ static void evolparticle(const RtaSetup& S, Part& p, double vx, double vy,
             pboxes.push_back(c);
     }
     if (pboxes.empty()) {
-        process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode);
+        process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode, p.score);
         p.t += tf;
         int bid = p.box;
         std::vector<int> cids;
@@ -353,7 +412,7 @@ static void evolparticle(const RtaSetup& S, Part& p, double vx, double vy,
             iborder = bids[0];
         }
         auto& ub = sys[p.box].borders[iborder];
-        p.mode = S.bounce(ub.npx, ub.npy, p.mode, rng);
+        p.mode = S.reflect(ub.npx, ub.npy, p.mode, rng, p.score);
         p.x = rx;
         p.y = ry;
         C.events++;
@@ -363,13 +422,13 @@ static void evolparticle(const RtaSetup& S, Part& p, double vx, double vy,
     if (pboxes.size() == 1) {
         p.x = rx;
         p.y = ry;
-        process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode);
+        process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode, p.score);
         p.t += tf;
         p.box = pboxes[0];
         C.crossings++;
         return;
     }
-    process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode);
+    process_segment(S, T, p.box, p.t, p.t + tf, p.sign, vx, vy, p.mode, p.score);
     p.t += tf;
     p.x = rx;
     p.y = ry;
@@ -405,6 +464,8 @@ int rta_run(RtaSetup& S, uint64_t nparticles, uint64_t seed, int nthreads,
     const size_t nloc = S.spec_boxes.size();
     if (nloc)
         for (auto& t : tallies) t.spectral(nloc, S.nomega, S.m.nq);
+    if (S.gmask)
+        for (auto& t : tallies) t.dgz.assign((size_t)4 * nb * nt, 0.);
     std::vector<Counters> cnt(nth);
     const double maxtime = S.maxtime;
     std::string err;
@@ -418,7 +479,7 @@ int rta_run(RtaSetup& S, uint64_t nparticles, uint64_t seed, int nthreads,
         while (time < maxtime) {
             double vx, vy;
             get_v(S.m, p.mode, vx, vy);
-            double w = S.m.w[p.mode];
+            double w = S.m.w[p.mode] * S.wscale;
             if (w == 0.) throw value_error("invalid scattering rate");
             double u = 1.0 - rng.uniform();
             double dt_ = -std::log(u) / w;
@@ -434,6 +495,22 @@ int rta_run(RtaSetup& S, uint64_t nparticles, uint64_t seed, int nthreads,
         }
     }
     if (!err.empty()) throw geometry_error(err);
+    // dT/dp [side][tbin][box]: the score-weighted tallies post-processed like T
+    S.grad_out.assign((size_t)4 * nt * nb, 0.);
+    if (S.gmask) {
+        const double vcorr = S.thickness / S.m.lattz;
+        for (int q = 0; q < 4; q++)
+            for (int b = 0; b < nb; b++) {
+                const double vbox = S.sys[b].area * S.thickness;
+                double acc = 0.;
+                for (int k = 0; k < nt; k++) {
+                    double v = 0.;
+                    for (int t = 0; t < nth; t++) v += tallies[t].dgz[((size_t)q * nb + b) * nt + k];
+                    acc += v * S.Eeff / vbox;
+                    S.grad_out[((size_t)q * nt + k) * nb + b] = acc / (S.cv / vcorr);
+                }
+            }
+    }
     std::vector<double> gz(nb * nt, 0.), jx(nb * nt, 0.), jy(nb * nt, 0.);
     Counters tot;
     for (int t = 0; t < nth; t++) {

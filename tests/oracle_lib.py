@@ -188,10 +188,13 @@ def set_periodic(nbox, periodic):
 
 
 def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, ballistic=False, sigma=0.25,
-            nthreads=0, periodic=None, spectral=None):
+            nthreads=0, periodic=None, spectral=None, wscale=1.0, specularity=None, grad=False):
     """spectral: {"boxes": [...], "nomega": K, "sigma": per-mode array or
     None}; the result then holds "spectral": one dict per location with
-    omega, dT, jx, jy [nt][K] and fd [nt][nq]"""
+    omega, dT, jx, jy [nt][K] and fd [nt][nq]. wscale multiplies every
+    scattering rate; specularity (left, bottom, top, right) is the specular
+    probability of each side; grad adds "dTdp" [4][nt][nbox] for the sides
+    with 0 < p < 1 (zeros elsewhere)."""
     rect, Teq = _f(rect), _f(Teq)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
@@ -211,6 +214,8 @@ def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, 
     cnt = (C.c_uint64 * 3)()
     eeff = C.c_double()
     om, vx, vy, w = (_f(mat[k]) for k in ("omega", "vx", "vy", "w"))
+    sp = _f(specularity if specularity is not None else (0.0, 0.0, 0.0, 0.0))
+    lib().orc_rta_ext(C.c_double(wscale), _dp(sp), 15 if grad else 0)
     rc = lib().orc_rta_run(len(om), _dp(om), _dp(vx), _dp(vy), _dp(w), C.c_double(mat["V"]),
                            C.c_double(mat.get("lattz", 1.0)), nq, nbox, _dp(rect), kind.ctypes.data_as(ip32),
                            _dp(Teq), C.c_double(thickness), C.c_double(T0), C.c_double(dt), C.c_double(maxtime),
@@ -220,6 +225,11 @@ def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, 
         raise RuntimeError(lib().orc_last_error().decode())
     out = {"T": T, "jx": jx, "jy": jy, "events": cnt[0], "crossings": cnt[1], "nudges": cnt[2], "raw_gz": raw,
            "Eeff": eeff.value}
+    if grad:
+        g = np.zeros((4, nt, nbox))
+        if lib().orc_rta_grad_get(_dp(g)) != 0:
+            raise RuntimeError("oracle returned no dT/dp")
+        out["dTdp"] = g
     if spectral:
         K = int(spectral["nomega"])
         out["spectral"] = []
