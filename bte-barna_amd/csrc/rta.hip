@@ -73,6 +73,12 @@ struct RtaParams {
     const uint64_t* lrow_cell;
     int32_t lrows, pad1;
     int32_t lt_goff[4];
+    // tally mode 2: the LDS holds FP64 gz, jx, jy of the first lbins time
+    // bins of every LDS row ([3][lrows][lbins]); each workgroup writes its
+    // block to part[blockIdx.x] at the end
+    int32_t lbins, pad2;
+    double* part;
+    double idt;  // nt / maxtime: time -> bin estimate on the uniform grid
 };
 
 // Per-XCD copies of the f64 tallies. Device-wide f64 atomics to a few
@@ -81,39 +87,90 @@ struct RtaParams {
 #define RTA_NCOPY 8
 // the XCD (XCC) this wave runs on, from the hardware register HW_REG_XCC_ID
 
-__device__ inline int time_bin(const double* grid, int nt, double target) {
-    // get_bin_index (RTA_MC2D.cpp:140-148): lower_bound - 1 clamped to [0, nt-1]
-    int lo = 0, hi = nt + 1;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (grid[mid] < target) lo = mid + 1; else hi = mid;
+// get_bin_index (RTA_MC2D.cpp:140-148): lower_bound - 1 clamped to
+// [0, nt-1]. The grid is uniform (t_i = i maxtime / nt), so the bin is
+// estimated from target * nt / maxtime and corrected against the grid
+// values themselves, which keeps lower_bound's result exactly (a target
+// on a grid point t_j belongs to bin j - 1).
+__device__ inline int time_bin(const double* grid, int nt, double idt, double target) {
+    const double e = target * idt;
+    int k = e < 0. ? 0 : (e > (double)nt ? nt : (int)e);
+    while (k > 0 && grid[k] >= target) k--;
+    while (k < nt && grid[k + 1] < target) k++;
+    return k > nt - 1 ? nt - 1 : k;
+}
+
+// A lane's running sum of consecutive contributions to one tally cell
+// (box, time bin); it is added to the tallies when the lane's next
+// contribution falls in another cell and when the lane's histories end.
+// Successive segments of a history often share a cell (several flights in
+// one box within one time bin), and each cell update costs three adds.
+struct TallyAcc {
+    int box = -1, bin = 0;
+    double g = 0., x = 0., y = 0.;
+};
+
+// Adds the accumulator into the tallies: under tally mode 2 the first
+// lbins time bins of a box go to the workgroup's FP64 LDS block, later
+// bins (and every bin under mode 0) to the XCD's copy by device atomics.
+template <int TM>
+__device__ inline void acc_flush(const RtaParams& P, double* L, TallyAcc& a) {
+    if (a.box < 0) return;
+    if (TM == 2 && a.bin < P.lbins) {
+        const uint32_t lc = (uint32_t)P.lrows * (uint32_t)P.lbins;
+        const uint32_t i = P.box_lrow[a.box] * (uint32_t)P.lbins + (uint32_t)a.bin;
+        atomicAdd(L + i, a.g);
+        atomicAdd(L + lc + i, a.x);
+        atomicAdd(L + 2 * lc + i, a.y);
+    } else {
+        const uint64_t k = P.box_cell[a.box] + (uint64_t)a.bin + (uint64_t)(xcd_id() % RTA_NCOPY) * 3 * P.cells;
+        atomicAdd(P.gz + k, a.g);
+        atomicAdd(P.jx + k, a.x);
+        atomicAdd(P.jy + k, a.y);
     }
-    int b = lo - 1;
-    return b < 0 ? 0 : (b > nt - 1 ? nt - 1 : b);
+    a.box = -1;
 }
 
 // process_segment (RTA_MC2D.cpp:699-769); gradient runs also add the
 // segment weighted by the particle's running score for each active side.
 // TM (tally mode) 0: FP64 device atomics. 1: the workgroup's FP32 partial
 // sums in LDS (lds), added into the FP64 tallies once at the end of the
-// kernel (lt_flush). 2: the workgroup's FP64 partial sums of gz in LDS,
-// jx and jy by device atomics.
+// kernel (lt_flush). 2: the workgroup's FP64 partial sums of gz, jx and jy
+// for the first lbins time bins in LDS, later bins by device atomics.
+// Modes 0 (without dT/dp) and 2 merge consecutive contributions to one cell
+// in the lane's accumulator.
 template <bool GRAD, int TM>
-__device__ inline void tally_segment(const RtaParams& P, void* lds, int box, double t0, double t1,
+__device__ inline void tally_segment(const RtaParams& P, void* lds, TallyAcc& acc, int box, double t0, double t1,
                                      int sign, double vx, double vy, int gmask, const float* score, int mode) {
-    int b0 = time_bin(P.timegrid, P.nt, t0), b1 = time_bin(P.timegrid, P.nt, t1);
+    int b0 = time_bin(P.timegrid, P.nt, P.idt, t0), b1 = time_bin(P.timegrid, P.nt, P.idt, t1);
     double delta = t1 - t0;
-    if constexpr (TM == 2) {
-        double* L = (double*)lds + P.box_lrow[box] * (uint32_t)P.nt;
-        const uint64_t k0 = P.box_cell[box] + (xcd_id() % RTA_NCOPY) * 3 * P.cells;
+    if constexpr (TM == 2 || (TM == 0 && !GRAD)) {
+        // spectral locations tally per mode; the Gamma-broadened spectra are
+        // one matrix product after the run (the reference adds the broadened
+        // weights of every segment, RTA_MC2D.cpp:718-766)
+        double* sp = nullptr;
+        if (TM == 0 && P.spec) {
+            const int l = P.box_loc[box];
+            if (l >= 0) sp = P.spec + (size_t)l * P.nt * P.n + mode;
+        }
         for (int i = b0; i <= b1; i++) {
             double lo = fmax(P.timegrid[i] - t0, 0.);
             double hi = fmin(P.timegrid[i + 1] - t0, delta);
             double c = sign * fmax(hi - lo, 0.);
             if (c == 0.) continue;
-            atomicAdd(L + i, c);
-            atomicAdd(P.jx + k0 + i, vx * c);
-            atomicAdd(P.jy + k0 + i, vy * c);
+            if (TM == 0 && sp) atomicAdd(sp + (size_t)i * P.n, c);
+            if (box == acc.box && i == acc.bin) {
+                acc.g += c;
+                acc.x += vx * c;
+                acc.y += vy * c;
+            } else {
+                acc_flush<TM>(P, (double*)lds, acc);
+                acc.box = box;
+                acc.bin = i;
+                acc.g = c;
+                acc.x = vx * c;
+                acc.y = vy * c;
+            }
         }
         return;
     }
@@ -284,7 +341,7 @@ __device__ inline void rta_watch(const RtaParams& P, RtaState& S, RtaCounts& C) 
 // One step of evolparticle (RTA_MC2D.cpp:990-1386) after the flight draw of
 // the history loop (2262-2288).
 template <bool GRAD, int TM>
-__device__ inline void rta_event(const RtaParams& P, void* lds, RtaState& S, RtaCounts& C) {
+__device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, RtaState& S, RtaCounts& C) {
     const double4 md = P.modes[S.mode];
     if (S.need_flight) {
         // exponential flight, truncated at maxtime
@@ -326,7 +383,7 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, RtaState& S, Rta
         tf = box_exit(B, S.x, S.y, vx, vy, iv.x, iv.y, mask, ex, ey);
     }
     if (inside) {
-        tally_segment<GRAD, TM>(P, lds, S.box, S.t, S.t + S.dtrem, S.sign, vx, vy, gmask, S.score, S.mode);
+        tally_segment<GRAD, TM>(P, lds, A, S.box, S.t, S.t + S.dtrem, S.sign, vx, vy, gmask, S.score, S.mode);
         S.x = nx;
         S.y = ny;
         S.t += S.dtrem;
@@ -340,7 +397,7 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, RtaState& S, Rta
     }
     double left = S.dtrem - tf;
     if (almost_equal(left, 0.)) left = 0.;
-    tally_segment<GRAD, TM>(P, lds, S.box, S.t, S.t + tf, S.sign, vx, vy, gmask, S.score, S.mode);
+    tally_segment<GRAD, TM>(P, lds, A, S.box, S.t, S.t + tf, S.sign, vx, vy, gmask, S.score, S.mode);
     S.t += tf;
     int side = 0;
     const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
@@ -437,10 +494,11 @@ __global__ __launch_bounds__(TMK ? 768 : 512) void rta_history_kernel(RtaParams 
     } else if constexpr (TMK == 2) {
         lt = rta_lt;
         double* g = (double*)rta_lt;
-        const uint32_t nf = (uint32_t)P.lrows * (uint32_t)P.nt;
+        const uint32_t nf = 3u * (uint32_t)P.lrows * (uint32_t)P.lbins;
         for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) g[j] = 0.;
         __syncthreads();
     }
+    TallyAcc A;
     RtaState S;
     S.alive = 0;
     RtaCounts C;
@@ -472,24 +530,24 @@ __global__ __launch_bounds__(TMK ? 768 : 512) void rta_history_kernel(RtaParams 
             if (pid >= P.total) break;
             rta_start<GRAD>(P, S, pid);
         }
-        rta_event<GRAD, TMK>(P, lt, S, C);
+        rta_event<GRAD, TMK>(P, lt, A, S, C);
     }
     if constexpr (TMK == 1) {
         __syncthreads();  // every history of the workgroup has ended
         lt_flush<GRAD>(P, lt);
         lt_counters(P.counters, C.events, C.crossings, C.stuck, C.badmirror, C.badtrans);
     } else if constexpr (TMK == 2) {
-        __syncthreads();
-        const uint32_t nt = (uint32_t)P.nt, lc = (uint32_t)P.lrows * nt;
-        double* tz = P.gz + (xcd_id() % RTA_NCOPY) * 3 * P.cells;
+        acc_flush<2>(P, (double*)lt, A);
+        __syncthreads();  // every lane has added its last cell
+        // the workgroup's block, stored whole (rta_sum_parts_kernel adds the
+        // blocks of all workgroups)
+        const uint32_t nf = 3u * (uint32_t)P.lrows * (uint32_t)P.lbins;
         const double* g = (const double*)lt;
-        for (uint32_t j = threadIdx.x; j < lc; j += blockDim.x) {
-            const uint32_t row = j / nt;
-            const double v = g[j];
-            if (v != 0.) atomicAdd(tz + P.lrow_cell[row] + (j - row * nt), v);
-        }
+        double* dst = P.part + (size_t)blockIdx.x * nf;
+        for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) dst[j] = g[j];
         lt_counters(P.counters, C.events, C.crossings, C.stuck, C.badmirror, C.badtrans);
     } else {
+        if constexpr (!GRAD) acc_flush<0>(P, nullptr, A);
         block_add_u64(P.counters + 0, C.events);
         block_add_u64(P.counters + 1, C.crossings);
         block_add_u64(P.counters + 3, C.stuck);
@@ -516,6 +574,22 @@ __global__ __launch_bounds__(256) void rta_spec_cumsum_kernel(double* S, int nlo
     for (int t = 0; t < nt; t++) {
         acc += p[(size_t)t * n];
         p[(size_t)t * n] = acc;
+    }
+}
+
+// Tally mode 2: the LDS blocks of all workgroups, part[blk][3][lrows][lbins],
+// summed per cell and added into copy 0 of the tallies (cells of the first
+// lbins time bins receive no device atomics, so the add is a plain store)
+__global__ __launch_bounds__(256) void rta_sum_parts_kernel(double* tally, const double* part, int nblk,
+                                                            uint32_t lrows, uint32_t lbins,
+                                                            const uint64_t* lrow_cell, uint64_t cells) {
+    const uint32_t lc = lrows * lbins, nf = 3u * lc;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nf; j += gridDim.x * blockDim.x) {
+        double v = 0.;
+        for (int b = 0; b < nblk; b++) v += part[(size_t)b * nf + j];
+        const uint32_t a = j / lc, r = j - a * lc, row = r / lbins;
+        double* dst = tally + (uint64_t)a * cells + lrow_cell[row] + (r - row * lbins);
+        *dst += v;
     }
 }
 
@@ -565,12 +639,15 @@ struct RtaBatch {
     DevBuf<double> d_spec, d_specw, d_specout;
     int block = 256, blocks = 0;
     // tally: 0 FP64 device atomics into per-XCD copies; 1 FP32 per-workgroup
-    // partial sums in LDS (one 768-thread workgroup per CU), added in FP64
+    // partial sums in LDS (one 768-thread workgroup per CU), added in FP64;
+    // 2 FP64 per-workgroup partial sums of the first lbins time bins in LDS
+    // (one 768-thread workgroup per CU), later bins by device atomics
     int tally = 0, lt_goff[4] = {-1, -1, -1, -1};
-    uint32_t lrows = 0;
+    uint32_t lrows = 0, lbins = 0;
     size_t lt_bytes = 0, lg_bytes = 0;  // LDS of modes 1 and 2
     DevBuf<uint32_t> d_box_lrow;
     DevBuf<uint64_t> d_lrow_cell;
+    DevBuf<double> d_part;  // mode 2: [blocks][3][lrows][lbins]
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -699,6 +776,9 @@ static RtaParams batch_params(RtaBatch* b, const RtaRunDev* runs, uint64_t total
     P.lrow_cell = b->d_lrow_cell.p;
     P.lrows = (int32_t)b->lrows;
     for (int k = 0; k < 4; k++) P.lt_goff[k] = b->lt_goff[k];
+    P.lbins = (int32_t)b->lbins;
+    P.part = b->d_part.p;
+    P.idt = (double)(b->ntime - 1) / b->maxtime;
     return P;
 }
 
@@ -713,6 +793,9 @@ static void launch_batch(RtaBatch* b, const RtaParams& P, int block, int blocks,
                            (uint64_t)b->nloc * (b->ntime - 1) * b->m->n, b->d_counters.p);
     if (b->tally == 2) {
         hipLaunchKernelGGL((rta_history_kernel<false, 2>), dim3(blocks), dim3(block), b->lg_bytes, s, P);
+        const uint32_t nf = 3u * b->lrows * b->lbins;
+        hipLaunchKernelGGL(rta_sum_parts_kernel, dim3(std::min<uint32_t>((nf + 255) / 256, 4096u)), dim3(256), 0, s,
+                           b->d_tally.p, b->d_part.p, blocks, b->lrows, b->lbins, b->d_lrow_cell.p, b->cells);
     } else if (b->tally == 1) {
         if (b->any_grad)
             hipLaunchKernelGGL((rta_history_kernel<true, 1>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
@@ -870,7 +953,11 @@ static void batch_create(RtaBatch* b) {
     for (int k = 0; k < 4; k++)
         if ((gunion >> k) & 1) b->lt_goff[k] = narr++ * (int)(b->lrows * (uint32_t)nt);
     b->lt_bytes = std::max<size_t>((size_t)narr * b->lrows * nt * sizeof(float), 16 * 5 * sizeof(uint64_t));
-    b->lg_bytes = std::max<size_t>((size_t)b->lrows * nt * sizeof(double), 16 * 5 * sizeof(uint64_t));
+    // mode 2: as many leading time bins of every LDS row as the workgroup's
+    // LDS holds in FP64 for gz, jx and jy (the early bins carry most of the
+    // segments: particles enter at t = 0 and leave through the reservoirs)
+    b->lbins = b->lrows ? (uint32_t)std::min<size_t>((size_t)nt, c->lds_per_block / (3 * sizeof(double) * b->lrows)) : 0;
+    b->lg_bytes = std::max<size_t>((size_t)3 * b->lrows * b->lbins * sizeof(double), 16 * 5 * sizeof(uint64_t));
     b->d_gens.upload(gens, s);
     b->d_gen_sel.upload(gsel, s);
     b->d_gen_tab.upload(gtab, s);
@@ -1011,14 +1098,16 @@ static void batch_set_tally(RtaBatch* b, int mode) {
     if (mode == 2) {
         if (b->nloc) throw Error(BTE_EINVAL, "LDS tallies do not cover spectral locations");
         if (b->any_grad) throw Error(BTE_EINVAL, "tally mode 2 does not carry dT/dp; use mode 1 or 0");
-        if (b->lg_bytes > b->ctx->lds_per_block)
-            throw Error(BTE_EINVAL, "FP64 LDS tallies of T need " + std::to_string(b->lg_bytes) +
-                                        " bytes of LDS, the device has " + std::to_string(b->ctx->lds_per_block) +
-                                        " per workgroup");
+        if (b->lbins == 0)
+            throw Error(BTE_EINVAL, "FP64 LDS tallies need " + std::to_string(3 * sizeof(double) * b->lrows) +
+                                        " bytes of LDS for one time bin, the device has " +
+                                        std::to_string(b->ctx->lds_per_block) + " per workgroup");
         HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&rta_history_kernel<false, 2>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lg_bytes));
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->ctx->lds_per_block));
         b->block = 768;
         b->blocks = b->ctx->cu_count;
+        const size_t np = (size_t)b->blocks * 3 * b->lrows * b->lbins;
+        if (b->d_part.n < np) b->d_part.alloc(np);  // kept: a captured graph holds its address
     } else if (mode == 1) {
         if (b->nloc) throw Error(BTE_EINVAL, "FP32 LDS tallies do not cover spectral locations");
         if (b->lt_bytes > b->ctx->lds_per_block)
@@ -1026,7 +1115,7 @@ static void batch_set_tally(RtaBatch* b, int mode) {
                                         std::to_string(b->ctx->lds_per_block) + " per workgroup");
         const void* k = b->any_grad ? reinterpret_cast<const void*>(&rta_history_kernel<true, 1>)
                                     : reinterpret_cast<const void*>(&rta_history_kernel<false, 1>);
-        HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lt_bytes));
+        HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->ctx->lds_per_block));
         b->block = 768;
         b->blocks = b->ctx->cu_count;
     } else if (b->tally != 0) {
@@ -1175,10 +1264,10 @@ int bte_rta_execute(bte_rta* rr, double* T_out, double* jx_out, double* jy_out, 
     auto* b = reinterpret_cast<RtaBatch*>(rr);
     if (!b || !T_out || !jx_out || !jy_out) throw Error(BTE_EINVAL, "null argument");
     HIPCHECK(hipSetDevice(b->ctx->device));
-    // T in FP64 LDS partial sums when the run's cells fit (tally mode 2:
-    // the same histories and FP64 sums, a third of the device atomics);
-    // device atomics otherwise and with spectral locations
-    const int want = (!b->nloc && !b->any_grad && b->lg_bytes <= b->ctx->lds_per_block) ? 2 : 0;
+    // FP64 LDS partial sums of the leading time bins (tally mode 2: the same
+    // histories and FP64 sums) whenever one time bin of every box fits;
+    // device atomics with spectral locations
+    const int want = (!b->nloc && !b->any_grad && b->lbins > 0) ? 2 : 0;
     if (b->tally != want) batch_set_tally(b, want);
     bte_rta_batch_report br{};
     double E = 0;
