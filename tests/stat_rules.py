@@ -54,7 +54,10 @@ def exact_z(x, mean, sigma):
 
 
 def three_sigma_verdict(z, alpha=1e-3):
-    """(ok, message) for a family of z-scores under rules (a) and (b)"""
+    """(ok, message) for a family of z-scores under rules (a) and (b); z is
+    an array or a list of arrays and scalars (one family)"""
+    if isinstance(z, (list, tuple)):
+        z = np.concatenate([np.ravel(np.asarray(v, float)) for v in z])
     z = np.abs(np.ravel(np.asarray(z, float)))
     K = z.size
     n3 = int((z > 3.0).sum())

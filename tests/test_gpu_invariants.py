@@ -5,6 +5,7 @@ import pytest
 import scipy.linalg
 
 import oracle_lib as orc
+from stat_rules import assert_3sigma, exact_z
 from test_oracle_mc import _bert_inputs, bar, horizontal_material
 
 pytestmark = pytest.mark.gpu
@@ -84,8 +85,7 @@ def test_bert_mode_distribution_follows_P(pkg, ctx):
     n300 = orc.bose_einstein(m["omega"], 300.0)
     h0 = m["omega"] * np.abs(n310 - n300)
     expect = np.linalg.matrix_power(P, 3) @ (h0 / h0.sum()) * h.sum()
-    z = (h - expect) / np.sqrt(expect / 4 + 1)
-    assert np.abs(z).max() < 5, z
+    assert_3sigma(exact_z(h, expect, np.sqrt(expect / 4 + 1)), "P^3 H0 per mode")
 
 
 def test_bert_cancellation_keeps_net_counts(pkg, ctx):

@@ -1,27 +1,26 @@
 """GPU parity of the HIP path against the oracle on the same seeded inputs.
 
 Monte Carlo results are compared statistically: R independent replicas on
-each side give per-cell means and standard errors. Every compared cell must
-satisfy |mean_gpu - mean_oracle| < 5 sigma_diff, and a few scalar summaries
-of each run (spatial means, gradients, event counts) must agree within
-4 sigma_diff. Cells of one run are correlated (a replica with more flux has
-more flux everywhere), so no statistic pools z^2 over cells."""
+each side give per-cell means and standard errors, and every compared cell
+and a few scalar summaries of each run (spatial means, gradients, event
+counts) form one family of z-scores that must pass the 3-sigma rule of
+stat_rules.py (the count beyond 3 sigma within its binomial bound, the
+largest within the Bonferroni bound). Cells of one run are correlated (a
+replica with more flux has more flux everywhere), so no statistic pools z^2
+over cells."""
 import numpy as np
 import pytest
 
 import oracle_lib as orc
+from stat_rules import assert_3sigma, welch_z
 
 pytestmark = pytest.mark.gpu
 
 
 def zstats(a, b):
-    """a, b: (R, ...) replicas -> z-scores of the difference of means"""
-    ma, mb = a.mean(0), b.mean(0)
-    sa = a.std(0, ddof=1) / np.sqrt(a.shape[0])
-    sb = b.std(0, ddof=1) / np.sqrt(b.shape[0])
-    s = np.sqrt(sa ** 2 + sb ** 2)
-    s = np.where(s > 0, s, np.inf)
-    return (ma - mb) / s
+    """a, b: (R, ...) replicas -> normal-equivalent z of the difference of
+    means (Welch)"""
+    return welch_z(a, b)
 
 
 def _rta_setup(pkg, ctx, nbins=20, nmodes=600):
@@ -39,11 +38,11 @@ def test_rta_matches_oracle(pkg, ctx):
     R, N = 16, 20000
     g = [pkg.rtamc2d(ctx, modes, geom, nparticles=N, seed=100 + r, sigma_bounce=1.0, **kw) for r in range(R)]
     o = [orc.rta_run(mat, rect, kind, teq, nparticles=N, seed=200 + r, sigma=1.0, **kw) for r in range(R)]
+    zs = []
     for key in ("T", "jx"):
         ga = np.array([x[key][[4, -1], :20] for x in g])
         oa = np.array([x[key][[4, -1], :20] for x in o])
-        z = zstats(ga, oa)
-        assert np.abs(z).max() < 5.0, (key, z)
+        zs.append(zstats(ga, oa))
     summaries = {
         "jx_mean": lambda x: x["jx"][-1, :20].mean(),
         "T_gradient": lambda x: x["T"][-1, :10].mean() - x["T"][-1, 10:20].mean(),
@@ -52,8 +51,8 @@ def test_rta_matches_oracle(pkg, ctx):
         "crossings": lambda x: float(x["crossings"]),
     }
     for name, f in summaries.items():
-        z = zstats(np.array([f(x) for x in g]), np.array([f(x) for x in o]))
-        assert abs(z) < 4.0, (name, z)
+        zs.append(zstats(np.array([f(x) for x in g]), np.array([f(x) for x in o])))
+    assert_3sigma(zs, "RTAMC2D vs oracle")
 
 
 def test_bert_matches_oracle(pkg, ctx):
@@ -70,18 +69,18 @@ def test_bert_matches_oracle(pkg, ctx):
     g = [pkg.bertamc2d(ctx, modes, geom, prop, seed=10 + r, **kw) for r in range(R)]
     o = [orc.bert_run(mat, rect, kind, teq, tin, P=P, seed=20 + r, **kw) for r in range(R)]
     assert g[0]["nsteps"] == o[0]["nsteps"] == 20
+    zs = []
     for q in (0, 1):
         ga = np.array([x["props"][[3, 10, 20], :, q] for x in g])
         oa = np.array([x["props"][[3, 10, 20], :, q] for x in o])
-        z = zstats(ga, oa)
-        assert np.abs(z).max() < 5.0, (q, z)
+        zs.append(zstats(ga, oa))
     # centre of energy and mean flux at the last step
     xc = 0.5 * (rect[:, 0] + rect[:, 1])
     for name, f in {"centre": lambda x: (x["props"][-1, :, 0] * xc).sum() / x["props"][-1, :, 0].sum(),
                     "jx_mean": lambda x: x["props"][-1, :, 1].mean(),
                     "npart_last": lambda x: float(x["npart"][-1])}.items():
-        z = zstats(np.array([f(x) for x in g]), np.array([f(x) for x in o]))
-        assert abs(z) < 4.0, (name, z)
+        zs.append(zstats(np.array([f(x) for x in g]), np.array([f(x) for x in o])))
+    assert_3sigma(zs, "beRTAMC2D vs oracle")
     # P >= 0 and adiabatic walls conserve particles + pending wall counts
     # exactly (every particle here is positive)
     for x in g + o:
@@ -137,8 +136,7 @@ def test_bert_reservoir_sources_match_oracle(pkg, ctx):
     g = [pkg.bertamc2d(ctx, modes, geom, prop, seed=70 + r, **kw) for r in range(R)]
     o = [orc.bert_run(mat, rect, kind, teq, teq, P=P, seed=90 + r, **kw) for r in range(R)]
     assert g[0]["npart"][-1] > 20000  # several thousand injected per step
-    for q in (0, 1):
-        z = zstats(np.array([x["props"][[5, -1], :6, q] for x in g]), np.array([x["props"][[5, -1], :6, q] for x in o]))
-        assert np.abs(z).max() < 5.0, (q, z)
-    z = zstats(np.array([float(x["npart"][-1]) for x in g]), np.array([float(x["npart"][-1]) for x in o]))
-    assert abs(z) < 4.0, z
+    zs = [zstats(np.array([x["props"][[5, -1], :6, q] for x in g]), np.array([x["props"][[5, -1], :6, q] for x in o]))
+          for q in (0, 1)]
+    zs.append(zstats(np.array([float(x["npart"][-1]) for x in g]), np.array([float(x["npart"][-1]) for x in o])))
+    assert_3sigma(zs, "reservoir-driven beRTAMC2D vs oracle")

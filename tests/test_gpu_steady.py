@@ -5,7 +5,9 @@ import numpy as np
 import pytest
 
 import oracle_lib as orc
+from stat_rules import assert_3sigma
 from test_gpu_parity import zstats
+from test_oracle_pins import onesample_z
 from test_oracle_mc import bar, bulk_flux, inversion_symmetric, periodic_cell
 
 pytestmark = pytest.mark.gpu
@@ -23,10 +25,9 @@ def test_steady_periodic_cell_gives_bulk_flux(pkg, ctx):
                            sigma_bounce=5.0, max_generations=60)
         js.append((o["jx"][0], o["jy"][0]))
     js = np.array(js)
-    for k, j0 in enumerate(bulk_flux(m, g)):
-        se = js[:, k].std(ddof=1) / np.sqrt(len(js))
-        assert abs(js[:, k].mean() - j0) < 5 * se, (k, js[:, k].mean(), j0, se)
-        assert se < 0.03 * abs(j0)
+    j0 = np.asarray(bulk_flux(m, g))
+    assert_3sigma(onesample_z(js, j0), "bulk flux against -kappa grad T")
+    assert (js.std(0, ddof=1) / np.sqrt(len(js)) < 0.03 * np.abs(j0)).all()
 
 
 def test_steady_bar_matches_oracle(pkg, ctx):
@@ -39,8 +40,6 @@ def test_steady_bar_matches_oracle(pkg, ctx):
     R = 10
     g = [pkg.rta_steady(ctx, modes, geom, seed=100 + r, sigma_bounce=2.0, **kw) for r in range(R)]
     o = [orc.rta_steady_run(mat, rect, kind, teq, seed=200 + r, sigma=2.0, **kw) for r in range(R)]
-    for key in ("T", "jx"):
-        z = zstats(np.array([x[key][:8] for x in g]), np.array([x[key][:8] for x in o]))
-        assert np.abs(z).max() < 5.0, (key, z)
-    z = zstats(np.array([x["jx"][:8].mean() for x in g]), np.array([x["jx"][:8].mean() for x in o]))
-    assert abs(z) < 4.0, z
+    zs = [zstats(np.array([x[key][:8] for x in g]), np.array([x[key][:8] for x in o])) for key in ("T", "jx")]
+    zs.append(zstats(np.array([x["jx"][:8].mean() for x in g]), np.array([x["jx"][:8].mean() for x in o])))
+    assert_3sigma(zs, "steady bar vs oracle")
