@@ -64,39 +64,107 @@ def measured_traffic():
     return d.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(pkg, mat, P, cfg, nparticles, nsteps):
-    """The oracle (C++ restatement of the reference loop, OpenMP) on a bounded
-    sample of the same workload: fewer initial particles and steps."""
+def host_cpus():
+    """Threads the CPU baselines use and what the host has: the process's
+    CPU allotment (OMP_NUM_THREADS where the launcher sets it, else the CPUs
+    this process may run on), os.cpu_count() and the CPU model name as
+    /proc/cpuinfo (lscpu's source) gives it."""
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = env if env > 0 else affinity
+    return {"threads": threads, "nproc": os.cpu_count(), "affinity": affinity, "omp_num_threads": env or None,
+            "model": model}
+
+
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    nthreads = max(1, min(nthreads, 16))
-    t0 = time.perf_counter()
-    o = orc.bert_run(mat, cfg["rect"], cfg["kind"], cfg["Teq"], cfg["Tinit"], thickness=cfg["thickness"],
-                     dt=cfg["dt"], maxtime=cfg["dt"] * (nsteps - 1), P=P, Efactor=nparticles,
-                     Tmax=cfg["Tmax"], Tmin=cfg["Tmin"], seed=12345, max_steps=nsteps, nthreads=nthreads)
-    el = time.perf_counter() - t0
-    ps = int(o["npart"][:-1].sum())  # particles entering each step
-    return {"value": ps / el, "unit": "particle-events/s", "cores": nthreads, "kind": "port",
-            "sample": f"oracle beRTAMC2D, same material/propagator/geometry, {nparticles} initial particles x "
-                      f"{nsteps} steps ({ps} particle-events in {el:.2f} s), {platform.processor() or platform.machine()}"}
+    return orc
+
+
+def _with_host(d, host, one):
+    """adds the host description and the 1-thread figure to a baseline"""
+    d.update({"cores": host["threads"], "nproc": host["nproc"], "affinity": host["affinity"],
+              "cpu_model": host["model"], "value_1thread": one["value"], "sample_1thread": one["sample"]})
+    return d
+
+
+def cpu_baseline(pkg, mat, P, cfg, nparticles, nsteps):
+    """The oracle (C++ restatement of the reference loop, OpenMP) on a bounded
+    sample of the same workload: fewer initial particles and steps; on every
+    allotted thread, and on one thread over a tenth of the particles and a
+    quarter of the steps."""
+    orc = _oracle()
+    host = host_cpus()
+
+    def leg(npart, nst, nthreads):
+        t0 = time.perf_counter()
+        o = orc.bert_run(mat, cfg["rect"], cfg["kind"], cfg["Teq"], cfg["Tinit"], thickness=cfg["thickness"],
+                         dt=cfg["dt"], maxtime=cfg["dt"] * (nst - 1), P=P, Efactor=npart,
+                         Tmax=cfg["Tmax"], Tmin=cfg["Tmin"], seed=12345, max_steps=nst, nthreads=nthreads)
+        el = time.perf_counter() - t0
+        ps = int(o["npart"][:-1].sum())  # particles entering each step
+        return {"value": ps / el, "sample": f"oracle beRTAMC2D, same material/propagator/geometry, {npart} initial "
+                                            f"particles x {nst} steps ({ps} particle-events in {el:.2f} s)"}
+
+    full = leg(nparticles, nsteps, host["threads"])
+    one = leg(max(1, nparticles // 10), max(2, nsteps // 4), 1)
+    return _with_host({"value": full["value"], "unit": "particle-events/s", "kind": "port",
+                       "sample": full["sample"]}, host, one)
 
 
 def rta_cpu_baseline(c, nparticles):
     """The oracle's RTAMC2D history loop (OpenMP over particles) on a bounded
-    particle sample of configuration 0."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as orc
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    nthreads = max(1, min(nthreads, 16))
+    particle sample of configuration 0; on every allotted thread, and on one
+    thread over a tenth of the particles."""
+    orc = _oracle()
+    host = host_cpus()
+
+    def leg(npart, nthreads):
+        t0 = time.perf_counter()
+        o = orc.rta_run(c["material"], c["rect"], c["kind"], c["Teq"], thickness=c["thickness"], T0=c["T0"],
+                        dt=c["dt"], maxtime=c["maxtime"], nparticles=npart, seed=12345, sigma=c["sigma_bounce"],
+                        nthreads=nthreads)
+        el = time.perf_counter() - t0
+        return {"value": o["events"] / el, "sample": f"oracle RTAMC2D, configuration 0 with {npart} particles "
+                                                     f"({o['events']} events in {el:.2f} s)"}
+
+    full = leg(nparticles, host["threads"])
+    one = leg(max(1, nparticles // 10), 1)
+    return _with_host({"value": full["value"], "unit": "particle-events/s", "kind": "port",
+                       "sample": full["sample"]}, host, one)
+
+
+def single_replica(pkg, ctx, modes, prop, cfg1, steps):
+    """Configuration 1 as stated: one ribbon, ~1e6 initial particles, one run
+    per timed step (the headline advances several replicas per run)."""
+    geom = pkg.Geometry(ctx, cfg1["rect"], cfg1["kind"], cfg1["Teq"], cfg1["Tinit"])
+    runs = [pkg.BertRun(ctx, modes, geom, prop, thickness=cfg1["thickness"], dt=cfg1["dt"], maxtime=cfg1["maxtime"],
+                        max_steps=cfg1["max_steps"], Efactor=cfg1["Efactor"], Tmax=cfg1["Tmax"], Tmin=cfg1["Tmin"],
+                        seed=5000 + s) for s in range(steps + 1)]
+    runs[0].execute()
+    ctx.sync()
     t0 = time.perf_counter()
-    o = orc.rta_run(c["material"], c["rect"], c["kind"], c["Teq"], thickness=c["thickness"], T0=c["T0"], dt=c["dt"],
-                    maxtime=c["maxtime"], nparticles=nparticles, seed=12345, sigma=c["sigma_bounce"],
-                    nthreads=nthreads)
+    ev = sum(r.execute()["particle_steps"] for r in runs[1:])
+    ctx.sync()
     el = time.perf_counter() - t0
-    return {"value": o["events"] / el, "unit": "particle-events/s", "cores": nthreads, "kind": "port",
-            "sample": f"oracle RTAMC2D, configuration 0 with {nparticles} particles ({o['events']} events in "
-                      f"{el:.2f} s)"}
+    for r in runs:
+        r.close()
+    geom.close()
+    return {"workload": "configuration 1, one replica per run", "runs": steps, "seconds": el,
+            "ms_per_run": el / steps * 1e3, "particle_events_per_s": ev / el}
 
 
 def rta_extra(pkg, ctx, cpu_particles=0):
@@ -180,12 +248,16 @@ def batch_extra(pkg, ctx):
             "kappa_W_per_mK": {"first": kappa[0], "last": kappa[-1], "min": min(kappa), "max": max(kappa)}}
 
 
-def grad_extra(pkg, ctx, fd_dp=0.01):
+def grad_extra(pkg, ctx, fd_dp=0.01, seeds=8):
     """Differentiable transient RTAMC2D: T(x, t) and dT/dp_top, dT/dp_bottom
     from one run with score-function tallies, against plain runs and a
     central-difference estimate with common random numbers. Plain and
     gradient runs are timed with both tally modes: FP64 device atomics, and
-    FP32 per-workgroup partial sums in LDS with an FP64 final sum."""
+    FP32 per-workgroup partial sums in LDS with an FP64 final sum. Over
+    `seeds` independent seeds (each: one gradient run and four
+    central-difference runs on the same streams) the two estimators of the
+    left-half mean temperature derivative get means and standard errors on
+    both edges."""
     W = pkg.workloads
     c = W.config4()
     m = c["material"]
@@ -205,37 +277,54 @@ def grad_extra(pkg, ctx, fd_dp=0.01):
             outs[(tally, grad)] = o
             events = int(rep["events"])
             b.close()
-    fd_runs = []
-    for side in (1, 2):
-        for sgn in (+1, -1):
-            q = p.copy()
-            q[side] += sgn * fd_dp
-            fd_runs.append(dict(base, specularity=tuple(q)))
-    f = pkg.RtaBatch(ctx, modes, fd_runs, **kw)
-    fo, _ = f.execute()
-    f.close()
-    go = outs[("fp32-lds", True)]
     obs = lambda T: float(T[-1, :64].mean())  # left half, last time
-    out = {}
-    for i, (side, name) in enumerate(((2, "top"), (1, "bottom"))):
-        k = 0 if side == 1 else 2
-        fd = (obs(fo[k]["T"]) - obs(fo[k + 1]["T"])) / (2 * fd_dp)
-        out[name] = {"score_dTdp_left_half_K": obs(go["dTdp"][side]), "central_fd_K": fd}
+    sides = ((2, "top"), (1, "bottom"))
+    score = {n: [] for _, n in sides}
+    fdiff = {n: [] for _, n in sides}
+    for k in range(seeds):
+        sd = dict(base, seed=1000 + k)
+        g = pkg.RtaBatch(ctx, modes, [dict(sd, specularity=tuple(p), grad=True)], tally="fp32-lds", **kw)
+        (go,), _ = g.execute(grad=True)
+        g.close()
+        for side, name in sides:
+            score[name].append(obs(go["dTdp"][side]))
+            T = []
+            for sgn in (+1, -1):
+                q = p.copy()
+                q[side] += sgn * fd_dp
+                f = pkg.RtaBatch(ctx, modes, [dict(sd, specularity=tuple(q))], tally="fp32-lds", **kw)
+                (fo,), _ = f.execute()
+                f.close()
+                T.append(obs(fo["T"]))
+            fdiff[name].append((T[0] - T[1]) / (2 * fd_dp))
+    t_plain = times[("fp32-lds", False)]
+    t_grad = times[("fp32-lds", True)]
+    out, ratio = {}, 0.0
+    for _, name in sides:
+        a, b_ = np.array(score[name]), np.array(fdiff[name])
+        se_a, se_b = a.std(ddof=1) / np.sqrt(seeds), b_.std(ddof=1) / np.sqrt(seeds)
+        d = a - b_
+        out[name] = {"score_mean_K": float(a.mean()), "score_se_K": float(se_a),
+                     "central_fd_mean_K": float(b_.mean()), "central_fd_se_K": float(se_b),
+                     "paired_difference_K": float(d.mean()), "paired_difference_se_K": float(d.std(ddof=1) / np.sqrt(seeds)),
+                     "z": float(d.mean() / (d.std(ddof=1) / np.sqrt(seeds)))}
+        ratio = max(ratio, (se_b / se_a) ** 2)
     g64 = outs[("fp64", True)]
-    t_plain = min(times[("fp64", False)], times[("fp32-lds", False)])
-    t_grad = min(times[("fp64", True)], times[("fp32-lds", True)])
+    go = outs[("fp32-lds", True)]
     return {"workload": "1 um x 100 nm, 128 bins x 64 time bins, 5000 mirror-pair modes, 1e7 particles, "
                         "p_top 0.3, p_bottom 0.7, walls 300 +- 0.5 K",
             "runs_s": {f"{t}_{'grad' if g else 'plain'}": v for (t, g), v in times.items()},
             "grad_run_s": t_grad, "plain_run_s": t_plain, "events": events,
             "useful_run_speedup": 5 * t_plain / t_grad,
-            "useful_run_speedup_vs_fp64_plain": 5 * times[("fp64", False)] / t_grad,
+            "useful_run_speedup_equal_se": (t_plain + 4 * t_plain * ratio) / t_grad,
+            "fd_over_score_variance": ratio,
             "fp32_vs_fp64_tallies": {
                 "max_abs_T_diff_K": float(np.abs(go["T"] - g64["T"]).max()),
                 "max_abs_dTdp_diff_K": float(np.abs(go["dTdp"] - g64["dTdp"]).max())},
-            "note": "one gradient run replaces the base run and four central-difference runs; "
-                    "speedups use the faster tally mode of each form",
-            "derivatives": out, "fd_dp": fd_dp}
+            "note": "both forms with FP32 LDS tallies; per run, one gradient run replaces the base run and four "
+                    "central-difference runs (at most 5x); at equal standard error of dT/dp the four "
+                    "central-difference runs are repeated (se_fd / se_score)^2 times",
+            "derivatives": out, "seeds": seeds, "fd_dp": fd_dp}
 
 
 def krylov_extra(pkg, ctx, N=8192, m=30, cpu_cols=0):
@@ -283,19 +372,25 @@ def krylov_cpu_baseline(B, m, ncols, gpu_ms):
     """The oracle's per-column FP64 Arnoldi + Pade (OpenMP over columns, the
     reference's build_P loop) on a bounded sample of columns, extrapolated to
     all N columns; compared with the GPU build of the same Arnoldi."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib as orc
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
-    nthreads = max(1, min(nthreads, 16))
+    orc = _oracle()
+    host = host_cpus()
     N = B.shape[0]
     A = np.asfortranarray(B)
-    t0 = time.perf_counter()
-    orc.build_P_cols(A, 5.0, m, 0, 0, ncols, nthreads)
-    el = time.perf_counter() - t0
-    est = el * N / ncols
-    return {"value": est, "unit": "s per full propagator (extrapolated)", "cores": nthreads, "kind": "port",
-            "sample": f"oracle build_P, reference Arnoldi, FP64, columns 0..{ncols - 1} of {N} in {el:.2f} s",
-            "gpu_fp64_reference_arnoldi_s": gpu_ms * 1e-3, "gpu_speedup": est / (gpu_ms * 1e-3)}
+
+    def leg(nc, nthreads):
+        t0 = time.perf_counter()
+        orc.build_P_cols(A, 5.0, m, 0, 0, nc, nthreads)
+        el = time.perf_counter() - t0
+        return {"value": el * N / nc,
+                "sample": f"oracle build_P, reference Arnoldi, FP64, columns 0..{nc - 1} of {N} in {el:.2f} s"}
+
+    full = leg(ncols, host["threads"])
+    one = leg(max(1, ncols // 8), 1)
+    d = _with_host({"value": full["value"], "unit": "s per full propagator (extrapolated)", "kind": "port",
+                    "sample": full["sample"]}, host, one)
+    d.update({"gpu_fp64_reference_arnoldi_s": gpu_ms * 1e-3, "gpu_speedup": full["value"] / (gpu_ms * 1e-3),
+              "gpu_speedup_vs_1thread": one["value"] / (gpu_ms * 1e-3)})
+    return d
 
 
 def main():
@@ -407,11 +502,19 @@ def main():
     }
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
-        line["cpu_baseline"] = cpu_baseline(pkg, mat, P, cfg1, args.cpu_particles, args.cpu_steps)
+        cb = cpu_baseline(pkg, mat, P, cfg1, args.cpu_particles, args.cpu_steps)
+        cb["gpu_over_cpu"] = value / cb["value"]
+        cb["gpu_over_cpu_1thread"] = value / cb["value_1thread"]
+        line["cpu_baseline"] = cb
         log("cpu baseline done")
     if not args.no_extras:
+        ex = {"config1_single_replica": single_replica(pkg, ctx, modes, prop, cfg1, args.steps)}
         log("extra: RTAMC2D config ...")
-        ex = {"rta_steady": rta_extra(pkg, ctx, cpu_particles=0 if args.no_cpu else 200000)}
+        ex["rta_steady"] = rta_extra(pkg, ctx, cpu_particles=0 if args.no_cpu else 200000)
+        rs = ex["rta_steady"]
+        if "cpu_baseline" in rs:
+            rs["cpu_baseline"]["gpu_over_cpu"] = rs["kernel_events_per_s"] / rs["cpu_baseline"]["value"]
+            rs["cpu_baseline"]["gpu_over_cpu_1thread"] = rs["kernel_events_per_s"] / rs["cpu_baseline"]["value_1thread"]
         log("extra: batched RTAMC2D sweep ...")
         ex["rta_batch"] = batch_extra(pkg, ctx)
         log("extra: specularity gradient ...")
