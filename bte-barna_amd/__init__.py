@@ -97,6 +97,13 @@ class ModeTable(_Handle):
         self.n = len(self.omega)
         check(lib().bte_modes_create(ctx.h, self.n, self.nq, _dp(self.omega), _dp(self.vx), _dp(self.vy),
                                      _dp(self.w), self.V, self.lattz, C.byref(self.h)))
+        self.sigma = None
+
+    def set_sigma(self, sigma):
+        """per-mode broadening [rad/ps] of the diffuse bounce (base_sigma);
+        None clears it (runs then use their constant sigma_bounce)"""
+        self.sigma = None if sigma is None else _f64(sigma)
+        check(lib().bte_modes_set_sigma(self.h, None if self.sigma is None else _dp(self.sigma)))
 
 
 class Geometry(_Handle):
@@ -300,14 +307,29 @@ class BertRun(_Handle):
 
     def __init__(self, ctx, modes, geom, prop, thickness, dt, maxtime, max_steps=0, Ed=-1.0,
                  Efactor=0, Tmax=0.0, Tmin=0.0, rta=False, pcancel_each=4, seed=1, capacity=0,
-                 gradient=None, prop_interp=0):
+                 gradient=None, prop_interp=0, init_dist=None):
         """gradient: (gx, gy) [K/nm] in every regular box (<gradient x y>);
-        None derives per-box gradients from the box temperatures"""
+        None derives per-box gradients from the box temperatures.
+        init_dist: {box: signed distribution over the modes after the
+        propagator's Gamma offset} (<init_distribution box file>)"""
         super().__init__()
         gx, gy = gradient if gradient is not None else (0.0, 0.0)
         self.cfg = BertCfg(thickness, dt, maxtime, max_steps, Ed, Efactor, Tmax, Tmin, int(rta),
                            pcancel_each, seed, capacity, int(gradient is not None), (C.c_double * 2)(gx, gy),
                            int(prop_interp))
+        self._init = None
+        if init_dist:
+            boxes = sorted(init_dist)
+            lens = {len(init_dist[b]) for b in boxes}
+            if len(lens) != 1:
+                raise ValueError("given distributions of different lengths")
+            ib = np.ascontiguousarray(boxes, np.int32)
+            fd = np.ascontiguousarray(np.concatenate([np.asarray(init_dist[b], np.float64) for b in boxes]))
+            self._init = (ib, fd)
+            self.cfg.ninit = len(boxes)
+            self.cfg.init_len = lens.pop()
+            self.cfg.init_boxes = ib.ctypes.data_as(C.POINTER(C.c_int32))
+            self.cfg.init_dist = _dp(fd)
         self.nbox = geom.nbox
         nst = 0
         t = 0.0

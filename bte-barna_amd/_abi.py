@@ -59,7 +59,9 @@ class BertCfg(C.Structure):
                 ("max_steps", C.c_int), ("Ed", C.c_double), ("Efactor", C.c_uint64),
                 ("Tmax", C.c_double), ("Tmin", C.c_double), ("rta", C.c_int),
                 ("pcancel_each", C.c_int), ("seed", C.c_uint64), ("capacity", C.c_uint64),
-                ("fixed_gradient", C.c_int), ("gradient", C.c_double * 2), ("prop_interp", C.c_int)]
+                ("fixed_gradient", C.c_int), ("gradient", C.c_double * 2), ("prop_interp", C.c_int),
+                ("ninit", C.c_int), ("init_len", C.c_int), ("init_boxes", C.POINTER(C.c_int32)),
+                ("init_dist", C.POINTER(C.c_double))]
 
 
 class BertReport(C.Structure):
@@ -96,6 +98,7 @@ SIGNATURES = {
     "bte_synthetic_B": (C.c_int, [C.c_int, dp, dp, C.c_double, C.c_uint64, C.c_int, dp]),
     "bte_alias_build": (C.c_int, [C.c_int, dp, up32, up32]),
     "bte_modes_create": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp, C.c_double, C.c_double, pvp]),
+    "bte_modes_set_sigma": (C.c_int, [vp, dp]),
     "bte_modes_destroy": (C.c_int, [vp]),
     "bte_geom_create_rects": (C.c_int, [vp, C.c_int, dp, ip32, dp, dp, pvp]),
     "bte_geom_destroy": (C.c_int, [vp]),

@@ -72,6 +72,7 @@ static int run(int argc, char** argv) {
     check(bte_modes_create(ctx, M.n, M.nq, M.omega.data(), M.vx.data(), M.vy.data(), M.w.data(), M.V, M.lattz,
                            &modes),
           "mode table");
+    if (!M.sigma.empty()) check(bte_modes_set_sigma(modes, M.sigma.data()), "per-mode sigma");
     bte_geom* geom = make_geom(ctx, B);
     bte_rta_cfg cfg{};
     cfg.thickness = M.thickness;

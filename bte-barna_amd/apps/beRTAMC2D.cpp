@@ -48,7 +48,6 @@ static int run(int argc, char** argv) {
     const xml::Node* eef = root->child("Eeff");
     if (!geo || !tim || !mat || !eef) throw std::runtime_error("<beRTAMC2D> needs geometry, time, material and Eeff");
     if (root->children("material").size() > 1) throw std::runtime_error("This is currently not supported (several materials)");
-    if (root->child("init_distribution")) throw std::runtime_error("<init_distribution> is not supported");
     if (root->child("spectral")) std::cout << "# <spectral> is read by dist_reader, not by the solver" << std::endl;
     Boxes B = read_geometry(geo->str("file"));
     Material M = read_material(*mat);
@@ -59,6 +58,7 @@ static int run(int argc, char** argv) {
     check(bte_modes_create(ctx, M.n, M.nq, M.omega.data(), M.vx.data(), M.vy.data(), M.w.data(), M.V, M.lattz,
                            &modes),
           "mode table");
+    if (!M.sigma.empty()) check(bte_modes_set_sigma(modes, M.sigma.data()), "per-mode sigma");
     bte_geom* geom = make_geom(ctx, B);
     const double dt = tim->num("dt");
     bte_prop* prop = nullptr;
@@ -104,9 +104,43 @@ static int run(int argc, char** argv) {
         check(bte_prop_create_multi(ctx, (int)rows, (int)Ts.size(), Ps.data(), Ts.data(), M.n - (int)rows, &prop),
               "propagator");
     }
+    // <init_distribution box file> (beyondRTA_MC2d.cpp:1411-1417): column 0
+    // of a tab-separated file, lines holding '#' skipped (get_column,
+    // beyondRTA_MC2d.cpp:1270-1291)
+    std::vector<int32_t> init_boxes;
+    std::vector<double> init_dist;
+    int init_len = -1;
+    for (const xml::Node* d : root->children("init_distribution")) {
+        const int ib = (int)d->num("box");
+        const std::string fn = d->str("file");
+        std::ifstream f(fn);
+        if (!f) throw std::runtime_error("cannot open init_distribution file " + fn);
+        std::vector<double> col;
+        std::string line;
+        while (std::getline(f, line)) {
+            if (line.find('#') != std::string::npos) continue;
+            std::istringstream ls(line);
+            double v;
+            if (ls >> v) col.push_back(v);
+        }
+        if (init_len >= 0 && (int)col.size() != init_len)
+            throw std::runtime_error("init_distribution files of different lengths");
+        init_len = (int)col.size();
+        init_boxes.push_back(ib);
+        init_dist.insert(init_dist.end(), col.begin(), col.end());
+    }
+    if (!init_boxes.empty())
+        std::cout << "# init_distribution: " << init_boxes.size()
+                  << " given distributions, kept together with the particles of boxes at Tinit != Teq (the "
+                     "reference's second init_particles discards them, beyondRTA_MC2d.cpp:2096-2105)"
+                  << std::endl;
     bte_bert_cfg cfg{};
     cfg.thickness = M.thickness;
     cfg.dt = dt;
+    cfg.ninit = (int)init_boxes.size();
+    cfg.init_len = init_len < 0 ? 0 : init_len;
+    cfg.init_boxes = init_boxes.empty() ? nullptr : init_boxes.data();
+    cfg.init_dist = init_dist.empty() ? nullptr : init_dist.data();
     cfg.maxtime = tim->num("maxtime");
     cfg.Ed = eef->num("Ed", -1.0);
     cfg.Efactor = (uint64_t)eef->num("particles", 0);

@@ -70,6 +70,7 @@ static int run(int argc, char** argv) {
     bte_modes* modes = nullptr;
     check(bte_modes_create(ctx, n, nq, M.omega.data(), M.vx.data(), M.vy.data(), M.w.data(), M.V, M.lattz, &modes),
           "mode table");
+    if (!M.sigma.empty()) check(bte_modes_set_sigma(modes, M.sigma.data()), "per-mode sigma");
     const double vuc = M.V * M.thickness / M.lattz;
     const int ncols = 3 * nbins + nq;
     std::ifstream in(argv[2], std::ios::binary);

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -106,13 +107,17 @@ inline double be_kernel(double omega, double T) {
 //   <material name="m" thickness="1" T0="300" modetable="synthetic" nmodes="3000" seed="1"
 //             [cv="1.6e6"] [sigma_bounce="0.25"]/>
 //   <material name="m" thickness="1" T0="300" modetable="file.modes"/>
-// A .modes file holds "n nq V lattz" then n lines "omega vx vy w"
-// (rad/ps, nm/ps, nm^3, nm, 1/ps).
+// A .modes file holds "n nq V lattz" then n lines "omega vx vy w [sigma]"
+// (rad/ps, nm/ps, nm^3, nm, 1/ps, rad/ps). The optional fifth column is the
+// per-mode broadening of the diffuse bounce (base_sigma of the q-grid the
+// table came from, qpoint_grid.hpp:179-181); without it sigma_bounce applies
+// to every mode.
 struct Material {
     std::string name;
     int n = 0, nq = 1;
     double V = 0, lattz = 0, thickness = 0, T0 = 300.0, sigma_bounce = 0.25;
     std::vector<double> omega, vx, vy, w;
+    std::vector<double> sigma;  // per-mode broadening, empty when not given
     double cv(double T) const {  // J/(K nm^3), calc_cv (bulk_properties.hpp:32-47)
         double s = 0;
         for (int i = 0; i < n; i++) s += be_kernel(omega[i], T);
@@ -169,9 +174,26 @@ inline Material read_material(const xml::Node& m) {
         M.vx.resize(M.n);
         M.vy.resize(M.n);
         M.w.resize(M.n);
-        for (int i = 0; i < M.n; i++)
-            if (!(f >> M.omega[i] >> M.vx[i] >> M.vy[i] >> M.w[i]))
-                throw std::runtime_error(mt + ": short mode list");
+        std::string line;
+        std::getline(f, line);  // rest of the header line
+        int cols = 0;
+        for (int i = 0; i < M.n;) {
+            if (!std::getline(f, line)) throw std::runtime_error(mt + ": short mode list");
+            std::istringstream ls(line);
+            double v[6];
+            int k = 0;
+            while (k < 6 && ls >> v[k]) k++;
+            if (k == 0) continue;  // blank line
+            if (k != 4 && k != 5) throw std::runtime_error(mt + ": mode lines hold 4 or 5 numbers");
+            if (cols == 0) cols = k;
+            if (k != cols) throw std::runtime_error(mt + ": mode lines differ in their number of columns");
+            M.omega[i] = v[0];
+            M.vx[i] = v[1];
+            M.vy[i] = v[2];
+            M.w[i] = v[3];
+            if (k == 5) M.sigma.push_back(v[4]);
+            i++;
+        }
     }
     return M;
 }

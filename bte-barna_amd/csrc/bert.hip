@@ -1089,7 +1089,9 @@ __global__ __launch_bounds__(256) void bert_init_kernel(BertParams P, const Bert
         double y = I.ymin + rng.uniform() * (I.ymax - I.ymin);
         bool ng;
         int mode = (int)alias_pick(itab + I.tab, (uint32_t)P.n, rng, ng);
-        BPart q{x, y, (uint32_t)mode, ((uint32_t)I.box << 1) | (I.sign < 0 ? 1u : 0u), pid};
+        // tables of given distributions carry each mode's sign in the alias word
+        const int sg = ng ? -I.sign : I.sign;
+        BPart q{x, y, (uint32_t)mode, ((uint32_t)I.box << 1) | (sg < 0 ? 1u : 0u), pid};
         P.pin[(pid % P.nsegs) * P.segcap + pid / P.nsegs] = q;
     }
 }
@@ -1488,6 +1490,44 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     std::vector<double> pmf(n);
     std::vector<uint32_t> th(n), al(n);
     uint64_t total = 0;
+    // given distributions first (init_particles, beyondRTA_MC2d.cpp:142-193;
+    // ref_distribution, sampling.cpp:70-114): pmf |fd| over the modes after
+    // the Gamma offset, the sign of fd in the alias word, Ntogenerate(Vbox /
+    // Vucell, Eeff) = vol sum|fd| / (Eeff nq) particles
+    if (cfg->ninit > 0) {
+        const int ngam = b->p ? b->p->ngamma : 0;
+        if (!cfg->init_boxes || !cfg->init_dist) throw Error(BTE_EINVAL, "init_distribution without data");
+        if (cfg->init_len + ngam != n)
+            throw Error(BTE_EINVAL, "init_distribution holds " + std::to_string(cfg->init_len) + " entries, the modes "
+                                        "after the Gamma offset are " + std::to_string(n - ngam));
+        std::vector<int8_t> neg(n);
+        for (int q = 0; q < cfg->ninit; q++) {
+            const int i = cfg->init_boxes[q];
+            if (i < 0 || i >= nbox) throw Error(BTE_EINVAL, "init_distribution box out of range");
+            const RectBox& bx = g->boxes[i];
+            if (bx.kind != 0) continue;  // reservoirs and periodic boxes are skipped
+            const double* fd = cfg->init_dist + (size_t)q * cfg->init_len;
+            double cs = 0;
+            for (int k = 0; k < n; k++) {
+                pmf[k] = k < ngam ? 0. : std::fabs(fd[k - ngam]);
+                neg[k] = k >= ngam && std::signbit(fd[k - ngam]) ? 1 : 0;
+                if (!std::isfinite(pmf[k])) throw Error(BTE_EINVAL, "init_distribution entries must be finite");
+                cs += pmf[k];
+            }
+            const double gen = (bx.area() * cfg->thickness / vuc) * cs / (b->Eeff * m->nq);
+            uint64_t ng = (uint64_t)gen;
+            if (hr.uniform() <= gen - std::floor(gen)) ng++;
+            if (!(cs > 0.)) continue;
+            build_alias(pmf.data(), n, th.data(), al.data());
+            BertInit I{bx.xmin, bx.xmax, bx.ymin, bx.ymax, total, ng, i, 1, (int32_t)itab.size(), 0};
+            for (int k = 0; k < n; k++) {
+                const uint32_t a = al[k];
+                itab.push_back(make_uint2(th[k], a | (neg[k] ? 0x80000000u : 0u) | (neg[a] ? 0x40000000u : 0u)));
+            }
+            inits.push_back(I);
+            total += ng;
+        }
+    }
     for (int i = 0; i < nbox; i++) {
         const RectBox& bx = g->boxes[i];
         if (bx.kind != 0 || almost_equal(bx.Teq, bx.Tinit)) continue;

@@ -3,6 +3,7 @@
 // build device tables.
 #include "runtime.hpp"
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <limits>
@@ -407,6 +408,21 @@ int bte_modes_create(bte_ctx* ctx, int n, int nq, const double* omega, const dou
     m->d_omega.upload(m->omega, c->stream);
     HIPCHECK(hipStreamSynchronize(c->stream));
     *out = reinterpret_cast<bte_modes*>(m);
+    BTE_GUARD_END
+}
+
+int bte_modes_set_sigma(bte_modes* mm, const double* sigma) {
+    BTE_GUARD_BEGIN
+    auto* m = reinterpret_cast<Modes*>(mm);
+    if (!m) throw Error(BTE_EINVAL, "null mode table");
+    if (!sigma) {
+        m->sigma.clear();
+        return BTE_OK;
+    }
+    for (int i = 0; i < m->n; i++)
+        if (!(sigma[i] >= 0.) || !std::isfinite(sigma[i]))
+            throw Error(BTE_EINVAL, "per-mode sigma must be finite and nonnegative (mode " + std::to_string(i) + ")");
+    m->sigma.assign(sigma, sigma + m->n);
     BTE_GUARD_END
 }
 

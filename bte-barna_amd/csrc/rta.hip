@@ -667,14 +667,30 @@ static void upload_alias(DevBuf<uint2>& buf, const std::vector<uint32_t>& th,
     buf.upload(v, s);
 }
 
-// Lambert tables (RTA_MC2D.cpp:556-626 with a constant broadening sigma per
-// mode): for mode i the partners are the modes within nsigma*sqrt(2)*sigma in
-// omega (a contiguous window of the omega-sorted order); for each wall side s
-// the weights are max(-n_s.v_f, 0) * N(omega_i - omega_f; 0, sqrt(2) sigma).
+// Lambert tables (RTA_MC2D.cpp:556-626): for mode i the partners f are the
+// modes with nonzero omega within nsigma * hypot(sigma_i, sigma_f) in omega,
+// weighted for wall side s by max(-n_s.v_f, 0) times the normal pdf of
+// omega_i - omega_f with width hypot(sigma_i, sigma_f). sigma_i is the mode
+// table's per-mode broadening (base_sigma of a q-grid) raised to the floor
+// exp(p25 - 1.5 (p75 - p25)) of log sigma (RTA_MC2D.cpp:566-576), or the
+// constant sigma for every mode when the table has none. The partners of i
+// lie in one contiguous window of the omega-sorted order (the window of the
+// widest pair); pairs outside their own cut get weight 0.
 static void build_lambert(const Modes& m, double sigma, std::vector<int4>& info,
                           std::vector<uint2>& tab, std::vector<int32_t>& perm) {
     const double nsig = 2.828427124746190097603377448419396157138;
+    const double rt2pi = 2.506628274631000502415765284811045253;
     int n = m.n;
+    std::vector<double> bs(n, sigma);
+    if (!m.sigma.empty()) {
+        std::vector<double> ls(n);
+        for (int i = 0; i < n; i++) ls[i] = m.sigma[i] == 0. ? std::numeric_limits<double>::lowest() : std::log(m.sigma[i]);
+        std::sort(ls.begin(), ls.end());
+        const double p0 = ls[25 * (size_t)n / 100], p1 = ls[75 * (size_t)n / 100];
+        const double lb = std::exp(p0 - 1.5 * (p1 - p0));
+        for (int i = 0; i < n; i++) bs[i] = std::max(m.sigma[i], lb);
+    }
+    const double bmax = *std::max_element(bs.begin(), bs.end());
     perm.resize(n);
     std::iota(perm.begin(), perm.end(), 0);
     std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return m.omega[a] < m.omega[b]; });
@@ -682,14 +698,13 @@ static void build_lambert(const Modes& m, double sigma, std::vector<int4>& info,
     for (int k = 0; k < n; k++) pos[perm[k]] = k;
     std::vector<double> sw(n);
     for (int k = 0; k < n; k++) sw[k] = m.omega[perm[k]];
-    double sig = std::hypot(sigma, sigma);
-    double cut = nsig * sig;
     info.assign(n, make_int4(0, 0, 0, 0));
     tab.clear();
     std::vector<double> pmf;
     std::vector<uint32_t> th, al;
     for (int i = 0; i < n; i++) {
         double oi = m.omega[i];
+        const double cut = nsig * std::hypot(bs[i], bmax);
         int lo = (int)(std::lower_bound(sw.begin(), sw.end(), oi - cut) - sw.begin());
         int hi = (int)(std::upper_bound(sw.begin(), sw.end(), oi + cut) - sw.begin());
         // the float window edges may admit a partner just outside |d| <= cut
@@ -706,10 +721,13 @@ static void build_lambert(const Modes& m, double sigma, std::vector<int4>& info,
             side_normal(s, nx, ny);
             for (int k = 0; k < len; k++) {
                 int f = perm[lo + k];
+                const double sig = std::hypot(bs[i], bs[f]);
                 double d = std::fabs(oi - m.omega[f]);
                 double vn = -(nx * m.vx[f] + ny * m.vy[f]);
-                bool ok = !(vn < 0. || almost_equal(vn, 0.)) && !almost_equal(0., m.omega[f]);
-                pmf[k] = ok ? vn * std::exp(-0.5 * d * d / (sig * sig)) : 0.;
+                bool ok = !(vn < 0. || almost_equal(vn, 0.)) && !almost_equal(0., m.omega[f]) && d <= nsig * sig;
+                // the pdf's 1 / (sig sqrt(2 pi)) differs between pairs only with per-mode sigma
+                const double pre = m.sigma.empty() ? 1.0 : 1.0 / (sig * rt2pi);
+                pmf[k] = ok ? vn * std::exp(-0.5 * d * d / (sig * sig)) * pre : 0.;
             }
             if (!build_alias(pmf.data(), len, th.data(), al.data())) {
                 // a mode that can reach this side needs a partner leaving it
@@ -1309,7 +1327,8 @@ int bte_rta_spectral(bte_rta* rr, int nloc, const int32_t* boxes, int nomega, co
         b->omegagrid[i] = (nomega > 1 ? maxomega * i / (nomega - 1) : maxomega) + 0.5 * delta;
     const int ncols = 3 * nomega + nq;
     std::vector<double> W;
-    spectral_weights(*m, sigma, b->sigma, nomega, b->omegagrid.data(), W);
+    spectral_weights(*m, sigma ? sigma : (m->sigma.empty() ? nullptr : m->sigma.data()), b->sigma, nomega,
+                     b->omegagrid.data(), W);
     const int nt = b->ntime - 1;
     b->d_box_loc.upload(loc, s);
     b->d_specw.upload(W, s);
@@ -1367,7 +1386,8 @@ int bte_spectra(bte_ctx* ctx, const bte_modes* mm, const double* sigma, double s
     HIPCHECK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     std::vector<double> W;
-    spectral_weights(*m, sigma, sigma_default, nomega, grid, W);
+    spectral_weights(*m, sigma ? sigma : (m->sigma.empty() ? nullptr : m->sigma.data()), sigma_default, nomega,
+                     grid, W);
     const int n = m->n, ncols = 3 * nomega + m->nq;
     DevBuf<double> dW, dX, dO;
     dW.upload(W, s);

@@ -88,6 +88,7 @@ struct Modes {
     Ctx* ctx = nullptr;
     int n = 0, nq = 1;
     std::vector<double> omega, vx, vy, w;
+    std::vector<double> sigma;  // per-mode broadening (base_sigma), empty when not given
     double V = 1.0, lattz = 1.0;
     DevBuf<double> d_vx, d_vy, d_w, d_omega;
 };

@@ -65,6 +65,14 @@ int bte_alias_build(int n, const double* pmf, uint32_t* thresh, uint32_t* alias)
 int bte_modes_create(bte_ctx* ctx, int n, int nq, const double* omega, const double* vx,
                      const double* vy, const double* w, double V, double lattz,
                      bte_modes** out);
+/* Per-mode broadening sigma[n] [rad/ps] of the diffuse bounce: base_sigma
+ * of a real q-grid, ||v^T dq|| / sqrt(12) (qpoint_grid.hpp:179-181), given
+ * with the mode table since this engine reads no q-grid. Lambert tables of
+ * RTAMC2D runs created afterwards use hypot(sigma_i, sigma_f) per pair after
+ * the reference's low-outlier floor (RTA_MC2D.cpp:566-618) instead of the
+ * constant sigma_bounce; spectral broadening without an explicit sigma uses
+ * it too. NULL clears it. */
+int bte_modes_set_sigma(bte_modes* m, const double* sigma);
 int bte_modes_destroy(bte_modes* m);
 
 /* ---- geometry: axis-aligned rectangular boxes (replaces
@@ -116,6 +124,7 @@ int bte_rta_create(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bt
                    bte_rta** out);
 int bte_rta_execute(bte_rta* r, double* T_out, double* jx_out, double* jy_out,
                     bte_rta_report* rep);
+
 /* Spectral decomposition (<spectral><resolution ticks/><location bin/>,
  * RTA_MC2D.cpp:1930-1945, 1978-2052; process_segment 718-766; outputs
  * 2349-2405, 2473-2541), set before bte_rta_execute: the segments in the nloc
@@ -264,6 +273,18 @@ typedef struct {
      *     temperature not below Teq and the next one, extrapolated down),
      * 1 = the adjacent temperatures bracketing Teq */
     int prop_interp;
+    /* <init_distribution box file> (beyondRTA_MC2d.cpp:1411-1417, sampled
+     * as init_particles 142-193 with ref_distribution, sampling.cpp:70-114):
+     * ninit regular boxes init_boxes[ninit], each with a signed distribution
+     * over the modes after the propagator's Gamma offset,
+     * init_dist[ninit][init_len] (init_len = modes - ngamma); 0 / NULL for
+     * none. These particles are kept together with those of boxes at
+     * Tinit != Teq (the reference's later init_particles clears them,
+     * beyondRTA_MC2d.cpp:2096-2105). */
+    int ninit;
+    int init_len;
+    const int32_t* init_boxes;
+    const double* init_dist;
 } bte_bert_cfg;
 
 typedef struct {
@@ -309,7 +330,9 @@ int bte_bert_destroy(bte_bert* b);
 typedef struct {
     double build_ms;   /* device time of the whole build */
     double gemm_ms;    /* device time in the block GEMMs */
-    double gemm_tflops;
+    double gemm_tflops; /* algorithmic: 2 N^3 per step over gemm_ms, whatever the
+                           number of partial products (BF16 forms each product
+                           from three) */
     double max_col_sum_err; /* max_j |sum_i P_ij - 1| after the build */
     int breakdowns;    /* columns whose Krylov space closed early */
 } bte_krylov_report;

@@ -107,6 +107,9 @@ struct BertSetup {
     uint64_t Efactor = 0;        // number of particles for calc_Eeff
     double Tmax = 0, Tmin = 0;
     int pcancel_each = 4;
+    // <init_distribution box file> (beyondRTA_MC2d.cpp:1411-1417): a given
+    // signed distribution per regular box (n - ngamma entries)
+    std::map<int, std::vector<double>> init_dist;
 
     double Eeff = 0;
 };
@@ -394,6 +397,25 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
 
     pcg32 hostrng(seed, 0xb0a7u);  // the serial generators' stream
     std::vector<BPart> parts;
+    // init_particles from given distributions (beyondRTA_MC2d.cpp:142-193,
+    // ref_distribution sampling.cpp:70-114): Ntogenerate(Vbox/Vucell, Eeff)
+    // particles at uniform points, mode and sign from the distribution. The
+    // reference's second init_particles (from Tinit, below) clears the list
+    // first (beyondRTA_MC2d.cpp:2096-2105, 218), so given distributions never
+    // survive there; both sets are kept here.
+    for (auto& [ib, fd] : S.init_dist) {
+        if (ib < 0 || ib >= (int)sys.size()) throw input_error("init_distribution box out of range");
+        const Box& s = sys[ib];
+        if (s.reservoir || s.periodic) continue;
+        GridDist d = ref_distribution(m, fd, S.ngamma);
+        size_t ngen = n_to_generate((s.area * S.thickness / vuc) * d.cumulsum / (Eeff * m.nq), hostrng);
+        for (size_t i = 0; i < ngen; i++) {
+            double x = s.xmin + hostrng.uniform() * (s.xmax - s.xmin);
+            double y = s.ymin + hostrng.uniform() * (s.ymax - s.ymin);
+            size_t pos = d.sample(hostrng);
+            parts.push_back({x, y, (int)pos, s.id, d.signs[pos], false});
+        }
+    }
     // init_particles from Tinit != Teq (beyondRTA_MC2d.cpp:211-269)
     for (auto& s : sys) {
         if (s.reservoir || s.periodic) continue;

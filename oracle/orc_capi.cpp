@@ -359,6 +359,13 @@ void orc_rta_ext(double wscale, const double* spec4, int gmask) {
     for (int k = 0; k < 4; k++) g_ext_spec[k] = spec4 ? spec4[k] : 0.;
     g_ext_gmask = gmask;
 }
+// per-mode base_sigma of the Lambert bounce of the next orc_rta_run (NULL:
+// the constant sigma)
+static thread_local std::vector<double> g_ext_msig;
+void orc_rta_mode_sigma(int n, const double* sigma) {
+    if (sigma) g_ext_msig.assign(sigma, sigma + n);
+    else g_ext_msig.clear();
+}
 // dT/dp of the last run: [4][nt][nbox]
 int orc_rta_grad_get(double* out) {
     if (g_grad_out.empty()) return 1;
@@ -384,6 +391,9 @@ int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, 
         S.wscale = g_ext_wscale;
         for (int k = 0; k < 4; k++) S.spec[k] = g_ext_spec[k];
         S.gmask = g_ext_gmask;
+        S.mode_sigma.swap(g_ext_msig);
+        g_ext_msig.clear();
+        if (!S.mode_sigma.empty() && (int)S.mode_sigma.size() != n) throw input_error("per-mode sigma of the wrong size");
         g_ext_wscale = 1.0;
         for (int k = 0; k < 4; k++) g_ext_spec[k] = 0.;
         g_ext_gmask = 0;
@@ -411,12 +421,13 @@ int orc_rta_run(int n, const double* omega, const double* vx, const double* vy, 
 
 // diffusive_bouncing alone: ns bounces of `mode` at a side with outward
 // normal (npx, npy), stream k of `seed` for bounce k; out: the new modes
+// (msig: per-mode base_sigma or NULL for sigma on every mode)
 int orc_lambert_sample(int n, const double* omega, const double* vx, const double* vy, double sigma, double npx,
-                       double npy, int mode, uint64_t seed, int ns, int32_t* out) {
+                       double npy, int mode, uint64_t seed, int ns, int32_t* out, const double* msig) {
     try {
         std::vector<double> w(n, 1.0);
         Modes m = mk_modes(n, omega, vx, vy, w.data(), 1.0, 1.0, 1);
-        const LambertTable L = lambert_table(m, sigma);
+        const LambertTable L = lambert_table(m, sigma, msig ? std::vector<double>(msig, msig + n) : std::vector<double>());
         for (int k = 0; k < ns; k++) {
             pcg32 rng(seed, (uint64_t)k);
             out[k] = lambert_bounce(m, L, npx, npy, mode, rng);
@@ -444,7 +455,7 @@ int orc_rta_event(int n, const double* omega, const double* vx, const double* vy
         S.sys = mk_sys(nbox, rect, kind, Teq.data(), nullptr);
         S.ballistic = true;
         S.sigma = sigma;
-        S.lambert = lambert_table(S.m, sigma);
+        S.lambert = lambert_table(S.m, sigma, {});
         S.timegrid = {0.0, 1.0e30};
         S.ntime = 2;
         if (ibox < 0 || ibox >= nbox || mode < 0 || mode >= n) throw input_error("bad particle");
@@ -510,6 +521,13 @@ static thread_local std::vector<double> g_bT;
 static thread_local std::vector<std::vector<double>> g_bP;
 static thread_local int g_brule = 0, g_bfixed = 0, g_bcancel = 4;
 static thread_local double g_bgx = 0., g_bgy = 0.;
+// given initial distributions of the next orc_bert_run: ninit boxes, each
+// with len entries (the modes after the Gamma offset)
+static thread_local std::map<int, std::vector<double>> g_binit;
+void orc_bert_init_dist(int ninit, const int32_t* boxes, int len, const double* fd) {
+    g_binit.clear();
+    for (int k = 0; k < ninit; k++) g_binit[boxes[k]].assign(fd + (size_t)k * len, fd + (size_t)(k + 1) * len);
+}
 void orc_bert_options(int ncol, int ntemp, const double* Ts, const double* Ps, int rule, int fixed_gradient,
                       double gx, double gy, int pcancel_each) {
     std::vector<int> ord(ntemp);
@@ -555,6 +573,8 @@ int orc_bert_run(int n, const double* omega, const double* vx, const double* vy,
         S.gx = g_bgx;
         S.gy = g_bgy;
         S.pcancel_each = g_bcancel;
+        S.init_dist.swap(g_binit);
+        g_binit.clear();
         g_bT.clear();
         g_bP.clear();
         g_brule = g_bfixed = 0;

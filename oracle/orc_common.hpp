@@ -198,6 +198,27 @@ inline size_t n_to_generate(double gen, pcg32& rng) {
 }
 
 // outTref_distribution (sampling.cpp:271-330)
+// ref_distribution (sampling.cpp:70-93): a given signed distribution fd over
+// the modes after the ngamma Gamma entries (mode ngamma + j <- fd[j]);
+// pmf |fd|, sign of fd, cumulsum = sum |fd|
+inline GridDist ref_distribution(const Modes& m, const std::vector<double>& fd, int ngamma) {
+    if ((int)fd.size() + ngamma != m.n) throw value_error("given distribution of the wrong length");
+    GridDist d;
+    d.nq = m.nq;
+    std::vector<double> pmf(m.n, 0.);
+    d.signs.assign(m.n, 1);
+    double cs = 0;
+    for (size_t j = 0; j < fd.size(); j++) {
+        const int k = (int)j + ngamma;
+        d.signs[k] = std::signbit(fd[j]) ? -1 : 1;
+        pmf[k] = std::fabs(fd[j]);
+        cs += std::fabs(fd[j]);
+    }
+    d.cumulsum = cs;
+    d.fill_cumulative(pmf);
+    return d;
+}
+
 inline GridDist out_tref(const Modes& m, double Tinit, double Tref) {
     if (Tinit <= 0 || Tref <= 0) throw value_error("Invalid temperature value");
     GridDist d;

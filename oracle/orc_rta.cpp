@@ -32,20 +32,37 @@ namespace orc {
 
 using LambertTable = std::vector<std::vector<std::pair<int, double>>>;
 
+// The broadening floor of the reference: modes below exp(p25 - 1.5 (p75 -
+// p25)) of log sigma are raised to it (calc_percentiles_log,
+// isotopic_scattering.cpp:31-50, applied at RTA_MC2D.cpp:572-576).
+static std::vector<double> floored_sigma(std::vector<double> sig) {
+    const size_t n = sig.size();
+    std::vector<double> ls(n);
+    for (size_t i = 0; i < n; i++) ls[i] = sig[i] == 0. ? std::numeric_limits<double>::lowest() : std::log(sig[i]);
+    std::sort(ls.begin(), ls.end());
+    const double p0 = ls[25 * n / 100], p1 = ls[75 * n / 100];
+    const double lb = std::exp(p0 - 1.5 * (p1 - p0));
+    for (auto& v : sig) v = std::max(v, lb);
+    return sig;
+}
+
 // diffusive_bouncing table (RTA_MC2D.cpp:556-626): for mode i every mode f
 // with nonzero omega within nsigma * hypot(sigma_i, sigma_f) in frequency,
-// weighted by the normal pdf of |omega_i - omega_f|; sigma_i = sigma for all
-// modes (a synthetic table has no q-grid for base_sigma).
-static LambertTable lambert_table(const Modes& m, double sigma) {
+// weighted by the normal pdf of |omega_i - omega_f| with that width. The
+// per-mode sigma_i is base_sigma (RTA_MC2D.cpp:566-570) after the floor of
+// floored_sigma; a table without it (a synthetic one has no q-grid) uses
+// sigma for every mode.
+static LambertTable lambert_table(const Modes& m, double sigma, const std::vector<double>& mode_sigma) {
     LambertTable tab(m.n);
-    double sig = std::hypot(sigma, sigma);
-    double norm = 1.0 / (sig * std::sqrt(2 * PI));
+    std::vector<double> bs = mode_sigma.empty() ? std::vector<double>(m.n, sigma) : floored_sigma(mode_sigma);
     for (int i = 0; i < m.n; i++) {
         if (almost_equal(0., m.omega[i])) continue;
         for (int f = 0; f < m.n; f++) {
             if (almost_equal(0., m.omega[f])) continue;
+            const double sig = std::hypot(bs[i], bs[f]);
             double d = std::fabs(m.omega[i] - m.omega[f]);
-            if (d <= nsigma * sig) tab[i].push_back({f, norm * std::exp(-0.5 * d * d / (sig * sig))});
+            if (d <= nsigma * sig)
+                tab[i].push_back({f, std::exp(-0.5 * d * d / (sig * sig)) / (sig * std::sqrt(2 * PI))});
         }
     }
     return tab;
@@ -82,6 +99,7 @@ struct RtaSetup {
     double thickness = 1.0, T0 = 300.0, dt = 1.0, maxtime = 10.0;
     bool ballistic = false;
     double sigma = 0.25;  // Lambert broadening per mode [rad/ps]
+    std::vector<double> mode_sigma;  // per-mode base_sigma (empty: sigma for all)
     double wscale = 1.0;  // scattering-rate factor
     double spec[4] = {0., 0., 0., 0.};  // specular probability: left, bottom, top, right
     int gmask = 0;        // sides whose specularity is differentiated (0 < p < 1)
@@ -128,7 +146,8 @@ struct RtaSetup {
         for (int i = 0; i < nomega; i++)
             omegagrid[i] = (nomega > 1 ? maxomega * i / (nomega - 1) : maxomega) + 0.5 * delta;
         bsig.resize(m.n);
-        for (int i = 0; i < m.n; i++) bsig[i] = spec_sigma.empty() ? sigma : spec_sigma[i];
+        for (int i = 0; i < m.n; i++)
+            bsig[i] = !spec_sigma.empty() ? spec_sigma[i] : (!mode_sigma.empty() ? mode_sigma[i] : sigma);
         // outlier floor exp(p25 - 1.5 (p75 - p25)) of log sigma
         // (calc_percentiles_log, isotopic_scattering.cpp:31-50)
         std::vector<double> ls(m.n);
@@ -183,7 +202,7 @@ struct RtaSetup {
         }
         Eeff = einfo1 / (double)nparticles;
         // Lambert table (RTA_MC2D.cpp:556-626) with sigma_i = sigma for all modes
-        lambert = lambert_table(m, sigma);
+        lambert = lambert_table(m, sigma, mode_sigma);
         // mirror partners: same omega, one velocity component negated, exact match
         mirror.assign(2 * (size_t)m.n, -1);
         bool anyspec = false;

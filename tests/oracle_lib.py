@@ -188,13 +188,14 @@ def set_periodic(nbox, periodic):
 
 
 def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, ballistic=False, sigma=0.25,
-            nthreads=0, periodic=None, spectral=None, wscale=1.0, specularity=None, grad=False):
+            nthreads=0, periodic=None, spectral=None, wscale=1.0, specularity=None, grad=False, mode_sigma=None):
     """spectral: {"boxes": [...], "nomega": K, "sigma": per-mode array or
     None}; the result then holds "spectral": one dict per location with
     omega, dT, jx, jy [nt][K] and fd [nt][nq]. wscale multiplies every
     scattering rate; specularity (left, bottom, top, right) is the specular
     probability of each side; grad adds "dTdp" [4][nt][nbox] for the sides
-    with 0 < p < 1 (zeros elsewhere)."""
+    with 0 < p < 1 (zeros elsewhere). mode_sigma: per-mode base broadening of
+    the Lambert bounce (None: sigma for every mode)."""
     rect, Teq = _f(rect), _f(Teq)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
@@ -216,6 +217,8 @@ def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, 
     om, vx, vy, w = (_f(mat[k]) for k in ("omega", "vx", "vy", "w"))
     sp = _f(specularity if specularity is not None else (0.0, 0.0, 0.0, 0.0))
     lib().orc_rta_ext(C.c_double(wscale), _dp(sp), 15 if grad else 0)
+    ms = None if mode_sigma is None else _f(mode_sigma)
+    lib().orc_rta_mode_sigma(len(mat["omega"]), None if ms is None else _dp(ms))
     rc = lib().orc_rta_run(len(om), _dp(om), _dp(vx), _dp(vy), _dp(w), C.c_double(mat["V"]),
                            C.c_double(mat.get("lattz", 1.0)), nq, nbox, _dp(rect), kind.ctypes.data_as(ip32),
                            _dp(Teq), C.c_double(thickness), C.c_double(T0), C.c_double(dt), C.c_double(maxtime),
@@ -269,9 +272,10 @@ def rta_steady_run(mat, rect, kind, Teq, thickness, T0, gradient, nparticles, se
 
 def bert_run(mat, rect, kind, Teq, Tinit, thickness, dt, maxtime, P=None, ngamma=0, rta=False, Ed=-1.0,
              Efactor=0, Tmax=0.0, Tmin=0.0, seed=1, max_steps=0, nthreads=0, periodic=None, Ps=None, Ts=None,
-             prop_rule=0, gradient=None, pcancel_each=4):
+             prop_rule=0, gradient=None, pcancel_each=4, init_dist=None):
     """Ps/Ts: propagators at several temperatures (replace P); gradient:
-    (gx, gy) in every regular box instead of calculate_gradientT"""
+    (gx, gy) in every regular box instead of calculate_gradientT; init_dist:
+    {box: signed distribution over the modes after the ngamma offset}"""
     rect, Teq, Tinit = _f(rect), _f(Teq), _f(Tinit)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
@@ -288,6 +292,15 @@ def bert_run(mat, rect, kind, Teq, Tinit, thickness, dt, maxtime, P=None, ngamma
                            int(pcancel_each))
     set_periodic(nbox, periodic)
     n = len(mat["omega"])
+    if init_dist:
+        ib = np.ascontiguousarray(sorted(init_dist), np.int32)
+        lens = {len(init_dist[b]) for b in ib}
+        if len(lens) != 1:
+            raise ValueError("given distributions of different lengths")
+        fd = np.ascontiguousarray(np.concatenate([np.asarray(init_dist[b], np.float64) for b in ib]))
+        lib().orc_bert_init_dist(len(ib), ib.ctypes.data_as(ip32), lens.pop(), _dp(fd))
+    else:
+        lib().orc_bert_init_dist(0, None, 0, None)
     nst = 0
     t = 0.0
     while t <= maxtime:
@@ -378,13 +391,15 @@ def project_columns(B):
     return B
 
 
-def lambert_sample(omega, vx, vy, sigma, npx, npy, mode, seed, ns):
-    """ns diffuse bounces of `mode` at a side with outward normal (npx, npy)."""
+def lambert_sample(omega, vx, vy, sigma, npx, npy, mode, seed, ns, mode_sigma=None):
+    """ns diffuse bounces of `mode` at a side with outward normal (npx, npy);
+    mode_sigma: per-mode base broadening (None: sigma for every mode)."""
     om, ux, uy = _f(omega), _f(vx), _f(vy)
     out = np.zeros(ns, np.int32)
+    ms = None if mode_sigma is None else _f(mode_sigma)
     if lib().orc_lambert_sample(len(om), _dp(om), _dp(ux), _dp(uy), C.c_double(sigma), C.c_double(npx),
                                 C.c_double(npy), int(mode), u64(seed), int(ns),
-                                out.ctypes.data_as(ip32)) != 0:
+                                out.ctypes.data_as(ip32), None if ms is None else _dp(ms)) != 0:
         raise RuntimeError(lib().orc_last_error().decode())
     return out
 

@@ -291,3 +291,55 @@ def test_reference_example_bar(pkg, tmp_path):
     run([pkg.app_path("dist_reader"), "berta.xml", "properties.msgpack.bin"], tmp_path)
     d = np.loadtxt(tmp_path / "deltaT_omega_5.csv", delimiter=",")
     assert d.shape == (3, 51) and d[1, 0] == 2.0 and d[2, 0] == 5.0
+
+
+def test_rtamc2d_reads_per_mode_sigma_from_the_mode_file(pkg, ctx, tmp_path):
+    """A .modes file with a fifth column gives the per-mode bounce broadening
+    (base_sigma): RTAMC2D then writes what the library gives with
+    ModeTable.set_sigma on the same seeds, and something else than with the
+    constant sigma_bounce."""
+    W = pkg.workloads
+    rect, kind, teq, _ = W.ribbon(60.0, 20.0, 6, 300.0, reservoirs=(300.5, 299.5))
+    write_ribbon(tmp_path / "g.xml", rect, kind, teq, teq)
+    mat = W.material(300, seed=4)
+    sig = 0.2 + 0.3 * np.hypot(mat["vx"], mat["vy"])
+    rows = "\n".join(" ".join(repr(float(v)) for v in r) for r in
+                     zip(mat["omega"], mat["vx"], mat["vy"], mat["w"], sig))
+    (tmp_path / "m.modes").write_text(f"300 1 {float(mat['V'])!r} {float(mat['lattz'])!r}\n{rows}\n")
+    (tmp_path / "in.xml").write_text(
+        '<RTA_MC2d>\n<geometry file="g.xml"/>\n<time dt="20" maxtime="200"/>\n'
+        '<material name="m" modetable="m.modes" thickness="1" T0="300" sigma_bounce="2.0"/>\n'
+        '<particles N="20000" seed="5"/>\n</RTA_MC2d>\n')
+    run([pkg.app_path("RTAMC2D"), "in.xml", "8", "1"], tmp_path)
+    T = np.loadtxt(tmp_path / "temperature_300K_run_0.csv", delimiter=",")
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    kw = dict(thickness=1.0, T0=300.0, dt=20.0, maxtime=200.0, nparticles=20000, seed=5, sigma_bounce=2.0)
+    const = pkg.rtamc2d(ctx, modes, geom, **kw)
+    modes.set_sigma(sig)
+    o = pkg.rtamc2d(ctx, modes, geom, **kw)
+    assert np.allclose(T[:, 1:], o["T"], rtol=1e-5, atol=1e-9)
+    assert not np.allclose(o["T"], const["T"], rtol=0, atol=1e-9)
+
+
+def test_bertamc2d_reads_init_distribution(pkg, ctx, tmp_path):
+    """<init_distribution box file/>: column 0 of a tab-separated file,
+    '#' lines skipped (get_column, beyondRTA_MC2d.cpp:1270-1291). The
+    executable starts from the same particles as the library run with the
+    same distribution and seed (equal initial count and properties)."""
+    W = pkg.workloads
+    rect, kind, teq, tin = W.ribbon(60.0, 30.0, 6, 300.0)
+    write_ribbon(tmp_path / "g.xml", rect, kind, teq, tin)
+    mat = W.material(200, seed=2)
+    n = len(mat["omega"])
+    fd = 1e-23 * (1.0 + np.arange(n) % 7) * np.where(np.arange(n) % 5 == 0, -1.0, 1.0)
+    (tmp_path / "fd.dat").write_text("# given distribution\n" + "\n".join(f"{float(v)!r}\t0" for v in fd) + "\n")
+    (tmp_path / "in.xml").write_text(
+        '<beRTAMC2D>\n<geometry file="g.xml"/>\n<time dt="5" maxtime="20"/>\n'
+        '<material name="m" modetable="synthetic" nmodes="200" seed="2" thickness="1" T0="300">\n'
+        '  <propagator T="300" file="synthetic" density="0.05" seed="1" m="20"/>\n</material>\n'
+        '<Eeff Ed="1e-22"/>\n<init_distribution box="2" file="fd.dat"/>\n<seed value="9"/>\n</beRTAMC2D>\n')
+    r = run([pkg.app_path("beRTAMC2D"), "in.xml", "2"], tmp_path)
+    assert "init_distribution" in r.stdout
+    first = [l for l in r.stdout.splitlines() if l and not l.startswith("#")][0].split("\t")
+    assert int(first[2]) > 100   # particles at step 0

@@ -55,6 +55,32 @@ def test_rta_matches_oracle(pkg, ctx):
     assert_3sigma(zs, "RTAMC2D vs oracle")
 
 
+def test_rta_per_mode_sigma_matches_oracle(pkg, ctx):
+    """Per-mode bounce broadening (ModeTable.set_sigma, the reference's
+    base_sigma with its low-outlier floor and pair width hypot(sigma_i,
+    sigma_f), RTA_MC2D.cpp:566-618) on a narrow ribbon where most events are
+    wall bounces: sigma grows with the mode's speed and a few modes sit far
+    below the floor. 16 replicas per side."""
+    W = pkg.workloads
+    mat = W.material(2000, seed=12)
+    sig = 0.3 + 1.5 * np.hypot(mat["vx"], mat["vy"]) / 8.0
+    sig[::100] = 1e-5
+    rect, kind, teq, _ = W.ribbon(200.0, 30.0, 20, 300.0, reservoirs=(300.5, 299.5))
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    modes.set_sigma(sig)
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    kw = dict(thickness=1.0, T0=300.0, dt=20.0, maxtime=400.0)
+    R, N = 16, 20000
+    g = [pkg.rtamc2d(ctx, modes, geom, nparticles=N, seed=500 + r, sigma_bounce=1.0, **kw) for r in range(R)]
+    o = [orc.rta_run(mat, rect, kind, teq, nparticles=N, seed=600 + r, sigma=1.0, mode_sigma=sig, **kw)
+         for r in range(R)]
+    zs = [zstats(np.array([x[key][[4, -1], :20] for x in g]), np.array([x[key][[4, -1], :20] for x in o]))
+          for key in ("T", "jx")]
+    for k in ("events", "crossings"):
+        zs.append(zstats(np.array([float(x[k]) for x in g]), np.array([float(x[k]) for x in o])))
+    assert_3sigma(zs, "RTAMC2D with per-mode sigma vs oracle")
+
+
 def test_bert_matches_oracle(pkg, ctx):
     W = pkg.workloads
     mat, B = W.config1_inputs(nmodes=300, seed=5)
@@ -86,6 +112,42 @@ def test_bert_matches_oracle(pkg, ctx):
     for x in g + o:
         tot = x["npart"] + x["pending"]
         assert (tot == tot[0]).all(), tot
+
+
+def test_bert_given_initial_distribution_matches_oracle(pkg, ctx):
+    """<init_distribution> (beyondRTA_MC2d.cpp:142-193, ref_distribution
+    sampling.cpp:70-114): box 0 starts from a given signed distribution and
+    box 3 from Tinit != Teq on top of it; both sets of particles are kept.
+    The initial count, the energy and flux of every box at three steps and
+    the particle count at the end against the oracle, 12 replicas per
+    side."""
+    W = pkg.workloads
+    mat, B = W.config1_inputs(nmodes=240, seed=7)
+    import scipy.linalg
+    P = scipy.linalg.expm(4.0 * B)
+    rect, kind, teq, tin = W.ribbon(100.0, 40.0, 8, 300.0)
+    tin = tin.copy()
+    tin[3] = 301.0
+    n = len(mat["omega"])
+    rng = np.random.default_rng(3)
+    nb = orc.bose_einstein(mat["omega"], 302.0) - orc.bose_einstein(mat["omega"], 300.0)
+    fd = 1e12 * 1.054571726e-34 * mat["omega"] * nb * np.where(rng.random(n) < 0.3, -1.0, 1.0)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, rect, kind, teq, tin)
+    prop = pkg.Propagator(ctx, P)
+    kw = dict(thickness=1.0, dt=4.0, maxtime=4.0 * 19, Efactor=20000, Tmax=302.0, Tmin=300.0)
+    R = 12
+    g = [pkg.bertamc2d(ctx, modes, geom, prop, seed=40 + r, init_dist={0: fd}, **kw) for r in range(R)]
+    o = [orc.bert_run(mat, rect, kind, teq, tin, P=P, seed=60 + r, init_dist={0: fd}, **kw) for r in range(R)]
+    assert g[0]["npart"][0] > 1000 and np.abs(g[0]["props"][0, 0, 0]) > 0
+    zs = [zstats(np.array([x["props"][[0, 5, 20], :8, q] for x in g]), np.array([x["props"][[0, 5, 20], :8, q] for x in o]))
+          for q in (0, 1)]
+    for f in (lambda x: float(x["npart"][0]), lambda x: float(x["npart"][-1])):
+        zs.append(zstats(np.array([f(x) for x in g]), np.array([f(x) for x in o])))
+    assert_3sigma(zs, "given initial distribution vs oracle")
+    # a distribution of the wrong length is refused before any launch
+    with pytest.raises(pkg.BteError, match="init_distribution"):
+        pkg.BertRun(ctx, modes, geom, prop, seed=1, init_dist={0: fd[:-1]}, **kw)
 
 
 def test_krylov_matches_oracle(pkg, ctx):

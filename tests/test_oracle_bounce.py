@@ -169,3 +169,59 @@ def test_void_corner_between_two_boxes():
         assert (vy < 0) if box == 1 else (vx < 0)
         seen[box] += 1
     assert abs(seen[1] - R / 2) < 5 * np.sqrt(R / 4), seen
+
+
+def _three_groups(sig_i, sig_a, sig_b, d=0.5, n=16):
+    """mode group i at omega 5 and partner groups a at 5 + d and b at 5 - d,
+    each of n modes with the given per-mode broadening"""
+    g = [ring_modes(n, w, seed=s) for w, s in ((5.0, 21), (5.0 + d, 22), (5.0 - d, 23))]
+    om, vx, vy = (np.concatenate(x) for x in zip(*g))
+    sig = np.concatenate([np.full(n, sig_i), np.full(n, sig_a), np.full(n, sig_b)])
+    return om, vx, vy, sig
+
+
+def test_bounce_with_per_mode_sigma_follows_the_pair_width():
+    """Per-mode broadening (base_sigma, RTA_MC2D.cpp:566-618): a partner f of
+    mode i at frequency distance d has weight N(d; 0, hypot(sigma_i,
+    sigma_f)), the normal pdf with its 1/sigma factor, times the cosine
+    weight. With a wide group above and a narrow one below, the three
+    outcomes follow those odds; dropping the pdf's 1/sigma (which a constant
+    sigma would not notice) is rejected."""
+    d, si, sa, sb = 0.5, 0.3, 0.6, 0.2
+    om, vx, vy, sig = _three_groups(si, sa, sb, d)
+    w = np.maximum(-vy, 0.0)   # top side, n = (0, 1)
+    pdf = lambda x, s: np.exp(-0.5 * x * x / (s * s)) / (s * np.sqrt(2 * np.pi))
+    W = [w[:16].sum(), w[16:32].sum(), w[32:].sum()]
+    wt = np.array([W[0] * pdf(0.0, np.hypot(si, si)), W[1] * pdf(d, np.hypot(si, sa)), W[2] * pdf(d, np.hypot(si, sb))])
+    ns = 200000
+    out = orc.lambert_sample(om, vx, vy, 0.25, 0.0, 1.0, mode=2, seed=31, ns=ns, mode_sigma=sig)
+    cnt = np.bincount(out // 16, minlength=3)
+    assert stats.chisquare(cnt, ns * wt / wt.sum()).pvalue > 1e-4
+    # without the 1/sigma of the pdf the odds shift by far more than the noise
+    nopre = wt * np.array([np.hypot(si, si), np.hypot(si, sa), np.hypot(si, sb)])
+    assert stats.chisquare(cnt, ns * nopre / nopre.sum()).pvalue < 1e-12
+
+
+def test_bounce_sigma_floor_raises_low_outliers():
+    """Per-mode sigma below exp(p25 - 1.5 (p75 - p25)) of log sigma is
+    raised to that floor (calc_percentiles_log, isotopic_scattering.cpp:
+    31-50, applied at RTA_MC2D.cpp:572-576): a table whose outlier group is
+    far below the floor bounces exactly like the same table with that group
+    at the floor (same streams); had the floor been skipped, the outliers'
+    weights and so the draws would differ."""
+    om, vx, vy, sig = _three_groups(0.3, 0.6, 0.2)
+    sig = np.concatenate([sig, np.full(8, 1e-4)])          # 8 outliers far below the rest
+    extra = ring_modes(8, 5.3, seed=24)
+    om, vx, vy = (np.concatenate([a, b]) for a, b in zip((om, vx, vy), extra))
+    ls = np.sort(np.log(sig))
+    n = len(sig)
+    lb = np.exp(ls[25 * n // 100] - 1.5 * (ls[75 * n // 100] - ls[25 * n // 100]))
+    assert 1e-4 < lb < 0.2
+    floored = np.maximum(sig, lb)
+    a = orc.lambert_sample(om, vx, vy, 0.25, 0.0, 1.0, mode=2, seed=5, ns=20000, mode_sigma=sig)
+    b = orc.lambert_sample(om, vx, vy, 0.25, 0.0, 1.0, mode=2, seed=5, ns=20000, mode_sigma=floored)
+    assert np.array_equal(a, b)
+    # and the outliers do take part in the odds: at their own width the draws differ
+    c = orc.lambert_sample(om, vx, vy, 0.25, 0.0, 1.0, mode=2, seed=5, ns=20000,
+                           mode_sigma=np.concatenate([sig[:48], np.full(8, 0.2)]))
+    assert not np.array_equal(a, c)

@@ -259,3 +259,48 @@ def test_calc_Eeff_closed_form():
                       Tmax=310.0, Tmin=295.0, seed=1, max_steps=1)
     dE2 = (1e12 * HBAR * om * (np.abs(n_be(om, 310.0) - n_be(om, Tref)) + np.abs(n_be(om, 295.0) - n_be(om, Tref)))).sum()
     assert o2["Eeff"] == pytest.approx(vols.min() * dE2 / 1000, rel=1e-12)
+
+
+def test_bert_given_initial_distribution():
+    """<init_distribution> (beyondRTA_MC2d.cpp:142-193, ref_distribution
+    sampling.cpp:70-114): a box given a signed distribution fd over the
+    modes after the Gamma offset starts with Ntogenerate(Vbox/Vucell, Eeff) =
+    floor or ceil of Vbox/Vucell sum|fd| / (Eeff nq) particles, mode j +
+    ngamma with probability |fd_j| / sum|fd| and the sign of fd_j. Modes
+    with v = 0 and P = I stay put, so the histogram after one step is the
+    initial one. A box at Tinit != Teq adds its outTref particles on top:
+    both sets are kept (the reference clears the given ones,
+    beyondRTA_MC2d.cpp:2096-2105, 218)."""
+    n, ng = 40, 3
+    m = orc.synthetic_modes(n, 31)
+    m["vx"] = np.zeros(n)
+    m["vy"] = np.zeros(n)
+    m["omega"][:ng] = 0.0   # the Gamma acoustic modes of the offset
+    m["V"], m["lattz"] = 20.0, 1.0
+    rng = np.random.default_rng(5)
+    fd = rng.uniform(0.2, 1.0, n - ng) * np.where(rng.random(n - ng) < 0.4, -1.0, 1.0) * 1e-22
+    rect = np.array([[0.0, 50.0, 0.0, 40.0], [50.0, 100.0, 0.0, 40.0]])
+    kind = np.array([0, 0], np.int32)
+    teq = np.array([300.0, 300.0])
+    tin = np.array([300.0, 301.0])
+    Ed = 2e-23
+    P = np.eye(n - ng)
+    vol = 50.0 * 40.0 * 1.0 / (m["V"] * 1.0 / m["lattz"])
+    gen = vol * np.abs(fd).sum() / (Ed * 1)
+    counts = []
+    for seed in range(6):
+        o = orc.bert_run(m, rect, kind, teq, tin, thickness=1.0, dt=1.0, maxtime=0.5, P=P, ngamma=ng, Ed=Ed,
+                         seed=seed, max_steps=1, init_dist={0: fd})
+        h = o["H_last"][0].astype(float)
+        assert abs(h).sum() in (np.floor(gen), np.floor(gen) + 1)
+        assert (h[:ng] == 0).all()
+        nz = h[ng:] != 0
+        assert (np.sign(h[ng:][nz]) == np.sign(fd[nz])).all()
+        counts.append(np.abs(h[ng:]))
+        assert o["H_last"][1].sum() > 0     # box 1's outTref particles are there too
+    cnt = np.sum(counts, 0)
+    p = np.abs(fd) / np.abs(fd).sum()
+    assert stats.chisquare(cnt, cnt.sum() * p).pvalue > 1e-4
+    with pytest.raises(RuntimeError, match="length"):
+        orc.bert_run(m, rect, kind, teq, tin, thickness=1.0, dt=1.0, maxtime=0.5, P=P, ngamma=ng, Ed=Ed, seed=1,
+                     max_steps=1, init_dist={0: fd[:-1]})
