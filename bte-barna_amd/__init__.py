@@ -190,6 +190,15 @@ class RtaRun(_Handle):
         check(lib().bte_rta_spectral(self.h, len(b), b.ctypes.data_as(C.POINTER(C.c_int32)), int(nomega), sg))
         self.spectral_boxes, self.nomega = list(b), int(nomega)
 
+    def execute_noalloc(self, T_ptr, jx_ptr, jy_ptr, counts_ptr):
+        """bte_rta_execute_noalloc: the same run into device buffers (T, jx,
+        jy: (ntime-1) x nbox doubles; counts: 6 uint64 = events, crossings,
+        ids taken, stuck, no mirror, no translation), asynchronous on the
+        context's stream, with no device allocation. Pointers are device
+        addresses (ints)."""
+        check(lib().bte_rta_execute_noalloc(self.h, C.c_void_p(T_ptr), C.c_void_p(jx_ptr), C.c_void_p(jy_ptr),
+                                            C.c_void_p(counts_ptr)))
+
     def execute(self):
         nt = self.ntime - 1
         T = np.zeros((nt, self.nbox))

@@ -107,6 +107,7 @@ SIGNATURES = {
     "bte_geom_gradient_T": (C.c_int, [vp, dp]),
     "bte_rta_create": (C.c_int, [vp, vp, vp, C.POINTER(RtaCfg), pvp]),
     "bte_rta_execute": (C.c_int, [vp, dp, dp, dp, C.POINTER(RtaReport)]),
+    "bte_rta_execute_noalloc": (C.c_int, [vp, vp, vp, vp, vp]),
     "bte_rta_destroy": (C.c_int, [vp]),
     "bte_rta_spectral": (C.c_int, [vp, C.c_int, ip32, C.c_int, dp]),
     "bte_rta_spectral_get": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp]),

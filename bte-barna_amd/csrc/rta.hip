@@ -15,6 +15,7 @@
 #include <numeric>
 #include <tuple>
 #include <limits>
+#include <mutex>
 #include <rocblas/rocblas.h>
 
 namespace bte {
@@ -1192,6 +1193,113 @@ static void batch_tune(RtaBatch* b, double fraction) {
     }
 }
 
+// ---- the allocation-free variant of bte_rta_execute ----
+//
+// Its tally copies, counters and per-workgroup LDS blocks live in this
+// file-scope device array instead of buffers of the handle. Every GPU the
+// library runs on holds one copy until the process ends; the host reaches it
+// only through the address hipGetSymbolAddress gives for the current device
+// (the array's name is not a device address in host code).
+#define RTA_SCRATCH_BYTES (64ull << 20)
+static __device__ __align__(256) unsigned char scratch[RTA_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // per device, looked up once
+
+// the scratch array's device address on device dev (current device)
+static void* scratch_address(int dev) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (dev < 0) throw Error(BTE_EINVAL, "bad device");
+    if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(BTE_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+// Post-processing of one run on the device (RTA_MC2D.cpp:2336-2435, the
+// same arithmetic as batch_execute): per box the cumulative sums over time
+// of gz E / vbox, jx E 1e30 / vbox, jy E 1e30 / vbox, T = sum / (Cv / vcorr)
+// + T0; outputs [nt][nbox]. One thread per box.
+__global__ __launch_bounds__(256) void rta_post_kernel(const double* tally, uint64_t cells, const DevBox* boxes,
+                                                       int nbox, int nt, double E, double th, double cv,
+                                                       double vcorr, double T0, double* T, double* JX,
+                                                       double* JY, const unsigned long long* cnt,
+                                                       unsigned long long* cnt_out) {
+    const int bx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x < 6) cnt_out[threadIdx.x] = cnt[threadIdx.x];
+    if (bx >= nbox) return;
+    const DevBox b = boxes[bx];
+    const double vbox = (b.xmax - b.xmin) * (b.ymax - b.ymin) * th;
+    const double* gz = tally + (uint64_t)bx * nt;
+    const double* jx = gz + cells;
+    const double* jy = jx + cells;
+    double agz = 0., ajx = 0., ajy = 0.;
+    for (int k = 0; k < nt; k++) {
+        agz += gz[k] * E / vbox;
+        ajx += jx[k] * E * 1e30 / vbox;
+        ajy += jy[k] * E * 1e30 / vbox;
+        const size_t o = (size_t)k * nbox + bx;
+        T[o] = agz / (cv / vcorr) + T0;
+        JX[o] = ajx;
+        JY[o] = ajy;
+    }
+}
+
+static void rta_execute_noalloc(RtaBatch* b, double* d_T, double* d_jx, double* d_jy, unsigned long long* d_counts) {
+    Ctx* c = b->ctx;
+    if (b->nruns != 1) throw Error(BTE_EINVAL, "bte_rta_execute_noalloc takes a single-run handle");
+    if (b->nloc) throw Error(BTE_EINVAL, "bte_rta_execute_noalloc does not tally spectral locations");
+    if (b->any_grad) throw Error(BTE_EINVAL, "bte_rta_execute_noalloc does not tally the gradient sides");
+    if (!d_T || !d_jx || !d_jy || !d_counts) throw Error(BTE_EINVAL, "null argument");
+    HIPCHECK(hipSetDevice(c->device));
+    double* base = (double*)scratch_address(c->device);
+    // the tally mode bte_rta_execute picks, without touching the handle's
+    const int mode = b->lbins > 0 ? 2 : 0;
+    const int block = mode == 2 ? 768 : 256, blocks = mode == 2 ? c->cu_count : c->cu_count * 8;
+    const size_t ncopy = (size_t)RTA_NCOPY * 3 * b->cells;
+    const size_t ncnt = 8;  // the counters, as unsigned 64-bit words
+    const size_t npart = mode == 2 ? (size_t)blocks * 3 * b->lrows * b->lbins : 0;
+    const size_t need = (ncopy + ncnt + npart) * sizeof(double);
+    if (need > RTA_SCRATCH_BYTES)
+        throw Error(BTE_EINVAL, "bte_rta_execute_noalloc needs " + std::to_string(need) +
+                                    " bytes of scratch for this run's tallies; the limit is " +
+                                    std::to_string(RTA_SCRATCH_BYTES));
+    RtaParams P = batch_params(b, b->d_runs.p, b->total);
+    P.gz = base;
+    P.jx = base + b->cells;
+    P.jy = base + 2 * b->cells;
+    P.dgz = nullptr;
+    P.counters = (unsigned long long*)(base + ncopy);
+    P.part = base + ncopy + ncnt;
+    hipStream_t s = c->stream;
+    // the scratch keeps the previous call's contents: the tally copies and
+    // the counters start at zero (a kernel, not a memset, so that a graph
+    // capture records no memset node); the partial blocks are overwritten
+    hipLaunchKernelGGL(rta_clear_kernel, dim3(1024), dim3(256), 0, s, base, (uint64_t)ncopy, P.counters);
+    if (mode == 2) {
+        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&rta_history_kernel<false, 2>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_per_block));
+        hipLaunchKernelGGL((rta_history_kernel<false, 2>), dim3(blocks), dim3(block), b->lg_bytes, s, P);
+        const uint32_t nf = 3u * b->lrows * b->lbins;
+        hipLaunchKernelGGL(rta_sum_parts_kernel, dim3(std::min<uint32_t>((nf + 255) / 256, 4096u)), dim3(256), 0, s,
+                           base, P.part, blocks, b->lrows, b->lbins, b->d_lrow_cell.p, b->cells);
+    } else {
+        hipLaunchKernelGGL((rta_history_kernel<false, 0>), dim3(blocks), dim3(block), 0, s, P);
+    }
+    hipLaunchKernelGGL(rta_sum_copies_kernel, dim3(1024), dim3(256), 0, s, base, 3 * b->cells);
+    const int nb = b->nbox[0], nt = b->ntime - 1;
+    const double th = b->thickness;
+    hipLaunchKernelGGL(rta_post_kernel, dim3((nb + 255) / 256), dim3(256), 0, s, base, b->cells, b->d_boxes.p, nb, nt,
+                       b->Eeff[0], th, b->cv[0], th / b->m->lattz, b->desc[0].T0, d_T, d_jx, d_jy, P.counters,
+                       d_counts);
+    HIPCHECK(hipGetLastError());
+}
+
 }  // namespace bte
 
 using namespace bte;
@@ -1401,6 +1509,14 @@ int bte_spectra(bte_ctx* ctx, const bte_modes* mm, const double* sigma, double s
         throw Error(BTE_EHIP, "rocblas_dgemm failed");
     HIPCHECK(hipMemcpyAsync(out, dO.p, (size_t)nrows * ncols * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+    BTE_GUARD_END
+}
+
+int bte_rta_execute_noalloc(bte_rta* rr, double* d_T, double* d_jx, double* d_jy, unsigned long long* d_counts) {
+    BTE_GUARD_BEGIN
+    auto* b = reinterpret_cast<RtaBatch*>(rr);
+    if (!b) throw Error(BTE_EINVAL, "null handle");
+    rta_execute_noalloc(b, d_T, d_jx, d_jy, d_counts);
     BTE_GUARD_END
 }
 

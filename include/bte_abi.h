@@ -124,7 +124,21 @@ int bte_rta_create(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bt
                    bte_rta** out);
 int bte_rta_execute(bte_rta* r, double* T_out, double* jx_out, double* jy_out,
                     bte_rta_report* rep);
-
+/* No counterpart in the reference: bte_rta_execute for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops). The same run (same streams, tally mode and post-processing) with
+ * its tally copies, counters and per-workgroup blocks in the library's
+ * static device scratch array of 64 MiB per GPU instead of buffers it
+ * allocates; runs whose tallies need more fail with BTE_EINVAL before any
+ * launch. Outputs are device pointers: T, jx, jy [(ntime-1) x nbox] doubles
+ * as bte_rta_execute writes them, counts[6] uint64 (events, crossings, ids
+ * taken, stuck histories, missing mirror partners, failed translations; the
+ * last three are errors of the run). Asynchronous on the context's stream;
+ * it makes no device allocation and no host synchronisation. Single-run
+ * handles without spectral locations or gradient sides only. Calls that
+ * overlap on one GPU share the scratch array and race: serialise them (one
+ * stream, or the caller's own ordering). */
+int bte_rta_execute_noalloc(bte_rta* r, double* T, double* jx, double* jy, unsigned long long* counts);
 /* Spectral decomposition (<spectral><resolution ticks/><location bin/>,
  * RTA_MC2D.cpp:1930-1945, 1978-2052; process_segment 718-766; outputs
  * 2349-2405, 2473-2541), set before bte_rta_execute: the segments in the nloc
