@@ -126,13 +126,17 @@ def test_bert_capacity_overflow_is_an_error(pkg, ctx):
                       Tmax=305.0, Tmin=300.0, seed=1, pcancel_each=1000, capacity=1 << 21)
 
 
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 5e-5), ("bf16", 5e-3)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-11), ("fp32", 2e-5), ("bf16", 5e-5)])
 def test_krylov_precisions_on_golden_operator(pkg, ctx, prec, tol):
+    """BF16 within the tolerance that hi-only BF16 arithmetic misses by more
+    than ten times (test_krylov_bf16_control.py)"""
     import os
     fix = np.load(os.path.join(os.path.dirname(__file__), "golden", "expm_fixtures.npz"))
     A, E = fix["ecop_n128_A"], fix["ecop_n128_E"]
     P, rep = pkg.build_P(ctx, A, 1.0, m=40, precision=prec, arnoldi="textbook")
-    assert np.abs(P - E).sum(0).max() < tol
+    l1 = np.abs(P - E).sum(0).max()
+    print(f"{prec}: max column L1 {l1:.3e}", flush=True)
+    assert l1 < tol
     np.testing.assert_allclose(P.sum(0), 1.0, atol=1e-12)
 
 

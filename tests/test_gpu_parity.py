@@ -159,9 +159,10 @@ def test_krylov_matches_oracle(pkg, ctx):
         assert np.abs(Pg - Po).max() < 1e-10, (mode, np.abs(Pg - Po).max())
     import scipy.linalg
     Pe = scipy.linalg.expm(5.0 * B)
-    for prec, tol in (("fp64", 1e-11), ("fp32", 2e-5), ("bf16", 2e-3)):
+    for prec, tol in (("fp64", 1e-11), ("fp32", 2e-5), ("bf16", 5e-5)):
         Pg, rep = pkg.build_P(ctx, B, 5.0, m=30, precision=prec, arnoldi="textbook")
         l1 = np.abs(Pg - Pe).sum(0).max()
+        print(f"{prec}: max column L1 {l1:.3e}", flush=True)
         assert l1 < tol, (prec, l1)
         assert np.abs(Pg.sum(0) - 1).max() < 1e-12
 
