@@ -868,81 +868,89 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
     block_add_u64(P.stats + 1, done);
 }
 
-// Fixed-rate sources, one or more workgroups each with a fixed alias table:
+// Fixed-rate sources, one or more parts each with a fixed alias table:
 // reservoir injection (Isothermal_walls, beyondRTA_MC2d.cpp:446-585) on a
 // contact segment p1 + t nd with a fixed sign, and gradient sources
 // (source_term_gradient, 597-676) at uniform points of a box (area: p1 the
-// lower corner, nd the box size) with the sign in the alias entry.
+// lower corner, nd the box size) with the sign in the alias entry. At most
+// BERT_SRC_BLOCKS workgroups run the parts, each a contiguous range of
+// them into its own buffer segment, so the segment count does not grow
+// with the number of sources.
 struct BertSource {
     double p1x, p1y, ndx, ndy;
     double expected;  // particles per step
     int32_t box, sign;
     int32_t tab;      // alias table offset (in entries)
     int32_t src;      // source index (keys its random streams)
-    int32_t part, nparts;  // this workgroup's share of the source's particles
+    int32_t part, nparts;  // this part's share of the source's particles
     int32_t area, pad;
 };
+#define BERT_SRC_BLOCKS 256
 template <bool PER>
-__global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src,
+__global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src, int nparts_all,
                                                           const uint2* stab, uint32_t step, int cur, int hist) {
     __shared__ uint32_t lcount;
-    const BertSource S = src[blockIdx.x];
     const int myseg = P.seg_src0 + blockIdx.x;
     uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
     if (threadIdx.x == 0) lcount = 0;
     double* sbox = box_lds_begin(P);
     __syncthreads();
     int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
-    // every part of a source draws the same count (Ntogenerate,
-    // sampling.cpp:230-245) and takes its slice of the particle indices
-    Pcg32 cr;
-    seed_stream(cr, stream_key(P.seed, step, 2), 0x5ce0000000000000ULL + (uint64_t)S.src);
-    const uint64_t skey = stream_key(P.seed, step, 3);
-    const double fl = floor(S.expected);
-    const int ng = (int)fl + (cr.uniform() <= S.expected - fl ? 1 : 0);
-    const int klo = (int)((int64_t)ng * S.part / S.nparts), khi = (int)((int64_t)ng * (S.part + 1) / S.nparts);
     unsigned long long* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
-    const int nloop = (khi - klo + 63) & ~63;
     unsigned long long done = 0;
-    for (int q = threadIdx.x; q - (int)(threadIdx.x & 63) < nloop; q += blockDim.x) {
-        const int k = klo + q;
-        bool valid = k < khi;
-        Pcg32 rng;
-        Pick0 f0{0u, 0u, make_uint2(0u, 0u)};
-        int mode = 0, bx = S.box, sign = S.sign;
-        double x = 0, y = 0;
-        bool alive = false;
-        uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)S.src << 20) ^ (uint64_t)k ^ 0x15a0ULL);
-        if (valid) {
-            seed_stream(rng, skey, pid);
-            double t = rng.uniform();
-            if (S.area) {
-                // get_random_point of a rectangle: x, then y
-                x = S.p1x + t * S.ndx;
-                y = S.p1y + rng.uniform() * S.ndy;
-            } else {
-                x = S.p1x + t * S.ndx;
-                y = S.p1y + t * S.ndy;
-                const DevBox B = P.boxes[S.box];
-                if (!box_inside(B, x, y)) {
-                    double dx = 0.5 * (B.xmin + B.xmax) - x, dy = 0.5 * (B.ymin + B.ymax) - y;
-                    double dn = sqrt(dx * dx + dy * dy);
-                    x += 1.0e-6 * dx / dn;
-                    y += 1.0e-6 * dy / dn;
+    const int j0 = (int)((int64_t)nparts_all * blockIdx.x / gridDim.x);
+    const int j1 = (int)((int64_t)nparts_all * (blockIdx.x + 1) / gridDim.x);
+    for (int j = j0; j < j1; j++) {
+        const BertSource S = src[j];
+        // every part of a source draws the same count (Ntogenerate,
+        // sampling.cpp:230-245) and takes its slice of the particle indices
+        Pcg32 cr;
+        seed_stream(cr, stream_key(P.seed, step, 2), 0x5ce0000000000000ULL + (uint64_t)S.src);
+        const uint64_t skey = stream_key(P.seed, step, 3);
+        const double fl = floor(S.expected);
+        const int ng = (int)fl + (cr.uniform() <= S.expected - fl ? 1 : 0);
+        const int klo = (int)((int64_t)ng * S.part / S.nparts), khi = (int)((int64_t)ng * (S.part + 1) / S.nparts);
+        const int nloop = (khi - klo + 63) & ~63;
+        for (int q = threadIdx.x; q - (int)(threadIdx.x & 63) < nloop; q += blockDim.x) {
+            const int k = klo + q;
+            bool valid = k < khi;
+            Pcg32 rng;
+            Pick0 f0{0u, 0u, make_uint2(0u, 0u)};
+            int mode = 0, bx = S.box, sign = S.sign;
+            double x = 0, y = 0;
+            bool alive = false;
+            uint64_t pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)S.src << 20) ^ (uint64_t)k ^ 0x15a0ULL);
+            if (valid) {
+                seed_stream(rng, skey, pid);
+                double t = rng.uniform();
+                if (S.area) {
+                    // get_random_point of a rectangle: x, then y
+                    x = S.p1x + t * S.ndx;
+                    y = S.p1y + rng.uniform() * S.ndy;
+                } else {
+                    x = S.p1x + t * S.ndx;
+                    y = S.p1y + t * S.ndy;
+                    const DevBox B = P.boxes[S.box];
+                    if (!box_inside(B, x, y)) {
+                        double dx = 0.5 * (B.xmin + B.xmax) - x, dy = 0.5 * (B.ymin + B.ymax) - y;
+                        double dn = sqrt(dx * dx + dy * dy);
+                        x += 1.0e-6 * dx / dn;
+                        y += 1.0e-6 * dy / dn;
+                    }
                 }
+                bool ngf;
+                mode = (int)alias_pick(stab + S.tab, (uint32_t)P.n, rng, ngf);
+                if (S.area) sign = ngf ? -1 : 1;
+                const double2 v = P.vel[mode];
+                const double2 iv = P.ivel[mode];
+                double rdt = P.dt * rng.uniform();
+                if (nslots_of<false>(P) == 1) f0 = first_pick<false>(P, rng, mode, 0);
+                alive = advect<PER>(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
+                if (nslots_of<false>(P) > 1 && alive) f0 = first_pick<false>(P, rng, mode, bx);
             }
-            bool ngf;
-            mode = (int)alias_pick(stab + S.tab, (uint32_t)P.n, rng, ngf);
-            if (S.area) sign = ngf ? -1 : 1;
-            const double2 v = P.vel[mode];
-            const double2 iv = P.ivel[mode];
-            double rdt = P.dt * rng.uniform();
-            if (nslots_of<false>(P) == 1) f0 = first_pick<false>(P, rng, mode, 0);
-            alive = advect<PER>(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
-            if (nslots_of<false>(P) > 1 && alive) f0 = first_pick<false>(P, rng, mode, bx);
+            scatter_out<false>(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+            done += alive ? 1 : 0;
         }
-        scatter_out<false>(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
-        done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
     box_lds_flush(P, myseg);
@@ -1140,7 +1148,7 @@ struct Bert {
     DevBuf<int32_t> d_pslot;
     const uint2* palias = nullptr;
     int nslots = 1;
-    int nsrc = 0, ninit = 0;
+    int nsrc = 0, src_blocks = 0, ninit = 0;  // source parts, workgroups running them
     bool geo_lds = false;
     bool periodic = false;  // some box is periodic: launch the PER kernels
     bool p1 = false;        // expected to launch the P1 kernels (sizes the step grid)
@@ -1635,15 +1643,28 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         one_table = temps.size() <= 1;
     }
     b->p1 = !cfg->rta && b->p != nullptr && one_table;
-    b->step_blocks = std::min((b->periodic ? 3 : (b->p1 ? 5 : 4)) * c->cu_count, 2048);
-    b->emit_blocks = std::min(2 * c->cu_count, 1024);
-    b->nsegs = b->step_blocks + b->emit_blocks + b->nsrc;
+    // step, emission and source workgroups together stay within
+    // BERT_MAXSEG buffer segments on any device
+    b->step_blocks = std::min((b->periodic ? 3 : (b->p1 ? 5 : 4)) * c->cu_count, BERT_MAXSEG - 512 - BERT_SRC_BLOCKS);
+    b->emit_blocks = std::min(2 * c->cu_count, 512);
+    b->src_blocks = std::min(b->nsrc, BERT_SRC_BLOCKS);
+    b->nsegs = b->step_blocks + b->emit_blocks + b->src_blocks;
     if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
     // generous by default (32 B per particle and buffer): reservoir-driven
     // runs grow to a steady population, and every segment keeps >= 4096 slots
+    // the busiest source workgroup's expected particles per step, twice over,
+    // fit one segment
+    double src_wg_max = 0;
+    for (int w = 0; w < b->src_blocks; w++) {
+        double e = 0;
+        const int j0 = (int)((int64_t)b->nsrc * w / b->src_blocks), j1 = (int)((int64_t)b->nsrc * (w + 1) / b->src_blocks);
+        for (int j = j0; j < j1; j++) e += srcs[j].expected / srcs[j].nparts;
+        src_wg_max = std::max(src_wg_max, e);
+    }
     b->cap = cfg->capacity ? cfg->capacity
                            : std::max<uint64_t>({1ull << 22, 3 * total, src_total > 0 ? (1ull << 25) : 0ull,
-                                                 4096ull * (uint64_t)b->nsegs});
+                                                 4096ull * (uint64_t)b->nsegs,
+                                                 (uint64_t)(2.0 * src_wg_max + 4096.0) * (uint64_t)b->nsegs});
     b->cap = (b->cap + b->nsegs - 1) / b->nsegs * b->nsegs;
     if (b->cap >= 0xffffffffULL) throw Error(BTE_EINVAL, "capacity above 2^32 particles");
     if ((total + b->nsegs - 1) / b->nsegs > b->cap / b->nsegs)
@@ -1867,8 +1888,9 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         }
         if (timed) HIPCHECK(hipEventRecord(b->kev[2 * istep + 1], s));
         if (b->nsrc)
-            hipLaunchKernelGGL(b->periodic ? bert_source_kernel<true> : bert_source_kernel<false>, dim3(b->nsrc),
-                               dim3(256), b->lds_bytes, s, P, b->d_src.p, b->d_src_tab.p, (uint32_t)istep, cur, hist);
+            hipLaunchKernelGGL(b->periodic ? bert_source_kernel<true> : bert_source_kernel<false>, dim3(b->src_blocks),
+                               dim3(256), b->lds_bytes, s, P, b->d_src.p, b->nsrc, b->d_src_tab.p, (uint32_t)istep,
+                               cur, hist);
         hipLaunchKernelGGL(bert_wallcdf_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep);
         {
             auto k = emit_kernel_tab[b->geo_lds][b->periodic][p1];

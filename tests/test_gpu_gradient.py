@@ -89,6 +89,34 @@ def test_gradient_sources_match_the_oracle(pkg, ctx):
     assert np.mean([x["props"][-1, :, 1].mean() for x in g]) < 0
 
 
+def test_six_hundred_gradient_boxes_match_the_oracle(pkg, ctx):
+    """A 600-box ribbon with a linear Teq profile: 600 gradient sources
+    (more parts than source workgroups, which run contiguous ranges of them
+    into a bounded number of buffer segments) against the oracle, every box
+    at two times."""
+    mat, B = pkg.workloads.config1_inputs(nmodes=200, seed=9)
+    P = scipy.linalg.expm(2.0 * B)
+    rect, kind, _, _ = pkg.workloads.ribbon(6000.0, 40.0, 600, 300.0)
+    assert len(rect) == 600
+    cx = 0.5 * (rect[:, 0] + rect[:, 1])
+    teq = 300.0 + 0.002 * cx
+    modes = _modes(pkg, ctx, mat)
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    prop = pkg.Propagator(ctx, P)
+    kw = dict(thickness=1.0, dt=2.0, maxtime=2.0 * 11, Efactor=100000, Tmax=312.0, Tmin=300.0)
+    R = 8
+    g = [pkg.bertamc2d(ctx, modes, geom, prop, seed=70 + r, **kw) for r in range(R)]
+    o = [orc.bert_run(mat, rect, kind, teq, teq, P=P, seed=90 + r, **kw) for r in range(R)]
+    assert g[0]["npart"][-1] > 20000
+    zs = [zstats(np.array([x["props"][[4, -1], :, q] for x in g]), np.array([x["props"][[4, -1], :, q] for x in o]))
+          for q in (0, 1)]
+    for name, f in {"npart": lambda x: float(x["npart"][-1]),
+                    "jx_mean": lambda x: x["props"][-1, :, 1].mean()}.items():
+        zs.append(zstats(np.array([f(x) for x in g]), np.array([f(x) for x in o])))
+    print(assert_3sigma(zs, "600 gradient boxes vs oracle"))
+    assert np.mean([x["props"][-1, :, 1].mean() for x in g]) < 0
+
+
 def _still_material(n=40, seed=8):
     rng = np.random.default_rng(seed)
     return {"omega": rng.uniform(1.0, 10.0, n), "vx": np.zeros(n), "vy": np.zeros(n),
