@@ -107,22 +107,35 @@ class ModeTable(_Handle):
 
 
 class Geometry(_Handle):
-    """Rectangular boxes: rect rows (xmin, xmax, ymin, ymax) [nm];
-    kind 0 box, 1 reservoir, 2 periodic; periodic = {box: (target, tx, ty)}
-    gives each periodic box its translation."""
+    """Rectangular boxes: rect rows (xmin, xmax, ymin, ymax) [nm]; or convex
+    polygons: polys, a list of (n_i, 2) vertex arrays [nm] (rect is then
+    ignored). kind 0 box, 1 reservoir, 2 periodic; periodic = {box:
+    (target, tx, ty)} gives each periodic box its translation."""
     _destroy = "bte_geom_destroy"
 
-    def __init__(self, ctx, rect, kind, Teq, Tinit=None, periodic=None):
+    def __init__(self, ctx, rect, kind, Teq, Tinit=None, periodic=None, polys=None):
         super().__init__()
         self.ctx = ctx
-        self.rect = _f64(rect).reshape(-1, 4)
         self.kind = np.ascontiguousarray(kind, dtype=np.int32)
         self.Teq = _f64(Teq)
         self.Tinit = None if Tinit is None else _f64(Tinit)
         self.nbox = len(self.kind)
-        check(lib().bte_geom_create_rects(ctx.h, self.nbox, _dp(self.rect),
-                                          self.kind.ctypes.data_as(C.POINTER(C.c_int32)), _dp(self.Teq),
-                                          None if self.Tinit is None else _dp(self.Tinit), C.byref(self.h)))
+        tin = None if self.Tinit is None else _dp(self.Tinit)
+        kp = self.kind.ctypes.data_as(C.POINTER(C.c_int32))
+        if polys is not None:
+            if len(polys) != self.nbox:
+                raise ValueError("one vertex array per box")
+            self.polys = [_f64(p).reshape(-1, 2) for p in polys]
+            self.nvert = np.array([len(p) for p in self.polys], np.int32)
+            self.xy = _f64(np.concatenate(self.polys))
+            self.rect = np.array([[p[:, 0].min(), p[:, 0].max(), p[:, 1].min(), p[:, 1].max()] for p in self.polys])
+            check(lib().bte_geom_create_polys(ctx.h, self.nbox, self.nvert.ctypes.data_as(C.POINTER(C.c_int32)),
+                                              _dp(self.xy), kp, _dp(self.Teq), tin, C.byref(self.h)))
+        else:
+            self.polys = None
+            self.rect = _f64(rect).reshape(-1, 4)
+            check(lib().bte_geom_create_rects(ctx.h, self.nbox, _dp(self.rect), kp, _dp(self.Teq), tin,
+                                              C.byref(self.h)))
         for b, (t, tx, ty) in (periodic or {}).items():
             check(lib().bte_geom_set_periodic(self.h, int(b), int(t), float(tx), float(ty)))
 

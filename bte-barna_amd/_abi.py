@@ -101,6 +101,7 @@ SIGNATURES = {
     "bte_modes_set_sigma": (C.c_int, [vp, dp]),
     "bte_modes_destroy": (C.c_int, [vp]),
     "bte_geom_create_rects": (C.c_int, [vp, C.c_int, dp, ip32, dp, dp, pvp]),
+    "bte_geom_create_polys": (C.c_int, [vp, C.c_int, ip32, dp, ip32, dp, dp, pvp]),
     "bte_geom_destroy": (C.c_int, [vp]),
     "bte_geom_set_periodic": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double]),
     "bte_geom_contacts": (C.c_int, [vp, C.c_int, ip32, ip32, dp]),

@@ -334,4 +334,184 @@ __device__ __host__ inline int translate_rect(const DevBox* boxes, const DevCont
     return chosen;
 }
 
+// ---- convex polygon boxes ----
+// Geometries with a box that is not an axis-aligned rectangle run the POLY
+// kernel instantiations. Every box then has a range [first, first + count)
+// of sides (int2 srange per box); each side is a hull edge p1 -> p2 in
+// counter-clockwise order, with its outward unit normal, the box sharing the
+// whole side (SIDE_VOID / SIDE_MIXED as for rectangles) and the id of its
+// normal among the geometry's distinct normals (one diffuse-bounce table per
+// normal). Per-side arrays are indexed by the global side index.
+#define POLY_MAXSIDES 16
+struct DevSide {
+    double p1x, p1y, p2x, p2y, nx, ny;
+    int32_t nbr, nid;
+};
+
+// covered_by of the closed convex polygon (geometry_2d.cpp:155-158)
+__device__ __host__ inline bool poly_inside(const DevSide* S, int2 r, double x, double y) {
+    for (int k = r.x; k < r.x + r.y; k++) {
+        const DevSide s = S[k];
+        if ((x - s.p1x) * s.nx + (y - s.p1y) * s.ny > 0.) return false;
+    }
+    return true;
+}
+
+// get_inter_side (geometry_2d.cpp:160-293) on a convex polygon: the first
+// side with n.v > 0 that the ray r + t v reaches (a side already behind the
+// start counts at t = 0); mask gets bit k for every side k (local index)
+// reached at that time.
+__device__ __host__ inline double poly_exit(const DevSide* S, int2 r, double x, double y, double vx, double vy,
+                                            int& mask, double& ex, double& ey) {
+    double t = 1e300;
+    mask = 0;
+    for (int k = 0; k < r.y; k++) {
+        const DevSide s = S[r.x + k];
+        const double vn = s.nx * vx + s.ny * vy;
+        if (!(vn > 0.)) continue;
+        double tk = ((s.p1x - x) * s.nx + (s.p1y - y) * s.ny) / vn;
+        if (tk < 0.) tk = 0.;
+        if (tk < t) {
+            t = tk;
+            mask = 1 << k;
+        } else if (tk == t) {
+            mask |= 1 << k;
+        }
+    }
+    if (mask == 0) t = 0.;  // no side ahead (v = 0): stay
+    ex = x + t * vx;
+    ey = y + t * vy;
+    if (almost_equal(t, 0.)) t = 0.;
+    return t;
+}
+
+// fast path of the neighbour search (see side_fast): a single side reached
+// away from its end points decides by its precomputed neighbour
+__device__ __host__ inline int poly_side_fast(const DevSide* S, int2 r, int mask, double ex, double ey, double vx,
+                                              double vy, int& side) {
+    if (__builtin_popcount(mask) != 1) return SIDE_MIXED;
+    side = __builtin_ctz(mask);
+    const DevSide s = S[r.x + side];
+    const double dx = s.p2x - s.p1x, dy = s.p2y - s.p1y;
+    const double u = ((ex - s.p1x) * dx + (ey - s.p1y) * dy) / (dx * dx + dy * dy);
+    if (!(u > 1.0e-9 && u < 1.0 - 1.0e-9) || almost_equal(s.nx * vx + s.ny * vy, 0.)) return SIDE_MIXED;
+    return s.nbr;
+}
+
+// find_pboxes on polygons (RTA_MC2D.cpp:1130-1163)
+__device__ __host__ inline int poly_find_pboxes(const DevSide* S, const int2* srange, const DevContact* cts,
+                                                const DevBox& cur, double rx, double ry, double vx, double vy,
+                                                Ids4& out) {
+    const double vn = sqrt(vx * vx + vy * vy);
+    const double ex = rx + 1.0e-6 * vx / vn, ey = ry + 1.0e-6 * vy / vn;
+    int n = 0;
+    for (int c = cur.c0; c < cur.c1; c++) {
+        const DevContact ct = cts[c];
+        const double d = ct.npx * vx + ct.npy * vy;
+        if (!(d > 0.) || almost_equal(d, 0.)) continue;
+        const int2 nr = srange[ct.box];
+        if (poly_inside(S, nr, rx, ry) || poly_inside(S, nr, ex, ey)) {
+            out.set(n, ct.box);
+            n++;
+        }
+    }
+    return n;
+}
+
+// void-side choice among tied sides (RTA_MC2D.cpp:1205-1236) on polygons;
+// returns the local side index
+__device__ __host__ inline int poly_pick_void_side(const DevSide* S, const int2* srange, const DevContact* cts,
+                                                   const DevBox& cur, int2 r, int mask, double rx, double ry,
+                                                   uint32_t rnd) {
+    int c0 = -1, c1 = -1, first = -1;
+    for (int k = 0; k < r.y; k++) {
+        if (!((mask >> k) & 1)) continue;
+        if (first < 0) first = k;
+        const DevSide s = S[r.x + k];
+        const double cx = rx + 1.0e-4 * s.nx, cy = ry + 1.0e-4 * s.ny;
+        bool isvoid = true;
+        for (int c = cur.c0; c < cur.c1; c++)
+            if (poly_inside(S, srange[cts[c].box], cx, cy)) { isvoid = false; break; }
+        if (isvoid) {
+            if (c0 < 0) c0 = k;
+            else if (c1 < 0) c1 = k;
+        }
+    }
+    if (c0 < 0) return first;
+    return (c1 < 0 || (rnd & 1u) == 0) ? c0 : c1;
+}
+
+// geometry_2d::translate (include/geometry_2d.hpp:366-494) on polygons: as
+// translate_rect, with the target box's sides (each extended by 1e-6 of its
+// length) as its borders
+template <class Rng>
+__device__ __host__ inline int translate_poly(const DevBox* boxes, const DevSide* S, const int2* srange,
+                                              const DevContact* cts, const double4* cseg, const double2* ptrans,
+                                              int id, double& x, double& y, double vx, double vy, Rng& rng) {
+    const DevBox b = boxes[id];
+    const double2 T = ptrans[id];
+    const double Tx = 1.0005 * T.x, Ty = 1.0005 * T.y;
+    const int tib = b.tgt;
+    if (tib < 0) return -1;
+    const DevBox tb = boxes[tib];
+    const int2 tr = srange[tib];
+    const double s0x = x, s0y = y, s1x = x + Tx, s1y = y + Ty;
+    double px = 0., py = 0.;
+    bool found = false;
+    for (int k = tr.x; k < tr.x + tr.y && !found; k++) {
+        const DevSide sd = S[k];
+        const double ndx = sd.p2x - sd.p1x, ndy = sd.p2y - sd.p1y;
+        found = seg_cross(s0x, s0y, s1x, s1y, sd.p1x - 1.0e-6 * ndx, sd.p1y - 1.0e-6 * ndy, sd.p2x + 1.0e-6 * ndx,
+                          sd.p2y + 1.0e-6 * ndy, px, py);
+    }
+    if (!found) return -1;
+    const double tn = sqrt(T.x * T.x + T.y * T.y), vn = sqrt(vx * vx + vy * vy);
+    const double pTx = px - 1.0e-6 * T.x / tn, pTy = py - 1.0e-6 * T.y / tn;
+    const double pVx = px + 1.0e-6 * vx / vn, pVy = py + 1.0e-6 * vy / vn;
+    int nb = 0, first = -1, chosen = -1, nkeep = 0;
+    double fx = 0., fy = 0., cx = 0., cy = 0.;
+    uint32_t rsel = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        int seen = 0;
+        for (int c = tb.c0; c < tb.c1; c++) {
+            const DevContact ct = cts[c];
+            const int ibc = ct.box;
+            if (ibc == tib || boxes[ibc].kind != 0) continue;
+            const int2 nr = srange[ibc];
+            const double4 sg = cseg[c];
+            double ix = 0., iy = 0.;
+            const bool here_i = seg_cross(s0x, s0y, s1x, s1y, sg.x, sg.y, sg.z, sg.w, ix, iy);
+            const bool here_T = poly_inside(S, nr, px, py);
+            const bool here_Tv = poly_inside(S, nr, pVx, pVy);
+            const bool here_TT = poly_inside(S, nr, pTx, pTy);
+            if (!(here_i || here_T || here_Tv || here_TT)) continue;
+            double rx, ry;
+            if (here_i) { rx = ix; ry = iy; }
+            else if (here_T) { rx = px; ry = py; }
+            else if (here_Tv) { rx = pVx; ry = pVy; }
+            else { rx = pTx; ry = pTy; }
+            const bool keep = poly_inside(S, nr, rx + 1.0e-6 * vx, ry + 1.0e-6 * vy);
+            if (pass == 0) {
+                if (nb == 0) { first = ibc; fx = rx; fy = ry; }
+                nb++;
+                if (keep) nkeep++;
+            } else if (keep) {
+                if ((uint32_t)seen == rsel) { chosen = ibc; cx = rx; cy = ry; break; }
+                seen++;
+            }
+        }
+        if (pass == 0) {
+            if (nb == 0) return -1;
+            if (nb == 1) { x = fx; y = fy; return first; }
+            if (nkeep == 0) return -1;
+            rsel = rng.bounded((uint32_t)nkeep);
+        }
+    }
+    if (chosen < 0) return -1;
+    x = cx;
+    y = cy;
+    return chosen;
+}
+
+
 }  // namespace bte

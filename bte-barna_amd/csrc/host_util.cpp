@@ -432,25 +432,122 @@ int bte_modes_destroy(bte_modes* mm) {
     BTE_GUARD_END
 }
 
-int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int32_t* kind,
-                          const double* Teq, const double* Tinit, bte_geom** out) {
-    BTE_GUARD_BEGIN
-    auto* c = reinterpret_cast<Ctx*>(ctx);
-    if (!c || !out || nbox <= 0 || !rect || !kind || !Teq) throw Error(BTE_EINVAL, "bad arguments");
-    auto* g = new Geom();
-    g->ctx = c;
-    g->boxes.resize(nbox);
-    for (int i = 0; i < nbox; i++) {
-        RectBox& b = g->boxes[i];
-        b.xmin = rect[4 * i]; b.xmax = rect[4 * i + 1]; b.ymin = rect[4 * i + 2]; b.ymax = rect[4 * i + 3];
-        if (!(b.xmax > b.xmin && b.ymax > b.ymin)) { delete g; throw Error(BTE_EGEOM, "degenerate box"); }
-        if (kind[i] < 0 || kind[i] > 2) { delete g; throw Error(BTE_EINVAL, "bad box kind"); }
-        b.kind = kind[i];
-        b.Teq = Teq[i];
-        b.Tinit = Tinit ? Tinit[i] : Teq[i];
+namespace {
+
+double cross2(double ax, double ay, double bx, double by) { return ax * by - ay * bx; }
+
+// Andrew's monotone chain: hull vertices counter-clockwise, collinear and
+// repeated points dropped
+std::vector<double2> convex_hull(std::vector<double2> p) {
+    std::sort(p.begin(), p.end(), [](const double2& a, const double2& b) { return a.x < b.x || (a.x == b.x && a.y < b.y); });
+    p.erase(std::unique(p.begin(), p.end(), [](const double2& a, const double2& b) { return a.x == b.x && a.y == b.y; }),
+            p.end());
+    if (p.size() < 3) return {};
+    std::vector<double2> h(2 * p.size());
+    size_t k = 0;
+    for (size_t i = 0; i < p.size(); i++) {
+        while (k >= 2 && cross2(h[k - 1].x - h[k - 2].x, h[k - 1].y - h[k - 2].y, p[i].x - h[k - 2].x,
+                                p[i].y - h[k - 2].y) <= 0.)
+            k--;
+        h[k++] = p[i];
     }
-    // calculate_contacts (geometry_2d.cpp:295-381) for rectangles
-    g->contacts.assign(nbox, {});
+    for (size_t i = p.size() - 1, t = k + 1; i-- > 0;) {
+        while (k >= t && cross2(h[k - 1].x - h[k - 2].x, h[k - 1].y - h[k - 2].y, p[i].x - h[k - 2].x,
+                                p[i].y - h[k - 2].y) <= 0.)
+            k--;
+        h[k++] = p[i];
+    }
+    h.resize(k - 1);
+    return h;
+}
+
+// p within tol of the closed segment a-b
+bool near_segment(const double2& p, const double2& a, const double2& b, double tol) {
+    const double dx = b.x - a.x, dy = b.y - a.y, L = std::hypot(dx, dy);
+    if (L == 0.) return std::hypot(p.x - a.x, p.y - a.y) <= tol;
+    if (std::fabs(cross2(dx, dy, p.x - a.x, p.y - a.y)) / L > tol) return false;
+    const double u = ((p.x - a.x) * dx + (p.y - a.y) * dy) / L;
+    return u >= -tol && u <= L + tol;
+}
+
+std::vector<double2> box_vertices(const RectBox& b) {
+    if (!b.hull.empty()) return b.hull;
+    return {make_double2(b.xmin, b.ymin), make_double2(b.xmax, b.ymin), make_double2(b.xmax, b.ymax),
+            make_double2(b.xmin, b.ymax)};
+}
+
+// calculate_contacts (geometry_2d.cpp:295-381) for convex polygons: the
+// boundary points two boxes share (a vertex of one on the other's
+// boundary), a segment between the two farthest of them or a single point;
+// the border normal points away from the owner's centre. Boxes whose
+// interiors overlap (no separating edge normal) are an error.
+bool poly_contact(const RectBox& a, const RectBox& e, double tol, HostContact& hc) {
+    const std::vector<double2> A = box_vertices(a), E = box_vertices(e);
+    // separating-axis test over the edge normals of both boxes: apart (no
+    // contact), touching along some axis, or overlapping interiors
+    bool touch = false;
+    for (int pass = 0; pass < 2 && !touch; pass++) {
+        const std::vector<double2>& P = pass ? E : A;
+        for (size_t k = 0; k < P.size() && !touch; k++) {
+            const double2 p1 = P[k], p2 = P[(k + 1) % P.size()];
+            double nx = p2.y - p1.y, ny = -(p2.x - p1.x);
+            const double nn = std::hypot(nx, ny);
+            nx /= nn;
+            ny /= nn;
+            double amin = 1e300, amax = -1e300, emin = 1e300, emax = -1e300;
+            for (auto& v : A) { const double d = v.x * nx + v.y * ny; amin = std::min(amin, d); amax = std::max(amax, d); }
+            for (auto& v : E) { const double d = v.x * nx + v.y * ny; emin = std::min(emin, d); emax = std::max(emax, d); }
+            if (amax < emin - tol || emax < amin - tol) return false;
+            touch = amax <= emin + tol || emax <= amin + tol;
+        }
+    }
+    if (!touch) throw Error(BTE_EGEOM, "overlapping boxes");
+    std::vector<double2> pts;
+    auto add = [&](const double2& q) {
+        for (auto& r : pts)
+            if (std::hypot(q.x - r.x, q.y - r.y) <= tol) return;
+        pts.push_back(q);
+    };
+    for (auto& v : A)
+        for (size_t k = 0; k < E.size(); k++)
+            if (near_segment(v, E[k], E[(k + 1) % E.size()], tol)) { add(v); break; }
+    for (auto& v : E)
+        for (size_t k = 0; k < A.size(); k++)
+            if (near_segment(v, A[k], A[(k + 1) % A.size()], tol)) { add(v); break; }
+    if (pts.empty()) return false;
+    size_t i0 = 0, i1 = 0;
+    double best = -1.;
+    for (size_t i = 0; i < pts.size(); i++)
+        for (size_t j = i + 1; j < pts.size(); j++) {
+            const double d = std::hypot(pts[i].x - pts[j].x, pts[i].y - pts[j].y);
+            if (d > best) { best = d; i0 = i; i1 = j; }
+        }
+    hc.p1x = pts[i0].x; hc.p1y = pts[i0].y; hc.p2x = pts[i1].x; hc.p2y = pts[i1].y;
+    if (pts.size() == 1 || best < 1.0e-6) {
+        const double px = pts.size() == 1 ? pts[0].x : 0.5 * (pts[i0].x + pts[i1].x);
+        const double py = pts.size() == 1 ? pts[0].y : 0.5 * (pts[i0].y + pts[i1].y);
+        hc.p1x = hc.p2x = px;
+        hc.p1y = hc.p2y = py;
+        const double nx = px - a.cx(), ny = py - a.cy(), nn = std::hypot(nx, ny);
+        hc.npx = nx / nn;
+        hc.npy = ny / nn;
+        return true;
+    }
+    const double ndx = hc.p2x - hc.p1x, ndy = hc.p2y - hc.p1y;
+    const double mx = 0.5 * (hc.p1x + hc.p2x) - a.cx(), my = 0.5 * (hc.p1y + hc.p2y) - a.cy();
+    double nx = ndy, ny = -ndx;
+    if (nx * mx + ny * my < 0.) { nx = -nx; ny = -ny; }
+    const double nn = std::hypot(nx, ny);
+    hc.npx = nx / nn;
+    hc.npy = ny / nn;
+    return true;
+}
+}  // namespace
+
+// contacts of rectangles (geometry_2d.cpp:295-381): overlap of the closed
+// boxes
+static void rect_contacts(Geom* g) {
+    const int nbox = (int)g->boxes.size();
     for (int i = 0; i < nbox; i++) {
         const RectBox& a = g->boxes[i];
         for (int j = 0; j < nbox; j++) {
@@ -465,7 +562,6 @@ int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int3
                 hc.npx = nx / nn;
                 hc.npy = ny / nn;
             } else if (x0 != x1 && y0 != y1) {
-                delete g;
                 throw Error(BTE_EGEOM, "overlapping boxes");
             } else {
                 double ndx = x1 - x0, ndy = y1 - y0;
@@ -478,11 +574,32 @@ int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int3
             }
             g->contacts[i].push_back(hc);
         }
-        if (g->contacts[i].empty() && nbox > 1) {
-            delete g;
-            throw Error(BTE_EGEOM, "There are unconnected boxes");
-        }
     }
+}
+
+// Device tables of a geometry whose boxes and contacts are set: boxes with
+// their contact ranges and side neighbours, contact segments, and the sides
+// (rectangles: left, bottom, top, right; polygons: hull edges) with the ids
+// of their outward normals.
+static void finish_geom(Geom* g) {
+    const int nbox = (int)g->boxes.size();
+    for (int i = 0; i < nbox; i++)
+        if (g->contacts[i].empty() && nbox > 1) throw Error(BTE_EGEOM, "There are unconnected boxes");
+    double scale = 0.;
+    for (auto& b : g->boxes) scale = std::max({scale, std::fabs(b.xmin), std::fabs(b.xmax), std::fabs(b.ymin), std::fabs(b.ymax)});
+    const double tol = 1.0e-9 * std::max(scale, 1.0);
+    auto normal_id = [&](double nx, double ny) {
+        for (size_t k = 0; k < g->normals.size(); k++)
+            if (std::hypot(g->normals[k].x - nx, g->normals[k].y - ny) < 1.0e-9) return (int32_t)k;
+        g->normals.push_back(make_double2(nx, ny));
+        return (int32_t)g->normals.size() - 1;
+    };
+    if (!g->poly)
+        for (int k = 0; k < 4; k++) {
+            double nx, ny;
+            side_normal(k, nx, ny);
+            normal_id(nx, ny);
+        }
     g->dboxes.resize(nbox);
     for (int i = 0; i < nbox; i++) {
         DevBox& d = g->dboxes[i];
@@ -500,33 +617,161 @@ int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int3
         }
         d.c1 = (int32_t)g->dcontacts.size();
         d.tgt = -1;
-        // side classification for the neighbour fast path
-        for (int sd = 0; sd < 4; sd++) {
-            const bool xside = (sd == 0 || sd == 3);
-            const double sv = sd == 0 ? b.xmin : sd == 3 ? b.xmax : sd == 1 ? b.ymin : b.ymax;
-            const double lo = xside ? b.ymin : b.xmin, hi = xside ? b.ymax : b.xmax;
+        // the sides, and for each the box sharing all of it: VOID when no
+        // contact segment of positive length lies on it, MIXED when several
+        // or one not covering it
+        std::vector<DevSide> sd;
+        if (b.hull.empty()) {
+            for (int k = 0; k < 4; k++) {
+                DevSide q{};
+                q.p1x = k == 3 ? b.xmax : b.xmin;
+                q.p1y = k == 2 || k == 3 ? b.ymax : b.ymin;
+                q.p2x = k == 0 ? b.xmin : b.xmax;
+                q.p2y = k == 0 || k == 2 ? b.ymax : b.ymin;
+                side_normal(k, q.nx, q.ny);
+                sd.push_back(q);
+            }
+        } else {
+            const size_t m = b.hull.size();
+            for (size_t k = 0; k < m; k++) {
+                DevSide q{};
+                q.p1x = b.hull[k].x; q.p1y = b.hull[k].y;
+                q.p2x = b.hull[(k + 1) % m].x; q.p2y = b.hull[(k + 1) % m].y;
+                const double L = std::hypot(q.p2x - q.p1x, q.p2y - q.p1y);
+                q.nx = (q.p2y - q.p1y) / L;   // counter-clockwise: outward is to the right
+                q.ny = -(q.p2x - q.p1x) / L;
+                sd.push_back(q);
+            }
+        }
+        for (size_t k = 0; k < sd.size(); k++) {
+            DevSide& q = sd[k];
+            const double2 a = make_double2(q.p1x, q.p1y), c = make_double2(q.p2x, q.p2y);
             int full = -1, touching = 0;
             for (auto& hc : g->contacts[i]) {
-                // a positive-length shared segment lying on this side
-                bool on = xside ? (hc.p1x == sv && hc.p2x == sv && hc.p1y != hc.p2y)
-                                : (hc.p1y == sv && hc.p2y == sv && hc.p1x != hc.p2x);
-                if (!on) continue;
+                if (hc.length() == 0.) continue;
+                const double2 h1 = make_double2(hc.p1x, hc.p1y), h2 = make_double2(hc.p2x, hc.p2y);
+                if (!near_segment(h1, a, c, tol) || !near_segment(h2, a, c, tol)) continue;
                 touching++;
-                double a = xside ? std::min(hc.p1y, hc.p2y) : std::min(hc.p1x, hc.p2x);
-                double c = xside ? std::max(hc.p1y, hc.p2y) : std::max(hc.p1x, hc.p2x);
-                if (a == lo && c == hi) full = hc.box;
+                const bool same = (std::hypot(h1.x - a.x, h1.y - a.y) <= tol && std::hypot(h2.x - c.x, h2.y - c.y) <= tol) ||
+                                  (std::hypot(h1.x - c.x, h1.y - c.y) <= tol && std::hypot(h2.x - a.x, h2.y - a.y) <= tol);
+                if (same) full = hc.box;
             }
-            d.side_nbr[sd] = touching == 0 ? SIDE_VOID : (touching == 1 && full >= 0 ? full : SIDE_MIXED);
+            q.nbr = touching == 0 ? SIDE_VOID : (touching == 1 && full >= 0 ? full : SIDE_MIXED);
+            q.nid = normal_id(q.nx, q.ny);
+            if (k < 4) d.side_nbr[k] = b.hull.empty() ? q.nbr : SIDE_MIXED;
         }
+        for (size_t k = sd.size(); k < 4; k++) d.side_nbr[k] = SIDE_MIXED;
+        g->srange.push_back(make_int2((int)g->sides.size(), (int)sd.size()));
+        g->sides.insert(g->sides.end(), sd.begin(), sd.end());
     }
     g->ptrans.assign(nbox, make_double2(0., 0.));
+    Ctx* c = g->ctx;
     HIPCHECK(hipSetDevice(c->device));
     g->d_boxes.upload(g->dboxes, c->stream);
     g->d_contacts.upload(g->dcontacts.empty() ? std::vector<DevContact>(1) : g->dcontacts, c->stream);
     g->d_cseg.upload(g->cseg.empty() ? std::vector<double4>(1) : g->cseg, c->stream);
     g->d_ptrans.upload(g->ptrans, c->stream);
+    g->d_sides.upload(g->sides, c->stream);
+    g->d_srange.upload(g->srange, c->stream);
     HIPCHECK(hipStreamSynchronize(c->stream));
-    *out = reinterpret_cast<bte_geom*>(g);
+}
+
+static void check_kind(int k) {
+    if (k < 0 || k > 2) throw Error(BTE_EINVAL, "bad box kind");
+}
+
+int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int32_t* kind,
+                          const double* Teq, const double* Tinit, bte_geom** out) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !out || nbox <= 0 || !rect || !kind || !Teq) throw Error(BTE_EINVAL, "bad arguments");
+    std::unique_ptr<Geom> g(new Geom());
+    g->ctx = c;
+    g->boxes.resize(nbox);
+    for (int i = 0; i < nbox; i++) {
+        RectBox& b = g->boxes[i];
+        b.xmin = rect[4 * i]; b.xmax = rect[4 * i + 1]; b.ymin = rect[4 * i + 2]; b.ymax = rect[4 * i + 3];
+        if (!(b.xmax > b.xmin && b.ymax > b.ymin)) throw Error(BTE_EGEOM, "degenerate box");
+        check_kind(kind[i]);
+        b.kind = kind[i];
+        b.Teq = Teq[i];
+        b.Tinit = Tinit ? Tinit[i] : Teq[i];
+    }
+    g->contacts.assign(nbox, {});
+    rect_contacts(g.get());
+    finish_geom(g.get());
+    *out = reinterpret_cast<bte_geom*>(g.release());
+    BTE_GUARD_END
+}
+
+int bte_geom_create_polys(bte_ctx* ctx, int nbox, const int32_t* nvert, const double* xy, const int32_t* kind,
+                          const double* Teq, const double* Tinit, bte_geom** out) {
+    BTE_GUARD_BEGIN
+    auto* c = reinterpret_cast<Ctx*>(ctx);
+    if (!c || !out || nbox <= 0 || !nvert || !xy || !kind || !Teq) throw Error(BTE_EINVAL, "bad arguments");
+    std::unique_ptr<Geom> g(new Geom());
+    g->ctx = c;
+    g->poly = true;
+    g->boxes.resize(nbox);
+    size_t off = 0;
+    for (int i = 0; i < nbox; i++) {
+        RectBox& b = g->boxes[i];
+        if (nvert[i] < 3) throw Error(BTE_EGEOM, "box " + std::to_string(i) + ": fewer than 3 vertices");
+        std::vector<double2> v(nvert[i]);
+        for (int k = 0; k < nvert[i]; k++) {
+            v[k] = make_double2(xy[2 * (off + k)], xy[2 * (off + k) + 1]);
+            if (!std::isfinite(v[k].x) || !std::isfinite(v[k].y)) throw Error(BTE_EINVAL, "vertices must be finite");
+        }
+        off += nvert[i];
+        b.hull = convex_hull(v);
+        if (b.hull.size() < 3) throw Error(BTE_EGEOM, "box " + std::to_string(i) + ": degenerate polygon");
+        if (b.hull.size() > POLY_MAXSIDES)
+            throw Error(BTE_EGEOM, "box " + std::to_string(i) + ": more than " + std::to_string(POLY_MAXSIDES) + " sides");
+        // a vertex on a side between two hull corners makes the reference
+        // keep only part of that side as a border (get_sides keeps the
+        // first vertex pair of each normal): refused
+        for (auto& p : v) {
+            bool corner = false;
+            for (auto& h : b.hull) corner = corner || (p.x == h.x && p.y == h.y);
+            if (corner) continue;
+            for (size_t k = 0; k < b.hull.size(); k++)
+                if (near_segment(p, b.hull[k], b.hull[(k + 1) % b.hull.size()], 0.))
+                    throw Error(BTE_EGEOM, "box " + std::to_string(i) + ": a vertex lies on a side");
+        }
+        double A = 0., cx = 0., cy = 0.;
+        b.xmin = b.ymin = 1e300;
+        b.xmax = b.ymax = -1e300;
+        for (size_t k = 0; k < b.hull.size(); k++) {
+            const double2 p = b.hull[k], q = b.hull[(k + 1) % b.hull.size()];
+            const double w = cross2(p.x, p.y, q.x, q.y);
+            A += w;
+            cx += (p.x + q.x) * w;
+            cy += (p.y + q.y) * w;
+            b.xmin = std::min(b.xmin, p.x); b.xmax = std::max(b.xmax, p.x);
+            b.ymin = std::min(b.ymin, p.y); b.ymax = std::max(b.ymax, p.y);
+        }
+        b.parea = 0.5 * A;
+        b.pcx = cx / (3. * A);
+        b.pcy = cy / (3. * A);
+        check_kind(kind[i]);
+        b.kind = kind[i];
+        b.Teq = Teq[i];
+        b.Tinit = Tinit ? Tinit[i] : Teq[i];
+    }
+    double scale = 0.;
+    for (auto& b : g->boxes) scale = std::max({scale, std::fabs(b.xmin), std::fabs(b.xmax), std::fabs(b.ymin), std::fabs(b.ymax)});
+    const double tol = 1.0e-9 * std::max(scale, 1.0);
+    g->contacts.assign(nbox, {});
+    for (int i = 0; i < nbox; i++)
+        for (int j = 0; j < nbox; j++) {
+            if (j == i) continue;
+            const RectBox &a = g->boxes[i], &e = g->boxes[j];
+            if (a.xmax < e.xmin - tol || e.xmax < a.xmin - tol || a.ymax < e.ymin - tol || e.ymax < a.ymin - tol) continue;
+            HostContact hc{j, 0, 0, 0, 0, 0, 0};
+            if (poly_contact(a, e, tol, hc)) g->contacts[i].push_back(hc);
+        }
+    finish_geom(g.get());
+    *out = reinterpret_cast<bte_geom*>(g.release());
     BTE_GUARD_END
 }
 

@@ -93,13 +93,17 @@ struct Modes {
     DevBuf<double> d_vx, d_vy, d_w, d_omega;
 };
 
+// A box: an axis-aligned rectangle, or a convex polygon (hull not empty:
+// its vertices counter-clockwise, xmin..ymax its bounding box)
 struct RectBox {
     double xmin, xmax, ymin, ymax;
     int kind;
     double Teq, Tinit;
-    double cx() const { return 0.5 * (xmin + xmax); }
-    double cy() const { return 0.5 * (ymin + ymax); }
-    double area() const { return (xmax - xmin) * (ymax - ymin); }
+    std::vector<double2> hull;
+    double parea = 0., pcx = 0., pcy = 0.;  // polygon area and centroid
+    double cx() const { return hull.empty() ? 0.5 * (xmin + xmax) : pcx; }
+    double cy() const { return hull.empty() ? 0.5 * (ymin + ymax) : pcy; }
+    double area() const { return hull.empty() ? (xmax - xmin) * (ymax - ymin) : parea; }
 };
 struct HostContact {
     int box;
@@ -115,10 +119,18 @@ struct Geom {
     std::vector<DevContact> dcontacts;
     std::vector<double4> cseg;     // per contact: shared segment extended by 1e-6
     std::vector<double2> ptrans;   // per box: periodic translation (0 otherwise)
+    // sides of every box (rectangles: left, bottom, top, right at 4 i ..
+    // 4 i + 3), their ranges per box and the distinct outward normals
+    bool poly = false;             // some box is not an axis-aligned rectangle
+    std::vector<DevSide> sides;
+    std::vector<int2> srange;
+    std::vector<double2> normals;
     DevBuf<DevBox> d_boxes;
     DevBuf<DevContact> d_contacts;
     DevBuf<double4> d_cseg;
     DevBuf<double2> d_ptrans;
+    DevBuf<DevSide> d_sides;
+    DevBuf<int2> d_srange;
 };
 
 // Propagators of one material at ascending temperatures; a single entry with

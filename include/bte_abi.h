@@ -83,6 +83,17 @@ int bte_modes_destroy(bte_modes* m);
  * Tinit may be NULL (Tinit = Teq). ---- */
 int bte_geom_create_rects(bte_ctx* ctx, int nbox, const double* rect, const int32_t* kind,
                           const double* Teq, const double* Tinit, bte_geom** out);
+/* Convex polygon boxes (read_geometry_XML with <Vertices npoints=n>,
+ * geometry_2d.cpp:127-153 and 408-522): box i has nvert[i] vertices, its
+ * (x, y) pairs consecutive in xy. Each box is the convex hull of its
+ * vertices (at most 16 sides); a vertex on a side between two hull corners
+ * is refused (the reference keeps only part of such a side as a border).
+ * Contacts are the boundary segments or points two boxes share
+ * (geometry_2d.cpp:295-381); overlapping boxes are an error. Runs on such a
+ * geometry use the polygon kernels (diffuse sides; specular sides and
+ * dT/dp need rectangles). */
+int bte_geom_create_polys(bte_ctx* ctx, int nbox, const int32_t* nvert, const double* xy, const int32_t* kind,
+                          const double* Teq, const double* Tinit, bte_geom** out);
 int bte_geom_destroy(bte_geom* g);
 /* Periodic box (kind 2, <Translate_to id> tx ty </Translate_to>,
  * geometry_2d.cpp:465-473): a particle entering it is moved by 1.0005 (tx, ty)

@@ -410,8 +410,7 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
         GridDist d = ref_distribution(m, fd, S.ngamma);
         size_t ngen = n_to_generate((s.area * S.thickness / vuc) * d.cumulsum / (Eeff * m.nq), hostrng);
         for (size_t i = 0; i < ngen; i++) {
-            double x = s.xmin + hostrng.uniform() * (s.xmax - s.xmin);
-            double y = s.ymin + hostrng.uniform() * (s.ymax - s.ymin);
+            const auto [x, y] = s.random_point(hostrng);
             size_t pos = d.sample(hostrng);
             parts.push_back({x, y, (int)pos, s.id, d.signs[pos], false});
         }
@@ -423,8 +422,7 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
         GridDist d = out_tref(m, s.Treal, s.Teq);
         size_t ngen = n_to_generate((s.area * S.thickness / vuc) * d.cumulsum / (Eeff * m.nq), hostrng);
         for (size_t i = 0; i < ngen; i++) {
-            double x = s.xmin + hostrng.uniform() * (s.xmax - s.xmin);
-            double y = s.ymin + hostrng.uniform() * (s.ymax - s.ymin);
+            const auto [x, y] = s.random_point(hostrng);
             size_t pos = d.sample(hostrng);
             parts.push_back({x, y, (int)pos, s.id, d.signs[pos], false});
         }
@@ -509,8 +507,7 @@ int bert_run(BertSetup& S, uint64_t seed, int nthreads, int max_steps,
                 const GridDist& g = ggen[k];
                 size_t ng = n_to_generate(S.dt * volsf[s.id] * g.cumulsum / (Eeff * m.nq), hostrng);
                 for (size_t i = 0; i < ng; i++) {
-                    double x = s.xmin + hostrng.uniform() * (s.xmax - s.xmin);
-                    double y = s.ymin + hostrng.uniform() * (s.ymax - s.ymin);
+                    const auto [x, y] = s.random_point(hostrng);
                     size_t pos = g.sample(hostrng);
                     np_.push_back({x, y, (int)pos, s.id, g.signs[pos], false});
                 }

@@ -237,13 +237,29 @@ void orc_propH_get_sc(int n, const double* sc, int nr, const double* R, int32_t*
 // set by orc_set_periodic and consumed by mk_sys
 static thread_local std::vector<int32_t> g_ptarget;
 static thread_local std::vector<double> g_ptrans;
+// convex polygon boxes for the next run (vertex counts and x, y pairs), set
+// by orc_set_polygons and consumed by mk_sys in place of the rectangles
+static thread_local std::vector<int32_t> g_nvert;
+static thread_local std::vector<double> g_xy;
 
 static std::vector<Box> mk_sys(int nbox, const double* rect, const int32_t* kind,
                                const double* Teq, const double* Tinit) {
     std::vector<Box> sys(nbox);
+    if (!g_nvert.empty() && (int)g_nvert.size() != nbox) throw geometry_error("polygon count differs from boxes");
+    size_t voff = 0;
     for (int i = 0; i < nbox; i++) {
         sys[i].id = i;
-        sys[i].make(rect[4 * i + 0], rect[4 * i + 1], rect[4 * i + 2], rect[4 * i + 3]);
+        if (g_nvert.empty()) {
+            sys[i].make(rect[4 * i + 0], rect[4 * i + 1], rect[4 * i + 2], rect[4 * i + 3]);
+        } else {
+            std::vector<double> vx(g_nvert[i]), vy(g_nvert[i]);
+            for (int k = 0; k < g_nvert[i]; k++) {
+                vx[k] = g_xy[2 * (voff + k)];
+                vy[k] = g_xy[2 * (voff + k) + 1];
+            }
+            voff += g_nvert[i];
+            sys[i].make_poly(vx, vy);
+        }
         sys[i].reservoir = kind[i] == 1;
         sys[i].periodic = kind[i] == 2;
         sys[i].Teq = Teq[i];
@@ -257,8 +273,16 @@ static std::vector<Box> mk_sys(int nbox, const double* rect, const int32_t* kind
     }
     g_ptarget.clear();
     g_ptrans.clear();
+    g_nvert.clear();
+    g_xy.clear();
     calculate_contacts(sys);
     return sys;
+}
+void orc_set_polygons(int nbox, const int32_t* nvert, const double* xy) {
+    g_nvert.assign(nvert, nvert + nbox);
+    size_t nv = 0;
+    for (int i = 0; i < nbox; i++) nv += (size_t)nvert[i];
+    g_xy.assign(xy, xy + 2 * nv);
 }
 void orc_set_periodic(int nbox, const int32_t* target, const double* trans) {
     g_ptarget.assign(target, target + nbox);
@@ -300,6 +324,69 @@ int orc_inter_side(int nbox, const double* rect, int ibox, double x0, double y0,
     }
 }
 // contacts: for each box, list (neighbour, npx, npy, p1x, p1y, p2x, p2y)
+int orc_box_info(int nbox, const double* rect, double* out) {
+    std::vector<int32_t> kind(nbox, 0);
+    std::vector<double> T(nbox, 300.);
+    try {
+        auto sys = mk_sys(nbox, rect, kind.data(), T.data(), nullptr);
+        for (int i = 0; i < nbox; i++) {
+            double* o = out + 8 * (size_t)i;
+            o[0] = sys[i].area; o[1] = sys[i].cx; o[2] = sys[i].cy;
+            o[3] = sys[i].xmin; o[4] = sys[i].xmax; o[5] = sys[i].ymin; o[6] = sys[i].ymax;
+            o[7] = (double)sys[i].borders.size();
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+int orc_borders(int nbox, const double* rect, int ibox, int maxb, double* out) {
+    std::vector<int32_t> kind(nbox, 0);
+    std::vector<double> T(nbox, 300.);
+    try {
+        auto sys = mk_sys(nbox, rect, kind.data(), T.data(), nullptr);
+        int k = 0;
+        for (auto& b : sys[ibox].borders) {
+            if (k >= maxb) break;
+            double* o = out + 6 * (size_t)k++;
+            o[0] = b.p1x; o[1] = b.p1y; o[2] = b.p2x; o[3] = b.p2y; o[4] = b.npx; o[5] = b.npy;
+        }
+        return k;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+int orc_inside(int nbox, const double* rect, int ibox, int npts, const double* xy, int32_t* out) {
+    std::vector<int32_t> kind(nbox, 0);
+    std::vector<double> T(nbox, 300.);
+    try {
+        auto sys = mk_sys(nbox, rect, kind.data(), T.data(), nullptr);
+        for (int k = 0; k < npts; k++) out[k] = sys[ibox].inside(xy[2 * k], xy[2 * k + 1]) ? 1 : 0;
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+int orc_random_points(int nbox, const double* rect, int ibox, int n, uint64_t seed, double* xy) {
+    std::vector<int32_t> kind(nbox, 0);
+    std::vector<double> T(nbox, 300.);
+    try {
+        auto sys = mk_sys(nbox, rect, kind.data(), T.data(), nullptr);
+        pcg32 rng(seed, 7u);
+        for (int k = 0; k < n; k++) std::tie(xy[2 * k], xy[2 * k + 1]) = sys[ibox].random_point(rng);
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
 int orc_contacts(int nbox, const double* rect, int maxc, int32_t* ncont, int32_t* nbr, double* geo) {
     std::vector<int32_t> kind(nbox, 0);
     std::vector<double> T(nbox, 300.);

@@ -1,5 +1,6 @@
 // Oracle (test infrastructure only): restatement of alma::geometry_2d for
-// rectangular boxes with the reference's semantics.
+// rectangular and convex polygon boxes with the reference's semantics.
+//   hull, centre:   geometry_2d.cpp:127-153 (convex hull, area centroid)
 //   borders:        geometry_2d.cpp:55-125 (get_sides; sbl = side extended by
 //                   1e-6*nd, outward normal np)
 //   inside:         geometry_2d.cpp:155-158 (closed region)
@@ -56,7 +57,8 @@ inline bool point_on_segment(double px, double py, double ax, double ay,
 
 struct Box {
     int id = 0;
-    double xmin, xmax, ymin, ymax, cx, cy, area;
+    double xmin, xmax, ymin, ymax, cx, cy, area;  // bounding box for polygons
+    std::vector<std::pair<double, double>> hull;  // convex polygons: hull, counter-clockwise
     bool reservoir = false, periodic = false;
     int target = -1;             // periodic: box2translate
     double tx = 0., ty = 0.;     // periodic: translation vector
@@ -89,8 +91,94 @@ struct Box {
             borders.push_back(b);
         }
     }
+    // geometry_2d(vertices) for a convex polygon (geometry_2d.cpp:127-153):
+    // hull, area centroid, area; borders by get_sides' loop over the vertex
+    // pairs (i, j >= i) of the given order: a pair is a border when its
+    // midpoint moved 1e-4 away from the centre leaves the hull, the normal is
+    // the perpendicular pointing away from the centre, a normal already
+    // present is skipped (geometry_2d.cpp:55-125)
+    void make_poly(const std::vector<double>& vx, const std::vector<double>& vy) {
+        std::vector<std::pair<double, double>> p;
+        for (size_t i = 0; i < vx.size(); i++) p.emplace_back(vx[i], vy[i]);
+        std::sort(p.begin(), p.end());
+        p.erase(std::unique(p.begin(), p.end()), p.end());
+        hull.clear();
+        // monotone chain, lower then upper hull, collinear points dropped
+        for (int pass = 0; pass < 2; pass++) {
+            const size_t start = hull.size();
+            for (size_t q = 0; q < p.size(); q++) {
+                const auto& pt = pass ? p[p.size() - 1 - q] : p[q];
+                while (hull.size() >= start + 2) {
+                    const auto& a = hull[hull.size() - 2];
+                    const auto& b = hull[hull.size() - 1];
+                    if (cross2(b.first - a.first, b.second - a.second, pt.first - a.first, pt.second - a.second) > 0.)
+                        break;
+                    hull.pop_back();
+                }
+                hull.push_back(pt);
+            }
+            hull.pop_back();
+        }
+        double A = 0., sx = 0., sy = 0.;
+        xmin = ymin = 1e300;
+        xmax = ymax = -1e300;
+        for (size_t k = 0; k < hull.size(); k++) {
+            const auto& a = hull[k];
+            const auto& b = hull[(k + 1) % hull.size()];
+            const double w = cross2(a.first, a.second, b.first, b.second);
+            A += w;
+            sx += (a.first + b.first) * w;
+            sy += (a.second + b.second) * w;
+            xmin = std::min(xmin, a.first); xmax = std::max(xmax, a.first);
+            ymin = std::min(ymin, a.second); ymax = std::max(ymax, a.second);
+        }
+        area = 0.5 * A;
+        cx = sx / (3. * A);
+        cy = sy / (3. * A);
+        borders.clear();
+        for (size_t i = 0; i < vx.size(); i++)
+            for (size_t j = i; j < vx.size(); j++) {
+                if (std::hypot(vx[j] - vx[i], vy[j] - vy[i]) < 1.0e-6) continue;
+                const double scx = 0.5 * (vx[i] + vx[j]), scy = 0.5 * (vy[i] + vy[j]);
+                double ncx = scx - cx, ncy = scy - cy;
+                const double nn = std::hypot(ncx, ncy);
+                if (nn == 0.) continue;
+                ncx *= 1.0e-4 / nn;
+                ncy *= 1.0e-4 / nn;
+                if (inside(ncx + scx, ncy + scy)) continue;
+                Border b{};
+                b.p1x = vx[i]; b.p1y = vy[i]; b.p2x = vx[j]; b.p2y = vy[j];
+                b.ndx = b.p2x - b.p1x; b.ndy = b.p2y - b.p1y;
+                double nx = b.ndy, ny = -b.ndx;
+                if (nx * ncx + ny * ncy < 0.) { nx = -nx; ny = -ny; }
+                const double L = std::hypot(nx, ny);
+                b.npx = nx / L; b.npy = ny / L;
+                b.ax = b.p1x - 1.0e-6 * b.ndx; b.ay = b.p1y - 1.0e-6 * b.ndy;
+                b.bx = b.p2x + 1.0e-6 * b.ndx; b.by = b.p2y + 1.0e-6 * b.ndy;
+                bool dup = false;
+                for (auto& o : borders) dup = dup || std::hypot(o.npx - b.npx, o.npy - b.npy) < 1.0e-6;
+                if (!dup) borders.push_back(b);
+                if (borders.size() > vx.size()) throw geometry_error("To much borders");
+            }
+    }
     bool inside(double x, double y) const {
-        return x >= xmin && x <= xmax && y >= ymin && y <= ymax;
+        if (hull.empty()) return x >= xmin && x <= xmax && y >= ymin && y <= ymax;
+        for (size_t k = 0; k < hull.size(); k++) {
+            const auto& a = hull[k];
+            const auto& b = hull[(k + 1) % hull.size()];
+            if (cross2(b.first - a.first, b.second - a.second, x - a.first, y - a.second) < 0.) return false;
+        }
+        return true;
+    }
+    // get_random_point (geometry_2d.hpp:332-348): uniform in the bounding box
+    // (x, then y) until covered by the box
+    template <class Rng>
+    std::pair<double, double> random_point(Rng& rng) const {
+        while (true) {
+            const double x = xmin + rng.uniform() * (xmax - xmin);
+            const double y = ymin + rng.uniform() * (ymax - ymin);
+            if (inside(x, y)) return {x, y};
+        }
     }
 
     struct Hit { double time; double rx, ry; std::vector<int> bids; };
@@ -148,11 +236,85 @@ struct Box {
 // calculate_contacts for rectangles (geometry_2d.cpp:295-381): a shared edge
 // of positive length gives a border whose normal points away from this box;
 // a single shared point gives a point contact with np = p - centre.
+// Shared boundary of two convex polygons with disjoint interiors: the end
+// points of the overlaps of collinear side pairs and the meeting points of
+// the others, as a segment between the two farthest points or one point;
+// two sides crossing inside both is an overlap (an error).
+inline bool poly_shared(const Box& a, const Box& e, double tol, double& x0, double& y0, double& x1, double& y1) {
+    std::vector<std::pair<double, double>> pts;
+    auto add = [&](double x, double y) {
+        for (auto& q : pts)
+            if (std::hypot(q.first - x, q.second - y) <= tol) return;
+        pts.emplace_back(x, y);
+    };
+    for (auto& ba : a.borders)
+        for (auto& be : e.borders) {
+            const double dx = ba.ndx, dy = ba.ndy, L = std::hypot(dx, dy);
+            const double c1 = cross2(dx, dy, be.p1x - ba.p1x, be.p1y - ba.p1y) / L;
+            const double c2 = cross2(dx, dy, be.p2x - ba.p1x, be.p2y - ba.p1y) / L;
+            if (std::fabs(c1) <= tol && std::fabs(c2) <= tol) {
+                // collinear: overlap of the parameter intervals along ba
+                double u1 = ((be.p1x - ba.p1x) * dx + (be.p1y - ba.p1y) * dy) / (L * L);
+                double u2 = ((be.p2x - ba.p1x) * dx + (be.p2y - ba.p1y) * dy) / (L * L);
+                const double lo = std::max(0., std::min(u1, u2)), hi = std::min(1., std::max(u1, u2));
+                if (hi < lo - tol / L) continue;
+                add(ba.p1x + lo * dx, ba.p1y + lo * dy);
+                add(ba.p1x + hi * dx, ba.p1y + hi * dy);
+                continue;
+            }
+            double px, py;
+            if (!seg_intersect(ba.p1x, ba.p1y, ba.p2x, ba.p2y, be.p1x, be.p1y, be.p2x, be.p2y, px, py)) continue;
+            const bool end_a = std::hypot(px - ba.p1x, py - ba.p1y) <= tol || std::hypot(px - ba.p2x, py - ba.p2y) <= tol;
+            const bool end_e = std::hypot(px - be.p1x, py - be.p1y) <= tol || std::hypot(px - be.p2x, py - be.p2y) <= tol;
+            if (!end_a && !end_e) throw geometry_error("overlapping boxes");
+            add(px, py);
+        }
+    if (pts.empty()) return false;
+    size_t i0 = 0, i1 = 0;
+    double best = -1.;
+    for (size_t i = 0; i < pts.size(); i++)
+        for (size_t j = i; j < pts.size(); j++) {
+            const double d = std::hypot(pts[i].first - pts[j].first, pts[i].second - pts[j].second);
+            if (d > best) { best = d; i0 = i; i1 = j; }
+        }
+    x0 = pts[i0].first; y0 = pts[i0].second; x1 = pts[i1].first; y1 = pts[i1].second;
+    return true;
+}
+
 inline void calculate_contacts(std::vector<Box>& sys) {
+    double scale = 1.;
+    for (auto& b : sys) scale = std::max({scale, std::fabs(b.xmin), std::fabs(b.xmax), std::fabs(b.ymin), std::fabs(b.ymax)});
+    const double tol = 1.0e-9 * scale;
     for (auto& a : sys) {
         a.contacts.clear();
         for (auto& e : sys) {
             if (e.id == a.id) continue;
+            if (!a.hull.empty() || !e.hull.empty()) {
+                double x0, y0, x1, y1;
+                if (!poly_shared(a, e, tol, x0, y0, x1, y1)) continue;
+                Border cb{};
+                if (std::hypot(x1 - x0, y1 - y0) < 1.0e-6) {
+                    const double px = 0.5 * (x0 + x1), py = 0.5 * (y0 + y1);
+                    cb.p1x = cb.p2x = cb.ax = cb.bx = px;
+                    cb.p1y = cb.p2y = cb.ay = cb.by = py;
+                    const double nx = px - a.cx, ny = py - a.cy, nn = std::hypot(nx, ny);
+                    cb.npx = nx / nn; cb.npy = ny / nn;
+                    a.contacts[e.id] = cb;
+                    continue;
+                }
+                cb.p1x = x0; cb.p1y = y0; cb.p2x = x1; cb.p2y = y1;
+                cb.ndx = x1 - x0; cb.ndy = y1 - y0;
+                const double mx = 0.5 * (x0 + x1) - a.cx, my = 0.5 * (y0 + y1) - a.cy;
+                double nx = cb.ndy, ny = -cb.ndx;
+                if (nx * mx + ny * my < 0.) { nx = -nx; ny = -ny; }
+                const double nn = std::hypot(nx, ny);
+                cb.npx = nx / nn; cb.npy = ny / nn;
+                const double L = std::hypot(cb.ndx, cb.ndy);
+                cb.ax = x0 - 1.0e-6 * cb.ndx / L; cb.ay = y0 - 1.0e-6 * cb.ndy / L;
+                cb.bx = x1 + 1.0e-6 * cb.ndx / L; cb.by = y1 + 1.0e-6 * cb.ndy / L;
+                a.contacts[e.id] = cb;
+                continue;
+            }
             double x0 = std::max(a.xmin, e.xmin), x1 = std::min(a.xmax, e.xmax);
             double y0 = std::max(a.ymin, e.ymin), y1 = std::min(a.ymax, e.ymax);
             if (x0 > x1 || y0 > y1) continue;  // disjoint

@@ -232,8 +232,7 @@ int rta_steady_run(SteadySetup& SS, uint64_t nparticles, uint64_t seed, int nthr
             size_t k = std::lower_bound(SS.gcum.begin(), SS.gcum.end(), u) - SS.gcum.begin();
             if (k >= SS.gbox.size()) k = SS.gbox.size() - 1;
             const Box& b = sys[SS.gbox[k]];
-            q.x = b.xmin + rng.uniform() * (b.xmax - b.xmin);
-            q.y = b.ymin + rng.uniform() * (b.ymax - b.ymin);
+            std::tie(q.x, q.y) = b.random_point(rng);
             q.box = b.id;
             const NablaT& g = SS.ggen[k];
             int sg = (i & 1) ? 1 : -1;
@@ -301,8 +300,7 @@ int rta_steady_run(SteadySetup& SS, uint64_t nparticles, uint64_t seed, int nthr
             pcg32 rng(gen_key(seed ^ 0x5eedULL, gen), (uint64_t)b);
             for (long k = 0; k < cnt; k++) {
                 SPart q{};
-                q.x = sys[b].xmin + rng.uniform() * (sys[b].xmax - sys[b].xmin);
-                q.y = sys[b].ymin + rng.uniform() * (sys[b].ymax - sys[b].ymin);
+                std::tie(q.x, q.y) = sys[b].random_point(rng);
                 q.mode = (int)S.sampler.sample(rng);
                 q.box = b;
                 q.sign = sg;

@@ -139,9 +139,10 @@ def prop_interp(Ps, Ts, Tx, rule=0):
     return sc
 
 
-def gradient_T(rect, kind, Teq):
+def gradient_T(rect, kind, Teq, polys=None):
     """calculate_gradientT: (nbox, 2) gradients [K/nm]"""
-    rect, Teq = _f(rect), _f(Teq)
+    rect, Teq = _f(poly_rect(polys) if polys is not None else rect), _f(Teq)
+    set_polygons(polys)
     kind = np.ascontiguousarray(kind, np.int32)
     g = np.zeros((len(kind), 2))
     rc = lib().orc_gradient_T(len(kind), _dp(rect), kind.ctypes.data_as(ip32), _dp(Teq), g.ctypes.data_as(dp))
@@ -150,8 +151,9 @@ def gradient_T(rect, kind, Teq):
     return g
 
 
-def inter_side(rect, ibox, x0, y0, vx, vy, dt):
-    rect = _f(rect)
+def inter_side(rect, ibox, x0, y0, vx, vy, dt, polys=None):
+    rect = _f(poly_rect(polys) if polys is not None else rect)
+    set_polygons(polys)
     out = np.zeros(3)
     bids = np.zeros(4, np.int32)
     nb = C.c_int32()
@@ -163,8 +165,9 @@ def inter_side(rect, ibox, x0, y0, vx, vy, dt):
     return out[0], out[1], out[2], list(bids[:nb.value])
 
 
-def contacts(rect, maxc=8):
-    rect = _f(rect)
+def contacts(rect, maxc=8, polys=None):
+    rect = _f(poly_rect(polys) if polys is not None else rect)
+    set_polygons(polys)
     nbox = len(rect)
     nc = np.zeros(nbox, np.int32)
     nbr = np.zeros((nbox, maxc), np.int32)
@@ -173,6 +176,63 @@ def contacts(rect, maxc=8):
     if rc != 0:
         raise RuntimeError(lib().orc_last_error().decode())
     return nc, nbr, geo
+
+
+def poly_rect(polys):
+    """bounding boxes (xmin, xmax, ymin, ymax) of polygon vertex arrays"""
+    return np.array([[np.min(p[:, 0]), np.max(p[:, 0]), np.min(p[:, 1]), np.max(p[:, 1])]
+                     for p in (np.asarray(q, np.float64).reshape(-1, 2) for q in polys)])
+
+
+def set_polygons(polys):
+    """convex polygon boxes (list of (n_i, 2) vertex arrays, in the given
+    vertex order) for the next oracle call in place of its rectangles"""
+    if polys is None:
+        return
+    ps = [np.asarray(q, np.float64).reshape(-1, 2) for q in polys]
+    nv = np.array([len(q) for q in ps], np.int32)
+    xy = _f(np.concatenate(ps).ravel())
+    lib().orc_set_polygons(len(ps), nv.ctypes.data_as(ip32), _dp(xy))
+
+
+def box_info(rect=None, polys=None):
+    """per box: area, centre (cx, cy), bounding box and border count"""
+    rect = _f(poly_rect(polys) if polys is not None else rect)
+    out = np.zeros((len(rect), 8))
+    set_polygons(polys)
+    if lib().orc_box_info(len(rect), _dp(rect), _dp(out)) != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out
+
+
+def borders(ibox, rect=None, polys=None, maxb=32):
+    """borders of box ibox in the oracle's order: rows p1x p1y p2x p2y npx npy"""
+    rect = _f(poly_rect(polys) if polys is not None else rect)
+    out = np.zeros((maxb, 6))
+    set_polygons(polys)
+    n = lib().orc_borders(len(rect), _dp(rect), ibox, maxb, _dp(out))
+    if n < 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out[:n]
+
+
+def inside(ibox, xy, rect=None, polys=None):
+    rect = _f(poly_rect(polys) if polys is not None else rect)
+    xy = _f(np.asarray(xy, np.float64).reshape(-1, 2))
+    out = np.zeros(len(xy), np.int32)
+    set_polygons(polys)
+    if lib().orc_inside(len(rect), _dp(rect), ibox, len(xy), _dp(xy), out.ctypes.data_as(ip32)) != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out.astype(bool)
+
+
+def random_points(ibox, n, seed, rect=None, polys=None):
+    rect = _f(poly_rect(polys) if polys is not None else rect)
+    out = np.zeros((n, 2))
+    set_polygons(polys)
+    if lib().orc_random_points(len(rect), _dp(rect), ibox, n, C.c_uint64(seed), _dp(out)) != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+    return out
 
 
 def set_periodic(nbox, periodic):
@@ -188,7 +248,8 @@ def set_periodic(nbox, periodic):
 
 
 def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, ballistic=False, sigma=0.25,
-            nthreads=0, periodic=None, spectral=None, wscale=1.0, specularity=None, grad=False, mode_sigma=None):
+            nthreads=0, periodic=None, spectral=None, wscale=1.0, specularity=None, grad=False, mode_sigma=None,
+            polys=None):
     """spectral: {"boxes": [...], "nomega": K, "sigma": per-mode array or
     None}; the result then holds "spectral": one dict per location with
     omega, dT, jx, jy [nt][K] and fd [nt][nq]. wscale multiplies every
@@ -196,10 +257,11 @@ def rta_run(mat, rect, kind, Teq, thickness, T0, dt, maxtime, nparticles, seed, 
     probability of each side; grad adds "dTdp" [4][nt][nbox] for the sides
     with 0 < p < 1 (zeros elsewhere). mode_sigma: per-mode base broadening of
     the Lambert bounce (None: sigma for every mode)."""
-    rect, Teq = _f(rect), _f(Teq)
+    rect, Teq = _f(poly_rect(polys) if polys is not None else rect), _f(Teq)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
     set_periodic(nbox, periodic)
+    set_polygons(polys)
     nq = int(mat.get("nq", 1))
     if spectral:
         sb = np.ascontiguousarray(spectral["boxes"], np.int32)
@@ -249,12 +311,13 @@ def gamma_pdf(k, theta, x):
 
 
 def rta_steady_run(mat, rect, kind, Teq, thickness, T0, gradient, nparticles, seed, sigma=0.25, eps_energy=-42.0,
-                   eps_flux=-42.0, max_generations=0, nthreads=0, periodic=None):
+                   eps_flux=-42.0, max_generations=0, nthreads=0, periodic=None, polys=None):
     """steady branch: per box T, jx, jy and the number of generations"""
-    rect, Teq = _f(rect), _f(Teq)
+    rect, Teq = _f(poly_rect(polys) if polys is not None else rect), _f(Teq)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
     set_periodic(nbox, periodic)
+    set_polygons(polys)
     T, jx, jy = np.zeros(nbox), np.zeros(nbox), np.zeros(nbox)
     gens = C.c_int32()
     eeff = C.c_double()
@@ -272,11 +335,11 @@ def rta_steady_run(mat, rect, kind, Teq, thickness, T0, gradient, nparticles, se
 
 def bert_run(mat, rect, kind, Teq, Tinit, thickness, dt, maxtime, P=None, ngamma=0, rta=False, Ed=-1.0,
              Efactor=0, Tmax=0.0, Tmin=0.0, seed=1, max_steps=0, nthreads=0, periodic=None, Ps=None, Ts=None,
-             prop_rule=0, gradient=None, pcancel_each=4, init_dist=None):
+             prop_rule=0, gradient=None, pcancel_each=4, init_dist=None, polys=None):
     """Ps/Ts: propagators at several temperatures (replace P); gradient:
     (gx, gy) in every regular box instead of calculate_gradientT; init_dist:
     {box: signed distribution over the modes after the ngamma offset}"""
-    rect, Teq, Tinit = _f(rect), _f(Teq), _f(Tinit)
+    rect, Teq, Tinit = _f(poly_rect(polys) if polys is not None else rect), _f(Teq), _f(Tinit)
     kind = np.ascontiguousarray(kind, np.int32)
     nbox = len(kind)
     if Ps is not None:
@@ -291,6 +354,7 @@ def bert_run(mat, rect, kind, Teq, Tinit, thickness, dt, maxtime, P=None, ngamma
                            _dp(Pa), prop_rule, int(gradient is not None), C.c_double(gx), C.c_double(gy),
                            int(pcancel_each))
     set_periodic(nbox, periodic)
+    set_polygons(polys)
     n = len(mat["omega"])
     if init_dist:
         ib = np.ascontiguousarray(sorted(init_dist), np.int32)
