@@ -136,8 +136,7 @@ static int run(int argc, char** argv) {
         for (size_t l = 0; l < loc.size(); l++) {
             const int b = loc[l];
             if (B.kind[b] == 1) continue;  // reservoirs are not printed
-            const double* r = &B.rect[4 * (size_t)b];
-            const double vol = (r[1] - r[0]) * (r[3] - r[2]) * M.thickness;
+            const double vol = B.area[b] * M.thickness;
             const double* o = out.data() + l * ncols;
             double Ebox = 0;
             for (int i = 0; i < n; i++) Ebox += X[l * n + i];

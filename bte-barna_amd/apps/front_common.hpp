@@ -3,6 +3,7 @@
 #pragma once
 #include "bte_abi.h"
 #include "xml_lite.hpp"
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -25,10 +26,16 @@ inline void check(int rc, const char* what) {
     throw std::runtime_error(std::string(what) + ": " + msg);
 }
 
-// read_geometry_XML (src/geometry_2d.cpp:408-522) for axis-aligned
-// rectangles: boxes in file order, ids assigned in that order.
+// read_geometry_XML (src/geometry_2d.cpp:408-522): boxes in file order,
+// ids assigned in that order. Axis-aligned rectangles go to the rectangle
+// kernels; a geometry with any other convex box goes to the polygon kernels
+// with every box as its vertex list.
 struct Boxes {
-    std::vector<double> rect;  // xmin xmax ymin ymax per box
+    std::vector<double> rect;  // bounding box xmin xmax ymin ymax per box
+    std::vector<int32_t> nvert;
+    std::vector<double> xy;    // vertices, (x, y) pairs per box
+    std::vector<double> area;  // convex hull area per box
+    bool poly = false;
     std::vector<int32_t> kind;
     std::vector<int32_t> target;       // periodic boxes: <Translate_to id>
     std::vector<double> tx, ty;        // and their translation
@@ -36,6 +43,34 @@ struct Boxes {
     std::vector<std::string> material;
     int n() const { return (int)kind.size(); }
 };
+
+// area of the convex hull of a vertex list (monotone chain, shoelace)
+inline double hull_area(std::vector<std::pair<double, double>> p) {
+    std::sort(p.begin(), p.end());
+    p.erase(std::unique(p.begin(), p.end()), p.end());
+    if (p.size() < 3) return 0.;
+    auto turn = [](const std::pair<double, double>& a, const std::pair<double, double>& b,
+                   const std::pair<double, double>& c) {
+        return (b.first - a.first) * (c.second - a.second) - (b.second - a.second) * (c.first - a.first);
+    };
+    std::vector<std::pair<double, double>> h;
+    for (int pass = 0; pass < 2; pass++) {
+        const size_t start = h.size();
+        for (size_t q = 0; q < p.size(); q++) {
+            const auto& pt = pass ? p[p.size() - 1 - q] : p[q];
+            while (h.size() >= start + 2 && turn(h[h.size() - 2], h.back(), pt) <= 0.) h.pop_back();
+            h.push_back(pt);
+        }
+        h.pop_back();
+    }
+    double A = 0.;
+    for (size_t k = 0; k < h.size(); k++) {
+        const auto& a = h[k];
+        const auto& b = h[(k + 1) % h.size()];
+        A += a.first * b.second - a.second * b.first;
+    }
+    return 0.5 * A;
+}
 
 inline Boxes read_geometry(const std::string& fn) {
     auto root = xml::parse_file(fn);
@@ -61,9 +96,12 @@ inline Boxes read_geometry(const std::string& fn) {
         bool rect = np == 4 && x1 > x0 && y1 > y0;
         for (int k = 0; rect && k < np; k++)
             rect = (xy[2 * k] == x0 || xy[2 * k] == x1) && (xy[2 * k + 1] == y0 || xy[2 * k + 1] == y1);
-        if (!rect)
-            throw std::runtime_error("box " + std::to_string(id) +
-                                     ": only axis-aligned rectangles are supported by the GPU engine");
+        g.poly = g.poly || !rect;
+        g.nvert.push_back(np);
+        g.xy.insert(g.xy.end(), xy.begin(), xy.end());
+        std::vector<std::pair<double, double>> pts;
+        for (int k = 0; k < np; k++) pts.emplace_back(xy[2 * k], xy[2 * k + 1]);
+        g.area.push_back(rect ? (x1 - x0) * (y1 - y0) : hull_area(pts));
         int tgt = -1;
         double ptx = 0, pty = 0;
         if (const xml::Node* tr = b->child("Translate_to")) {
@@ -71,9 +109,12 @@ inline Boxes read_geometry(const std::string& fn) {
             std::stringstream ts(tr->text);
             if (!(ts >> ptx >> pty)) throw std::runtime_error("box " + std::to_string(id) + ": bad <Translate_to>");
         }
+        // <theta angle> sets geometry_2d::rotmat (geometry_2d.cpp:496-509),
+        // which neither solver reads: accepted and not applied
         if (const xml::Node* th = b->child("theta"))
             if (th->num("angle") != 0.)
-                throw std::runtime_error("box " + std::to_string(id) + ": rotated materials are not supported");
+                std::cerr << "# box " << id << ": theta " << th->num("angle")
+                          << " is read and, as in the reference, not applied" << std::endl;
         const xml::Node* ic = b->child("initCnd");
         if (!ic) throw std::runtime_error("box " + std::to_string(id) + ": no <initCnd>");
         const double Teq = ic->num("Teq");
@@ -128,8 +169,13 @@ struct Material {
 // the geometry handle with periodic translations applied
 inline bte_geom* make_geom(bte_ctx* ctx, const Boxes& B) {
     bte_geom* geom = nullptr;
-    check(bte_geom_create_rects(ctx, B.n(), B.rect.data(), B.kind.data(), B.Teq.data(), B.Tinit.data(), &geom),
-          "geometry");
+    if (B.poly)
+        check(bte_geom_create_polys(ctx, B.n(), B.nvert.data(), B.xy.data(), B.kind.data(), B.Teq.data(),
+                                    B.Tinit.data(), &geom),
+              "geometry");
+    else
+        check(bte_geom_create_rects(ctx, B.n(), B.rect.data(), B.kind.data(), B.Teq.data(), B.Tinit.data(), &geom),
+              "geometry");
     for (int i = 0; i < B.n(); i++)
         if (B.kind[i] == 2) check(bte_geom_set_periodic(geom, i, B.target[i], B.tx[i], B.ty[i]), "periodic box");
     return geom;

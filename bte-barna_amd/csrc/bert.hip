@@ -53,21 +53,29 @@ struct BertParams {
     const int32_t* box_rta_slot;
     const double* box_Teq;
     const double* spl;       // spline coefficients: per table 6 x nseg
-    const int32_t* side_spl; // per (box, side) spline table index, -1 none
+    const int32_t* side_spl; // per side spline table index, -1 none
+    // every geometry: the sides (rectangles: side s of box i at 4 i + s),
+    // their box, the side range, centre and area of each box; the POLY
+    // kernels take the box geometry from these
+    const DevSide* sides;
+    const int32_t* side_box;
+    const int2* srange;
+    const double2* center;
+    const double* box_area;
     BPart* pin;
     BPart* pout;
     uint32_t* seg;           // [2][nsegs] live count per buffer segment
     unsigned long long* H;   // [2][nbox][n] particle counts: positive | negative << 32
     double* pbox;            // [nsegs][nbox][3] per-workgroup (sum h, sum h vx, sum h vy)
     uint32_t* tickets;       // [nbox][n]
-    int32_t* Eb;             // [2][nbox*4]
+    int32_t* Eb;             // [2][nsd]
     double* props;           // [steps+1][nbox][3]
     int64_t* npart;          // [steps+1]
     int64_t* pending;        // [steps+1] signed wall counts awaiting re-emission
-    double* wcdf;            // [nbox*4][n] emission cumulative per box side, normalised to 1
-    int32_t* wguide;         // [nbox*4][BERT_GUIDE] first cumulative index per key
-    uint32_t* wcount;        // [nbox*4] particles to emit this step
-    int32_t* wsign;          // [nbox*4] their sign
+    double* wcdf;            // [nsd][n] emission cumulative per side, normalised to 1
+    int32_t* wguide;         // [nsd][BERT_GUIDE] first cumulative index per key
+    uint32_t* wcount;        // [nsd] particles to emit this step
+    int32_t* wsign;          // [nsd] their sign
     uint32_t* flags;         // [0] overflow, [1] bad mode, [2] stuck, [3] failed translation
     unsigned long long* stats;  // [0] step-kernel particle-steps, [1] emitted
     uint64_t cap;
@@ -78,13 +86,14 @@ struct BertParams {
     uint64_t seed;
     double dt, Eeff, thickness;
     int32_t n, ngamma, ncol, nbox, nseg;
+    int32_t nsd, poly;       // sides over all boxes; polygon boxes present
     int32_t rta, max_children;
     int32_t nslots;          // propagator slots (box temperatures)
     int32_t lds_props;       // properties from per-workgroup box sums (nbox <= BERT_LDS_BOXES)
     // dynamic LDS layout of the particle kernels (byte offsets)
     int32_t lds_pre, lds_eb, lds_sbox, lds_boxes, lds_cts;
     int32_t cts_lds;         // contacts staged in LDS too (GL instantiations)
-    int32_t eb_lds;          // wall counters in LDS (nbox*4 <= BERT_EB_LDS)
+    int32_t eb_lds;          // wall counters in LDS (nsd <= BERT_EB_LDS)
 };
 
 // The stream of particle pid in a given step and kernel: PCG32 with
@@ -126,8 +135,9 @@ __device__ inline void geo_stage(const BertParams& P, const DevBox*& boxes, cons
 // Eborder and ends the particle. Returns false if the particle ended.
 // PER compiles in the periodic translation; without periodic boxes the
 // kernels use the instantiation without it, which needs about 50 fewer
-// registers per lane (4 resident waves per SIMD instead of 3).
-template <bool PER>
+// registers per lane (4 resident waves per SIMD instead of 3). POLY: convex
+// polygon boxes (the geometry from the side tables).
+template <bool PER, bool POLY>
 __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const DevContact* cts, double& x,
                               double& y, int& box, int sign, double vx, double vy, double ivx, double ivy,
                               double dt, Pcg32& rng, int32_t* Eb) {
@@ -140,7 +150,8 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
                 return false;
             }
             // periodic box: translate and continue the flight (beyondRTA_MC2d.cpp:729-737)
-            const int nb = translate_rect(boxes, cts, P.cseg, P.ptrans, box, x, y, vx, vy, rng);
+            const int nb = POLY ? translate_poly(boxes, P.sides, P.srange, cts, P.cseg, P.ptrans, box, x, y, vx, vy, rng)
+                                : translate_rect(boxes, cts, P.cseg, P.ptrans, box, x, y, vx, vy, rng);
             if (nb < 0) {
                 atomicOr(P.flags + 3, 1u);
                 return false;
@@ -149,20 +160,24 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
             continue;
         }
         double nx = x + dt * vx, ny = y + dt * vy;
-        if (box_inside(B, nx, ny)) {
+        int2 sr = make_int2(4 * box, 4);
+        if constexpr (POLY) sr = P.srange[box];
+        if (POLY ? poly_inside(P.sides, sr, nx, ny) : box_inside(B, nx, ny)) {
             x = nx;
             y = ny;
             return true;
         }
         int mask;
         double ex, ey;
-        double tf = box_exit(B, x, y, vx, vy, ivx, ivy, mask, ex, ey);
+        double tf = POLY ? poly_exit(P.sides, sr, x, y, vx, vy, mask, ex, ey)
+                         : box_exit(B, x, y, vx, vy, ivx, ivy, mask, ex, ey);
         double left = dt - tf;
         if (almost_equal(left, 0.)) left = 0.;
         int side = 0;
-        const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
+        const int fast = POLY ? poly_side_fast(P.sides, sr, mask, ex, ey, vx, vy, side)
+                              : side_fast(B, mask, ex, ey, vx, vy, side);
         if (fast == SIDE_VOID) {
-            atomicAdd(Eb + box * 4 + side, sign);
+            atomicAdd(Eb + sr.x + side, sign);
             return false;
         }
         if (fast >= 0) {
@@ -174,11 +189,13 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
             continue;
         }
         Ids4 pb{};
-        int npb = find_pboxes(boxes, cts, B, ex, ey, vx, vy, pb);
+        int npb = POLY ? poly_find_pboxes(P.sides, P.srange, cts, B, ex, ey, vx, vy, pb)
+                       : find_pboxes(boxes, cts, B, ex, ey, vx, vy, pb);
         if (npb == 0) {
             int s = __popc(mask) == 1 ? __ffs(mask) - 1
-                                      : pick_void_side(boxes, cts, B, mask, ex, ey, rng.next());
-            atomicAdd(Eb + box * 4 + s, sign);
+                    : (POLY ? poly_pick_void_side(P.sides, P.srange, cts, B, sr, mask, ex, ey, rng.next())
+                            : pick_void_side(boxes, cts, B, mask, ex, ey, rng.next()));
+            atomicAdd(Eb + sr.x + s, sign);
             return false;
         }
         x = ex;
@@ -190,12 +207,18 @@ __device__ inline bool advect(const BertParams& P, const DevBox* boxes, const De
             int nc = 0;
             double tau = 1.0e-6;
             while (nc == 0 && tau >= 1.0e-15) {
-                for (int k = 0; k < npb && k < 4; k++)
-                    if (box_inside(boxes[pb.get(k)], ex + tau * vx, ey + tau * vy)) cand.set(nc++, pb.get(k));
+                for (int k = 0; k < npb && k < 4; k++) {
+                    const double tx = ex + tau * vx, ty = ey + tau * vy;
+                    const bool in = POLY ? poly_inside(P.sides, P.srange[pb.get(k)], tx, ty)
+                                         : box_inside(boxes[pb.get(k)], tx, ty);
+                    if (in) cand.set(nc++, pb.get(k));
+                }
                 tau /= 10.0;
             }
             if (nc == 0) {
-                double dx = 0.5 * (B.xmin + B.xmax) - ex, dy = 0.5 * (B.ymin + B.ymax) - ey;
+                const double ccx = POLY ? P.center[box].x : 0.5 * (B.xmin + B.xmax);
+                const double ccy = POLY ? P.center[box].y : 0.5 * (B.ymin + B.ymax);
+                double dx = ccx - ex, dy = ccy - ey;
                 double dn = sqrt(dx * dx + dy * dy);
                 x = ex + 1.0e-6 * dx / dn;
                 y = ey + 1.0e-6 * dy / dn;
@@ -518,7 +541,7 @@ __device__ inline void box_lds_flush(const BertParams& P, int myseg) {
 // w-th equal slice of the live particle list (all input segments).
 // P1 without periodic boxes fits 96 VGPRs: five workgroups (waves per SIMD)
 // stay resident per CU
-template <bool GL, bool PER, bool P1>
+template <bool GL, bool PER, bool P1, bool POLY>
 __global__ __launch_bounds__(256, (P1 && !PER) ? 5 : 1) void bert_step_kernel(BertParams P, uint32_t step, int cur,
                                                         int cancel_prev, int hist) {
     __shared__ uint32_t tsum[16];
@@ -527,7 +550,7 @@ __global__ __launch_bounds__(256, (P1 && !PER) ? 5 : 1) void bert_step_kernel(Be
     const DevBox* boxes;
     const DevContact* cts;
     geo_stage<GL>(P, boxes, cts);
-    const int nb4 = P.nbox * 4;
+    const int nb4 = P.nsd;
     int32_t* Eg = P.Eb + (size_t)(step & 1) * nb4;
     int32_t* Eb = eb_lds_begin(P, Eg, nb4);
     if (threadIdx.x == 0) lcount = 0;
@@ -591,7 +614,7 @@ __global__ __launch_bounds__(256, (P1 && !PER) ? 5 : 1) void bert_step_kernel(Be
                 if (nslots_of<P1>(P) == 1) f0 = first_pick<P1>(P, rng, mode, 0);
                 const double2 v = P.vel[mode];
                 const double2 iv = P.ivel[mode];
-                alive = advect<PER>(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
+                alive = advect<PER, POLY>(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
                 if (nslots_of<P1>(P) > 1 && alive) f0 = first_pick<P1>(P, rng, mode, box);
             }
         }
@@ -638,9 +661,9 @@ __device__ inline double block_scan_256(double v, double* wtot, double& tot) {
 __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_t step) {
     __shared__ double wtot[4];
     __shared__ int kcarry[4];
-    const int b = blockIdx.x;  // box*4 + side
-    const int box = b >> 2, side = b & 3;
-    int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
+    const int b = blockIdx.x;  // side
+    const int box = P.side_box[b];
+    int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nsd;
     const int E = Eb[b];
     const int si = P.side_spl[b];
     if (E == 0 || si < 0) {
@@ -652,9 +675,9 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     }
     __syncthreads();
     if (threadIdx.x == 0) Eb[b] = 0;
-    const DevBox B = P.boxes[box];
+    const DevSide sd = P.sides[b];
     const double Teq = P.box_Teq[box];
-    const double L = (side == 0 || side == 3) ? (B.ymax - B.ymin) : (B.xmax - B.xmin);
+    const double L = hypot(sd.p2x - sd.p1x, sd.p2y - sd.p1y);
     const double J = P.Eeff * (double)E / P.dt / (L * P.thickness);
     const double* S = P.spl + (size_t)si * 6 * P.nseg;  // a b c d x0 xe
     // lower bound of J among the ascending knots: the count of knots below J
@@ -666,10 +689,7 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     if (lo >= P.nseg) lo = P.nseg - 1;
     const double dx = J - S[4 * P.nseg + lo];
     const double Tb = S[lo] + dx * (S[P.nseg + lo] + dx * (S[2 * P.nseg + lo] + S[3 * P.nseg + lo] * dx));
-    double nx, ny;
-    side_normal(side, nx, ny);
-    nx = -nx;  // emission normal points into the box
-    ny = -ny;
+    const double nx = -sd.nx, ny = -sd.ny;  // emission normal points into the box
     const int n = P.n;
     double* cdf = P.wcdf + (size_t)b * n;
     int32_t* G = P.wguide + (size_t)b * BERT_GUIDE;
@@ -794,12 +814,12 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
 // kernel's segment walk).
 #define BERT_EMIT_MAXB 4096
 #define BERT_EMIT_THREADS 512
-template <bool GL, bool PER, bool P1>
+template <bool GL, bool PER, bool P1, bool POLY>
 __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
     __shared__ uint32_t tsum[16];
     __shared__ uint32_t lcount;
     uint32_t* pre = (uint32_t*)(bert_dyn + P.lds_pre);
-    const int nb4 = P.nbox * 4;
+    const int nb4 = P.nsd;
     const int myseg = P.seg_emit0 + blockIdx.x;
     uint32_t* segout = P.seg + (size_t)(cur ^ 1) * P.nsegs;
     if (threadIdx.x == 0) lcount = 0;
@@ -832,7 +852,7 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
             int b = bw;
             while (b < nb4 - 1 && k >= pre[b]) b++;
             const uint32_t j = k - (b > 0 ? pre[b - 1] : 0u);
-            bx = b >> 2;
+            bx = POLY ? P.side_box[b] : b >> 2;
             const int side = b & 3;
             sign = P.wsign[b];
             pid = mix64(P.seed ^ ((uint64_t)step << 40) ^ ((uint64_t)b << 20) ^ (uint64_t)j ^ 0xad1aULL);
@@ -843,12 +863,17 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
             int l = P.wguide[(size_t)b * BERT_GUIDE + guide_key(u, (double)BERT_GUIDE)];
             while (l < P.n - 1 && cdf[l] < u) l++;
             mode = l;
-            const DevBox B = boxes[bx];
             double p1x, p1y, ndx, ndy;
-            if (side == 0) { p1x = B.xmin; p1y = B.ymin; ndx = 0; ndy = B.ymax - B.ymin; }
-            else if (side == 1) { p1x = B.xmin; p1y = B.ymin; ndx = B.xmax - B.xmin; ndy = 0; }
-            else if (side == 2) { p1x = B.xmin; p1y = B.ymax; ndx = B.xmax - B.xmin; ndy = 0; }
-            else { p1x = B.xmax; p1y = B.ymax; ndx = 0; ndy = B.ymin - B.ymax; }
+            if constexpr (POLY) {
+                const DevSide sd = P.sides[b];
+                p1x = sd.p1x; p1y = sd.p1y; ndx = sd.p2x - sd.p1x; ndy = sd.p2y - sd.p1y;
+            } else {
+                const DevBox B = boxes[bx];
+                if (side == 0) { p1x = B.xmin; p1y = B.ymin; ndx = 0; ndy = B.ymax - B.ymin; }
+                else if (side == 1) { p1x = B.xmin; p1y = B.ymin; ndx = B.xmax - B.xmin; ndy = 0; }
+                else if (side == 2) { p1x = B.xmin; p1y = B.ymax; ndx = B.xmax - B.xmin; ndy = 0; }
+                else { p1x = B.xmax; p1y = B.ymax; ndx = 0; ndy = B.ymin - B.ymax; }
+            }
             double t = rng.uniform();
             x = p1x + t * ndx;
             y = p1y + t * ndy;
@@ -856,7 +881,7 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
             const double2 iv = P.ivel[mode];
             double rdt = P.dt * rng.uniform();
             if (nslots_of<P1>(P) == 1) f0 = first_pick<P1>(P, rng, mode, 0);
-            alive = advect<PER>(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
+            alive = advect<PER, POLY>(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
             if (nslots_of<P1>(P) > 1 && alive) f0 = first_pick<P1>(P, rng, mode, bx);
         }
         scatter_out<P1>(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
@@ -886,7 +911,7 @@ struct BertSource {
     int32_t area, pad;
 };
 #define BERT_SRC_BLOCKS 256
-template <bool PER>
+template <bool PER, bool POLY>
 __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const BertSource* src, int nparts_all,
                                                           const uint2* stab, uint32_t step, int cur, int hist) {
     __shared__ uint32_t lcount;
@@ -895,7 +920,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
     if (threadIdx.x == 0) lcount = 0;
     double* sbox = box_lds_begin(P);
     __syncthreads();
-    int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nbox * 4;
+    int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nsd;
     unsigned long long* H = hist ? P.H + (size_t)cur * P.nbox * P.n : nullptr;
     unsigned long long done = 0;
     const int j0 = (int)((int64_t)nparts_all * blockIdx.x / gridDim.x);
@@ -924,15 +949,33 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
                 seed_stream(rng, skey, pid);
                 double t = rng.uniform();
                 if (S.area) {
-                    // get_random_point of a rectangle: x, then y
+                    // get_random_point (geometry_2d.hpp:332-348): x, then y
+                    // in the bounding box until covered by the box
                     x = S.p1x + t * S.ndx;
                     y = S.p1y + rng.uniform() * S.ndy;
+                    if constexpr (POLY) {
+                        const int2 sr = P.srange[S.box];
+                        int tries = 0;
+                        while (!poly_inside(P.sides, sr, x, y)) {
+                            if (++tries > 4096) {  // a sliver in its bounding box: reported
+                                atomicOr(P.flags + 3, 1u);
+                                x = P.center[S.box].x;
+                                y = P.center[S.box].y;
+                                break;
+                            }
+                            x = S.p1x + rng.uniform() * S.ndx;
+                            y = S.p1y + rng.uniform() * S.ndy;
+                        }
+                    }
                 } else {
                     x = S.p1x + t * S.ndx;
                     y = S.p1y + t * S.ndy;
                     const DevBox B = P.boxes[S.box];
-                    if (!box_inside(B, x, y)) {
-                        double dx = 0.5 * (B.xmin + B.xmax) - x, dy = 0.5 * (B.ymin + B.ymax) - y;
+                    const bool in = POLY ? poly_inside(P.sides, P.srange[S.box], x, y) : box_inside(B, x, y);
+                    if (!in) {
+                        const double ccx = POLY ? P.center[S.box].x : 0.5 * (B.xmin + B.xmax);
+                        const double ccy = POLY ? P.center[S.box].y : 0.5 * (B.ymin + B.ymax);
+                        double dx = ccx - x, dy = ccy - y;
                         double dn = sqrt(dx * dx + dy * dy);
                         x += 1.0e-6 * dx / dn;
                         y += 1.0e-6 * dy / dn;
@@ -945,7 +988,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
                 const double2 iv = P.ivel[mode];
                 double rdt = P.dt * rng.uniform();
                 if (nslots_of<false>(P) == 1) f0 = first_pick<false>(P, rng, mode, 0);
-                alive = advect<PER>(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
+                alive = advect<PER, POLY>(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
                 if (nslots_of<false>(P) > 1 && alive) f0 = first_pick<false>(P, rng, mode, bx);
             }
             scatter_out<false>(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
@@ -1027,7 +1070,7 @@ __global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, 
     }
     if (threadIdx.x == 0) {
         const DevBox B = P.boxes[box];
-        double vol = (B.xmax - B.xmin) * (B.ymax - B.ymin) * P.thickness;
+        double vol = P.box_area[box] * P.thickness;
         bool skip = B.kind != 0;
         double* o = P.props + ((size_t)row * P.nbox + box) * 3;
         o[0] = skip ? 0. : P.Eeff / vol * 1.0e27 * red[0][0];
@@ -1037,8 +1080,10 @@ __global__ __launch_bounds__(256) void bert_props_kernel(BertParams P, int row, 
         // the step cancels, else every particle written this step
         if (cancel_now && !skip)
             atomicAdd((unsigned long long*)(P.npart + row), (unsigned long long)redc[1][0]);
-        const int32_t* E = P.Eb + (size_t)ebn * P.nbox * 4 + (size_t)box * 4;
-        long long pend = (long long)E[0] + E[1] + E[2] + E[3];
+        const int2 sr = P.srange[box];
+        const int32_t* E = P.Eb + (size_t)ebn * P.nsd + sr.x;
+        long long pend = 0;
+        for (int k = 0; k < sr.y; k++) pend += E[k];
         if (pend) atomicAdd((unsigned long long*)(P.pending + row), (unsigned long long)pend);
     }
     if (box == 0) {
@@ -1095,6 +1140,21 @@ __global__ __launch_bounds__(256) void bert_init_kernel(BertParams P, const Bert
         seed_stream(rng, stream_key(P.seed, 0xffffffu, 4), pid);
         double x = I.xmin + rng.uniform() * (I.xmax - I.xmin);
         double y = I.ymin + rng.uniform() * (I.ymax - I.ymin);
+        if (P.poly) {
+            // get_random_point: the bounding box until covered by the box
+            const int2 sr = P.srange[I.box];
+            int tries = 0;
+            while (!poly_inside(P.sides, sr, x, y)) {
+                if (++tries > 4096) {  // a sliver in its bounding box: reported
+                    atomicOr(P.flags + 3, 1u);
+                    x = P.center[I.box].x;
+                    y = P.center[I.box].y;
+                    break;
+                }
+                x = I.xmin + rng.uniform() * (I.xmax - I.xmin);
+                y = I.ymin + rng.uniform() * (I.ymax - I.ymin);
+            }
+        }
         bool ng;
         int mode = (int)alias_pick(itab + I.tab, (uint32_t)P.n, rng, ng);
         // tables of given distributions carry each mode's sign in the alias word
@@ -1104,19 +1164,26 @@ __global__ __launch_bounds__(256) void bert_init_kernel(BertParams P, const Bert
     }
 }
 
-// particle kernels by [geometry in LDS][periodic boxes][P1]
+// particle kernels by [geometry in LDS][periodic boxes][P1]; polygon
+// geometries by [periodic boxes][P1] (geometry in global memory)
 using StepKernel = void (*)(BertParams, uint32_t, int, int, int);
 using EmitKernel = void (*)(BertParams, uint32_t, int, int);
 static const StepKernel step_kernel_tab[2][2][2] = {
-    {{bert_step_kernel<false, false, false>, bert_step_kernel<false, false, true>},
-     {bert_step_kernel<false, true, false>, bert_step_kernel<false, true, true>}},
-    {{bert_step_kernel<true, false, false>, bert_step_kernel<true, false, true>},
-     {bert_step_kernel<true, true, false>, bert_step_kernel<true, true, true>}}};
+    {{bert_step_kernel<false, false, false, false>, bert_step_kernel<false, false, true, false>},
+     {bert_step_kernel<false, true, false, false>, bert_step_kernel<false, true, true, false>}},
+    {{bert_step_kernel<true, false, false, false>, bert_step_kernel<true, false, true, false>},
+     {bert_step_kernel<true, true, false, false>, bert_step_kernel<true, true, true, false>}}};
 static const EmitKernel emit_kernel_tab[2][2][2] = {
-    {{bert_emit_kernel<false, false, false>, bert_emit_kernel<false, false, true>},
-     {bert_emit_kernel<false, true, false>, bert_emit_kernel<false, true, true>}},
-    {{bert_emit_kernel<true, false, false>, bert_emit_kernel<true, false, true>},
-     {bert_emit_kernel<true, true, false>, bert_emit_kernel<true, true, true>}}};
+    {{bert_emit_kernel<false, false, false, false>, bert_emit_kernel<false, false, true, false>},
+     {bert_emit_kernel<false, true, false, false>, bert_emit_kernel<false, true, true, false>}},
+    {{bert_emit_kernel<true, false, false, false>, bert_emit_kernel<true, false, true, false>},
+     {bert_emit_kernel<true, true, false, false>, bert_emit_kernel<true, true, true, false>}}};
+static const StepKernel step_kernel_poly_tab[2][2] = {
+    {bert_step_kernel<false, false, false, true>, bert_step_kernel<false, false, true, true>},
+    {bert_step_kernel<false, true, false, true>, bert_step_kernel<false, true, true, true>}};
+static const EmitKernel emit_kernel_poly_tab[2][2] = {
+    {bert_emit_kernel<false, false, false, true>, bert_emit_kernel<false, false, true, true>},
+    {bert_emit_kernel<false, true, false, true>, bert_emit_kernel<false, true, true, true>}};
 
 struct Bert {
     Ctx* ctx = nullptr;
@@ -1131,9 +1198,10 @@ struct Bert {
     DevBuf<double2> d_vel, d_ivel;
     DevBuf<BPart> d_parts;
     DevBuf<uint32_t> d_seg, d_tickets, d_flags;
-    DevBuf<int32_t> d_Eb, d_side_spl, d_rta_slot;
+    DevBuf<int32_t> d_Eb, d_side_spl, d_rta_slot, d_side_box;
     DevBuf<unsigned long long> d_H;
-    DevBuf<double> d_props, d_spl, d_Teq, d_pbox;
+    DevBuf<double> d_props, d_spl, d_Teq, d_pbox, d_box_area;
+    DevBuf<double2> d_center;
     DevBuf<int64_t> d_npart, d_pending;
     DevBuf<double> d_wcdf;
     DevBuf<int32_t> d_wguide;
@@ -1273,6 +1341,13 @@ static BertParams make_params(Bert* b) {
     P.cts = b->g->d_contacts.p;
     P.cseg = b->g->d_cseg.p;
     P.ptrans = b->g->d_ptrans.p;
+    P.sides = b->g->d_sides.p;
+    P.srange = b->g->d_srange.p;
+    P.side_box = b->d_side_box.p;
+    P.center = b->d_center.p;
+    P.box_area = b->d_box_area.p;
+    P.nsd = (int32_t)b->g->sides.size();
+    P.poly = b->g->poly ? 1 : 0;
     P.palias = b->palias;
     P.box_pslot = b->d_pslot.p;
     P.nslots = b->nslots;
@@ -1316,7 +1391,7 @@ static BertParams make_params(Bert* b) {
     P.pbox = b->d_pbox.p;
     P.lds_props = (int)b->g->boxes.size() <= BERT_LDS_BOXES ? 1 : 0;
     // dynamic LDS layout
-    const int nb4 = P.nbox * 4;
+    const int nb4 = P.nsd;
     auto up16 = [](size_t v) { return (v + 15) & ~(size_t)15; };
     size_t off = 0;
     P.lds_pre = (int32_t)off;
@@ -1327,7 +1402,7 @@ static BertParams make_params(Bert* b) {
     P.lds_sbox = (int32_t)off;
     if (P.lds_props) off = up16(off + sizeof(double) * 3 * (size_t)P.nbox);
     const size_t geo = sizeof(DevBox) * (size_t)P.nbox + sizeof(DevContact) * (size_t)P.ncts;
-    b->geo_lds = geo <= BERT_GEO_LDS_BYTES;
+    b->geo_lds = geo <= BERT_GEO_LDS_BYTES && !b->g->poly;  // polygon kernels read the geometry globally
     P.lds_boxes = (int32_t)off;
     P.lds_cts = (int32_t)up16(off + sizeof(DevBox) * (size_t)P.nbox);
     P.cts_lds = 1;
@@ -1442,7 +1517,8 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     if (!cfg->rta && !p) throw Error(BTE_EINVAL, "a propagator is required unless rta is set");
     if (p && p->ngamma + p->ncol != m->n) throw Error(BTE_EINVAL, "propagator size does not match the mode table");
     if (!(cfg->dt > 0) || !(cfg->thickness > 0)) throw Error(BTE_EINVAL, "bad beRTA configuration");
-    if ((size_t)g->boxes.size() * 4 > BERT_EMIT_MAXB) throw Error(BTE_EINVAL, "more than 1024 boxes");
+    if (g->sides.size() > BERT_EMIT_MAXB)
+        throw Error(BTE_EINVAL, "more than " + std::to_string(BERT_EMIT_MAXB) + " box sides");
     for (size_t i = 0; i < g->boxes.size(); i++)
         if (g->dboxes[i].kind == 2 && g->dboxes[i].tgt < 0)
             throw Error(BTE_EGEOM, "periodic box " + std::to_string(i) + " has no translation");
@@ -1684,17 +1760,28 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     b->d_H.alloc((size_t)2 * nbox * n);
     b->d_tickets.alloc((size_t)nbox * n);
     if (nbox <= BERT_LDS_BOXES) b->d_pbox.alloc((size_t)b->nsegs * nbox * 3);
-    b->d_Eb.alloc((size_t)2 * nbox * 4);
+    b->d_Eb.alloc((size_t)2 * g->sides.size());
     b->d_props.alloc((size_t)(b->max_steps + 1) * nbox * 3);
     b->d_npart.alloc((size_t)(b->max_steps + 1));
     b->d_pending.alloc((size_t)(b->max_steps + 1));
-    b->d_wcdf.alloc((size_t)nbox * 4 * n);
-    b->d_wguide.alloc((size_t)nbox * 4 * BERT_GUIDE);
-    b->d_wcount.alloc((size_t)nbox * 4);
-    b->d_wsign.alloc((size_t)nbox * 4);
-    std::vector<double> Teq(nbox);
-    for (int i = 0; i < nbox; i++) Teq[i] = g->boxes[i].Teq;
+    const size_t nsd = g->sides.size();
+    b->d_wcdf.alloc(nsd * n);
+    b->d_wguide.alloc(nsd * BERT_GUIDE);
+    b->d_wcount.alloc(nsd);
+    b->d_wsign.alloc(nsd);
+    std::vector<double> Teq(nbox), area(nbox);
+    std::vector<double2> centre(nbox);
+    std::vector<int32_t> side_box(nsd);
+    for (int i = 0; i < nbox; i++) {
+        Teq[i] = g->boxes[i].Teq;
+        area[i] = g->boxes[i].area();
+        centre[i] = make_double2(g->boxes[i].cx(), g->boxes[i].cy());
+        for (int k = 0; k < g->srange[i].y; k++) side_box[g->srange[i].x + k] = i;
+    }
     b->d_Teq.upload(Teq, s);
+    b->d_box_area.upload(area, s);
+    b->d_center.upload(centre, s);
+    b->d_side_box.upload(side_box, s);
     if (!inits.empty()) {
         b->d_init.upload(inits, s);
         b->d_init_tab.upload(itab, s);
@@ -1776,15 +1863,15 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     for (int i = 0; i < 100; i++) Ts[i] = 10.0 + 10.0 * i;
     std::map<std::pair<double, int>, int> spl_id;
     std::vector<double> spl;
-    std::vector<int32_t> side_spl((size_t)nbox * 4, -1);
+    std::vector<int32_t> side_spl(g->sides.size(), -1);
     for (int i = 0; i < nbox; i++) {
         const RectBox& bx = g->boxes[i];
         if (bx.kind != 0) continue;
-        for (int sd = 0; sd < 4; sd++) {
-            auto key = std::make_pair(bx.Teq, sd);
+        for (int sd = g->srange[i].x; sd < g->srange[i].x + g->srange[i].y; sd++) {
+            // one J(T) table per box temperature and side normal
+            auto key = std::make_pair(bx.Teq, (int)g->sides[sd].nid);
             if (!spl_id.count(key)) {
-                double nx, ny;
-                side_normal(sd, nx, ny);
+                const double nx = g->sides[sd].nx, ny = g->sides[sd].ny;
                 std::vector<double> J(100);
                 for (int t = 0; t < 100; t++) {
                     double cum = wall_pmf(*m, Ts[t], bx.Teq, -nx, -ny, pmf.data());
@@ -1797,7 +1884,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
                 spl_id[key] = id;
                 for (auto* v : {&sp.a, &sp.b, &sp.c, &sp.d, &sp.x0, &sp.xe}) spl.insert(spl.end(), v->begin(), v->end());
             }
-            side_spl[(size_t)i * 4 + sd] = spl_id[key];
+            side_spl[sd] = spl_id[key];
         }
     }
     b->d_spl.upload(spl.empty() ? std::vector<double>(1, 0.) : spl, s);
@@ -1830,7 +1917,7 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     BPart* buf[2] = {b->d_parts.p, b->d_parts.p + b->cap};
     HIPCHECK(hipMemsetAsync(b->d_H.p, 0, (size_t)2 * nbox * n * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(b->d_tickets.p, 0, (size_t)nbox * n * sizeof(uint32_t), s));
-    HIPCHECK(hipMemsetAsync(b->d_Eb.p, 0, (size_t)2 * nbox * 4 * sizeof(int32_t), s));
+    HIPCHECK(hipMemsetAsync(b->d_Eb.p, 0, (size_t)2 * b->g->sides.size() * sizeof(int32_t), s));
     HIPCHECK(hipMemsetAsync(b->d_flags.p, 0, 4 * sizeof(uint32_t), s));
     HIPCHECK(hipMemsetAsync(b->d_stats.p, 0, 2 * sizeof(unsigned long long), s));
     HIPCHECK(hipMemsetAsync(b->d_npart.p, 0, (size_t)(b->max_steps + 1) * sizeof(int64_t), s));
@@ -1882,18 +1969,20 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         const bool timed = (istep % pc) == ((istep / pc) % pc);
         if (timed) HIPCHECK(hipEventRecord(b->kev[2 * istep], s));
         {
-            auto k = step_kernel_tab[b->geo_lds][b->periodic][p1];
+            auto k = b->g->poly ? step_kernel_poly_tab[b->periodic][p1] : step_kernel_tab[b->geo_lds][b->periodic][p1];
             hipLaunchKernelGGL(k, dim3(step_blocks), dim3(256), b->lds_bytes, s, P, (uint32_t)istep, cur, cancel_prev,
                                hist);
         }
         if (timed) HIPCHECK(hipEventRecord(b->kev[2 * istep + 1], s));
         if (b->nsrc)
-            hipLaunchKernelGGL(b->periodic ? bert_source_kernel<true> : bert_source_kernel<false>, dim3(b->src_blocks),
+            hipLaunchKernelGGL(b->g->poly ? (b->periodic ? bert_source_kernel<true, true> : bert_source_kernel<false, true>)
+                                          : (b->periodic ? bert_source_kernel<true, false> : bert_source_kernel<false, false>),
+                               dim3(b->src_blocks),
                                dim3(256), b->lds_bytes, s, P, b->d_src.p, b->nsrc, b->d_src_tab.p, (uint32_t)istep,
                                cur, hist);
-        hipLaunchKernelGGL(bert_wallcdf_kernel, dim3(nbox * 4), dim3(256), 0, s, P, (uint32_t)istep);
+        hipLaunchKernelGGL(bert_wallcdf_kernel, dim3((unsigned)b->g->sides.size()), dim3(256), 0, s, P, (uint32_t)istep);
         {
-            auto k = emit_kernel_tab[b->geo_lds][b->periodic][p1];
+            auto k = b->g->poly ? emit_kernel_poly_tab[b->periodic][p1] : emit_kernel_tab[b->geo_lds][b->periodic][p1];
             hipLaunchKernelGGL(k, dim3(emit_blocks), dim3(BERT_EMIT_THREADS), b->lds_bytes, s, P, (uint32_t)istep, cur,
                                hist);
         }

@@ -44,6 +44,10 @@ struct RtaParams {
     const DevContact* cts;
     const double4* cseg;    // contact segments (periodic translation)
     const double2* ptrans;  // periodic translation per box
+    // polygon geometries (POLY kernels): sides, side range and centre per box
+    const DevSide* sides;
+    const int2* srange;
+    const double2* center;
     const RtaGen* gens;
     const uint2* gen_sel;  // alias over each run's generators
     const uint2* gen_tab;  // per generator, n entries
@@ -297,7 +301,7 @@ struct RtaCounts {
 };
 
 // Init_generator (RTA_MC2D.cpp:436-484): particle pid of its run at t = 0
-template <bool GRAD>
+template <bool GRAD, bool POLY>
 __device__ inline void rta_start(const RtaParams& P, RtaState& S, uint64_t pid) {
     S.budget = 0;
     S.still = 0;
@@ -313,12 +317,22 @@ __device__ inline void rta_start(const RtaParams& P, RtaState& S, uint64_t pid) 
     double u = S.rng.uniform();
     S.x = g.p1x + u * g.ndx;
     S.y = g.p1y + u * g.ndy;
-    const DevBox bi = P.boxes[g.idi];
-    if (!box_inside(bi, S.x, S.y)) {
-        double dx = 0.5 * (bi.xmin + bi.xmax) - S.x, dy = 0.5 * (bi.ymin + bi.ymax) - S.y;
-        double dn = sqrt(dx * dx + dy * dy);
-        S.x += 1.0e-6 * dx / dn;
-        S.y += 1.0e-6 * dy / dn;
+    if constexpr (POLY) {
+        if (!poly_inside(P.sides, P.srange[g.idi], S.x, S.y)) {
+            const double2 c = P.center[g.idi];
+            double dx = c.x - S.x, dy = c.y - S.y;
+            double dn = sqrt(dx * dx + dy * dy);
+            S.x += 1.0e-6 * dx / dn;
+            S.y += 1.0e-6 * dy / dn;
+        }
+    } else {
+        const DevBox bi = P.boxes[g.idi];
+        if (!box_inside(bi, S.x, S.y)) {
+            double dx = 0.5 * (bi.xmin + bi.xmax) - S.x, dy = 0.5 * (bi.ymin + bi.ymax) - S.y;
+            double dn = sqrt(dx * dx + dy * dy);
+            S.x += 1.0e-6 * dx / dn;
+            S.y += 1.0e-6 * dy / dn;
+        }
     }
     S.mode = (int)alias_pick(P.gen_tab + (size_t)gi * P.n, (uint32_t)P.n, S.rng, ng);
     S.sign = g.sign;
@@ -341,7 +355,8 @@ __device__ inline void rta_watch(const RtaParams& P, RtaState& S, RtaCounts& C) 
 
 // One step of evolparticle (RTA_MC2D.cpp:990-1386) after the flight draw of
 // the history loop (2262-2288).
-template <bool GRAD, int TM>
+// POLY: the geometry has convex polygon boxes (diffuse sides only)
+template <bool GRAD, int TM, bool POLY>
 __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, RtaState& S, RtaCounts& C) {
     const double4 md = P.modes[S.mode];
     if (S.need_flight) {
@@ -359,7 +374,9 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, Rta
     const DevBox B = P.boxes[S.box];
     if (B.kind == 2) {
         // periodic box: translate, keep the remaining flight (RTA_MC2D.cpp:1034-1043)
-        const int nb = translate_rect(P.boxes, P.cts, P.cseg, P.ptrans, S.box, S.x, S.y, md.x, md.y, S.rng);
+        const int nb = POLY ? translate_poly(P.boxes, P.sides, P.srange, P.cts, P.cseg, P.ptrans, S.box, S.x, S.y,
+                                             md.x, md.y, S.rng)
+                            : translate_rect(P.boxes, P.cts, P.cseg, P.ptrans, S.box, S.x, S.y, md.x, md.y, S.rng);
         if (nb < 0) {
             C.badtrans++;
             S.alive = 0;
@@ -376,12 +393,18 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, Rta
     const int gmask = GRAD ? P.runs[S.run].gmask : 0;
     const double vx = md.x, vy = md.y;
     double nx = S.x + S.dtrem * vx, ny = S.y + S.dtrem * vy;
-    const bool inside = box_inside(B, nx, ny);
+    int2 sr = make_int2(0, 0);
+    if constexpr (POLY) sr = P.srange[S.box];
+    const bool inside = POLY ? poly_inside(P.sides, sr, nx, ny) : box_inside(B, nx, ny);
     int mask = 0;
     double ex = 0., ey = 0., tf = 0.;
     if (!inside) {
-        const double2 iv = P.ivel[S.mode];
-        tf = box_exit(B, S.x, S.y, vx, vy, iv.x, iv.y, mask, ex, ey);
+        if constexpr (POLY) {
+            tf = poly_exit(P.sides, sr, S.x, S.y, vx, vy, mask, ex, ey);
+        } else {
+            const double2 iv = P.ivel[S.mode];
+            tf = box_exit(B, S.x, S.y, vx, vy, iv.x, iv.y, mask, ex, ey);
+        }
     }
     if (inside) {
         tally_segment<GRAD, TM>(P, lds, A, S.box, S.t, S.t + S.dtrem, S.sign, vx, vy, gmask, S.score, S.mode);
@@ -401,22 +424,27 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, Rta
     tally_segment<GRAD, TM>(P, lds, A, S.box, S.t, S.t + tf, S.sign, vx, vy, gmask, S.score, S.mode);
     S.t += tf;
     int side = 0;
-    const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
+    const int fast = POLY ? poly_side_fast(P.sides, sr, mask, ex, ey, vx, vy, side)
+                          : side_fast(B, mask, ex, ey, vx, vy, side);
     Ids4 pb{};
     int npb;
     if (fast >= 0) {
         pb.a = fast;
         npb = 1;
+    } else if (fast == SIDE_VOID) {
+        npb = 0;
     } else {
-        npb = fast == SIDE_VOID ? 0 : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
+        npb = POLY ? poly_find_pboxes(P.sides, P.srange, P.cts, B, ex, ey, vx, vy, pb)
+                   : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
     }
     if (npb == 0) {
         int s = fast == SIDE_VOID ? side
               : (__popc(mask) == 1 ? __ffs(mask) - 1
-                                   : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, S.rng.next()));
+                 : (POLY ? poly_pick_void_side(P.sides, P.srange, P.cts, B, sr, mask, ex, ey, S.rng.next())
+                         : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, S.rng.next())));
         // specular with probability p (mirror partner of the mode), else
-        // diffuse; a draw is made only when 0 < p < 1
-        const float p = P.runs[S.run].spec[s];
+        // diffuse; a draw is made only when 0 < p < 1 (polygon runs: diffuse)
+        const float p = POLY ? 0.f : P.runs[S.run].spec[s];
         bool specular = p >= 1.f;
         if (p > 0.f && p < 1.f) {
             specular = S.rng.uniform() < (double)p;
@@ -440,8 +468,9 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, Rta
         } else {
             // Lambert re-emission (RTA_MC2D.cpp:635-678)
             const int4 li = P.lam_info[S.mode];
+            const int tabk = POLY ? P.sides[sr.x + s].nid : s;
             bool ng;
-            int k = (int)alias_pick(P.lam_tab + li.z + (size_t)s * li.y, (uint32_t)li.y, S.rng, ng);
+            int k = (int)alias_pick(P.lam_tab + li.z + (size_t)tabk * li.y, (uint32_t)li.y, S.rng, ng);
             S.mode = P.perm[li.x + k];
         }
         // the remaining flight is discarded (RTA_MC2D.cpp:1259)
@@ -460,14 +489,25 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, Rta
         Ids4 cand{};
         int nc = 0;
         double cx = ex + 1.0e-6 * vx, cy = ey + 1.0e-6 * vy;
-        for (int k = 0; k < npb && k < 4; k++)
-            if (box_inside(P.boxes[pb.get(k)], cx, cy)) cand.set(nc++, pb.get(k));
+        for (int k = 0; k < npb && k < 4; k++) {
+            const bool in = POLY ? poly_inside(P.sides, P.srange[pb.get(k)], cx, cy)
+                                 : box_inside(P.boxes[pb.get(k)], cx, cy);
+            if (in) cand.set(nc++, pb.get(k));
+        }
         if (nc > 0) {
             S.box = cand.get((int)S.rng.bounded((uint32_t)nc));
         } else {
             S.box = pb.get((int)S.rng.bounded((uint32_t)(npb < 4 ? npb : 4)));
-            const DevBox nb = P.boxes[S.box];
-            double dx = 0.5 * (nb.xmin + nb.xmax) - S.x, dy = 0.5 * (nb.ymin + nb.ymax) - S.y;
+            double ccx, ccy;
+            if constexpr (POLY) {
+                ccx = P.center[S.box].x;
+                ccy = P.center[S.box].y;
+            } else {
+                const DevBox nb = P.boxes[S.box];
+                ccx = 0.5 * (nb.xmin + nb.xmax);
+                ccy = 0.5 * (nb.ymin + nb.ymax);
+            }
+            double dx = ccx - S.x, dy = ccy - S.y;
             double dn = sqrt(dx * dx + dy * dy);
             S.x += 1.0e-6 * dx / dn;
             S.y += 1.0e-6 * dy / dn;
@@ -482,7 +522,7 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, Rta
 // One lane follows one particle history at a time and takes the next id
 // from a per-wave pool when it ends; ids of all runs form one range, so a
 // batch of runs of any sizes fills the GPU in one launch.
-template <bool GRAD, int TMK>
+template <bool GRAD, int TMK, bool POLY = false>
 __global__ __launch_bounds__(TMK ? 768 : 512) void rta_history_kernel(RtaParams P) {
     float* lt = nullptr;
     if constexpr (TMK == 1) {
@@ -529,9 +569,9 @@ __global__ __launch_bounds__(TMK ? 768 : 512) void rta_history_kernel(RtaParams 
         }
         if (!S.alive) {
             if (pid >= P.total) break;
-            rta_start<GRAD>(P, S, pid);
+            rta_start<GRAD, POLY>(P, S, pid);
         }
-        rta_event<GRAD, TMK>(P, lt, A, S, C);
+        rta_event<GRAD, TMK, POLY>(P, lt, A, S, C);
     }
     if constexpr (TMK == 1) {
         __syncthreads();  // every history of the workgroup has ended
@@ -616,7 +656,11 @@ struct RtaBatch {
     std::vector<uint64_t> cell0;   // first tally cell of each run
     uint64_t cells = 0, total = 0;
     bool any_grad = false;
+    bool poly = false;  // some run's geometry has polygon boxes: POLY kernels
     std::vector<RtaRunDev> runs;
+    DevBuf<DevSide> d_sides;
+    DevBuf<int2> d_srange;
+    DevBuf<double2> d_center;
     DevBuf<double4> d_modes;
     DevBuf<double2> d_ivel;
     DevBuf<int32_t> d_mirror, d_perm;
@@ -670,14 +714,15 @@ static void upload_alias(DevBuf<uint2>& buf, const std::vector<uint32_t>& th,
 
 // Lambert tables (RTA_MC2D.cpp:556-626): for mode i the partners f are the
 // modes with nonzero omega within nsigma * hypot(sigma_i, sigma_f) in omega,
-// weighted for wall side s by max(-n_s.v_f, 0) times the normal pdf of
+// weighted for each outward side normal n of the geometry (the four sides of
+// rectangles; table k of a mode for normal k) by max(-n.v_f, 0) times the normal pdf of
 // omega_i - omega_f with width hypot(sigma_i, sigma_f). sigma_i is the mode
 // table's per-mode broadening (base_sigma of a q-grid) raised to the floor
 // exp(p25 - 1.5 (p75 - p25)) of log sigma (RTA_MC2D.cpp:566-576), or the
 // constant sigma for every mode when the table has none. The partners of i
 // lie in one contiguous window of the omega-sorted order (the window of the
 // widest pair); pairs outside their own cut get weight 0.
-static void build_lambert(const Modes& m, double sigma, std::vector<int4>& info,
+static void build_lambert(const Modes& m, double sigma, const std::vector<double2>& normals, std::vector<int4>& info,
                           std::vector<uint2>& tab, std::vector<int32_t>& perm) {
     const double nsig = 2.828427124746190097603377448419396157138;
     const double rt2pi = 2.506628274631000502415765284811045253;
@@ -717,9 +762,8 @@ static void build_lambert(const Modes& m, double sigma, std::vector<int4>& info,
         pmf.resize(len);
         th.resize(len);
         al.resize(len);
-        for (int s = 0; s < 4; s++) {
-            double nx, ny;
-            side_normal(s, nx, ny);
+        for (const double2& nrm : normals) {
+            const double nx = nrm.x, ny = nrm.y;
             for (int k = 0; k < len; k++) {
                 int f = perm[lo + k];
                 const double sig = std::hypot(bs[i], bs[f]);
@@ -768,6 +812,9 @@ static RtaParams batch_params(RtaBatch* b, const RtaRunDev* runs, uint64_t total
     P.cts = b->d_cts.p;
     P.cseg = b->d_cseg.p;
     P.ptrans = b->d_ptrans.p;
+    P.sides = b->d_sides.p;
+    P.srange = b->d_srange.p;
+    P.center = b->d_center.p;
     P.gens = b->d_gens.p;
     P.gen_sel = b->d_gen_sel.p;
     P.gen_tab = b->d_gen_tab.p;
@@ -811,15 +858,22 @@ static void launch_batch(RtaBatch* b, const RtaParams& P, int block, int blocks,
         hipLaunchKernelGGL(rta_clear_kernel, dim3(1024), dim3(256), 0, s, b->d_spec.p,
                            (uint64_t)b->nloc * (b->ntime - 1) * b->m->n, b->d_counters.p);
     if (b->tally == 2) {
-        hipLaunchKernelGGL((rta_history_kernel<false, 2>), dim3(blocks), dim3(block), b->lg_bytes, s, P);
+        if (b->poly)
+            hipLaunchKernelGGL((rta_history_kernel<false, 2, true>), dim3(blocks), dim3(block), b->lg_bytes, s, P);
+        else
+            hipLaunchKernelGGL((rta_history_kernel<false, 2>), dim3(blocks), dim3(block), b->lg_bytes, s, P);
         const uint32_t nf = 3u * b->lrows * b->lbins;
         hipLaunchKernelGGL(rta_sum_parts_kernel, dim3(std::min<uint32_t>((nf + 255) / 256, 4096u)), dim3(256), 0, s,
                            b->d_tally.p, b->d_part.p, blocks, b->lrows, b->lbins, b->d_lrow_cell.p, b->cells);
     } else if (b->tally == 1) {
         if (b->any_grad)
             hipLaunchKernelGGL((rta_history_kernel<true, 1>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
+        else if (b->poly)
+            hipLaunchKernelGGL((rta_history_kernel<false, 1, true>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
         else
             hipLaunchKernelGGL((rta_history_kernel<false, 1>), dim3(blocks), dim3(block), b->lt_bytes, s, P);
+    } else if (b->poly) {
+        hipLaunchKernelGGL((rta_history_kernel<false, 0, true>), dim3(blocks), dim3(block), 0, s, P);
     } else if (b->any_grad) {
         hipLaunchKernelGGL((rta_history_kernel<true, 0>), dim3(blocks), dim3(block), 0, s, P);
     } else {
@@ -867,6 +921,20 @@ static void batch_create(RtaBatch* b) {
     std::vector<uint2> gsel, gtab, scat;
     std::vector<double> pmf(n);
     std::vector<uint32_t> th(n), al(n);
+    // polygon geometries: the sides of every run's boxes, their ranges and
+    // centres, and the distinct outward normals of the batch (one Lambert
+    // table per normal and mode); rectangles use the four axis normals
+    std::vector<DevSide> sides;
+    std::vector<int2> srange;
+    std::vector<double2> center, normals;
+    b->poly = false;
+    for (const Geom* g : b->geoms) b->poly = b->poly || (g && g->poly);
+    if (!b->poly)
+        for (int k = 0; k < 4; k++) {
+            double nx, ny;
+            side_normal(k, nx, ny);
+            normals.push_back(make_double2(nx, ny));
+        }
     b->nruns = (int)b->desc.size();
     b->total = 0;
     b->cells = 0;
@@ -892,6 +960,8 @@ static void batch_create(RtaBatch* b) {
                 throw Error(BTE_EINVAL, "specularity must lie in [0, 1]");
             R.spec[k] = (float)D.specularity[k];
             if (D.grad && D.specularity[k] > 0. && D.specularity[k] < 1.) R.gmask |= 1 << k;
+            if (b->poly && (D.specularity[k] > 0. || D.grad))
+                throw Error(BTE_EINVAL, "specular sides and dT/dp need rectangular boxes (diffuse sides only on polygons)");
         }
         if (R.gmask) b->any_grad = true;
         const int bo = (int)boxes.size(), co = (int)cts.size();
@@ -909,6 +979,24 @@ static void batch_create(RtaBatch* b) {
             for (int k = 0; k < 4; k++)
                 if (d.side_nbr[k] >= 0) d.side_nbr[k] += bo;
             boxes.push_back(d);
+            if (b->poly) {
+                const int2 sr = g->srange[i];
+                srange.push_back(make_int2((int)sides.size(), sr.y));
+                for (int k = 0; k < sr.y; k++) {
+                    DevSide q = g->sides[sr.x + k];
+                    if (q.nbr >= 0) q.nbr += bo;
+                    int id = -1;
+                    for (size_t u = 0; u < normals.size() && id < 0; u++)
+                        if (std::hypot(normals[u].x - q.nx, normals[u].y - q.ny) < 1.0e-9) id = (int)u;
+                    if (id < 0) {
+                        id = (int)normals.size();
+                        normals.push_back(make_double2(q.nx, q.ny));
+                    }
+                    q.nid = id;
+                    sides.push_back(q);
+                }
+                center.push_back(make_double2(g->boxes[i].cx(), g->boxes[i].cy()));
+            }
             box_cell.push_back(b->cells + (uint64_t)i * nt);
             // only regular boxes tally (reservoirs absorb, periodic boxes translate)
             box_lrow.push_back(d.kind == 0 ? b->lrows++ : ~0u);
@@ -958,6 +1046,11 @@ static void batch_create(RtaBatch* b) {
         b->total += D.nparticles;
     }
     b->d_boxes.upload(boxes, s);
+    if (b->poly) {
+        b->d_sides.upload(sides, s);
+        b->d_srange.upload(srange, s);
+        b->d_center.upload(center, s);
+    }
     b->d_cts.upload(cts.empty() ? std::vector<DevContact>(1) : cts, s);
     b->d_cseg.upload(cseg.empty() ? std::vector<double4>(1) : cseg, s);
     b->d_ptrans.upload(ptrans, s);
@@ -986,7 +1079,7 @@ static void batch_create(RtaBatch* b) {
     std::vector<int4> info;
     std::vector<uint2> ltab;
     std::vector<int32_t> perm;
-    build_lambert(*m, b->sigma > 0 ? b->sigma : 0.25, info, ltab, perm);
+    build_lambert(*m, b->sigma > 0 ? b->sigma : 0.25, normals, info, ltab, perm);
     b->d_lam_info.upload(info, s);
     b->d_lam_tab.upload(ltab, s);
     b->d_perm.upload(perm, s);
@@ -1121,7 +1214,8 @@ static void batch_set_tally(RtaBatch* b, int mode) {
             throw Error(BTE_EINVAL, "FP64 LDS tallies need " + std::to_string(3 * sizeof(double) * b->lrows) +
                                         " bytes of LDS for one time bin, the device has " +
                                         std::to_string(b->ctx->lds_per_block) + " per workgroup");
-        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&rta_history_kernel<false, 2>),
+        HIPCHECK(hipFuncSetAttribute(b->poly ? reinterpret_cast<const void*>(&rta_history_kernel<false, 2, true>)
+                                             : reinterpret_cast<const void*>(&rta_history_kernel<false, 2>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->ctx->lds_per_block));
         b->block = 768;
         b->blocks = b->ctx->cu_count;
@@ -1133,6 +1227,7 @@ static void batch_set_tally(RtaBatch* b, int mode) {
             throw Error(BTE_EINVAL, "FP32 LDS tallies need " + std::to_string(b->lt_bytes) + " bytes of LDS, the device has " +
                                         std::to_string(b->ctx->lds_per_block) + " per workgroup");
         const void* k = b->any_grad ? reinterpret_cast<const void*>(&rta_history_kernel<true, 1>)
+                        : b->poly   ? reinterpret_cast<const void*>(&rta_history_kernel<false, 1, true>)
                                     : reinterpret_cast<const void*>(&rta_history_kernel<false, 1>);
         HIPCHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->ctx->lds_per_block));
         b->block = 768;
@@ -1255,6 +1350,7 @@ static void rta_execute_noalloc(RtaBatch* b, double* d_T, double* d_jx, double* 
     if (b->nruns != 1) throw Error(BTE_EINVAL, "bte_rta_execute_noalloc takes a single-run handle");
     if (b->nloc) throw Error(BTE_EINVAL, "bte_rta_execute_noalloc does not tally spectral locations");
     if (b->any_grad) throw Error(BTE_EINVAL, "bte_rta_execute_noalloc does not tally the gradient sides");
+    if (b->poly) throw Error(BTE_EINVAL, "bte_rta_execute_noalloc takes rectangular boxes only");
     if (!d_T || !d_jx || !d_jy || !d_counts) throw Error(BTE_EINVAL, "null argument");
     HIPCHECK(hipSetDevice(c->device));
     double* base = (double*)scratch_address(c->device);
@@ -1799,6 +1895,7 @@ extern "C" int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* mm, const bte_g
     auto* g = reinterpret_cast<const Geom*>(gg);
     if (!c || !m || !g || !cfg || !T_out || !jx_out || !jy_out) throw Error(BTE_EINVAL, "null argument");
     if (cfg->nparticles == 0 || !(cfg->thickness > 0)) throw Error(BTE_EINVAL, "bad steady configuration");
+    if (g->poly) throw Error(BTE_EINVAL, "the steady branch takes rectangular boxes only");
     HIPCHECK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int n = m->n, nbox = (int)g->boxes.size();
@@ -1908,7 +2005,7 @@ extern "C" int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* mm, const bte_g
     std::vector<int4> info;
     std::vector<uint2> ltab;
     std::vector<int32_t> perm;
-    build_lambert(*m, cfg->sigma_bounce > 0 ? cfg->sigma_bounce : 0.25, info, ltab, perm);
+    build_lambert(*m, cfg->sigma_bounce > 0 ? cfg->sigma_bounce : 0.25, g->normals, info, ltab, perm);
     DevBuf<int4> d_lam_info;
     d_lam_info.upload(info, s);
     d_lam_tab.upload(ltab, s);

@@ -49,7 +49,6 @@ def test_rtamc2d_rejects_hdf5_material(pkg, tmp_path):
 
 @pytest.mark.parametrize("extra,msg", [
     ('<Translate_to id="0"> 1.0 </Translate_to>', "bad <Translate_to>"),
-    ('<theta angle="0.3"/>', "rotated"),
 ])
 def test_geometry_features_rejected(pkg, tmp_path, extra, msg):
     (tmp_path / "g.xml").write_text(geometry_xml([((0, 10, 0, 5), extra)]))
@@ -61,16 +60,17 @@ def test_geometry_features_rejected(pkg, tmp_path, extra, msg):
     assert r.returncode == 1 and msg in r.stderr
 
 
-def test_non_rectangle_rejected(pkg, tmp_path):
-    (tmp_path / "g.xml").write_text(
-        '<Geometry><Box><BoxID id="0"/><Vertices dim="2" npoints="3"> 0 0 1 0 0 1 </Vertices>'
-        '<initCnd Teq="300"/></Box></Geometry>')
+def test_theta_is_read_and_not_applied(pkg, tmp_path):
+    """<theta angle> sets geometry_2d::rotmat, which neither solver reads
+    (geometry_2d.cpp:496-509): the reader accepts it and says so"""
+    (tmp_path / "g.xml").write_text(geometry_xml([((0, 10, 0, 5), '<theta angle="0.3"/>')]))
     (tmp_path / "in.xml").write_text(
-        '<beRTAMC2D><geometry file="g.xml"/><time dt="1" maxtime="10"/>'
-        '<material name="m" modetable="synthetic" nmodes="20" thickness="1"><propagator T="300" file="p.bin"/>'
-        '</material><Eeff particles="100"/></beRTAMC2D>')
-    r = run([app(pkg, "beRTAMC2D"), "in.xml", "2"], tmp_path)
-    assert r.returncode == 1 and "rectangles" in r.stderr
+        '<RTA_MC2d><geometry file="g.xml"/><time dt="0.25" maxtime="10"/>'
+        '<material name="m" modetable="synthetic" nmodes="50" thickness="1" T0="300"/>'
+        '<particles N="10"/></RTA_MC2d>')
+    r = run([app(pkg, "RTAMC2D"), "in.xml", "4"], tmp_path)
+    assert "theta 0.3 is read and, as in the reference, not applied" in r.stderr
+    assert "rotated" not in r.stderr
 
 
 def test_xml_errors_name_the_line(pkg, tmp_path):

@@ -343,3 +343,35 @@ def test_bertamc2d_reads_init_distribution(pkg, ctx, tmp_path):
     assert "init_distribution" in r.stdout
     first = [l for l in r.stdout.splitlines() if l and not l.startswith("#")][0].split("\t")
     assert int(first[2]) > 100   # particles at step 0
+
+
+def test_rtamc2d_runs_polygon_geometries(pkg, ctx, tmp_path):
+    """A geometry file with quadrilateral boxes (slanted shared sides and a
+    zigzag top wall) between two rectangular reservoirs: RTAMC2D writes what
+    the library's polygon geometry gives on the same seed."""
+    from test_gpu_polygons import zigzag_ribbon
+    polys = zigzag_ribbon(6, 60.0, 30.0, 3.0, 4.0, reservoirs=True)
+    kind = np.array([0] * 6 + [1, 1], np.int32)
+    teq = np.array([300.0] * 6 + [300.5, 299.5])
+    boxes = []
+    for i, p in enumerate(polys):
+        pts = " ".join(f"{float(x)!r} {float(y)!r}" for x, y in p)
+        res = "\n    <Reservoir/>" if kind[i] == 1 else ""
+        boxes.append(f'  <Box>\n    <BoxID id="{i}"/>\n    <MaterialID name="m"/>\n'
+                     f'    <Vertices dim="2" npoints="{len(p)}"> {pts} </Vertices>\n'
+                     f'    <initCnd Teq="{float(teq[i])!r}"/>{res}\n  </Box>')
+    (tmp_path / "g.xml").write_text(f'<Geometry>\n  <number_of_boxes Ngeom="{len(polys)}"/>\n' +
+                                    "\n".join(boxes) + "\n</Geometry>\n")
+    (tmp_path / "in.xml").write_text(
+        '<RTA_MC2d>\n<geometry file="g.xml"/>\n<time dt="20" maxtime="200"/>\n'
+        '<material name="m" modetable="synthetic" nmodes="300" seed="4" thickness="1" T0="300" sigma_bounce="1.0"/>\n'
+        '<particles N="20000" seed="5"/>\n</RTA_MC2d>\n')
+    run([pkg.app_path("RTAMC2D"), "in.xml", "8", "1"], tmp_path)
+    T = np.loadtxt(tmp_path / "temperature_300K_run_0.csv", delimiter=",")
+    mat = pkg.workloads.material(300, seed=4)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    geom = pkg.Geometry(ctx, None, kind, teq, polys=polys)
+    o = pkg.rtamc2d(ctx, modes, geom, thickness=1.0, T0=300.0, dt=20.0, maxtime=200.0, nparticles=20000, seed=5,
+                    sigma_bounce=1.0)
+    assert np.allclose(T[:, 1:], o["T"], rtol=1e-5, atol=1e-9)
+    assert np.abs(o["T"][:, :6] - 300.0).max() > 1e-3
