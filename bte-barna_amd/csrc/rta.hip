@@ -1659,6 +1659,9 @@ struct RtaSteadyParams {
     const DevContact* cts;
     const double4* cseg;
     const double2* ptrans;
+    const DevSide* sides;     // polygon geometries: sides, side range and centre per box
+    const int2* srange;
+    const double2* center;
     const int4* lam_info;
     const uint2* lam_tab;
     const int32_t* perm;
@@ -1692,8 +1695,36 @@ __device__ inline void steady_append(const RtaSteadyParams& P, bool want, const 
     if (want) P.pout[base + __popcll(m & ((1ULL << lane) - 1ULL))] = q;
 }
 
+// get_random_point of box b (geometry_2d.hpp:332-348): x, then y in the
+// bounding box, until covered by the box (polygons; a rectangle takes the
+// first draw). A box that a few thousand draws miss is counted as stuck and
+// gets its centre.
+template <bool POLY>
+__device__ inline void steady_box_point(const RtaSteadyParams& P, int b, Pcg32& rng, double& x, double& y,
+                                        unsigned long long& stuck) {
+    const DevBox B = P.boxes[b];
+    x = B.xmin + rng.uniform() * (B.xmax - B.xmin);
+    y = B.ymin + rng.uniform() * (B.ymax - B.ymin);
+    if constexpr (POLY) {
+        const int2 sr = P.srange[b];
+        int tries = 0;
+        while (!poly_inside(P.sides, sr, x, y)) {
+            if (++tries > 4096) {
+                stuck++;
+                x = P.center[b].x;
+                y = P.center[b].y;
+                return;
+            }
+            x = B.xmin + rng.uniform() * (B.xmax - B.xmin);
+            y = B.ymin + rng.uniform() * (B.ymax - B.ymin);
+        }
+    }
+}
+
 // initial particles (Init_generator, RTA_MC2D.cpp:436-533)
+template <bool POLY>
 __global__ __launch_bounds__(256) void rta_steady_init_kernel(RtaSteadyParams P, uint64_t np) {
+    unsigned long long stuck = 0;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (uint64_t)gridDim.x * blockDim.x) {
         Pcg32 rng;
         rng.seed(P.seed, i);
@@ -1706,8 +1737,10 @@ __global__ __launch_bounds__(256) void rta_steady_init_kernel(RtaSteadyParams P,
             const double u = rng.uniform();
             double x = g.p1x + u * g.ndx, y = g.p1y + u * g.ndy;
             const DevBox bi = P.boxes[g.idi];
-            if (!box_inside(bi, x, y)) {
-                double dx = 0.5 * (bi.xmin + bi.xmax) - x, dy = 0.5 * (bi.ymin + bi.ymax) - y;
+            if (!(POLY ? poly_inside(P.sides, P.srange[g.idi], x, y) : box_inside(bi, x, y))) {
+                const double ccx = POLY ? P.center[g.idi].x : 0.5 * (bi.xmin + bi.xmax);
+                const double ccy = POLY ? P.center[g.idi].y : 0.5 * (bi.ymin + bi.ymax);
+                double dx = ccx - x, dy = ccy - y;
                 double dn = sqrt(dx * dx + dy * dy);
                 x += 1.0e-6 * dx / dn;
                 y += 1.0e-6 * dy / dn;
@@ -1719,9 +1752,7 @@ __global__ __launch_bounds__(256) void rta_steady_init_kernel(RtaSteadyParams P,
         } else {
             const int k = (int)alias_pick(P.gbox_sel, (uint32_t)P.ngbox, rng, ng);
             const int b = P.gbox[k];
-            const DevBox B = P.boxes[b];
-            q.x = B.xmin + rng.uniform() * (B.xmax - B.xmin);
-            q.y = B.ymin + rng.uniform() * (B.ymax - B.ymin);
+            steady_box_point<POLY>(P, b, rng, q.x, q.y, stuck);
             // signs alternate with the id; a class without modes gives way
             const int cls = P.gclass[k];
             int neg = (i & 1) ? 0 : 1;
@@ -1732,9 +1763,11 @@ __global__ __launch_bounds__(256) void rta_steady_init_kernel(RtaSteadyParams P,
         }
         P.pout[i] = q;
     }
+    if (POLY) block_add_u64(P.counters + 2, stuck);
 }
 
 // one flight per particle (steady evolparticle, RTA_MC2D.cpp:1406-1799)
+template <bool POLY>
 __global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams P) {
     extern __shared__ __align__(16) unsigned char sdyn[];
     double* sacc = (double*)sdyn;  // [4][nbox]: gz, jx, jy, phi
@@ -1769,7 +1802,9 @@ __global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams 
                 if (++it > P.max_iters || still > 64) { stuck++; break; }
                 const DevBox B = P.boxes[box];
                 if (B.kind == 2) {
-                    const int nb = translate_rect(P.boxes, P.cts, P.cseg, P.ptrans, box, x, y, md.x, md.y, rng);
+                    const int nb = POLY ? translate_poly(P.boxes, P.sides, P.srange, P.cts, P.cseg, P.ptrans, box, x,
+                                                         y, md.x, md.y, rng)
+                                        : translate_rect(P.boxes, P.cts, P.cseg, P.ptrans, box, x, y, md.x, md.y, rng);
                     if (nb < 0) { badtrans++; break; }
                     box = nb;
                     continue;
@@ -1777,7 +1812,9 @@ __global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams 
                 if (B.kind == 1) break;  // absorbed
                 const double vx = md.x, vy = md.y;
                 const double nx = x + dt * vx, ny = y + dt * vy;
-                if (box_inside(B, nx, ny)) {
+                int2 sr = make_int2(0, 0);
+                if constexpr (POLY) sr = P.srange[box];
+                if (POLY ? poly_inside(P.sides, sr, nx, ny) : box_inside(B, nx, ny)) {
                     atomicAdd(tgz + box, sign * dt);
                     atomicAdd(tjx + box, vx * sign * dt);
                     atomicAdd(tjy + box, vy * sign * dt);
@@ -1786,8 +1823,13 @@ __global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams 
                 }
                 int mask;
                 double ex, ey;
-                const double2 iv = P.ivel[mode];
-                const double tf = box_exit(B, x, y, vx, vy, iv.x, iv.y, mask, ex, ey);
+                double tf;
+                if constexpr (POLY) {
+                    tf = poly_exit(P.sides, sr, x, y, vx, vy, mask, ex, ey);
+                } else {
+                    const double2 iv = P.ivel[mode];
+                    tf = box_exit(B, x, y, vx, vy, iv.x, iv.y, mask, ex, ey);
+                }
                 double left = dt - tf;
                 if (almost_equal(left, 0.)) left = 0.;
                 still = tf == 0. ? still + 1 : 0;
@@ -1795,23 +1837,29 @@ __global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams 
                 atomicAdd(tjx + box, vx * sign * tf);
                 atomicAdd(tjy + box, vy * sign * tf);
                 int side = 0;
-                const int fast = side_fast(B, mask, ex, ey, vx, vy, side);
+                const int fast = POLY ? poly_side_fast(P.sides, sr, mask, ex, ey, vx, vy, side)
+                                      : side_fast(B, mask, ex, ey, vx, vy, side);
                 Ids4 pb{};
                 int npb;
                 if (fast >= 0) {
                     pb.a = fast;
                     npb = 1;
+                } else if (fast == SIDE_VOID) {
+                    npb = 0;
                 } else {
-                    npb = fast == SIDE_VOID ? 0 : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
+                    npb = POLY ? poly_find_pboxes(P.sides, P.srange, P.cts, B, ex, ey, vx, vy, pb)
+                               : find_pboxes(P.boxes, P.cts, B, ex, ey, vx, vy, pb);
                 }
                 if (npb == 0) {
                     // diffuse wall: Lambert bounce, carried to the next generation
                     const int s = fast == SIDE_VOID ? side
                                 : (__popc(mask) == 1 ? __ffs(mask) - 1
-                                                     : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next()));
+                                   : (POLY ? poly_pick_void_side(P.sides, P.srange, P.cts, B, sr, mask, ex, ey, rng.next())
+                                           : pick_void_side(P.boxes, P.cts, B, mask, ex, ey, rng.next())));
                     const int4 li = P.lam_info[mode];
+                    const int tabk = POLY ? P.sides[sr.x + s].nid : s;
                     bool ng;
-                    const int k = (int)alias_pick(P.lam_tab + li.z + (size_t)s * li.y, (uint32_t)li.y, rng, ng);
+                    const int k = (int)alias_pick(P.lam_tab + li.z + (size_t)tabk * li.y, (uint32_t)li.y, rng, ng);
                     mode = P.perm[li.x + k];
                     x = ex;
                     y = ey;
@@ -1826,15 +1874,20 @@ __global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams 
                     Ids4 cand{};
                     int nc = 0;
                     const double cx = ex + 1.0e-6 * vx, cy = ey + 1.0e-6 * vy;
-                    for (int k = 0; k < npb && k < 4; k++)
-                        if (box_inside(P.boxes[pb.get(k)], cx, cy)) cand.set(nc++, pb.get(k));
+                    for (int k = 0; k < npb && k < 4; k++) {
+                        const bool in = POLY ? poly_inside(P.sides, P.srange[pb.get(k)], cx, cy)
+                                             : box_inside(P.boxes[pb.get(k)], cx, cy);
+                        if (in) cand.set(nc++, pb.get(k));
+                    }
                     if (nc > 0) {
                         box = cand.get((int)rng.bounded((uint32_t)nc));
                     } else {
                         box = pb.get((int)rng.bounded((uint32_t)(npb < 4 ? npb : 4)));
                         const DevBox nb = P.boxes[box];
                         if (nb.kind != 2) {
-                            double dx = 0.5 * (nb.xmin + nb.xmax) - x, dy = 0.5 * (nb.ymin + nb.ymax) - y;
+                            const double ccx = POLY ? P.center[box].x : 0.5 * (nb.xmin + nb.xmax);
+                            const double ccy = POLY ? P.center[box].y : 0.5 * (nb.ymin + nb.ymax);
+                            double dx = ccx - x, dy = ccy - y;
                             double dn = sqrt(dx * dx + dy * dy);
                             x += 1.0e-6 * dx / dn;
                             y += 1.0e-6 * dy / dn;
@@ -1863,7 +1916,9 @@ __global__ __launch_bounds__(256) void rta_steady_flight_kernel(RtaSteadyParams 
 }
 
 // re-emission of |phi(box)| particles per box (RTA_MC2D.cpp:2637-2654)
+template <bool POLY>
 __global__ __launch_bounds__(256) void rta_steady_regen_kernel(RtaSteadyParams P, uint64_t out0, uint64_t total) {
+    unsigned long long stuck = 0;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (uint64_t)gridDim.x * blockDim.x) {
         int lo = 0, hi = P.nbox - 1;  // box b holds [regen[b], regen[b+1])
         while (lo < hi) {
@@ -1873,16 +1928,15 @@ __global__ __launch_bounds__(256) void rta_steady_regen_kernel(RtaSteadyParams P
         const int b = lo;
         Pcg32 rng;
         rng.seed(P.key ^ 0x5eed5eed5eed5eedULL, k);
-        const DevBox B = P.boxes[b];
         RtaSPart q;
-        q.x = B.xmin + rng.uniform() * (B.xmax - B.xmin);
-        q.y = B.ymin + rng.uniform() * (B.ymax - B.ymin);
+        steady_box_point<POLY>(P, b, rng, q.x, q.y, stuck);
         bool ng;
         q.mode = alias_pick(P.scat, (uint32_t)P.n, rng, ng);
         q.boxsign = ((uint32_t)b << 1) | (P.phi[b] < 0. ? 1u : 0u);
         q.id = out0 + k;
         P.pout[out0 + k] = q;
     }
+    if (POLY) block_add_u64(P.counters + 2, stuck);
 }
 
 }  // namespace bte
@@ -1895,7 +1949,6 @@ extern "C" int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* mm, const bte_g
     auto* g = reinterpret_cast<const Geom*>(gg);
     if (!c || !m || !g || !cfg || !T_out || !jx_out || !jy_out) throw Error(BTE_EINVAL, "null argument");
     if (cfg->nparticles == 0 || !(cfg->thickness > 0)) throw Error(BTE_EINVAL, "bad steady configuration");
-    if (g->poly) throw Error(BTE_EINVAL, "the steady branch takes rectangular boxes only");
     HIPCHECK(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     const int n = m->n, nbox = (int)g->boxes.size();
@@ -2029,6 +2082,13 @@ extern "C" int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* mm, const bte_g
     P.cts = g->d_contacts.p;
     P.cseg = g->d_cseg.p;
     P.ptrans = g->d_ptrans.p;
+    P.sides = g->d_sides.p;
+    P.srange = g->d_srange.p;
+    std::vector<double2> centre(nbox);
+    for (int i = 0; i < nbox; i++) centre[i] = make_double2(g->boxes[i].cx(), g->boxes[i].cy());
+    DevBuf<double2> d_centre;
+    d_centre.upload(centre, s);
+    P.center = d_centre.p;
     P.lam_info = d_lam_info.p;
     P.lam_tab = d_lam_tab.p;
     P.perm = d_perm.p;
@@ -2056,7 +2116,8 @@ extern "C" int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* mm, const bte_g
     RtaSPart* buf[2] = {d_parts.p, d_parts.p + cap};
     int cur = 0;
     P.pout = buf[cur];
-    hipLaunchKernelGGL(rta_steady_init_kernel, dim3(blocks), dim3(256), 0, s, P, cap);
+    hipLaunchKernelGGL(g->poly ? rta_steady_init_kernel<true> : rta_steady_init_kernel<false>, dim3(blocks), dim3(256),
+                       0, s, P, cap);
     HIPCHECK(hipGetLastError());
     uint64_t nin = cap;
     std::vector<double> acc(3 * (size_t)nbox, 0.), old(3 * (size_t)nbox), phi(nbox);
@@ -2082,7 +2143,8 @@ extern "C" int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* mm, const bte_g
         HIPCHECK(hipMemsetAsync(d_phi.p, 0, nbox * sizeof(double), s));
         HIPCHECK(hipMemsetAsync(d_cnt.p, 0, 4 * sizeof(unsigned long long), s));
         HIPCHECK(hipEventRecord(e0, s));
-        hipLaunchKernelGGL(rta_steady_flight_kernel, dim3(blocks), dim3(256), lds, s, P);
+        hipLaunchKernelGGL(g->poly ? rta_steady_flight_kernel<true> : rta_steady_flight_kernel<false>, dim3(blocks),
+                           dim3(256), lds, s, P);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(e1, s));
         unsigned long long cnt[4];
@@ -2107,7 +2169,8 @@ extern "C" int bte_rta_steady_run(bte_ctx* ctx, const bte_modes* mm, const bte_g
         if (nb + nre > cap) throw Error(BTE_ECAPACITY, "steady generation larger than the first");
         if (nre) {
             d_regen.upload(pre.data(), nbox + 1, s);
-            hipLaunchKernelGGL(rta_steady_regen_kernel, dim3(blocks), dim3(256), 0, s, P, nb, nre);
+            hipLaunchKernelGGL(g->poly ? rta_steady_regen_kernel<true> : rta_steady_regen_kernel<false>, dim3(blocks),
+                               dim3(256), 0, s, P, nb, nre);
             HIPCHECK(hipGetLastError());
         }
         // convergence (RTA_MC2D.cpp:2660-2692)

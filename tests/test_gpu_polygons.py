@@ -155,3 +155,23 @@ def test_bert_polygon_reservoirs_and_gradient_sources_match_oracle(pkg, ctx):
           for q in (0, 1, 2)]
     zs.append(zstats(np.array([float(x["npart"][-1]) for x in g]), np.array([float(x["npart"][-1]) for x in o])))
     print(assert_3sigma(zs, "polygon reservoirs and gradient sources vs oracle"))
+
+
+def test_rta_steady_on_zigzag_polygons_matches_oracle(pkg, ctx):
+    """steady branch with a gradient (RTA_MC2D.cpp:1406-1799, 2544-2842):
+    gradient generators and re-emission at uniform points of polygon boxes,
+    flights through slanted sides, diffuse zigzag walls"""
+    mat = pkg.workloads.material(300, seed=5)
+    polys = zigzag_ribbon(8, 80.0, 30.0, 3.0, 4.0, reservoirs=True)
+    kind = np.array([0] * 8 + [1, 1], np.int32)
+    teq = np.array([300.0] * 8 + [300.5, 299.5])
+    modes = _modes(pkg, ctx, mat)
+    geom = pkg.Geometry(ctx, None, kind, teq, polys=polys)
+    kw = dict(thickness=1.0, T0=300.0, gradient=(-2.0e-3, 0.0), nparticles=20000, max_generations=40)
+    R = 10
+    rect = orc.poly_rect(polys)
+    g = [pkg.rta_steady(ctx, modes, geom, seed=100 + r, sigma_bounce=2.0, **kw) for r in range(R)]
+    o = [orc.rta_steady_run(mat, rect, kind, teq, seed=200 + r, sigma=2.0, polys=polys, **kw) for r in range(R)]
+    zs = [zstats(np.array([x[key][:8] for x in g]), np.array([x[key][:8] for x in o])) for key in ("T", "jx", "jy")]
+    zs.append(zstats(np.array([x["jx"][:8].mean() for x in g]), np.array([x["jx"][:8].mean() for x in o])))
+    print(assert_3sigma(zs, "steady zigzag polygons vs oracle"))
