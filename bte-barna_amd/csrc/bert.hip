@@ -1418,6 +1418,12 @@ static BertParams make_params(Bert* b) {
         }
     }
     b->lds_bytes = off;
+    // the particle kernels' dynamic LDS and their static arrays (under 1 KB)
+    if (off + 1024 > b->ctx->lds_per_block)
+        throw Error(BTE_EINVAL, "the particle kernels need " + std::to_string(off + 1024) + " bytes of LDS for " +
+                                    std::to_string(P.nbox) + " boxes, " + std::to_string(P.nsd) + " sides and " +
+                                    std::to_string(P.nsegs) + " buffer segments; the device has " +
+                                    std::to_string(b->ctx->lds_per_block) + " per workgroup");
     return P;
 }
 
@@ -1899,6 +1905,7 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     for (auto& e : b->kev) HIPCHECK(hipEventCreate(&e));
     HIPCHECK(hipEventCreate(&b->ev0));
     HIPCHECK(hipEventCreate(&b->ev1));
+    (void)make_params(b);  // refuses a layout that exceeds the LDS before any run
     HIPCHECK(hipStreamSynchronize(s));
     *out = reinterpret_cast<bte_bert*>(guard.release());
     BTE_GUARD_END

@@ -72,15 +72,42 @@ struct Basis {
     }
 };
 
-// c = exp(H_i) e_1 for every column: one wave per column, H in LDS;
-// exp(H) e_1 = (exp(H/2^s))^(2^s) e_1 with each factor a Taylor polynomial
-// of degree <= 24 applied to the vector (||H/2^s||_1 <= 1/2).
+// c = exp(H_i) e_1 for every column: one wave per column, H in LDS.
+// Scaling and squaring (the method of Eigen's MatrixExponential, which the
+// reference's build_P calls): with ||H/2^s||_1 <= 1/2 each factor
+// exp(H/2^s) is a Taylor polynomial of degree <= 24. When 2^s <= m the
+// factor is applied 2^s times to the vector; otherwise E = exp(H/2^s) is
+// formed column by column and squared s times, and c is its first column
+// (cost O(s m^3) instead of O(2^s m^2) for large norms).
+__device__ inline void kr_taylor_apply(const double* Hs, double* v, double* t, int m, int l) {
+    double acc = l < m ? v[l] : 0.;
+    double term = acc;
+    for (int kk = 1; kk <= 24; kk++) {
+        if (l < m) t[l] = term;
+        __syncthreads();
+        double nt = 0;
+        if (l < m)
+            for (int c = 0; c < m; c++) nt += Hs[l + c * m] * t[c];
+        nt /= kk;
+        __syncthreads();
+        term = nt;
+        acc += nt;
+        double mag = fabs(nt);
+        for (int d = 32; d > 0; d >>= 1) mag = fmax(mag, __shfl_xor(mag, d, 64));
+        if (mag < 1e-18) break;
+    }
+    if (l < m) v[l] = acc;
+    __syncthreads();
+}
+
 __global__ __launch_bounds__(64) void kr_expm_e1(const double* __restrict__ H, double* __restrict__ C,
                                                  int m) {
     extern __shared__ double sh[];
     double* Hs = sh;            // m*m
     double* v = sh + m * m;     // m
     double* t = v + m;          // m
+    double* E = t + m;          // m*m (squaring path)
+    double* W = E + m * m;      // m*m (squaring path)
     const int i = blockIdx.x, l = threadIdx.x;
     const double* Hi = H + (size_t)i * m * m;
     for (int q = l; q < m * m; q += 64) Hs[q] = Hi[q];
@@ -90,8 +117,8 @@ __global__ __launch_bounds__(64) void kr_expm_e1(const double* __restrict__ H, d
     if (l < m)
         for (int r = 0; r < m; r++) cs += fabs(Hs[r + l * m]);
     for (int d = 32; d > 0; d >>= 1) cs = fmax(cs, __shfl_xor(cs, d, 64));
-    // a non-finite or huge Hessenberg norm yields NaN instead of an unbounded loop
-    if (!(cs < 8.0e6)) {
+    // a non-finite Hessenberg matrix yields NaN
+    if (!(cs < 1.0e300)) {
         if (l < m) C[(size_t)i * m + l] = __builtin_nan("");
         return;
     }
@@ -99,30 +126,34 @@ __global__ __launch_bounds__(64) void kr_expm_e1(const double* __restrict__ H, d
     if (cs > 0.5) s = (int)ceil(log2(cs / 0.5));
     const double sc = ldexp(1.0, -s);
     for (int q = l; q < m * m; q += 64) Hs[q] *= sc;
-    if (l < m) v[l] = l == 0 ? 1.0 : 0.0;
     __syncthreads();
-    const long reps = 1L << s;
-    for (long rp = 0; rp < reps; rp++) {
-        double acc = l < m ? v[l] : 0.;
-        double term = acc;
-        for (int kk = 1; kk <= 24; kk++) {
-            if (l < m) t[l] = term;
-            __syncthreads();
-            double nt = 0;
-            if (l < m)
-                for (int c = 0; c < m; c++) nt += Hs[l + c * m] * t[c];
-            nt /= kk;
-            __syncthreads();
-            term = nt;
-            acc += nt;
-            double mag = fabs(nt);
-            for (int d = 32; d > 0; d >>= 1) mag = fmax(mag, __shfl_xor(mag, d, 64));
-            if (mag < 1e-18) break;
-        }
-        if (l < m) v[l] = acc;
+    if (s < 31 && (1L << s) <= m) {
+        if (l < m) v[l] = l == 0 ? 1.0 : 0.0;
+        __syncthreads();
+        for (long rp = 0; rp < (1L << s); rp++) kr_taylor_apply(Hs, v, t, m, l);
+        if (l < m) C[(size_t)i * m + l] = v[l];
+        return;
+    }
+    for (int j = 0; j < m; j++) {
+        if (l < m) v[l] = l == j ? 1.0 : 0.0;
+        __syncthreads();
+        kr_taylor_apply(Hs, v, t, m, l);
+        if (l < m) E[l + j * m] = v[l];
+    }
+    __syncthreads();
+    for (int q = 0; q < s; q++) {
+        // thread l forms column l of E E
+        if (l < m)
+            for (int r = 0; r < m; r++) {
+                double a = 0.;
+                for (int c = 0; c < m; c++) a += E[r + c * m] * E[c + l * m];
+                W[r + l * m] = a;
+            }
+        __syncthreads();
+        for (int u = l; u < m * m; u += 64) E[u] = W[u];
         __syncthreads();
     }
-    if (l < m) C[(size_t)i * m + l] = v[l];
+    if (l < m) C[(size_t)i * m + l] = E[l];
 }
 
 // Wide variants: one 1024-thread workgroup per column, element r of the
@@ -672,7 +703,12 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
         }
         HIPCHECK(hipGetLastError());
     }
-    hipLaunchKernelGGL(kr_expm_e1, dim3(N), dim3(64), (size_t)(m * m + 2 * m) * sizeof(double), s, dH.p, dC.p, m);
+    {
+        const size_t lds = (size_t)(3 * m * m + 2 * m) * sizeof(double);
+        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&kr_expm_e1),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kr_expm_e1, dim3(N), dim3(64), lds, s, dH.p, dC.p, m);
+    }
     HIPCHECK(hipGetLastError());
     if (vec) {
         combine_step(s, N, V, dC.p, dP.p, dErr.p, m);

@@ -1217,8 +1217,10 @@ static void batch_set_tally(RtaBatch* b, int mode) {
         HIPCHECK(hipFuncSetAttribute(b->poly ? reinterpret_cast<const void*>(&rta_history_kernel<false, 2, true>)
                                              : reinterpret_cast<const void*>(&rta_history_kernel<false, 2>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->ctx->lds_per_block));
+        // two 768-thread workgroups per CU when two blocks fit its LDS (small
+        // geometries), else one
         b->block = 768;
-        b->blocks = b->ctx->cu_count;
+        b->blocks = b->ctx->cu_count * (2 * b->lg_bytes <= b->ctx->lds_per_block ? 2 : 1);
         const size_t np = (size_t)b->blocks * 3 * b->lrows * b->lbins;
         if (b->d_part.n < np) b->d_part.alloc(np);  // kept: a captured graph holds its address
     } else if (mode == 1) {

@@ -133,6 +133,10 @@ typedef struct {
  * the time column, RTA_MC2D.cpp:2437-2471). */
 int bte_rta_create(bte_ctx* ctx, const bte_modes* m, const bte_geom* g, const bte_rta_cfg* cfg,
                    bte_rta** out);
+/* bte_rta_execute switches the handle to tally mode 2 (FP64 LDS sums of
+ * the leading time bins, one 768-thread workgroup per CU) whenever the
+ * run's cells fit and no spectral location is set, and back to mode 0
+ * (256-thread workgroups, eight per CU) otherwise. */
 int bte_rta_execute(bte_rta* r, double* T_out, double* jx_out, double* jy_out,
                     bte_rta_report* rep);
 /* No counterpart in the reference: bte_rta_execute for callers that cannot
@@ -245,7 +249,7 @@ int bte_rta_batch_create(bte_ctx* ctx, const bte_modes* m, int nruns, const bte_
  * Not with dT/dp (BTE_EINVAL). */
 int bte_rta_batch_set_tally(bte_rta_batch* b, int mode);
 /* Times the batch scaled to `fraction` of each run's particles over a set of
- * launch shapes and keeps the fastest (no-op under tally mode 1). */
+ * launch shapes and keeps the fastest (no-op under tally modes 1 and 2). */
 int bte_rta_batch_tune(bte_rta_batch* b, double fraction, int* block, int* blocks);
 /* T, jx, jy: per run r, [ntime-1][nbox_r] blocks one after another; dTdp (may
  * be NULL): per run [4 sides][ntime-1][nbox_r]; Eeff (may be NULL): [nruns]. */

@@ -266,3 +266,39 @@ def test_single_run_switches_tally_mode_around_spectral_locations(pkg, ctx):
     h.spectral([], 0)
     close(h.execute(), fresh)
     h.close()
+
+
+def test_rta_execute_switches_tally_modes_on_one_handle(pkg, ctx):
+    """bte_rta_execute uses tally mode 2 while the cells fit and no spectral
+    location is set, and mode 0 (256-thread workgroups) with one: a handle
+    executed plain, with a spectral location and plain again gives each time
+    the histories of a fresh handle (equal event and crossing counts) and T,
+    jx, jy equal up to the order of the FP64 sums."""
+    W = pkg.workloads
+    mat = W.material(400, seed=6)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    rect, kind, teq, _ = W.ribbon(200.0, 50.0, 20, 300.0, reservoirs=(300.5, 299.5))
+    geom = pkg.Geometry(ctx, rect, kind, teq)
+    kw = dict(thickness=1.0, T0=300.0, dt=10.0, maxtime=300.0, nparticles=50000, seed=9, sigma_bounce=2.0)
+
+    def fresh(spec):
+        h = pkg.RtaRun(ctx, modes, geom, **kw)
+        if spec:
+            h.spectral([3], 50)
+        o = h.execute()
+        h.close()
+        return o
+
+    h = pkg.RtaRun(ctx, modes, geom, **kw)
+    a = h.execute()
+    h.spectral([3], 50)
+    b = h.execute()
+    h.spectral([], 1)
+    c = h.execute()
+    h.close()
+    for got, ref in ((a, fresh(False)), (b, fresh(True)), (c, fresh(False))):
+        assert (got["events"], got["crossings"]) == (ref["events"], ref["crossings"])
+        for key, off in (("T", 300.0), ("jx", 0.0), ("jy", 0.0)):
+            scale = np.abs(ref[key] - off).max()
+            assert scale > 0
+            np.testing.assert_allclose(got[key] - off, ref[key] - off, rtol=0, atol=1e-10 * scale)

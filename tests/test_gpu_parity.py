@@ -203,3 +203,22 @@ def test_bert_reservoir_sources_match_oracle(pkg, ctx):
           for q in (0, 1)]
     zs.append(zstats(np.array([float(x["npart"][-1]) for x in g]), np.array([float(x["npart"][-1]) for x in o])))
     assert_3sigma(zs, "reservoir-driven beRTAMC2D vs oracle")
+
+
+def test_krylov_exponential_of_a_large_norm_hessenberg(pkg, ctx):
+    """dt = 40 ps gives ||dt H||_1 = 68 on this operator (s = 8 halvings,
+    2^s = 256 > m = 30), so the per-column exponential forms exp(H / 2^s)
+    and squares it s times, while the oracle takes the exponential of its
+    Arnoldi matrices by Pade scaling and squaring. The Krylov build has
+    converged at this step: both agree with each other and with the full
+    matrix exponential."""
+    import scipy.linalg
+    W = pkg.workloads
+    mat, B = W.config1_inputs(nmodes=128, seed=7)
+    Po = orc.build_P(B, 40.0, 30, 1)
+    Pg, rep = pkg.build_P(ctx, B, 40.0, m=30, precision="fp64", arnoldi="textbook")
+    Pe = scipy.linalg.expm(40.0 * B)
+    l1o, l1e = np.abs(Pg - Po).sum(0).max(), np.abs(Pg - Pe).sum(0).max()
+    print(f"dt = 40 ps: max column L1 to the oracle {l1o:.3e}, to expm {l1e:.3e}", flush=True)
+    assert l1o < 1e-10 and l1e < 1e-10
+    assert np.abs(Pg.sum(0) - 1).max() < 1e-12
