@@ -175,3 +175,75 @@ def test_rta_steady_on_zigzag_polygons_matches_oracle(pkg, ctx):
     zs = [zstats(np.array([x[key][:8] for x in g]), np.array([x[key][:8] for x in o])) for key in ("T", "jx", "jy")]
     zs.append(zstats(np.array([x["jx"][:8].mean() for x in g]), np.array([x["jx"][:8].mean() for x in o])))
     print(assert_3sigma(zs, "steady zigzag polygons vs oracle"))
+
+
+def sheared_strip(width=20.0, nrows=5, dy=10.0, shear=0.5, TB=300.5, TT=299.5):
+    """periodic_strip sheared by x -> x + shear * y: a column of
+    parallelograms periodic in x through parallelogram ghost boxes on both
+    sides (each translates onto the other), reservoirs below and above"""
+    H = nrows * dy
+
+    def X(x, y):
+        return x + shear * y
+
+    polys = [np.array([[X(0, a), a], [X(width, a), a], [X(width, b), b], [X(0, b), b]])
+             for a, b in ((k * dy, (k + 1) * dy) for k in range(nrows))]
+    polys.append(np.array([[X(-2, 0), 0], [X(0, 0), 0], [X(0, H), H], [X(-2, H), H]]))
+    polys.append(np.array([[X(width, 0), 0], [X(width + 2, 0), 0], [X(width + 2, H), H], [X(width, H), H]]))
+    polys.append(np.array([[-2.0, -2.0], [width + 2.0, -2.0], [width + 2.0, 0.0], [-2.0, 0.0]]))
+    polys.append(np.array([[X(-2, H), H], [X(width + 2, H), H], [X(width + 2, H), H + 2.0], [X(-2, H), H + 2.0]]))
+    kind = np.array([0] * nrows + [2, 2, 1, 1], np.int32)
+    teq = np.array([300.0] * (nrows + 2) + [TB, TT])
+    left, right = nrows, nrows + 1
+    return polys, kind, teq, {left: (right, width, 0.0), right: (left, -width, 0.0)}
+
+
+def test_sheared_periodic_strip_ballistic_flux_is_exact(pkg, ctx):
+    """translate on polygon ghost boxes: ballistic particles from the bottom
+    reservoir wrap around the sheared periodic direction until they reach
+    the top, so every row carries the flux (|J_B| + |J_T|) 1e30 (each
+    particle crosses each row once, dy / vy in it)"""
+    from test_oracle_mc import tilted_y_material
+    m = tilted_y_material()
+    polys, kind, teq, per = sheared_strip()
+    modes = _modes(pkg, ctx, m)
+    geom = pkg.Geometry(ctx, None, kind, teq, periodic=per, polys=polys)
+    out = pkg.rtamc2d(ctx, modes, geom, thickness=1.0, T0=300.0, dt=50.0, maxtime=1000.0, nparticles=100000,
+                      seed=5, ballistic=True, sigma_bounce=5.0)
+    vuc = m["V"] / m["lattz"]
+    JB = orc.wall_dist(m["omega"], m["vx"], m["vy"], 300.5, 300.0, 0.0, 1.0, vuc)[2]
+    JT = orc.wall_dist(m["omega"], m["vx"], m["vy"], 299.5, 300.0, 0.0, -1.0, vuc)[2]
+    np.testing.assert_allclose(out["jy"][-1, :5], (abs(JB) + abs(JT)) * 1e30, rtol=1e-8)
+    assert np.abs(out["jy"][-1, 5:7]).max() == 0.0
+    assert out["events"] > 0
+
+
+def test_sheared_periodic_strip_matches_oracle(pkg, ctx):
+    mat = pkg.workloads.material(400, seed=8)
+    polys, kind, teq, per = sheared_strip(width=30.0, nrows=6, dy=15.0)
+    modes = _modes(pkg, ctx, mat)
+    geom = pkg.Geometry(ctx, None, kind, teq, periodic=per, polys=polys)
+    kw = dict(thickness=1.0, T0=300.0, dt=20.0, maxtime=300.0)
+    R, N = 12, 20000
+    rect = orc.poly_rect(polys)
+    g = [pkg.rtamc2d(ctx, modes, geom, nparticles=N, seed=300 + r, sigma_bounce=2.0, **kw) for r in range(R)]
+    o = [orc.rta_run(mat, rect, kind, teq, nparticles=N, seed=400 + r, sigma=2.0, periodic=per, polys=polys, **kw)
+         for r in range(R)]
+    zs = [zstats(np.array([x[key][[3, -1], :6] for x in g]), np.array([x[key][[3, -1], :6] for x in o]))
+          for key in ("T", "jx", "jy")]
+    for k in ("events", "crossings"):
+        zs.append(zstats(np.array([float(x[k]) for x in g]), np.array([float(x[k]) for x in o])))
+    print(assert_3sigma(zs, "sheared periodic strip vs oracle"))
+
+
+def test_polygon_geometry_errors(pkg, ctx):
+    """a vertex between two hull corners (the reference would keep part of
+    that side as a border), a degenerate box and too many sides are refused"""
+    k1, t1 = np.zeros(1, np.int32), np.full(1, 300.0)
+    with pytest.raises(pkg.BteError, match="vertex lies on a side"):
+        pkg.Geometry(ctx, None, k1, t1, polys=[np.array([[0.0, 0.0], [5.0, 0.0], [10.0, 0.0], [10.0, 5.0], [0.0, 5.0]])])
+    with pytest.raises(pkg.BteError, match="degenerate"):
+        pkg.Geometry(ctx, None, k1, t1, polys=[np.array([[0.0, 0.0], [1.0, 1.0], [2.0, 2.0]])])
+    a = 2 * np.pi * np.arange(20) / 20
+    with pytest.raises(pkg.BteError, match="more than 16 sides"):
+        pkg.Geometry(ctx, None, k1, t1, polys=[np.stack([np.cos(a), np.sin(a)], 1)])
