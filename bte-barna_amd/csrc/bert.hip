@@ -52,6 +52,8 @@ struct BertParams {
     const uint2* rta_alias;  // BE_derivative alias per box-temperature slot
     const int32_t* box_rta_slot;
     const double* box_Teq;
+    const double* beq;       // [distinct regular-box Teq][n] Bose-Einstein occupations at Teq
+    const int32_t* box_beq;  // per box its row of beq
     const double* spl;       // spline coefficients: per table 6 x nseg
     const int32_t* side_spl; // per side spline table index, -1 none
     // every geometry: the sides (rectangles: side s of box i at 4 i + s),
@@ -691,6 +693,11 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     const double Tb = S[lo] + dx * (S[P.nseg + lo] + dx * (S[2 * P.nseg + lo] + S[3 * P.nseg + lo] * dx));
     const double nx = -sd.nx, ny = -sd.ny;  // emission normal points into the box
     const int n = P.n;
+    // n(omega, Tb) - n(omega, Teq): the first from exp(omega pf / Tb), the
+    // second from the box's row of the equilibrium table
+    constexpr double pf = 1e12 * hbar / kB;
+    const double sTb = pf / Tb;
+    const double* beq = P.beq + (size_t)P.box_beq[box] * n;
     double* cdf = P.wcdf + (size_t)b * n;
     int32_t* G = P.wguide + (size_t)b * BERT_GUIDE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -711,7 +718,7 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
                 const double om = P.omega[i];
                 const double vn = nx * v.x + ny * v.y;
                 if (vn > 0. && !almost_equal(0., om))
-                    pm = vn * om * fabs(bose_einstein(om, Tb) - bose_einstein(om, Teq));
+                    pm = vn * om * fabs(1. / (exp(sTb * om) - 1.) - beq[i]);
             }
             run += pm;
             c[k] = run;
@@ -762,7 +769,7 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
             const double2 v = P.vel[i];
             const double om = P.omega[i];
             const double vn = nx * v.x + ny * v.y;
-            if (vn > 0. && !almost_equal(0., om)) pm = vn * om * fabs(bose_einstein(om, Tb) - bose_einstein(om, Teq));
+            if (vn > 0. && !almost_equal(0., om)) pm = vn * om * fabs(1. / (exp(sTb * om) - 1.) - beq[i]);
         }
         double ctot;
         const double incl = block_scan_256(pm, wtot, ctot);
@@ -1200,7 +1207,8 @@ struct Bert {
     DevBuf<uint32_t> d_seg, d_tickets, d_flags;
     DevBuf<int32_t> d_Eb, d_side_spl, d_rta_slot, d_side_box;
     DevBuf<unsigned long long> d_H;
-    DevBuf<double> d_props, d_spl, d_Teq, d_pbox, d_box_area;
+    DevBuf<double> d_props, d_spl, d_Teq, d_pbox, d_box_area, d_beq;
+    DevBuf<int32_t> d_box_beq;
     DevBuf<double2> d_center;
     DevBuf<int64_t> d_npart, d_pending;
     DevBuf<double> d_wcdf;
@@ -1354,6 +1362,8 @@ static BertParams make_params(Bert* b) {
     P.rta_alias = b->d_rta_alias.p;
     P.box_rta_slot = b->d_rta_slot.p;
     P.box_Teq = b->d_Teq.p;
+    P.beq = b->d_beq.p;
+    P.box_beq = b->d_box_beq.p;
     P.spl = b->d_spl.p;
     P.side_spl = b->d_side_spl.p;
     P.seg = b->d_seg.p;
@@ -1785,6 +1795,24 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         for (int k = 0; k < g->srange[i].y; k++) side_box[g->srange[i].x + k] = i;
     }
     b->d_Teq.upload(Teq, s);
+    {
+        // equilibrium occupations per distinct box temperature
+        std::map<double, int> rows;
+        std::vector<double> beq;
+        std::vector<int32_t> box_beq(nbox, 0);
+        for (int i = 0; i < nbox; i++) {
+            const double T = g->boxes[i].Teq;
+            if (!(T > 0.)) continue;  // no walls emit there
+            if (!rows.count(T)) {
+                rows[T] = (int)rows.size();
+                for (int k = 0; k < n; k++) beq.push_back(m->omega[k] > 0. ? h_bose_einstein(m->omega[k], T) : 0.);
+            }
+            box_beq[i] = rows[T];
+        }
+        if (beq.empty()) beq.assign((size_t)n, 0.);
+        b->d_beq.upload(beq, s);
+        b->d_box_beq.upload(box_beq, s);
+    }
     b->d_box_area.upload(area, s);
     b->d_center.upload(centre, s);
     b->d_side_box.upload(side_box, s);
