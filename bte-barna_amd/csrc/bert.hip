@@ -655,6 +655,20 @@ __device__ inline double block_scan_256(double v, double* wtot, double& tot) {
     return v + off;
 }
 
+// inclusive max-scan over the 256 threads of a workgroup (wave scans by
+// shuffles, the four wave maxima through LDS)
+__device__ inline int block_max_scan_256(int v, int* wmax) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(v, d, 64);
+        if (lane >= d) v = max(v, u);
+    }
+    if (lane == 63) wmax[w] = v;
+    __syncthreads();
+    for (int k = 0; k < w; k++) v = max(v, wmax[k]);
+    return v;
+}
+
 #define BERT_WCDF_PER 8
 // Adiabatic diffuse walls (beyondRTA_MC2d.cpp:281-435), phase 1: one
 // workgroup per (box, side) with a nonzero Eborder computes the border
@@ -663,7 +677,26 @@ __device__ inline double block_scan_256(double v, double* wtot, double& tot) {
 __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_t step) {
     __shared__ double wtot[4];
     __shared__ int kcarry[4];
+    // register path: the weight of every mode, then its guide key
+    __shared__ double spm[256 * BERT_WCDF_PER];
+    __shared__ int skey[256 * BERT_WCDF_PER];
+    __shared__ int grun[BERT_GUIDE];  // mode starting the run of keys at each guide entry, else 0
     const int b = blockIdx.x;  // side
+    const int n = P.n;
+    const bool regs = n <= 256 * BERT_WCDF_PER;
+    // The kernel is a chain of dependent loads (side -> box -> spline ->
+    // modes), so the mode velocities and frequencies, which depend on
+    // nothing, are requested first, coalesced: mode k * 256 + thread.
+    double2 mv[BERT_WCDF_PER];
+    double mom[BERT_WCDF_PER];
+    if (regs) {
+#pragma unroll
+        for (int k = 0; k < BERT_WCDF_PER; k++) {
+            const int i = k * 256 + threadIdx.x;
+            mv[k] = i < n ? P.vel[i] : make_double2(0., 0.);
+            mom[k] = i < n ? P.omega[i] : 0.;
+        }
+    }
     const int box = P.side_box[b];
     int32_t* Eb = P.Eb + (size_t)(step & 1) * P.nsd;
     const int E = Eb[b];
@@ -679,6 +712,15 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     if (threadIdx.x == 0) Eb[b] = 0;
     const DevSide sd = P.sides[b];
     const double Teq = P.box_Teq[box];
+    const double* beq = P.beq + (size_t)P.box_beq[box] * n;
+    double mbq[BERT_WCDF_PER];
+    if (regs) {
+#pragma unroll
+        for (int k = 0; k < BERT_WCDF_PER; k++) {
+            const int i = k * 256 + threadIdx.x;
+            mbq[k] = i < n ? beq[i] : 0.;
+        }
+    }
     const double L = hypot(sd.p2x - sd.p1x, sd.p2y - sd.p1y);
     const double J = P.Eeff * (double)E / P.dt / (L * P.thickness);
     const double* S = P.spl + (size_t)si * 6 * P.nseg;  // a b c d x0 xe
@@ -692,47 +734,45 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
     const double dx = J - S[4 * P.nseg + lo];
     const double Tb = S[lo] + dx * (S[P.nseg + lo] + dx * (S[2 * P.nseg + lo] + S[3 * P.nseg + lo] * dx));
     const double nx = -sd.nx, ny = -sd.ny;  // emission normal points into the box
-    const int n = P.n;
     // n(omega, Tb) - n(omega, Teq): the first from exp(omega pf / Tb), the
     // second from the box's row of the equilibrium table
     constexpr double pf = 1e12 * hbar / kB;
     const double sTb = pf / Tb;
-    const double* beq = P.beq + (size_t)P.box_beq[box] * n;
     double* cdf = P.wcdf + (size_t)b * n;
     int32_t* G = P.wguide + (size_t)b * BERT_GUIDE;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (n <= 256 * BERT_WCDF_PER) {
-        // each thread owns up to BERT_WCDF_PER consecutive modes in
-        // registers: one load phase, one block scan, the guide table from
-        // the same values
+    if (regs) {
+#pragma unroll
+        for (int k = 0; k < BERT_WCDF_PER; k++) {
+            const int i = k * 256 + threadIdx.x;
+            const double vn = nx * mv[k].x + ny * mv[k].y;
+            double pm = 0.;
+            if (vn > 0. && !almost_equal(0., mom[k]))
+                pm = vn * mom[k] * fabs(1. / (exp(sTb * mom[k]) - 1.) - mbq[k]);
+            if (i < n) spm[i] = pm;
+        }
+        for (int j = threadIdx.x; j < BERT_GUIDE; j += blockDim.x) grun[j] = 0;
+        __syncthreads();
+        // the scan: each thread sums up to BERT_WCDF_PER consecutive modes
+        // from LDS, one block scan adds the preceding threads' totals
         const int per = (n + 255) >> 8;
         const int i0 = threadIdx.x * per;
         double c[BERT_WCDF_PER];
         double run = 0.;
 #pragma unroll
         for (int k = 0; k < BERT_WCDF_PER; k++) {
-            const int i = i0 + k;
-            double pm = 0.;
-            if (k < per && i < n) {
-                const double2 v = P.vel[i];
-                const double om = P.omega[i];
-                const double vn = nx * v.x + ny * v.y;
-                if (vn > 0. && !almost_equal(0., om))
-                    pm = vn * om * fabs(1. / (exp(sTb * om) - 1.) - beq[i]);
-            }
-            run += pm;
+            if (k < per && i0 + k < n) run += spm[i0 + k];
             c[k] = run;
         }
         double tot;
         const double excl = block_scan_256(run, wtot, tot) - run;
-        int mylast = -1;  // key of this thread's last mode (-1: it has none)
         const double inv = tot > 0. ? 1.0 / tot : 0.;
 #pragma unroll
         for (int k = 0; k < BERT_WCDF_PER; k++) {
             if (k < per && i0 + k < n) {
                 c[k] = (c[k] + excl) * inv;
-                cdf[i0 + k] = c[k];
-                mylast = guide_key(c[k], (double)BERT_GUIDE);
+                spm[i0 + k] = c[k];
+                skey[i0 + k] = guide_key(c[k], (double)BERT_GUIDE);
             }
         }
         if (threadIdx.x == 0) {
@@ -740,23 +780,41 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
             P.wsign[b] = (Tb - Teq) >= 0 ? 1 : -1;
         }
         if (!(tot > 0.)) return;  // uniform over the workgroup
-        // G[j] = first mode whose key reaches j: each mode fills the keys
-        // above its predecessor's (threads without modes come last)
-        int kp = __shfl_up(mylast, 1, 64);
-        if (lane == 63) kcarry[w] = mylast;
         __syncthreads();
-        if (lane == 0) kp = w > 0 ? kcarry[w - 1] : -1;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) cdf[i] = spm[i];  // coalesced
+        // G[j] = first mode whose key reaches j (the keys ascend with the
+        // cumulative), the last mode when none does. Mode i owns the keys
+        // (key[i-1], key[i]]: it marks the first of them, the last mode marks
+        // the key above its own, and a max-scan over the entries carries each
+        // mark up to the next one (the marking modes ascend with the entries).
 #pragma unroll
         for (int k = 0; k < BERT_WCDF_PER; k++) {
-            if (k < per && i0 + k < n) {
-                const int kk = guide_key(c[k], (double)BERT_GUIDE);
-                for (int j = kp + 1; j <= kk; j++) G[j] = i0 + k;
-                kp = kk;
-                // keys above the last cumulative value (rounding) point at the last mode
-                if (i0 + k == n - 1)
-                    for (int j = kk + 1; j < BERT_GUIDE; j++) G[j] = n - 1;
+            const int i = i0 + k;
+            if (k < per && i < n) {
+                const int kp = i > 0 ? skey[i - 1] : -1;
+                if (skey[i] > kp) grun[kp + 1] = i;
             }
         }
+        if (threadIdx.x == 0 && skey[n - 1] + 1 < BERT_GUIDE) grun[skey[n - 1] + 1] = n - 1;
+        __syncthreads();
+        constexpr int GPT = BERT_GUIDE / 256;  // entries per thread
+        int gv[GPT];
+        int gm = 0;
+#pragma unroll
+        for (int q = 0; q < GPT; q++) {
+            gm = max(gm, grun[threadIdx.x * GPT + q]);
+            gv[q] = gm;
+        }
+        const int gpre = block_max_scan_256(gm, kcarry);
+        // the entries before this thread's: its predecessor's scan value
+        // (for a wave's first lane, the maximum of the earlier waves)
+        int gprev = __shfl_up(gpre, 1, 64);
+        if (lane == 0) {
+            gprev = 0;
+            for (int k = 0; k < w; k++) gprev = max(gprev, kcarry[k]);
+        }
+        static_assert(GPT == 4, "one 16-byte store of guide entries per thread");
+        *(int4*)(G + threadIdx.x * GPT) = make_int4(max(gv[0], gprev), max(gv[1], gprev), max(gv[2], gprev), max(gv[3], gprev));
         return;
     }
     // larger tables: the cumulative in chunks of 256 consecutive modes, one mode per thread
