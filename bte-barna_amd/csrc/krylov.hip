@@ -3,8 +3,9 @@
 // Reference: build_P (src/collision_operator.cpp:1015-1064) runs one
 // Arnoldi_algorithm (815-851) per column with b = e_i and keeps
 // P(:,i) = (V exp(H))(:,0). Here every column advances together: step k is
-// one N x N x N GEMM Z = dt*B*V_k (rocBLAS, FP64 / FP32 / BF16 hi+lo split
-// with FP32 accumulation), followed by a per-column FP64 modified
+// one N x N x N GEMM Z = dt*B*V_k (FP64 and FP32 on rocBLAS; BF16 hi+lo
+// split as one hand-written MFMA kernel with FP32 accumulation, A_hi V_hi +
+// A_hi V_lo + A_lo V_hi), followed by a per-column FP64 modified
 // Gram-Schmidt kernel, a per-column FP64 exponential of the small Hessenberg
 // matrix and a combine kernel that forms P(:,i) and applies the FP64 column
 // energy projection (collision_operator.cpp:786-807 applied to P: columns
@@ -31,14 +32,199 @@ __global__ void kr_to_f32(const double* __restrict__ a, float* __restrict__ o, s
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         o[i] = (float)(a[i] * scale);
 }
-__global__ void kr_to_bf16x2(const double* __restrict__ a, __hip_bfloat16* __restrict__ hi,
-                             __hip_bfloat16* __restrict__ lo, size_t n, double scale) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        float v = (float)(a[i] * scale);
-        __hip_bfloat16 h = __float2bfloat16(v);
-        hi[i] = h;
-        lo[i] = __float2bfloat16(v - __bfloat162float(h));
+// column-major N x N a -> Np x Np FP32, column stride ldo (rows and columns
+// past N are not written)
+__global__ void kr_to_f32_ld(const double* __restrict__ a, float* __restrict__ o, int N, size_t ldo, double scale) {
+    const size_t n = (size_t)N * N;
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        o[(q / N) * ldo + q % N] = (float)(a[q] * scale);
+}
+// column-major N x N a -> the row-major hi/lo-interleaved operand of
+// kr_bf16x3_gemm: row r holds, for every 8 consecutive k, the 8 bf16 hi
+// parts of (float)(scale * a[r + k N]) then the 8 lo parts; zero past N up to
+// Np. Rows ld bf16 apart. 32 x 32 tiles transposed through LDS (grid Np/32 x
+// Np/32, 256 threads).
+__global__ __launch_bounds__(256) void kr_to_bf16_il(const double* __restrict__ a, __hip_bfloat16* __restrict__ A,
+                                                     int N, size_t ld, double scale) {
+    __shared__ float t[32][33];
+    const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int kk = ty; kk < 32; kk += 8) {
+        const int r = r0 + tx, k = k0 + kk;
+        t[kk][tx] = (r < N && k < N) ? (float)(a[r + (size_t)k * N] * scale) : 0.f;
     }
+    __syncthreads();
+    for (int rr = ty; rr < 32; rr += 8) {
+        const int r = r0 + rr, k = k0 + tx;
+        const float v = t[tx][rr];
+        const __hip_bfloat16 h = __float2bfloat16(v);
+        __hip_bfloat16* e = A + (size_t)r * ld + 2 * (k & ~7) + (k & 7);
+        e[0] = h;
+        e[8] = __float2bfloat16(v - __bfloat162float(h));
+    }
+}
+
+// Z = A_hi V_hi + A_hi V_lo + A_lo V_hi in FP32 on MFMA: the compensated BF16
+// form of the block product dt*B*V_k that replaces Arnoldi_algorithm's
+// per-column t*A*q (collision_operator.cpp:832). A (Np x Np, rows ld_a bf16
+// apart) and V_k (Np columns, ld_v bf16 apart) are hi/lo interleaved per 8
+// consecutive k, so one 32-deep stage of a row is one 128-byte line holding
+// both parts. Z: column-major Np x Np. Np a multiple of 256.
+// Workgroup tile 256 x 256, 8 waves of 128 x 64 (4 x 2 blocks of
+// v_mfma_f32_32x32x16_bf16, three products each per 16-deep step); stages of
+// K = 32 (64 KB) through a 2-stage LDS ring filled by 16-byte direct-to-LDS
+// loads, issued for stage s+1 while stage s is multiplied. In LDS a row is 8
+// chunks of 16 bytes, chunk c stored at c ^ ((row >> 1) & 7): the 16 lanes
+// of a 128-bit fragment read then cover all 64 banks.
+namespace gk {
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void lds_void;
+constexpr int WM = 2, WN = 4, MB = 4, NB = 2;       // waves and blocks per wave
+constexpr int TM = WM * MB * 32, TN = WN * NB * 32;  // 256 x 256
+constexpr int NW = WM * WN, NT = 64 * NW;            // 8 waves
+constexpr int ROWB = 128, STAGE = (TM + TN) * ROWB;  // 64 KB
+constexpr int LDS = 2 * STAGE;
+constexpr int LPW = STAGE / 1024 / NW;               // 1 KB direct loads per wave per stage
+__device__ inline int swz(int r) { return (r >> 1) & 7; }
+}  // namespace gk
+
+__global__ __launch_bounds__(gk::NT) void kr_bf16x3_gemm(const __hip_bfloat16* __restrict__ A, size_t lda,
+                                                        const __hip_bfloat16* __restrict__ V, size_t ldv,
+                                                        float* __restrict__ Z, int Np) {
+    using namespace gk;
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int tmn = Np / TM, tnn = Np / TN, total = tmn * tnn;
+    // workgroups go round-robin to the 8 XCDs: each XCD takes a contiguous
+    // eighth of the tiles, walked in bands of 4 row tiles, so the 32 tiles
+    // an XCD runs at once share 4 row panels and 8 column panels in its L2
+    const int b = blockIdx.x;
+    const int u = (total % 8 == 0) ? (b % 8) * (total / 8) + b / 8 : b;
+    int tr, tc;
+    if (tmn % 4 == 0) {
+        const int band = u / (4 * tnn), w = u % (4 * tnn);
+        tr = band * 4 + w % 4;
+        tc = w / 4;
+    } else {
+        tr = u / tnn;
+        tc = u % tnn;
+    }
+    const int row0 = tr * TM, col0 = tc * TN;
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // direct load q of this wave fills 1 KB block blk of a stage: 8 rows,
+    // lane i writes 16 bytes at 16 i: row i / 8, stored chunk i % 8, which
+    // holds chunk (i % 8) ^ swz(row) of the row's 128 bytes
+    const __hip_bfloat16* gsrc[LPW];
+    int ldsoff[LPW];
+#pragma unroll
+    for (int q = 0; q < LPW; q++) {
+        const int blk = q * NW + wv;
+        const __hip_bfloat16* base;
+        int r;
+        if (blk < TM / 8) {
+            r = blk * 8 + (lane >> 3);
+            base = A + (size_t)(row0 + r) * lda;
+        } else {
+            r = (blk - TM / 8) * 8 + (lane >> 3);
+            base = V + (size_t)(col0 + r) * ldv;
+        }
+        gsrc[q] = base + ((lane & 7) ^ swz(r)) * 8;
+        ldsoff[q] = blk * 1024;
+    }
+    auto issue = [&](int s, int buf) {
+#pragma unroll
+        for (int q = 0; q < LPW; q++)
+            __builtin_amdgcn_global_load_lds((const void*)(gsrc[q] + (size_t)s * 64),
+                                             (lds_void*)(lds + buf * STAGE + ldsoff[q]), 16, 0, 0);
+    };
+    f32x16 acc[MB][NB];
+#pragma unroll
+    for (int x = 0; x < MB; x++)
+#pragma unroll
+        for (int y = 0; y < NB; y++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[x][y][e] = 0.f;
+    const int wm = wv % WM, wn = wv / WM;
+    const int fr = lane & 31, fh = lane >> 5;
+    // fragment of 16-deep step ks for this lane: k chunk j = 2 ks + fh of
+    // row fr, hi at chunk 2 j and lo at chunk 2 j + 1
+    int ahoff[MB][2], aloff[MB][2], bhoff[NB][2], bloff[NB][2];
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++) {
+        const int r = wm * MB * 32 + mb * 32 + fr;
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const int j = 2 * ks + fh;
+            ahoff[mb][ks] = r * ROWB + (((2 * j) ^ swz(r)) * 16);
+            aloff[mb][ks] = r * ROWB + (((2 * j + 1) ^ swz(r)) * 16);
+        }
+    }
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) {
+        const int r = wn * NB * 32 + nb * 32 + fr;
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const int j = 2 * ks + fh;
+            bhoff[nb][ks] = TM * ROWB + r * ROWB + (((2 * j) ^ swz(r)) * 16);
+            bloff[nb][ks] = TM * ROWB + r * ROWB + (((2 * j + 1) ^ swz(r)) * 16);
+        }
+    }
+    const int nk = Np / 32;
+    issue(0, 0);
+    for (int s = 0; s < nk; s++) {
+        const int buf = s & 1;
+        // this wave's loads of stage s have landed (s_waitcnt vmcnt(0));
+        // after the barrier every wave's have, and every wave is done with
+        // stage s-1, whose buffer then takes stage s+1. A bare barrier: the
+        // fence of __syncthreads would not add anything here.
+        __builtin_amdgcn_s_waitcnt(0x3F70);
+        __builtin_amdgcn_s_barrier();
+        if (s + 1 < nk) issue(s + 1, buf ^ 1);
+        const unsigned char* sb = lds + buf * STAGE;
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            bf16x8 ah[MB], al[MB], bh[NB], bl[NB];
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++) {
+                ah[mb] = *reinterpret_cast<const bf16x8*>(sb + ahoff[mb][ks]);
+                al[mb] = *reinterpret_cast<const bf16x8*>(sb + aloff[mb][ks]);
+            }
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) {
+                bh[nb] = *reinterpret_cast<const bf16x8*>(sb + bhoff[nb][ks]);
+                bl[nb] = *reinterpret_cast<const bf16x8*>(sb + bloff[nb][ks]);
+            }
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mb], bh[nb], acc[mb][nb], 0, 0, 0);
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[mb], bl[nb], acc[mb][nb], 0, 0, 0);
+#pragma unroll
+            for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[mb], bh[nb], acc[mb][nb], 0, 0, 0);
+        }
+    }
+    // a 32 x 32 block: lane holds column lane & 31, rows 8 q + 4 (lane >> 5) + e
+#pragma unroll
+    for (int mb = 0; mb < MB; mb++)
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) {
+            const int col = col0 + wn * NB * 32 + nb * 32 + fr;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int row = row0 + wm * MB * 32 + mb * 32 + 8 * q + 4 * fh;
+                *reinterpret_cast<float4*>(Z + (size_t)col * Np + row) = make_float4(
+                    acc[mb][nb][4 * q], acc[mb][nb][4 * q + 1], acc[mb][nb][4 * q + 2], acc[mb][nb][4 * q + 3]);
+            }
+        }
 }
 __global__ void kr_scale64(const double* __restrict__ a, double* __restrict__ o, size_t n, double scale) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -49,28 +235,23 @@ __global__ void kr_f32_to_f64(const float* __restrict__ a, double* __restrict__ 
         o[i] = (double)a[i];
 }
 
-// Basis storage: FP64 (prec 0), FP32 (1) or BF16 hi/lo (2); column i of V_j
-// starts at (j*N + i)*N.
+// Basis storage: FP64 (prec 0), FP32 (1) or BF16 hi/lo (2). Element r of
+// column i of V_j has the logical index (j*nc + i)*cs + r (nc columns per
+// step, cs a multiple of 8). FP64/FP32: stored at that index (nc = cs = N).
+// BF16: every 8 consecutive elements are stored as 8 hi then 8 lo values,
+// so logical index x lives at hi[il(x)] and lo[il(x)] = hi[il(x) + 8] with
+// il(x) = 2 (x & ~7) + (x & 7); nc = Np (N rounded up to the product
+// kernel's tile), cs = Np + 32 (columns 2 Np + 64 bf16 apart in memory).
 struct Basis {
     double* f64;
     float* f32;
     __hip_bfloat16* hi;
-    __hip_bfloat16* lo;
+    __hip_bfloat16* lo;  // hi + 8
     int prec;
-    __device__ inline double load(size_t idx) const {
-        if (prec == 0) return f64[idx];
-        if (prec == 1) return (double)f32[idx];
-        return (double)__bfloat162float(hi[idx]) + (double)__bfloat162float(lo[idx]);
-    }
-    __device__ inline void store(size_t idx, double v) const {
-        if (prec == 0) { f64[idx] = v; return; }
-        if (prec == 1) { f32[idx] = (float)v; return; }
-        float f = (float)v;
-        __hip_bfloat16 h = __float2bfloat16(f);
-        hi[idx] = h;
-        lo[idx] = __float2bfloat16(f - __bfloat162float(h));
-    }
+    int nc;
+    size_t cs;
 };
+__device__ inline size_t il(size_t x) { return 2 * (x & ~(size_t)7) + (x & 7); }
 
 // c = exp(H_i) e_1 for every column: one wave per column, H in LDS.
 // Scaling and squaring (the method of Eigen's MatrixExponential, which the
@@ -165,7 +346,7 @@ template <int PREC>
 __device__ inline double bload(const Basis& V, size_t i) {
     if constexpr (PREC == 0) return V.f64[i];
     else if constexpr (PREC == 1) return (double)V.f32[i];
-    else return (double)__bfloat162float(V.hi[i]) + (double)__bfloat162float(V.lo[i]);
+    else return (double)__bfloat162float(V.hi[il(i)]) + (double)__bfloat162float(V.lo[il(i)]);
 }
 template <int PREC>
 __device__ inline void bstore(const Basis& V, size_t i, double v) {
@@ -174,8 +355,8 @@ __device__ inline void bstore(const Basis& V, size_t i, double v) {
     else {
         float f = (float)v;
         __hip_bfloat16 h = __float2bfloat16(f);
-        V.hi[i] = h;
-        V.lo[i] = __float2bfloat16(f - __bfloat162float(h));
+        V.hi[il(i)] = h;
+        V.lo[il(i)] = __float2bfloat16(f - __bfloat162float(h));
     }
 }
 // sum over the 16 waves of a 1024-thread workgroup; buf holds 16 doubles
@@ -193,14 +374,14 @@ __device__ inline double wide_sum(double v, double* buf) {
 template <int E, int PREC>
 __global__ __launch_bounds__(1024) void kr_mgs_wide(const float* __restrict__ Z32, const double* __restrict__ Z64,
                                                     Basis V, double* __restrict__ H, int* __restrict__ closed,
-                                                    int N, int m, int k, int textbook) {
+                                                    int N, size_t ldz, int m, int k, int textbook) {
     __shared__ double red[3][16];
     __shared__ double hz;
     const int i = blockIdx.x;
     if (closed[i]) return;
     const int tid = threadIdx.x;
     double z[E], v[E];
-    const size_t zc = (size_t)i * N;
+    const size_t zc = (size_t)i * ldz;
 #pragma unroll
     for (int e = 0; e < E; e++) {
         const int r = tid + 1024 * e;
@@ -213,7 +394,7 @@ __global__ __launch_bounds__(1024) void kr_mgs_wide(const float* __restrict__ Z3
     for (int e = 0; e < E; e++)
         if (tid + 1024 * e == i) hz = z[e];
     if (jmax > 1) {
-        const size_t vc = ((size_t)1 * N + i) * N;
+        const size_t vc = ((size_t)1 * V.nc + i) * V.cs;
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int r = tid + 1024 * e;
@@ -234,7 +415,7 @@ __global__ __launch_bounds__(1024) void kr_mgs_wide(const float* __restrict__ Z3
         for (int e = 0; e < E; e++) d += v[e] * z[e];
         double vn[E];
         if (j + 1 < jmax) {
-            const size_t vc = ((size_t)(j + 1) * N + i) * N;
+            const size_t vc = ((size_t)(j + 1) * V.nc + i) * V.cs;
 #pragma unroll
             for (int e = 0; e < E; e++) {
                 const int r = tid + 1024 * e;
@@ -262,7 +443,7 @@ __global__ __launch_bounds__(1024) void kr_mgs_wide(const float* __restrict__ Z3
     }
     if (k + 1 < m) {
         if (tid == 0) Hi[(k + 1) + (size_t)k * m] = zn;
-        const size_t vc = ((size_t)(k + 1) * N + i) * N;
+        const size_t vc = ((size_t)(k + 1) * V.nc + i) * V.cs;
         const double inv = 1.0 / zn;
 #pragma unroll
         for (int e = 0; e < E; e++) {
@@ -285,7 +466,7 @@ __global__ __launch_bounds__(1024) void kr_combine_wide(Basis V, const double* _
     for (int k = 1; k < m; k++) {
         const double ck = c[k];
         if (ck == 0.) continue;
-        const size_t vc = ((size_t)k * N + i) * N;
+        const size_t vc = ((size_t)k * V.nc + i) * V.cs;
 #pragma unroll
         for (int e = 0; e < E; e++) {
             const int r = tid + 1024 * e;
@@ -327,8 +508,9 @@ __device__ inline void vload8(const Basis& V, size_t i, double* o) {
         o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
         o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
     } else {
-        const uint4 h = *reinterpret_cast<const uint4*>(V.hi + i);
-        const uint4 l = *reinterpret_cast<const uint4*>(V.lo + i);
+        // i is a multiple of 8: 16 bytes of hi then 16 of lo at 2 i
+        const uint4 h = *reinterpret_cast<const uint4*>(V.hi + 2 * i);
+        const uint4 l = *reinterpret_cast<const uint4*>(V.lo + 2 * i);
         const uint32_t hh[4] = {h.x, h.y, h.z, h.w}, ll[4] = {l.x, l.y, l.z, l.w};
 #pragma unroll
         for (int w = 0; w < 4; w++) {
@@ -364,8 +546,8 @@ __device__ inline void vstore8(const Basis& V, size_t i, const double* v) {
             hh[w] = hb[0] | (hb[1] << 16);
             ll[w] = lb[0] | (lb[1] << 16);
         }
-        *reinterpret_cast<uint4*>(V.hi + i) = make_uint4(hh[0], hh[1], hh[2], hh[3]);
-        *reinterpret_cast<uint4*>(V.lo + i) = make_uint4(ll[0], ll[1], ll[2], ll[3]);
+        *reinterpret_cast<uint4*>(V.hi + 2 * i) = make_uint4(hh[0], hh[1], hh[2], hh[3]);
+        *reinterpret_cast<uint4*>(V.lo + 2 * i) = make_uint4(ll[0], ll[1], ll[2], ll[3]);
     }
 }
 // sum over the NT/64 waves of the workgroup (buf: NT/64 doubles, not
@@ -384,7 +566,7 @@ __device__ inline double vec_sum(double v, double* buf) {
 template <int NT, int E, int PREC>
 __global__ __launch_bounds__(NT) void kr_mgs_vec(const float* __restrict__ Z32, const double* __restrict__ Z64,
                                                  Basis V, double* __restrict__ H, int* __restrict__ closed, int N,
-                                                 int m, int k, int textbook) {
+                                                 size_t ldz, int m, int k, int textbook) {
     __shared__ double red[3][NT / 64];
     const int i = blockIdx.x;
     if (closed[i]) return;
@@ -395,7 +577,7 @@ __global__ __launch_bounds__(NT) void kr_mgs_vec(const float* __restrict__ Z32, 
         const int r = r0 + 8 * c;
         if (r < N) {
             if (PREC == 0) {
-                const double2* p = reinterpret_cast<const double2*>(Z64 + (size_t)i * N + r);
+                const double2* p = reinterpret_cast<const double2*>(Z64 + (size_t)i * ldz + r);
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
                     const double2 v = p[q];
@@ -403,7 +585,7 @@ __global__ __launch_bounds__(NT) void kr_mgs_vec(const float* __restrict__ Z32, 
                     z[8 * c + 2 * q + 1] = v.y;
                 }
             } else {
-                const float4* p = reinterpret_cast<const float4*>(Z32 + (size_t)i * N + r);
+                const float4* p = reinterpret_cast<const float4*>(Z32 + (size_t)i * ldz + r);
                 const float4 a = p[0], b = p[1];
                 z[8 * c] = a.x; z[8 * c + 1] = a.y; z[8 * c + 2] = a.z; z[8 * c + 3] = a.w;
                 z[8 * c + 4] = b.x; z[8 * c + 5] = b.y; z[8 * c + 6] = b.z; z[8 * c + 7] = b.w;
@@ -427,7 +609,7 @@ __global__ __launch_bounds__(NT) void kr_mgs_vec(const float* __restrict__ Z32, 
     }
     for (int j = 1; j < jmax; j++) {
         double v[E];
-        const size_t vc = ((size_t)j * N + i) * N;
+        const size_t vc = ((size_t)j * V.nc + i) * V.cs;
 #pragma unroll
         for (int c = 0; c < E / 8; c++) {
             if (r0 + 8 * c < N) vload8<PREC>(V, vc + r0 + 8 * c, v + 8 * c);
@@ -454,7 +636,7 @@ __global__ __launch_bounds__(NT) void kr_mgs_vec(const float* __restrict__ Z32, 
     }
     if (k + 1 < m) {
         if (threadIdx.x == 0) Hi[(k + 1) + (size_t)k * m] = zn;
-        const size_t vc = ((size_t)(k + 1) * N + i) * N;
+        const size_t vc = ((size_t)(k + 1) * V.nc + i) * V.cs;
         const double inv = 1.0 / zn;
 #pragma unroll
         for (int c = 0; c < E / 8; c++) {
@@ -479,7 +661,7 @@ __global__ __launch_bounds__(NT) void kr_combine_vec(Basis V, const double* __re
     for (int k = 1; k < m; k++) {
         const double ck = c[k];
         if (ck == 0.) continue;
-        const size_t vc = ((size_t)k * N + i) * N;
+        const size_t vc = ((size_t)k * V.nc + i) * V.cs;
 #pragma unroll
         for (int q = 0; q < E / 8; q++) {
             if (r0 + 8 * q >= N) continue;
@@ -512,14 +694,14 @@ static void vec_shape(int N, int& nt, int& e) {
     else { nt = 1024; e = 16; }
 }
 template <int NT, int E>
-static void launch_vec(hipStream_t s, int N, const float* z32, const double* z64, Basis V, double* H, int* closed,
-                       int m, int k, int tb) {
+static void launch_vec(hipStream_t s, int N, size_t ldz, const float* z32, const double* z64, Basis V, double* H,
+                       int* closed, int m, int k, int tb) {
     if (V.prec == 0)
-        hipLaunchKernelGGL((kr_mgs_vec<NT, E, 0>), dim3(N), dim3(NT), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+        hipLaunchKernelGGL((kr_mgs_vec<NT, E, 0>), dim3(N), dim3(NT), 0, s, z32, z64, V, H, closed, N, ldz, m, k, tb);
     else if (V.prec == 1)
-        hipLaunchKernelGGL((kr_mgs_vec<NT, E, 1>), dim3(N), dim3(NT), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+        hipLaunchKernelGGL((kr_mgs_vec<NT, E, 1>), dim3(N), dim3(NT), 0, s, z32, z64, V, H, closed, N, ldz, m, k, tb);
     else
-        hipLaunchKernelGGL((kr_mgs_vec<NT, E, 2>), dim3(N), dim3(NT), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+        hipLaunchKernelGGL((kr_mgs_vec<NT, E, 2>), dim3(N), dim3(NT), 0, s, z32, z64, V, H, closed, N, ldz, m, k, tb);
 }
 template <int NT, int E>
 static void launch_combine_vec(hipStream_t s, int N, Basis V, const double* C, double* P, double* err, int m) {
@@ -530,14 +712,14 @@ static void launch_combine_vec(hipStream_t s, int N, Basis V, const double* C, d
     else
         hipLaunchKernelGGL((kr_combine_vec<NT, E, 2>), dim3(N), dim3(NT), 0, s, V, C, P, err, N, m, 1);
 }
-static void mgs_step(hipStream_t s, int N, const float* z32, const double* z64, Basis V, double* H, int* closed,
-                     int m, int k, int tb) {
+static void mgs_step(hipStream_t s, int N, size_t ldz, const float* z32, const double* z64, Basis V, double* H,
+                     int* closed, int m, int k, int tb) {
     int nt, e;
     vec_shape(N, nt, e);
-    if (nt == 256) launch_vec<256, 8>(s, N, z32, z64, V, H, closed, m, k, tb);
-    else if (nt == 512 && e == 8) launch_vec<512, 8>(s, N, z32, z64, V, H, closed, m, k, tb);
-    else if (nt == 512) launch_vec<512, 16>(s, N, z32, z64, V, H, closed, m, k, tb);
-    else launch_vec<1024, 16>(s, N, z32, z64, V, H, closed, m, k, tb);
+    if (nt == 256) launch_vec<256, 8>(s, N, ldz, z32, z64, V, H, closed, m, k, tb);
+    else if (nt == 512 && e == 8) launch_vec<512, 8>(s, N, ldz, z32, z64, V, H, closed, m, k, tb);
+    else if (nt == 512) launch_vec<512, 16>(s, N, ldz, z32, z64, V, H, closed, m, k, tb);
+    else launch_vec<1024, 16>(s, N, ldz, z32, z64, V, H, closed, m, k, tb);
 }
 static void combine_step(hipStream_t s, int N, Basis V, const double* C, double* P, double* err, int m) {
     int nt, e;
@@ -549,14 +731,14 @@ static void combine_step(hipStream_t s, int N, Basis V, const double* C, double*
 }
 
 template <int E>
-static void launch_mgs_wide(hipStream_t s, int N, const float* z32, const double* z64, Basis V, double* H,
-                            int* closed, int m, int k, int tb) {
+static void launch_mgs_wide(hipStream_t s, int N, size_t ldz, const float* z32, const double* z64, Basis V,
+                            double* H, int* closed, int m, int k, int tb) {
     if (V.prec == 0)
-        hipLaunchKernelGGL((kr_mgs_wide<E, 0>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+        hipLaunchKernelGGL((kr_mgs_wide<E, 0>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, ldz, m, k, tb);
     else if (V.prec == 1)
-        hipLaunchKernelGGL((kr_mgs_wide<E, 1>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+        hipLaunchKernelGGL((kr_mgs_wide<E, 1>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, ldz, m, k, tb);
     else
-        hipLaunchKernelGGL((kr_mgs_wide<E, 2>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, m, k, tb);
+        hipLaunchKernelGGL((kr_mgs_wide<E, 2>), dim3(N), dim3(1024), 0, s, z32, z64, V, H, closed, N, ldz, m, k, tb);
 }
 template <int E>
 static void launch_combine_wide(hipStream_t s, int N, Basis V, const double* C, double* P, double* err, int m) {
@@ -603,10 +785,14 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
     DevBuf<double> dB, dZ64, dH, dC, dP, dErr;
     DevBuf<float> dA32, dZ32, dV32;
     DevBuf<double> dV64;
-    DevBuf<__hip_bfloat16> dAhi, dAlo, dVhi, dVlo;
+    DevBuf<__hip_bfloat16> dAil, dVil;
     DevBuf<int> dClosed;
     dB.upload(B, NN, s);
-    Basis V{nullptr, nullptr, nullptr, nullptr, prec};
+    Basis V{nullptr, nullptr, nullptr, nullptr, prec, N, (size_t)N};
+    // BF16: N padded to the product kernel's 256 with zero rows and columns
+    const int Np = prec == 2 ? (N + gk::TM - 1) / gk::TM * gk::TM : N;
+    const size_t ldz = (size_t)Np;
+    const size_t lda_il = 2 * (size_t)Np + 64;  // bf16 per row of A (the pad spreads rows over L2 channels)
     const int cb = 4 * c->cu_count;
     // A = dt*B in the compute format; Z_0 = dt*B*V_0 = dt*B
     if (prec == 0) {
@@ -622,15 +808,18 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
         hipLaunchKernelGGL(kr_to_f32, dim3(cb), dim3(256), 0, s, dB.p, dA32.p, NN, dt);
         HIPCHECK(hipMemcpyAsync(dZ32.p, dA32.p, NN * sizeof(float), hipMemcpyDeviceToDevice, s));
     } else {
-        dAhi.alloc(NN);
-        dAlo.alloc(NN);
-        dZ32.alloc(NN);
-        dVhi.alloc(NN * m);
-        dVlo.alloc(NN * m);
-        V.hi = dVhi.p;
-        V.lo = dVlo.p;
-        hipLaunchKernelGGL(kr_to_bf16x2, dim3(cb), dim3(256), 0, s, dB.p, dAhi.p, dAlo.p, NN, dt);
-        hipLaunchKernelGGL(kr_to_f32, dim3(cb), dim3(256), 0, s, dB.p, dZ32.p, NN, dt);
+        const size_t NNp = (size_t)Np * Np;
+        dAil.alloc((size_t)Np * lda_il);
+        dZ32.alloc(NNp);
+        V.nc = Np;
+        V.cs = (size_t)Np + 32;
+        dVil.alloc((size_t)m * Np * 2 * V.cs);
+        V.hi = dVil.p;
+        V.lo = dVil.p + 8;
+        hipLaunchKernelGGL(kr_to_bf16_il, dim3(Np / 32, Np / 32), dim3(256), 0, s, dB.p, dAil.p, N, lda_il, dt);
+        hipLaunchKernelGGL(kr_to_f32_ld, dim3(cb), dim3(256), 0, s, dB.p, dZ32.p, N, ldz, dt);
+        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&kr_bf16x3_gemm),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, gk::LDS));
     }
     HIPCHECK(hipGetLastError());
     dH.alloc((size_t)N * m * m);
@@ -644,10 +833,8 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
     // columns of a basis that closes early stay zero
     if (prec == 0) HIPCHECK(hipMemsetAsync(dV64.p, 0, NN * m * sizeof(double), s));
     if (prec == 1) HIPCHECK(hipMemsetAsync(dV32.p, 0, NN * m * sizeof(float), s));
-    if (prec == 2) {
-        HIPCHECK(hipMemsetAsync(dVhi.p, 0, NN * m * sizeof(__hip_bfloat16), s));
-        HIPCHECK(hipMemsetAsync(dVlo.p, 0, NN * m * sizeof(__hip_bfloat16), s));
-    }
+    // BF16: also the padding rows and columns, which the product reads
+    if (prec == 2) HIPCHECK(hipMemsetAsync(dVil.p, 0, (size_t)m * Np * 2 * V.cs * sizeof(__hip_bfloat16), s));
     // events bracket the build and every GEMM; they are read after the final
     // synchronisation, so the host never waits inside the loop
     struct EvGuard {
@@ -676,16 +863,10 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
                 RBCHECK(rocblas_sgemm(hb, rocblas_operation_none, rocblas_operation_none, N, N, N, &onef, dA32.p,
                                       N, dV32.p + vo, N, &zerof, dZ32.p, N));
             } else {
-                const __hip_bfloat16* As[3] = {dAhi.p, dAhi.p, dAlo.p};
-                const __hip_bfloat16* Vs[3] = {dVhi.p + vo, dVlo.p + vo, dVhi.p + vo};
-                for (int q = 0; q < 3; q++) {
-                    const float beta = q == 0 ? zerof : onef;
-                    RBCHECK(rocblas_gemm_ex(hb, rocblas_operation_none, rocblas_operation_none, N, N, N, &onef,
-                                            As[q], rocblas_datatype_bf16_r, N, Vs[q], rocblas_datatype_bf16_r, N,
-                                            &beta, dZ32.p, rocblas_datatype_f32_r, N, dZ32.p,
-                                            rocblas_datatype_f32_r, N, rocblas_datatype_f32_r,
-                                            rocblas_gemm_algo_standard, 0, 0));
-                }
+                // A_hi V_hi + A_hi V_lo + A_lo V_hi on MFMA; V_k's columns
+                // start at logical (k Np) cs, stored at twice that
+                hipLaunchKernelGGL(kr_bf16x3_gemm, dim3((Np / gk::TM) * (Np / gk::TN)), dim3(gk::NT), gk::LDS, s,
+                                   dAil.p, lda_il, dVil.p + 2 * (size_t)k * Np * V.cs, 2 * V.cs, dZ32.p, Np);
             }
             HIPCHECK(hipEventRecord(eg.ev[3 + 2 * gemms], s));
             gemms++;
@@ -693,13 +874,13 @@ extern "C" int bte_krylov_build(bte_ctx* ctx, int N, const double* B, double dt,
         const float* z32 = prec == 0 ? nullptr : dZ32.p;
         const double* z64 = prec == 0 ? dZ64.p : nullptr;
         if (vec) {
-            mgs_step(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode);
+            mgs_step(s, N, ldz, z32, z64, V, dH.p, dClosed.p, m, k, mode);
         } else switch (WE) {
-            case 1: launch_mgs_wide<1>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
-            case 2: launch_mgs_wide<2>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
-            case 4: launch_mgs_wide<4>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
-            case 8: launch_mgs_wide<8>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
-            default: launch_mgs_wide<16>(s, N, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 1: launch_mgs_wide<1>(s, N, ldz, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 2: launch_mgs_wide<2>(s, N, ldz, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 4: launch_mgs_wide<4>(s, N, ldz, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            case 8: launch_mgs_wide<8>(s, N, ldz, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
+            default: launch_mgs_wide<16>(s, N, ldz, z32, z64, V, dH.p, dClosed.p, m, k, mode); break;
         }
         HIPCHECK(hipGetLastError());
     }
