@@ -95,13 +95,24 @@ __global__ __launch_bounds__(gk::NT) void kr_bf16x3_gemm(const __hip_bfloat16* _
     using namespace gk;
     extern __shared__ __align__(16) unsigned char lds[];
     const int tmn = Np / TM, tnn = Np / TN, total = tmn * tnn;
-    // workgroups go round-robin to the 8 XCDs: each XCD takes a contiguous
-    // eighth of the tiles, walked in bands of 4 row tiles, so the 32 tiles
-    // an XCD runs at once share 4 row panels and 8 column panels in its L2
+    // workgroups go round-robin to the 8 XCDs (workgroup b to XCD b % 8).
+    // From 16 x 16 tiles up, XCD x owns a quarter of the tile rows times half
+    // of the tile columns and walks it in groups of 4 row tiles x 8 column
+    // tiles: the 32 tiles it runs at once share 4 row panels and 8 column
+    // panels in its L2, and over the whole product it reads a quarter of A and
+    // half of V (a band of whole tile rows per XCD would read all of V).
+    // Smaller grids: contiguous eighths in bands of 4 row tiles, or row order.
     const int b = blockIdx.x;
     const int u = (total % 8 == 0) ? (b % 8) * (total / 8) + b / 8 : b;
     int tr, tc;
-    if (tmn % 4 == 0) {
+    if (tmn % 16 == 0 && tnn % 16 == 0) {
+        const int x = b % 8, t = b / 8;
+        const int bm = tmn / 4, bn = tnn / 2;  // the XCD's block of tiles
+        const int gpr = bn / 8;                 // groups of 8 columns per band of 4 rows
+        const int grp = t / 32, w = t % 32;
+        tr = (x / 2) * bm + (grp / gpr) * 4 + w % 4;
+        tc = (x % 2) * bn + (grp % gpr) * 8 + w / 4;
+    } else if (tmn % 4 == 0) {
         const int band = u / (4 * tnn), w = u % (4 * tnn);
         tr = band * 4 + w % 4;
         tc = w / 4;
