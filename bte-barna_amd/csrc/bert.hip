@@ -274,13 +274,17 @@ __device__ inline Pick0 first_pick(const BertParams& P, Pcg32& rng, int mode, in
 // counts in one 64-bit word, so that one atomic tallies a particle and the
 // cancellation knows how many candidates of each sign a bin holds.
 __device__ inline int h_net(unsigned long long v) { return (int)(uint32_t)v - (int)(uint32_t)(v >> 32); }
-__device__ inline unsigned long long h_unit(int sign) { return sign > 0 ? 1ull : (1ull << 32); }
+// one more particle of the given sign in histogram cell h: a 32-bit atomic on
+// the half that changes (positive count in the low word, negative in the high)
+__device__ inline void h_add(unsigned long long* h, int sign) {
+    atomicAdd(reinterpret_cast<unsigned int*>(h) + (sign > 0 ? 0 : 1), 1u);
+}
 
 // A child written in this step enters the (box, mode) histogram when that
 // step feeds a cancellation or a frame (H not null) and the box sums behind
 // calculate_prop (sbox in LDS, not null when the boxes fit there).
 __device__ inline void tally_child(const BertParams& P, unsigned long long* H, double* sbox, int box, int nm, int ns) {
-    if (H) atomicAdd(H + (size_t)box * P.n + nm, h_unit(ns));
+    if (H) h_add(H + (size_t)box * P.n + nm, ns);
     if (sbox) {
         const double2 v = P.vel[nm];
         atomicAdd(sbox + box * 3 + 0, (double)ns);
@@ -1183,7 +1187,7 @@ __global__ __launch_bounds__(256) void bert_hist_kernel(BertParams P, int cur) {
         const BPart p = P.pin[(uint64_t)sg * P.segcap + ((uint32_t)i - (sg > 0 ? pre[sg - 1] : 0u))];
         int box = (int)(p.boxsign >> 1);
         int sign = (p.boxsign & 1u) ? -1 : 1;
-        atomicAdd(H + (size_t)box * P.n + p.mode, h_unit(sign));
+        h_add(H + (size_t)box * P.n + p.mode, sign);
     }
 }
 
