@@ -210,8 +210,10 @@ def rta_extra(pkg, ctx, cpu_particles=0):
 
 def batch_extra(pkg, ctx):
     """64 independent RTAMC2D runs (widths 50 nm - 2 um, 100 - 500 K, 1e5 - 1e7
-    particles) in one tuned, graph-replayed launch, against the same runs
-    launched one after another."""
+    particles) in one graph-replayed launch with FP64 LDS tallies per run
+    (tally mode 3), against the same batch with FP64 device-atomic tallies
+    (mode 0, tuned launch shape) and against the 64 runs launched one after
+    another, each with its best single-run tallies (mode 2)."""
     W = pkg.workloads
     c = W.config3()
     m = c["material"]
@@ -221,17 +223,27 @@ def batch_extra(pkg, ctx):
         runs.append(dict(geom=pkg.Geometry(ctx, r["rect"], r["kind"], r["Teq"]), T0=r["T0"],
                          nparticles=r["nparticles"], seed=r["seed"], wscale=r["wscale"]))
     kw = dict(thickness=c["thickness"], dt=c["dt"], maxtime=c["maxtime"], sigma_bounce=c["sigma_bounce"])
-    b = pkg.RtaBatch(ctx, modes, runs, use_graph=True, **kw)
-    block, blocks = b.tune(fraction=0.02)
+    b = pkg.RtaBatch(ctx, modes, runs, use_graph=True, tally="fp64-lds-runs", **kw)
+    b.execute()  # captures the graph
     ctx.sync()
     t0 = time.perf_counter()
     res, rep = b.execute()
     t_batch = time.perf_counter() - t0
     b.close()
+    # the same batch with device-atomic tallies and a tuned launch shape
+    a = pkg.RtaBatch(ctx, modes, runs, use_graph=True, **kw)
+    block, blocks = a.tune(fraction=0.02)
+    ctx.sync()
+    t0 = time.perf_counter()
+    res0, rep0 = a.execute()
+    t_atomics = time.perf_counter() - t0
+    a.close()
+    same = (int(rep0["events"]), int(rep0["crossings"])) == (int(rep["events"]), int(rep["crossings"]))
+    dT = max(float(np.abs(x["T"] - y["T"]).max()) for x, y in zip(res, res0))
     # plain form: each run in its own launch
     t_seq, ev_seq = 0.0, 0
     for r in runs:
-        s = pkg.RtaBatch(ctx, modes, [r], **kw)
+        s = pkg.RtaBatch(ctx, modes, [r], tally="fp64-lds", **kw)
         t0 = time.perf_counter()
         _, rr = s.execute()
         t_seq += time.perf_counter() - t0
@@ -242,8 +254,11 @@ def batch_extra(pkg, ctx):
     return {"workload": "64 RTAMC2D runs: widths 50 nm-2 um (log), 100-500 K, 1e5-1e7 particles (seeded), "
                         "3000 modes, 1 um long, 100 bins, dt 25 ps, maxtime 5 ns, tau ~ 1/T",
             "particles": int(sum(r["nparticles"] for r in runs)), "events": int(rep["events"]),
-            "batched_s": t_batch, "batched_kernel_ms": rep["kernel_ms"], "launch": [block, blocks],
-            "sequential_s": t_seq, "batched_events_per_s": rep["events"] / t_batch,
+            "batched_s": t_batch, "batched_kernel_ms": rep["kernel_ms"], "tally": "fp64-lds-runs (mode 3)",
+            "batched_events_per_s": rep["events"] / t_batch,
+            "batched_fp64_atomics_s": t_atomics, "fp64_atomics_launch": [block, blocks],
+            "same_histories_as_fp64_atomics": bool(same), "max_abs_T_diff_vs_fp64_atomics_K": dT,
+            "sequential_s": t_seq, "sequential_tally": "fp64-lds (mode 2) per run",
             "speedup_vs_sequential": t_seq / t_batch,
             "kappa_W_per_mK": {"first": kappa[0], "last": kappa[-1], "min": min(kappa), "max": max(kappa)}}
 

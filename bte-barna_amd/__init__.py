@@ -264,9 +264,12 @@ class RtaBatch(_Handle):
 
     tally: "fp64" (device atomics, the default), "fp32-lds" (per-workgroup
     FP32 partial sums in LDS added into the FP64 tallies at the end; raises
-    when the regular boxes' cells do not fit in LDS) or "fp64-lds" (FP64
+    when the regular boxes' cells do not fit in LDS), "fp64-lds" (FP64
     per-workgroup sums of T, jx and jy in LDS for the leading time bins that
-    fit, later bins by device atomics; no dT/dp)."""
+    fit, later bins by device atomics; no dT/dp) or "fp64-lds-runs" (the same
+    with each workgroup's LDS holding the rows of one run at a time: work
+    items of one run's particles, so batches of many runs keep as many time
+    bins in LDS as a single run; no dT/dp)."""
     _destroy = "bte_rta_batch_destroy"
 
     def __init__(self, ctx, modes, runs, thickness, dt, maxtime, sigma_bounce=0.25, use_graph=False,
@@ -290,7 +293,7 @@ class RtaBatch(_Handle):
         self._keep = (ctx, modes, [r["geom"] for r in runs])
         check(lib().bte_rta_batch_create(ctx.h, modes.h, self.nruns, self.descs, thickness, dt, maxtime,
                                          sigma_bounce, int(use_graph), C.byref(self.h)))
-        modes_ = {"fp64": 0, "fp32-lds": 1, "fp64-lds": 2}
+        modes_ = {"fp64": 0, "fp32-lds": 1, "fp64-lds": 2, "fp64-lds-runs": 3}
         if tally not in modes_:
             raise ValueError(f"tally must be one of {sorted(modes_)}")
         if tally != "fp64":

@@ -35,6 +35,16 @@ struct RtaRunDev {
     float spec[4];                         // specular probability per side
 };
 
+// Tally mode 3: a contiguous range of one run's particle ids, taken whole by
+// one workgroup, and the run's rows in the workgroup's FP64 LDS block.
+struct RtaItem {
+    uint64_t lo, hi;     // particle ids [lo, hi)
+    int32_t lrow0;       // global LDS row of the run's first regular box
+    int32_t lrows;       // the run's regular boxes
+    int32_t lbins;       // leading time bins of each row held in LDS
+    int32_t run;
+};
+
 struct RtaParams {
     const double4* modes;  // (vx, vy, w, omega)
     const double2* ivel;   // (1/vx, 1/vy), 0 where the component is 0
@@ -84,6 +94,9 @@ struct RtaParams {
     int32_t lbins, pad2;
     double* part;
     double idt;  // nt / maxtime: time -> bin estimate on the uniform grid
+    // tally mode 3: the work items, taken in order through counters[6]
+    const RtaItem* items;
+    int32_t nitems, pad3;
 };
 
 // Per-XCD copies of the f64 tallies. Device-wide f64 atomics to a few
@@ -121,7 +134,16 @@ struct TallyAcc {
 template <int TM>
 __device__ inline void acc_flush(const RtaParams& P, double* L, TallyAcc& a) {
     if (a.box < 0) return;
-    if (TM == 2 && a.bin < P.lbins) {
+    // mode 3: the block holds [3][lrows][lbins] of the current item's run,
+    // after a header of ints (lrows, lbins, first global row)
+    const int* hd = reinterpret_cast<const int*>(L) - 4;
+    if (TM == 3 && a.bin < hd[1]) {
+        const uint32_t lc = (uint32_t)hd[0] * (uint32_t)hd[1];
+        const uint32_t i = (P.box_lrow[a.box] - (uint32_t)hd[2]) * (uint32_t)hd[1] + (uint32_t)a.bin;
+        atomicAdd(L + i, a.g);
+        atomicAdd(L + lc + i, a.x);
+        atomicAdd(L + 2 * lc + i, a.y);
+    } else if (TM == 2 && a.bin < P.lbins) {
         const uint32_t lc = (uint32_t)P.lrows * (uint32_t)P.lbins;
         const uint32_t i = P.box_lrow[a.box] * (uint32_t)P.lbins + (uint32_t)a.bin;
         atomicAdd(L + i, a.g);
@@ -142,14 +164,15 @@ __device__ inline void acc_flush(const RtaParams& P, double* L, TallyAcc& a) {
 // sums in LDS (lds), added into the FP64 tallies once at the end of the
 // kernel (lt_flush). 2: the workgroup's FP64 partial sums of gz, jx and jy
 // for the first lbins time bins in LDS, later bins by device atomics.
-// Modes 0 (without dT/dp) and 2 merge consecutive contributions to one cell
-// in the lane's accumulator.
+// 3: as 2 with the LDS block holding only the rows of the run of the
+// workgroup's current work item. Modes 0 (without dT/dp), 2 and 3 merge
+// consecutive contributions to one cell in the lane's accumulator.
 template <bool GRAD, int TM>
 __device__ inline void tally_segment(const RtaParams& P, void* lds, TallyAcc& acc, int box, double t0, double t1,
                                      int sign, double vx, double vy, int gmask, const float* score, int mode) {
     int b0 = time_bin(P.timegrid, P.nt, P.idt, t0), b1 = time_bin(P.timegrid, P.nt, P.idt, t1);
     double delta = t1 - t0;
-    if constexpr (TM == 2 || (TM == 0 && !GRAD)) {
+    if constexpr (TM == 2 || TM == 3 || (TM == 0 && !GRAD)) {
         // spectral locations tally per mode; the Gamma-broadened spectra are
         // one matrix product after the run (the reference adds the broadened
         // weights of every segment, RTA_MC2D.cpp:718-766)
@@ -597,6 +620,89 @@ __global__ __launch_bounds__(TMK ? 768 : 512) void rta_history_kernel(RtaParams 
     }
 }
 
+// Tally mode 3: persistent workgroups (768 threads, one per CU) take work
+// items in order; the histories of an item (ids of one run) run with the
+// run's rows, as many leading time bins as fit, in an FP64 LDS block, which
+// is added into the XCD's tally copy once the item's last history has
+// ended. Later bins go to the copies by device atomics, as under mode 2. A
+// batch whose runs each fit a block of their own thus gets LDS sums where
+// one block for all runs would not hold a single time bin.
+template <bool POLY>
+__global__ __launch_bounds__(768) void rta_history_items_kernel(RtaParams P) {
+    int* hd = reinterpret_cast<int*>(rta_lt);  // lrows, lbins, first row, item
+    double* L = reinterpret_cast<double*>(rta_lt) + 2;
+    __shared__ unsigned long long next_id;
+    TallyAcc A;
+    RtaState S;
+    S.alive = 0;
+    RtaCounts C;
+    const uint64_t xc = xcd_id() % RTA_NCOPY;
+    for (;;) {
+        __syncthreads();  // the previous item's block has been read
+        if (threadIdx.x == 0) {
+            const unsigned long long it = atomicAdd(P.counters + 6, 1ULL);
+            hd[3] = it < (unsigned long long)P.nitems ? (int)it : P.nitems;
+            if (it < (unsigned long long)P.nitems) {
+                const RtaItem I = P.items[it];
+                hd[0] = I.lrows;
+                hd[1] = I.lbins;
+                hd[2] = I.lrow0;
+                next_id = I.lo;
+            }
+        }
+        __syncthreads();
+        const int it = hd[3];
+        if (it >= P.nitems) break;
+        const RtaItem I = P.items[it];
+        const uint32_t lc = (uint32_t)I.lrows * (uint32_t)I.lbins, nf = 3u * lc;
+        for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) L[j] = 0.;
+        __syncthreads();
+        // ids of the item from per-wave pools of 64, taken from next_id
+        unsigned long long pool_lo = 0, pool_hi = 0;
+        for (;;) {
+            if (S.alive) rta_watch(P, S, C);
+            unsigned long long pid = ~0ULL;
+            unsigned long long want = __ballot(!S.alive);
+            while (want) {
+                if (pool_lo == pool_hi) {
+                    unsigned long long base = 0;
+                    const int leader = __ffsll((long long)want) - 1;
+                    if ((int)(threadIdx.x & 63) == leader) base = atomicAdd(&next_id, 64ULL);
+                    base = __shfl(base, leader, 64);
+                    pool_lo = base;
+                    pool_hi = base + 64;
+                }
+                const unsigned long long below = want & ((1ULL << (threadIdx.x & 63)) - 1ULL);
+                const unsigned long long rank = (unsigned long long)__popcll(below);
+                const unsigned long long take = min((unsigned long long)__popcll(want), pool_hi - pool_lo);
+                const bool mine = (want >> (threadIdx.x & 63)) & 1ULL;
+                if (mine && rank < take) pid = pool_lo + rank;
+                pool_lo += take;
+                want = __ballot(!S.alive && pid == ~0ULL);
+            }
+            if (!S.alive) {
+                if (pid >= I.hi) break;
+                rta_start<false, POLY>(P, S, pid);
+            }
+            rta_event<false, 3, POLY>(P, L, A, S, C);
+        }
+        acc_flush<3>(P, L, A);
+        __syncthreads();  // every history of the item has ended and added its last cell
+        for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) {
+            const double v = L[j];
+            if (v == 0.) continue;
+            const uint32_t a = j / lc, r = j - a * lc, row = r / (uint32_t)I.lbins;
+            const uint64_t cell = P.lrow_cell[I.lrow0 + row] + (r - row * (uint32_t)I.lbins);
+            atomicAdd(P.gz + xc * 3 * P.cells + (uint64_t)a * P.cells + cell, v);
+        }
+    }
+    block_add_u64(P.counters + 0, C.events);
+    block_add_u64(P.counters + 1, C.crossings);
+    block_add_u64(P.counters + 3, C.stuck);
+    block_add_u64(P.counters + 4, C.badmirror);
+    block_add_u64(P.counters + 5, C.badtrans);
+}
+
 // clears the tallies and counters (a kernel rather than a memset so that the
 // launch sequence can be captured in a graph)
 __global__ __launch_bounds__(256) void rta_clear_kernel(double* a, uint64_t n, unsigned long long* cnt) {
@@ -693,6 +799,12 @@ struct RtaBatch {
     DevBuf<uint32_t> d_box_lrow;
     DevBuf<uint64_t> d_lrow_cell;
     DevBuf<double> d_part;  // mode 2: [blocks][3][lrows][lbins]
+    // 3: per run, the first LDS row and the rows of its regular boxes; the
+    // work items and the LDS bytes of the largest run block (with header)
+    std::vector<int32_t> run_lrow0, run_lrows;
+    std::vector<RtaItem> items;
+    DevBuf<RtaItem> d_items;
+    size_t l3_bytes = 0;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -845,6 +957,8 @@ static RtaParams batch_params(RtaBatch* b, const RtaRunDev* runs, uint64_t total
     P.lbins = (int32_t)b->lbins;
     P.part = b->d_part.p;
     P.idt = (double)(b->ntime - 1) / b->maxtime;
+    P.items = b->d_items.p;
+    P.nitems = (int32_t)b->items.size();
     return P;
 }
 
@@ -857,7 +971,12 @@ static void launch_batch(RtaBatch* b, const RtaParams& P, int block, int blocks,
     if (b->nloc)
         hipLaunchKernelGGL(rta_clear_kernel, dim3(1024), dim3(256), 0, s, b->d_spec.p,
                            (uint64_t)b->nloc * (b->ntime - 1) * b->m->n, b->d_counters.p);
-    if (b->tally == 2) {
+    if (b->tally == 3) {
+        if (b->poly)
+            hipLaunchKernelGGL(rta_history_items_kernel<true>, dim3(blocks), dim3(block), b->l3_bytes, s, P);
+        else
+            hipLaunchKernelGGL(rta_history_items_kernel<false>, dim3(blocks), dim3(block), b->l3_bytes, s, P);
+    } else if (b->tally == 2) {
         if (b->poly)
             hipLaunchKernelGGL((rta_history_kernel<false, 2, true>), dim3(blocks), dim3(block), b->lg_bytes, s, P);
         else
@@ -968,6 +1087,7 @@ static void batch_create(RtaBatch* b) {
         b->box0.push_back(bo);
         b->nbox.push_back(nb);
         b->cell0.push_back(b->cells);
+        b->run_lrow0.push_back((int32_t)b->lrows);
         for (int i = 0; i < nb; i++) {
             DevBox d = g->dboxes[i];
             if (d.kind == 2 && d.tgt < 0)
@@ -1008,6 +1128,7 @@ static void batch_create(RtaBatch* b) {
         }
         cseg.insert(cseg.end(), g->cseg.begin(), g->cseg.end());
         b->cells += (uint64_t)nb * nt;
+        b->run_lrows.push_back((int32_t)b->lrows - b->run_lrow0.back());
         // BE_derivative sampler at the run temperature (sampling.cpp:47-68)
         for (int i = 0; i < n; i++) pmf[i] = m->omega[i] <= 0. ? 0. : h_be_kernel(m->omega[i], D.T0) * m->w[i];
         if (!build_alias(pmf.data(), n, th.data(), al.data())) throw Error(BTE_EINVAL, "empty scattering pmf");
@@ -1205,9 +1326,61 @@ static void batch_execute(RtaBatch* b, double* T_out, double* jx_out, double* jy
 // CU) and keeps the fastest.
 // FP32 LDS tallies on (mode 1) or off (mode 0); see bte_rta_batch_set_tally.
 static void batch_set_tally(RtaBatch* b, int mode) {
-    if (mode < 0 || mode > 2)
-        throw Error(BTE_EINVAL, "tally mode must be 0 (FP64), 1 (FP32 LDS) or 2 (FP64 T in LDS)");
-    if (mode == 2) {
+    if (mode < 0 || mode > 3)
+        throw Error(BTE_EINVAL,
+                    "tally mode must be 0 (FP64 atomics), 1 (FP32 LDS), 2 (FP64 LDS, leading time bins) or 3 "
+                    "(FP64 LDS per run, leading time bins)");
+    if (mode == 3) {
+        if (b->nloc) throw Error(BTE_EINVAL, "LDS tallies do not cover spectral locations");
+        if (b->any_grad) throw Error(BTE_EINVAL, "tally mode 3 does not carry dT/dp; use mode 1 or 0");
+        const int nt = b->ntime - 1;
+        // the block: a 16-byte header, then [3][rows][bins] doubles; a margin
+        // for the kernel's static LDS
+        const size_t avail = b->ctx->lds_per_block - 1024;
+        b->l3_bytes = 16 * 5 * sizeof(uint64_t);
+        std::vector<int32_t> lb(b->nruns, 0);
+        for (int r = 0; r < b->nruns; r++) {
+            const size_t rows = (size_t)b->run_lrows[r];
+            if (rows == 0) continue;
+            if (16 + 3 * sizeof(double) * rows > avail)
+                throw Error(BTE_EINVAL, "run " + std::to_string(r) + ": FP64 LDS tallies need " +
+                                            std::to_string(16 + 3 * sizeof(double) * rows) +
+                                            " bytes of LDS for one time bin, the device has " +
+                                            std::to_string(avail) + " per workgroup");
+            lb[r] = (int32_t)std::min<size_t>((size_t)nt, (avail - 16) / (3 * sizeof(double) * rows));
+            b->l3_bytes = std::max(b->l3_bytes, 16 + 3 * sizeof(double) * rows * (size_t)lb[r]);
+        }
+        HIPCHECK(hipFuncSetAttribute(b->poly ? reinterpret_cast<const void*>(&rta_history_items_kernel<true>)
+                                             : reinterpret_cast<const void*>(&rta_history_items_kernel<false>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->l3_bytes));
+        b->block = 768;
+        b->blocks = b->ctx->cu_count;
+        // about four items per workgroup, at least 4096 ids each; the runs
+        // with most particles first, so that the last items taken are short
+        const uint64_t isz = std::max<uint64_t>(4096, (b->total + 4 * (uint64_t)b->blocks - 1) / (4 * (uint64_t)b->blocks));
+        std::vector<int> order(b->nruns);
+        for (int r = 0; r < b->nruns; r++) order[r] = r;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int x, int y) { return b->runs[x].count > b->runs[y].count; });
+        b->items.clear();
+        for (int r : order) {
+            const RtaRunDev& R = b->runs[r];
+            if (R.count == 0) continue;
+            const uint64_t k = (R.count + isz - 1) / isz;
+            for (uint64_t q = 0; q < k; q++) {
+                RtaItem I{};
+                I.lo = R.first + R.count * q / k;
+                I.hi = R.first + R.count * (q + 1) / k;
+                I.lrow0 = b->run_lrow0[r];
+                I.lrows = b->run_lrows[r];
+                I.lbins = lb[r];
+                I.run = r;
+                b->items.push_back(I);
+            }
+        }
+        // the same buffer while the list does not grow: a captured graph holds its address
+        if (!b->items.empty()) b->d_items.upload(b->items.data(), b->items.size(), b->ctx->stream);
+    } else if (mode == 2) {
         if (b->nloc) throw Error(BTE_EINVAL, "LDS tallies do not cover spectral locations");
         if (b->any_grad) throw Error(BTE_EINVAL, "tally mode 2 does not carry dT/dp; use mode 1 or 0");
         if (b->lbins == 0)

@@ -243,13 +243,16 @@ int bte_rta_batch_create(bte_ctx* ctx, const bte_modes* m, int nruns, const bte_
  * arrays in LDS (T, jx, jy and each differentiated side over the regular
  * boxes' cells) and adds them into the FP64 tallies once at the end; one
  * 768-thread workgroup per CU. BTE_EINVAL when those arrays exceed the
- * device's LDS per workgroup. mode 2: the same for T alone in FP64
- * (regular boxes x (ntime - 1) x 8 bytes of LDS), jx and jy by FP64 device
- * atomics: FP64 sums throughout, a third of the device atomics of mode 0.
- * Not with dT/dp (BTE_EINVAL). */
+ * device's LDS per workgroup. mode 2: FP64 sums of T, jx and jy in LDS for
+ * as many leading time bins of all regular boxes as fit, later bins by FP64
+ * device atomics. mode 3: as mode 2 with each workgroup working through
+ * items of one run's particles at a time and its LDS holding only that
+ * run's boxes (added into the tallies after each item), so a batch of many
+ * runs keeps as many bins in LDS as one run. Modes 2 and 3: FP64 sums
+ * throughout; not with dT/dp or spectral locations (BTE_EINVAL). */
 int bte_rta_batch_set_tally(bte_rta_batch* b, int mode);
 /* Times the batch scaled to `fraction` of each run's particles over a set of
- * launch shapes and keeps the fastest (no-op under tally modes 1 and 2). */
+ * launch shapes and keeps the fastest (no-op under tally modes 1, 2 and 3). */
 int bte_rta_batch_tune(bte_rta_batch* b, double fraction, int* block, int* blocks);
 /* T, jx, jy: per run r, [ntime-1][nbox_r] blocks one after another; dTdp (may
  * be NULL): per run [4 sides][ntime-1][nbox_r]; Eeff (may be NULL): [nruns]. */

@@ -233,6 +233,51 @@ def test_fp64_lds_tallies_follow_the_same_histories(pkg, ctx):
                      tally="fp64-lds", **kw)
 
 
+def test_fp64_run_lds_tallies_follow_the_same_histories(pkg, ctx):
+    """Tally mode 3 (FP64 LDS sums per work item of one run's particles, the
+    block holding that run's boxes only) on a batch of five runs of different
+    geometries (10 to 80 bins), temperatures, rate factors, specular sides
+    and particle counts (one run below one item of 4096 ids, others split in
+    up to 30 items): the same histories as mode 0 (equal event and crossing
+    counts, T, jx, jy equal up to the order of the FP64 sums) on two graph
+    replays; dT/dp is refused before any launch."""
+    W = pkg.workloads
+    mat = W.material(300, seed=3, mirror=True)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    specs = [(400.0, 40.0, 40, 300.0, 60000, 21, 1.0, (0.0, 0.0, 0.0, 0.0)),
+             (200.0, 20.0, 20, 150.0, 5000, 22, 0.5, (0.0, 0.0, 0.0, 0.0)),
+             (800.0, 100.0, 80, 450.0, 120000, 23, 1.5, (0.0, 0.0, 0.0, 0.0)),
+             (100.0, 50.0, 10, 300.0, 700, 24, 1.0, (0.0, 0.0, 0.0, 0.0)),
+             (400.0, 40.0, 40, 300.0, 30000, 25, 1.0, (0.0, 1.0, 1.0, 0.0))]
+    runs = []
+    for (length, width, nb, T, N, seed, ws, sp) in specs:
+        rect, kind, teq, _ = W.ribbon(length, width, nb, T, reservoirs=(T + 0.5, T - 0.5))
+        runs.append(dict(geom=pkg.Geometry(ctx, rect, kind, teq), T0=T, nparticles=N, seed=seed, wscale=ws,
+                         specularity=sp))
+    kw = dict(thickness=1.0, dt=5.0, maxtime=1000.0, sigma_bounce=2.0)
+    ref_b = pkg.RtaBatch(ctx, modes, runs, **kw)
+    ref, rrep = ref_b.execute()
+    lds = pkg.RtaBatch(ctx, modes, runs, use_graph=True, tally="fp64-lds-runs", **kw)
+    for replay in range(2):
+        got, grep = lds.execute()
+        print(f"replay {replay}: events {grep['events']} crossings {grep['crossings']}", flush=True)
+        assert (grep["events"], grep["crossings"]) == (rrep["events"], rrep["crossings"])
+        assert grep["block"] == 768
+        for r, (a, b) in enumerate(zip(got, ref)):
+            T0 = specs[r][3]
+            for key, off in (("T", T0), ("jx", 0.0), ("jy", 0.0)):
+                scale = np.abs(b[key] - off).max()
+                assert scale > 0, (r, key)
+                np.testing.assert_allclose(a[key] - off, b[key] - off, rtol=0, atol=1e-10 * scale)
+        ref_b.execute()  # eager work between replays
+        ctx.sync()
+    ref_b.close()
+    lds.close()
+    with pytest.raises(Exception, match="dT/dp"):
+        pkg.RtaBatch(ctx, modes, [dict(runs[0], specularity=(0.0, 0.5, 0.5, 0.0), grad=True)],
+                     tally="fp64-lds-runs", **kw)
+
+
 def test_single_run_switches_tally_mode_around_spectral_locations(pkg, ctx):
     """bte_rta_execute keeps T, jx and jy in FP64 LDS sums (tally mode 2)
     unless spectral locations are set (then mode 0, device atomics, with the
