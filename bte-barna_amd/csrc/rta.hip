@@ -651,10 +651,12 @@ __global__ __launch_bounds__(768) void rta_history_items_kernel(RtaParams P) {
             }
         }
         __syncthreads();
-        const int it = hd[3];
+        const int it = __builtin_amdgcn_readfirstlane(hd[3]);
         if (it >= P.nitems) break;
-        const RtaItem I = P.items[it];
-        const uint32_t lc = (uint32_t)I.lrows * (uint32_t)I.lbins, nf = 3u * lc;
+        // the item's fields as uniform scalars
+        const uint64_t ihi = P.items[it].hi;
+        const uint32_t lbins = (uint32_t)P.items[it].lbins, lrow0 = (uint32_t)P.items[it].lrow0;
+        const uint32_t lc = (uint32_t)P.items[it].lrows * lbins, nf = 3u * lc;
         for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) L[j] = 0.;
         __syncthreads();
         // ids of the item from per-wave pools of 64, taken from next_id
@@ -681,7 +683,7 @@ __global__ __launch_bounds__(768) void rta_history_items_kernel(RtaParams P) {
                 want = __ballot(!S.alive && pid == ~0ULL);
             }
             if (!S.alive) {
-                if (pid >= I.hi) break;
+                if (pid >= ihi) break;
                 rta_start<false, POLY>(P, S, pid);
             }
             rta_event<false, 3, POLY>(P, L, A, S, C);
@@ -691,8 +693,8 @@ __global__ __launch_bounds__(768) void rta_history_items_kernel(RtaParams P) {
         for (uint32_t j = threadIdx.x; j < nf; j += blockDim.x) {
             const double v = L[j];
             if (v == 0.) continue;
-            const uint32_t a = j / lc, r = j - a * lc, row = r / (uint32_t)I.lbins;
-            const uint64_t cell = P.lrow_cell[I.lrow0 + row] + (r - row * (uint32_t)I.lbins);
+            const uint32_t a = j / lc, r = j - a * lc, row = r / lbins;
+            const uint64_t cell = P.lrow_cell[lrow0 + row] + (r - row * lbins);
             atomicAdd(P.gz + xc * 3 * P.cells + (uint64_t)a * P.cells + cell, v);
         }
     }
