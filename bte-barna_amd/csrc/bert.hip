@@ -930,7 +930,18 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
             const double* cdf = P.wcdf + (size_t)b * P.n;
             const double u = rng.uniform();
             int l = P.wguide[(size_t)b * BERT_GUIDE + guide_key(u, (double)BERT_GUIDE)];
-            while (l < P.n - 1 && cdf[l] < u) l++;
+            // first l at or after the guide with cdf[l] >= u (the last mode when
+            // none): four entries per round of loads instead of one
+            const int nl = P.n - 1;
+            for (;;) {
+                const int l1 = min(l + 1, nl), l2 = min(l + 2, nl), l3 = min(l + 3, nl);
+                const double c0 = cdf[l], c1 = cdf[l1], c2 = cdf[l2], c3 = cdf[l3];
+                if (l >= nl || c0 >= u) break;
+                if (l1 >= nl || c1 >= u) { l = l1; break; }
+                if (l2 >= nl || c2 >= u) { l = l2; break; }
+                if (l3 >= nl || c3 >= u) { l = l3; break; }
+                l += 4;
+            }
             mode = l;
             double p1x, p1y, ndx, ndy;
             if constexpr (POLY) {
