@@ -187,7 +187,8 @@ def rta_extra(pkg, ctx, cpu_particles=0):
     R = 16
     b = pkg.RtaBatch(ctx, modes, [dict(geom=geom, T0=c["T0"], nparticles=c["nparticles"], seed=100 + k)
                                   for k in range(R)],
-                     thickness=c["thickness"], dt=c["dt"], maxtime=c["maxtime"], sigma_bounce=c["sigma_bounce"])
+                     thickness=c["thickness"], dt=c["dt"], maxtime=c["maxtime"], sigma_bounce=c["sigma_bounce"],
+                     tally="fp64-lds-runs")
     b.execute()
     ctx.sync()
     t0 = time.perf_counter()
@@ -204,7 +205,7 @@ def rta_extra(pkg, ctx, cpu_particles=0):
             "T_left_right": [float(T[0]), float(T[-1])],
             "sigma_T": {"replicas": R, "particles_per_replica": c["nparticles"], "max_bin_sigma_K": sig,
                         "target_K": 0.002, "met": sig <= 0.002, "seconds": t_rep,
-                        "events_per_s": rep["events"] / t_rep},
+                        "events_per_s": rep["events"] / t_rep, "tally": "fp64-lds-runs (mode 3)"},
             **({"cpu_baseline": rta_cpu_baseline(c, cpu_particles)} if cpu_particles else {})}
 
 
