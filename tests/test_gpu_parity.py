@@ -167,13 +167,14 @@ def test_krylov_matches_oracle(pkg, ctx):
         assert np.abs(Pg.sum(0) - 1).max() < 1e-12
 
 
-@pytest.mark.parametrize("n", [300, 520])
+@pytest.mark.parametrize("n", [2, 7, 257, 300, 520])
 def test_krylov_bf16_sizes_off_the_product_tile(pkg, ctx, n):
     """The BF16 product runs on 256 x 256 tiles with the operator and basis
     padded by zero rows and columns: N = 300 (padded to 512; N not a multiple
     of 8, so the element-wise Gram-Schmidt kernels) and N = 520 (padded to
     768; the vector kernels) give the exact exponential within the BF16
-    tolerance, columns summing to 1."""
+    tolerance, columns summing to 1; so do N = 2 and 7, below the Krylov
+    dimension m = 30, and N = 257, one past a tile."""
     import scipy.linalg
     W = pkg.workloads
     mat, B = W.config1_inputs(nmodes=n, seed=11)
