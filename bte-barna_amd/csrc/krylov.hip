@@ -185,12 +185,12 @@ __global__ __launch_bounds__(gk::NT) void kr_bf16x3_gemm(const __hip_bfloat16* _
     issue(0, 0);
     for (int s = 0; s < nk; s++) {
         const int buf = s & 1;
-        // this wave's loads of stage s have landed (s_waitcnt vmcnt(0));
-        // after the barrier every wave's have, and every wave is done with
-        // stage s-1, whose buffer then takes stage s+1. A bare barrier: the
-        // fence of __syncthreads would not add anything here.
-        __builtin_amdgcn_s_waitcnt(0x3F70);
-        __builtin_amdgcn_s_barrier();
+        // every wave's loads of stage s have landed (the release fence of
+        // __syncthreads waits for this wave's outstanding loads, which with
+        // two buffers are exactly stage s), and every wave is done with stage
+        // s-1, whose buffer then takes stage s+1; the fences also keep the
+        // compiler from moving LDS reads across the barrier
+        __syncthreads();
         if (s + 1 < nk) issue(s + 1, buf ^ 1);
         const unsigned char* sb = lds + buf * STAGE;
 #pragma unroll
