@@ -112,10 +112,65 @@ static int run(int argc, char** argv) {
         std::cout << "# Writing steady state" << std::endl;
         write_csv(name, 3, B.n(), out.data());
     }
-    for (int irun = 0; irun < nruns && !steady; irun++) {
+    // rows: (i + 0.5) dt for i < ntime - 1 with ntime = floor(maxtime/dt) + 1
+    const int nt = (int)std::floor(cfg.maxtime / cfg.dt + 1.0e-6);
+    // temperature_, jx_ and jy_<T0>K_run_<irun>.csv from [rows][nbox] blocks
+    auto write_run = [&](int irun, int rows, const double* T, const double* jx, const double* jy) {
+        char tag[64];
+        std::snprintf(tag, sizeof tag, "%gK_run_%i.csv", cfg.T0, irun);
+        const char* names[3] = {"temperature_", "jx_", "jy_"};
+        const double* data[3] = {T, jx, jy};
+        for (int q = 0; q < 3; q++) {
+            std::vector<double> out((size_t)rows * (B.n() + 1));
+            for (int i = 0; i < rows; i++) {
+                out[(size_t)i * (B.n() + 1)] = (i + 0.5) * cfg.dt;
+                for (int b = 0; b < B.n(); b++) out[(size_t)i * (B.n() + 1) + 1 + b] = data[q][(size_t)i * B.n() + b];
+            }
+            std::cout << "# Writing " << names[q] << tag << std::endl;
+            write_csv(std::string(names[q]) + tag, rows, B.n() + 1, out.data());
+        }
+    };
+    // The reference runs its nruns one after another (RTA_MC2D.cpp:2188-2471);
+    // here they go to the GPU as one batch: run i keeps the seed seed + i and
+    // the random streams of its own single run (keyed by run seed and particle
+    // index), so the histories are those of nruns single runs and only the
+    // order of the FP64 tally sums differs. Each run's time bins are summed in
+    // FP64 LDS per run (tally mode 3) where one bin of its boxes fits, else by
+    // device atomics. Spectral locations need single-run handles.
+    const bool batched = !steady && nruns > 1 && locs.empty();
+    if (batched) {
+        std::vector<bte_rta_run_desc> d(nruns);
+        for (int irun = 0; irun < nruns; irun++) {
+            d[irun] = bte_rta_run_desc{};
+            d[irun].geom = geom;
+            d[irun].T0 = cfg.T0;
+            d[irun].nparticles = cfg.nparticles;
+            d[irun].seed = seed0 + (uint64_t)irun;
+            d[irun].ballistic = cfg.ballistic;
+            d[irun].wscale = 1.0;
+        }
+        auto t0 = std::chrono::steady_clock::now();
+        bte_rta_batch* bt = nullptr;
+        check(bte_rta_batch_create(ctx, modes, nruns, d.data(), cfg.thickness, cfg.dt, cfg.maxtime, cfg.sigma_bounce,
+                                   0, &bt),
+              "RTAMC2D setup");
+        if (bte_rta_batch_set_tally(bt, 3) != BTE_OK) check(bte_rta_batch_set_tally(bt, 0), "RTAMC2D tallies");
+        const size_t per = (size_t)nt * B.n();
+        std::vector<double> T(per * nruns), jx(T.size()), jy(T.size()), Eeff(nruns);
+        bte_rta_batch_report rep{};
+        check(bte_rta_batch_execute(bt, T.data(), jx.data(), jy.data(), nullptr, Eeff.data(), &rep), "RTAMC2D runs");
+        double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        check(bte_rta_batch_destroy(bt), "RTAMC2D teardown");
+        if (rep.ntime - 1 != nt) throw std::runtime_error("time bins of the batch differ from the input's");
+        std::cout << "# runs 0-" << nruns - 1 << " as one batch: " << rep.events << " events, " << rep.crossings
+                  << " crossings, " << el << " s" << std::endl;
+        for (int irun = 0; irun < nruns; irun++) {
+            std::cout << "# run " << irun << ": Eeff " << Eeff[irun] << std::endl;
+            write_run(irun, nt, T.data() + per * irun, jx.data() + per * irun, jy.data() + per * irun);
+        }
+    }
+    for (int irun = 0; irun < nruns && !steady && !batched; irun++) {
         cfg.seed = seed0 + (uint64_t)irun;
-        // rows: (i + 0.5) dt for i < ntime - 1 with ntime = floor(maxtime/dt) + 1
-        const int nt = (int)std::floor(cfg.maxtime / cfg.dt + 1.0e-6);
         std::vector<double> T((size_t)nt * B.n()), jx(T.size()), jy(T.size());
         bte_rta_report rep{};
         auto t0 = std::chrono::steady_clock::now();
@@ -128,19 +183,7 @@ static int run(int argc, char** argv) {
         const int rows = rep.ntime - 1;
         std::cout << "# run " << irun << ": " << rep.events << " events, " << rep.crossings << " crossings, Eeff "
                   << rep.Eeff << ", " << el << " s" << std::endl;
-        char tag[64];
-        std::snprintf(tag, sizeof tag, "%gK_run_%i.csv", cfg.T0, irun);
-        const char* names[3] = {"temperature_", "jx_", "jy_"};
-        const std::vector<double>* data[3] = {&T, &jx, &jy};
-        for (int q = 0; q < 3; q++) {
-            std::vector<double> out((size_t)rows * (B.n() + 1));
-            for (int i = 0; i < rows; i++) {
-                out[(size_t)i * (B.n() + 1)] = (i + 0.5) * cfg.dt;
-                for (int b = 0; b < B.n(); b++) out[(size_t)i * (B.n() + 1) + 1 + b] = (*data[q])[(size_t)i * B.n() + b];
-            }
-            std::cout << "# Writing " << names[q] << tag << std::endl;
-            write_csv(std::string(names[q]) + tag, rows, B.n() + 1, out.data());
-        }
+        write_run(irun, rows, T.data(), jx.data(), jy.data());
         if (!locs.empty()) {
             // RTA_MC2D.cpp:2473-2541: deltaT_omega, jx_omega, jy_omega as
             // (ntime) x (nomega + 1) with the tick row first (time -1) and

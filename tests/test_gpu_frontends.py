@@ -170,17 +170,21 @@ def test_rtamc2d_csv(pkg, ctx, tmp_path):
         '<RTA_MC2d>\n<geometry file="g.xml"/>\n<time dt="20" maxtime="200"/>\n'
         '<material name="m" modetable="synthetic" nmodes="400" seed="2" thickness="1" T0="300" sigma_bounce="2.0"/>\n'
         '<particles N="20000" seed="5"/>\n</RTA_MC2d>\n')
-    run([pkg.app_path("RTAMC2D"), "in.xml", "8", "2"], tmp_path)
+    # three runs go to the GPU as one batch: the histories of three single
+    # runs with seeds 5, 6, 7; only the order of the FP64 tally sums differs
+    r = run([pkg.app_path("RTAMC2D"), "in.xml", "8", "3"], tmp_path)
+    assert "as one batch" in r.stdout
     mat = W.material(400, seed=2)
     modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
     geom = pkg.Geometry(ctx, rect, kind, teq)
-    for irun in range(2):
+    for irun in range(3):
         T = np.loadtxt(tmp_path / f"temperature_300K_run_{irun}.csv", delimiter=",")
         jx = np.loadtxt(tmp_path / f"jx_300K_run_{irun}.csv", delimiter=",")
         assert T.shape == (10, 1 + 8) and jx.shape == T.shape
         assert np.allclose(T[:, 0], (np.arange(10) + 0.5) * 20.0)
         o = pkg.rtamc2d(ctx, modes, geom, thickness=1.0, T0=300.0, dt=20.0, maxtime=200.0, nparticles=20000,
                         seed=5 + irun, sigma_bounce=2.0)
+        # the CSVs hold six significant digits
         assert np.allclose(T[:, 1:], o["T"], rtol=1e-5, atol=1e-9)
         assert np.allclose(jx[:, 1:], o["jx"], rtol=1e-5, atol=1e-6 * np.abs(o["jx"]).max())
 
