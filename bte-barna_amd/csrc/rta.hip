@@ -315,7 +315,8 @@ struct RtaState {
     Pcg32 rng;
     double x, y, t, dtrem, wscale, tlast;
     unsigned long long budget;
-    int mode, box, sign, run, still, alive, need_flight, pad;
+    int mode, box, sign, run, still, alive, need_flight, scat0;
+    int ballistic, pad0;
     float score[4];
 };
 
@@ -333,6 +334,8 @@ __device__ inline void rta_start(const RtaParams& P, RtaState& S, uint64_t pid) 
     const RtaRunDev& R = P.runs[S.run];
     S.rng.seed(R.seed, pid - R.first);
     S.wscale = R.wscale;
+    S.scat0 = R.scat0;
+    S.ballistic = R.ballistic;
     if (GRAD) S.score[0] = S.score[1] = S.score[2] = S.score[3] = 0.f;
     bool ng;
     int gi = R.gen0 + (int)alias_pick(P.gen_sel + R.gsel0, (uint32_t)R.ngen, S.rng, ng);
@@ -382,19 +385,31 @@ __device__ inline void rta_watch(const RtaParams& P, RtaState& S, RtaCounts& C) 
 template <bool GRAD, int TM, bool POLY>
 __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, RtaState& S, RtaCounts& C) {
     const double4 md = P.modes[S.mode];
+    const DevBox B = P.boxes[S.box];
+    double u = 0.;
     if (S.need_flight) {
         // exponential flight, truncated at maxtime
         if (!(S.t < P.maxtime)) {
             S.alive = 0;
             return;
         }
-        double u = 1.0 - S.rng.uniform();
+        u = 1.0 - S.rng.uniform();
+    }
+    // The scattering that ends a flight inside the box draws the next two
+    // numbers of the stream (alias_pick); they are taken here on a copy of
+    // the generator and the alias entry is loaded while the flight and the
+    // exit test are computed. The copy replaces the generator only on that
+    // path, so every stream is consumed as without the look-ahead.
+    Pcg32 r2 = S.rng;
+    const uint32_t sk = r2.bounded((uint32_t)P.n);
+    const uint32_t su = r2.next();
+    const uint2 se = P.scat[(size_t)S.scat0 + sk];
+    if (S.need_flight) {
         S.dtrem = -log(u) / (md.z * S.wscale);
         if (S.t + S.dtrem > P.maxtime) S.dtrem = P.maxtime - S.t + 1.0e-8;
         if (almost_equal(S.dtrem, 0.)) return;
         S.need_flight = 0;
     }
-    const DevBox B = P.boxes[S.box];
     if (B.kind == 2) {
         // periodic box: translate, keep the remaining flight (RTA_MC2D.cpp:1034-1043)
         const int nb = POLY ? translate_poly(P.boxes, P.sides, P.srange, P.cts, P.cseg, P.ptrans, S.box, S.x, S.y,
@@ -434,9 +449,10 @@ __device__ inline void rta_event(const RtaParams& P, void* lds, TallyAcc& A, Rta
         S.x = nx;
         S.y = ny;
         S.t += S.dtrem;
-        if (!P.runs[S.run].ballistic) {
-            bool ng;
-            S.mode = (int)alias_pick(P.scat + P.runs[S.run].scat0, (uint32_t)P.n, S.rng, ng);
+        if (!S.ballistic) {
+            // alias_pick with the draws taken ahead
+            S.mode = su < se.x ? (int)sk : (int)(se.y & 0x3fffffffu);
+            S.rng = r2;
         }
         C.events++;
         S.need_flight = 1;
@@ -1357,8 +1373,9 @@ static void batch_set_tally(RtaBatch* b, int mode) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->l3_bytes));
         b->block = 768;
         b->blocks = b->ctx->cu_count;
-        // about eight items per workgroup (configuration 3: 1.24 s, against
-        // 1.33 s with four, 1.54 s with two and 1.24 s with sixteen), at
+        // about eight items per workgroup (configuration 3 without the
+        // scattering look-ahead: 1.24 s, against 1.33 s with four, 1.54 s
+        // with two and 1.24 s with sixteen), at
         // least 4096 ids each; the runs with most particles first, so that
         // the last items taken are short
         const uint64_t isz = std::max<uint64_t>(4096, (b->total + 8 * (uint64_t)b->blocks - 1) / (8 * (uint64_t)b->blocks));
