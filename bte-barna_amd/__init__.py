@@ -332,16 +332,19 @@ class BertRun(_Handle):
 
     def __init__(self, ctx, modes, geom, prop, thickness, dt, maxtime, max_steps=0, Ed=-1.0,
                  Efactor=0, Tmax=0.0, Tmin=0.0, rta=False, pcancel_each=4, seed=1, capacity=0,
-                 gradient=None, prop_interp=0, init_dist=None):
+                 gradient=None, prop_interp=0, init_dist=None, step_shape=0):
         """gradient: (gx, gy) [K/nm] in every regular box (<gradient x y>);
         None derives per-box gradients from the box temperatures.
         init_dist: {box: signed distribution over the modes after the
-        propagator's Gamma offset} (<init_distribution box file>)"""
+        propagator's Gamma offset} (<init_distribution box file>).
+        step_shape: 0 chooses the step kernel's launch shape (the mode
+        velocities in LDS when they fit), 1 keeps them in global memory."""
         super().__init__()
         gx, gy = gradient if gradient is not None else (0.0, 0.0)
         self.cfg = BertCfg(thickness, dt, maxtime, max_steps, Ed, Efactor, Tmax, Tmin, int(rta),
                            pcancel_each, seed, capacity, int(gradient is not None), (C.c_double * 2)(gx, gy),
                            int(prop_interp))
+        self.cfg.step_shape = int(step_shape)
         self._init = None
         if init_dist:
             boxes = sorted(init_dist)
@@ -381,7 +384,7 @@ class BertRun(_Handle):
                 "particle_steps": rep.particle_steps, "step_kernel_particle_steps": rep.step_kernel_particle_steps,
                 "max_particles": rep.max_particles, "loop_ms": rep.loop_ms,
                 "step_kernel_ms": rep.step_kernel_ms, "step_kernel_launches": rep.step_kernel_launches,
-                "step_kernel_timed": rep.step_kernel_timed}
+                "step_kernel_timed": rep.step_kernel_timed, "step_threads": rep.step_threads}
 
 
 def bertamc2d(ctx, modes, geom, prop, **kw):

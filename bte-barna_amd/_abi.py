@@ -61,14 +61,14 @@ class BertCfg(C.Structure):
                 ("pcancel_each", C.c_int), ("seed", C.c_uint64), ("capacity", C.c_uint64),
                 ("fixed_gradient", C.c_int), ("gradient", C.c_double * 2), ("prop_interp", C.c_int),
                 ("ninit", C.c_int), ("init_len", C.c_int), ("init_boxes", C.POINTER(C.c_int32)),
-                ("init_dist", C.POINTER(C.c_double))]
+                ("init_dist", C.POINTER(C.c_double)), ("step_shape", C.c_int)]
 
 
 class BertReport(C.Structure):
     _fields_ = [("nsteps", C.c_int), ("Eeff", C.c_double), ("particle_steps", C.c_uint64),
                 ("step_kernel_particle_steps", C.c_uint64), ("max_particles", C.c_int64),
                 ("loop_ms", C.c_double), ("step_kernel_ms", C.c_double), ("step_kernel_launches", C.c_int),
-                ("step_kernel_timed", C.c_int)]
+                ("step_kernel_timed", C.c_int), ("step_threads", C.c_int)]
 
 
 class KrylovReport(C.Structure):

@@ -93,7 +93,7 @@ struct BertParams {
     int32_t nslots;          // propagator slots (box temperatures)
     int32_t lds_props;       // properties from per-workgroup box sums (nbox <= BERT_LDS_BOXES)
     // dynamic LDS layout of the particle kernels (byte offsets)
-    int32_t lds_pre, lds_eb, lds_sbox, lds_boxes, lds_cts;
+    int32_t lds_pre, lds_eb, lds_sbox, lds_boxes, lds_cts, lds_vel;
     int32_t cts_lds;         // contacts staged in LDS too (GL instantiations)
     int32_t eb_lds;          // wall counters in LDS (nsd <= BERT_EB_LDS)
 };
@@ -547,15 +547,31 @@ __device__ inline void box_lds_flush(const BertParams& P, int myseg) {
 // w-th equal slice of the live particle list (all input segments).
 // P1 without periodic boxes fits 96 VGPRs: five workgroups (waves per SIMD)
 // stay resident per CU
-template <bool GL, bool PER, bool P1, bool POLY>
-__global__ __launch_bounds__(256, (P1 && !PER) ? 5 : 1) void bert_step_kernel(BertParams P, uint32_t step, int cur,
-                                                        int cancel_prev, int hist) {
+// VL: one 1024-thread workgroup per CU with the mode velocities and their
+// inverses (32 B per mode) copied into LDS after the shared layout, so the
+// per-particle table reads are LDS reads; otherwise 256-thread workgroups,
+// five per CU for the P1 kernels, reading the tables from global memory.
+template <bool GL, bool PER, bool P1, bool POLY, bool VL = false>
+__global__ __launch_bounds__(VL ? 1024 : 256, VL ? 1 : ((P1 && !PER) ? 5 : 1)) void bert_step_kernel(
+    BertParams P, uint32_t step, int cur, int cancel_prev, int hist) {
     __shared__ uint32_t tsum[16];
     __shared__ uint32_t lcount;
     uint32_t* pre = (uint32_t*)(bert_dyn + P.lds_pre);
     const DevBox* boxes;
     const DevContact* cts;
     geo_stage<GL>(P, boxes, cts);
+    const double2* vtab = P.vel;
+    const double2* ivtab = P.ivel;
+    if constexpr (VL) {
+        // visible after seg_prefix's barrier below
+        double2* sv = (double2*)(bert_dyn + P.lds_vel);
+        for (int q = threadIdx.x; q < P.n; q += blockDim.x) {
+            sv[q] = P.vel[q];
+            sv[P.n + q] = P.ivel[q];
+        }
+        vtab = sv;
+        ivtab = sv + P.n;
+    }
     const int nb4 = P.nsd;
     int32_t* Eg = P.Eb + (size_t)(step & 1) * nb4;
     int32_t* Eb = eb_lds_begin(P, Eg, nb4);
@@ -618,8 +634,8 @@ __global__ __launch_bounds__(256, (P1 && !PER) ? 5 : 1) void bert_step_kernel(Be
             if (alive) {
                 seed_stream(rng, skey, p.pid);
                 if (nslots_of<P1>(P) == 1) f0 = first_pick<P1>(P, rng, mode, 0);
-                const double2 v = P.vel[mode];
-                const double2 iv = P.ivel[mode];
+                const double2 v = vtab[mode];
+                const double2 iv = ivtab[mode];
                 alive = advect<PER, POLY>(P, boxes, cts, p.x, p.y, box, sign, v.x, v.y, iv.x, iv.y, P.dt, rng, Eb);
                 if (nslots_of<P1>(P) > 1 && alive) f0 = first_pick<P1>(P, rng, mode, box);
             }
@@ -1261,6 +1277,12 @@ static const EmitKernel emit_kernel_tab[2][2][2] = {
 static const StepKernel step_kernel_poly_tab[2][2] = {
     {bert_step_kernel<false, false, false, true>, bert_step_kernel<false, false, true, true>},
     {bert_step_kernel<false, true, false, true>, bert_step_kernel<false, true, true, true>}};
+// the VL step kernels (no periodic boxes): [GL][P1] for rectangles, [P1] for polygons
+static const StepKernel step_kernel_vl_tab[2][2] = {
+    {bert_step_kernel<false, false, false, false, true>, bert_step_kernel<false, false, true, false, true>},
+    {bert_step_kernel<true, false, false, false, true>, bert_step_kernel<true, false, true, false, true>}};
+static const StepKernel step_kernel_poly_vl_tab[2] = {bert_step_kernel<false, false, false, true, true>,
+                                                       bert_step_kernel<false, false, true, true, true>};
 static const EmitKernel emit_kernel_poly_tab[2][2] = {
     {bert_emit_kernel<false, false, false, true>, bert_emit_kernel<false, false, true, true>},
     {bert_emit_kernel<false, true, false, true>, bert_emit_kernel<false, true, true, true>}};
@@ -1299,6 +1321,8 @@ struct Bert {
     int nslots = 1;
     int nsrc = 0, src_blocks = 0, ninit = 0;  // source parts, workgroups running them
     bool geo_lds = false;
+    bool vl = false;        // step kernels with the mode velocities in LDS (one 1024-thread workgroup per CU)
+    size_t lds_vl_bytes = 0;
     bool periodic = false;  // some box is periodic: launch the PER kernels
     bool p1 = false;        // expected to launch the P1 kernels (sizes the step grid)
     size_t lds_bytes = 0;
@@ -1491,16 +1515,21 @@ static BertParams make_params(Bert* b) {
     P.cts_lds = 1;
     if (b->geo_lds) {
         off = up16(P.lds_cts + sizeof(DevContact) * (size_t)P.ncts);
-        // five step workgroups per CU share its LDS: when the layout does
-        // not fit a fifth, the contacts (read only for sides that several
-        // boxes share and for periodic translations) stay in global memory
-        const size_t per_wg = b->ctx->lds_per_block / 5;
-        if (b->p1 && !b->periodic && off + 256 > per_wg && (size_t)P.lds_cts + 256 <= per_wg) {
+        // the step workgroups share the CU's LDS: five 256-thread ones, or
+        // one VL workgroup with the velocity tables after the layout. When
+        // the layout does not fit that share, the contacts (read only for
+        // sides that several boxes share and for periodic translations) stay
+        // in global memory
+        const long long per_wg = b->vl ? (long long)b->ctx->lds_per_block - 1024 - 32 * (long long)P.n
+                                       : (long long)(b->ctx->lds_per_block / 5);
+        if ((b->vl || b->p1) && !b->periodic && (long long)off + 256 > per_wg && (long long)P.lds_cts + 256 <= per_wg) {
             P.cts_lds = 0;
             off = (size_t)P.lds_cts;
         }
     }
     b->lds_bytes = off;
+    P.lds_vel = (int32_t)up16(off);
+    b->lds_vl_bytes = up16(off) + 32 * (size_t)P.n;
     // the particle kernels' dynamic LDS and their static arrays (under 1 KB)
     if (off + 1024 > b->ctx->lds_per_block)
         throw Error(BTE_EINVAL, "the particle kernels need " + std::to_string(off + 1024) + " bytes of LDS for " +
@@ -1810,9 +1839,21 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     b->p1 = !cfg->rta && b->p != nullptr && one_table;
     // step, emission and source workgroups together stay within
     // BERT_MAXSEG buffer segments on any device
-    b->step_blocks = std::min((b->periodic ? 3 : (b->p1 ? 5 : 4)) * c->cu_count, BERT_MAXSEG - 512 - BERT_SRC_BLOCKS);
     b->emit_blocks = std::min(2 * c->cu_count, 512);
     b->src_blocks = std::min(b->nsrc, BERT_SRC_BLOCKS);
+    // the VL step kernels (one 1024-thread workgroup per CU, mode velocities
+    // in LDS) whenever no box is periodic and the tables fit next to the
+    // particle kernels' layout; else 256-thread workgroups
+    if (cfg->step_shape < 0 || cfg->step_shape > 1) throw Error(BTE_EINVAL, "step_shape must be 0 or 1");
+    b->vl = !b->periodic && cfg->step_shape == 0;
+    if (b->vl) {
+        b->step_blocks = std::min(c->cu_count, BERT_MAXSEG - 512 - BERT_SRC_BLOCKS);
+        b->nsegs = b->step_blocks + b->emit_blocks + b->src_blocks;
+        (void)make_params(b);  // the layout for these segments
+        b->vl = b->lds_vl_bytes + 1024 <= c->lds_per_block;
+    }
+    if (!b->vl)
+        b->step_blocks = std::min((b->periodic ? 3 : (b->p1 ? 5 : 4)) * c->cu_count, BERT_MAXSEG - 512 - BERT_SRC_BLOCKS);
     b->nsegs = b->step_blocks + b->emit_blocks + b->src_blocks;
     if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
     // generous by default (32 B per particle and buffer): reservoir-driven
@@ -1826,9 +1867,13 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
         for (int j = j0; j < j1; j++) e += srcs[j].expected / srcs[j].nparts;
         src_wg_max = std::max(src_wg_max, e);
     }
+    // (every step workgroup's segment takes twice its share of the initial
+    // particles; with sources, its share of a steady population of 0.7 x 2^25)
+    const uint64_t seg_step = std::max<uint64_t>(2 * total, src_total > 0 ? (uint64_t)(0.7 * (1ull << 25)) : 0ull) /
+                                  (uint64_t)b->step_blocks + 4096;
     b->cap = cfg->capacity ? cfg->capacity
                            : std::max<uint64_t>({1ull << 22, 3 * total, src_total > 0 ? (1ull << 25) : 0ull,
-                                                 4096ull * (uint64_t)b->nsegs,
+                                                 4096ull * (uint64_t)b->nsegs, seg_step * (uint64_t)b->nsegs,
                                                  (uint64_t)(2.0 * src_wg_max + 4096.0) * (uint64_t)b->nsegs});
     b->cap = (b->cap + b->nsegs - 1) / b->nsegs * b->nsegs;
     if (b->cap >= 0xffffffffULL) throw Error(BTE_EINVAL, "capacity above 2^32 particles");
@@ -2058,6 +2103,12 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     // one propagator table and no RTA scattering: the P1 particle kernels
     const int p1 = (!b->cfg.rta && P.nslots == 1) ? 1 : 0;
     const int pc = b->cfg.pcancel_each;
+    StepKernel step_k = b->g->poly ? step_kernel_poly_tab[b->periodic][p1] : step_kernel_tab[b->geo_lds][b->periodic][p1];
+    if (b->vl) {
+        step_k = b->g->poly ? step_kernel_poly_vl_tab[p1] : step_kernel_vl_tab[b->geo_lds][p1];
+        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(step_k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)b->lds_vl_bytes));
+    }
     auto t0 = std::chrono::steady_clock::now();
     HIPCHECK(hipEventRecord(b->ev0, s));
     int istep = 0;
@@ -2076,11 +2127,8 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         // between two kernels leaves the device idle for ~5.6 us
         const bool timed = (istep % pc) == ((istep / pc) % pc);
         if (timed) HIPCHECK(hipEventRecord(b->kev[2 * istep], s));
-        {
-            auto k = b->g->poly ? step_kernel_poly_tab[b->periodic][p1] : step_kernel_tab[b->geo_lds][b->periodic][p1];
-            hipLaunchKernelGGL(k, dim3(step_blocks), dim3(256), b->lds_bytes, s, P, (uint32_t)istep, cur, cancel_prev,
-                               hist);
-        }
+        hipLaunchKernelGGL(step_k, dim3(step_blocks), dim3(b->vl ? 1024 : 256), b->vl ? b->lds_vl_bytes : b->lds_bytes,
+                           s, P, (uint32_t)istep, cur, cancel_prev, hist);
         if (timed) HIPCHECK(hipEventRecord(b->kev[2 * istep + 1], s));
         if (b->nsrc)
             hipLaunchKernelGGL(b->g->poly ? (b->periodic ? bert_source_kernel<true, true> : bert_source_kernel<false, true>)
@@ -2152,6 +2200,7 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
         rep->loop_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
         rep->step_kernel_ms = kms;
         rep->step_kernel_timed = ktimed;
+        rep->step_threads = b->vl ? 1024 : 256;
     }
     BTE_GUARD_END
 }

@@ -317,6 +317,13 @@ typedef struct {
     int init_len;
     const int32_t* init_boxes;
     const double* init_dist;
+    /* launch shape of the step kernel (no counterpart in the reference; the
+     * histories are the same either way): 0 = automatic, the mode
+     * velocities and their inverses staged in LDS (one 1024-thread
+     * workgroup per CU) when no box is periodic and 32 B per mode fit next
+     * to the particle kernels' LDS layout, else 1; 1 = 256-thread
+     * workgroups reading them from global memory */
+    int step_shape;
 } bte_bert_cfg;
 
 typedef struct {
@@ -332,6 +339,8 @@ typedef struct {
      * the cancellation cycle so every step type is sampled equally (an
      * event pair costs about 11 us of idle device time per step) */
     int step_kernel_timed;
+    int step_threads;         /* threads per step-kernel workgroup: 1024 with the
+                                 mode velocities in LDS, else 256 */
 } bte_bert_report;
 
 /* Frame callback: called after each step with the (box, mode) histogram

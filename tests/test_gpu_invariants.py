@@ -161,3 +161,57 @@ def test_bert_replicas_in_one_geometry_are_isolated(pkg, ctx, config1):
     assert np.abs(o["props"][-1, :50, 0]).sum() > 0
     tot = o["npart"] + o["pending"]
     assert (tot == tot[0]).all()
+
+
+@pytest.mark.parametrize("layout", ["ribbon", "zigzag", "many_modes"])
+def test_bert_step_shapes_follow_the_same_histories(pkg, ctx, layout):
+    """The step kernel with the mode velocities in LDS (one 1024-thread
+    workgroup per CU) and with them in global memory (256-thread
+    workgroups) differ only in which workgroup takes which particle. With a
+    nonnegative propagator every particle keeps its sign, cancellation keeps
+    every candidate, and the histories do not depend on that order: equal
+    particle and pending counts and (box, mode) histograms at every step,
+    box sums equal to rounding. 5000 modes do not fit the LDS next to the
+    layout, so the automatic choice falls back to 256 threads."""
+    from test_gpu_polygons import zigzag_ribbon
+    W = pkg.workloads
+    nmodes = 5000 if layout == "many_modes" else 400
+    mat, B = W.config1_inputs(nmodes=nmodes, seed=3)
+    P, _ = pkg.expm(ctx, B, 5.0)
+    P = np.maximum(P, 0.0)
+    P /= P.sum(0)
+    modes = pkg.ModeTable(ctx, mat["omega"], mat["vx"], mat["vy"], mat["w"], mat["V"], mat["lattz"])
+    if layout == "zigzag":
+        polys = zigzag_ribbon(12, 120.0, 40.0, 3.0, 5.0)
+        kind = np.zeros(len(polys), np.int32)
+        teq = np.full(len(polys), 300.0)
+        tin = teq.copy()
+        tin[0] = 301.0
+        geom = pkg.Geometry(ctx, None, kind, teq, tin, polys=polys)
+    else:
+        c = W.config1(nbins=20)
+        geom = pkg.Geometry(ctx, c["rect"], c["kind"], c["Teq"], c["Tinit"])
+    prop = pkg.Propagator(ctx, P)
+    kw = dict(thickness=1.0, dt=5.0, maxtime=5.0 * 15, Efactor=100000, Tmax=301.0, Tmin=300.0, seed=9)
+    out = []
+    for shape in (0, 1):
+        frames = []
+        r = pkg.BertRun(ctx, modes, geom, prop, step_shape=shape, **kw)
+        o = r.execute(frame_cb=lambda s, t, e, H: frames.append(H) and False)
+        r.close()
+        out.append((o, frames))
+    (a, fa), (b, fb) = out
+    print(f"{layout}: step threads {a['step_threads']} / {b['step_threads']}, "
+          f"{a['particle_steps']} particle-steps, {len(fa)} frames", flush=True)
+    assert a["step_threads"] == (256 if layout == "many_modes" else 1024)
+    assert b["step_threads"] == 256
+    assert a["nsteps"] == b["nsteps"] == 16
+    np.testing.assert_array_equal(a["npart"], b["npart"])
+    np.testing.assert_array_equal(a["pending"], b["pending"])
+    assert a["particle_steps"] == b["particle_steps"]
+    assert len(fa) == len(fb) > 0
+    for x, y in zip(fa, fb):
+        np.testing.assert_array_equal(x, y)
+    scale = np.abs(a["props"]).max()
+    assert scale > 0
+    np.testing.assert_allclose(a["props"], b["props"], rtol=0, atol=1e-12 * scale)
