@@ -282,11 +282,13 @@ __device__ inline void h_add(unsigned long long* h, int sign) {
 
 // A child written in this step enters the (box, mode) histogram when that
 // step feeds a cancellation or a frame (H not null) and the box sums behind
-// calculate_prop (sbox in LDS, not null when the boxes fit there).
-__device__ inline void tally_child(const BertParams& P, unsigned long long* H, double* sbox, int box, int nm, int ns) {
+// calculate_prop (sbox in LDS, not null when the boxes fit there); vt holds
+// the mode velocities (in LDS in the VL step kernels).
+__device__ inline void tally_child(const BertParams& P, unsigned long long* H, double* sbox, const double2* vt, int box,
+                                   int nm, int ns) {
     if (H) h_add(H + (size_t)box * P.n + nm, ns);
     if (sbox) {
-        const double2 v = P.vel[nm];
+        const double2 v = vt[nm];
         atomicAdd(sbox + box * 3 + 0, (double)ns);
         atomicAdd(sbox + box * 3 + 1, ns * v.x);
         atomicAdd(sbox + box * 3 + 2, ns * v.y);
@@ -299,7 +301,8 @@ __device__ inline void tally_child(const BertParams& P, unsigned long long* H, d
 // them when there is more than one. Children of a sign flip re-queue the
 // parent twice (1189-1192).
 template <bool EMIT, bool P1>
-__device__ inline int scatter(const BertParams& P, unsigned long long* H, double* sbox, Pcg32 rng, const Pick0& f0, int mode,
+__device__ inline int scatter(const BertParams& P, unsigned long long* H, double* sbox, const double2* vt, Pcg32 rng,
+                              const Pick0& f0, int mode,
                               int sign, int box, BPart* out, uint64_t slot, uint64_t limit, double x, double y,
                               uint64_t pid, uint32_t step, int& nm0, int& ns0) {
     if (rta_on<P1>(P)) {
@@ -311,7 +314,7 @@ __device__ inline int scatter(const BertParams& P, unsigned long long* H, double
             nm = (int)alias_pick(P.rta_alias + (size_t)sl * P.n, (uint32_t)P.n, rng, ng);
         }
         if (EMIT) {
-            tally_child(P, H, sbox, box, nm, sign);
+            tally_child(P, H, sbox, vt, box, nm, sign);
             if (slot < limit) {
                 BPart q{x, y, (uint32_t)nm, ((uint32_t)box << 1) | (sign < 0 ? 1u : 0u), pid};
                 out[slot] = q;
@@ -347,7 +350,7 @@ __device__ inline int scatter(const BertParams& P, unsigned long long* H, double
             ns0 = ns;
         }
         if (EMIT) {
-            tally_child(P, H, sbox, box, nm, ns);
+            tally_child(P, H, sbox, vt, box, nm, ns);
             uint64_t s = slot + nchild;
             if (s < limit) {
                 uint64_t cpid = nchild == 0 ? pid : mix64(pid ^ ((uint64_t)step << 40) ^ (uint64_t)nchild);
@@ -447,21 +450,22 @@ __device__ inline void seg_close(const BertParams& P, uint32_t* segout, int myse
 // count the children, reserve their slots in the workgroup's segment and
 // write them; a single child (the common case) is written from pass 0's result
 template <bool P1>
-__device__ inline void scatter_out(const BertParams& P, unsigned long long* H, double* sbox, Pcg32& rng, const Pick0& f0,
+__device__ inline void scatter_out(const BertParams& P, unsigned long long* H, double* sbox, const double2* vt, Pcg32& rng,
+                                   const Pick0& f0,
                                    bool alive, int mode, int sign, int box, double x, double y, uint64_t pid,
                                    uint32_t step, uint32_t* lds_count, int myseg) {
     int nm0 = 0, ns0 = 1;
-    int c = alive ? scatter<false, P1>(P, H, sbox, rng, f0, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
+    int c = alive ? scatter<false, P1>(P, H, sbox, vt, rng, f0, mode, sign, box, nullptr, 0, 0, 0, 0, 0, 0, nm0, ns0) : 0;
     uint64_t limit;
     uint64_t slot = wave_reserve(lds_count, myseg, P.segcap, c, P.flags, limit);
     if (c == 1) {
-        tally_child(P, H, sbox, box, nm0, ns0);
+        tally_child(P, H, sbox, vt, box, nm0, ns0);
         if (slot < limit) {
             BPart q{x, y, (uint32_t)nm0, ((uint32_t)box << 1) | (ns0 < 0 ? 1u : 0u), pid};
             P.pout[slot] = q;
         }
     } else if (c > 1) {
-        scatter<true, P1>(P, H, sbox, rng, f0, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
+        scatter<true, P1>(P, H, sbox, vt, rng, f0, mode, sign, box, P.pout, slot, limit, x, y, pid, step, nm0, ns0);
     }
 }
 
@@ -640,7 +644,7 @@ __global__ __launch_bounds__(VL ? 1024 : 256, VL ? 1 : ((P1 && !PER) ? 5 : 1)) v
                 if (nslots_of<P1>(P) > 1 && alive) f0 = first_pick<P1>(P, rng, mode, box);
             }
         }
-        scatter_out<P1>(P, H, sbox, rng, f0, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
+        scatter_out<P1>(P, H, sbox, vtab, rng, f0, alive, mode, sign, box, p.x, p.y, p.pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -980,7 +984,7 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
             alive = advect<PER, POLY>(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
             if (nslots_of<P1>(P) > 1 && alive) f0 = first_pick<P1>(P, rng, mode, bx);
         }
-        scatter_out<P1>(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out<P1>(P, H, sbox, P.vel, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -1087,7 +1091,7 @@ __global__ __launch_bounds__(256) void bert_source_kernel(BertParams P, const Be
                 alive = advect<PER, POLY>(P, P.boxes, P.cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Eb);
                 if (nslots_of<false>(P) > 1 && alive) f0 = first_pick<false>(P, rng, mode, bx);
             }
-            scatter_out<false>(P, H, sbox, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+            scatter_out<false>(P, H, sbox, P.vel, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
             done += alive ? 1 : 0;
         }
     }
