@@ -903,8 +903,11 @@ __global__ __launch_bounds__(256) void bert_wallcdf_kernel(BertParams P, uint32_
 // kernel's segment walk).
 #define BERT_EMIT_MAXB 4096
 #define BERT_EMIT_THREADS 512
-template <bool GL, bool PER, bool P1, bool POLY>
-__global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams P, uint32_t step, int cur, int hist) {
+// VL: as in the step kernel, one 1024-thread workgroup per CU with the mode
+// velocities and their inverses in LDS.
+template <bool GL, bool PER, bool P1, bool POLY, bool VL = false>
+__global__ __launch_bounds__(VL ? 1024 : BERT_EMIT_THREADS) void bert_emit_kernel(BertParams P, uint32_t step, int cur,
+                                                                                  int hist) {
     __shared__ uint32_t tsum[16];
     __shared__ uint32_t lcount;
     uint32_t* pre = (uint32_t*)(bert_dyn + P.lds_pre);
@@ -915,6 +918,18 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
     const DevBox* boxes;
     const DevContact* cts;
     geo_stage<GL>(P, boxes, cts);
+    const double2* vtab = P.vel;
+    const double2* ivtab = P.ivel;
+    if constexpr (VL) {
+        // visible after seg_prefix's barrier below
+        double2* sv = (double2*)(bert_dyn + P.lds_vel);
+        for (int q = threadIdx.x; q < P.n; q += blockDim.x) {
+            sv[q] = P.vel[q];
+            sv[P.n + q] = P.ivel[q];
+        }
+        vtab = sv;
+        ivtab = sv + P.n;
+    }
     int32_t* Eg = P.Eb + (size_t)((step + 1) & 1) * nb4;
     int32_t* Ebn = eb_lds_begin(P, Eg, nb4);
     double* sbox = box_lds_begin(P);
@@ -977,14 +992,14 @@ __global__ __launch_bounds__(BERT_EMIT_THREADS) void bert_emit_kernel(BertParams
             double t = rng.uniform();
             x = p1x + t * ndx;
             y = p1y + t * ndy;
-            const double2 v = P.vel[mode];
-            const double2 iv = P.ivel[mode];
+            const double2 v = vtab[mode];
+            const double2 iv = ivtab[mode];
             double rdt = P.dt * rng.uniform();
             if (nslots_of<P1>(P) == 1) f0 = first_pick<P1>(P, rng, mode, 0);
             alive = advect<PER, POLY>(P, boxes, cts, x, y, bx, sign, v.x, v.y, iv.x, iv.y, rdt, rng, Ebn);
             if (nslots_of<P1>(P) > 1 && alive) f0 = first_pick<P1>(P, rng, mode, bx);
         }
-        scatter_out<P1>(P, H, sbox, P.vel, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
+        scatter_out<P1>(P, H, sbox, vtab, rng, f0, alive, mode, sign, bx, x, y, pid, step, &lcount, myseg);
         done += alive ? 1 : 0;
     }
     seg_close(P, segout, myseg, &lcount);
@@ -1287,6 +1302,11 @@ static const StepKernel step_kernel_vl_tab[2][2] = {
     {bert_step_kernel<true, false, false, false, true>, bert_step_kernel<true, false, true, false, true>}};
 static const StepKernel step_kernel_poly_vl_tab[2] = {bert_step_kernel<false, false, false, true, true>,
                                                        bert_step_kernel<false, false, true, true, true>};
+static const EmitKernel emit_kernel_vl_tab[2][2] = {
+    {bert_emit_kernel<false, false, false, false, true>, bert_emit_kernel<false, false, true, false, true>},
+    {bert_emit_kernel<true, false, false, false, true>, bert_emit_kernel<true, false, true, false, true>}};
+static const EmitKernel emit_kernel_poly_vl_tab[2] = {bert_emit_kernel<false, false, false, true, true>,
+                                                       bert_emit_kernel<false, false, true, true, true>};
 static const EmitKernel emit_kernel_poly_tab[2][2] = {
     {bert_emit_kernel<false, false, false, true>, bert_emit_kernel<false, false, true, true>},
     {bert_emit_kernel<false, true, false, true>, bert_emit_kernel<false, true, true, true>}};
@@ -1843,21 +1863,24 @@ int bte_bert_create(bte_ctx* ctx, const bte_modes* mm, const bte_geom* gg, const
     b->p1 = !cfg->rta && b->p != nullptr && one_table;
     // step, emission and source workgroups together stay within
     // BERT_MAXSEG buffer segments on any device
-    b->emit_blocks = std::min(2 * c->cu_count, 512);
     b->src_blocks = std::min(b->nsrc, BERT_SRC_BLOCKS);
-    // the VL step kernels (one 1024-thread workgroup per CU, mode velocities
-    // in LDS) whenever no box is periodic and the tables fit next to the
-    // particle kernels' layout; else 256-thread workgroups
+    // the VL step and emission kernels (one 1024-thread workgroup per CU,
+    // mode velocities in LDS) whenever no box is periodic and the tables fit
+    // next to the particle kernels' layout; else 256-thread step and
+    // 512-thread emission workgroups
     if (cfg->step_shape < 0 || cfg->step_shape > 1) throw Error(BTE_EINVAL, "step_shape must be 0 or 1");
     b->vl = !b->periodic && cfg->step_shape == 0;
     if (b->vl) {
         b->step_blocks = std::min(c->cu_count, BERT_MAXSEG - 512 - BERT_SRC_BLOCKS);
+        b->emit_blocks = std::min(c->cu_count, 512);
         b->nsegs = b->step_blocks + b->emit_blocks + b->src_blocks;
         (void)make_params(b);  // the layout for these segments
         b->vl = b->lds_vl_bytes + 1024 <= c->lds_per_block;
     }
-    if (!b->vl)
+    if (!b->vl) {
         b->step_blocks = std::min((b->periodic ? 3 : (b->p1 ? 5 : 4)) * c->cu_count, BERT_MAXSEG - 512 - BERT_SRC_BLOCKS);
+        b->emit_blocks = std::min(2 * c->cu_count, 512);
+    }
     b->nsegs = b->step_blocks + b->emit_blocks + b->src_blocks;
     if (b->nsegs > BERT_MAXSEG) throw Error(BTE_EINVAL, "too many buffer segments");
     // generous by default (32 B per particle and buffer): reservoir-driven
@@ -2108,9 +2131,13 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
     const int p1 = (!b->cfg.rta && P.nslots == 1) ? 1 : 0;
     const int pc = b->cfg.pcancel_each;
     StepKernel step_k = b->g->poly ? step_kernel_poly_tab[b->periodic][p1] : step_kernel_tab[b->geo_lds][b->periodic][p1];
+    EmitKernel emit_k = b->g->poly ? emit_kernel_poly_tab[b->periodic][p1] : emit_kernel_tab[b->geo_lds][b->periodic][p1];
     if (b->vl) {
         step_k = b->g->poly ? step_kernel_poly_vl_tab[p1] : step_kernel_vl_tab[b->geo_lds][p1];
+        emit_k = b->g->poly ? emit_kernel_poly_vl_tab[p1] : emit_kernel_vl_tab[b->geo_lds][p1];
         HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(step_k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)b->lds_vl_bytes));
+        HIPCHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(emit_k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                      (int)b->lds_vl_bytes));
     }
     auto t0 = std::chrono::steady_clock::now();
@@ -2141,11 +2168,8 @@ int bte_bert_execute(bte_bert* bb, double* props, int64_t* npart, int64_t* pendi
                                dim3(256), b->lds_bytes, s, P, b->d_src.p, b->nsrc, b->d_src_tab.p, (uint32_t)istep,
                                cur, hist);
         hipLaunchKernelGGL(bert_wallcdf_kernel, dim3((unsigned)b->g->sides.size()), dim3(256), 0, s, P, (uint32_t)istep);
-        {
-            auto k = b->g->poly ? emit_kernel_poly_tab[b->periodic][p1] : emit_kernel_tab[b->geo_lds][b->periodic][p1];
-            hipLaunchKernelGGL(k, dim3(emit_blocks), dim3(BERT_EMIT_THREADS), b->lds_bytes, s, P, (uint32_t)istep, cur,
-                               hist);
-        }
+        hipLaunchKernelGGL(emit_k, dim3(emit_blocks), dim3(b->vl ? 1024 : BERT_EMIT_THREADS),
+                           b->vl ? b->lds_vl_bytes : b->lds_bytes, s, P, (uint32_t)istep, cur, hist);
         // H of this step lives in H[cur]; H[cur^1] (previous step) is cleared
         // after the step kernel used it for cancellation
         hipLaunchKernelGGL(bert_props_kernel, dim3(nbox), dim3(256), 0, s, P, istep + 1, cur, cancel_now,

@@ -317,12 +317,13 @@ typedef struct {
     int init_len;
     const int32_t* init_boxes;
     const double* init_dist;
-    /* launch shape of the step kernel (no counterpart in the reference; the
-     * histories are the same either way): 0 = automatic, the mode
-     * velocities and their inverses staged in LDS (one 1024-thread
-     * workgroup per CU) when no box is periodic and 32 B per mode fit next
-     * to the particle kernels' LDS layout, else 1; 1 = 256-thread
-     * workgroups reading them from global memory */
+    /* launch shape of the step and emission kernels (no counterpart in the
+     * reference; the histories are the same either way): 0 = automatic,
+     * the mode velocities and their inverses staged in LDS (one
+     * 1024-thread workgroup per CU) when no box is periodic and 32 B per
+     * mode fit next to the particle kernels' LDS layout, else 1; 1 = 256-
+     * (step) and 512-thread (emission) workgroups reading them from global
+     * memory */
     int step_shape;
 } bte_bert_cfg;
 
